@@ -1,0 +1,217 @@
+"""Benchmark: DRR/s for forward + analytic backward (pose gradients) on one MI355X.
+
+Workload (fwd+bwd on a 256^3 volume): seeded 256^3 phantom at 1 mm, 4 label
+channels (int8 labels), 16 poses (ZXY +-15 deg, +-20 mm about the isocentre
+600 mm from the source), 200 x 200 detector at 1 mm, sdd 1020 mm, 256
+samples per ray. One step = render(...) of the 16 poses plus the backward of
+a weighted sum to the poses; DRR/s = 16 * steps / elapsed. The volume is
+benchmarked stored as bfloat16 (the headline `value`) and as float32; both
+accumulate in float32.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints one JSON line. `roofline` covers the dominant kernel (the backward
+march) timed with HIP events on the stream it runs on; `cpu_baseline` times
+the pure-PyTorch grid_sample restatement (fp32) of the same workload on the
+host cores for a bounded sample of DRRs.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+import torch  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+SIZE, SPACING, N_CLASSES, BATCH, DET, S = 256, 1.0, 4, 16, 200, 256
+
+
+def make_workload(dtype, device):
+    from nanodrr_amd.phantom import detector, make_phantom, phantom_subject, sample_poses
+
+    ph = make_phantom((SIZE,) * 3, (SPACING,) * 3, seed=0)
+    subject = phantom_subject(ph, n_classes=N_CLASSES, dtype=dtype, device=device)
+    rt_inv, _, _ = sample_poses(BATCH, seed=1, isocenter=subject.isocenter)
+    k_inv, sdd = detector(1020.0, 1.0, DET, DET)
+    return subject, k_inv.to(device), rt_inv.to(device), sdd.to(device), ph
+
+
+def step_fn(subject, k_inv, rt_inv, sdd, wts):
+    from nanodrr_amd import render
+
+    rt = rt_inv.detach().requires_grad_(True)
+    out = render(subject, k_inv, rt, sdd, DET, DET, n_samples=S)
+    (out * wts).sum().backward()
+    return rt.grad
+
+
+def time_steps(fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(steps):
+        fn()
+    t1.record()
+    torch.cuda.synchronize()
+    return t0.elapsed_time(t1) / steps  # ms per step
+
+
+def kernel_times(subject, k_inv, rt_inv, sdd, wts, reps):
+    """Average duration (ms) of one forward and one backward op launch, HIP events
+    recorded on the stream the ops launch on (torch's current stream)."""
+    from nanodrr_amd._ops import fused_raymarch, fused_raymarch_bwd
+    from nanodrr_amd.renderer import _make_src, _make_tgt
+
+    vol = subject.image[0, 0]
+    lab = subject.label_i8
+    M = (subject.world_to_voxel.double() @ rt_inv.double())[:, :3, :].float().contiguous()
+    tgt = _make_tgt(k_inv, sdd, DET, DET, k_inv.device, torch.float32).contiguous()
+    src = _make_src(False, tgt, sdd).contiguous()
+    step = ((tgt - src).norm(dim=-1) / (S - 1)).contiguous()
+    go = wts.reshape(1, N_CLASSES, -1).expand(M.shape[0], -1, -1).contiguous()
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for _ in range(reps):
+            fn()
+        t1.record(stream)
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1) / reps
+
+    ms_fwd = timed(lambda: fused_raymarch(vol, lab, M, src, tgt, step, S, N_CLASSES, DET))
+    ms_bwd = timed(lambda: fused_raymarch_bwd(go, vol, lab, M, src, tgt, step, S, N_CLASSES, DET, False, False))
+    return ms_fwd, ms_bwd, (M, src, tgt)
+
+
+def algorithmic_bytes(subject, geom, dtype):
+    from nanodrr_amd.roofline import inside_samples, march_bytes
+
+    M, src, tgt = geom
+    n_in = int(inside_samples(M, src, tgt, S, tuple(subject.image.shape[-3:])).sum())
+    n_rays = M.shape[0] * tgt.shape[1]
+    fwd = march_bytes(n_in, n_rays, M.shape[0], N_CLASSES, dtype)
+    bwd = march_bytes(n_in, n_rays, M.shape[0], N_CLASSES, dtype, backward=True)
+    return n_in, fwd, bwd
+
+
+def cpu_baseline(ph, budget_s=12.0):
+    """Pure-PyTorch grid_sample restatement (oracle.torch_ref, fp32) of one DRR of the
+    workload (fwd + pose backward), repeated for about `budget_s` seconds."""
+    from nanodrr_amd.phantom import detector, phantom_subject, sample_poses
+    from oracle import torch_ref
+
+    threads = min(16, os.cpu_count() or 1)
+    torch.set_num_threads(threads)
+    subject = phantom_subject(ph, n_classes=N_CLASSES)
+    rt_all, _, _ = sample_poses(BATCH, seed=1, isocenter=subject.isocenter)
+    k_inv, sdd = detector(1020.0, 1.0, DET, DET)
+    image, label, w2g = subject.image, subject.label, subject.world_to_grid
+    wts = torch.randn(1, N_CLASSES, DET, DET, generator=torch.Generator().manual_seed(0))
+
+    def one(i):
+        rt = rt_all[i % BATCH: i % BATCH + 1].clone().requires_grad_(True)
+        out = torch_ref.render_torch(image, label, N_CLASSES, w2g, k_inv, rt, sdd, DET, DET, S)
+        (out * wts).sum().backward()
+
+    one(0)  # warm-up
+    times = []
+    t_start = time.perf_counter()
+    i = 1
+    while (time.perf_counter() - t_start < budget_s and i <= 40) or len(times) < 2:
+        t0 = time.perf_counter()
+        one(i)
+        times.append(time.perf_counter() - t0)
+        i += 1
+    med = statistics.median(times)
+    return {"value": 1.0 / med, "unit": "DRR/s", "cores": threads, "kind": "port",
+            "sample": f"{len(times)} DRRs (1 pose each) of the 256^3/C=4/200x200/S=256 workload, fwd + pose "
+                      f"backward, oracle.torch_ref fp32 grid_sample on {threads} threads, median of runs"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--dtypes", default="bf16,fp32")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(2)
+    dev = torch.device("cuda:0")
+    dtypes = {"bf16": torch.bfloat16, "fp32": torch.float32, "fp16": torch.float16}
+    results = {}
+    ph = None
+    for name in args.dtypes.split(","):
+        dtype = dtypes[name]
+        subject, k_inv, rt_inv, sdd, ph = make_workload(dtype, dev)
+        wts = torch.randn(1, N_CLASSES, DET, DET, generator=torch.Generator().manual_seed(0)).to(dev)
+        ms = time_steps(lambda: step_fn(subject, k_inv, rt_inv, sdd, wts), args.steps, args.warmup)
+        ms_fwd, ms_bwd, geom = kernel_times(subject, k_inv, rt_inv, sdd, wts, max(10, args.steps))
+        n_in, b_fwd, b_bwd = algorithmic_bytes(subject, geom, dtype)
+        results[name] = dict(ms_per_step=ms, drr_per_s=BATCH * 1000.0 / ms, ms_fwd=ms_fwd, ms_bwd=ms_bwd,
+                             inside_samples=n_in, bytes_fwd=b_fwd, bytes_bwd=b_bwd,
+                             gbs_fwd=b_fwd / ms_fwd / 1e6, gbs_bwd=b_bwd / ms_bwd / 1e6)
+        print(f"# {name}: {results[name]}", file=sys.stderr, flush=True)
+        del subject
+        torch.cuda.empty_cache()
+
+    head = "bf16" if "bf16" in results else next(iter(results))
+    r = results[head]
+    dominant = "bwd" if r["ms_bwd"] >= r["ms_fwd"] else "fwd"
+    achieved = r[f"gbs_{dominant}"]
+    line = {
+        "metric": "DRR/s fwd+bwd 200x200 @256^3 (fp32 vs bf16)",
+        "value": r["drr_per_s"],
+        "unit": "DRR/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": head,
+        "data": "synthetic (seeded 256^3 phantom, 4 labels, 16 random poses)",
+        "config": {
+            "workload": "fwd+bwd (pose grads) 256^3 @1mm, C=4, B=16, 200x200, S=256",
+            "volume_storage": head, "accumulate": "fp32", "global_batch": BATCH, "n_samples": S,
+            "per_dtype": {k: {"drr_per_s": v["drr_per_s"], "ms_per_step": v["ms_per_step"],
+                              "ms_fwd_kernel": v["ms_fwd"], "ms_bwd_kernel": v["ms_bwd"],
+                              "gbs_fwd": v["gbs_fwd"], "gbs_bwd": v["gbs_bwd"]} for k, v in results.items()},
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": f"k_{dominant} ({head} volume)", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "algorithmic_bytes_per_launch": r[f"bytes_{dominant}"], "inside_samples": r["inside_samples"],
+        },
+    }
+    if "fp32" in results and "bf16" in results:
+        line["config"]["bf16_over_fp32"] = results["bf16"]["drr_per_s"] / results["fp32"]["drr_per_s"]
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(ph)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,131 @@
+/*
+ * nanodrr_amd — C ABI of the MI355X (gfx950) fused differentiable DRR ray march.
+ *
+ * Every entry point takes raw device pointers, element counts and a HIP stream
+ * handle (passed as `void*`; NULL means the legacy default stream). Nothing in
+ * these signatures depends on PyTorch. Launches are asynchronous on `stream`;
+ * no entry point synchronises the host or allocates device memory. Callers own
+ * every buffer.
+ *
+ * Return value: 0 on success, a negative NDRR_E* code for an argument error
+ * detected before any launch, or a positive hipError_t from the HIP runtime.
+ * `ndrr_strerror` turns either into text.
+ *
+ * Geometry conventions (all row-major, fp32):
+ *   M    [B][3][4]   camera space -> voxel-index space (voxel centres are the
+ *                    integers; x indexes W, y indexes H, z indexes D)
+ *   src  ray sources in camera space, element (b, n) at
+ *        src[b * src_batch_stride + n * src_ray_stride + {0,1,2}]
+ *        (perspective rays share one source per pose: src_ray_stride = 0)
+ *   tgt  ray targets, element (b, n) at tgt[b * tgt_batch_stride + 3 n + {0,1,2}]
+ *        (tgt_batch_stride = 0 shares one detector across the pose batch)
+ *   step per-ray Riemann weight, element (b, n) at step[b * step_batch_stride + n]
+ *   out  [B][C][N] fp32 line integrals
+ *   vol  [D][H][W] in NDRR_F32 / NDRR_BF16 / NDRR_F16, W fastest
+ *   lab  [D][H][W] int8 class per voxel (values outside [0, C) are dropped);
+ *        ignored when n_classes == 1
+ * Sample s of ray (b, n) sits at M[b] @ [lerp(src, tgt, s / (S - 1)), 1].
+ * Rays are mapped to 16 x 16 detector tiles of a `width`-wide image when
+ * width > 0 (N must be a multiple of width), else to runs of 256 rays.
+ */
+#ifndef NANODRR_AMD_H
+#define NANODRR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Volume element types. */
+enum {
+  NDRR_F32 = 0,
+  NDRR_BF16 = 1,
+  NDRR_F16 = 2
+};
+
+/* Argument errors (negative return codes). */
+enum {
+  NDRR_OK = 0,
+  NDRR_EINVAL = -1,  /* null pointer, non-positive size, S < 2, bad width */
+  NDRR_EDTYPE = -2,  /* unknown volume element type */
+  NDRR_ECLASSES = -3,/* n_classes outside [1, NDRR_MAX_CLASSES] */
+  NDRR_ERANGE = -4   /* a size exceeds the kernel's 32-bit index range */
+};
+
+#define NDRR_MAX_CLASSES 8
+
+/* Library version as major * 10000 + minor * 100 + patch. */
+int ndrr_version(void);
+
+/* Text for a return code of any entry point below. Never NULL. */
+const char* ndrr_strerror(int code);
+
+/*
+ * Forward ray march: out[b][c][n] = step(b,n) * sum_s trilinear(vol, p_s) * [lab(p_s) == c]
+ * with zero padding outside the volume and nearest (round-half-even) label
+ * lookup. Rays with n_samples >= 512 accumulate with Neumaier-compensated
+ * fp32 sums. Output is fp32 whatever the volume type.
+ * Replaces the forward launch of nanodrr's fused op:
+ *   src/nanodrr/drr/_fused.py:415-460 (fused_raymarch) and the
+ *   BACKWARD=0 specialisation of _fused_kernel, _fused.py:36-331.
+ */
+int ndrr_raymarch_fwd(const void* vol, int vol_dtype, const int8_t* lab,
+                      const float* M, const float* src, const float* tgt,
+                      const float* step, float* out,
+                      int B, int N, int D, int H, int W,
+                      int n_samples, int n_classes, int width,
+                      int64_t src_batch_stride, int64_t src_ray_stride,
+                      int64_t tgt_batch_stride, int64_t step_batch_stride,
+                      void* stream);
+
+/*
+ * Number of fp32 elements of the pose-gradient workspace that
+ * ndrr_raymarch_bwd needs for B poses of N rays at this width
+ * (12 per pose per detector tile). Pure host arithmetic.
+ */
+size_t ndrr_bwd_workspace_floats(int B, int N, int width);
+
+/*
+ * Analytic adjoint of ndrr_raymarch_fwd for an upstream gradient
+ * go[B][C][N] (fp32):
+ *   gM[b][3][4]  = sum over the pose's rays and samples of
+ *                  g * grad_p trilinear (x) [q; 1], g = go[b][cls][n] * step,
+ *                  q = lerp(src, tgt, t) — deterministic: per-tile partials in
+ *                  `workspace` (ndrr_bwd_workspace_floats) reduced in fixed order;
+ *   gsrc, gtgt   [B][N][3] per-ray gradients of the ray endpoints, and
+ *   gstep        [B][N], written when need_gcam != 0 (else may be NULL);
+ *   gvol         [D][H][W] fp32, accumulated with atomics when gvol != NULL
+ *                (the caller zeroes it; the result is not bitwise reproducible).
+ * Replaces the backward launch of nanodrr's fused op:
+ *   src/nanodrr/drr/_fused.py:477-531 (_fused_raymarch_bwd) and the
+ *   BACKWARD=1 specialisation of _fused_kernel, _fused.py:161-186,230-325.
+ */
+int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype,
+                      const int8_t* lab, const float* M, const float* src,
+                      const float* tgt, const float* step,
+                      float* workspace, float* gM,
+                      float* gsrc, float* gtgt, float* gstep, float* gvol,
+                      int need_gcam,
+                      int B, int N, int D, int H, int W,
+                      int n_samples, int n_classes, int width,
+                      int64_t src_batch_stride, int64_t src_ray_stride,
+                      int64_t tgt_batch_stride, int64_t step_batch_stride,
+                      void* stream);
+
+/*
+ * Label map for the kernels: dst[i] = trunc(src[i]) when that lies in
+ * [0, n_classes), else -1 (a class the march drops). `src` holds the
+ * reference's float label volume.
+ * Replaces the float label read of _fused.py:222-228 (`.to(tl.int32)` of
+ * the nearest float label) with a one-off int8 conversion.
+ */
+int ndrr_labels_to_i8(const float* src, int8_t* dst, int64_t numel,
+                      int n_classes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NANODRR_AMD_H */

@@ -1,0 +1,64 @@
+"""ctypes binding of the C ABI declared in ``include/nanodrr_amd.h``.
+
+The library is built in-tree (``nanodrr_amd/libnanodrr_amd.so``). There is no
+fallback: when it is missing, every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import functools
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().with_name("libnanodrr_amd.so")
+
+NDRR_F32, NDRR_BF16, NDRR_F16 = 0, 1, 2
+MAX_CLASSES = 8
+
+_c_int, _c_i64, _c_p, _c_size = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_size_t
+
+# name -> (restype, argtypes); mirrors include/nanodrr_amd.h
+SIGNATURES: dict[str, tuple] = {
+    "ndrr_version": (_c_int, []),
+    "ndrr_strerror": (ctypes.c_char_p, [_c_int]),
+    "ndrr_raymarch_fwd": (
+        _c_int,
+        [_c_p, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
+         _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+         _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
+    ),
+    "ndrr_bwd_workspace_floats": (_c_size, [_c_int, _c_int, _c_int]),
+    "ndrr_raymarch_bwd": (
+        _c_int,
+        [_c_p, _c_p, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p,
+         _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_int,
+         _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+         _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
+    ),
+    "ndrr_labels_to_i8": (_c_int, [_c_p, _c_p, _c_i64, _c_int, _c_p]),
+}
+
+
+class NativeError(RuntimeError):
+    """A non-zero return code from the native library."""
+
+
+@functools.cache
+def lib() -> ctypes.CDLL:
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"nanodrr_amd native library not found at {LIB_PATH}; build it with "
+            "`python -m nanodrr_amd.build` (hipcc, gfx950). There is no CPU fallback."
+        )
+    handle = ctypes.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(handle, name)
+        fn.restype = res
+        fn.argtypes = args
+    return handle
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().ndrr_strerror(rc).decode()
+        raise NativeError(f"{what} failed with code {rc}: {msg}")

@@ -1,0 +1,565 @@
+// Fused differentiable DRR ray march for AMD Instinct MI355X (gfx950).
+//
+// One thread marches one ray. A 256-thread workgroup covers a 16 x 16
+// detector tile; each of its four wave64s owns an 8 x 8 quadrant, so the 64
+// rays of a wave stay spatially coherent and their trilinear gathers land in
+// few cache lines. Per ray the kernel clips the sample range to the samples
+// whose 2x2x2 footprint can touch the volume (skipped samples contribute
+// exactly zero, so the clip changes no bit of the result), then marches with
+// eight zero-padded corner gathers per sample and a nearest-label route.
+//
+// The arithmetic of the march and of its adjoint follows nanodrr's fused
+// kernel (src/nanodrr/drr/_fused.py:133-300); the launch layout, clip,
+// reductions and compensated accumulation are this library's own.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nanodrr_amd.h"
+
+namespace {
+
+constexpr int kThreads = 256;  // rays per workgroup (16 x 16 tile)
+constexpr int kTile = 16;      // tile edge in detector pixels
+constexpr float kMargin = 1.0f;  // voxels of slack around the footprint box
+constexpr int kKahanMinSamples = 512;
+
+// ---------------------------------------------------------------------------
+// Volume element loads (fp32 math whatever the storage type)
+// ---------------------------------------------------------------------------
+template <int DT> struct Vox;
+template <> struct Vox<NDRR_F32> {
+  using T = float;
+  __device__ static inline float ld(const T* p, int i) { return p[i]; }
+};
+template <> struct Vox<NDRR_BF16> {
+  using T = uint16_t;
+  __device__ static inline float ld(const T* p, int i) {
+    return __uint_as_float(static_cast<uint32_t>(p[i]) << 16);
+  }
+};
+template <> struct Vox<NDRR_F16> {
+  using T = _Float16;
+  __device__ static inline float ld(const T* p, int i) { return static_cast<float>(p[i]); }
+};
+
+struct Params {
+  const void* vol;
+  const int8_t* lab;
+  const float* M;
+  const float* src;
+  const float* tgt;
+  const float* step;
+  const float* go;
+  float* out;
+  float* gpart;  // [B][nblocks][12] pose-gradient partials
+  float* gsrc;
+  float* gtgt;
+  float* gstep;
+  float* gvol;
+  int64_t src_bs, src_rs, tgt_bs, step_bs;
+  int B, N, D, H, W, S, C;
+  int IW, IH, tiles_w, nblocks;
+  float inv_s;   // 1 / (S - 1)
+  float s_m1;    // S - 1
+};
+
+// Ray index of this thread and whether it is a real detector pixel.
+__device__ inline int ray_index(const Params& p, int blk, int tid, bool& valid) {
+  if (p.IW > 0) {
+    const int ty = blk / p.tiles_w;
+    const int tx = blk - ty * p.tiles_w;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    const int px = tx * kTile + (wave & 1) * 8 + (lane & 7);
+    const int py = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
+    valid = (px < p.IW) && (py < p.IH);
+    return py * p.IW + px;
+  }
+  const int n = blk * kThreads + tid;
+  valid = n < p.N;
+  return n;
+}
+
+struct Ray {
+  float sx, sy, sz;     // source, camera space
+  float ex, ey, ez;     // tgt - src, camera space
+  float p0x, p0y, p0z;  // source, voxel space
+  float dx, dy, dz;     // tgt - src, voxel space
+  float step;
+  int s_lo, s_hi;       // samples [s_lo, s_hi) can touch the volume
+};
+
+__device__ inline void axis_range(float p0, float d, float lo, float hi, float& t0, float& t1) {
+  const float sd = fabsf(d) < 1e-12f ? 1e-12f : d;
+  const float a = (lo - p0) / sd;
+  const float b = (hi - p0) / sd;
+  t0 = fminf(a, b);
+  t1 = fmaxf(a, b);
+}
+
+__device__ inline Ray setup_ray(const Params& p, const float* __restrict__ Mb, int b, int n, bool valid) {
+  Ray r;
+  const int nn = valid ? n : 0;
+  const float* sp = p.src + b * p.src_bs + nn * p.src_rs;
+  const float* tp = p.tgt + b * p.tgt_bs + 3 * static_cast<int64_t>(nn);
+  r.sx = sp[0]; r.sy = sp[1]; r.sz = sp[2];
+  const float tx = tp[0], ty = tp[1], tz = tp[2];
+  r.step = p.step[b * p.step_bs + nn];
+  r.ex = tx - r.sx; r.ey = ty - r.sy; r.ez = tz - r.sz;
+  r.p0x = Mb[0] * r.sx + Mb[1] * r.sy + Mb[2] * r.sz + Mb[3];
+  r.p0y = Mb[4] * r.sx + Mb[5] * r.sy + Mb[6] * r.sz + Mb[7];
+  r.p0z = Mb[8] * r.sx + Mb[9] * r.sy + Mb[10] * r.sz + Mb[11];
+  r.dx = Mb[0] * r.ex + Mb[1] * r.ey + Mb[2] * r.ez;
+  r.dy = Mb[4] * r.ex + Mb[5] * r.ey + Mb[6] * r.ez;
+  r.dz = Mb[8] * r.ex + Mb[9] * r.ey + Mb[10] * r.ez;
+
+  float ax0, ax1, ay0, ay1, az0, az1;
+  axis_range(r.p0x, r.dx, -1.0f - kMargin, static_cast<float>(p.W) + kMargin, ax0, ax1);
+  axis_range(r.p0y, r.dy, -1.0f - kMargin, static_cast<float>(p.H) + kMargin, ay0, ay1);
+  axis_range(r.p0z, r.dz, -1.0f - kMargin, static_cast<float>(p.D) + kMargin, az0, az1);
+  const float t_lo = fmaxf(fmaxf(ax0, ay0), az0);
+  const float t_hi = fminf(fminf(ax1, ay1), az1);
+  r.s_lo = 0;
+  r.s_hi = 0;
+  if (valid && t_lo <= t_hi) {
+    // Two samples of slack below, three above: every sample whose footprint
+    // reaches the volume is kept; the extra ones add exact zeros.
+    const float f_lo = fmaxf(floorf(t_lo * p.s_m1) - 2.0f, 0.0f);
+    const float f_hi = fminf(ceilf(t_hi * p.s_m1) + 3.0f, static_cast<float>(p.S));
+    if (f_lo < f_hi) {
+      r.s_lo = static_cast<int>(f_lo);
+      r.s_hi = static_cast<int>(f_hi);
+    }
+  }
+  return r;
+}
+
+// The 2x2x2 stencil of one sample: corner values (zero outside the volume)
+// and fractional offsets. Corner vABC has z offset A, y offset B, x offset C.
+struct Stencil {
+  float v000, v001, v010, v011, v100, v101, v110, v111;
+  float fx, fy, fz;
+  int ix, iy, iz;
+};
+
+template <int DT>
+__device__ inline Stencil gather(const typename Vox<DT>::T* __restrict__ vol, int D, int H, int W,
+                                 float x, float y, float z) {
+  Stencil st;
+  const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
+  st.fx = x - xf; st.fy = y - yf; st.fz = z - zf;
+  st.ix = static_cast<int>(xf); st.iy = static_cast<int>(yf); st.iz = static_cast<int>(zf);
+  const bool bx0 = static_cast<unsigned>(st.ix) < static_cast<unsigned>(W);
+  const bool bx1 = static_cast<unsigned>(st.ix + 1) < static_cast<unsigned>(W);
+  const bool by0 = static_cast<unsigned>(st.iy) < static_cast<unsigned>(H);
+  const bool by1 = static_cast<unsigned>(st.iy + 1) < static_cast<unsigned>(H);
+  const bool bz0 = static_cast<unsigned>(st.iz) < static_cast<unsigned>(D);
+  const bool bz1 = static_cast<unsigned>(st.iz + 1) < static_cast<unsigned>(D);
+  // Clamped addresses keep every load in bounds; the masks zero the padding.
+  const int cx0 = min(max(st.ix, 0), W - 1), cx1 = min(max(st.ix + 1, 0), W - 1);
+  const int cy0 = min(max(st.iy, 0), H - 1), cy1 = min(max(st.iy + 1, 0), H - 1);
+  const int cz0 = min(max(st.iz, 0), D - 1), cz1 = min(max(st.iz + 1, 0), D - 1);
+  const int r00 = (cz0 * H + cy0) * W, r01 = (cz0 * H + cy1) * W;
+  const int r10 = (cz1 * H + cy0) * W, r11 = (cz1 * H + cy1) * W;
+  const float a000 = Vox<DT>::ld(vol, r00 + cx0), a001 = Vox<DT>::ld(vol, r00 + cx1);
+  const float a010 = Vox<DT>::ld(vol, r01 + cx0), a011 = Vox<DT>::ld(vol, r01 + cx1);
+  const float a100 = Vox<DT>::ld(vol, r10 + cx0), a101 = Vox<DT>::ld(vol, r10 + cx1);
+  const float a110 = Vox<DT>::ld(vol, r11 + cx0), a111 = Vox<DT>::ld(vol, r11 + cx1);
+  st.v000 = (bz0 && by0 && bx0) ? a000 : 0.0f;
+  st.v001 = (bz0 && by0 && bx1) ? a001 : 0.0f;
+  st.v010 = (bz0 && by1 && bx0) ? a010 : 0.0f;
+  st.v011 = (bz0 && by1 && bx1) ? a011 : 0.0f;
+  st.v100 = (bz1 && by0 && bx0) ? a100 : 0.0f;
+  st.v101 = (bz1 && by0 && bx1) ? a101 : 0.0f;
+  st.v110 = (bz1 && by1 && bx0) ? a110 : 0.0f;
+  st.v111 = (bz1 && by1 && bx1) ? a111 : 0.0f;
+  return st;
+}
+
+__device__ inline float lerp_value(const Stencil& s) {
+  const float c00 = s.v000 * (1.0f - s.fx) + s.v001 * s.fx;
+  const float c01 = s.v010 * (1.0f - s.fx) + s.v011 * s.fx;
+  const float c10 = s.v100 * (1.0f - s.fx) + s.v101 * s.fx;
+  const float c11 = s.v110 * (1.0f - s.fx) + s.v111 * s.fx;
+  const float c0 = c00 * (1.0f - s.fy) + c01 * s.fy;
+  const float c1 = c10 * (1.0f - s.fy) + c11 * s.fy;
+  return c0 * (1.0f - s.fz) + c1 * s.fz;
+}
+
+// Nearest label (round half to even, zero padded: outside the volume is class 0).
+__device__ inline int nearest_label(const int8_t* __restrict__ lab, int D, int H, int W,
+                                    float x, float y, float z) {
+  const int jx = static_cast<int>(rintf(x));
+  const int jy = static_cast<int>(rintf(y));
+  const int jz = static_cast<int>(rintf(z));
+  const bool inb = static_cast<unsigned>(jx) < static_cast<unsigned>(W) &&
+                   static_cast<unsigned>(jy) < static_cast<unsigned>(H) &&
+                   static_cast<unsigned>(jz) < static_cast<unsigned>(D);
+  const int cx = min(max(jx, 0), W - 1), cy = min(max(jy, 0), H - 1), cz = min(max(jz, 0), D - 1);
+  const int l = static_cast<int>(lab[(cz * H + cy) * W + cx]);
+  return inb ? l : 0;
+}
+
+// Neumaier-compensated accumulation: sum + comp carries ~2x fp32 precision.
+__device__ inline void comp_add(float& sum, float& comp, float x) {
+  const float t = sum + x;
+  comp += (fabsf(sum) >= fabsf(x)) ? ((sum - t) + x) : ((x - t) + sum);
+  sum = t;
+}
+
+// ---------------------------------------------------------------------------
+// Forward
+// ---------------------------------------------------------------------------
+// CV == 1: single channel (no label map). CV == 8: label map, runtime C <= 8.
+template <int DT, int CV, bool KAHAN>
+__global__ __launch_bounds__(kThreads) void k_fwd(Params p) {
+  const int b = blockIdx.y;
+  bool valid;
+  const int n = ray_index(p, blockIdx.x, threadIdx.x, valid);
+  const float* __restrict__ Mb = p.M + 12 * b;
+  const Ray r = setup_ray(p, Mb, b, n, valid);
+  const auto* __restrict__ vol = static_cast<const typename Vox<DT>::T*>(p.vol);
+
+  float acc[CV];
+  float comp[CV];
+#pragma unroll
+  for (int c = 0; c < CV; ++c) { acc[c] = 0.0f; comp[c] = 0.0f; }
+
+  for (int s = r.s_lo; s < r.s_hi; ++s) {
+    const float t = static_cast<float>(s) * p.inv_s;
+    const float x = r.p0x + t * r.dx;
+    const float y = r.p0y + t * r.dy;
+    const float z = r.p0z + t * r.dz;
+    const Stencil st = gather<DT>(vol, p.D, p.H, p.W, x, y, z);
+    const float val = lerp_value(st);
+    if constexpr (CV == 1) {
+      if constexpr (KAHAN) comp_add(acc[0], comp[0], val); else acc[0] += val;
+    } else {
+      const int cls = nearest_label(p.lab, p.D, p.H, p.W, x, y, z);
+#pragma unroll
+      for (int c = 0; c < CV; ++c) {
+        const float contrib = (cls == c) ? val : 0.0f;
+        if constexpr (KAHAN) comp_add(acc[c], comp[c], contrib); else acc[c] += contrib;
+      }
+    }
+  }
+  if (!valid) return;
+  float* o = p.out + (static_cast<int64_t>(b) * p.C) * p.N + n;
+#pragma unroll
+  for (int c = 0; c < CV; ++c) {
+    if (c < p.C) {
+      const float total = KAHAN ? (acc[c] + comp[c]) : acc[c];
+      o[static_cast<int64_t>(c) * p.N] = total * r.step;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Backward (analytic adjoint)
+// ---------------------------------------------------------------------------
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <int DT, int CV, bool GCAM, bool GVOL>
+__global__ __launch_bounds__(kThreads) void k_bwd(Params p) {
+  const int b = blockIdx.y;
+  bool valid;
+  const int n = ray_index(p, blockIdx.x, threadIdx.x, valid);
+  const float* __restrict__ Mb = p.M + 12 * b;
+  const Ray r = setup_ray(p, Mb, b, n, valid);
+  const auto* __restrict__ vol = static_cast<const typename Vox<DT>::T*>(p.vol);
+  const float m00 = Mb[0], m01 = Mb[1], m02 = Mb[2];
+  const float m10 = Mb[4], m11 = Mb[5], m12 = Mb[6];
+  const float m20 = Mb[8], m21 = Mb[9], m22 = Mb[10];
+
+  // Upstream gradient of this ray, one value per class.
+  float goc[CV];
+  {
+    const int nn = valid ? n : 0;
+    const float* g = p.go + (static_cast<int64_t>(b) * p.C) * p.N + nn;
+#pragma unroll
+    for (int c = 0; c < CV; ++c)
+      goc[c] = (valid && c < p.C) ? g[static_cast<int64_t>(c) * p.N] : 0.0f;
+  }
+
+  float a00 = 0.f, a01 = 0.f, a02 = 0.f, a03 = 0.f;
+  float a10 = 0.f, a11 = 0.f, a12 = 0.f, a13 = 0.f;
+  float a20 = 0.f, a21 = 0.f, a22 = 0.f, a23 = 0.f;
+  float gsx = 0.f, gsy = 0.f, gsz = 0.f, gtx = 0.f, gty = 0.f, gtz = 0.f, astep = 0.f;
+
+  for (int s = r.s_lo; s < r.s_hi; ++s) {
+    const float t = static_cast<float>(s) * p.inv_s;
+    const float x = r.p0x + t * r.dx;
+    const float y = r.p0y + t * r.dy;
+    const float z = r.p0z + t * r.dz;
+    const Stencil st = gather<DT>(vol, p.D, p.H, p.W, x, y, z);
+
+    float gg;
+    if constexpr (CV == 1) {
+      gg = goc[0];
+    } else {
+      const int cls = nearest_label(p.lab, p.D, p.H, p.W, x, y, z);
+      gg = 0.0f;
+#pragma unroll
+      for (int c = 0; c < CV; ++c) gg = (cls == c) ? goc[c] : gg;
+    }
+    const float g = gg * r.step;
+    const float fx = st.fx, fy = st.fy, fz = st.fz;
+
+    const float dvx = ((st.v001 - st.v000) * (1.0f - fy) + (st.v011 - st.v010) * fy) * (1.0f - fz) +
+                      ((st.v101 - st.v100) * (1.0f - fy) + (st.v111 - st.v110) * fy) * fz;
+    const float dvy = ((st.v010 - st.v000) * (1.0f - fx) + (st.v011 - st.v001) * fx) * (1.0f - fz) +
+                      ((st.v110 - st.v100) * (1.0f - fx) + (st.v111 - st.v101) * fx) * fz;
+    const float dvz = ((st.v100 - st.v000) * (1.0f - fx) + (st.v101 - st.v001) * fx) * (1.0f - fy) +
+                      ((st.v110 - st.v010) * (1.0f - fx) + (st.v111 - st.v011) * fx) * fy;
+    const float gx = g * dvx, gy = g * dvy, gz = g * dvz;
+
+    const float qx = r.sx + t * r.ex;
+    const float qy = r.sy + t * r.ey;
+    const float qz = r.sz + t * r.ez;
+    a00 += gx * qx; a01 += gx * qy; a02 += gx * qz; a03 += gx;
+    a10 += gy * qx; a11 += gy * qy; a12 += gy * qz; a13 += gy;
+    a20 += gz * qx; a21 += gz * qy; a22 += gz * qz; a23 += gz;
+
+    if constexpr (GCAM) {
+      const float gqx = m00 * gx + m10 * gy + m20 * gz;
+      const float gqy = m01 * gx + m11 * gy + m21 * gz;
+      const float gqz = m02 * gx + m12 * gy + m22 * gz;
+      gsx += (1.0f - t) * gqx; gsy += (1.0f - t) * gqy; gsz += (1.0f - t) * gqz;
+      gtx += t * gqx; gty += t * gqy; gtz += t * gqz;
+      astep += gg * lerp_value(st);
+    }
+    if constexpr (GVOL) {
+      if (g != 0.0f) {
+        const float wx[2] = {1.0f - fx, fx};
+        const float wy[2] = {1.0f - fy, fy};
+        const float wz[2] = {1.0f - fz, fz};
+#pragma unroll
+        for (int corner = 0; corner < 8; ++corner) {
+          const int cx = corner & 1, cy = (corner >> 1) & 1, cz = corner >> 2;
+          const int jx = st.ix + cx, jy = st.iy + cy, jz = st.iz + cz;
+          const bool inb = static_cast<unsigned>(jx) < static_cast<unsigned>(p.W) &&
+                           static_cast<unsigned>(jy) < static_cast<unsigned>(p.H) &&
+                           static_cast<unsigned>(jz) < static_cast<unsigned>(p.D);
+          if (inb) unsafeAtomicAdd(p.gvol + (jz * p.H + jy) * p.W + jx, g * (wx[cx] * wy[cy] * wz[cz]));
+        }
+      }
+    }
+  }
+
+  if constexpr (GCAM) {
+    if (valid) {
+      const int64_t ray = static_cast<int64_t>(b) * p.N + n;
+      p.gsrc[3 * ray + 0] = gsx; p.gsrc[3 * ray + 1] = gsy; p.gsrc[3 * ray + 2] = gsz;
+      p.gtgt[3 * ray + 0] = gtx; p.gtgt[3 * ray + 1] = gty; p.gtgt[3 * ray + 2] = gtz;
+      p.gstep[ray] = astep;
+    }
+  }
+
+  // Deterministic per-tile pose-gradient partials: butterfly within each
+  // wave, then the four wave sums in fixed order.
+  __shared__ float red[kThreads / 64][12];
+  float a[12] = {a00, a01, a02, a03, a10, a11, a12, a13, a20, a21, a22, a23};
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int j = 0; j < 12; ++j) {
+    const float v = wave_sum(a[j]);
+    if (lane == 0) red[wave][j] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 12) {
+    const int j = threadIdx.x;
+    float v = red[0][j];
+#pragma unroll
+    for (int w = 1; w < kThreads / 64; ++w) v += red[w][j];
+    p.gpart[(static_cast<int64_t>(b) * p.nblocks + blockIdx.x) * 12 + j] = v;
+  }
+}
+
+// gM[b][j] = sum over tiles of gpart[b][tile][j], in tile order (compensated).
+__global__ void k_reduce_pose(const float* __restrict__ part, float* __restrict__ gM, int nblocks) {
+  const int b = blockIdx.x;
+  const int j = threadIdx.x;
+  if (j >= 12) return;
+  const float* pp = part + static_cast<int64_t>(b) * nblocks * 12 + j;
+  float sum = 0.0f, comp = 0.0f;
+  for (int k = 0; k < nblocks; ++k) comp_add(sum, comp, pp[static_cast<int64_t>(k) * 12]);
+  gM[b * 12 + j] = sum + comp;
+}
+
+__global__ void k_labels_to_i8(const float* __restrict__ src, int8_t* __restrict__ dst, int64_t numel, int C) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= numel) return;
+  const float t = truncf(src[i]);
+  dst[i] = (t >= 0.0f && t < static_cast<float>(C)) ? static_cast<int8_t>(t) : static_cast<int8_t>(-1);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+int nblocks_of(int N, int width, int& tiles_w, int& IH) {
+  if (width > 0) {
+    IH = N / width;
+    tiles_w = (width + kTile - 1) / kTile;
+    return tiles_w * ((IH + kTile - 1) / kTile);
+  }
+  tiles_w = 0;
+  IH = 0;
+  return (N + kThreads - 1) / kThreads;
+}
+
+int check_common(const void* vol, int vol_dtype, const int8_t* lab, const float* M, const float* src,
+                 const float* tgt, const float* step, int B, int N, int D, int H, int W, int S, int C,
+                 int width) {
+  if (!vol || !M || !src || !tgt || !step) return NDRR_EINVAL;
+  if (B <= 0 || N <= 0 || D <= 0 || H <= 0 || W <= 0 || S < 2 || width < 0) return NDRR_EINVAL;
+  if (width > 0 && N % width != 0) return NDRR_EINVAL;
+  if (vol_dtype != NDRR_F32 && vol_dtype != NDRR_BF16 && vol_dtype != NDRR_F16) return NDRR_EDTYPE;
+  if (C < 1 || C > NDRR_MAX_CLASSES) return NDRR_ECLASSES;
+  if (C > 1 && !lab) return NDRR_EINVAL;
+  if (B > 65535) return NDRR_ERANGE;
+  const int64_t vox = static_cast<int64_t>(D) * H * W;
+  if (vox >= (int64_t{1} << 31) || static_cast<int64_t>(C) * B * N >= (int64_t{1} << 31) ||
+      3 * static_cast<int64_t>(B) * N >= (int64_t{1} << 31))
+    return NDRR_ERANGE;
+  return NDRR_OK;
+}
+
+Params make_params(const void* vol, const int8_t* lab, const float* M, const float* src, const float* tgt,
+                   const float* step, int B, int N, int D, int H, int W, int S, int C, int width,
+                   int64_t src_bs, int64_t src_rs, int64_t tgt_bs, int64_t step_bs) {
+  Params p{};
+  p.vol = vol; p.lab = lab; p.M = M; p.src = src; p.tgt = tgt; p.step = step;
+  p.src_bs = src_bs; p.src_rs = src_rs; p.tgt_bs = tgt_bs; p.step_bs = step_bs;
+  p.B = B; p.N = N; p.D = D; p.H = H; p.W = W; p.S = S; p.C = C;
+  p.IW = width;
+  p.nblocks = nblocks_of(N, width, p.tiles_w, p.IH);
+  p.inv_s = 1.0f / static_cast<float>(S - 1);
+  p.s_m1 = static_cast<float>(S - 1);
+  return p;
+}
+
+int launch_status() {
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? NDRR_OK : static_cast<int>(e);
+}
+
+template <int DT, int CV>
+void launch_fwd_dt(const Params& p, dim3 grid, hipStream_t st) {
+  if (p.S >= kKahanMinSamples)
+    hipLaunchKernelGGL((k_fwd<DT, CV, true>), grid, dim3(kThreads), 0, st, p);
+  else
+    hipLaunchKernelGGL((k_fwd<DT, CV, false>), grid, dim3(kThreads), 0, st, p);
+}
+
+template <int CV>
+void launch_fwd_cv(const Params& p, int dt, dim3 grid, hipStream_t st) {
+  switch (dt) {
+    case NDRR_F32: launch_fwd_dt<NDRR_F32, CV>(p, grid, st); break;
+    case NDRR_BF16: launch_fwd_dt<NDRR_BF16, CV>(p, grid, st); break;
+    default: launch_fwd_dt<NDRR_F16, CV>(p, grid, st); break;
+  }
+}
+
+template <int DT, int CV>
+void launch_bwd_dt(const Params& p, bool gcam, bool gvol, dim3 grid, hipStream_t st) {
+  if (gcam && gvol) hipLaunchKernelGGL((k_bwd<DT, CV, true, true>), grid, dim3(kThreads), 0, st, p);
+  else if (gcam) hipLaunchKernelGGL((k_bwd<DT, CV, true, false>), grid, dim3(kThreads), 0, st, p);
+  else if (gvol) hipLaunchKernelGGL((k_bwd<DT, CV, false, true>), grid, dim3(kThreads), 0, st, p);
+  else hipLaunchKernelGGL((k_bwd<DT, CV, false, false>), grid, dim3(kThreads), 0, st, p);
+}
+
+template <int CV>
+void launch_bwd_cv(const Params& p, int dt, bool gcam, bool gvol, dim3 grid, hipStream_t st) {
+  switch (dt) {
+    case NDRR_F32: launch_bwd_dt<NDRR_F32, CV>(p, gcam, gvol, grid, st); break;
+    case NDRR_BF16: launch_bwd_dt<NDRR_BF16, CV>(p, gcam, gvol, grid, st); break;
+    default: launch_bwd_dt<NDRR_F16, CV>(p, gcam, gvol, grid, st); break;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ndrr_version(void) { return 100; }
+
+const char* ndrr_strerror(int code) {
+  switch (code) {
+    case NDRR_OK: return "success";
+    case NDRR_EINVAL: return "invalid argument (null pointer, non-positive size, n_samples < 2 or bad width)";
+    case NDRR_EDTYPE: return "unsupported volume element type";
+    case NDRR_ECLASSES: return "n_classes outside [1, 8]";
+    case NDRR_ERANGE: return "a size exceeds the kernel's 32-bit index range";
+    default: break;
+  }
+  if (code > 0) return hipGetErrorString(static_cast<hipError_t>(code));
+  return "unknown error";
+}
+
+int ndrr_raymarch_fwd(const void* vol, int vol_dtype, const int8_t* lab, const float* M, const float* src,
+                      const float* tgt, const float* step, float* out, int B, int N, int D, int H, int W,
+                      int n_samples, int n_classes, int width, int64_t src_batch_stride,
+                      int64_t src_ray_stride, int64_t tgt_batch_stride, int64_t step_batch_stride,
+                      void* stream) {
+  const int rc = check_common(vol, vol_dtype, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width);
+  if (rc != NDRR_OK) return rc;
+  if (!out) return NDRR_EINVAL;
+  Params p = make_params(vol, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width,
+                         src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride);
+  p.out = out;
+  const dim3 grid(p.nblocks, B);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (n_classes == 1) launch_fwd_cv<1>(p, vol_dtype, grid, st);
+  else launch_fwd_cv<8>(p, vol_dtype, grid, st);
+  return launch_status();
+}
+
+size_t ndrr_bwd_workspace_floats(int B, int N, int width) {
+  if (B <= 0 || N <= 0 || width < 0) return 0;
+  int tw, ih;
+  return static_cast<size_t>(B) * nblocks_of(N, width, tw, ih) * 12;
+}
+
+int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, const int8_t* lab, const float* M,
+                      const float* src, const float* tgt, const float* step, float* workspace, float* gM,
+                      float* gsrc, float* gtgt, float* gstep, float* gvol, int need_gcam, int B, int N, int D,
+                      int H, int W, int n_samples, int n_classes, int width, int64_t src_batch_stride,
+                      int64_t src_ray_stride, int64_t tgt_batch_stride, int64_t step_batch_stride,
+                      void* stream) {
+  const int rc = check_common(vol, vol_dtype, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width);
+  if (rc != NDRR_OK) return rc;
+  if (!go || !workspace || !gM) return NDRR_EINVAL;
+  if (need_gcam && (!gsrc || !gtgt || !gstep)) return NDRR_EINVAL;
+  Params p = make_params(vol, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width,
+                         src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride);
+  p.go = go;
+  p.gpart = workspace;
+  p.gsrc = gsrc; p.gtgt = gtgt; p.gstep = gstep; p.gvol = gvol;
+  const dim3 grid(p.nblocks, B);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool gcam = need_gcam != 0, gv = gvol != nullptr;
+  if (n_classes == 1) launch_bwd_cv<1>(p, vol_dtype, gcam, gv, grid, st);
+  else launch_bwd_cv<8>(p, vol_dtype, gcam, gv, grid, st);
+  int r = launch_status();
+  if (r != NDRR_OK) return r;
+  hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(64), 0, st, workspace, gM, p.nblocks);
+  return launch_status();
+}
+
+int ndrr_labels_to_i8(const float* src, int8_t* dst, int64_t numel, int n_classes, void* stream) {
+  if (!src || !dst || numel < 0) return NDRR_EINVAL;
+  if (n_classes < 1 || n_classes > 127) return NDRR_ECLASSES;
+  if (numel == 0) return NDRR_OK;
+  const int64_t blocks = (numel + kThreads - 1) / kThreads;
+  if (blocks > 0x7fffffff) return NDRR_ERANGE;
+  hipLaunchKernelGGL(k_labels_to_i8, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), 0,
+                     static_cast<hipStream_t>(stream), src, dst, numel, n_classes);
+  return launch_status();
+}
+
+}  // extern "C"

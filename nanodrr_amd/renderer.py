@@ -1,0 +1,121 @@
+"""`render(...)`: differentiable DRRs through the native HIP ray march.
+
+Same call signature, ray construction and errors as nanodrr's renderer
+(src/nanodrr/drr/renderer.py:21-132) and fused backend
+(src/nanodrr/drr/backends.py:66-117). Backends: "hip" (the native kernels)
+and "auto" (= "hip"); "triton" is accepted as an alias of "hip" so callers of
+the reference's fused backend switch unchanged. The reference's pure-PyTorch
+`grid_sample` backend is not part of this package: "torch" raises.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from ._ops import fused_raymarch
+
+_FUSED_DTYPES = (torch.float32, torch.float16, torch.bfloat16)
+_BACKENDS = ("auto", "hip", "triton")
+
+
+def fused_supported(subject, B: int, n_pixels: int) -> bool:
+    """Hard limits of the kernels: one volume, 32-bit indexing, <= 8 classes (backends.py:66-69)."""
+    N = B * n_pixels
+    return (
+        subject.image.shape[0] == 1
+        and max(subject.image.numel(), subject.n_classes * N, 3 * N) < 2**31
+        and 1 <= subject.n_classes <= _lib.MAX_CLASSES
+        and B <= 65535
+    )
+
+
+def _make_tgt(k_inv, sdd, height, width, device, dtype) -> torch.Tensor:
+    """tgt[b, n] = sdd_b * K^-1_b [u + 1/2, v + 1/2, 1], n = v * width + u (renderer.py:101-117)."""
+    v, u = torch.meshgrid(
+        torch.arange(height, device=device, dtype=dtype) + 0.5,
+        torch.arange(width, device=device, dtype=dtype) + 0.5,
+        indexing="ij",
+    )
+    uv1 = torch.stack([u, v, torch.ones_like(u)], dim=-1).reshape(height * width, 3)
+    return sdd[:, None, None] * torch.einsum("bij,nj->bni", k_inv, uv1)
+
+
+def _make_src(orthographic: bool, tgt: torch.Tensor, sdd: torch.Tensor) -> torch.Tensor:
+    """Perspective: one source at the camera origin per pose, (B, 1, 3).
+    Orthographic: one source per ray, sdd behind its target (renderer.py:120-132)."""
+    if orthographic:
+        src = tgt.clone()
+        src[..., 2] -= sdd[:, None]
+        return src
+    return torch.zeros(tgt.shape[0], 1, 3, device=tgt.device, dtype=tgt.dtype)
+
+
+def render(
+    subject,
+    k_inv: torch.Tensor,
+    rt_inv: torch.Tensor,
+    sdd: torch.Tensor,
+    height: int,
+    width: int,
+    n_samples: int = 500,
+    orthographic: bool = False,
+    src: torch.Tensor | None = None,
+    tgt: torch.Tensor | None = None,
+    backend: str = "auto",
+) -> torch.Tensor:
+    """Differentiable ray marching through `subject.image` (and its label map).
+
+    Returns (B, C, H, W) fp32 line integrals, one channel per label class.
+    Gradients flow to `rt_inv`, `k_inv`, `sdd`, `src`, `tgt` and, when it
+    requires grad, the volume.
+    """
+    if n_samples < 2:
+        raise ValueError("n_samples must be at least 2")
+    if backend not in _BACKENDS:
+        if backend == "torch":
+            raise ValueError(
+                "backend='torch' (the grid_sample path) is not part of nanodrr_amd; "
+                "use backend='hip' (or 'auto')"
+            )
+        raise ValueError(f"Unknown backend {backend!r}; expected 'auto', 'hip' or 'triton'")
+    device, dtype = rt_inv.device, rt_inv.dtype
+    if tgt is None:
+        tgt = _make_tgt(k_inv, sdd, height, width, device, dtype)
+    if src is None:
+        src = _make_src(orthographic, tgt, sdd)
+    step_size = (tgt - src).norm(dim=-1) / float(n_samples - 1)
+    return render_hip(subject, rt_inv, src, tgt, step_size, n_samples, height, width)
+
+
+def render_hip(subject, rt_inv, src, tgt, step_size, n_samples, height, width) -> torch.Tensor:
+    """Native backend (the counterpart of render_fused, backends.py:72-117)."""
+    image = subject.image
+    if not (rt_inv.is_cuda and image.is_cuda):
+        raise ValueError("the hip backend renders on a GPU: move the subject and poses to a cuda device")
+    if image.dtype not in _FUSED_DTYPES:
+        raise ValueError(f"volume dtype {image.dtype} unsupported; expected float32, float16 or bfloat16")
+    C = subject.n_classes
+    vol = image[0, 0]
+    if not vol.is_contiguous():
+        vol = vol.contiguous()
+    lab = subject.label_i8 if C > 1 else vol.new_empty(0, dtype=torch.int8)
+    # Camera -> voxel-index affine, composed in float64 and rounded once.
+    M = (subject.world_to_voxel.to(torch.float64) @ rt_inv.to(torch.float64))[:, :3, :].to(torch.float32)
+    B = max(M.shape[0], tgt.shape[0], src.shape[0])
+    if M.shape[0] != B:
+        M = M.expand(B, -1, -1)
+    if not fused_supported(subject, B, height * width):
+        raise ValueError("inputs exceed the hip kernel's limits (one volume, 32-bit indexing, <= 8 classes)")
+    out = fused_raymarch(
+        vol,
+        lab,
+        M.contiguous(),
+        src.to(torch.float32).contiguous(),
+        tgt.to(torch.float32).contiguous(),
+        step_size.to(torch.float32).contiguous(),
+        n_samples,
+        C,
+        width,
+    )
+    return out.reshape(B, C, height, width)

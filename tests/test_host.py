@@ -1,0 +1,130 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports what
+include/*.h declares, argument errors come back before any launch, and the
+render() front end validates like the reference (tests/drr/test_fused.py:229-251)."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+import torch
+
+from nanodrr_amd import _lib, render
+from nanodrr_amd.camera import make_k_inv, make_rt_inv
+from tests.helpers import random_subject
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared_symbols():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names.update(re.findall(r"\b(ndrr_\w+)\s*\(", text))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = _declared_symbols()
+    for n in ("ndrr_raymarch_fwd", "ndrr_raymarch_bwd", "ndrr_labels_to_i8", "ndrr_strerror"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.lib()
+    for name in _declared_symbols():
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+
+
+def test_version_and_error_strings():
+    lib = _lib.lib()
+    assert lib.ndrr_version() >= 100
+    for code in (0, -1, -2, -3, -4):
+        assert lib.ndrr_strerror(code)
+    assert b"invalid" in lib.ndrr_strerror(-1)
+
+
+def test_workspace_size_is_12_floats_per_tile():
+    lib = _lib.lib()
+    # 200 x 200 detector: 13 x 13 tiles of 16 x 16
+    assert lib.ndrr_bwd_workspace_floats(16, 40000, 200) == 16 * 169 * 12
+    assert lib.ndrr_bwd_workspace_floats(2, 1000, 0) == 2 * 4 * 12  # runs of 256 rays
+    assert lib.ndrr_bwd_workspace_floats(0, 10, 0) == 0
+
+
+def _fwd(lib, **kw):
+    args = dict(vol=1, dt=0, lab=None, M=1, src=1, tgt=1, step=1, out=1, B=1, N=16, D=4, H=4, W=4, S=8, C=1,
+                width=4, sbs=0, srs=0, tbs=0, pbs=0, stream=None)
+    args.update(kw)
+    return lib.ndrr_raymarch_fwd(*args.values())
+
+
+def test_argument_errors_return_before_launch():
+    """Invalid arguments are rejected on the host (no pointer is dereferenced)."""
+    lib = _lib.lib()
+    assert _fwd(lib, vol=None) == -1
+    assert _fwd(lib, S=1) == -1
+    assert _fwd(lib, width=5) == -1  # 16 rays are not a multiple of width 5
+    assert _fwd(lib, dt=7) == -2
+    assert _fwd(lib, C=9) == -3
+    assert _fwd(lib, C=3, lab=None) == -1
+    assert _fwd(lib, B=70000) == -4
+    assert _fwd(lib, D=2048, H=2048, W=1024) == -4
+    assert lib.ndrr_labels_to_i8(None, None, 10, 2, None) == -1
+    assert lib.ndrr_labels_to_i8(1, 1, 10, 0, None) == -3
+
+
+def test_check_raises_native_error():
+    with pytest.raises(_lib.NativeError, match="n_classes"):
+        _lib.check(-3, "probe")
+
+
+def _camera():
+    k_inv = make_k_inv(100.0, 1.0, 1.0, 0.0, 0.0, 16, 16)
+    rt_inv = make_rt_inv(torch.tensor([[5.0, -3.0, 8.0]]), torch.tensor([[1.0, 80.0, -2.0]]), "AP", torch.zeros(3))
+    return k_inv, rt_inv, torch.tensor([100.0]), 16, 16
+
+
+def test_invalid_n_samples_raises():
+    subject = random_subject()
+    k_inv, rt_inv, sdd, h, w = _camera()
+    with pytest.raises(ValueError, match="n_samples"):
+        render(subject, k_inv, rt_inv, sdd, h, w, n_samples=1)
+
+
+def test_unknown_backend_raises():
+    subject = random_subject()
+    k_inv, rt_inv, sdd, h, w = _camera()
+    with pytest.raises(ValueError, match="backend"):
+        render(subject, k_inv, rt_inv, sdd, h, w, backend="cuda")
+
+
+def test_torch_backend_is_not_shipped():
+    subject = random_subject()
+    k_inv, rt_inv, sdd, h, w = _camera()
+    with pytest.raises(ValueError, match="not part of nanodrr_amd"):
+        render(subject, k_inv, rt_inv, sdd, h, w, backend="torch")
+
+
+def test_cpu_tensors_fail_loudly():
+    """No silent CPU fallback: host tensors are an error, not a slow path."""
+    subject = random_subject()
+    k_inv, rt_inv, sdd, h, w = _camera()
+    for backend in ("auto", "hip", "triton"):
+        with pytest.raises(ValueError, match="GPU"):
+            render(subject, k_inv, rt_inv, sdd, h, w, n_samples=64, backend=backend)
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "LIB_PATH", tmp_path / "libnanodrr_amd.so")
+    _lib.lib.cache_clear()
+    try:
+        with pytest.raises(RuntimeError, match="no CPU fallback"):
+            _lib.lib()
+    finally:
+        monkeypatch.undo()
+        _lib.lib.cache_clear()
+    assert isinstance(_lib.lib(), ctypes.CDLL)
