@@ -129,11 +129,11 @@ def fused_raymarch_bwd(
     lib = _lib.lib()
     ws = torch.empty(lib.ndrr_bwd_workspace_floats(B, N, width), device=dev, dtype=torch.float32)
     gM = torch.empty(B, 3, 4, device=dev, dtype=torch.float32)
-    empty = gM.new_empty(0)
-    gvol = torch.zeros(vol.shape, device=dev, dtype=torch.float32) if need_gvol else empty
-    gsrc = torch.empty(B, N, 3, device=dev, dtype=torch.float32) if need_gcam else empty
-    gtgt = torch.empty(B, N, 3, device=dev, dtype=torch.float32) if need_gcam else empty
-    gstep = torch.empty(B, N, device=dev, dtype=torch.float32) if need_gcam else empty
+    # unused outputs are distinct empty tensors: custom-op outputs may not alias
+    gvol = torch.zeros(vol.shape, device=dev, dtype=torch.float32) if need_gvol else gM.new_empty(0)
+    gsrc = torch.empty(B, N, 3, device=dev, dtype=torch.float32) if need_gcam else gM.new_empty(0)
+    gtgt = torch.empty(B, N, 3, device=dev, dtype=torch.float32) if need_gcam else gM.new_empty(0)
+    gstep = torch.empty(B, N, device=dev, dtype=torch.float32) if need_gcam else gM.new_empty(0)
     with torch.cuda.device(dev):
         rc = lib.ndrr_raymarch_bwd(
             go.data_ptr(), vol.data_ptr(), _DTYPE_CODE[vol.dtype], _ptr(lab) if n_classes > 1 else None,
