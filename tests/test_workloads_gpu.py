@@ -1,0 +1,217 @@
+"""Parity at the benchmark workloads' real sizes (GPU).
+
+The oracle checks sampled rays (rays are independent, so a subset is an
+exact check of those rays); size-independent properties cover the rest:
+exact linearity in the volume, per-pose independence of the batch, the
+channel-sum invariant and the adjoint identity of the volume gradient.
+Errors of the bf16/fp16 volumes are stated against the float64 oracle.
+"""
+
+from __future__ import annotations
+
+import functools
+
+import numpy as np
+import pytest
+import torch
+
+from nanodrr_amd import render
+from nanodrr_amd._ops import fused_raymarch, fused_raymarch_bwd
+from nanodrr_amd.phantom import detector, make_phantom, phantom_subject, sample_poses
+from nanodrr_amd.renderer import _make_src, _make_tgt
+from oracle import raymarch as orm
+from tests.helpers import rel_to_max
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda")
+
+
+@functools.cache
+def _phantom(shape, spacing, seed=0):
+    return make_phantom(shape, spacing, seed=seed)
+
+
+def _workload(shape, spacing, n_classes, batch, det, det_spacing=1.0, dtype=torch.float32, pose_seed=1):
+    ph = _phantom(shape, spacing)
+    subject = phantom_subject(ph, n_classes=n_classes, dtype=dtype, device=DEV)
+    rt_inv, _, _ = sample_poses(batch, seed=pose_seed, isocenter=subject.isocenter)
+    k_inv, sdd = detector(1020.0, det_spacing, det, det)
+    return ph, subject, k_inv.to(DEV), rt_inv.to(DEV), sdd.to(DEV)
+
+
+def _op_inputs(subject, k_inv, rt_inv, sdd, det, S):
+    """The fp32 tensors render() hands to the op (renderer.render_hip)."""
+    tgt = _make_tgt(k_inv, sdd, det, det, DEV, torch.float32)
+    src = _make_src(False, tgt, sdd)
+    step = (tgt - src).norm(dim=-1) / float(S - 1)
+    M = (subject.world_to_voxel.double() @ rt_inv.double())[:, :3, :].float().contiguous()
+    return M, src.contiguous(), tgt.contiguous(), step.contiguous()
+
+
+def _oracle_vol(ph, n_classes, dtype=torch.float32):
+    vol = ph.mu.to(dtype).double().numpy()
+    lab = ph.labels.long().clamp(max=n_classes - 1).numpy() if n_classes > 1 else None
+    return vol, lab
+
+
+def _np(t):
+    return t.detach().double().cpu().numpy()
+
+
+def _subset(tgt, step, rays):
+    idx = torch.as_tensor(rays, device=DEV)
+    return tgt[:, idx].contiguous(), step[:, idx].contiguous()
+
+
+def _pick_rays(n, k, seed):
+    return np.sort(np.random.default_rng(seed).choice(n, size=k, replace=False))
+
+
+def _check_channels(ref, out, n_classes, tol):
+    scale = np.abs(ref).max()
+    if n_classes == 1:
+        assert np.abs(ref - out).max() / scale < tol
+        return
+    assert np.abs(ref.sum(1) - out.sum(1)).max() / scale < tol
+    bad = (np.abs(ref - out).max(axis=1) / scale) > tol
+    assert bad.mean() < 2e-3, bad.mean()
+
+
+# --- 128^3 @ 1.5 mm, one pose, 200 x 200, S = 256 (the CPU-runnable workload) ---------------------
+
+def test_small_workload_forward_and_pose_gradient():
+    ph, subject, k_inv, rt_inv, sdd = _workload((128,) * 3, (1.5,) * 3, 1, 1, 200)
+    M, src, tgt, step = _op_inputs(subject, k_inv, rt_inv, sdd, 200, 256)
+    out = fused_raymarch(subject.image[0, 0], subject.label_i8.new_empty(0), M, src, tgt, step, 256, 1, 200)
+    vol, _ = _oracle_vol(ph, 1)
+    rays = _pick_rays(40000, 4000, 0)
+    ref = orm.forward(vol, None, 1, _np(M), _np(src), _np(tgt), _np(step), 256, rays=rays)
+    assert rel_to_max(ref, _np(out)[:, :, rays]) < 1e-5
+    # pose gradient of a ray subset rendered on its own (width 0: runs of 256 rays)
+    tsub, ssub = _subset(tgt, step, rays)
+    go = torch.randn(1, 1, len(rays), generator=torch.Generator().manual_seed(1)).to(DEV)
+    _, gM, gsrc, gtgt, gstep = fused_raymarch_bwd(go, subject.image[0, 0], subject.label_i8.new_empty(0), M, src,
+                                                  tsub, ssub, 256, 1, 0, False, True)
+    g = orm.backward(_np(go), vol, None, 1, _np(M), _np(src), _np(tsub), _np(ssub), 256, need_gvol=False)
+    assert rel_to_max(g["gM"], _np(gM)) < 1e-4
+    assert rel_to_max(g["gtgt"], _np(gtgt)) < 1e-4
+    assert rel_to_max(g["gstep"], _np(gstep)) < 1e-4
+
+
+# --- 256^3 @ 1 mm, 4 labels, 16 poses, 200 x 200, S = 256 (the benchmark workload) ------------------
+
+@pytest.fixture(scope="module")
+def bench_fp32():
+    return _workload((256,) * 3, (1.0,) * 3, 4, 16, 200)
+
+
+def test_bench_workload_forward_sampled_rays(bench_fp32):
+    ph, subject, k_inv, rt_inv, sdd = bench_fp32
+    M, src, tgt, step = _op_inputs(subject, k_inv, rt_inv, sdd, 200, 256)
+    out = fused_raymarch(subject.image[0, 0], subject.label_i8, M, src, tgt, step, 256, 4, 200)
+    vol, lab = _oracle_vol(ph, 4)
+    rays = _pick_rays(40000, 500, 2)
+    ref = orm.forward(vol, lab, 4, _np(M), _np(src), _np(tgt), _np(step), 256, rays=rays)
+    _check_channels(ref, _np(out)[:, :, rays], 4, 1e-5)
+
+
+def test_bench_workload_pose_gradient_ray_subset(bench_fp32):
+    ph, subject, k_inv, rt_inv, sdd = bench_fp32
+    M, src, tgt, step = _op_inputs(subject, k_inv, rt_inv, sdd, 200, 256)
+    M = M[:4].contiguous()
+    rays = _pick_rays(40000, 768, 3)
+    tsub, ssub = _subset(tgt, step, rays)
+    go = torch.randn(4, 4, len(rays), generator=torch.Generator().manual_seed(4)).to(DEV)
+    _, gM, _, gtgt, gstep = fused_raymarch_bwd(go, subject.image[0, 0], subject.label_i8, M, src, tsub, ssub, 256,
+                                               4, 0, False, True)
+    vol, lab = _oracle_vol(ph, 4)
+    g = orm.backward(_np(go), vol, lab, 4, _np(M), _np(src), _np(tsub), _np(ssub), 256, need_gvol=False)
+    assert rel_to_max(g["gM"], _np(gM)) < 2e-3  # label flips at tissue boundaries move single samples
+    assert rel_to_max(g["gstep"], _np(gstep)) < 2e-3
+
+
+def test_bench_workload_exact_linearity(bench_fp32):
+    """Scaling the fp32 volume by 2 is exact, and so is every step of the march."""
+    _, subject, k_inv, rt_inv, sdd = bench_fp32
+    M, src, tgt, step = _op_inputs(subject, k_inv, rt_inv, sdd, 200, 256)
+    vol = subject.image[0, 0]
+    a = fused_raymarch(vol, subject.label_i8, M, src, tgt, step, 256, 4, 200)
+    b = fused_raymarch((vol * 2).contiguous(), subject.label_i8, M, src, tgt, step, 256, 4, 200)
+    torch.testing.assert_close(b, 2 * a, rtol=0, atol=0)
+
+
+def test_bench_workload_poses_are_independent(bench_fp32):
+    """Each pose of the batch renders bitwise as it does alone."""
+    _, subject, k_inv, rt_inv, sdd = bench_fp32
+    M, src, tgt, step = _op_inputs(subject, k_inv, rt_inv, sdd, 200, 256)
+    full = fused_raymarch(subject.image[0, 0], subject.label_i8, M, src, tgt, step, 256, 4, 200)
+    for b in (0, 7, 15):
+        one = fused_raymarch(subject.image[0, 0], subject.label_i8, M[b:b + 1].contiguous(), src, tgt, step, 256,
+                             4, 200)
+        torch.testing.assert_close(one[0], full[b], rtol=0, atol=0)
+
+
+def test_bench_workload_channel_sum_and_adjoint(bench_fp32):
+    _, subject, k_inv, rt_inv, sdd = bench_fp32
+    M, src, tgt, step = _op_inputs(subject, k_inv, rt_inv, sdd, 200, 256)
+    vol = subject.image[0, 0]
+    multi = fused_raymarch(vol, subject.label_i8, M, src, tgt, step, 256, 4, 200)
+    single = fused_raymarch(vol, subject.label_i8.new_empty(0), M, src, tgt, step, 256, 1, 200)
+    assert rel_to_max(_np(single[:, 0]), _np(multi.sum(1))) < 1e-6
+    # <go, A v> = <A^T go, v> at full size (fp64 dot products of fp32 results)
+    go = torch.randn(16, 1, 40000, generator=torch.Generator().manual_seed(5)).to(DEV)
+    gvol = fused_raymarch_bwd(go, vol, subject.label_i8.new_empty(0), M, src, tgt, step, 256, 1, 200, True,
+                              False)[0]
+    lhs = (go.double() * single.double()).sum().item()
+    rhs = (gvol.double() * vol.double()).sum().item()
+    assert lhs == pytest.approx(rhs, rel=1e-5)
+
+
+def test_render_fwd_bwd_matches_op_path(bench_fp32):
+    """render() + autograd reaches the same kernels: gradient wrt rt_inv is finite and deterministic."""
+    _, subject, k_inv, rt_inv, sdd = bench_fp32
+    grads = []
+    for _ in range(2):
+        rt = rt_inv.clone().requires_grad_(True)
+        out = render(subject, k_inv, rt, sdd, 200, 200, n_samples=256)
+        out.sum().backward()
+        grads.append(rt.grad)
+    assert torch.isfinite(grads[0]).all()
+    torch.testing.assert_close(grads[0], grads[1], rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
+def test_half_volume_error_against_float64(dtype, bench_fp32):
+    """bf16/fp16 storage, fp32 accumulation: image and pose-gradient error vs the float64
+    oracle of the full-precision volume stays <= 1e-3 relative to the largest value."""
+    ph, _, k_inv, rt_inv, sdd = bench_fp32
+    subject = phantom_subject(ph, n_classes=4, dtype=dtype, device=DEV)
+    M, src, tgt, step = _op_inputs(subject, k_inv, rt_inv, sdd, 200, 256)
+    out = fused_raymarch(subject.image[0, 0], subject.label_i8, M, src, tgt, step, 256, 4, 200)
+    vol, lab = _oracle_vol(ph, 4)
+    rays = _pick_rays(40000, 400, 6)
+    ref = orm.forward(vol, lab, 4, _np(M), _np(src), _np(tgt), _np(step), 256, rays=rays)
+    err = rel_to_max(ref.sum(1), _np(out)[:, :, rays].sum(1))
+    print(f"{dtype} image rel-to-max error vs fp64 (fp32 volume): {err:.2e}")
+    assert err < 1e-3
+    tsub, ssub = _subset(tgt, step, rays)
+    Ms = M[:2].contiguous()
+    go = torch.randn(2, 4, len(rays), generator=torch.Generator().manual_seed(7)).to(DEV)
+    gM = fused_raymarch_bwd(go, subject.image[0, 0], subject.label_i8, Ms, src, tsub, ssub, 256, 4, 0, False,
+                            False)[1]
+    g = orm.backward(_np(go), vol, lab, 4, _np(Ms), _np(src), _np(tsub), _np(ssub), 256, need_gvol=False)
+    gerr = rel_to_max(g["gM"], _np(gM))
+    print(f"{dtype} pose-gradient rel-to-max error vs fp64: {gerr:.2e}")
+    assert gerr < 1e-2
+
+
+def test_compensated_accumulation_long_rays(bench_fp32):
+    """S = 1024 takes the compensated (Neumaier) accumulation path; error stays at fp32 level."""
+    ph, subject, k_inv, rt_inv, sdd = bench_fp32
+    M, src, tgt, step = _op_inputs(subject, k_inv, rt_inv, sdd, 200, 1024)
+    M = M[:2].contiguous()
+    out = fused_raymarch(subject.image[0, 0], subject.label_i8.new_empty(0), M, src, tgt, step, 1024, 1, 200)
+    vol, _ = _oracle_vol(ph, 1)
+    rays = _pick_rays(40000, 300, 8)
+    ref = orm.forward(vol, None, 1, _np(M), _np(src), _np(tgt), _np(step), 1024, rays=rays)
+    assert rel_to_max(ref, _np(out)[:, :, rays]) < 1e-5
