@@ -68,13 +68,14 @@ def time_steps(fn, steps, warmup):
     return t0.elapsed_time(t1) / steps  # ms per step
 
 
-def kernel_times(subject, k_inv, rt_inv, sdd, wts, reps):
+def kernel_times(subject, k_inv, rt_inv, sdd, wts, reps, layout):
     """Average duration (ms) of one forward and one backward op launch, HIP events
     recorded on the stream the ops launch on (torch's current stream)."""
     from nanodrr_amd._ops import fused_raymarch, fused_raymarch_bwd
     from nanodrr_amd.renderer import _make_src, _make_tgt
 
     vol = subject.image[0, 0]
+    packed = subject.stencil_volume() if layout == "stencil" else None
     lab = subject.label_i8
     M = (subject.world_to_voxel.double() @ rt_inv.double())[:, :3, :].float().contiguous()
     tgt = _make_tgt(k_inv, sdd, DET, DET, k_inv.device, torch.float32).contiguous()
@@ -94,8 +95,9 @@ def kernel_times(subject, k_inv, rt_inv, sdd, wts, reps):
         torch.cuda.synchronize()
         return t0.elapsed_time(t1) / reps
 
-    ms_fwd = timed(lambda: fused_raymarch(vol, lab, M, src, tgt, step, S, N_CLASSES, DET))
-    ms_bwd = timed(lambda: fused_raymarch_bwd(go, vol, lab, M, src, tgt, step, S, N_CLASSES, DET, False, False))
+    ms_fwd = timed(lambda: fused_raymarch(vol, lab, M, src, tgt, step, S, N_CLASSES, DET, packed))
+    ms_bwd = timed(lambda: fused_raymarch_bwd(go, vol, lab, M, src, tgt, step, S, N_CLASSES, DET, False, False,
+                                              packed))
     return ms_fwd, ms_bwd, (M, src, tgt)
 
 
@@ -151,6 +153,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--dtypes", default="bf16,fp32")
+    ap.add_argument("--layout", default="stencil", choices=["stencil", "linear"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -158,6 +161,9 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(2)
+    import nanodrr_amd.renderer as renderer
+
+    renderer.VOLUME_LAYOUT = args.layout
     dev = torch.device("cuda:0")
     dtypes = {"bf16": torch.bfloat16, "fp32": torch.float32, "fp16": torch.float16}
     results = {}
@@ -167,7 +173,7 @@ def main():
         subject, k_inv, rt_inv, sdd, ph = make_workload(dtype, dev)
         wts = torch.randn(1, N_CLASSES, DET, DET, generator=torch.Generator().manual_seed(0)).to(dev)
         ms = time_steps(lambda: step_fn(subject, k_inv, rt_inv, sdd, wts), args.steps, args.warmup)
-        ms_fwd, ms_bwd, geom = kernel_times(subject, k_inv, rt_inv, sdd, wts, max(10, args.steps))
+        ms_fwd, ms_bwd, geom = kernel_times(subject, k_inv, rt_inv, sdd, wts, max(10, args.steps), args.layout)
         n_in, b_fwd, b_bwd = algorithmic_bytes(subject, geom, dtype)
         results[name] = dict(ms_per_step=ms, drr_per_s=BATCH * 1000.0 / ms, ms_fwd=ms_fwd, ms_bwd=ms_bwd,
                              inside_samples=n_in, bytes_fwd=b_fwd, bytes_bwd=b_bwd,
@@ -195,13 +201,14 @@ def main():
         "data": "synthetic (seeded 256^3 phantom, 4 labels, 16 random poses)",
         "config": {
             "workload": "fwd+bwd (pose grads) 256^3 @1mm, C=4, B=16, 200x200, S=256",
-            "volume_storage": head, "accumulate": "fp32", "global_batch": BATCH, "n_samples": S,
+            "volume_storage": head, "volume_layout": args.layout, "accumulate": "fp32", "global_batch": BATCH,
+            "n_samples": S,
             "per_dtype": {k: {"drr_per_s": v["drr_per_s"], "ms_per_step": v["ms_per_step"],
                               "ms_fwd_kernel": v["ms_fwd"], "ms_bwd_kernel": v["ms_bwd"],
                               "gbs_fwd": v["gbs_fwd"], "gbs_bwd": v["gbs_bwd"]} for k, v in results.items()},
         },
         "roofline": {
-            "bound": "hbm", "kernel": f"k_{dominant} ({head} volume)", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "bound": "hbm", "kernel": f"k_{dominant} ({head} volume, {args.layout} layout)", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "algorithmic_bytes_per_launch": r[f"bytes_{dominant}"], "inside_samples": r["inside_samples"],
         },

@@ -21,7 +21,9 @@
  *        (tgt_batch_stride = 0 shares one detector across the pose batch)
  *   step per-ray Riemann weight, element (b, n) at step[b * step_batch_stride + n]
  *   out  [B][C][N] fp32 line integrals
- *   vol  [D][H][W] in NDRR_F32 / NDRR_BF16 / NDRR_F16, W fastest
+ *   vol  the volume in NDRR_F32 / NDRR_BF16 / NDRR_F16, laid out as
+ *        NDRR_LAYOUT_LINEAR:  [D][H][W], W fastest, or
+ *        NDRR_LAYOUT_STENCIL: stencil records made by ndrr_pack_stencil
  *   lab  [D][H][W] int8 class per voxel (values outside [0, C) are dropped);
  *        ignored when n_classes == 1
  * Sample s of ray (b, n) sits at M[b] @ [lerp(src, tgt, s / (S - 1)), 1].
@@ -43,6 +45,12 @@ enum {
   NDRR_F32 = 0,
   NDRR_BF16 = 1,
   NDRR_F16 = 2
+};
+
+/* Volume layouts. */
+enum {
+  NDRR_LAYOUT_LINEAR = 0,  /* [D][H][W] elements */
+  NDRR_LAYOUT_STENCIL = 1  /* one record of 8 trilinear corners per cell */
 };
 
 /* Argument errors (negative return codes). */
@@ -71,7 +79,7 @@ const char* ndrr_strerror(int code);
  *   src/nanodrr/drr/_fused.py:415-460 (fused_raymarch) and the
  *   BACKWARD=0 specialisation of _fused_kernel, _fused.py:36-331.
  */
-int ndrr_raymarch_fwd(const void* vol, int vol_dtype, const int8_t* lab,
+int ndrr_raymarch_fwd(const void* vol, int vol_dtype, int vol_layout, const int8_t* lab,
                       const float* M, const float* src, const float* tgt,
                       const float* step, float* out,
                       int B, int N, int D, int H, int W,
@@ -103,7 +111,7 @@ size_t ndrr_bwd_workspace_floats(int B, int N, int width);
  *   BACKWARD=1 specialisation of _fused_kernel, _fused.py:161-186,230-325.
  */
 int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype,
-                      const int8_t* lab, const float* M, const float* src,
+                      int vol_layout, const int8_t* lab, const float* M, const float* src,
                       const float* tgt, const float* step,
                       float* workspace, float* gM,
                       float* gsrc, float* gtgt, float* gstep, float* gvol,
@@ -113,6 +121,21 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype,
                       int64_t src_batch_stride, int64_t src_ray_stride,
                       int64_t tgt_batch_stride, int64_t step_batch_stride,
                       void* stream);
+
+/*
+ * Stencil-record layout of a [D][H][W] volume: one record per trilinear cell
+ * (x0, y0, z0), x0 in [-1, W-1], y0 in [-1, H-1], z0 in [-1, D-1], x fastest,
+ * each holding the cell's eight corner values (zero outside the volume) in
+ * the order (z, y, x) = 000, 001, 010, 011, 100, 101, 110, 111: 16 bytes for
+ * bf16/fp16, 32 bytes for fp32. A march sample reads one record instead of
+ * gathering eight voxels. ndrr_stencil_bytes gives the size of `out`
+ * (0 for invalid arguments); ndrr_pack_stencil fills it from the linear volume.
+ * Replaces the per-sample gathers of _fused.py:205-212 (_sample, :11-16) with
+ * a one-off re-layout of the subject's volume.
+ */
+size_t ndrr_stencil_bytes(int D, int H, int W, int vol_dtype);
+int ndrr_pack_stencil(const void* vol, int vol_dtype, int D, int H, int W,
+                      void* out, void* stream);
 
 /*
  * Label map for the kernels: dst[i] = trunc(src[i]) when that lies in

@@ -13,6 +13,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().with_name("libnanodrr_amd.so")
 
 NDRR_F32, NDRR_BF16, NDRR_F16 = 0, 1, 2
+LAYOUT_LINEAR, LAYOUT_STENCIL = 0, 1
 MAX_CLASSES = 8
 
 _c_int, _c_i64, _c_p, _c_size = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_size_t
@@ -23,19 +24,21 @@ SIGNATURES: dict[str, tuple] = {
     "ndrr_strerror": (ctypes.c_char_p, [_c_int]),
     "ndrr_raymarch_fwd": (
         _c_int,
-        [_c_p, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
+        [_c_p, _c_int, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
          _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
          _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
     ),
     "ndrr_bwd_workspace_floats": (_c_size, [_c_int, _c_int, _c_int]),
     "ndrr_raymarch_bwd": (
         _c_int,
-        [_c_p, _c_p, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p,
+        [_c_p, _c_p, _c_int, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p,
          _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_int,
          _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
          _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
     ),
     "ndrr_labels_to_i8": (_c_int, [_c_p, _c_p, _c_i64, _c_int, _c_p]),
+    "ndrr_stencil_bytes": (_c_size, [_c_int, _c_int, _c_int, _c_int]),
+    "ndrr_pack_stencil": (_c_int, [_c_p, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p]),
 }
 
 

@@ -9,10 +9,15 @@ differences:
     pose) instead of being expanded and copied per pose;
   * the output is always fp32 (an fp32 accumulation rounded to a bf16 output
     alone costs ~2^-9 relative error);
+  * an optional `packed` volume (stencil records of `vol`, see
+    `pack_stencil`) is what the kernels read when given; `vol` still fixes
+    the shape, dtype and the layout of the volume gradient;
   * the launch goes through the C ABI of include/nanodrr_amd.h.
 """
 
 from __future__ import annotations
+
+from typing import Optional
 
 import torch
 
@@ -27,6 +32,33 @@ def _ptr(t: torch.Tensor | None) -> int | None:
 
 def _stream(device: torch.device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
+
+
+def _volume_arg(vol, packed):
+    """(pointer, layout) the kernels read: the stencil records when given, else `vol`."""
+    if packed is None or packed.numel() == 0:
+        return vol.data_ptr(), _lib.LAYOUT_LINEAR
+    D, H, W = vol.shape
+    need = _lib.lib().ndrr_stencil_bytes(D, H, W, _DTYPE_CODE[vol.dtype])
+    if packed.dtype != torch.uint8 or packed.numel() != need or not packed.is_contiguous() \
+            or packed.device != vol.device:
+        raise ValueError(f"packed must be a contiguous uint8 tensor of {need} bytes on {vol.device}")
+    return packed.data_ptr(), _lib.LAYOUT_STENCIL
+
+
+def pack_stencil(vol: torch.Tensor) -> torch.Tensor:
+    """Stencil records (uint8 bytes) of a contiguous (D, H, W) device volume (ndrr_pack_stencil)."""
+    if not vol.is_cuda or vol.dim() != 3 or vol.dtype not in _DTYPE_CODE:
+        raise ValueError("pack_stencil needs a (D, H, W) float32/bfloat16/float16 device tensor")
+    vol = vol.detach().contiguous()
+    D, H, W = vol.shape
+    lib = _lib.lib()
+    out = torch.empty(lib.ndrr_stencil_bytes(D, H, W, _DTYPE_CODE[vol.dtype]), device=vol.device, dtype=torch.uint8)
+    with torch.cuda.device(vol.device):
+        rc = lib.ndrr_pack_stencil(vol.data_ptr(), _DTYPE_CODE[vol.dtype], D, H, W, out.data_ptr(),
+                                   _stream(vol.device))
+    _lib.check(rc, "ndrr_pack_stencil")
+    return out
 
 
 def _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width):
@@ -80,13 +112,15 @@ def fused_raymarch(
     n_samples: int,
     n_classes: int,
     width: int,
+    packed: Optional[torch.Tensor] = None,
 ) -> torch.Tensor:
     """Line integrals (B, C, N) fp32 of the rays through `vol` (see ndrr_raymarch_fwd)."""
     B, N, D, H, W, strides = _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width)
+    vptr, layout = _volume_arg(vol, packed)
     out = torch.empty(B, n_classes, N, device=vol.device, dtype=torch.float32)
     with torch.cuda.device(vol.device):
         rc = _lib.lib().ndrr_raymarch_fwd(
-            vol.data_ptr(), _DTYPE_CODE[vol.dtype], _ptr(lab) if n_classes > 1 else None,
+            vptr, _DTYPE_CODE[vol.dtype], layout, _ptr(lab) if n_classes > 1 else None,
             M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(), out.data_ptr(),
             B, N, D, H, W, n_samples, n_classes, width, *strides, _stream(vol.device),
         )
@@ -95,7 +129,7 @@ def fused_raymarch(
 
 
 @fused_raymarch.register_fake
-def _(vol, lab, M, src, tgt, step, n_samples, n_classes, width):
+def _(vol, lab, M, src, tgt, step, n_samples, n_classes, width, packed=None):
     return torch.empty(M.shape[0], n_classes, tgt.shape[1], device=vol.device, dtype=torch.float32)
 
 
@@ -113,6 +147,7 @@ def fused_raymarch_bwd(
     width: int,
     need_gvol: bool,
     need_gcam: bool,
+    packed: Optional[torch.Tensor] = None,
 ) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
     """Analytic adjoint (see ndrr_raymarch_bwd). Returns (gvol|empty, gM (B,3,4),
     gsrc (B,N,3)|empty, gtgt (B,N,3)|empty, gstep (B,N)|empty), all fp32."""
@@ -122,6 +157,7 @@ def fused_raymarch_bwd(
             "disable deterministic algorithms to request them"
         )
     B, N, D, H, W, strides = _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width)
+    vptr, layout = _volume_arg(vol, packed)
     go = go.to(torch.float32).contiguous()
     if go.shape != (B, n_classes, N):
         raise ValueError(f"go must be (B, C, N) = {(B, n_classes, N)}, got {tuple(go.shape)}")
@@ -136,7 +172,7 @@ def fused_raymarch_bwd(
     gstep = torch.empty(B, N, device=dev, dtype=torch.float32) if need_gcam else gM.new_empty(0)
     with torch.cuda.device(dev):
         rc = lib.ndrr_raymarch_bwd(
-            go.data_ptr(), vol.data_ptr(), _DTYPE_CODE[vol.dtype], _ptr(lab) if n_classes > 1 else None,
+            go.data_ptr(), vptr, _DTYPE_CODE[vol.dtype], layout, _ptr(lab) if n_classes > 1 else None,
             M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(),
             ws.data_ptr(), gM.data_ptr(), _ptr(gsrc), _ptr(gtgt), _ptr(gstep), _ptr(gvol),
             int(need_gcam), B, N, D, H, W, n_samples, n_classes, width, *strides, _stream(dev),
@@ -146,7 +182,7 @@ def fused_raymarch_bwd(
 
 
 @fused_raymarch_bwd.register_fake
-def _(go, vol, lab, M, src, tgt, step, n_samples, n_classes, width, need_gvol, need_gcam):
+def _(go, vol, lab, M, src, tgt, step, n_samples, n_classes, width, need_gvol, need_gcam, packed=None):
     B, N = M.shape[0], tgt.shape[1]
     gM = torch.empty(B, 3, 4, device=M.device, dtype=torch.float32)
     gvol = torch.empty(vol.shape, device=M.device, dtype=torch.float32) if need_gvol else gM.new_empty(0)
@@ -157,8 +193,8 @@ def _(go, vol, lab, M, src, tgt, step, n_samples, n_classes, width, need_gvol, n
 
 
 def _setup_context(ctx, inputs, output):
-    vol, lab, M, src, tgt, step, n_samples, n_classes, width = inputs
-    ctx.save_for_backward(vol, lab, M, src, tgt, step)
+    vol, lab, M, src, tgt, step, n_samples, n_classes, width, packed = inputs
+    ctx.save_for_backward(vol, lab, M, src, tgt, step, packed)
     ctx.n_samples, ctx.n_classes, ctx.width = n_samples, n_classes, width
 
 
@@ -171,11 +207,11 @@ def _reduce_to(g: torch.Tensor, like: torch.Tensor) -> torch.Tensor:
 
 
 def _backward(ctx, go):
-    vol, lab, M, src, tgt, step = ctx.saved_tensors
+    vol, lab, M, src, tgt, step, packed = ctx.saved_tensors
     need_gvol = ctx.needs_input_grad[0]
     need_gcam = any(ctx.needs_input_grad[i] for i in (3, 4, 5))
     gvol, gM, gsrc, gtgt, gstep = fused_raymarch_bwd(
-        go, vol, lab, M, src, tgt, step, ctx.n_samples, ctx.n_classes, ctx.width, need_gvol, need_gcam
+        go, vol, lab, M, src, tgt, step, ctx.n_samples, ctx.n_classes, ctx.width, need_gvol, need_gcam, packed
     )
     return (
         gvol.to(vol.dtype) if need_gvol else None,
@@ -184,9 +220,10 @@ def _backward(ctx, go):
         _reduce_to(gsrc, src) if need_gcam else None,
         _reduce_to(gtgt, tgt) if need_gcam else None,
         _reduce_to(gstep, step) if need_gcam else None,
-        None,
-        None,
-        None,
+        None,  # n_samples
+        None,  # n_classes
+        None,  # width
+        None,  # packed: a re-layout of vol; its gradient is vol's
     )
 
 

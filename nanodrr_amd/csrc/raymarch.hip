@@ -144,8 +144,8 @@ struct Stencil {
 };
 
 template <int DT>
-__device__ inline Stencil gather(const typename Vox<DT>::T* __restrict__ vol, int D, int H, int W,
-                                 float x, float y, float z) {
+__device__ inline Stencil gather_linear(const typename Vox<DT>::T* __restrict__ vol, int D, int H, int W,
+                                        float x, float y, float z) {
   Stencil st;
   const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
   st.fx = x - xf; st.fy = y - yf; st.fz = z - zf;
@@ -175,6 +175,63 @@ __device__ inline Stencil gather(const typename Vox<DT>::T* __restrict__ vol, in
   st.v110 = (bz1 && by1 && bx0) ? a110 : 0.0f;
   st.v111 = (bz1 && by1 && bx1) ? a111 : 0.0f;
   return st;
+}
+
+// Stencil-record layout: one record per trilinear cell (x0, y0, z0) with
+// x0 in [-1, W-1], y0 in [-1, H-1], z0 in [-1, D-1], holding the cell's eight
+// corner values (zero outside the volume) in the order v000 v001 v010 v011
+// v100 v101 v110 v111 (x fastest); records are x-fastest too. A sample then
+// costs one 16-byte load (bf16/fp16) or two (fp32) instead of eight gathers.
+template <int DT> struct Rec { static constexpr int kBytes = 16; };
+template <> struct Rec<NDRR_F32> { static constexpr int kBytes = 32; };
+
+__device__ inline float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ inline float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+__device__ inline float f16_lo(uint32_t w) {
+  return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w & 0xffffu)));
+}
+__device__ inline float f16_hi(uint32_t w) {
+  return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w >> 16)));
+}
+
+template <int DT>
+__device__ inline Stencil gather_stencil(const void* __restrict__ rec, int D, int H, int W,
+                                         float x, float y, float z) {
+  Stencil st;
+  const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
+  st.fx = x - xf; st.fy = y - yf; st.fz = z - zf;
+  st.ix = static_cast<int>(xf); st.iy = static_cast<int>(yf); st.iz = static_cast<int>(zf);
+  // cells -1 .. dim-1 exist; any other cell has all eight corners outside
+  const bool in = static_cast<unsigned>(st.ix + 1) <= static_cast<unsigned>(W) &&
+                  static_cast<unsigned>(st.iy + 1) <= static_cast<unsigned>(H) &&
+                  static_cast<unsigned>(st.iz + 1) <= static_cast<unsigned>(D);
+  const int cx = min(max(st.ix + 1, 0), W), cy = min(max(st.iy + 1, 0), H), cz = min(max(st.iz + 1, 0), D);
+  const int64_t r = (static_cast<int64_t>(cz) * (H + 1) + cy) * (W + 1) + cx;
+  if constexpr (DT == NDRR_F32) {
+    const float4* p = static_cast<const float4*>(rec) + 2 * r;
+    const float4 a = p[0], b = p[1];
+    st.v000 = in ? a.x : 0.0f; st.v001 = in ? a.y : 0.0f; st.v010 = in ? a.z : 0.0f; st.v011 = in ? a.w : 0.0f;
+    st.v100 = in ? b.x : 0.0f; st.v101 = in ? b.y : 0.0f; st.v110 = in ? b.z : 0.0f; st.v111 = in ? b.w : 0.0f;
+  } else {
+    const uint4 q = static_cast<const uint4*>(rec)[r];
+    const uint4 w = in ? q : make_uint4(0u, 0u, 0u, 0u);  // +0 in both bf16 and fp16
+    if constexpr (DT == NDRR_BF16) {
+      st.v000 = bf16_lo(w.x); st.v001 = bf16_hi(w.x); st.v010 = bf16_lo(w.y); st.v011 = bf16_hi(w.y);
+      st.v100 = bf16_lo(w.z); st.v101 = bf16_hi(w.z); st.v110 = bf16_lo(w.w); st.v111 = bf16_hi(w.w);
+    } else {
+      st.v000 = f16_lo(w.x); st.v001 = f16_hi(w.x); st.v010 = f16_lo(w.y); st.v011 = f16_hi(w.y);
+      st.v100 = f16_lo(w.z); st.v101 = f16_hi(w.z); st.v110 = f16_lo(w.w); st.v111 = f16_hi(w.w);
+    }
+  }
+  return st;
+}
+
+template <int DT, int L>
+__device__ inline Stencil gather(const void* __restrict__ vol, int D, int H, int W, float x, float y, float z) {
+  if constexpr (L == NDRR_LAYOUT_STENCIL)
+    return gather_stencil<DT>(vol, D, H, W, x, y, z);
+  else
+    return gather_linear<DT>(static_cast<const typename Vox<DT>::T*>(vol), D, H, W, x, y, z);
 }
 
 __device__ inline float lerp_value(const Stencil& s) {
@@ -212,14 +269,13 @@ __device__ inline void comp_add(float& sum, float& comp, float x) {
 // Forward
 // ---------------------------------------------------------------------------
 // CV == 1: single channel (no label map). CV == 8: label map, runtime C <= 8.
-template <int DT, int CV, bool KAHAN>
+template <int DT, int L, int CV, bool KAHAN>
 __global__ __launch_bounds__(kThreads) void k_fwd(Params p) {
   const int b = blockIdx.y;
   bool valid;
   const int n = ray_index(p, blockIdx.x, threadIdx.x, valid);
   const float* __restrict__ Mb = p.M + 12 * b;
   const Ray r = setup_ray(p, Mb, b, n, valid);
-  const auto* __restrict__ vol = static_cast<const typename Vox<DT>::T*>(p.vol);
 
   float acc[CV];
   float comp[CV];
@@ -231,7 +287,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd(Params p) {
     const float x = r.p0x + t * r.dx;
     const float y = r.p0y + t * r.dy;
     const float z = r.p0z + t * r.dz;
-    const Stencil st = gather<DT>(vol, p.D, p.H, p.W, x, y, z);
+    const Stencil st = gather<DT, L>(p.vol, p.D, p.H, p.W, x, y, z);
     const float val = lerp_value(st);
     if constexpr (CV == 1) {
       if constexpr (KAHAN) comp_add(acc[0], comp[0], val); else acc[0] += val;
@@ -264,14 +320,13 @@ __device__ inline float wave_sum(float v) {
   return v;
 }
 
-template <int DT, int CV, bool GCAM, bool GVOL>
+template <int DT, int L, int CV, bool GCAM, bool GVOL>
 __global__ __launch_bounds__(kThreads) void k_bwd(Params p) {
   const int b = blockIdx.y;
   bool valid;
   const int n = ray_index(p, blockIdx.x, threadIdx.x, valid);
   const float* __restrict__ Mb = p.M + 12 * b;
   const Ray r = setup_ray(p, Mb, b, n, valid);
-  const auto* __restrict__ vol = static_cast<const typename Vox<DT>::T*>(p.vol);
   const float m00 = Mb[0], m01 = Mb[1], m02 = Mb[2];
   const float m10 = Mb[4], m11 = Mb[5], m12 = Mb[6];
   const float m20 = Mb[8], m21 = Mb[9], m22 = Mb[10];
@@ -296,7 +351,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd(Params p) {
     const float x = r.p0x + t * r.dx;
     const float y = r.p0y + t * r.dy;
     const float z = r.p0z + t * r.dz;
-    const Stencil st = gather<DT>(vol, p.D, p.H, p.W, x, y, z);
+    const Stencil st = gather<DT, L>(p.vol, p.D, p.H, p.W, x, y, z);
 
     float gg;
     if constexpr (CV == 1) {
@@ -380,15 +435,27 @@ __global__ __launch_bounds__(kThreads) void k_bwd(Params p) {
   }
 }
 
-// gM[b][j] = sum over tiles of gpart[b][tile][j], in tile order (compensated).
-__global__ void k_reduce_pose(const float* __restrict__ part, float* __restrict__ gM, int nblocks) {
+// gM[b][j] = sum over tiles of gpart[b][tile][j] in a fixed order: thread
+// (j, k) of the 192-thread block sums tiles k, k+16, ... (compensated), then
+// thread j adds the 16 partial sums in k order. Deterministic.
+constexpr int kReduceLanes = 16;
+__global__ __launch_bounds__(12 * kReduceLanes) void k_reduce_pose(const float* __restrict__ part,
+                                                                    float* __restrict__ gM, int nblocks) {
+  __shared__ float red[12][kReduceLanes];
   const int b = blockIdx.x;
-  const int j = threadIdx.x;
-  if (j >= 12) return;
+  const int j = threadIdx.x % 12;
+  const int k = threadIdx.x / 12;
   const float* pp = part + static_cast<int64_t>(b) * nblocks * 12 + j;
   float sum = 0.0f, comp = 0.0f;
-  for (int k = 0; k < nblocks; ++k) comp_add(sum, comp, pp[static_cast<int64_t>(k) * 12]);
-  gM[b * 12 + j] = sum + comp;
+  for (int t = k; t < nblocks; t += kReduceLanes) comp_add(sum, comp, pp[static_cast<int64_t>(t) * 12]);
+  red[j][k] = sum + comp;
+  __syncthreads();
+  if (threadIdx.x < 12) {
+    float s2 = 0.0f, c2 = 0.0f;
+#pragma unroll
+    for (int t = 0; t < kReduceLanes; ++t) comp_add(s2, c2, red[threadIdx.x][t]);
+    gM[b * 12 + threadIdx.x] = s2 + c2;
+  }
 }
 
 __global__ void k_labels_to_i8(const float* __restrict__ src, int8_t* __restrict__ dst, int64_t numel, int C) {
@@ -396,6 +463,43 @@ __global__ void k_labels_to_i8(const float* __restrict__ src, int8_t* __restrict
   if (i >= numel) return;
   const float t = truncf(src[i]);
   dst[i] = (t >= 0.0f && t < static_cast<float>(C)) ? static_cast<int8_t>(t) : static_cast<int8_t>(-1);
+}
+
+// One thread per stencil record: gather the cell's 8 corners (zero padded)
+// from the linear volume and store them as one record (raw element bits).
+template <int DT>
+__global__ void k_pack_stencil(const typename Vox<DT>::T* __restrict__ vol, void* __restrict__ rec,
+                               int D, int H, int W, int64_t n_rec) {
+  using T = typename Vox<DT>::T;
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n_rec) return;
+  const int cx = static_cast<int>(r % (W + 1));
+  const int64_t rest = r / (W + 1);
+  const int cy = static_cast<int>(rest % (H + 1));
+  const int cz = static_cast<int>(rest / (H + 1));
+  T v[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int x = cx - 1 + (c & 1), y = cy - 1 + ((c >> 1) & 1), z = cz - 1 + (c >> 2);
+    const bool inb = static_cast<unsigned>(x) < static_cast<unsigned>(W) &&
+                     static_cast<unsigned>(y) < static_cast<unsigned>(H) &&
+                     static_cast<unsigned>(z) < static_cast<unsigned>(D);
+    v[c] = inb ? vol[(static_cast<int64_t>(z) * H + y) * W + x] : T(0);
+  }
+  if constexpr (DT == NDRR_F32) {
+    float4* o = static_cast<float4*>(rec) + 2 * r;
+    o[0] = make_float4(v[0], v[1], v[2], v[3]);
+    o[1] = make_float4(v[4], v[5], v[6], v[7]);
+  } else {
+    uint32_t w[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const uint32_t lo = __builtin_bit_cast(uint16_t, v[2 * c]);
+      const uint32_t hi = __builtin_bit_cast(uint16_t, v[2 * c + 1]);
+      w[c] = lo | (hi << 16);
+    }
+    static_cast<uint4*>(rec)[r] = make_uint4(w[0], w[1], w[2], w[3]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -412,19 +516,26 @@ int nblocks_of(int N, int width, int& tiles_w, int& IH) {
   return (N + kThreads - 1) / kThreads;
 }
 
-int check_common(const void* vol, int vol_dtype, const int8_t* lab, const float* M, const float* src,
-                 const float* tgt, const float* step, int B, int N, int D, int H, int W, int S, int C,
-                 int width) {
+int64_t stencil_records(int D, int H, int W) {
+  return static_cast<int64_t>(D + 1) * (H + 1) * (W + 1);
+}
+
+bool valid_dtype(int dt) { return dt == NDRR_F32 || dt == NDRR_BF16 || dt == NDRR_F16; }
+
+int check_common(const void* vol, int vol_dtype, int vol_layout, const int8_t* lab, const float* M,
+                 const float* src, const float* tgt, const float* step, int B, int N, int D, int H, int W,
+                 int S, int C, int width) {
   if (!vol || !M || !src || !tgt || !step) return NDRR_EINVAL;
   if (B <= 0 || N <= 0 || D <= 0 || H <= 0 || W <= 0 || S < 2 || width < 0) return NDRR_EINVAL;
   if (width > 0 && N % width != 0) return NDRR_EINVAL;
-  if (vol_dtype != NDRR_F32 && vol_dtype != NDRR_BF16 && vol_dtype != NDRR_F16) return NDRR_EDTYPE;
+  if (vol_layout != NDRR_LAYOUT_LINEAR && vol_layout != NDRR_LAYOUT_STENCIL) return NDRR_EINVAL;
+  if (!valid_dtype(vol_dtype)) return NDRR_EDTYPE;
   if (C < 1 || C > NDRR_MAX_CLASSES) return NDRR_ECLASSES;
   if (C > 1 && !lab) return NDRR_EINVAL;
   if (B > 65535) return NDRR_ERANGE;
   const int64_t vox = static_cast<int64_t>(D) * H * W;
-  if (vox >= (int64_t{1} << 31) || static_cast<int64_t>(C) * B * N >= (int64_t{1} << 31) ||
-      3 * static_cast<int64_t>(B) * N >= (int64_t{1} << 31))
+  if (vox >= (int64_t{1} << 31) || stencil_records(D, H, W) >= (int64_t{1} << 31) ||
+      static_cast<int64_t>(C) * B * N >= (int64_t{1} << 31) || 3 * static_cast<int64_t>(B) * N >= (int64_t{1} << 31))
     return NDRR_ERANGE;
   return NDRR_OK;
 }
@@ -448,50 +559,56 @@ int launch_status() {
   return e == hipSuccess ? NDRR_OK : static_cast<int>(e);
 }
 
-template <int DT, int CV>
-void launch_fwd_dt(const Params& p, dim3 grid, hipStream_t st) {
+template <int DT, int L, int CV>
+void launch_fwd3(const Params& p, dim3 grid, hipStream_t st) {
   if (p.S >= kKahanMinSamples)
-    hipLaunchKernelGGL((k_fwd<DT, CV, true>), grid, dim3(kThreads), 0, st, p);
+    hipLaunchKernelGGL((k_fwd<DT, L, CV, true>), grid, dim3(kThreads), 0, st, p);
   else
-    hipLaunchKernelGGL((k_fwd<DT, CV, false>), grid, dim3(kThreads), 0, st, p);
+    hipLaunchKernelGGL((k_fwd<DT, L, CV, false>), grid, dim3(kThreads), 0, st, p);
 }
 
-template <int CV>
-void launch_fwd_cv(const Params& p, int dt, dim3 grid, hipStream_t st) {
-  switch (dt) {
-    case NDRR_F32: launch_fwd_dt<NDRR_F32, CV>(p, grid, st); break;
-    case NDRR_BF16: launch_fwd_dt<NDRR_BF16, CV>(p, grid, st); break;
-    default: launch_fwd_dt<NDRR_F16, CV>(p, grid, st); break;
-  }
+template <int DT, int L>
+void launch_fwd2(const Params& p, dim3 grid, hipStream_t st) {
+  if (p.C == 1) launch_fwd3<DT, L, 1>(p, grid, st);
+  else launch_fwd3<DT, L, 8>(p, grid, st);
 }
 
-template <int DT, int CV>
-void launch_bwd_dt(const Params& p, bool gcam, bool gvol, dim3 grid, hipStream_t st) {
-  if (gcam && gvol) hipLaunchKernelGGL((k_bwd<DT, CV, true, true>), grid, dim3(kThreads), 0, st, p);
-  else if (gcam) hipLaunchKernelGGL((k_bwd<DT, CV, true, false>), grid, dim3(kThreads), 0, st, p);
-  else if (gvol) hipLaunchKernelGGL((k_bwd<DT, CV, false, true>), grid, dim3(kThreads), 0, st, p);
-  else hipLaunchKernelGGL((k_bwd<DT, CV, false, false>), grid, dim3(kThreads), 0, st, p);
+template <int DT, int L, int CV>
+void launch_bwd3(const Params& p, bool gcam, bool gvol, dim3 grid, hipStream_t st) {
+  if (gcam && gvol) hipLaunchKernelGGL((k_bwd<DT, L, CV, true, true>), grid, dim3(kThreads), 0, st, p);
+  else if (gcam) hipLaunchKernelGGL((k_bwd<DT, L, CV, true, false>), grid, dim3(kThreads), 0, st, p);
+  else if (gvol) hipLaunchKernelGGL((k_bwd<DT, L, CV, false, true>), grid, dim3(kThreads), 0, st, p);
+  else hipLaunchKernelGGL((k_bwd<DT, L, CV, false, false>), grid, dim3(kThreads), 0, st, p);
 }
 
-template <int CV>
-void launch_bwd_cv(const Params& p, int dt, bool gcam, bool gvol, dim3 grid, hipStream_t st) {
-  switch (dt) {
-    case NDRR_F32: launch_bwd_dt<NDRR_F32, CV>(p, gcam, gvol, grid, st); break;
-    case NDRR_BF16: launch_bwd_dt<NDRR_BF16, CV>(p, gcam, gvol, grid, st); break;
-    default: launch_bwd_dt<NDRR_F16, CV>(p, gcam, gvol, grid, st); break;
-  }
+template <int DT, int L>
+void launch_bwd2(const Params& p, bool gcam, bool gvol, dim3 grid, hipStream_t st) {
+  if (p.C == 1) launch_bwd3<DT, L, 1>(p, gcam, gvol, grid, st);
+  else launch_bwd3<DT, L, 8>(p, gcam, gvol, grid, st);
+}
+
+template <int DT>
+void launch_fwd1(const Params& p, int layout, dim3 grid, hipStream_t st) {
+  if (layout == NDRR_LAYOUT_STENCIL) launch_fwd2<DT, NDRR_LAYOUT_STENCIL>(p, grid, st);
+  else launch_fwd2<DT, NDRR_LAYOUT_LINEAR>(p, grid, st);
+}
+
+template <int DT>
+void launch_bwd1(const Params& p, int layout, bool gcam, bool gvol, dim3 grid, hipStream_t st) {
+  if (layout == NDRR_LAYOUT_STENCIL) launch_bwd2<DT, NDRR_LAYOUT_STENCIL>(p, gcam, gvol, grid, st);
+  else launch_bwd2<DT, NDRR_LAYOUT_LINEAR>(p, gcam, gvol, grid, st);
 }
 
 }  // namespace
 
 extern "C" {
 
-int ndrr_version(void) { return 100; }
+int ndrr_version(void) { return 200; }
 
 const char* ndrr_strerror(int code) {
   switch (code) {
     case NDRR_OK: return "success";
-    case NDRR_EINVAL: return "invalid argument (null pointer, non-positive size, n_samples < 2 or bad width)";
+    case NDRR_EINVAL: return "invalid argument (null pointer, non-positive size, n_samples < 2, bad width or layout)";
     case NDRR_EDTYPE: return "unsupported volume element type";
     case NDRR_ECLASSES: return "n_classes outside [1, 8]";
     case NDRR_ERANGE: return "a size exceeds the kernel's 32-bit index range";
@@ -501,12 +618,13 @@ const char* ndrr_strerror(int code) {
   return "unknown error";
 }
 
-int ndrr_raymarch_fwd(const void* vol, int vol_dtype, const int8_t* lab, const float* M, const float* src,
-                      const float* tgt, const float* step, float* out, int B, int N, int D, int H, int W,
-                      int n_samples, int n_classes, int width, int64_t src_batch_stride,
+int ndrr_raymarch_fwd(const void* vol, int vol_dtype, int vol_layout, const int8_t* lab, const float* M,
+                      const float* src, const float* tgt, const float* step, float* out, int B, int N, int D,
+                      int H, int W, int n_samples, int n_classes, int width, int64_t src_batch_stride,
                       int64_t src_ray_stride, int64_t tgt_batch_stride, int64_t step_batch_stride,
                       void* stream) {
-  const int rc = check_common(vol, vol_dtype, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width);
+  const int rc = check_common(vol, vol_dtype, vol_layout, lab, M, src, tgt, step, B, N, D, H, W, n_samples,
+                              n_classes, width);
   if (rc != NDRR_OK) return rc;
   if (!out) return NDRR_EINVAL;
   Params p = make_params(vol, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width,
@@ -514,8 +632,11 @@ int ndrr_raymarch_fwd(const void* vol, int vol_dtype, const int8_t* lab, const f
   p.out = out;
   const dim3 grid(p.nblocks, B);
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (n_classes == 1) launch_fwd_cv<1>(p, vol_dtype, grid, st);
-  else launch_fwd_cv<8>(p, vol_dtype, grid, st);
+  switch (vol_dtype) {
+    case NDRR_F32: launch_fwd1<NDRR_F32>(p, vol_layout, grid, st); break;
+    case NDRR_BF16: launch_fwd1<NDRR_BF16>(p, vol_layout, grid, st); break;
+    default: launch_fwd1<NDRR_F16>(p, vol_layout, grid, st); break;
+  }
   return launch_status();
 }
 
@@ -525,13 +646,14 @@ size_t ndrr_bwd_workspace_floats(int B, int N, int width) {
   return static_cast<size_t>(B) * nblocks_of(N, width, tw, ih) * 12;
 }
 
-int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, const int8_t* lab, const float* M,
-                      const float* src, const float* tgt, const float* step, float* workspace, float* gM,
-                      float* gsrc, float* gtgt, float* gstep, float* gvol, int need_gcam, int B, int N, int D,
-                      int H, int W, int n_samples, int n_classes, int width, int64_t src_batch_stride,
+int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_layout, const int8_t* lab,
+                      const float* M, const float* src, const float* tgt, const float* step, float* workspace,
+                      float* gM, float* gsrc, float* gtgt, float* gstep, float* gvol, int need_gcam, int B, int N,
+                      int D, int H, int W, int n_samples, int n_classes, int width, int64_t src_batch_stride,
                       int64_t src_ray_stride, int64_t tgt_batch_stride, int64_t step_batch_stride,
                       void* stream) {
-  const int rc = check_common(vol, vol_dtype, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width);
+  const int rc = check_common(vol, vol_dtype, vol_layout, lab, M, src, tgt, step, B, N, D, H, W, n_samples,
+                              n_classes, width);
   if (rc != NDRR_OK) return rc;
   if (!go || !workspace || !gM) return NDRR_EINVAL;
   if (need_gcam && (!gsrc || !gtgt || !gstep)) return NDRR_EINVAL;
@@ -543,11 +665,44 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, const int
   const dim3 grid(p.nblocks, B);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const bool gcam = need_gcam != 0, gv = gvol != nullptr;
-  if (n_classes == 1) launch_bwd_cv<1>(p, vol_dtype, gcam, gv, grid, st);
-  else launch_bwd_cv<8>(p, vol_dtype, gcam, gv, grid, st);
+  switch (vol_dtype) {
+    case NDRR_F32: launch_bwd1<NDRR_F32>(p, vol_layout, gcam, gv, grid, st); break;
+    case NDRR_BF16: launch_bwd1<NDRR_BF16>(p, vol_layout, gcam, gv, grid, st); break;
+    default: launch_bwd1<NDRR_F16>(p, vol_layout, gcam, gv, grid, st); break;
+  }
   int r = launch_status();
   if (r != NDRR_OK) return r;
-  hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(64), 0, st, workspace, gM, p.nblocks);
+  hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(12 * kReduceLanes), 0, st, workspace, gM, p.nblocks);
+  return launch_status();
+}
+
+size_t ndrr_stencil_bytes(int D, int H, int W, int vol_dtype) {
+  if (D <= 0 || H <= 0 || W <= 0 || !valid_dtype(vol_dtype)) return 0;
+  const size_t per = vol_dtype == NDRR_F32 ? Rec<NDRR_F32>::kBytes : Rec<NDRR_BF16>::kBytes;
+  return static_cast<size_t>(stencil_records(D, H, W)) * per;
+}
+
+int ndrr_pack_stencil(const void* vol, int vol_dtype, int D, int H, int W, void* out, void* stream) {
+  if (!vol || !out || D <= 0 || H <= 0 || W <= 0) return NDRR_EINVAL;
+  if (!valid_dtype(vol_dtype)) return NDRR_EDTYPE;
+  const int64_t n = stencil_records(D, H, W);
+  if (n >= (int64_t{1} << 31)) return NDRR_ERANGE;
+  const dim3 grid(static_cast<unsigned>((n + kThreads - 1) / kThreads));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (vol_dtype) {
+    case NDRR_F32:
+      hipLaunchKernelGGL(k_pack_stencil<NDRR_F32>, grid, dim3(kThreads), 0, st,
+                         static_cast<const float*>(vol), out, D, H, W, n);
+      break;
+    case NDRR_BF16:
+      hipLaunchKernelGGL(k_pack_stencil<NDRR_BF16>, grid, dim3(kThreads), 0, st,
+                         static_cast<const uint16_t*>(vol), out, D, H, W, n);
+      break;
+    default:
+      hipLaunchKernelGGL(k_pack_stencil<NDRR_F16>, grid, dim3(kThreads), 0, st,
+                         static_cast<const _Float16*>(vol), out, D, H, W, n);
+      break;
+  }
   return launch_status();
 }
 

@@ -18,6 +18,12 @@ from ._ops import fused_raymarch
 _FUSED_DTYPES = (torch.float32, torch.float16, torch.bfloat16)
 _BACKENDS = ("auto", "hip", "triton")
 
+# Volume layout the kernels read: "stencil" (cached stencil records of the
+# volume, one load per sample), "linear" (the (D, H, W) volume itself) or
+# "auto" (stencil unless the volume requires grad, which would force a
+# re-layout after every update).
+VOLUME_LAYOUT = "auto"
+
 
 def fused_supported(subject, B: int, n_pixels: int) -> bool:
     """Hard limits of the kernels: one volume, 32-bit indexing, <= 8 classes (backends.py:66-69)."""
@@ -107,6 +113,12 @@ def render_hip(subject, rt_inv, src, tgt, step_size, n_samples, height, width) -
         M = M.expand(B, -1, -1)
     if not fused_supported(subject, B, height * width):
         raise ValueError("inputs exceed the hip kernel's limits (one volume, 32-bit indexing, <= 8 classes)")
+    layout = VOLUME_LAYOUT
+    if layout == "auto":
+        layout = "linear" if image.requires_grad else "stencil"
+    if layout not in ("linear", "stencil"):
+        raise ValueError(f"VOLUME_LAYOUT must be 'auto', 'linear' or 'stencil', got {VOLUME_LAYOUT!r}")
+    packed = subject.stencil_volume() if layout == "stencil" else None
     out = fused_raymarch(
         vol,
         lab,
@@ -117,5 +129,6 @@ def render_hip(subject, rt_inv, src, tgt, step_size, n_samples, height, width) -
         n_samples,
         C,
         width,
+        packed,
     )
     return out.reshape(B, C, height, width)

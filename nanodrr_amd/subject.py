@@ -75,6 +75,8 @@ class Subject(torch.nn.Module):
         self.n_classes = int(max_label + 1) if max_label is not None else int(self.label.max().item()) + 1
         self._label_i8_key = None
         self._label_i8 = None
+        self._stencil_key = None
+        self._stencil = None
 
     # -- volume ---------------------------------------------------------------
     @property
@@ -102,6 +104,8 @@ class Subject(torch.nn.Module):
             result._image_mu_cache = hu_to_mu(result._image_hu, result.mu_water, result.mu_bone, result.hu_bone)
         result._label_i8_key = None
         result._label_i8 = None
+        result._stencil_key = None
+        result._stencil = None
         return result
 
     def with_volume_dtype(self, dtype: torch.dtype) -> "Subject":
@@ -112,6 +116,18 @@ class Subject(torch.nn.Module):
         out = Subject(image, self.label, self.voxel_to_world, self.world_to_voxel, self.voxel_to_grid,
                       self.isocenter, max_label=self.n_classes - 1, convert_to_mu=False)
         return out
+
+    def stencil_volume(self) -> torch.Tensor:
+        """Stencil records of `image` (eight trilinear corners per cell, see
+        ndrr_pack_stencil), cached until the image tensor changes."""
+        from ._ops import pack_stencil
+
+        img = self.image
+        key = (img.data_ptr(), img.device, img.dtype, img._version, tuple(img.shape))
+        if self._stencil_key != key:
+            self._stencil = pack_stencil(img[0, 0])
+            self._stencil_key = key
+        return self._stencil
 
     # -- labels ---------------------------------------------------------------
     @property

@@ -56,7 +56,7 @@ def test_workspace_size_is_12_floats_per_tile():
 
 
 def _fwd(lib, **kw):
-    args = dict(vol=1, dt=0, lab=None, M=1, src=1, tgt=1, step=1, out=1, B=1, N=16, D=4, H=4, W=4, S=8, C=1,
+    args = dict(vol=1, dt=0, layout=0, lab=None, M=1, src=1, tgt=1, step=1, out=1, B=1, N=16, D=4, H=4, W=4, S=8, C=1,
                 width=4, sbs=0, srs=0, tbs=0, pbs=0, stream=None)
     args.update(kw)
     return lib.ndrr_raymarch_fwd(*args.values())

@@ -29,6 +29,15 @@ FWD_TOL = 1e-5
 GRAD_TOL = 1e-4
 
 
+@pytest.fixture(params=["stencil", "linear"], autouse=True)
+def volume_layout(request, monkeypatch):
+    """Every test runs on both volume layouts the kernels read."""
+    import nanodrr_amd.renderer as renderer
+
+    monkeypatch.setattr(renderer, "VOLUME_LAYOUT", request.param)
+    return request.param
+
+
 def camera(device=DEV, height=16, width=16):
     sdd = 100.0
     k_inv = make_k_inv(sdd, 1.0, 1.0, 0.0, 0.0, height, width, device=device)
@@ -70,10 +79,13 @@ def test_forward_matches_oracle(orthographic, n_classes):
 
 @pytest.mark.parametrize("name,ortho,C", [("persp_c1", False, 1), ("ortho_c1", True, 1),
                                           ("persp_c3", False, 3), ("ortho_c3", True, 3)])
-def test_against_golden_vectors(name, ortho, C):
+def test_against_golden_vectors(name, ortho, C, volume_layout):
     """Forward and pose/endpoint gradients vs the committed float64 fixtures."""
+    from nanodrr_amd._ops import pack_stencil
+
     g = np.load(GOLDEN)
     vol = torch.from_numpy(g["volume"]).to(DEV)
+    packed = pack_stencil(vol) if volume_layout == "stencil" else None
     lab = torch.from_numpy(g["label3"]).to(DEV) if C > 1 else torch.empty(0, dtype=torch.int8, device=DEV)
     D, H, W = vol.shape
     w2v = np.eye(4)
@@ -84,7 +96,7 @@ def test_against_golden_vectors(name, ortho, C):
     M = orm.camera_to_voxel(w2v, g["rt_inv"])
     t = {k: torch.tensor(v, dtype=torch.float32, device=DEV).requires_grad_(True)
          for k, v in (("M", M), ("src", src), ("tgt", tgt), ("step", step))}
-    out = fused_raymarch(vol, lab, t["M"], t["src"], t["tgt"], t["step"], 200, C, 16)
+    out = fused_raymarch(vol, lab, t["M"], t["src"], t["tgt"], t["step"], 200, C, 16, packed)
     _assert_parity(g[f"{name}_fwd"], out.detach().cpu().numpy(), C)
     (out * torch.tensor(g[f"{name}_go"], dtype=torch.float32, device=DEV)).sum().backward()
     tol = GRAD_TOL if C == 1 else 2e-3  # label flips move single samples between channels
@@ -132,6 +144,8 @@ def test_pose_gradients_match_reference_backend(orthographic):
 
 @pytest.mark.parametrize("n_classes", [1, 3])
 def test_volume_gradients_match_reference_backend(n_classes):
+    """A volume that requires grad: 'auto' would pick the linear layout; 'stencil'
+    re-packs it and must give the same gradient (the atomics target the linear volume)."""
     ref, wts = _torch_ref_grads(n_classes, False, ["image"], 2)
     subject = random_subject(n_classes=n_classes, device=DEV)
     subject._image_hu = subject._image_hu.detach().requires_grad_(True)
@@ -287,3 +301,20 @@ def test_hip_graph_replays_match_eager():
         torch.testing.assert_close(out_static, eager_out, rtol=0, atol=0)
         torch.testing.assert_close(rt_static.grad, eager_grad, rtol=0, atol=0)
         _ = (eager_out * 2).sum().item()  # eager work between replays
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16, torch.float16])
+def test_stencil_records_hold_the_eight_corners(dtype):
+    """ndrr_pack_stencil: record (x0, y0, z0) holds the zero-padded corners
+    vol[z0+a, y0+b, x0+c] in (a, b, c) order, bit for bit."""
+    from nanodrr_amd._ops import pack_stencil
+
+    torch.manual_seed(0)
+    D, H, W = 5, 6, 7
+    vol = torch.randn(D, H, W, device=DEV).to(dtype)
+    rec = pack_stencil(vol)
+    pad = torch.nn.functional.pad(vol.float(), (1, 1, 1, 1, 1, 1)).to(dtype)  # (D+2, H+2, W+2)
+    corners = [pad[a:a + D + 1, b:b + H + 1, c:c + W + 1] for a in (0, 1) for b in (0, 1) for c in (0, 1)]
+    expect = torch.stack(corners, dim=-1)  # (D+1, H+1, W+1, 8)
+    got = rec.view(dtype).reshape(D + 1, H + 1, W + 1, 8)
+    assert torch.equal(got.view(torch.uint8), expect.contiguous().view(torch.uint8))
