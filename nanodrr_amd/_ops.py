@@ -46,6 +46,7 @@ def _volume_arg(vol, packed):
     return packed.data_ptr(), _lib.LAYOUT_STENCIL
 
 
+@torch.library.custom_op("nanodrr_amd::pack_stencil", mutates_args=())
 def pack_stencil(vol: torch.Tensor) -> torch.Tensor:
     """Stencil records (uint8 bytes) of a contiguous (D, H, W) device volume (ndrr_pack_stencil)."""
     if not vol.is_cuda or vol.dim() != 3 or vol.dtype not in _DTYPE_CODE:
@@ -59,6 +60,13 @@ def pack_stencil(vol: torch.Tensor) -> torch.Tensor:
                                    _stream(vol.device))
     _lib.check(rc, "ndrr_pack_stencil")
     return out
+
+
+@pack_stencil.register_fake
+def _(vol):
+    D, H, W = vol.shape
+    per = 32 if vol.dtype == torch.float32 else 16
+    return torch.empty((D + 1) * (H + 1) * (W + 1) * per, device=vol.device, dtype=torch.uint8)
 
 
 def _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width):
@@ -230,6 +238,7 @@ def _backward(ctx, go):
 torch.library.register_autograd("nanodrr_amd::fused_raymarch", _backward, setup_context=_setup_context)
 
 
+@torch.library.custom_op("nanodrr_amd::labels_to_i8", mutates_args=())
 def labels_to_i8(label: torch.Tensor, n_classes: int) -> torch.Tensor:
     """int8 label map for the kernels: trunc(label) if in [0, n_classes) else -1."""
     if not label.is_cuda:
@@ -241,3 +250,8 @@ def labels_to_i8(label: torch.Tensor, n_classes: int) -> torch.Tensor:
                                           _stream(src.device))
     _lib.check(rc, "ndrr_labels_to_i8")
     return dst
+
+
+@labels_to_i8.register_fake
+def _(label, n_classes):
+    return torch.empty(label.shape, device=label.device, dtype=torch.int8)

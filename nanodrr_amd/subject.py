@@ -123,6 +123,8 @@ class Subject(torch.nn.Module):
         from ._ops import pack_stencil
 
         img = self.image
+        if torch.compiler.is_compiling():  # traced graphs re-pack (the cache is Python state)
+            return pack_stencil(img[0, 0])
         key = (img.data_ptr(), img.device, img.dtype, img._version, tuple(img.shape))
         if self._stencil_key != key:
             self._stencil = pack_stencil(img[0, 0])
@@ -137,6 +139,8 @@ class Subject(torch.nn.Module):
         from ._ops import labels_to_i8
 
         lab = self.label
+        if torch.compiler.is_compiling():  # traced graphs convert (the cache is Python state)
+            return labels_to_i8(lab[0, 0], self.n_classes)
         key = (lab.data_ptr(), lab.device, lab._version, tuple(lab.shape), self.n_classes)
         if self._label_i8_key != key:
             self._label_i8 = labels_to_i8(lab[0, 0], self.n_classes)

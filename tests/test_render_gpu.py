@@ -248,13 +248,14 @@ def test_volume_gradient_refused_under_determinism():
         torch.use_deterministic_algorithms(False)
 
 
-def test_compiles_as_opaque_custom_op():
+@pytest.mark.parametrize("n_classes", [1, 3])
+def test_compiles_as_opaque_custom_op(n_classes):
     """torch.compile(fullgraph=True) traces both launches as opaque ops (test_fused.py:147-169);
     aot_eager keeps the surrounding ops in eager ATen (no generated kernels)."""
-    subject = random_subject(device=DEV)
+    subject = random_subject(n_classes=n_classes, device=DEV)
     k_inv, rt_inv, sdd, h, w = camera()
     torch.manual_seed(4)
-    wts = torch.randn(1, 1, h, w, device=DEV)
+    wts = torch.randn(1, n_classes, h, w, device=DEV)
 
     def f(rt):
         return render(subject, k_inv, rt, sdd, h, w, n_samples=200)
