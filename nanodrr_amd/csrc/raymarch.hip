@@ -81,13 +81,61 @@ __device__ inline int ray_index(const Params& p, int blk, int tid, bool& valid) 
   return n;
 }
 
+// ---------------------------------------------------------------------------
+// Double-float (hi + lo) arithmetic for the per-ray set-up. Sample positions
+// are p(s) = A + k * delta with the anchor A = p(s_lo) and the per-sample
+// step delta = M (tgt - src) / (S - 1) both carried to ~2^-44 relative, so a
+// sample lands within ~1 ulp of its exact position. Plain fp32 (p0 + t d)
+// errs by ~1e-4 voxel at the ~1000-voxel ray lengths of a cone beam, enough
+// to move samples across cell faces, where the trilinear derivative jumps.
+// ---------------------------------------------------------------------------
+struct DF {
+  float hi, lo;
+};
+
+__device__ inline DF fast_two_sum(float a, float b) {  // |a| >= |b|
+#pragma clang fp contract(off)
+  const float s = a + b;
+  return {s, b - (s - a)};
+}
+
+__device__ inline DF two_sum(float a, float b) {
+#pragma clang fp contract(off)
+  const float s = a + b;
+  const float bb = s - a;
+  return {s, (a - (s - bb)) + (b - bb)};
+}
+
+__device__ inline DF two_prod(float a, float b) {
+#pragma clang fp contract(off)
+  const float p = a * b;
+  return {p, fmaf(a, b, -p)};
+}
+
+__device__ inline DF df_add(DF a, DF b) {
+#pragma clang fp contract(off)
+  const DF s = two_sum(a.hi, b.hi);
+  return fast_two_sum(s.hi, s.lo + (a.lo + b.lo));
+}
+
+__device__ inline DF df_mul(DF a, float b) {
+  DF p = two_prod(a.hi, b);
+  return fast_two_sum(p.hi, fmaf(a.lo, b, p.lo));
+}
+
+__device__ inline DF df_div(DF a, float b) {  // b an exact small integer
+  const float q = a.hi / b;
+  const float r = fmaf(-q, b, a.hi);  // exact remainder
+  return fast_two_sum(q, (r + a.lo) / b);
+}
+
 struct Ray {
   float sx, sy, sz;     // source, camera space
   float ex, ey, ez;     // tgt - src, camera space
-  float p0x, p0y, p0z;  // source, voxel space
-  float dx, dy, dz;     // tgt - src, voxel space
+  DF ax, ay, az;        // voxel-space position of sample s_lo
+  DF dx, dy, dz;        // voxel-space step per sample
   float step;
-  int s_lo, s_hi;       // samples [s_lo, s_hi) can touch the volume
+  int s_lo, n_s;        // samples [s_lo, s_lo + n_s) can touch the volume
 };
 
 __device__ inline void axis_range(float p0, float d, float lo, float hi, float& t0, float& t1) {
@@ -104,24 +152,26 @@ __device__ inline Ray setup_ray(const Params& p, const float* __restrict__ Mb, i
   const float* sp = p.src + b * p.src_bs + nn * p.src_rs;
   const float* tp = p.tgt + b * p.tgt_bs + 3 * static_cast<int64_t>(nn);
   r.sx = sp[0]; r.sy = sp[1]; r.sz = sp[2];
-  const float tx = tp[0], ty = tp[1], tz = tp[2];
   r.step = p.step[b * p.step_bs + nn];
-  r.ex = tx - r.sx; r.ey = ty - r.sy; r.ez = tz - r.sz;
-  r.p0x = Mb[0] * r.sx + Mb[1] * r.sy + Mb[2] * r.sz + Mb[3];
-  r.p0y = Mb[4] * r.sx + Mb[5] * r.sy + Mb[6] * r.sz + Mb[7];
-  r.p0z = Mb[8] * r.sx + Mb[9] * r.sy + Mb[10] * r.sz + Mb[11];
-  r.dx = Mb[0] * r.ex + Mb[1] * r.ey + Mb[2] * r.ez;
-  r.dy = Mb[4] * r.ex + Mb[5] * r.ey + Mb[6] * r.ez;
-  r.dz = Mb[8] * r.ex + Mb[9] * r.ey + Mb[10] * r.ez;
+  const DF ex = two_sum(tp[0], -r.sx), ey = two_sum(tp[1], -r.sy), ez = two_sum(tp[2], -r.sz);
+  r.ex = ex.hi; r.ey = ey.hi; r.ez = ez.hi;
+  DF p0[3], d[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float m0 = Mb[4 * i], m1 = Mb[4 * i + 1], m2 = Mb[4 * i + 2], m3 = Mb[4 * i + 3];
+    p0[i] = df_add(df_add(df_add(two_prod(m0, r.sx), two_prod(m1, r.sy)), two_prod(m2, r.sz)), DF{m3, 0.0f});
+    d[i] = df_add(df_add(df_mul(ex, m0), df_mul(ey, m1)), df_mul(ez, m2));
+  }
 
+  // Clip to the samples whose footprint can reach the volume (fp32 is ample here).
   float ax0, ax1, ay0, ay1, az0, az1;
-  axis_range(r.p0x, r.dx, -1.0f - kMargin, static_cast<float>(p.W) + kMargin, ax0, ax1);
-  axis_range(r.p0y, r.dy, -1.0f - kMargin, static_cast<float>(p.H) + kMargin, ay0, ay1);
-  axis_range(r.p0z, r.dz, -1.0f - kMargin, static_cast<float>(p.D) + kMargin, az0, az1);
+  axis_range(p0[0].hi, d[0].hi, -1.0f - kMargin, static_cast<float>(p.W) + kMargin, ax0, ax1);
+  axis_range(p0[1].hi, d[1].hi, -1.0f - kMargin, static_cast<float>(p.H) + kMargin, ay0, ay1);
+  axis_range(p0[2].hi, d[2].hi, -1.0f - kMargin, static_cast<float>(p.D) + kMargin, az0, az1);
   const float t_lo = fmaxf(fmaxf(ax0, ay0), az0);
   const float t_hi = fminf(fminf(ax1, ay1), az1);
   r.s_lo = 0;
-  r.s_hi = 0;
+  r.n_s = 0;
   if (valid && t_lo <= t_hi) {
     // Two samples of slack below, three above: every sample whose footprint
     // reaches the volume is kept; the extra ones add exact zeros.
@@ -129,53 +179,81 @@ __device__ inline Ray setup_ray(const Params& p, const float* __restrict__ Mb, i
     const float f_hi = fminf(ceilf(t_hi * p.s_m1) + 3.0f, static_cast<float>(p.S));
     if (f_lo < f_hi) {
       r.s_lo = static_cast<int>(f_lo);
-      r.s_hi = static_cast<int>(f_hi);
+      r.n_s = static_cast<int>(f_hi) - r.s_lo;
     }
+  }
+  const float s0 = static_cast<float>(r.s_lo);
+  DF* const an[3] = {&r.ax, &r.ay, &r.az};
+  DF* const st[3] = {&r.dx, &r.dy, &r.dz};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const DF delta = df_div(d[i], p.s_m1);
+    *st[i] = delta;
+    *an[i] = df_add(p0[i], df_mul(delta, s0));
   }
   return r;
 }
 
-// The 2x2x2 stencil of one sample: corner values (zero outside the volume)
-// and fractional offsets. Corner vABC has z offset A, y offset B, x offset C.
-struct Stencil {
-  float v000, v001, v010, v011, v100, v101, v110, v111;
-  float fx, fy, fz;
+// Voxel-space position of sample s_lo + k.
+__device__ inline float sample_coord(const DF& a, const DF& d, float k) {
+  return fmaf(k, d.hi, a.hi) + fmaf(k, d.lo, a.lo);
+}
+
+// Cell of a sample: integer corner (ix, iy, iz) and fractions.
+struct Cell {
   int ix, iy, iz;
+  float fx, fy, fz;
 };
 
-template <int DT>
-__device__ inline Stencil gather_linear(const typename Vox<DT>::T* __restrict__ vol, int D, int H, int W,
-                                        float x, float y, float z) {
-  Stencil st;
+__device__ inline Cell cell_of(float x, float y, float z) {
+  Cell c;
   const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
-  st.fx = x - xf; st.fy = y - yf; st.fz = z - zf;
-  st.ix = static_cast<int>(xf); st.iy = static_cast<int>(yf); st.iz = static_cast<int>(zf);
-  const bool bx0 = static_cast<unsigned>(st.ix) < static_cast<unsigned>(W);
-  const bool bx1 = static_cast<unsigned>(st.ix + 1) < static_cast<unsigned>(W);
-  const bool by0 = static_cast<unsigned>(st.iy) < static_cast<unsigned>(H);
-  const bool by1 = static_cast<unsigned>(st.iy + 1) < static_cast<unsigned>(H);
-  const bool bz0 = static_cast<unsigned>(st.iz) < static_cast<unsigned>(D);
-  const bool bz1 = static_cast<unsigned>(st.iz + 1) < static_cast<unsigned>(D);
-  // Clamped addresses keep every load in bounds; the masks zero the padding.
-  const int cx0 = min(max(st.ix, 0), W - 1), cx1 = min(max(st.ix + 1, 0), W - 1);
-  const int cy0 = min(max(st.iy, 0), H - 1), cy1 = min(max(st.iy + 1, 0), H - 1);
-  const int cz0 = min(max(st.iz, 0), D - 1), cz1 = min(max(st.iz + 1, 0), D - 1);
-  const int r00 = (cz0 * H + cy0) * W, r01 = (cz0 * H + cy1) * W;
-  const int r10 = (cz1 * H + cy0) * W, r11 = (cz1 * H + cy1) * W;
-  const float a000 = Vox<DT>::ld(vol, r00 + cx0), a001 = Vox<DT>::ld(vol, r00 + cx1);
-  const float a010 = Vox<DT>::ld(vol, r01 + cx0), a011 = Vox<DT>::ld(vol, r01 + cx1);
-  const float a100 = Vox<DT>::ld(vol, r10 + cx0), a101 = Vox<DT>::ld(vol, r10 + cx1);
-  const float a110 = Vox<DT>::ld(vol, r11 + cx0), a111 = Vox<DT>::ld(vol, r11 + cx1);
-  st.v000 = (bz0 && by0 && bx0) ? a000 : 0.0f;
-  st.v001 = (bz0 && by0 && bx1) ? a001 : 0.0f;
-  st.v010 = (bz0 && by1 && bx0) ? a010 : 0.0f;
-  st.v011 = (bz0 && by1 && bx1) ? a011 : 0.0f;
-  st.v100 = (bz1 && by0 && bx0) ? a100 : 0.0f;
-  st.v101 = (bz1 && by0 && bx1) ? a101 : 0.0f;
-  st.v110 = (bz1 && by1 && bx0) ? a110 : 0.0f;
-  st.v111 = (bz1 && by1 && bx1) ? a111 : 0.0f;
-  return st;
+  c.fx = x - xf; c.fy = y - yf; c.fz = z - zf;
+  c.ix = static_cast<int>(xf); c.iy = static_cast<int>(yf); c.iz = static_cast<int>(zf);
+  return c;
 }
+
+// Corner values vABC (z offset A, y offset B, x offset C), zero outside the volume.
+struct Corners {
+  float v000, v001, v010, v011, v100, v101, v110, v111;
+};
+
+// Raw data of one sample's cell as loaded; decoded into Corners at use, so
+// that several samples' loads can be in flight with few registers.
+template <int DT, int L> struct Fetch;
+
+// Linear layout: eight masked element loads (clamped addresses stay in bounds).
+template <int DT> struct Fetch<DT, NDRR_LAYOUT_LINEAR> {
+  static constexpr int kInFlight = 2;
+  float v[8];
+  __device__ inline void load(const void* __restrict__ base, int D, int H, int W, const Cell& c) {
+    const auto* __restrict__ vol = static_cast<const typename Vox<DT>::T*>(base);
+    const bool bx0 = static_cast<unsigned>(c.ix) < static_cast<unsigned>(W);
+    const bool bx1 = static_cast<unsigned>(c.ix + 1) < static_cast<unsigned>(W);
+    const bool by0 = static_cast<unsigned>(c.iy) < static_cast<unsigned>(H);
+    const bool by1 = static_cast<unsigned>(c.iy + 1) < static_cast<unsigned>(H);
+    const bool bz0 = static_cast<unsigned>(c.iz) < static_cast<unsigned>(D);
+    const bool bz1 = static_cast<unsigned>(c.iz + 1) < static_cast<unsigned>(D);
+    const int cx0 = min(max(c.ix, 0), W - 1), cx1 = min(max(c.ix + 1, 0), W - 1);
+    const int cy0 = min(max(c.iy, 0), H - 1), cy1 = min(max(c.iy + 1, 0), H - 1);
+    const int cz0 = min(max(c.iz, 0), D - 1), cz1 = min(max(c.iz + 1, 0), D - 1);
+    const int r00 = (cz0 * H + cy0) * W, r01 = (cz0 * H + cy1) * W;
+    const int r10 = (cz1 * H + cy0) * W, r11 = (cz1 * H + cy1) * W;
+    const float a000 = Vox<DT>::ld(vol, r00 + cx0), a001 = Vox<DT>::ld(vol, r00 + cx1);
+    const float a010 = Vox<DT>::ld(vol, r01 + cx0), a011 = Vox<DT>::ld(vol, r01 + cx1);
+    const float a100 = Vox<DT>::ld(vol, r10 + cx0), a101 = Vox<DT>::ld(vol, r10 + cx1);
+    const float a110 = Vox<DT>::ld(vol, r11 + cx0), a111 = Vox<DT>::ld(vol, r11 + cx1);
+    v[0] = (bz0 && by0 && bx0) ? a000 : 0.0f;
+    v[1] = (bz0 && by0 && bx1) ? a001 : 0.0f;
+    v[2] = (bz0 && by1 && bx0) ? a010 : 0.0f;
+    v[3] = (bz0 && by1 && bx1) ? a011 : 0.0f;
+    v[4] = (bz1 && by0 && bx0) ? a100 : 0.0f;
+    v[5] = (bz1 && by0 && bx1) ? a101 : 0.0f;
+    v[6] = (bz1 && by1 && bx0) ? a110 : 0.0f;
+    v[7] = (bz1 && by1 && bx1) ? a111 : 0.0f;
+  }
+  __device__ inline Corners corners() const { return {v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]}; }
+};
 
 // Stencil-record layout: one record per trilinear cell (x0, y0, z0) with
 // x0 in [-1, W-1], y0 in [-1, H-1], z0 in [-1, D-1], holding the cell's eight
@@ -194,54 +272,57 @@ __device__ inline float f16_hi(uint32_t w) {
   return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w >> 16)));
 }
 
-template <int DT>
-__device__ inline Stencil gather_stencil(const void* __restrict__ rec, int D, int H, int W,
-                                         float x, float y, float z) {
-  Stencil st;
-  const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
-  st.fx = x - xf; st.fy = y - yf; st.fz = z - zf;
-  st.ix = static_cast<int>(xf); st.iy = static_cast<int>(yf); st.iz = static_cast<int>(zf);
+__device__ inline int64_t record_index(int D, int H, int W, const Cell& c, bool& in) {
   // cells -1 .. dim-1 exist; any other cell has all eight corners outside
-  const bool in = static_cast<unsigned>(st.ix + 1) <= static_cast<unsigned>(W) &&
-                  static_cast<unsigned>(st.iy + 1) <= static_cast<unsigned>(H) &&
-                  static_cast<unsigned>(st.iz + 1) <= static_cast<unsigned>(D);
-  const int cx = min(max(st.ix + 1, 0), W), cy = min(max(st.iy + 1, 0), H), cz = min(max(st.iz + 1, 0), D);
-  const int64_t r = (static_cast<int64_t>(cz) * (H + 1) + cy) * (W + 1) + cx;
-  if constexpr (DT == NDRR_F32) {
-    const float4* p = static_cast<const float4*>(rec) + 2 * r;
-    const float4 a = p[0], b = p[1];
-    st.v000 = in ? a.x : 0.0f; st.v001 = in ? a.y : 0.0f; st.v010 = in ? a.z : 0.0f; st.v011 = in ? a.w : 0.0f;
-    st.v100 = in ? b.x : 0.0f; st.v101 = in ? b.y : 0.0f; st.v110 = in ? b.z : 0.0f; st.v111 = in ? b.w : 0.0f;
-  } else {
-    const uint4 q = static_cast<const uint4*>(rec)[r];
-    const uint4 w = in ? q : make_uint4(0u, 0u, 0u, 0u);  // +0 in both bf16 and fp16
-    if constexpr (DT == NDRR_BF16) {
-      st.v000 = bf16_lo(w.x); st.v001 = bf16_hi(w.x); st.v010 = bf16_lo(w.y); st.v011 = bf16_hi(w.y);
-      st.v100 = bf16_lo(w.z); st.v101 = bf16_hi(w.z); st.v110 = bf16_lo(w.w); st.v111 = bf16_hi(w.w);
-    } else {
-      st.v000 = f16_lo(w.x); st.v001 = f16_hi(w.x); st.v010 = f16_lo(w.y); st.v011 = f16_hi(w.y);
-      st.v100 = f16_lo(w.z); st.v101 = f16_hi(w.z); st.v110 = f16_lo(w.w); st.v111 = f16_hi(w.w);
-    }
+  in = static_cast<unsigned>(c.ix + 1) <= static_cast<unsigned>(W) &&
+       static_cast<unsigned>(c.iy + 1) <= static_cast<unsigned>(H) &&
+       static_cast<unsigned>(c.iz + 1) <= static_cast<unsigned>(D);
+  const int cx = min(max(c.ix + 1, 0), W), cy = min(max(c.iy + 1, 0), H), cz = min(max(c.iz + 1, 0), D);
+  return (static_cast<int64_t>(cz) * (H + 1) + cy) * (W + 1) + cx;
+}
+
+template <int DT> struct Fetch<DT, NDRR_LAYOUT_STENCIL> {  // bf16 / fp16: 16-byte records
+  static constexpr int kInFlight = 4;
+  uint4 q;
+  __device__ inline void load(const void* __restrict__ base, int D, int H, int W, const Cell& c) {
+    bool in;
+    const int64_t r = record_index(D, H, W, c, in);
+    const uint4 w = static_cast<const uint4*>(base)[r];
+    q = in ? w : make_uint4(0u, 0u, 0u, 0u);  // all-zero bits are +0 in bf16 and fp16
   }
-  return st;
-}
+  __device__ inline Corners corners() const {
+    if constexpr (DT == NDRR_BF16)
+      return {bf16_lo(q.x), bf16_hi(q.x), bf16_lo(q.y), bf16_hi(q.y),
+              bf16_lo(q.z), bf16_hi(q.z), bf16_lo(q.w), bf16_hi(q.w)};
+    else
+      return {f16_lo(q.x), f16_hi(q.x), f16_lo(q.y), f16_hi(q.y),
+              f16_lo(q.z), f16_hi(q.z), f16_lo(q.w), f16_hi(q.w)};
+  }
+};
 
-template <int DT, int L>
-__device__ inline Stencil gather(const void* __restrict__ vol, int D, int H, int W, float x, float y, float z) {
-  if constexpr (L == NDRR_LAYOUT_STENCIL)
-    return gather_stencil<DT>(vol, D, H, W, x, y, z);
-  else
-    return gather_linear<DT>(static_cast<const typename Vox<DT>::T*>(vol), D, H, W, x, y, z);
-}
+template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_STENCIL> {  // fp32: 32-byte records
+  static constexpr int kInFlight = 2;
+  float4 a, b;
+  __device__ inline void load(const void* __restrict__ base, int D, int H, int W, const Cell& c) {
+    bool in;
+    const int64_t r = record_index(D, H, W, c, in);
+    const float4* p = static_cast<const float4*>(base) + 2 * r;
+    const float4 x = p[0], y = p[1];
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    a = in ? x : z;
+    b = in ? y : z;
+  }
+  __device__ inline Corners corners() const { return {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w}; }
+};
 
-__device__ inline float lerp_value(const Stencil& s) {
-  const float c00 = s.v000 * (1.0f - s.fx) + s.v001 * s.fx;
-  const float c01 = s.v010 * (1.0f - s.fx) + s.v011 * s.fx;
-  const float c10 = s.v100 * (1.0f - s.fx) + s.v101 * s.fx;
-  const float c11 = s.v110 * (1.0f - s.fx) + s.v111 * s.fx;
-  const float c0 = c00 * (1.0f - s.fy) + c01 * s.fy;
-  const float c1 = c10 * (1.0f - s.fy) + c11 * s.fy;
-  return c0 * (1.0f - s.fz) + c1 * s.fz;
+__device__ inline float lerp_value(const Corners& v, const Cell& c) {
+  const float c00 = v.v000 * (1.0f - c.fx) + v.v001 * c.fx;
+  const float c01 = v.v010 * (1.0f - c.fx) + v.v011 * c.fx;
+  const float c10 = v.v100 * (1.0f - c.fx) + v.v101 * c.fx;
+  const float c11 = v.v110 * (1.0f - c.fx) + v.v111 * c.fx;
+  const float c0 = c00 * (1.0f - c.fy) + c01 * c.fy;
+  const float c1 = c10 * (1.0f - c.fy) + c11 * c.fy;
+  return c0 * (1.0f - c.fz) + c1 * c.fz;
 }
 
 // Nearest label (round half to even, zero padded: outside the volume is class 0).
@@ -260,10 +341,30 @@ __device__ inline int nearest_label(const int8_t* __restrict__ lab, int D, int H
 
 // Neumaier-compensated accumulation: sum + comp carries ~2x fp32 precision.
 __device__ inline void comp_add(float& sum, float& comp, float x) {
+#pragma clang fp contract(off)
   const float t = sum + x;
   comp += (fabsf(sum) >= fabsf(x)) ? ((sum - t) + x) : ((x - t) + sum);
   sum = t;
 }
+
+// One batch of the march: positions, cells, loads (and labels) of up to
+// kInFlight samples k0 .. k0 + kInFlight - 1 are issued before any is used.
+// Samples past the ray's range re-load its last sample and get weight 0.
+template <int DT, int L, bool LABELS>
+struct Batch {
+  static constexpr int U = Fetch<DT, L>::kInFlight;
+  Fetch<DT, L> f[U];
+  int cls[U];
+  __device__ inline void issue(const Params& p, const Ray& r, int k0) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const float k = static_cast<float>(min(k0 + u, r.n_s - 1));
+      const float x = sample_coord(r.ax, r.dx, k), y = sample_coord(r.ay, r.dy, k), z = sample_coord(r.az, r.dz, k);
+      f[u].load(p.vol, p.D, p.H, p.W, cell_of(x, y, z));
+      if constexpr (LABELS) cls[u] = nearest_label(p.lab, p.D, p.H, p.W, x, y, z);
+    }
+  }
+};
 
 // ---------------------------------------------------------------------------
 // Forward
@@ -282,21 +383,23 @@ __global__ __launch_bounds__(kThreads) void k_fwd(Params p) {
 #pragma unroll
   for (int c = 0; c < CV; ++c) { acc[c] = 0.0f; comp[c] = 0.0f; }
 
-  for (int s = r.s_lo; s < r.s_hi; ++s) {
-    const float t = static_cast<float>(s) * p.inv_s;
-    const float x = r.p0x + t * r.dx;
-    const float y = r.p0y + t * r.dy;
-    const float z = r.p0z + t * r.dz;
-    const Stencil st = gather<DT, L>(p.vol, p.D, p.H, p.W, x, y, z);
-    const float val = lerp_value(st);
-    if constexpr (CV == 1) {
-      if constexpr (KAHAN) comp_add(acc[0], comp[0], val); else acc[0] += val;
-    } else {
-      const int cls = nearest_label(p.lab, p.D, p.H, p.W, x, y, z);
+  using B = Batch<DT, L, (CV > 1)>;
+  for (int k0 = 0; k0 < r.n_s; k0 += B::U) {
+    B batch;
+    batch.issue(p, r, k0);
 #pragma unroll
-      for (int c = 0; c < CV; ++c) {
-        const float contrib = (cls == c) ? val : 0.0f;
-        if constexpr (KAHAN) comp_add(acc[c], comp[c], contrib); else acc[c] += contrib;
+    for (int u = 0; u < B::U; ++u) {
+      const float k = static_cast<float>(min(k0 + u, r.n_s - 1));
+      const Cell c = cell_of(sample_coord(r.ax, r.dx, k), sample_coord(r.ay, r.dy, k), sample_coord(r.az, r.dz, k));
+      const float val = (k0 + u < r.n_s) ? lerp_value(batch.f[u].corners(), c) : 0.0f;
+      if constexpr (CV == 1) {
+        if constexpr (KAHAN) comp_add(acc[0], comp[0], val); else acc[0] += val;
+      } else {
+#pragma unroll
+        for (int cl = 0; cl < CV; ++cl) {
+          const float contrib = (batch.cls[u] == cl) ? val : 0.0f;
+          if constexpr (KAHAN) comp_add(acc[cl], comp[cl], contrib); else acc[cl] += contrib;
+        }
       }
     }
   }
@@ -314,6 +417,10 @@ __global__ __launch_bounds__(kThreads) void k_fwd(Params p) {
 // ---------------------------------------------------------------------------
 // Backward (analytic adjoint)
 // ---------------------------------------------------------------------------
+// With q(t) = src + t e the camera-space sample, dL/dM[i][j] sums
+// gp_i q_j = gp_i src_j + (gp_i t) e_j over the ray, and dL/dsrc, dL/dtgt sum
+// (1 - t) M^T gp and t M^T gp: six per-ray sums G = sum gp and T = sum t gp
+// give all of them (the reference's kernel keeps 19, _fused.py:168-186).
 __device__ inline float wave_sum(float v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -327,9 +434,6 @@ __global__ __launch_bounds__(kThreads) void k_bwd(Params p) {
   const int n = ray_index(p, blockIdx.x, threadIdx.x, valid);
   const float* __restrict__ Mb = p.M + 12 * b;
   const Ray r = setup_ray(p, Mb, b, n, valid);
-  const float m00 = Mb[0], m01 = Mb[1], m02 = Mb[2];
-  const float m10 = Mb[4], m11 = Mb[5], m12 = Mb[6];
-  const float m20 = Mb[8], m21 = Mb[9], m22 = Mb[10];
 
   // Upstream gradient of this ray, one value per class.
   float goc[CV];
@@ -341,66 +445,53 @@ __global__ __launch_bounds__(kThreads) void k_bwd(Params p) {
       goc[c] = (valid && c < p.C) ? g[static_cast<int64_t>(c) * p.N] : 0.0f;
   }
 
-  float a00 = 0.f, a01 = 0.f, a02 = 0.f, a03 = 0.f;
-  float a10 = 0.f, a11 = 0.f, a12 = 0.f, a13 = 0.f;
-  float a20 = 0.f, a21 = 0.f, a22 = 0.f, a23 = 0.f;
-  float gsx = 0.f, gsy = 0.f, gsz = 0.f, gtx = 0.f, gty = 0.f, gtz = 0.f, astep = 0.f;
-
-  for (int s = r.s_lo; s < r.s_hi; ++s) {
-    const float t = static_cast<float>(s) * p.inv_s;
-    const float x = r.p0x + t * r.dx;
-    const float y = r.p0y + t * r.dy;
-    const float z = r.p0z + t * r.dz;
-    const Stencil st = gather<DT, L>(p.vol, p.D, p.H, p.W, x, y, z);
-
-    float gg;
-    if constexpr (CV == 1) {
-      gg = goc[0];
-    } else {
-      const int cls = nearest_label(p.lab, p.D, p.H, p.W, x, y, z);
-      gg = 0.0f;
+  float Gx = 0.f, Gy = 0.f, Gz = 0.f, Tx = 0.f, Ty = 0.f, Tz = 0.f, astep = 0.f;
+  using B = Batch<DT, L, (CV > 1)>;
+  for (int k0 = 0; k0 < r.n_s; k0 += B::U) {
+    B batch;
+    batch.issue(p, r, k0);
 #pragma unroll
-      for (int c = 0; c < CV; ++c) gg = (cls == c) ? goc[c] : gg;
-    }
-    const float g = gg * r.step;
-    const float fx = st.fx, fy = st.fy, fz = st.fz;
-
-    const float dvx = ((st.v001 - st.v000) * (1.0f - fy) + (st.v011 - st.v010) * fy) * (1.0f - fz) +
-                      ((st.v101 - st.v100) * (1.0f - fy) + (st.v111 - st.v110) * fy) * fz;
-    const float dvy = ((st.v010 - st.v000) * (1.0f - fx) + (st.v011 - st.v001) * fx) * (1.0f - fz) +
-                      ((st.v110 - st.v100) * (1.0f - fx) + (st.v111 - st.v101) * fx) * fz;
-    const float dvz = ((st.v100 - st.v000) * (1.0f - fx) + (st.v101 - st.v001) * fx) * (1.0f - fy) +
-                      ((st.v110 - st.v010) * (1.0f - fx) + (st.v111 - st.v011) * fx) * fy;
-    const float gx = g * dvx, gy = g * dvy, gz = g * dvz;
-
-    const float qx = r.sx + t * r.ex;
-    const float qy = r.sy + t * r.ey;
-    const float qz = r.sz + t * r.ez;
-    a00 += gx * qx; a01 += gx * qy; a02 += gx * qz; a03 += gx;
-    a10 += gy * qx; a11 += gy * qy; a12 += gy * qz; a13 += gy;
-    a20 += gz * qx; a21 += gz * qy; a22 += gz * qz; a23 += gz;
-
-    if constexpr (GCAM) {
-      const float gqx = m00 * gx + m10 * gy + m20 * gz;
-      const float gqy = m01 * gx + m11 * gy + m21 * gz;
-      const float gqz = m02 * gx + m12 * gy + m22 * gz;
-      gsx += (1.0f - t) * gqx; gsy += (1.0f - t) * gqy; gsz += (1.0f - t) * gqz;
-      gtx += t * gqx; gty += t * gqy; gtz += t * gqz;
-      astep += gg * lerp_value(st);
-    }
-    if constexpr (GVOL) {
-      if (g != 0.0f) {
-        const float wx[2] = {1.0f - fx, fx};
-        const float wy[2] = {1.0f - fy, fy};
-        const float wz[2] = {1.0f - fz, fz};
+    for (int u = 0; u < B::U; ++u) {
+      const bool live = k0 + u < r.n_s;
+      const float k = static_cast<float>(min(k0 + u, r.n_s - 1));
+      const Cell c = cell_of(sample_coord(r.ax, r.dx, k), sample_coord(r.ay, r.dy, k), sample_coord(r.az, r.dz, k));
+      const Corners v = batch.f[u].corners();
+      float gg;
+      if constexpr (CV == 1) {
+        gg = goc[0];
+      } else {
+        gg = 0.0f;
 #pragma unroll
-        for (int corner = 0; corner < 8; ++corner) {
-          const int cx = corner & 1, cy = (corner >> 1) & 1, cz = corner >> 2;
-          const int jx = st.ix + cx, jy = st.iy + cy, jz = st.iz + cz;
-          const bool inb = static_cast<unsigned>(jx) < static_cast<unsigned>(p.W) &&
-                           static_cast<unsigned>(jy) < static_cast<unsigned>(p.H) &&
-                           static_cast<unsigned>(jz) < static_cast<unsigned>(p.D);
-          if (inb) unsafeAtomicAdd(p.gvol + (jz * p.H + jy) * p.W + jx, g * (wx[cx] * wy[cy] * wz[cz]));
+        for (int cl = 0; cl < CV; ++cl) gg = (batch.cls[u] == cl) ? goc[cl] : gg;
+      }
+      gg = live ? gg : 0.0f;
+      const float g = gg * r.step;
+      const float fx = c.fx, fy = c.fy, fz = c.fz;
+      const float dvx = ((v.v001 - v.v000) * (1.0f - fy) + (v.v011 - v.v010) * fy) * (1.0f - fz) +
+                        ((v.v101 - v.v100) * (1.0f - fy) + (v.v111 - v.v110) * fy) * fz;
+      const float dvy = ((v.v010 - v.v000) * (1.0f - fx) + (v.v011 - v.v001) * fx) * (1.0f - fz) +
+                        ((v.v110 - v.v100) * (1.0f - fx) + (v.v111 - v.v101) * fx) * fz;
+      const float dvz = ((v.v100 - v.v000) * (1.0f - fx) + (v.v101 - v.v001) * fx) * (1.0f - fy) +
+                        ((v.v110 - v.v010) * (1.0f - fx) + (v.v111 - v.v011) * fx) * fy;
+      const float gx = g * dvx, gy = g * dvy, gz = g * dvz;
+      const float t = static_cast<float>(r.s_lo + k0 + u) * p.inv_s;
+      Gx += gx; Gy += gy; Gz += gz;
+      Tx += gx * t; Ty += gy * t; Tz += gz * t;
+      if constexpr (GCAM) astep += gg * lerp_value(v, c);
+      if constexpr (GVOL) {
+        if (g != 0.0f) {
+          const float wx[2] = {1.0f - fx, fx};
+          const float wy[2] = {1.0f - fy, fy};
+          const float wz[2] = {1.0f - fz, fz};
+#pragma unroll
+          for (int corner = 0; corner < 8; ++corner) {
+            const int cx = corner & 1, cy = (corner >> 1) & 1, cz = corner >> 2;
+            const int jx = c.ix + cx, jy = c.iy + cy, jz = c.iz + cz;
+            const bool inb = static_cast<unsigned>(jx) < static_cast<unsigned>(p.W) &&
+                             static_cast<unsigned>(jy) < static_cast<unsigned>(p.H) &&
+                             static_cast<unsigned>(jz) < static_cast<unsigned>(p.D);
+            if (inb) unsafeAtomicAdd(p.gvol + (jz * p.H + jy) * p.W + jx, g * (wx[cx] * wy[cy] * wz[cz]));
+          }
         }
       }
     }
@@ -408,9 +499,18 @@ __global__ __launch_bounds__(kThreads) void k_bwd(Params p) {
 
   if constexpr (GCAM) {
     if (valid) {
+      const float m00 = Mb[0], m01 = Mb[1], m02 = Mb[2];
+      const float m10 = Mb[4], m11 = Mb[5], m12 = Mb[6];
+      const float m20 = Mb[8], m21 = Mb[9], m22 = Mb[10];
+      const float qGx = m00 * Gx + m10 * Gy + m20 * Gz;  // M33^T G
+      const float qGy = m01 * Gx + m11 * Gy + m21 * Gz;
+      const float qGz = m02 * Gx + m12 * Gy + m22 * Gz;
+      const float qTx = m00 * Tx + m10 * Ty + m20 * Tz;  // M33^T T
+      const float qTy = m01 * Tx + m11 * Ty + m21 * Tz;
+      const float qTz = m02 * Tx + m12 * Ty + m22 * Tz;
       const int64_t ray = static_cast<int64_t>(b) * p.N + n;
-      p.gsrc[3 * ray + 0] = gsx; p.gsrc[3 * ray + 1] = gsy; p.gsrc[3 * ray + 2] = gsz;
-      p.gtgt[3 * ray + 0] = gtx; p.gtgt[3 * ray + 1] = gty; p.gtgt[3 * ray + 2] = gtz;
+      p.gsrc[3 * ray + 0] = qGx - qTx; p.gsrc[3 * ray + 1] = qGy - qTy; p.gsrc[3 * ray + 2] = qGz - qTz;
+      p.gtgt[3 * ray + 0] = qTx; p.gtgt[3 * ray + 1] = qTy; p.gtgt[3 * ray + 2] = qTz;
       p.gstep[ray] = astep;
     }
   }
@@ -418,7 +518,9 @@ __global__ __launch_bounds__(kThreads) void k_bwd(Params p) {
   // Deterministic per-tile pose-gradient partials: butterfly within each
   // wave, then the four wave sums in fixed order.
   __shared__ float red[kThreads / 64][12];
-  float a[12] = {a00, a01, a02, a03, a10, a11, a12, a13, a20, a21, a22, a23};
+  const float a[12] = {r.sx * Gx + r.ex * Tx, r.sy * Gx + r.ey * Tx, r.sz * Gx + r.ez * Tx, Gx,
+                       r.sx * Gy + r.ex * Ty, r.sy * Gy + r.ey * Ty, r.sz * Gy + r.ez * Ty, Gy,
+                       r.sx * Gz + r.ex * Tz, r.sy * Gz + r.ey * Tz, r.sz * Gz + r.ez * Tz, Gz};
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int j = 0; j < 12; ++j) {

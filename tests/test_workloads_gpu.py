@@ -93,11 +93,9 @@ def test_small_workload_forward_and_pose_gradient():
     _, gM, gsrc, gtgt, gstep = fused_raymarch_bwd(go, subject.image[0, 0], subject.label_i8.new_empty(0), M, src,
                                                   tsub, ssub, 256, 1, 0, False, True)
     g = orm.backward(_np(go), vol, None, 1, _np(M), _np(src), _np(tsub), _np(ssub), 256, need_gvol=False)
-    # fp32 sample positions (~1e-5 voxel) flip samples across cell faces, where the
-    # trilinear derivative jumps: the phantom's sharp edges make that visible in gM
     print(f"pose-gradient rel-to-max error: {rel_to_max(g['gM'], _np(gM)):.2e}")
-    assert rel_to_max(g["gM"], _np(gM)) < 5e-4
-    assert rel_to_max(g["gtgt"], _np(gtgt)) < 5e-4
+    assert rel_to_max(g["gM"], _np(gM)) < 1e-4
+    assert rel_to_max(g["gtgt"], _np(gtgt)) < 1e-4
     assert rel_to_max(g["gstep"], _np(gstep)) < 1e-4
 
 
@@ -118,19 +116,36 @@ def test_bench_workload_forward_sampled_rays(bench_fp32):
     _check_channels(ref, _np(out)[:, :, rays], 4, 1e-5)
 
 
-def test_bench_workload_pose_gradient_ray_subset(bench_fp32):
+@pytest.mark.parametrize("layout", ["stencil", "linear"])
+def test_bench_workload_pose_gradient_ray_subset(bench_fp32, layout):
+    """Pose gradient of 4 poses x 768 sampled rays vs the oracle. With the same
+    upstream gradient in every channel the label route cannot change a sample's
+    weight, so gM is checked tightly; with per-channel gradients a label flip at a
+    tissue boundary changes one sample's weight, so per-ray results are checked
+    by the fraction of rays that differ (as the reference's forward parity does)."""
+    from nanodrr_amd._ops import pack_stencil
+
     ph, subject, k_inv, rt_inv, sdd = bench_fp32
+    vol_t = subject.image[0, 0]
+    packed = pack_stencil(vol_t) if layout == "stencil" else None
     M, src, tgt, step = _op_inputs(subject, k_inv, rt_inv, sdd, 200, 256)
     M = M[:4].contiguous()
     rays = _pick_rays(40000, 768, 3)
     tsub, ssub = _subset(tgt, step, rays)
-    go = torch.randn(4, 4, len(rays), generator=torch.Generator().manual_seed(4)).to(DEV)
-    _, gM, _, gtgt, gstep = fused_raymarch_bwd(go, subject.image[0, 0], subject.label_i8, M, src, tsub, ssub, 256,
-                                               4, 0, False, True)
     vol, lab = _oracle_vol(ph, 4)
+    go1 = torch.randn(4, 1, len(rays), generator=torch.Generator().manual_seed(4)).to(DEV).expand(-1, 4, -1)
+    gM = fused_raymarch_bwd(go1, vol_t, subject.label_i8, M, src, tsub, ssub, 256, 4, 0, False, False, packed)[1]
+    g = orm.backward(_np(go1), vol, lab, 4, _np(M), _np(src), _np(tsub), _np(ssub), 256, need_gvol=False)
+    print(f"pose-gradient rel-to-max error (C=4, shared go): {rel_to_max(g['gM'], _np(gM)):.2e}")
+    assert rel_to_max(g["gM"], _np(gM)) < 1e-4
+    go = torch.randn(4, 4, len(rays), generator=torch.Generator().manual_seed(5)).to(DEV)
+    _, _, _, gtgt, gstep = fused_raymarch_bwd(go, vol_t, subject.label_i8, M, src, tsub, ssub, 256, 4, 0, False,
+                                              True, packed)
     g = orm.backward(_np(go), vol, lab, 4, _np(M), _np(src), _np(tsub), _np(ssub), 256, need_gvol=False)
-    assert rel_to_max(g["gM"], _np(gM)) < 2e-3  # label flips at tissue boundaries move single samples
-    assert rel_to_max(g["gstep"], _np(gstep)) < 2e-3
+    for key, got in (("gstep", gstep), ("gtgt", gtgt)):
+        ref = g[key].reshape(4, len(rays), -1)
+        err = np.abs(ref - _np(got).reshape(ref.shape)).max(axis=-1) / np.abs(ref).max()
+        assert (err > 1e-4).mean() < 5e-3, (key, (err > 1e-4).mean())
 
 
 def test_bench_workload_exact_linearity(bench_fp32):
@@ -185,27 +200,31 @@ def test_render_fwd_bwd_matches_op_path(bench_fp32):
 
 @pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
 def test_half_volume_error_against_float64(dtype, bench_fp32):
-    """bf16/fp16 storage, fp32 accumulation: image and pose-gradient error vs the float64
-    oracle of the full-precision volume stays <= 1e-3 relative to the largest value."""
+    """bf16/fp16 storage (stencil records), fp32 accumulation. Against the float64
+    oracle of the stored (rounded) volume the kernel's own error is <= 1e-5 (image)
+    and <= 1e-4 (pose gradient); against the full-precision volume the image error
+    stays <= 1e-3 and the pose-gradient error is what the storage rounding costs."""
+    from nanodrr_amd._ops import pack_stencil
+
     ph, _, k_inv, rt_inv, sdd = bench_fp32
     subject = phantom_subject(ph, n_classes=4, dtype=dtype, device=DEV)
+    vol_t = subject.image[0, 0]
+    packed = pack_stencil(vol_t)
     M, src, tgt, step = _op_inputs(subject, k_inv, rt_inv, sdd, 200, 256)
-    out = fused_raymarch(subject.image[0, 0], subject.label_i8, M, src, tgt, step, 256, 4, 200)
-    vol, lab = _oracle_vol(ph, 4)
+    out = fused_raymarch(vol_t, subject.label_i8, M, src, tgt, step, 256, 4, 200, packed)
     rays = _pick_rays(40000, 400, 6)
-    ref = orm.forward(vol, lab, 4, _np(M), _np(src), _np(tgt), _np(step), 256, rays=rays)
-    err = rel_to_max(ref.sum(1), _np(out)[:, :, rays].sum(1))
-    print(f"{dtype} image rel-to-max error vs fp64 (fp32 volume): {err:.2e}")
-    assert err < 1e-3
     tsub, ssub = _subset(tgt, step, rays)
     Ms = M[:2].contiguous()
-    go = torch.randn(2, 4, len(rays), generator=torch.Generator().manual_seed(7)).to(DEV)
-    gM = fused_raymarch_bwd(go, subject.image[0, 0], subject.label_i8, Ms, src, tsub, ssub, 256, 4, 0, False,
-                            False)[1]
-    g = orm.backward(_np(go), vol, lab, 4, _np(Ms), _np(src), _np(tsub), _np(ssub), 256, need_gvol=False)
-    gerr = rel_to_max(g["gM"], _np(gM))
-    print(f"{dtype} pose-gradient rel-to-max error vs fp64: {gerr:.2e}")
-    assert gerr < 1e-2
+    go = torch.randn(2, 1, len(rays), generator=torch.Generator().manual_seed(7)).to(DEV).expand(-1, 4, -1)
+    gM = fused_raymarch_bwd(go, vol_t, subject.label_i8, Ms, src, tsub, ssub, 256, 4, 0, False, False, packed)[1]
+    for name, vol_dtype, img_tol, grad_tol in (("stored", dtype, 1e-5, 1e-4), ("fp32", torch.float32, 1e-3, 5e-2)):
+        vol, lab = _oracle_vol(ph, 4, vol_dtype)
+        ref = orm.forward(vol, lab, 4, _np(M), _np(src), _np(tgt), _np(step), 256, rays=rays)
+        err = rel_to_max(ref.sum(1), _np(out)[:, :, rays].sum(1))
+        g = orm.backward(_np(go), vol, lab, 4, _np(Ms), _np(src), _np(tsub), _np(ssub), 256, need_gvol=False)
+        gerr = rel_to_max(g["gM"], _np(gM))
+        print(f"{dtype} vs float64 oracle of the {name} volume: image {err:.2e}, pose gradient {gerr:.2e}")
+        assert err < img_tol and gerr < grad_tol
 
 
 def test_compensated_accumulation_long_rays(bench_fp32):
