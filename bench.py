@@ -75,8 +75,8 @@ def kernel_times(subject, k_inv, rt_inv, sdd, wts, reps, layout):
     from nanodrr_amd.renderer import _make_src, _make_tgt
 
     vol = subject.image[0, 0]
-    packed = subject.stencil_volume() if layout == "stencil" else None
-    lab = subject.label_i8
+    packed = None if layout == "linear" else subject.packed_volume(layout)
+    lab = subject.packed_labels(layout)
     M = (subject.world_to_voxel.double() @ rt_inv.double())[:, :3, :].float().contiguous()
     tgt = _make_tgt(k_inv, sdd, DET, DET, k_inv.device, torch.float32).contiguous()
     src = _make_src(False, tgt, sdd).contiguous()
@@ -95,9 +95,9 @@ def kernel_times(subject, k_inv, rt_inv, sdd, wts, reps, layout):
         torch.cuda.synchronize()
         return t0.elapsed_time(t1) / reps
 
-    ms_fwd = timed(lambda: fused_raymarch(vol, lab, M, src, tgt, step, S, N_CLASSES, DET, packed))
+    ms_fwd = timed(lambda: fused_raymarch(vol, lab, M, src, tgt, step, S, N_CLASSES, DET, packed, layout))
     ms_bwd = timed(lambda: fused_raymarch_bwd(go, vol, lab, M, src, tgt, step, S, N_CLASSES, DET, False, False,
-                                              packed))
+                                              packed, layout))
     return ms_fwd, ms_bwd, (M, src, tgt)
 
 
@@ -153,7 +153,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--dtypes", default="bf16,fp32")
-    ap.add_argument("--layout", default="stencil", choices=["stencil", "linear"])
+    ap.add_argument("--layout", default="quad", choices=["quad", "stencil", "linear"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))

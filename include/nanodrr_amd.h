@@ -22,10 +22,12 @@
  *   step per-ray Riemann weight, element (b, n) at step[b * step_batch_stride + n]
  *   out  [B][C][N] fp32 line integrals
  *   vol  the volume in NDRR_F32 / NDRR_BF16 / NDRR_F16, laid out as
- *        NDRR_LAYOUT_LINEAR:  [D][H][W], W fastest, or
- *        NDRR_LAYOUT_STENCIL: stencil records made by ndrr_pack_stencil
- *   lab  [D][H][W] int8 class per voxel (values outside [0, C) are dropped);
- *        ignored when n_classes == 1
+ *        NDRR_LAYOUT_LINEAR:  [D][H][W], W fastest, or a packed copy made by
+ *        ndrr_pack_volume: NDRR_LAYOUT_QUAD or NDRR_LAYOUT_STENCIL records
+ *   lab  int8 class per voxel (values outside [0, C) are dropped) in the form
+ *        vol_layout reads: the [D][H][W] map for NDRR_LAYOUT_LINEAR and
+ *        NDRR_LAYOUT_STENCIL, its label quads (ndrr_pack_labels) for
+ *        NDRR_LAYOUT_QUAD; ignored when n_classes == 1
  * Sample s of ray (b, n) sits at M[b] @ [lerp(src, tgt, s / (S - 1)), 1].
  * Rays are mapped to 16 x 16 detector tiles of a `width`-wide image when
  * width > 0 (N must be a multiple of width), else to runs of 256 rays.
@@ -50,7 +52,8 @@ enum {
 /* Volume layouts. */
 enum {
   NDRR_LAYOUT_LINEAR = 0,  /* [D][H][W] elements */
-  NDRR_LAYOUT_STENCIL = 1  /* one record of 8 trilinear corners per cell */
+  NDRR_LAYOUT_STENCIL = 1, /* one record of 8 trilinear corners per cell */
+  NDRR_LAYOUT_QUAD = 2     /* one record of a (y, z) corner quad per x */
 };
 
 /* Argument errors (negative return codes). */
@@ -123,19 +126,37 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype,
                       void* stream);
 
 /*
- * Stencil-record layout of a [D][H][W] volume: one record per trilinear cell
- * (x0, y0, z0), x0 in [-1, W-1], y0 in [-1, H-1], z0 in [-1, D-1], x fastest,
- * each holding the cell's eight corner values (zero outside the volume) in
- * the order (z, y, x) = 000, 001, 010, 011, 100, 101, 110, 111: 16 bytes for
- * bf16/fp16, 32 bytes for fp32. A march sample reads one record instead of
- * gathering eight voxels. ndrr_stencil_bytes gives the size of `out`
- * (0 for invalid arguments); ndrr_pack_stencil fills it from the linear volume.
+ * Packed copies of a [D][H][W] volume that the march reads with one load per
+ * sample (two 16-byte loads for fp32):
+ *   NDRR_LAYOUT_QUAD: one record per (x, y0, z0), x in [-1, W], y0 in [-1, H-1],
+ *     z0 in [-1, D-1] (x fastest), holding v[z0+a][y0+b][x] for
+ *     (a, b) = 00, 01, 10, 11, zero outside the volume; a sample's cell is the
+ *     two adjacent records x0 and x0 + 1. 4 copies of each voxel.
+ *   NDRR_LAYOUT_STENCIL: one record per cell (x0, y0, z0), x0 in [-1, W-1] etc.,
+ *     holding its eight corners in the order (z, y, x) = 000, 001, 010, 011,
+ *     100, 101, 110, 111. 8 copies of each voxel.
+ * ndrr_packed_bytes gives the size of `out` (0 for invalid arguments; for
+ * NDRR_LAYOUT_LINEAR the volume's own size); ndrr_pack_volume fills it from the
+ * linear volume on `stream`.
  * Replaces the per-sample gathers of _fused.py:205-212 (_sample, :11-16) with
  * a one-off re-layout of the subject's volume.
  */
-size_t ndrr_stencil_bytes(int D, int H, int W, int vol_dtype);
-int ndrr_pack_stencil(const void* vol, int vol_dtype, int D, int H, int W,
-                      void* out, void* stream);
+size_t ndrr_packed_bytes(int D, int H, int W, int vol_dtype, int vol_layout);
+int ndrr_pack_volume(const void* vol, int vol_dtype, int vol_layout, int D, int H, int W,
+                     void* out, void* stream);
+
+/*
+ * Labels in the form a layout's march reads. For NDRR_LAYOUT_QUAD: one uint32
+ * per quad record (x, y0, z0) holding the int8 classes lab[z0+a][y0+b][x] in
+ * byte 2a + b (zero outside the volume), so the nearest-label lookup shares
+ * the volume's record index; for the other layouts the [D][H][W] int8 map
+ * itself (copied). ndrr_packed_label_bytes gives the size of `out`.
+ * Replaces the nearest-label gather of _fused.py:222-228 (rounding,
+ * zero padding, `.to(tl.int32)`), which the kernels then resolve from the
+ * sample's cell fractions.
+ */
+size_t ndrr_packed_label_bytes(int D, int H, int W, int vol_layout);
+int ndrr_pack_labels(const int8_t* lab, int vol_layout, int D, int H, int W, void* out, void* stream);
 
 /*
  * Label map for the kernels: dst[i] = trunc(src[i]) when that lies in

@@ -13,7 +13,8 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().with_name("libnanodrr_amd.so")
 
 NDRR_F32, NDRR_BF16, NDRR_F16 = 0, 1, 2
-LAYOUT_LINEAR, LAYOUT_STENCIL = 0, 1
+LAYOUT_LINEAR, LAYOUT_STENCIL, LAYOUT_QUAD = 0, 1, 2
+LAYOUTS = {"linear": LAYOUT_LINEAR, "stencil": LAYOUT_STENCIL, "quad": LAYOUT_QUAD}
 MAX_CLASSES = 8
 
 _c_int, _c_i64, _c_p, _c_size = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_size_t
@@ -37,8 +38,10 @@ SIGNATURES: dict[str, tuple] = {
          _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
     ),
     "ndrr_labels_to_i8": (_c_int, [_c_p, _c_p, _c_i64, _c_int, _c_p]),
-    "ndrr_stencil_bytes": (_c_size, [_c_int, _c_int, _c_int, _c_int]),
-    "ndrr_pack_stencil": (_c_int, [_c_p, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p]),
+    "ndrr_packed_bytes": (_c_size, [_c_int, _c_int, _c_int, _c_int, _c_int]),
+    "ndrr_pack_volume": (_c_int, [_c_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p]),
+    "ndrr_packed_label_bytes": (_c_size, [_c_int, _c_int, _c_int, _c_int]),
+    "ndrr_pack_labels": (_c_int, [_c_p, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p]),
 }
 
 

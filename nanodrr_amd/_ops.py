@@ -9,9 +9,9 @@ differences:
     pose) instead of being expanded and copied per pose;
   * the output is always fp32 (an fp32 accumulation rounded to a bf16 output
     alone costs ~2^-9 relative error);
-  * an optional `packed` volume (stencil records of `vol`, see
-    `pack_stencil`) is what the kernels read when given; `vol` still fixes
-    the shape, dtype and the layout of the volume gradient;
+  * an optional `packed` copy of `vol` in another layout (`pack_volume`) is
+    what the kernels read when given, with `layout` naming it; `vol` still
+    fixes the shape, dtype and the layout of the volume gradient;
   * the launch goes through the C ABI of include/nanodrr_amd.h.
 """
 
@@ -34,42 +34,75 @@ def _stream(device: torch.device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 
 
-def _volume_arg(vol, packed):
-    """(pointer, layout) the kernels read: the stencil records when given, else `vol`."""
+def _volume_arg(vol, packed, layout):
+    """(pointer, layout code) the kernels read: the packed copy when given, else `vol`."""
     if packed is None or packed.numel() == 0:
         return vol.data_ptr(), _lib.LAYOUT_LINEAR
+    code = _lib.LAYOUTS.get(layout)
+    if code is None or code == _lib.LAYOUT_LINEAR:
+        raise ValueError(f"layout must name the packed copy ('quad' or 'stencil'), got {layout!r}")
     D, H, W = vol.shape
-    need = _lib.lib().ndrr_stencil_bytes(D, H, W, _DTYPE_CODE[vol.dtype])
+    need = _lib.lib().ndrr_packed_bytes(D, H, W, _DTYPE_CODE[vol.dtype], code)
     if packed.dtype != torch.uint8 or packed.numel() != need or not packed.is_contiguous() \
             or packed.device != vol.device:
         raise ValueError(f"packed must be a contiguous uint8 tensor of {need} bytes on {vol.device}")
-    return packed.data_ptr(), _lib.LAYOUT_STENCIL
+    return packed.data_ptr(), code
 
 
-@torch.library.custom_op("nanodrr_amd::pack_stencil", mutates_args=())
-def pack_stencil(vol: torch.Tensor) -> torch.Tensor:
-    """Stencil records (uint8 bytes) of a contiguous (D, H, W) device volume (ndrr_pack_stencil)."""
+@torch.library.custom_op("nanodrr_amd::pack_volume", mutates_args=())
+def pack_volume(vol: torch.Tensor, layout: str) -> torch.Tensor:
+    """`layout` copy (uint8 bytes) of a (D, H, W) device volume (ndrr_pack_volume)."""
     if not vol.is_cuda or vol.dim() != 3 or vol.dtype not in _DTYPE_CODE:
-        raise ValueError("pack_stencil needs a (D, H, W) float32/bfloat16/float16 device tensor")
+        raise ValueError("pack_volume needs a (D, H, W) float32/bfloat16/float16 device tensor")
+    if layout not in _lib.LAYOUTS:
+        raise ValueError(f"unknown layout {layout!r}; expected one of {sorted(_lib.LAYOUTS)}")
     vol = vol.detach().contiguous()
     D, H, W = vol.shape
     lib = _lib.lib()
-    out = torch.empty(lib.ndrr_stencil_bytes(D, H, W, _DTYPE_CODE[vol.dtype]), device=vol.device, dtype=torch.uint8)
+    code, dt = _lib.LAYOUTS[layout], _DTYPE_CODE[vol.dtype]
+    out = torch.empty(lib.ndrr_packed_bytes(D, H, W, dt, code), device=vol.device, dtype=torch.uint8)
     with torch.cuda.device(vol.device):
-        rc = lib.ndrr_pack_stencil(vol.data_ptr(), _DTYPE_CODE[vol.dtype], D, H, W, out.data_ptr(),
-                                   _stream(vol.device))
-    _lib.check(rc, "ndrr_pack_stencil")
+        rc = lib.ndrr_pack_volume(vol.data_ptr(), dt, code, D, H, W, out.data_ptr(), _stream(vol.device))
+    _lib.check(rc, "ndrr_pack_volume")
     return out
 
 
-@pack_stencil.register_fake
-def _(vol):
+@pack_volume.register_fake
+def _(vol, layout):
     D, H, W = vol.shape
-    per = 32 if vol.dtype == torch.float32 else 16
-    return torch.empty((D + 1) * (H + 1) * (W + 1) * per, device=vol.device, dtype=torch.uint8)
+    e = 4 if vol.dtype == torch.float32 else 2
+    n = {"linear": D * H * W, "stencil": (D + 1) * (H + 1) * (W + 1) * 8, "quad": (D + 1) * (H + 1) * (W + 2) * 4}
+    return torch.empty(n[layout] * e, device=vol.device, dtype=torch.uint8)
 
 
-def _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width):
+@torch.library.custom_op("nanodrr_amd::pack_labels", mutates_args=())
+def pack_labels(lab: torch.Tensor, layout: str) -> torch.Tensor:
+    """Labels in the form `layout`'s march reads (ndrr_pack_labels): the label quads
+    for "quad", a copy of the (D, H, W) int8 map otherwise."""
+    if not lab.is_cuda or lab.dim() != 3 or lab.dtype != torch.int8:
+        raise ValueError("pack_labels needs a (D, H, W) int8 device tensor (Subject.label_i8)")
+    if layout not in _lib.LAYOUTS:
+        raise ValueError(f"unknown layout {layout!r}; expected one of {sorted(_lib.LAYOUTS)}")
+    lab = lab.contiguous()
+    D, H, W = lab.shape
+    lib = _lib.lib()
+    code = _lib.LAYOUTS[layout]
+    out = torch.empty(lib.ndrr_packed_label_bytes(D, H, W, code), device=lab.device, dtype=torch.int8)
+    with torch.cuda.device(lab.device):
+        rc = lib.ndrr_pack_labels(lab.data_ptr(), code, D, H, W, out.data_ptr(), _stream(lab.device))
+    _lib.check(rc, "ndrr_pack_labels")
+    return out if layout == "quad" else out.view(D, H, W)
+
+
+@pack_labels.register_fake
+def _(lab, layout):
+    D, H, W = lab.shape
+    if layout == "quad":
+        return torch.empty((D + 1) * (H + 1) * (W + 2) * 4, device=lab.device, dtype=torch.int8)
+    return torch.empty(D, H, W, device=lab.device, dtype=torch.int8)
+
+
+def _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width, layout="linear"):
     """Validate shapes/devices and return the launch geometry (raises before any launch)."""
     if not vol.is_cuda:
         raise ValueError("nanodrr_amd renders on a GPU: the volume must be a device tensor")
@@ -94,9 +127,15 @@ def _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width):
     for name, t in (("M", M), ("src", src), ("tgt", tgt), ("step", step)):
         if t.dtype != torch.float32 or not t.is_contiguous() or t.device != vol.device:
             raise ValueError(f"{name} must be a contiguous float32 tensor on {vol.device}")
-    if n_classes > 1 and (lab.shape != vol.shape or lab.dtype != torch.int8 or not lab.is_contiguous()
-                          or lab.device != vol.device):
-        raise ValueError("lab must be a contiguous int8 (D, H, W) tensor on the volume's device")
+    if n_classes > 1:
+        if lab.dtype != torch.int8 or not lab.is_contiguous() or lab.device != vol.device:
+            raise ValueError("lab must be a contiguous int8 tensor on the volume's device")
+        if layout == "quad":
+            need = _lib.lib().ndrr_packed_label_bytes(*vol.shape, _lib.LAYOUT_QUAD)
+            if lab.numel() != need:
+                raise ValueError(f"the quad layout reads label quads (pack_labels): {need} bytes, got {lab.numel()}")
+        elif lab.shape != vol.shape:
+            raise ValueError("lab must be the (D, H, W) int8 label map for this layout")
     if width < 0 or (width > 0 and N % width != 0):
         raise ValueError(f"width {width} must be 0 or divide the ray count {N}")
     D, H, W = vol.shape
@@ -121,14 +160,16 @@ def fused_raymarch(
     n_classes: int,
     width: int,
     packed: Optional[torch.Tensor] = None,
+    layout: str = "linear",
 ) -> torch.Tensor:
     """Line integrals (B, C, N) fp32 of the rays through `vol` (see ndrr_raymarch_fwd)."""
-    B, N, D, H, W, strides = _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width)
-    vptr, layout = _volume_arg(vol, packed)
+    B, N, D, H, W, strides = _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width,
+                                       layout if packed is not None else "linear")
+    vptr, code = _volume_arg(vol, packed, layout)
     out = torch.empty(B, n_classes, N, device=vol.device, dtype=torch.float32)
     with torch.cuda.device(vol.device):
         rc = _lib.lib().ndrr_raymarch_fwd(
-            vptr, _DTYPE_CODE[vol.dtype], layout, _ptr(lab) if n_classes > 1 else None,
+            vptr, _DTYPE_CODE[vol.dtype], code, _ptr(lab) if n_classes > 1 else None,
             M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(), out.data_ptr(),
             B, N, D, H, W, n_samples, n_classes, width, *strides, _stream(vol.device),
         )
@@ -137,7 +178,7 @@ def fused_raymarch(
 
 
 @fused_raymarch.register_fake
-def _(vol, lab, M, src, tgt, step, n_samples, n_classes, width, packed=None):
+def _(vol, lab, M, src, tgt, step, n_samples, n_classes, width, packed=None, layout="linear"):
     return torch.empty(M.shape[0], n_classes, tgt.shape[1], device=vol.device, dtype=torch.float32)
 
 
@@ -156,6 +197,7 @@ def fused_raymarch_bwd(
     need_gvol: bool,
     need_gcam: bool,
     packed: Optional[torch.Tensor] = None,
+    layout: str = "linear",
 ) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
     """Analytic adjoint (see ndrr_raymarch_bwd). Returns (gvol|empty, gM (B,3,4),
     gsrc (B,N,3)|empty, gtgt (B,N,3)|empty, gstep (B,N)|empty), all fp32."""
@@ -164,8 +206,9 @@ def fused_raymarch_bwd(
             "Volume gradients on the hip backend use nondeterministic atomics; "
             "disable deterministic algorithms to request them"
         )
-    B, N, D, H, W, strides = _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width)
-    vptr, layout = _volume_arg(vol, packed)
+    B, N, D, H, W, strides = _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width,
+                                       layout if packed is not None else "linear")
+    vptr, code = _volume_arg(vol, packed, layout)
     go = go.to(torch.float32).contiguous()
     if go.shape != (B, n_classes, N):
         raise ValueError(f"go must be (B, C, N) = {(B, n_classes, N)}, got {tuple(go.shape)}")
@@ -180,7 +223,7 @@ def fused_raymarch_bwd(
     gstep = torch.empty(B, N, device=dev, dtype=torch.float32) if need_gcam else gM.new_empty(0)
     with torch.cuda.device(dev):
         rc = lib.ndrr_raymarch_bwd(
-            go.data_ptr(), vptr, _DTYPE_CODE[vol.dtype], layout, _ptr(lab) if n_classes > 1 else None,
+            go.data_ptr(), vptr, _DTYPE_CODE[vol.dtype], code, _ptr(lab) if n_classes > 1 else None,
             M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(),
             ws.data_ptr(), gM.data_ptr(), _ptr(gsrc), _ptr(gtgt), _ptr(gstep), _ptr(gvol),
             int(need_gcam), B, N, D, H, W, n_samples, n_classes, width, *strides, _stream(dev),
@@ -190,7 +233,8 @@ def fused_raymarch_bwd(
 
 
 @fused_raymarch_bwd.register_fake
-def _(go, vol, lab, M, src, tgt, step, n_samples, n_classes, width, need_gvol, need_gcam, packed=None):
+def _(go, vol, lab, M, src, tgt, step, n_samples, n_classes, width, need_gvol, need_gcam, packed=None,
+      layout="linear"):
     B, N = M.shape[0], tgt.shape[1]
     gM = torch.empty(B, 3, 4, device=M.device, dtype=torch.float32)
     gvol = torch.empty(vol.shape, device=M.device, dtype=torch.float32) if need_gvol else gM.new_empty(0)
@@ -201,9 +245,9 @@ def _(go, vol, lab, M, src, tgt, step, n_samples, n_classes, width, need_gvol, n
 
 
 def _setup_context(ctx, inputs, output):
-    vol, lab, M, src, tgt, step, n_samples, n_classes, width, packed = inputs
+    vol, lab, M, src, tgt, step, n_samples, n_classes, width, packed, layout = inputs
     ctx.save_for_backward(vol, lab, M, src, tgt, step, packed)
-    ctx.n_samples, ctx.n_classes, ctx.width = n_samples, n_classes, width
+    ctx.n_samples, ctx.n_classes, ctx.width, ctx.layout = n_samples, n_classes, width, layout
 
 
 def _reduce_to(g: torch.Tensor, like: torch.Tensor) -> torch.Tensor:
@@ -219,7 +263,8 @@ def _backward(ctx, go):
     need_gvol = ctx.needs_input_grad[0]
     need_gcam = any(ctx.needs_input_grad[i] for i in (3, 4, 5))
     gvol, gM, gsrc, gtgt, gstep = fused_raymarch_bwd(
-        go, vol, lab, M, src, tgt, step, ctx.n_samples, ctx.n_classes, ctx.width, need_gvol, need_gcam, packed
+        go, vol, lab, M, src, tgt, step, ctx.n_samples, ctx.n_classes, ctx.width, need_gvol, need_gcam, packed,
+        ctx.layout,
     )
     return (
         gvol.to(vol.dtype) if need_gvol else None,
@@ -232,6 +277,7 @@ def _backward(ctx, go):
         None,  # n_classes
         None,  # width
         None,  # packed: a re-layout of vol; its gradient is vol's
+        None,  # layout
     )
 
 

@@ -315,6 +315,56 @@ template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_STENCIL> {  // fp32: 32-byte reco
   __device__ inline Corners corners() const { return {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w}; }
 };
 
+// Quad layout: one record per (x, y0, z0), x in [-1, W], y0 in [-1, H-1],
+// z0 in [-1, D-1], holding the (y, z) quad v[z0+a][y0+b][x] (zero outside the
+// volume) in the order (a, b) = 00, 01, 10, 11; records x-fastest. The cell
+// (x0, y0, z0) is records x0 and x0 + 1, adjacent in memory: one 16-byte load
+// (bf16/fp16) or two (fp32) per sample, with each voxel stored 4 times
+// instead of the stencil's 8, so a 256^3 bf16 volume (136 MB) stays resident
+// in the 256 MB Infinity Cache and adjacent rays share records along x.
+__device__ inline int64_t quad_index(int D, int H, int W, const Cell& c, bool& in) {
+  in = static_cast<unsigned>(c.ix + 1) <= static_cast<unsigned>(W) &&
+       static_cast<unsigned>(c.iy + 1) <= static_cast<unsigned>(H) &&
+       static_cast<unsigned>(c.iz + 1) <= static_cast<unsigned>(D);
+  const int cx = min(max(c.ix + 1, 0), W), cy = min(max(c.iy + 1, 0), H), cz = min(max(c.iz + 1, 0), D);
+  return (static_cast<int64_t>(cz) * (H + 1) + cy) * (W + 2) + cx;
+}
+
+template <int DT> struct Fetch<DT, NDRR_LAYOUT_QUAD> {  // bf16 / fp16: two 8-byte records
+  static constexpr int kInFlight = 4;
+  uint4 q;  // x: (z0,y0) (z0,y0+1) at x0; y: (z0+1,y0) (z0+1,y0+1) at x0; z, w: the same at x0 + 1
+  __device__ inline void load(const void* __restrict__ base, int D, int H, int W, const Cell& c) {
+    bool in;
+    const int64_t r = quad_index(D, H, W, c, in);
+    uint4 w;
+    __builtin_memcpy(&w, static_cast<const uint2*>(base) + r, sizeof(uint4));  // 8-byte aligned
+    q = in ? w : make_uint4(0u, 0u, 0u, 0u);
+  }
+  __device__ inline Corners corners() const {
+    if constexpr (DT == NDRR_BF16)
+      return {bf16_lo(q.x), bf16_lo(q.z), bf16_hi(q.x), bf16_hi(q.z),
+              bf16_lo(q.y), bf16_lo(q.w), bf16_hi(q.y), bf16_hi(q.w)};
+    else
+      return {f16_lo(q.x), f16_lo(q.z), f16_hi(q.x), f16_hi(q.z),
+              f16_lo(q.y), f16_lo(q.w), f16_hi(q.y), f16_hi(q.w)};
+  }
+};
+
+template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_QUAD> {  // fp32: two 16-byte records
+  static constexpr int kInFlight = 2;
+  float4 a, b;  // quads at x0 and x0 + 1: (z0,y0) (z0,y0+1) (z0+1,y0) (z0+1,y0+1)
+  __device__ inline void load(const void* __restrict__ base, int D, int H, int W, const Cell& c) {
+    bool in;
+    const int64_t r = quad_index(D, H, W, c, in);
+    const float4* p = static_cast<const float4*>(base) + r;
+    const float4 x = p[0], y = p[1];
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    a = in ? x : z;
+    b = in ? y : z;
+  }
+  __device__ inline Corners corners() const { return {a.x, b.x, a.y, b.y, a.z, b.z, a.w, b.w}; }
+};
+
 __device__ inline float lerp_value(const Corners& v, const Cell& c) {
   const float c00 = v.v000 * (1.0f - c.fx) + v.v001 * c.fx;
   const float c01 = v.v010 * (1.0f - c.fx) + v.v011 * c.fx;
@@ -339,6 +389,24 @@ __device__ inline int nearest_label(const int8_t* __restrict__ lab, int D, int H
   return inb ? l : 0;
 }
 
+// Nearest label from label quads (the quad layout's labels: one uint32 per
+// quad record holding the int8 classes lab[z0+a][y0+b][x] in byte 2a + b, zero
+// outside the volume). The nearest voxel is a corner of the sample's cell:
+// offset 1 along an axis when the fraction exceeds 1/2, or equals it with an
+// odd cell index (round half to even, as rintf and grid_sample's nearest).
+__device__ inline int quad_label(const int8_t* __restrict__ lq, int D, int H, int W, const Cell& c) {
+  bool in;
+  const int64_t r = quad_index(D, H, W, c, in);
+  uint2 w;
+  __builtin_memcpy(&w, reinterpret_cast<const uint32_t*>(lq) + r, sizeof(uint2));  // 4-byte aligned
+  const bool ox = c.fx > 0.5f || (c.fx == 0.5f && (c.ix & 1));
+  const bool oy = c.fy > 0.5f || (c.fy == 0.5f && (c.iy & 1));
+  const bool oz = c.fz > 0.5f || (c.fz == 0.5f && (c.iz & 1));
+  const uint32_t word = ox ? w.y : w.x;
+  const int l = static_cast<int8_t>(static_cast<uint8_t>(word >> ((oz ? 16 : 0) + (oy ? 8 : 0))));
+  return in ? l : 0;
+}
+
 // Neumaier-compensated accumulation: sum + comp carries ~2x fp32 precision.
 __device__ inline void comp_add(float& sum, float& comp, float x) {
 #pragma clang fp contract(off)
@@ -354,14 +422,21 @@ template <int DT, int L, bool LABELS>
 struct Batch {
   static constexpr int U = Fetch<DT, L>::kInFlight;
   Fetch<DT, L> f[U];
+  Cell cell[U];
   int cls[U];
   __device__ inline void issue(const Params& p, const Ray& r, int k0) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const float k = static_cast<float>(min(k0 + u, r.n_s - 1));
       const float x = sample_coord(r.ax, r.dx, k), y = sample_coord(r.ay, r.dy, k), z = sample_coord(r.az, r.dz, k);
-      f[u].load(p.vol, p.D, p.H, p.W, cell_of(x, y, z));
-      if constexpr (LABELS) cls[u] = nearest_label(p.lab, p.D, p.H, p.W, x, y, z);
+      cell[u] = cell_of(x, y, z);
+      f[u].load(p.vol, p.D, p.H, p.W, cell[u]);
+      if constexpr (LABELS) {
+        if constexpr (L == NDRR_LAYOUT_QUAD)
+          cls[u] = quad_label(p.lab, p.D, p.H, p.W, cell[u]);
+        else
+          cls[u] = nearest_label(p.lab, p.D, p.H, p.W, x, y, z);
+      }
     }
   }
 };
@@ -369,7 +444,7 @@ struct Batch {
 // ---------------------------------------------------------------------------
 // Forward
 // ---------------------------------------------------------------------------
-// CV == 1: single channel (no label map). CV == 8: label map, runtime C <= 8.
+// CV == 1: single channel (no label map). CV in {2, 4, 8}: label map, runtime C <= CV.
 template <int DT, int L, int CV, bool KAHAN>
 __global__ __launch_bounds__(kThreads) void k_fwd(Params p) {
   const int b = blockIdx.y;
@@ -389,8 +464,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd(Params p) {
     batch.issue(p, r, k0);
 #pragma unroll
     for (int u = 0; u < B::U; ++u) {
-      const float k = static_cast<float>(min(k0 + u, r.n_s - 1));
-      const Cell c = cell_of(sample_coord(r.ax, r.dx, k), sample_coord(r.ay, r.dy, k), sample_coord(r.az, r.dz, k));
+      const Cell& c = batch.cell[u];
       const float val = (k0 + u < r.n_s) ? lerp_value(batch.f[u].corners(), c) : 0.0f;
       if constexpr (CV == 1) {
         if constexpr (KAHAN) comp_add(acc[0], comp[0], val); else acc[0] += val;
@@ -453,8 +527,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd(Params p) {
 #pragma unroll
     for (int u = 0; u < B::U; ++u) {
       const bool live = k0 + u < r.n_s;
-      const float k = static_cast<float>(min(k0 + u, r.n_s - 1));
-      const Cell c = cell_of(sample_coord(r.ax, r.dx, k), sample_coord(r.ay, r.dy, k), sample_coord(r.az, r.dz, k));
+      const Cell& c = batch.cell[u];
       const Corners v = batch.f[u].corners();
       float gg;
       if constexpr (CV == 1) {
@@ -604,6 +677,59 @@ __global__ void k_pack_stencil(const typename Vox<DT>::T* __restrict__ vol, void
   }
 }
 
+// One thread per quad record (x, y0, z0): v[z0+a][y0+b][x] for (a, b) = 00, 01, 10, 11.
+template <int DT>
+__global__ void k_pack_quad(const typename Vox<DT>::T* __restrict__ vol, void* __restrict__ rec,
+                            int D, int H, int W, int64_t n_rec) {
+  using T = typename Vox<DT>::T;
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n_rec) return;
+  const int cx = static_cast<int>(r % (W + 2));
+  const int64_t rest = r / (W + 2);
+  const int cy = static_cast<int>(rest % (H + 1));
+  const int cz = static_cast<int>(rest / (H + 1));
+  T v[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int x = cx - 1, y = cy - 1 + (c & 1), z = cz - 1 + (c >> 1);
+    const bool inb = static_cast<unsigned>(x) < static_cast<unsigned>(W) &&
+                     static_cast<unsigned>(y) < static_cast<unsigned>(H) &&
+                     static_cast<unsigned>(z) < static_cast<unsigned>(D);
+    v[c] = inb ? vol[(static_cast<int64_t>(z) * H + y) * W + x] : T(0);
+  }
+  if constexpr (DT == NDRR_F32) {
+    static_cast<float4*>(rec)[r] = make_float4(v[0], v[1], v[2], v[3]);
+  } else {
+    const uint32_t w0 = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, v[0])) |
+                        (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, v[1])) << 16);
+    const uint32_t w1 = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, v[2])) |
+                        (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, v[3])) << 16);
+    static_cast<uint2*>(rec)[r] = make_uint2(w0, w1);
+  }
+}
+
+// One thread per label-quad record (x, y0, z0): bytes lab[z0+a][y0+b][x] at 2a + b.
+__global__ void k_pack_label_quads(const int8_t* __restrict__ lab, uint32_t* __restrict__ out,
+                                   int D, int H, int W, int64_t n_rec) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n_rec) return;
+  const int cx = static_cast<int>(r % (W + 2));
+  const int64_t rest = r / (W + 2);
+  const int cy = static_cast<int>(rest % (H + 1));
+  const int cz = static_cast<int>(rest / (H + 1));
+  uint32_t w = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int x = cx - 1, y = cy - 1 + (c & 1), z = cz - 1 + (c >> 1);
+    const bool inb = static_cast<unsigned>(x) < static_cast<unsigned>(W) &&
+                     static_cast<unsigned>(y) < static_cast<unsigned>(H) &&
+                     static_cast<unsigned>(z) < static_cast<unsigned>(D);
+    const uint32_t v = inb ? static_cast<uint8_t>(lab[(static_cast<int64_t>(z) * H + y) * W + x]) : 0u;
+    w |= v << (8 * c);
+  }
+  out[r] = w;
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -622,6 +748,10 @@ int64_t stencil_records(int D, int H, int W) {
   return static_cast<int64_t>(D + 1) * (H + 1) * (W + 1);
 }
 
+int64_t quad_records(int D, int H, int W) {
+  return static_cast<int64_t>(D + 1) * (H + 1) * (W + 2);
+}
+
 bool valid_dtype(int dt) { return dt == NDRR_F32 || dt == NDRR_BF16 || dt == NDRR_F16; }
 
 int check_common(const void* vol, int vol_dtype, int vol_layout, const int8_t* lab, const float* M,
@@ -630,13 +760,15 @@ int check_common(const void* vol, int vol_dtype, int vol_layout, const int8_t* l
   if (!vol || !M || !src || !tgt || !step) return NDRR_EINVAL;
   if (B <= 0 || N <= 0 || D <= 0 || H <= 0 || W <= 0 || S < 2 || width < 0) return NDRR_EINVAL;
   if (width > 0 && N % width != 0) return NDRR_EINVAL;
-  if (vol_layout != NDRR_LAYOUT_LINEAR && vol_layout != NDRR_LAYOUT_STENCIL) return NDRR_EINVAL;
+  if (vol_layout != NDRR_LAYOUT_LINEAR && vol_layout != NDRR_LAYOUT_STENCIL && vol_layout != NDRR_LAYOUT_QUAD)
+    return NDRR_EINVAL;
   if (!valid_dtype(vol_dtype)) return NDRR_EDTYPE;
   if (C < 1 || C > NDRR_MAX_CLASSES) return NDRR_ECLASSES;
   if (C > 1 && !lab) return NDRR_EINVAL;
   if (B > 65535) return NDRR_ERANGE;
   const int64_t vox = static_cast<int64_t>(D) * H * W;
   if (vox >= (int64_t{1} << 31) || stencil_records(D, H, W) >= (int64_t{1} << 31) ||
+      quad_records(D, H, W) >= (int64_t{1} << 31) ||
       static_cast<int64_t>(C) * B * N >= (int64_t{1} << 31) || 3 * static_cast<int64_t>(B) * N >= (int64_t{1} << 31))
     return NDRR_ERANGE;
   return NDRR_OK;
@@ -672,6 +804,8 @@ void launch_fwd3(const Params& p, dim3 grid, hipStream_t st) {
 template <int DT, int L>
 void launch_fwd2(const Params& p, dim3 grid, hipStream_t st) {
   if (p.C == 1) launch_fwd3<DT, L, 1>(p, grid, st);
+  else if (p.C == 2) launch_fwd3<DT, L, 2>(p, grid, st);
+  else if (p.C <= 4) launch_fwd3<DT, L, 4>(p, grid, st);
   else launch_fwd3<DT, L, 8>(p, grid, st);
 }
 
@@ -686,18 +820,22 @@ void launch_bwd3(const Params& p, bool gcam, bool gvol, dim3 grid, hipStream_t s
 template <int DT, int L>
 void launch_bwd2(const Params& p, bool gcam, bool gvol, dim3 grid, hipStream_t st) {
   if (p.C == 1) launch_bwd3<DT, L, 1>(p, gcam, gvol, grid, st);
+  else if (p.C == 2) launch_bwd3<DT, L, 2>(p, gcam, gvol, grid, st);
+  else if (p.C <= 4) launch_bwd3<DT, L, 4>(p, gcam, gvol, grid, st);
   else launch_bwd3<DT, L, 8>(p, gcam, gvol, grid, st);
 }
 
 template <int DT>
 void launch_fwd1(const Params& p, int layout, dim3 grid, hipStream_t st) {
   if (layout == NDRR_LAYOUT_STENCIL) launch_fwd2<DT, NDRR_LAYOUT_STENCIL>(p, grid, st);
+  else if (layout == NDRR_LAYOUT_QUAD) launch_fwd2<DT, NDRR_LAYOUT_QUAD>(p, grid, st);
   else launch_fwd2<DT, NDRR_LAYOUT_LINEAR>(p, grid, st);
 }
 
 template <int DT>
 void launch_bwd1(const Params& p, int layout, bool gcam, bool gvol, dim3 grid, hipStream_t st) {
   if (layout == NDRR_LAYOUT_STENCIL) launch_bwd2<DT, NDRR_LAYOUT_STENCIL>(p, gcam, gvol, grid, st);
+  else if (layout == NDRR_LAYOUT_QUAD) launch_bwd2<DT, NDRR_LAYOUT_QUAD>(p, gcam, gvol, grid, st);
   else launch_bwd2<DT, NDRR_LAYOUT_LINEAR>(p, gcam, gvol, grid, st);
 }
 
@@ -778,33 +916,77 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_l
   return launch_status();
 }
 
-size_t ndrr_stencil_bytes(int D, int H, int W, int vol_dtype) {
+size_t ndrr_packed_bytes(int D, int H, int W, int vol_dtype, int vol_layout) {
   if (D <= 0 || H <= 0 || W <= 0 || !valid_dtype(vol_dtype)) return 0;
-  const size_t per = vol_dtype == NDRR_F32 ? Rec<NDRR_F32>::kBytes : Rec<NDRR_BF16>::kBytes;
-  return static_cast<size_t>(stencil_records(D, H, W)) * per;
+  const size_t e = vol_dtype == NDRR_F32 ? 4 : 2;
+  switch (vol_layout) {
+    case NDRR_LAYOUT_LINEAR: return static_cast<size_t>(D) * H * W * e;
+    case NDRR_LAYOUT_STENCIL: return static_cast<size_t>(stencil_records(D, H, W)) * 8 * e;
+    case NDRR_LAYOUT_QUAD: return static_cast<size_t>(quad_records(D, H, W)) * 4 * e;
+    default: return 0;
+  }
 }
 
-int ndrr_pack_stencil(const void* vol, int vol_dtype, int D, int H, int W, void* out, void* stream) {
+int ndrr_pack_volume(const void* vol, int vol_dtype, int vol_layout, int D, int H, int W, void* out,
+                     void* stream) {
   if (!vol || !out || D <= 0 || H <= 0 || W <= 0) return NDRR_EINVAL;
   if (!valid_dtype(vol_dtype)) return NDRR_EDTYPE;
-  const int64_t n = stencil_records(D, H, W);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (vol_layout == NDRR_LAYOUT_LINEAR) {
+    const hipError_t e = hipMemcpyAsync(out, vol, ndrr_packed_bytes(D, H, W, vol_dtype, vol_layout),
+                                        hipMemcpyDeviceToDevice, st);
+    return e == hipSuccess ? NDRR_OK : static_cast<int>(e);
+  }
+  if (vol_layout != NDRR_LAYOUT_STENCIL && vol_layout != NDRR_LAYOUT_QUAD) return NDRR_EINVAL;
+  const bool quad = vol_layout == NDRR_LAYOUT_QUAD;
+  const int64_t n = quad ? quad_records(D, H, W) : stencil_records(D, H, W);
   if (n >= (int64_t{1} << 31)) return NDRR_ERANGE;
   const dim3 grid(static_cast<unsigned>((n + kThreads - 1) / kThreads));
-  hipStream_t st = static_cast<hipStream_t>(stream);
   switch (vol_dtype) {
     case NDRR_F32:
-      hipLaunchKernelGGL(k_pack_stencil<NDRR_F32>, grid, dim3(kThreads), 0, st,
-                         static_cast<const float*>(vol), out, D, H, W, n);
+      if (quad) hipLaunchKernelGGL(k_pack_quad<NDRR_F32>, grid, dim3(kThreads), 0, st,
+                                   static_cast<const float*>(vol), out, D, H, W, n);
+      else hipLaunchKernelGGL(k_pack_stencil<NDRR_F32>, grid, dim3(kThreads), 0, st,
+                              static_cast<const float*>(vol), out, D, H, W, n);
       break;
     case NDRR_BF16:
-      hipLaunchKernelGGL(k_pack_stencil<NDRR_BF16>, grid, dim3(kThreads), 0, st,
-                         static_cast<const uint16_t*>(vol), out, D, H, W, n);
+      if (quad) hipLaunchKernelGGL(k_pack_quad<NDRR_BF16>, grid, dim3(kThreads), 0, st,
+                                   static_cast<const uint16_t*>(vol), out, D, H, W, n);
+      else hipLaunchKernelGGL(k_pack_stencil<NDRR_BF16>, grid, dim3(kThreads), 0, st,
+                              static_cast<const uint16_t*>(vol), out, D, H, W, n);
       break;
     default:
-      hipLaunchKernelGGL(k_pack_stencil<NDRR_F16>, grid, dim3(kThreads), 0, st,
-                         static_cast<const _Float16*>(vol), out, D, H, W, n);
+      if (quad) hipLaunchKernelGGL(k_pack_quad<NDRR_F16>, grid, dim3(kThreads), 0, st,
+                                   static_cast<const _Float16*>(vol), out, D, H, W, n);
+      else hipLaunchKernelGGL(k_pack_stencil<NDRR_F16>, grid, dim3(kThreads), 0, st,
+                              static_cast<const _Float16*>(vol), out, D, H, W, n);
       break;
   }
+  return launch_status();
+}
+
+size_t ndrr_packed_label_bytes(int D, int H, int W, int vol_layout) {
+  if (D <= 0 || H <= 0 || W <= 0) return 0;
+  switch (vol_layout) {
+    case NDRR_LAYOUT_LINEAR:
+    case NDRR_LAYOUT_STENCIL: return static_cast<size_t>(D) * H * W;
+    case NDRR_LAYOUT_QUAD: return static_cast<size_t>(quad_records(D, H, W)) * 4;
+    default: return 0;
+  }
+}
+
+int ndrr_pack_labels(const int8_t* lab, int vol_layout, int D, int H, int W, void* out, void* stream) {
+  if (!lab || !out || D <= 0 || H <= 0 || W <= 0) return NDRR_EINVAL;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (vol_layout == NDRR_LAYOUT_LINEAR || vol_layout == NDRR_LAYOUT_STENCIL) {
+    const hipError_t e = hipMemcpyAsync(out, lab, static_cast<size_t>(D) * H * W, hipMemcpyDeviceToDevice, st);
+    return e == hipSuccess ? NDRR_OK : static_cast<int>(e);
+  }
+  if (vol_layout != NDRR_LAYOUT_QUAD) return NDRR_EINVAL;
+  const int64_t n = quad_records(D, H, W);
+  if (n >= (int64_t{1} << 31)) return NDRR_ERANGE;
+  hipLaunchKernelGGL(k_pack_label_quads, dim3(static_cast<unsigned>((n + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, st, lab, static_cast<uint32_t*>(out), D, H, W, n);
   return launch_status();
 }
 

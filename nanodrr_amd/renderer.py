@@ -18,11 +18,13 @@ from ._ops import fused_raymarch
 _FUSED_DTYPES = (torch.float32, torch.float16, torch.bfloat16)
 _BACKENDS = ("auto", "hip", "triton")
 
-# Volume layout the kernels read: "stencil" (cached stencil records of the
-# volume, one load per sample), "linear" (the (D, H, W) volume itself) or
-# "auto" (stencil unless the volume requires grad, which would force a
-# re-layout after every update).
+# Volume layout the kernels read: "quad" (a cached copy holding each voxel's
+# (y, z) corner quads, one load per sample), "stencil" (a cached copy with all
+# eight corners per cell), "linear" (the (D, H, W) volume itself) or "auto"
+# (quad unless the volume requires grad, which would force a re-layout after
+# every update).
 VOLUME_LAYOUT = "auto"
+_LAYOUTS = ("linear", "quad", "stencil")
 
 
 def fused_supported(subject, B: int, n_pixels: int) -> bool:
@@ -105,7 +107,6 @@ def render_hip(subject, rt_inv, src, tgt, step_size, n_samples, height, width) -
     vol = image[0, 0]
     if not vol.is_contiguous():
         vol = vol.contiguous()
-    lab = subject.label_i8 if C > 1 else vol.new_empty(0, dtype=torch.int8)
     # Camera -> voxel-index affine, composed in float64 and rounded once.
     M = (subject.world_to_voxel.to(torch.float64) @ rt_inv.to(torch.float64))[:, :3, :].to(torch.float32)
     B = max(M.shape[0], tgt.shape[0], src.shape[0])
@@ -115,10 +116,11 @@ def render_hip(subject, rt_inv, src, tgt, step_size, n_samples, height, width) -
         raise ValueError("inputs exceed the hip kernel's limits (one volume, 32-bit indexing, <= 8 classes)")
     layout = VOLUME_LAYOUT
     if layout == "auto":
-        layout = "linear" if image.requires_grad else "stencil"
-    if layout not in ("linear", "stencil"):
-        raise ValueError(f"VOLUME_LAYOUT must be 'auto', 'linear' or 'stencil', got {VOLUME_LAYOUT!r}")
-    packed = subject.stencil_volume() if layout == "stencil" else None
+        layout = "linear" if image.requires_grad else "quad"
+    if layout not in _LAYOUTS:
+        raise ValueError(f"VOLUME_LAYOUT must be 'auto' or one of {_LAYOUTS}, got {VOLUME_LAYOUT!r}")
+    packed = None if layout == "linear" else subject.packed_volume(layout)
+    lab = subject.packed_labels(layout) if C > 1 else vol.new_empty(0, dtype=torch.int8)
     out = fused_raymarch(
         vol,
         lab,
@@ -130,5 +132,6 @@ def render_hip(subject, rt_inv, src, tgt, step_size, n_samples, height, width) -
         C,
         width,
         packed,
+        layout,
     )
     return out.reshape(B, C, height, width)

@@ -75,8 +75,10 @@ class Subject(torch.nn.Module):
         self.n_classes = int(max_label + 1) if max_label is not None else int(self.label.max().item()) + 1
         self._label_i8_key = None
         self._label_i8 = None
-        self._stencil_key = None
-        self._stencil = None
+        self._packed_key = None
+        self._packed = None
+        self._packed_lab_key = None
+        self._packed_lab = None
 
     # -- volume ---------------------------------------------------------------
     @property
@@ -104,8 +106,10 @@ class Subject(torch.nn.Module):
             result._image_mu_cache = hu_to_mu(result._image_hu, result.mu_water, result.mu_bone, result.hu_bone)
         result._label_i8_key = None
         result._label_i8 = None
-        result._stencil_key = None
-        result._stencil = None
+        result._packed_key = None
+        result._packed = None
+        result._packed_lab_key = None
+        result._packed_lab = None
         return result
 
     def with_volume_dtype(self, dtype: torch.dtype) -> "Subject":
@@ -117,21 +121,39 @@ class Subject(torch.nn.Module):
                       self.isocenter, max_label=self.n_classes - 1, convert_to_mu=False)
         return out
 
-    def stencil_volume(self) -> torch.Tensor:
-        """Stencil records of `image` (eight trilinear corners per cell, see
-        ndrr_pack_stencil), cached until the image tensor changes."""
-        from ._ops import pack_stencil
+    def packed_volume(self, layout: str) -> torch.Tensor:
+        """`image` re-laid out for the kernels ("quad" or "stencil", see
+        ndrr_pack_volume), cached until the image tensor or the layout changes."""
+        from ._ops import pack_volume
 
         img = self.image
         if torch.compiler.is_compiling():  # traced graphs re-pack (the cache is Python state)
-            return pack_stencil(img[0, 0])
-        key = (img.data_ptr(), img.device, img.dtype, img._version, tuple(img.shape))
-        if self._stencil_key != key:
-            self._stencil = pack_stencil(img[0, 0])
-            self._stencil_key = key
-        return self._stencil
+            return pack_volume(img[0, 0], layout)
+        key = (img.data_ptr(), img.device, img.dtype, img._version, tuple(img.shape), layout)
+        if self._packed_key != key:
+            self._packed = None  # free the old copy first
+            self._packed = pack_volume(img[0, 0], layout)
+            self._packed_key = key
+        return self._packed
 
     # -- labels ---------------------------------------------------------------
+    def packed_labels(self, layout: str) -> torch.Tensor:
+        """`label_i8` in the form `layout`'s march reads (label quads for "quad",
+        see ndrr_pack_labels), cached until the labels or the layout change."""
+        from ._ops import pack_labels
+
+        if layout != "quad":
+            return self.label_i8
+        lab = self.label_i8
+        if torch.compiler.is_compiling():
+            return pack_labels(lab, layout)
+        key = (lab.data_ptr(), self._label_i8_key, layout)
+        if self._packed_lab_key != key:
+            self._packed_lab = None
+            self._packed_lab = pack_labels(lab, layout)
+            self._packed_lab_key = key
+        return self._packed_lab
+
     @property
     def label_i8(self) -> torch.Tensor:
         """(D, H, W) int8 class map used by the kernels: trunc(label) when it

@@ -29,7 +29,7 @@ FWD_TOL = 1e-5
 GRAD_TOL = 1e-4
 
 
-@pytest.fixture(params=["stencil", "linear"], autouse=True)
+@pytest.fixture(params=["quad", "stencil", "linear"], autouse=True)
 def volume_layout(request, monkeypatch):
     """Every test runs on both volume layouts the kernels read."""
     import nanodrr_amd.renderer as renderer
@@ -81,12 +81,14 @@ def test_forward_matches_oracle(orthographic, n_classes):
                                           ("persp_c3", False, 3), ("ortho_c3", True, 3)])
 def test_against_golden_vectors(name, ortho, C, volume_layout):
     """Forward and pose/endpoint gradients vs the committed float64 fixtures."""
-    from nanodrr_amd._ops import pack_stencil
+    from nanodrr_amd._ops import pack_labels, pack_volume
 
     g = np.load(GOLDEN)
     vol = torch.from_numpy(g["volume"]).to(DEV)
-    packed = pack_stencil(vol) if volume_layout == "stencil" else None
+    packed = None if volume_layout == "linear" else pack_volume(vol, volume_layout)
     lab = torch.from_numpy(g["label3"]).to(DEV) if C > 1 else torch.empty(0, dtype=torch.int8, device=DEV)
+    if C > 1:
+        lab = pack_labels(lab, volume_layout)
     D, H, W = vol.shape
     w2v = np.eye(4)
     w2v[:3, 3] = [(W - 1) / 2, (H - 1) / 2, (D - 1) / 2]
@@ -96,7 +98,7 @@ def test_against_golden_vectors(name, ortho, C, volume_layout):
     M = orm.camera_to_voxel(w2v, g["rt_inv"])
     t = {k: torch.tensor(v, dtype=torch.float32, device=DEV).requires_grad_(True)
          for k, v in (("M", M), ("src", src), ("tgt", tgt), ("step", step))}
-    out = fused_raymarch(vol, lab, t["M"], t["src"], t["tgt"], t["step"], 200, C, 16, packed)
+    out = fused_raymarch(vol, lab, t["M"], t["src"], t["tgt"], t["step"], 200, C, 16, packed, volume_layout)
     _assert_parity(g[f"{name}_fwd"], out.detach().cpu().numpy(), C)
     (out * torch.tensor(g[f"{name}_go"], dtype=torch.float32, device=DEV)).sum().backward()
     tol = GRAD_TOL if C == 1 else 2e-3  # label flips move single samples between channels
@@ -305,17 +307,41 @@ def test_hip_graph_replays_match_eager():
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16, torch.float16])
-def test_stencil_records_hold_the_eight_corners(dtype):
-    """ndrr_pack_stencil: record (x0, y0, z0) holds the zero-padded corners
-    vol[z0+a, y0+b, x0+c] in (a, b, c) order, bit for bit."""
-    from nanodrr_amd._ops import pack_stencil
+def test_packed_layouts_hold_the_corners(dtype, volume_layout):
+    """ndrr_pack_volume, bit for bit: stencil record (x0, y0, z0) holds the
+    zero-padded corners vol[z0+a, y0+b, x0+c] in (a, b, c) order; quad record
+    (x, y0, z0) holds vol[z0+a, y0+b, x] in (a, b) order."""
+    from nanodrr_amd._ops import pack_volume
 
     torch.manual_seed(0)
     D, H, W = 5, 6, 7
     vol = torch.randn(D, H, W, device=DEV).to(dtype)
-    rec = pack_stencil(vol)
+    rec = pack_volume(vol, volume_layout)
     pad = torch.nn.functional.pad(vol.float(), (1, 1, 1, 1, 1, 1)).to(dtype)  # (D+2, H+2, W+2)
-    corners = [pad[a:a + D + 1, b:b + H + 1, c:c + W + 1] for a in (0, 1) for b in (0, 1) for c in (0, 1)]
-    expect = torch.stack(corners, dim=-1)  # (D+1, H+1, W+1, 8)
-    got = rec.view(dtype).reshape(D + 1, H + 1, W + 1, 8)
+    if volume_layout == "stencil":
+        corners = [pad[a:a + D + 1, b:b + H + 1, c:c + W + 1] for a in (0, 1) for b in (0, 1) for c in (0, 1)]
+        expect = torch.stack(corners, dim=-1)  # (D+1, H+1, W+1, 8)
+    elif volume_layout == "quad":
+        corners = [pad[a:a + D + 1, b:b + H + 1, :] for a in (0, 1) for b in (0, 1)]
+        expect = torch.stack(corners, dim=-1)  # (D+1, H+1, W+2, 4)
+    else:
+        expect = vol
+    got = rec.view(dtype).reshape(expect.shape)
     assert torch.equal(got.view(torch.uint8), expect.contiguous().view(torch.uint8))
+
+
+def test_label_quads_hold_the_classes(volume_layout):
+    """ndrr_pack_labels: quad record (x, y0, z0) holds lab[z0+a, y0+b, x] in byte 2a + b
+    (zero outside); other layouts read the (D, H, W) map itself."""
+    from nanodrr_amd._ops import pack_labels
+
+    torch.manual_seed(1)
+    D, H, W = 4, 5, 6
+    lab = torch.randint(-1, 5, (D, H, W), dtype=torch.int8, device=DEV)
+    got = pack_labels(lab, volume_layout)
+    if volume_layout != "quad":
+        assert torch.equal(got, lab)
+        return
+    pad = torch.nn.functional.pad(lab.to(torch.int32), (1, 1, 1, 1, 1, 1)).to(torch.int8)
+    expect = torch.stack([pad[a:a + D + 1, b:b + H + 1, :] for a in (0, 1) for b in (0, 1)], dim=-1)
+    assert torch.equal(got.view(D + 1, H + 1, W + 2, 4), expect)
