@@ -21,7 +21,7 @@ namespace {
 
 constexpr int kThreads = 256;  // rays per workgroup (16 x 16 tile)
 constexpr int kTile = 16;      // tile edge in detector pixels
-constexpr float kMargin = 1.0f;  // voxels of slack around the footprint box
+constexpr float kMargin = 0.5f;  // voxels of slack around the footprint box
 constexpr int kKahanMinSamples = 512;
 
 // ---------------------------------------------------------------------------
@@ -58,6 +58,7 @@ struct Params {
   float* gstep;
   float* gvol;
   int64_t src_bs, src_rs, tgt_bs, step_bs;
+  uint32_t vol_bytes, lab_bytes;  // sizes of the packed volume / label buffers
   int B, N, D, H, W, S, C;
   int IW, IH, tiles_w, nblocks;
   float inv_s;   // 1 / (S - 1)
@@ -173,10 +174,11 @@ __device__ inline Ray setup_ray(const Params& p, const float* __restrict__ Mb, i
   r.s_lo = 0;
   r.n_s = 0;
   if (valid && t_lo <= t_hi) {
-    // Two samples of slack below, three above: every sample whose footprint
-    // reaches the volume is kept; the extra ones add exact zeros.
-    const float f_lo = fmaxf(floorf(t_lo * p.s_m1) - 2.0f, 0.0f);
-    const float f_hi = fminf(ceilf(t_hi * p.s_m1) + 3.0f, static_cast<float>(p.S));
+    // The slab is widened by kMargin voxels, far more than the rounding of
+    // t_lo, t_hi: every sample whose footprint reaches the volume is kept; the
+    // few extra ones add exact zeros.
+    const float f_lo = fmaxf(floorf(t_lo * p.s_m1), 0.0f);
+    const float f_hi = fminf(ceilf(t_hi * p.s_m1) + 1.0f, static_cast<float>(p.S));
     if (f_lo < f_hi) {
       r.s_lo = static_cast<int>(f_lo);
       r.n_s = static_cast<int>(f_hi) - r.s_lo;
@@ -218,6 +220,31 @@ struct Corners {
   float v000, v001, v010, v011, v100, v101, v110, v111;
 };
 
+// What the fetches read. The linear layout reads through plain pointers; the
+// packed layouts read through buffer resources with 32-bit byte offsets, and
+// a cell outside the packed grid gets an offset past the end of the buffer,
+// for which the hardware returns zeros (no clamping, no 64-bit address math).
+constexpr uint32_t kBufferFlags = 0x00020000u;  // raw buffer, 32-bit data (gfx9 family)
+
+struct Src {
+  const void* vol;
+  const int8_t* lab;
+  __amdgpu_buffer_rsrc_t vrs, lrs;
+  int D, H, W;
+};
+
+__device__ inline Src make_src(const Params& p) {
+  Src s;
+  s.vol = p.vol;
+  s.lab = p.lab;
+  s.vrs = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p.vol), 0, static_cast<int>(p.vol_bytes),
+                                            static_cast<int>(kBufferFlags));
+  s.lrs = __builtin_amdgcn_make_buffer_rsrc(const_cast<int8_t*>(p.lab), 0, static_cast<int>(p.lab_bytes),
+                                            static_cast<int>(kBufferFlags));
+  s.D = p.D; s.H = p.H; s.W = p.W;
+  return s;
+}
+
 // Raw data of one sample's cell as loaded; decoded into Corners at use, so
 // that several samples' loads can be in flight with few registers.
 template <int DT, int L> struct Fetch;
@@ -226,8 +253,9 @@ template <int DT, int L> struct Fetch;
 template <int DT> struct Fetch<DT, NDRR_LAYOUT_LINEAR> {
   static constexpr int kInFlight = 2;
   float v[8];
-  __device__ inline void load(const void* __restrict__ base, int D, int H, int W, const Cell& c) {
-    const auto* __restrict__ vol = static_cast<const typename Vox<DT>::T*>(base);
+  __device__ inline void load(const Src& s, const Cell& c) {
+    const int D = s.D, H = s.H, W = s.W;
+    const auto* __restrict__ vol = static_cast<const typename Vox<DT>::T*>(s.vol);
     const bool bx0 = static_cast<unsigned>(c.ix) < static_cast<unsigned>(W);
     const bool bx1 = static_cast<unsigned>(c.ix + 1) < static_cast<unsigned>(W);
     const bool by0 = static_cast<unsigned>(c.iy) < static_cast<unsigned>(H);
@@ -272,23 +300,38 @@ __device__ inline float f16_hi(uint32_t w) {
   return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w >> 16)));
 }
 
-__device__ inline int64_t record_index(int D, int H, int W, const Cell& c, bool& in) {
-  // cells -1 .. dim-1 exist; any other cell has all eight corners outside
-  in = static_cast<unsigned>(c.ix + 1) <= static_cast<unsigned>(W) &&
-       static_cast<unsigned>(c.iy + 1) <= static_cast<unsigned>(H) &&
-       static_cast<unsigned>(c.iz + 1) <= static_cast<unsigned>(D);
-  const int cx = min(max(c.ix + 1, 0), W), cy = min(max(c.iy + 1, 0), H), cz = min(max(c.iz + 1, 0), D);
-  return (static_cast<int64_t>(cz) * (H + 1) + cy) * (W + 1) + cx;
+// Whether cell (ix, iy, iz) exists in a packed grid (cells -1 .. dim-1);
+// every other cell has all eight corners outside the volume.
+__device__ inline bool cell_exists(const Src& s, const Cell& c) {
+  return static_cast<unsigned>(c.ix + 1) <= static_cast<unsigned>(s.W) &&
+         static_cast<unsigned>(c.iy + 1) <= static_cast<unsigned>(s.H) &&
+         static_cast<unsigned>(c.iz + 1) <= static_cast<unsigned>(s.D);
+}
+
+// Index of a cell's record in a grid of (D + 1) x (H + 1) x row records.
+__device__ inline uint32_t record_of(const Src& s, const Cell& c, int row) {
+  return (static_cast<uint32_t>(c.iz + 1) * static_cast<uint32_t>(s.H + 1) + static_cast<uint32_t>(c.iy + 1)) *
+             static_cast<uint32_t>(row) + static_cast<uint32_t>(c.ix + 1);
+}
+
+// Byte offset of a cell's record, or one past 2 GiB (so past the end of the
+// buffer) when the cell does not exist. Bit arithmetic rather than a select,
+// which the compiler would otherwise turn into a divergent branch.
+__device__ inline uint32_t record_offset(const Src& s, const Cell& c, int row, uint32_t bytes) {
+  const uint32_t gone = static_cast<uint32_t>(!cell_exists(s, c)) << 31;
+  return (record_of(s, c, row) * bytes) | gone;
+}
+
+__device__ inline uint4 load16(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+  return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
 }
 
 template <int DT> struct Fetch<DT, NDRR_LAYOUT_STENCIL> {  // bf16 / fp16: 16-byte records
   static constexpr int kInFlight = 4;
   uint4 q;
-  __device__ inline void load(const void* __restrict__ base, int D, int H, int W, const Cell& c) {
-    bool in;
-    const int64_t r = record_index(D, H, W, c, in);
-    const uint4 w = static_cast<const uint4*>(base)[r];
-    q = in ? w : make_uint4(0u, 0u, 0u, 0u);  // all-zero bits are +0 in bf16 and fp16
+  __device__ inline void load(const Src& s, const Cell& c) {
+    const uint32_t off = record_offset(s, c, s.W + 1, 16u);
+    q = load16(s.vrs, off);
   }
   __device__ inline Corners corners() const {
     if constexpr (DT == NDRR_BF16)
@@ -302,17 +345,16 @@ template <int DT> struct Fetch<DT, NDRR_LAYOUT_STENCIL> {  // bf16 / fp16: 16-by
 
 template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_STENCIL> {  // fp32: 32-byte records
   static constexpr int kInFlight = 2;
-  float4 a, b;
-  __device__ inline void load(const void* __restrict__ base, int D, int H, int W, const Cell& c) {
-    bool in;
-    const int64_t r = record_index(D, H, W, c, in);
-    const float4* p = static_cast<const float4*>(base) + 2 * r;
-    const float4 x = p[0], y = p[1];
-    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    a = in ? x : z;
-    b = in ? y : z;
+  uint4 a, b;
+  __device__ inline void load(const Src& s, const Cell& c) {
+    const uint32_t off = record_offset(s, c, s.W + 1, 32u);
+    a = load16(s.vrs, off);
+    b = load16(s.vrs, off + 16u);
   }
-  __device__ inline Corners corners() const { return {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w}; }
+  __device__ inline Corners corners() const {
+    return {__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), __uint_as_float(a.w),
+            __uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w)};
+  }
 };
 
 // Quad layout: one record per (x, y0, z0), x in [-1, W], y0 in [-1, H-1],
@@ -322,23 +364,12 @@ template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_STENCIL> {  // fp32: 32-byte reco
 // (bf16/fp16) or two (fp32) per sample, with each voxel stored 4 times
 // instead of the stencil's 8, so a 256^3 bf16 volume (136 MB) stays resident
 // in the 256 MB Infinity Cache and adjacent rays share records along x.
-__device__ inline int64_t quad_index(int D, int H, int W, const Cell& c, bool& in) {
-  in = static_cast<unsigned>(c.ix + 1) <= static_cast<unsigned>(W) &&
-       static_cast<unsigned>(c.iy + 1) <= static_cast<unsigned>(H) &&
-       static_cast<unsigned>(c.iz + 1) <= static_cast<unsigned>(D);
-  const int cx = min(max(c.ix + 1, 0), W), cy = min(max(c.iy + 1, 0), H), cz = min(max(c.iz + 1, 0), D);
-  return (static_cast<int64_t>(cz) * (H + 1) + cy) * (W + 2) + cx;
-}
-
 template <int DT> struct Fetch<DT, NDRR_LAYOUT_QUAD> {  // bf16 / fp16: two 8-byte records
   static constexpr int kInFlight = 4;
   uint4 q;  // x: (z0,y0) (z0,y0+1) at x0; y: (z0+1,y0) (z0+1,y0+1) at x0; z, w: the same at x0 + 1
-  __device__ inline void load(const void* __restrict__ base, int D, int H, int W, const Cell& c) {
-    bool in;
-    const int64_t r = quad_index(D, H, W, c, in);
-    uint4 w;
-    __builtin_memcpy(&w, static_cast<const uint2*>(base) + r, sizeof(uint4));  // 8-byte aligned
-    q = in ? w : make_uint4(0u, 0u, 0u, 0u);
+  __device__ inline void load(const Src& s, const Cell& c) {
+    const uint32_t off = record_offset(s, c, s.W + 2, 8u);
+    q = load16(s.vrs, off);
   }
   __device__ inline Corners corners() const {
     if constexpr (DT == NDRR_BF16)
@@ -352,17 +383,16 @@ template <int DT> struct Fetch<DT, NDRR_LAYOUT_QUAD> {  // bf16 / fp16: two 8-by
 
 template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_QUAD> {  // fp32: two 16-byte records
   static constexpr int kInFlight = 2;
-  float4 a, b;  // quads at x0 and x0 + 1: (z0,y0) (z0,y0+1) (z0+1,y0) (z0+1,y0+1)
-  __device__ inline void load(const void* __restrict__ base, int D, int H, int W, const Cell& c) {
-    bool in;
-    const int64_t r = quad_index(D, H, W, c, in);
-    const float4* p = static_cast<const float4*>(base) + r;
-    const float4 x = p[0], y = p[1];
-    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    a = in ? x : z;
-    b = in ? y : z;
+  uint4 a, b;  // quads at x0 and x0 + 1: (z0,y0) (z0,y0+1) (z0+1,y0) (z0+1,y0+1)
+  __device__ inline void load(const Src& s, const Cell& c) {
+    const uint32_t off = record_offset(s, c, s.W + 2, 16u);
+    a = load16(s.vrs, off);
+    b = load16(s.vrs, off + 16u);
   }
-  __device__ inline Corners corners() const { return {a.x, b.x, a.y, b.y, a.z, b.z, a.w, b.w}; }
+  __device__ inline Corners corners() const {
+    return {__uint_as_float(a.x), __uint_as_float(b.x), __uint_as_float(a.y), __uint_as_float(b.y),
+            __uint_as_float(a.z), __uint_as_float(b.z), __uint_as_float(a.w), __uint_as_float(b.w)};
+  }
 };
 
 __device__ inline float lerp_value(const Corners& v, const Cell& c) {
@@ -391,20 +421,18 @@ __device__ inline int nearest_label(const int8_t* __restrict__ lab, int D, int H
 
 // Nearest label from label quads (the quad layout's labels: one uint32 per
 // quad record holding the int8 classes lab[z0+a][y0+b][x] in byte 2a + b, zero
-// outside the volume). The nearest voxel is a corner of the sample's cell:
-// offset 1 along an axis when the fraction exceeds 1/2, or equals it with an
-// odd cell index (round half to even, as rintf and grid_sample's nearest).
-__device__ inline int quad_label(const int8_t* __restrict__ lq, int D, int H, int W, const Cell& c) {
-  bool in;
-  const int64_t r = quad_index(D, H, W, c, in);
-  uint2 w;
-  __builtin_memcpy(&w, reinterpret_cast<const uint32_t*>(lq) + r, sizeof(uint2));  // 4-byte aligned
-  const bool ox = c.fx > 0.5f || (c.fx == 0.5f && (c.ix & 1));
-  const bool oy = c.fy > 0.5f || (c.fy == 0.5f && (c.iy & 1));
-  const bool oz = c.fz > 0.5f || (c.fz == 0.5f && (c.iz & 1));
+// outside the volume). The nearest voxel, rint(p) (round half to even, as
+// grid_sample's nearest mode), is the corner of the sample's cell at offset 1
+// along each axis where rint(p) > floor(p). Branch-free.
+__device__ inline int quad_label(const Src& s, const Cell& c, float x, float y, float z) {
+  const uint32_t off = record_offset(s, c, s.W + 2, 4u);
+  const uint2 w = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(s.lrs, off, 0, 0));
+  const bool ox = rintf(x) > floorf(x);
+  const bool oy = rintf(y) > floorf(y);
+  const bool oz = rintf(z) > floorf(z);
   const uint32_t word = ox ? w.y : w.x;
-  const int l = static_cast<int8_t>(static_cast<uint8_t>(word >> ((oz ? 16 : 0) + (oy ? 8 : 0))));
-  return in ? l : 0;
+  const uint32_t shift = (oz ? 16u : 0u) | (oy ? 8u : 0u);
+  return static_cast<int8_t>(static_cast<uint8_t>(word >> shift));
 }
 
 // Neumaier-compensated accumulation: sum + comp carries ~2x fp32 precision.
@@ -424,18 +452,18 @@ struct Batch {
   Fetch<DT, L> f[U];
   Cell cell[U];
   int cls[U];
-  __device__ inline void issue(const Params& p, const Ray& r, int k0) {
+  __device__ inline void issue(const Src& s, const Ray& r, int k0) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const float k = static_cast<float>(min(k0 + u, r.n_s - 1));
       const float x = sample_coord(r.ax, r.dx, k), y = sample_coord(r.ay, r.dy, k), z = sample_coord(r.az, r.dz, k);
       cell[u] = cell_of(x, y, z);
-      f[u].load(p.vol, p.D, p.H, p.W, cell[u]);
+      f[u].load(s, cell[u]);
       if constexpr (LABELS) {
         if constexpr (L == NDRR_LAYOUT_QUAD)
-          cls[u] = quad_label(p.lab, p.D, p.H, p.W, cell[u]);
+          cls[u] = quad_label(s, cell[u], x, y, z);
         else
-          cls[u] = nearest_label(p.lab, p.D, p.H, p.W, x, y, z);
+          cls[u] = nearest_label(s.lab, s.D, s.H, s.W, x, y, z);
       }
     }
   }
@@ -452,6 +480,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd(Params p) {
   const int n = ray_index(p, blockIdx.x, threadIdx.x, valid);
   const float* __restrict__ Mb = p.M + 12 * b;
   const Ray r = setup_ray(p, Mb, b, n, valid);
+  const Src src = make_src(p);
 
   float acc[CV];
   float comp[CV];
@@ -461,7 +490,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd(Params p) {
   using B = Batch<DT, L, (CV > 1)>;
   for (int k0 = 0; k0 < r.n_s; k0 += B::U) {
     B batch;
-    batch.issue(p, r, k0);
+    batch.issue(src, r, k0);
 #pragma unroll
     for (int u = 0; u < B::U; ++u) {
       const Cell& c = batch.cell[u];
@@ -508,6 +537,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd(Params p) {
   const int n = ray_index(p, blockIdx.x, threadIdx.x, valid);
   const float* __restrict__ Mb = p.M + 12 * b;
   const Ray r = setup_ray(p, Mb, b, n, valid);
+  const Src src = make_src(p);
 
   // Upstream gradient of this ray, one value per class.
   float goc[CV];
@@ -523,7 +553,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd(Params p) {
   using B = Batch<DT, L, (CV > 1)>;
   for (int k0 = 0; k0 < r.n_s; k0 += B::U) {
     B batch;
-    batch.issue(p, r, k0);
+    batch.issue(src, r, k0);
 #pragma unroll
     for (int u = 0; u < B::U; ++u) {
       const bool live = k0 + u < r.n_s;
@@ -752,6 +782,20 @@ int64_t quad_records(int D, int H, int W) {
   return static_cast<int64_t>(D + 1) * (H + 1) * (W + 2);
 }
 
+int64_t packed_volume_bytes(int D, int H, int W, int dt, int layout) {
+  const int64_t e = dt == NDRR_F32 ? 4 : 2;
+  switch (layout) {
+    case NDRR_LAYOUT_LINEAR: return static_cast<int64_t>(D) * H * W * e;
+    case NDRR_LAYOUT_STENCIL: return stencil_records(D, H, W) * 8 * e;
+    case NDRR_LAYOUT_QUAD: return quad_records(D, H, W) * 4 * e;
+    default: return 0;
+  }
+}
+
+int64_t packed_label_bytes(int D, int H, int W, int layout) {
+  return layout == NDRR_LAYOUT_QUAD ? quad_records(D, H, W) * 4 : static_cast<int64_t>(D) * H * W;
+}
+
 bool valid_dtype(int dt) { return dt == NDRR_F32 || dt == NDRR_BF16 || dt == NDRR_F16; }
 
 int check_common(const void* vol, int vol_dtype, int vol_layout, const int8_t* lab, const float* M,
@@ -766,6 +810,9 @@ int check_common(const void* vol, int vol_dtype, int vol_layout, const int8_t* l
   if (C < 1 || C > NDRR_MAX_CLASSES) return NDRR_ECLASSES;
   if (C > 1 && !lab) return NDRR_EINVAL;
   if (B > 65535) return NDRR_ERANGE;
+  if (vol_layout != NDRR_LAYOUT_LINEAR && (packed_volume_bytes(D, H, W, vol_dtype, vol_layout) >= (int64_t{1} << 31) ||
+                                           packed_label_bytes(D, H, W, vol_layout) >= (int64_t{1} << 31)))
+    return NDRR_ERANGE;
   const int64_t vox = static_cast<int64_t>(D) * H * W;
   if (vox >= (int64_t{1} << 31) || stencil_records(D, H, W) >= (int64_t{1} << 31) ||
       quad_records(D, H, W) >= (int64_t{1} << 31) ||
@@ -869,6 +916,8 @@ int ndrr_raymarch_fwd(const void* vol, int vol_dtype, int vol_layout, const int8
   if (!out) return NDRR_EINVAL;
   Params p = make_params(vol, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width,
                          src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride);
+  p.vol_bytes = static_cast<uint32_t>(packed_volume_bytes(D, H, W, vol_dtype, vol_layout));
+  p.lab_bytes = n_classes > 1 ? static_cast<uint32_t>(packed_label_bytes(D, H, W, vol_layout)) : 0u;
   p.out = out;
   const dim3 grid(p.nblocks, B);
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -899,6 +948,8 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_l
   if (need_gcam && (!gsrc || !gtgt || !gstep)) return NDRR_EINVAL;
   Params p = make_params(vol, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width,
                          src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride);
+  p.vol_bytes = static_cast<uint32_t>(packed_volume_bytes(D, H, W, vol_dtype, vol_layout));
+  p.lab_bytes = n_classes > 1 ? static_cast<uint32_t>(packed_label_bytes(D, H, W, vol_layout)) : 0u;
   p.go = go;
   p.gpart = workspace;
   p.gsrc = gsrc; p.gtgt = gtgt; p.gstep = gstep; p.gvol = gvol;
@@ -918,13 +969,7 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_l
 
 size_t ndrr_packed_bytes(int D, int H, int W, int vol_dtype, int vol_layout) {
   if (D <= 0 || H <= 0 || W <= 0 || !valid_dtype(vol_dtype)) return 0;
-  const size_t e = vol_dtype == NDRR_F32 ? 4 : 2;
-  switch (vol_layout) {
-    case NDRR_LAYOUT_LINEAR: return static_cast<size_t>(D) * H * W * e;
-    case NDRR_LAYOUT_STENCIL: return static_cast<size_t>(stencil_records(D, H, W)) * 8 * e;
-    case NDRR_LAYOUT_QUAD: return static_cast<size_t>(quad_records(D, H, W)) * 4 * e;
-    default: return 0;
-  }
+  return static_cast<size_t>(packed_volume_bytes(D, H, W, vol_dtype, vol_layout));
 }
 
 int ndrr_pack_volume(const void* vol, int vol_dtype, int vol_layout, int D, int H, int W, void* out,
@@ -967,12 +1012,9 @@ int ndrr_pack_volume(const void* vol, int vol_dtype, int vol_layout, int D, int 
 
 size_t ndrr_packed_label_bytes(int D, int H, int W, int vol_layout) {
   if (D <= 0 || H <= 0 || W <= 0) return 0;
-  switch (vol_layout) {
-    case NDRR_LAYOUT_LINEAR:
-    case NDRR_LAYOUT_STENCIL: return static_cast<size_t>(D) * H * W;
-    case NDRR_LAYOUT_QUAD: return static_cast<size_t>(quad_records(D, H, W)) * 4;
-    default: return 0;
-  }
+  if (vol_layout != NDRR_LAYOUT_LINEAR && vol_layout != NDRR_LAYOUT_STENCIL && vol_layout != NDRR_LAYOUT_QUAD)
+    return 0;
+  return static_cast<size_t>(packed_label_bytes(D, H, W, vol_layout));
 }
 
 int ndrr_pack_labels(const int8_t* lab, int vol_layout, int D, int H, int W, void* out, void* stream) {
