@@ -4,11 +4,12 @@ Workload (fwd+bwd on a 256^3 volume): seeded 256^3 phantom at 1 mm, 4 label
 channels (int8 labels), 16 poses (ZXY +-15 deg, +-20 mm about the isocentre
 600 mm from the source), 200 x 200 detector at 1 mm, sdd 1020 mm, 256
 samples per ray. One step = render(...) of the 16 poses plus the backward of
-a weighted sum to the poses; DRR/s = 16 * steps / elapsed. The volume is
-benchmarked stored as bfloat16 (the headline `value`) and as float32; both
-accumulate in float32.
+a weighted sum to the poses; DRR/s = 16 * steps / elapsed. Cases (volume
+dtype : layout the kernels read): bf16:stencil (the headline `value`),
+fp32:linear (the plain fp32 kernel on the (D, H, W) volume) and fp32:quad;
+every case accumulates in float32.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cases bf16:stencil,...] [--no-cpu-baseline]
 
 Prints one JSON line. `roofline` covers the dominant kernel (the backward
 march) timed with HIP events on the stream it runs on; `cpu_baseline` times
@@ -152,8 +153,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--dtypes", default="bf16,fp32")
-    ap.add_argument("--layout", default="quad", choices=["quad", "stencil", "linear"])
+    ap.add_argument("--cases", default="bf16:stencil,fp32:linear,fp32:quad")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -163,26 +163,28 @@ def main():
         sys.exit(2)
     import nanodrr_amd.renderer as renderer
 
-    renderer.VOLUME_LAYOUT = args.layout
     dev = torch.device("cuda:0")
     dtypes = {"bf16": torch.bfloat16, "fp32": torch.float32, "fp16": torch.float16}
     results = {}
     ph = None
-    for name in args.dtypes.split(","):
+    for case in args.cases.split(","):
+        name, layout = case.split(":")
         dtype = dtypes[name]
+        renderer.VOLUME_LAYOUT = layout
         subject, k_inv, rt_inv, sdd, ph = make_workload(dtype, dev)
         wts = torch.randn(1, N_CLASSES, DET, DET, generator=torch.Generator().manual_seed(0)).to(dev)
         ms = time_steps(lambda: step_fn(subject, k_inv, rt_inv, sdd, wts), args.steps, args.warmup)
-        ms_fwd, ms_bwd, geom = kernel_times(subject, k_inv, rt_inv, sdd, wts, max(10, args.steps), args.layout)
+        ms_fwd, ms_bwd, geom = kernel_times(subject, k_inv, rt_inv, sdd, wts, max(10, args.steps), layout)
         n_in, b_fwd, b_bwd = algorithmic_bytes(subject, geom, dtype)
-        results[name] = dict(ms_per_step=ms, drr_per_s=BATCH * 1000.0 / ms, ms_fwd=ms_fwd, ms_bwd=ms_bwd,
+        results[case] = dict(ms_per_step=ms, drr_per_s=BATCH * 1000.0 / ms, ms_fwd=ms_fwd, ms_bwd=ms_bwd,
                              inside_samples=n_in, bytes_fwd=b_fwd, bytes_bwd=b_bwd,
                              gbs_fwd=b_fwd / ms_fwd / 1e6, gbs_bwd=b_bwd / ms_bwd / 1e6)
-        print(f"# {name}: {results[name]}", file=sys.stderr, flush=True)
+        print(f"# {case}: {results[case]}", file=sys.stderr, flush=True)
         del subject
         torch.cuda.empty_cache()
 
-    head = "bf16" if "bf16" in results else next(iter(results))
+    head = next(iter(results))
+    head_dtype, head_layout = head.split(":")
     r = results[head]
     dominant = "bwd" if r["ms_bwd"] >= r["ms_fwd"] else "fwd"
     achieved = r[f"gbs_{dominant}"]
@@ -197,24 +199,28 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": head,
+        "dtype": head_dtype,
         "data": "synthetic (seeded 256^3 phantom, 4 labels, 16 random poses)",
         "config": {
             "workload": "fwd+bwd (pose grads) 256^3 @1mm, C=4, B=16, 200x200, S=256",
-            "volume_storage": head, "volume_layout": args.layout, "accumulate": "fp32", "global_batch": BATCH,
+            "volume_storage": head_dtype, "volume_layout": head_layout, "accumulate": "fp32", "global_batch": BATCH,
             "n_samples": S,
-            "per_dtype": {k: {"drr_per_s": v["drr_per_s"], "ms_per_step": v["ms_per_step"],
+            "per_case": {k: {"drr_per_s": v["drr_per_s"], "ms_per_step": v["ms_per_step"],
                               "ms_fwd_kernel": v["ms_fwd"], "ms_bwd_kernel": v["ms_bwd"],
                               "gbs_fwd": v["gbs_fwd"], "gbs_bwd": v["gbs_bwd"]} for k, v in results.items()},
         },
         "roofline": {
-            "bound": "hbm", "kernel": f"k_{dominant} ({head} volume, {args.layout} layout)", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "bound": "hbm", "kernel": f"k_{dominant} ({head_dtype} volume, {head_layout} layout)", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "algorithmic_bytes_per_launch": r[f"bytes_{dominant}"], "inside_samples": r["inside_samples"],
         },
     }
-    if "fp32" in results and "bf16" in results:
-        line["config"]["bf16_over_fp32"] = results["bf16"]["drr_per_s"] / results["fp32"]["drr_per_s"]
+    if "fp32:linear" in results and head != "fp32:linear":
+        plain = results["fp32:linear"]
+        line["config"]["speedup_over_fp32_plain"] = {
+            "step": r["drr_per_s"] / plain["drr_per_s"],
+            "kernels": (plain["ms_fwd"] + plain["ms_bwd"]) / (r["ms_fwd"] + r["ms_bwd"]),
+        }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(ph)
     print(json.dumps(line), flush=True)

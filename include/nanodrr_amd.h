@@ -29,8 +29,9 @@
  *        NDRR_LAYOUT_STENCIL, its label quads (ndrr_pack_labels) for
  *        NDRR_LAYOUT_QUAD; ignored when n_classes == 1
  * Sample s of ray (b, n) sits at M[b] @ [lerp(src, tgt, s / (S - 1)), 1].
- * Rays are mapped to 16 x 16 detector tiles of a `width`-wide image when
- * width > 0 (N must be a multiple of width), else to runs of 256 rays.
+ * Rays are mapped to detector tiles of a `width`-wide image when width > 0
+ * (N must be a multiple of width), else to runs of consecutive rays; see
+ * ndrr_launch_tile. Packed volume and label copies must be smaller than 2 GiB.
  */
 #ifndef NANODRR_AMD_H
 #define NANODRR_AMD_H
@@ -90,6 +91,13 @@ int ndrr_raymarch_fwd(const void* vol, int vol_dtype, int vol_layout, const int8
                       int64_t src_batch_stride, int64_t src_ray_stride,
                       int64_t tgt_batch_stride, int64_t step_batch_stride,
                       void* stream);
+
+/*
+ * The launch tile: each workgroup marches a tile_width x tile_height patch of
+ * the detector (width > 0) or tile_rays consecutive rays (width == 0).
+ * Pure host query; any pointer may be NULL.
+ */
+void ndrr_launch_tile(int* tile_width, int* tile_height, int* tile_rays);
 
 /*
  * Number of fp32 elements of the pose-gradient workspace that

@@ -30,6 +30,7 @@ SIGNATURES: dict[str, tuple] = {
          _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
     ),
     "ndrr_bwd_workspace_floats": (_c_size, [_c_int, _c_int, _c_int]),
+    "ndrr_launch_tile": (None, [ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
     "ndrr_raymarch_bwd": (
         _c_int,
         [_c_p, _c_p, _c_int, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p,
@@ -68,3 +69,10 @@ def check(rc: int, what: str) -> None:
     if rc != 0:
         msg = lib().ndrr_strerror(rc).decode()
         raise NativeError(f"{what} failed with code {rc}: {msg}")
+
+
+def launch_tile() -> tuple[int, int, int]:
+    """(tile width, tile height, rays per tile) of the kernels' launch."""
+    w, h, r = _c_int(), _c_int(), _c_int()
+    lib().ndrr_launch_tile(ctypes.byref(w), ctypes.byref(h), ctypes.byref(r))
+    return w.value, h.value, r.value

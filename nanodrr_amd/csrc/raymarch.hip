@@ -1,12 +1,14 @@
 // Fused differentiable DRR ray march for AMD Instinct MI355X (gfx950).
 //
-// One thread marches one ray. A 256-thread workgroup covers a 16 x 16
-// detector tile; each of its four wave64s owns an 8 x 8 quadrant, so the 64
-// rays of a wave stay spatially coherent and their trilinear gathers land in
-// few cache lines. Per ray the kernel clips the sample range to the samples
-// whose 2x2x2 footprint can touch the volume (skipped samples contribute
-// exactly zero, so the clip changes no bit of the result), then marches with
-// eight zero-padded corner gathers per sample and a nearest-label route.
+// One thread marches one ray. A workgroup covers a 32 x 16 detector tile as
+// eight wave64s of 8 x 8 rays, so the rays sharing a CU's L1 are spatially
+// coherent and their gathers land in few cache lines; workgroups are remapped
+// so each XCD (own L2) runs neighbouring tiles. Per ray the kernel clips the
+// sample range to the samples whose 2x2x2 footprint can touch the volume
+// (skipped samples contribute exactly zero, so the clip changes no bit of the
+// result), then marches: per sample the trilinear stencil (eight zero-padded
+// corners, from the linear volume or one packed record) and a nearest-label
+// route, with several samples' loads in flight per lane.
 //
 // The arithmetic of the march and of its adjoint follows nanodrr's fused
 // kernel (src/nanodrr/drr/_fused.py:133-300); the launch layout, clip,
@@ -19,8 +21,32 @@
 
 namespace {
 
-constexpr int kThreads = 256;  // rays per workgroup (16 x 16 tile)
-constexpr int kTile = 16;      // tile edge in detector pixels
+constexpr int kWave = 64;        // rays per wave: an 8 x 8 detector patch (or 64 consecutive rays)
+// Tuning knobs (compile-time; the defaults are the measured best on MI355X).
+#ifndef NDRR_TILE_WAVES_X
+#define NDRR_TILE_WAVES_X 4
+#endif
+#ifndef NDRR_TILE_WAVES_Y
+#define NDRR_TILE_WAVES_Y 2
+#endif
+#ifndef NDRR_CHUNKS
+#define NDRR_CHUNKS 1
+#endif
+#ifndef NDRR_IN_FLIGHT
+#define NDRR_IN_FLIGHT 4
+#endif
+#ifndef NDRR_WAVES_PER_EU
+#define NDRR_WAVES_PER_EU 1
+#endif
+#ifndef NDRR_XCD_REMAP
+#define NDRR_XCD_REMAP 1
+#endif
+constexpr int kTwx = NDRR_TILE_WAVES_X, kTwy = NDRR_TILE_WAVES_Y;  // 8 x 8 wave patches per tile
+constexpr int kTileW = 8 * kTwx, kTileH = 8 * kTwy;                // tile size in detector pixels
+constexpr int kTileRays = kWave * kTwx * kTwy;
+constexpr int kChunks = NDRR_CHUNKS;  // wave groups per workgroup; group c marches chunk c of every ray's samples
+constexpr int kBlock = kTileRays * kChunks;
+constexpr int kThreads = 256;    // threads per block of the element-wise helper kernels
 constexpr float kMargin = 0.5f;  // voxels of slack around the footprint box
 constexpr int kKahanMinSamples = 512;
 
@@ -63,23 +89,46 @@ struct Params {
   int IW, IH, tiles_w, nblocks;
   float inv_s;   // 1 / (S - 1)
   float s_m1;    // S - 1
+  float inv_lo;  // 1 / (S - 1) - inv_s, so inv_s + inv_lo carries the reciprocal to ~2^-48
 };
 
-// Ray index of this thread and whether it is a real detector pixel.
-__device__ inline int ray_index(const Params& p, int blk, int tid, bool& valid) {
+// Ray of thread `tr` (0 .. kTileRays-1) in tile `blk` and whether it is a real
+// detector pixel. Each wave holds an 8 x 8 patch of the tile; with kChunks > 1
+// every chunk group holds the same rays (one chunk of samples each).
+__device__ inline int ray_index(const Params& p, int blk, int tr, bool& valid) {
   if (p.IW > 0) {
     const int ty = blk / p.tiles_w;
     const int tx = blk - ty * p.tiles_w;
-    const int wave = tid >> 6;
-    const int lane = tid & 63;
-    const int px = tx * kTile + (wave & 1) * 8 + (lane & 7);
-    const int py = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
+    const int wave = tr >> 6, lane = tr & 63;
+    const int px = tx * kTileW + (wave % kTwx) * 8 + (lane & 7);
+    const int py = ty * kTileH + (wave / kTwx) * 8 + (lane >> 3);
     valid = (px < p.IW) && (py < p.IH);
     return py * p.IW + px;
   }
-  const int n = blk * kThreads + tid;
+  const int n = blk * kTileRays + tr;
   valid = n < p.N;
   return n;
+}
+
+// (pose, tile) of this workgroup. Workgroups are dispatched round-robin over
+// the 8 XCDs (each with its own L2); with the remap, XCD k runs a contiguous
+// range of (pose, tile) pairs, so the tiles sharing an L2 are neighbours.
+constexpr int kXcds = 8;
+__device__ inline void block_coords(int& b, int& tile) {
+  if constexpr (NDRR_XCD_REMAP) {
+    const int ntiles = gridDim.x;
+    const int total = ntiles * gridDim.y;
+    const int id = blockIdx.y * ntiles + blockIdx.x;
+    const int per = total / kXcds, rem = total % kXcds;
+    const int xcd = id % kXcds, slot = id / kXcds;
+    // XCD k owns per (+1 for k < rem) consecutive ids starting at k * per + min(k, rem)
+    const int lin = xcd * per + min(xcd, rem) + slot;
+    b = lin / ntiles;
+    tile = lin - b * ntiles;
+  } else {
+    b = blockIdx.y;
+    tile = blockIdx.x;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -124,12 +173,6 @@ __device__ inline DF df_mul(DF a, float b) {
   return fast_two_sum(p.hi, fmaf(a.lo, b, p.lo));
 }
 
-__device__ inline DF df_div(DF a, float b) {  // b an exact small integer
-  const float q = a.hi / b;
-  const float r = fmaf(-q, b, a.hi);  // exact remainder
-  return fast_two_sum(q, (r + a.lo) / b);
-}
-
 struct Ray {
   float sx, sy, sz;     // source, camera space
   float ex, ey, ez;     // tgt - src, camera space
@@ -139,29 +182,50 @@ struct Ray {
   int s_lo, n_s;        // samples [s_lo, s_lo + n_s) can touch the volume
 };
 
+// Ray-parameter interval where lo <= p0 + t d <= hi (per axis). An approximate
+// reciprocal is ample: the slab is widened by kMargin voxels.
 __device__ inline void axis_range(float p0, float d, float lo, float hi, float& t0, float& t1) {
-  const float sd = fabsf(d) < 1e-12f ? 1e-12f : d;
-  const float a = (lo - p0) / sd;
-  const float b = (hi - p0) / sd;
+  const float inv = __builtin_amdgcn_rcpf(fabsf(d) < 1e-12f ? 1e-12f : d);
+  const float a = (lo - p0) * inv;
+  const float b = (hi - p0) * inv;
   t0 = fminf(a, b);
   t1 = fmaxf(a, b);
 }
 
-__device__ inline Ray setup_ray(const Params& p, const float* __restrict__ Mb, int b, int n, bool valid) {
+__device__ inline DF df_mul_df(DF a, DF b) {
+  DF p = two_prod(a.hi, b.hi);
+  return fast_two_sum(p.hi, fmaf(a.hi, b.lo, fmaf(a.lo, b.hi, p.lo)));
+}
+
+// Per-ray set-up for the samples of chunk `chunk` of `kChunks` equal parts of
+// the clipped sample range: the chunk's first sample (anchor) and its count.
+__device__ inline Ray setup_ray(const Params& p, const float* __restrict__ Mb, int b, int n, bool valid, int chunk) {
   Ray r;
   const int nn = valid ? n : 0;
   const float* sp = p.src + b * p.src_bs + nn * p.src_rs;
   const float* tp = p.tgt + b * p.tgt_bs + 3 * static_cast<int64_t>(nn);
   r.sx = sp[0]; r.sy = sp[1]; r.sz = sp[2];
   r.step = p.step[b * p.step_bs + nn];
-  const DF ex = two_sum(tp[0], -r.sx), ey = two_sum(tp[1], -r.sy), ez = two_sum(tp[2], -r.sz);
-  r.ex = ex.hi; r.ey = ey.hi; r.ez = ez.hi;
   DF p0[3], d[3];
+  if (r.sx == 0.0f && r.sy == 0.0f && r.sz == 0.0f) {
+    // A source at the camera origin (every perspective render): p0 is M's
+    // translation column and tgt - src is tgt, both exact.
+    r.ex = tp[0]; r.ey = tp[1]; r.ez = tp[2];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    const float m0 = Mb[4 * i], m1 = Mb[4 * i + 1], m2 = Mb[4 * i + 2], m3 = Mb[4 * i + 3];
-    p0[i] = df_add(df_add(df_add(two_prod(m0, r.sx), two_prod(m1, r.sy)), two_prod(m2, r.sz)), DF{m3, 0.0f});
-    d[i] = df_add(df_add(df_mul(ex, m0), df_mul(ey, m1)), df_mul(ez, m2));
+    for (int i = 0; i < 3; ++i) {
+      const float m0 = Mb[4 * i], m1 = Mb[4 * i + 1], m2 = Mb[4 * i + 2];
+      p0[i] = DF{Mb[4 * i + 3], 0.0f};
+      d[i] = df_add(df_add(two_prod(m0, r.ex), two_prod(m1, r.ey)), two_prod(m2, r.ez));
+    }
+  } else {
+    const DF ex = two_sum(tp[0], -r.sx), ey = two_sum(tp[1], -r.sy), ez = two_sum(tp[2], -r.sz);
+    r.ex = ex.hi; r.ey = ey.hi; r.ez = ez.hi;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const float m0 = Mb[4 * i], m1 = Mb[4 * i + 1], m2 = Mb[4 * i + 2], m3 = Mb[4 * i + 3];
+      p0[i] = df_add(df_add(df_add(two_prod(m0, r.sx), two_prod(m1, r.sy)), two_prod(m2, r.sz)), DF{m3, 0.0f});
+      d[i] = df_add(df_add(df_mul(ex, m0), df_mul(ey, m1)), df_mul(ez, m2));
+    }
   }
 
   // Clip to the samples whose footprint can reach the volume (fp32 is ample here).
@@ -171,8 +235,7 @@ __device__ inline Ray setup_ray(const Params& p, const float* __restrict__ Mb, i
   axis_range(p0[2].hi, d[2].hi, -1.0f - kMargin, static_cast<float>(p.D) + kMargin, az0, az1);
   const float t_lo = fmaxf(fmaxf(ax0, ay0), az0);
   const float t_hi = fminf(fminf(ax1, ay1), az1);
-  r.s_lo = 0;
-  r.n_s = 0;
+  int s_lo = 0, n_all = 0;
   if (valid && t_lo <= t_hi) {
     // The slab is widened by kMargin voxels, far more than the rounding of
     // t_lo, t_hi: every sample whose footprint reaches the volume is kept; the
@@ -180,16 +243,20 @@ __device__ inline Ray setup_ray(const Params& p, const float* __restrict__ Mb, i
     const float f_lo = fmaxf(floorf(t_lo * p.s_m1), 0.0f);
     const float f_hi = fminf(ceilf(t_hi * p.s_m1) + 1.0f, static_cast<float>(p.S));
     if (f_lo < f_hi) {
-      r.s_lo = static_cast<int>(f_lo);
-      r.n_s = static_cast<int>(f_hi) - r.s_lo;
+      s_lo = static_cast<int>(f_lo);
+      n_all = static_cast<int>(f_hi) - s_lo;
     }
   }
+  const int per = (n_all + kChunks - 1) / kChunks;
+  r.s_lo = s_lo + chunk * per;
+  r.n_s = max(0, min(per, n_all - chunk * per));
   const float s0 = static_cast<float>(r.s_lo);
+  const DF inv{p.inv_s, p.inv_lo};
   DF* const an[3] = {&r.ax, &r.ay, &r.az};
   DF* const st[3] = {&r.dx, &r.dy, &r.dz};
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    const DF delta = df_div(d[i], p.s_m1);
+    const DF delta = df_mul_df(d[i], inv);
     *st[i] = delta;
     *an[i] = df_add(p0[i], df_mul(delta, s0));
   }
@@ -327,7 +394,7 @@ __device__ inline uint4 load16(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
 }
 
 template <int DT> struct Fetch<DT, NDRR_LAYOUT_STENCIL> {  // bf16 / fp16: 16-byte records
-  static constexpr int kInFlight = 4;
+  static constexpr int kInFlight = NDRR_IN_FLIGHT;
   uint4 q;
   __device__ inline void load(const Src& s, const Cell& c) {
     const uint32_t off = record_offset(s, c, s.W + 1, 16u);
@@ -365,7 +432,7 @@ template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_STENCIL> {  // fp32: 32-byte reco
 // instead of the stencil's 8, so a 256^3 bf16 volume (136 MB) stays resident
 // in the 256 MB Infinity Cache and adjacent rays share records along x.
 template <int DT> struct Fetch<DT, NDRR_LAYOUT_QUAD> {  // bf16 / fp16: two 8-byte records
-  static constexpr int kInFlight = 4;
+  static constexpr int kInFlight = NDRR_IN_FLIGHT;
   uint4 q;  // x: (z0,y0) (z0,y0+1) at x0; y: (z0+1,y0) (z0+1,y0+1) at x0; z, w: the same at x0 + 1
   __device__ inline void load(const Src& s, const Cell& c) {
     const uint32_t off = record_offset(s, c, s.W + 2, 8u);
@@ -472,14 +539,21 @@ struct Batch {
 // ---------------------------------------------------------------------------
 // Forward
 // ---------------------------------------------------------------------------
+// A workgroup of kChunks waves marches one tile of 64 rays; wave c covers
+// chunk c of every ray's clipped sample range, so a launch holds kChunks x as
+// many short waves and the tail of the grid is short. The chunks' partial
+// sums are added in chunk order through LDS (deterministic).
 // CV == 1: single channel (no label map). CV in {2, 4, 8}: label map, runtime C <= CV.
 template <int DT, int L, int CV, bool KAHAN>
-__global__ __launch_bounds__(kThreads) void k_fwd(Params p) {
-  const int b = blockIdx.y;
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NDRR_WAVES_PER_EU)))
+void k_fwd(Params p) {
+  int b, tile;
+  block_coords(b, tile);
+  const int tr = threadIdx.x % kTileRays, chunk = threadIdx.x / kTileRays;
   bool valid;
-  const int n = ray_index(p, blockIdx.x, threadIdx.x, valid);
+  const int n = ray_index(p, tile, tr, valid);
   const float* __restrict__ Mb = p.M + 12 * b;
-  const Ray r = setup_ray(p, Mb, b, n, valid);
+  const Ray r = setup_ray(p, Mb, b, n, valid, chunk);
   const Src src = make_src(p);
 
   float acc[CV];
@@ -506,13 +580,26 @@ __global__ __launch_bounds__(kThreads) void k_fwd(Params p) {
       }
     }
   }
-  if (!valid) return;
+
+  __shared__ float part[kChunks > 1 ? kChunks - 1 : 1][CV][kTileRays];
+  if constexpr (kChunks > 1) {
+    if (chunk > 0) {
+#pragma unroll
+      for (int c = 0; c < CV; ++c) part[chunk - 1][c][tr] = KAHAN ? (acc[c] + comp[c]) : acc[c];
+    }
+    __syncthreads();
+  }
+  if (chunk != 0 || !valid) return;
   float* o = p.out + (static_cast<int64_t>(b) * p.C) * p.N + n;
 #pragma unroll
   for (int c = 0; c < CV; ++c) {
     if (c < p.C) {
-      const float total = KAHAN ? (acc[c] + comp[c]) : acc[c];
-      o[static_cast<int64_t>(c) * p.N] = total * r.step;
+      float total = acc[c], tcomp = comp[c];
+#pragma unroll
+      for (int k = 1; k < kChunks; ++k) {
+        if constexpr (KAHAN) comp_add(total, tcomp, part[k - 1][c][tr]); else total += part[k - 1][c][tr];
+      }
+      o[static_cast<int64_t>(c) * p.N] = (KAHAN ? (total + tcomp) : total) * r.step;
     }
   }
 }
@@ -531,12 +618,15 @@ __device__ inline float wave_sum(float v) {
 }
 
 template <int DT, int L, int CV, bool GCAM, bool GVOL>
-__global__ __launch_bounds__(kThreads) void k_bwd(Params p) {
-  const int b = blockIdx.y;
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NDRR_WAVES_PER_EU)))
+void k_bwd(Params p) {
+  int b, tile;
+  block_coords(b, tile);
+  const int tr = threadIdx.x % kTileRays, chunk = threadIdx.x / kTileRays;
   bool valid;
-  const int n = ray_index(p, blockIdx.x, threadIdx.x, valid);
+  const int n = ray_index(p, tile, tr, valid);
   const float* __restrict__ Mb = p.M + 12 * b;
-  const Ray r = setup_ray(p, Mb, b, n, valid);
+  const Ray r = setup_ray(p, Mb, b, n, valid, chunk);
   const Src src = make_src(p);
 
   // Upstream gradient of this ray, one value per class.
@@ -601,30 +691,48 @@ __global__ __launch_bounds__(kThreads) void k_bwd(Params p) {
   }
 
   if constexpr (GCAM) {
-    if (valid) {
+    // Per-ray totals over the chunks, in chunk order.
+    __shared__ float part[kChunks > 1 ? kChunks - 1 : 1][7][kTileRays];
+    if constexpr (kChunks > 1) {
+      if (chunk > 0) {
+        part[chunk - 1][0][tr] = Gx; part[chunk - 1][1][tr] = Gy; part[chunk - 1][2][tr] = Gz;
+        part[chunk - 1][3][tr] = Tx; part[chunk - 1][4][tr] = Ty; part[chunk - 1][5][tr] = Tz;
+        part[chunk - 1][6][tr] = astep;
+      }
+      __syncthreads();
+    }
+    if (chunk == 0 && valid) {
+      float sGx = Gx, sGy = Gy, sGz = Gz, sTx = Tx, sTy = Ty, sTz = Tz, sA = astep;
+#pragma unroll
+      for (int k = 1; k < kChunks; ++k) {
+        sGx += part[k - 1][0][tr]; sGy += part[k - 1][1][tr]; sGz += part[k - 1][2][tr];
+        sTx += part[k - 1][3][tr]; sTy += part[k - 1][4][tr]; sTz += part[k - 1][5][tr];
+        sA += part[k - 1][6][tr];
+      }
       const float m00 = Mb[0], m01 = Mb[1], m02 = Mb[2];
       const float m10 = Mb[4], m11 = Mb[5], m12 = Mb[6];
       const float m20 = Mb[8], m21 = Mb[9], m22 = Mb[10];
-      const float qGx = m00 * Gx + m10 * Gy + m20 * Gz;  // M33^T G
-      const float qGy = m01 * Gx + m11 * Gy + m21 * Gz;
-      const float qGz = m02 * Gx + m12 * Gy + m22 * Gz;
-      const float qTx = m00 * Tx + m10 * Ty + m20 * Tz;  // M33^T T
-      const float qTy = m01 * Tx + m11 * Ty + m21 * Tz;
-      const float qTz = m02 * Tx + m12 * Ty + m22 * Tz;
+      const float qGx = m00 * sGx + m10 * sGy + m20 * sGz;  // M33^T G
+      const float qGy = m01 * sGx + m11 * sGy + m21 * sGz;
+      const float qGz = m02 * sGx + m12 * sGy + m22 * sGz;
+      const float qTx = m00 * sTx + m10 * sTy + m20 * sTz;  // M33^T T
+      const float qTy = m01 * sTx + m11 * sTy + m21 * sTz;
+      const float qTz = m02 * sTx + m12 * sTy + m22 * sTz;
       const int64_t ray = static_cast<int64_t>(b) * p.N + n;
       p.gsrc[3 * ray + 0] = qGx - qTx; p.gsrc[3 * ray + 1] = qGy - qTy; p.gsrc[3 * ray + 2] = qGz - qTz;
       p.gtgt[3 * ray + 0] = qTx; p.gtgt[3 * ray + 1] = qTy; p.gtgt[3 * ray + 2] = qTz;
-      p.gstep[ray] = astep;
+      p.gstep[ray] = sA;
     }
   }
 
   // Deterministic per-tile pose-gradient partials: butterfly within each
-  // wave, then the four wave sums in fixed order.
-  __shared__ float red[kThreads / 64][12];
+  // wave, then the wave sums in wave order.
+  constexpr int kWaves = kBlock / kWave;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __shared__ float red[kWaves][12];
   const float a[12] = {r.sx * Gx + r.ex * Tx, r.sy * Gx + r.ey * Tx, r.sz * Gx + r.ez * Tx, Gx,
                        r.sx * Gy + r.ex * Ty, r.sy * Gy + r.ey * Ty, r.sz * Gy + r.ez * Ty, Gy,
                        r.sx * Gz + r.ex * Tz, r.sy * Gz + r.ey * Tz, r.sz * Gz + r.ez * Tz, Gz};
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int j = 0; j < 12; ++j) {
     const float v = wave_sum(a[j]);
@@ -635,8 +743,8 @@ __global__ __launch_bounds__(kThreads) void k_bwd(Params p) {
     const int j = threadIdx.x;
     float v = red[0][j];
 #pragma unroll
-    for (int w = 1; w < kThreads / 64; ++w) v += red[w][j];
-    p.gpart[(static_cast<int64_t>(b) * p.nblocks + blockIdx.x) * 12 + j] = v;
+    for (int w = 1; w < kWaves; ++w) v += red[w][j];
+    p.gpart[(static_cast<int64_t>(b) * p.nblocks + tile) * 12 + j] = v;
   }
 }
 
@@ -766,12 +874,12 @@ __global__ void k_pack_label_quads(const int8_t* __restrict__ lab, uint32_t* __r
 int nblocks_of(int N, int width, int& tiles_w, int& IH) {
   if (width > 0) {
     IH = N / width;
-    tiles_w = (width + kTile - 1) / kTile;
-    return tiles_w * ((IH + kTile - 1) / kTile);
+    tiles_w = (width + kTileW - 1) / kTileW;
+    return tiles_w * ((IH + kTileH - 1) / kTileH);
   }
   tiles_w = 0;
   IH = 0;
-  return (N + kThreads - 1) / kThreads;
+  return (N + kTileRays - 1) / kTileRays;
 }
 
 int64_t stencil_records(int D, int H, int W) {
@@ -832,6 +940,7 @@ Params make_params(const void* vol, const int8_t* lab, const float* M, const flo
   p.nblocks = nblocks_of(N, width, p.tiles_w, p.IH);
   p.inv_s = 1.0f / static_cast<float>(S - 1);
   p.s_m1 = static_cast<float>(S - 1);
+  p.inv_lo = static_cast<float>(1.0 / static_cast<double>(S - 1) - static_cast<double>(p.inv_s));
   return p;
 }
 
@@ -843,9 +952,9 @@ int launch_status() {
 template <int DT, int L, int CV>
 void launch_fwd3(const Params& p, dim3 grid, hipStream_t st) {
   if (p.S >= kKahanMinSamples)
-    hipLaunchKernelGGL((k_fwd<DT, L, CV, true>), grid, dim3(kThreads), 0, st, p);
+    hipLaunchKernelGGL((k_fwd<DT, L, CV, true>), grid, dim3(kBlock), 0, st, p);
   else
-    hipLaunchKernelGGL((k_fwd<DT, L, CV, false>), grid, dim3(kThreads), 0, st, p);
+    hipLaunchKernelGGL((k_fwd<DT, L, CV, false>), grid, dim3(kBlock), 0, st, p);
 }
 
 template <int DT, int L>
@@ -858,10 +967,10 @@ void launch_fwd2(const Params& p, dim3 grid, hipStream_t st) {
 
 template <int DT, int L, int CV>
 void launch_bwd3(const Params& p, bool gcam, bool gvol, dim3 grid, hipStream_t st) {
-  if (gcam && gvol) hipLaunchKernelGGL((k_bwd<DT, L, CV, true, true>), grid, dim3(kThreads), 0, st, p);
-  else if (gcam) hipLaunchKernelGGL((k_bwd<DT, L, CV, true, false>), grid, dim3(kThreads), 0, st, p);
-  else if (gvol) hipLaunchKernelGGL((k_bwd<DT, L, CV, false, true>), grid, dim3(kThreads), 0, st, p);
-  else hipLaunchKernelGGL((k_bwd<DT, L, CV, false, false>), grid, dim3(kThreads), 0, st, p);
+  if (gcam && gvol) hipLaunchKernelGGL((k_bwd<DT, L, CV, true, true>), grid, dim3(kBlock), 0, st, p);
+  else if (gcam) hipLaunchKernelGGL((k_bwd<DT, L, CV, true, false>), grid, dim3(kBlock), 0, st, p);
+  else if (gvol) hipLaunchKernelGGL((k_bwd<DT, L, CV, false, true>), grid, dim3(kBlock), 0, st, p);
+  else hipLaunchKernelGGL((k_bwd<DT, L, CV, false, false>), grid, dim3(kBlock), 0, st, p);
 }
 
 template <int DT, int L>
@@ -927,6 +1036,12 @@ int ndrr_raymarch_fwd(const void* vol, int vol_dtype, int vol_layout, const int8
     default: launch_fwd1<NDRR_F16>(p, vol_layout, grid, st); break;
   }
   return launch_status();
+}
+
+void ndrr_launch_tile(int* tile_width, int* tile_height, int* tile_rays) {
+  if (tile_width) *tile_width = kTileW;
+  if (tile_height) *tile_height = kTileH;
+  if (tile_rays) *tile_rays = kTileRays;
 }
 
 size_t ndrr_bwd_workspace_floats(int B, int N, int width) {

@@ -49,10 +49,30 @@ def test_version_and_error_strings():
 
 def test_workspace_size_is_12_floats_per_tile():
     lib = _lib.lib()
-    # 200 x 200 detector: 13 x 13 tiles of 16 x 16
-    assert lib.ndrr_bwd_workspace_floats(16, 40000, 200) == 16 * 169 * 12
-    assert lib.ndrr_bwd_workspace_floats(2, 1000, 0) == 2 * 4 * 12  # runs of 256 rays
+    tw, th, tr = _lib.launch_tile()
+    assert tw % 8 == 0 and th % 8 == 0 and tr == tw * th
+    tiles = -(-200 // tw) * -(-200 // th)
+    assert lib.ndrr_bwd_workspace_floats(16, 40000, 200) == 16 * tiles * 12
+    assert lib.ndrr_bwd_workspace_floats(2, 1000, 0) == 2 * -(-1000 // tr) * 12  # runs of tr rays
     assert lib.ndrr_bwd_workspace_floats(0, 10, 0) == 0
+
+
+def test_packed_sizes():
+    lib = _lib.lib()
+    D, H, W = 5, 6, 7
+    assert lib.ndrr_packed_bytes(D, H, W, 0, 0) == D * H * W * 4
+    assert lib.ndrr_packed_bytes(D, H, W, 1, 1) == (D + 1) * (H + 1) * (W + 1) * 16
+    assert lib.ndrr_packed_bytes(D, H, W, 0, 2) == (D + 1) * (H + 1) * (W + 2) * 16
+    assert lib.ndrr_packed_bytes(D, H, W, 2, 2) == (D + 1) * (H + 1) * (W + 2) * 8
+    assert lib.ndrr_packed_bytes(D, H, W, 1, 7) == 0
+    assert lib.ndrr_packed_label_bytes(D, H, W, 2) == (D + 1) * (H + 1) * (W + 2) * 4
+    assert lib.ndrr_packed_label_bytes(D, H, W, 0) == D * H * W
+
+
+def test_packed_layouts_limited_to_2gib():
+    """Packed layouts are read through 32-bit buffer offsets: larger copies are refused."""
+    lib = _lib.lib()
+    assert _fwd(lib, layout=1, dt=0, D=512, H=512, W=400) == -4  # fp32 stencil: 3.4 GB
 
 
 def _fwd(lib, **kw):
