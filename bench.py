@@ -4,7 +4,8 @@ Workload (fwd+bwd on a 256^3 volume): seeded 256^3 phantom at 1 mm, 4 label
 channels (int8 labels), 16 poses (ZXY +-15 deg, +-20 mm about the isocentre
 600 mm from the source), 200 x 200 detector at 1 mm, sdd 1020 mm, 256
 samples per ray. One step = render(...) of the 16 poses plus the backward of
-a weighted sum to the poses; DRR/s = 16 * steps / elapsed. Cases (volume
+a weighted sum to the poses, captured once in a HIP graph and replayed (the
+eager, uncaptured step is reported too); DRR/s = 16 * steps / elapsed. Cases (volume
 dtype : layout the kernels read): bf16:stencil (the headline `value`),
 fp32:linear (the plain fp32 kernel on the (D, H, W) volume) and fp32:quad;
 every case accumulates in float32.
@@ -52,8 +53,35 @@ def step_fn(subject, k_inv, rt_inv, sdd, wts):
 
     rt = rt_inv.detach().requires_grad_(True)
     out = render(subject, k_inv, rt, sdd, DET, DET, n_samples=S)
-    (out * wts).sum().backward()
+    out.backward(wts.expand_as(out))  # d(sum(out * wts))/d(out) = wts, without a reduction kernel
     return rt.grad
+
+
+def graph_step(subject, k_inv, rt_inv, sdd, wts):
+    """The same step captured once in a HIP graph (torch.cuda.CUDAGraph) and
+    replayed: the counterpart of the reference benchmark's CUDA-graph rows
+    (torch.compile mode="reduce-overhead", tests/benchmark/benchmark.py:226)."""
+    from nanodrr_amd import render
+
+    rt = rt_inv.detach().clone().requires_grad_(True)
+
+    def body():
+        rt.grad = None
+        out = render(subject, k_inv, rt, sdd, DET, DET, n_samples=S)
+        out.backward(wts.expand_as(out))
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(3):
+            body()
+    torch.cuda.current_stream().wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    rt.grad = None
+    with torch.cuda.graph(graph):
+        out = render(subject, k_inv, rt, sdd, DET, DET, n_samples=S)
+        out.backward(wts.expand_as(out))
+    return graph.replay
 
 
 def time_steps(fn, steps, warmup):
@@ -173,10 +201,12 @@ def main():
         renderer.VOLUME_LAYOUT = layout
         subject, k_inv, rt_inv, sdd, ph = make_workload(dtype, dev)
         wts = torch.randn(1, N_CLASSES, DET, DET, generator=torch.Generator().manual_seed(0)).to(dev)
-        ms = time_steps(lambda: step_fn(subject, k_inv, rt_inv, sdd, wts), args.steps, args.warmup)
+        ms_eager = time_steps(lambda: step_fn(subject, k_inv, rt_inv, sdd, wts), args.steps, args.warmup)
+        ms = time_steps(graph_step(subject, k_inv, rt_inv, sdd, wts), args.steps, args.warmup)
         ms_fwd, ms_bwd, geom = kernel_times(subject, k_inv, rt_inv, sdd, wts, max(10, args.steps), layout)
         n_in, b_fwd, b_bwd = algorithmic_bytes(subject, geom, dtype)
-        results[case] = dict(ms_per_step=ms, drr_per_s=BATCH * 1000.0 / ms, ms_fwd=ms_fwd, ms_bwd=ms_bwd,
+        results[case] = dict(ms_per_step=ms, drr_per_s=BATCH * 1000.0 / ms, ms_eager=ms_eager,
+                             drr_per_s_eager=BATCH * 1000.0 / ms_eager, ms_fwd=ms_fwd, ms_bwd=ms_bwd,
                              inside_samples=n_in, bytes_fwd=b_fwd, bytes_bwd=b_bwd,
                              gbs_fwd=b_fwd / ms_fwd / 1e6, gbs_bwd=b_bwd / ms_bwd / 1e6)
         print(f"# {case}: {results[case]}", file=sys.stderr, flush=True)
@@ -205,7 +235,9 @@ def main():
             "workload": "fwd+bwd (pose grads) 256^3 @1mm, C=4, B=16, 200x200, S=256",
             "volume_storage": head_dtype, "volume_layout": head_layout, "accumulate": "fp32", "global_batch": BATCH,
             "n_samples": S,
+            "step": "render + weighted-sum backward captured in one HIP graph, replayed (eager also given)",
             "per_case": {k: {"drr_per_s": v["drr_per_s"], "ms_per_step": v["ms_per_step"],
+                             "drr_per_s_eager": v["drr_per_s_eager"], "ms_per_step_eager": v["ms_eager"],
                               "ms_fwd_kernel": v["ms_fwd"], "ms_bwd_kernel": v["ms_bwd"],
                               "gbs_fwd": v["gbs_fwd"], "gbs_bwd": v["gbs_bwd"]} for k, v in results.items()},
         },
