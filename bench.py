@@ -175,6 +175,42 @@ def cpu_baseline(ph, budget_s=12.0):
                       f"backward, oracle.torch_ref fp32 grid_sample on {threads} threads, median of runs"}
 
 
+def counter_probe(case: str, reps: int = 4):
+    """Launch the fwd and bwd ops of one case `reps` times, then copy a 256 MiB
+    device buffer three times: the run that rocprofv3 --pmc passes profile (the
+    copy calibrates FETCH_SIZE / WRITE_SIZE against known bytes)."""
+    import nanodrr_amd.renderer as renderer
+
+    name, layout = case.split(":")
+    dtype = {"bf16": torch.bfloat16, "fp32": torch.float32, "fp16": torch.float16}[name]
+    renderer.VOLUME_LAYOUT = layout
+    subject, k_inv, rt_inv, sdd, _ = make_workload(dtype, torch.device("cuda:0"))
+    wts = torch.randn(1, N_CLASSES, DET, DET, generator=torch.Generator().manual_seed(0)).cuda()
+    f, b, _ = kernel_times(subject, k_inv, rt_inv, sdd, wts, reps, layout)
+    print(f"probe {case}: fwd {f * 1000:.1f} us, bwd {b * 1000:.1f} us", flush=True)
+    x = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print("probe: 3 copies of 256 MiB done", flush=True)
+
+
+def measured_traffic(case: str):
+    """HBM bytes per launch (read + write, MB) of the case's fwd and bwd kernels, from
+    the committed rocprofv3 counter summary of this workload, or None."""
+    name, layout = case.split(":")
+    path = ROOT / "profiles" / f"raymarch_{name}_{layout}_counters_bench256_c4_b16.md"
+    if not path.exists():
+        return None
+    out = {}
+    for line in path.read_text().splitlines():
+        if line.startswith("| k_fwd") or line.startswith("| k_bwd"):
+            cells = [c.strip() for c in line.strip("|").split("|")]
+            out[cells[0][:5]] = (float(cells[2]) + float(cells[4])) * 1e6
+    return out or None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,7 +218,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cases", default="bf16:stencil,fp32:linear,fp32:quad")
+    ap.add_argument("--probe", default=None, help="run only the counter probe of one case, e.g. bf16:stencil")
     args = ap.parse_args()
+    if args.probe:
+        counter_probe(args.probe)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -218,6 +258,7 @@ def main():
     r = results[head]
     dominant = "bwd" if r["ms_bwd"] >= r["ms_fwd"] else "fwd"
     achieved = r[f"gbs_{dominant}"]
+    traffic = measured_traffic(head)
     line = {
         "metric": "DRR/s fwd+bwd 200x200 @256^3 (fp32 vs bf16)",
         "value": r["drr_per_s"],
@@ -243,7 +284,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "kernel": f"k_{dominant} ({head_dtype} volume, {head_layout} layout)", "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None if traffic is None else traffic.get(f"k_{dominant}"),
+            "traffic_source": "profiles/ rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated" if traffic else None,
             "algorithmic_bytes_per_launch": r[f"bytes_{dominant}"], "inside_samples": r["inside_samples"],
         },
     }
