@@ -176,6 +176,41 @@ int ndrr_pack_labels(const int8_t* lab, int vol_layout, int D, int H, int W, voi
 int ndrr_labels_to_i8(const float* src, int8_t* dst, int64_t numel,
                       int n_classes, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Registration support
+ * ------------------------------------------------------------------------- */
+
+/*
+ * The SE(3) pose step in FP64 (the only FP64 on the GPU). For each of B
+ * problems with parameters w = rot[b*param_stride + 0..2] (rotation vector)
+ * and v = xyz[b*param_stride + 0..2]:
+ *   T = [exp([w]x), exp([w]x) v; 0 0 0 1],
+ *   M[b] = (prefix[b] @ T @ suffix[b])[:3, :]   (fp32, [B][3][4]),
+ * prefix/suffix are [B][4][4] float64 (for registration: prefix =
+ * world_to_voxel @ rt_inv @ pivot, suffix = pivot_inv). ndrr_pose_compose_bwd
+ * maps gM [B][3][4] to the parameter gradients grot, gxyz (same strides).
+ * Replaces Registration.pose and convert(..., "so3_log") of
+ * src/nanodrr/registration/registration.py:64-68 and
+ * src/nanodrr/geometry/se3.py:62-69,109-110, with the render's
+ * world_to_voxel @ rt_inv (src/nanodrr/drr/backends.py:96) folded in.
+ */
+int ndrr_pose_compose(const double* prefix, const double* suffix, const float* rot,
+                      const float* xyz, int64_t param_stride, float* M, int B,
+                      void* stream);
+int ndrr_pose_compose_bwd(const double* prefix, const double* suffix, const float* rot,
+                          const float* xyz, int64_t param_stride, const float* gM,
+                          float* grot, float* gxyz, int B, void* stream);
+
+/*
+ * Zero-normalised cross-correlation of B image pairs of N pixels (fp32):
+ * score[b] = mean(xh * yh), xh = (x - mean x) / sqrt(var x + eps) (population
+ * variance), yh likewise; grad_x (optional, [B][N]) = d score[b] / d x.
+ * Deterministic (fixed-order reductions). Replaces
+ * NormalizedCrossCorrelation2d.forward, src/nanodrr/metrics/ncc.py:16-37.
+ */
+int ndrr_zncc(const float* x, const float* y, int B, int N, float eps, float* score,
+              float* grad_x, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,9 @@ SIGNATURES: dict[str, tuple] = {
     "ndrr_pack_volume": (_c_int, [_c_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p]),
     "ndrr_packed_label_bytes": (_c_size, [_c_int, _c_int, _c_int, _c_int]),
     "ndrr_pack_labels": (_c_int, [_c_p, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p]),
+    "ndrr_pose_compose": (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_int, _c_p]),
+    "ndrr_pose_compose_bwd": (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_int, _c_p]),
+    "ndrr_zncc": (_c_int, [_c_p, _c_p, _c_int, _c_int, ctypes.c_float, _c_p, _c_p, _c_p]),
 }
 
 

@@ -115,3 +115,48 @@ def sample_poses(batch: int, seed: int, rot_deg: float = 15.0, trans_mm: float =
 def detector(sdd: float = 1020.0, spacing: float = 1.0, height: int = 200, width: int = 200, dtype=torch.float32):
     """(k_inv (1, 3, 3), sdd (1,)) for a centred pinhole detector (make_k_inv with x0 = y0 = 0)."""
     return make_k_inv(sdd, spacing, spacing, 0.0, 0.0, height, width, dtype=dtype), torch.tensor([sdd], dtype=dtype)
+
+
+@dataclass
+class RegistrationProblems:
+    rt_true: torch.Tensor     # (B, 4, 4) float64 camera-to-world target poses
+    rt_init: torch.Tensor     # (B, 4, 4) float64 perturbed starting poses
+    fiducials: torch.Tensor   # (K, 3) float64 world points for mTRE
+    k_inv: torch.Tensor       # (1, 3, 3)
+    sdd: torch.Tensor         # (1,)
+    height: int
+    width: int
+
+
+def registration_problems(subject: Subject, batch: int, seed: int, target_rot_deg: float = 30.0,
+                          target_trans_mm: float = 40.0, start_rot_deg: float = 10.0, start_trans_mm: float = 15.0,
+                          n_fiducials: int = 32, distance: float = 600.0, sdd: float = 1020.0, height: int = 200,
+                          width: int = 200) -> RegistrationProblems:
+    """Batched 2D/3D registration problems on `subject`.
+
+    Targets: ZXY Euler rotations in +-target_rot_deg and offsets in
+    +-target_trans_mm about the isocentre at `distance` (as sample_poses).
+    Starts: each target composed with a rigid perturbation about the
+    isocentre, rotation vector in +-start_rot_deg per axis and translation
+    in +-start_trans_mm per axis. Fiducials: uniform in the central 80 % of
+    the volume's world bounding box.
+    """
+    from .geometry import so3_exp
+
+    iso = subject.isocenter.detach().cpu().to(torch.float64)
+    rt_true, _, _ = sample_poses(batch, seed, target_rot_deg, target_trans_mm, distance, iso, torch.float64)
+    g = torch.Generator().manual_seed(int(seed) + 7919)
+    w = (torch.rand(batch, 3, generator=g, dtype=torch.float64) * 2 - 1) * (start_rot_deg * torch.pi / 180.0)
+    t = (torch.rand(batch, 3, generator=g, dtype=torch.float64) * 2 - 1) * start_trans_mm
+    P = torch.eye(4, dtype=torch.float64).repeat(batch, 1, 1)
+    R = so3_exp(w)
+    P[:, :3, :3] = R
+    P[:, :3, 3] = t + iso - (R @ iso[:, None])[..., 0]  # rotate about the isocentre, then shift
+    rt_init = P @ rt_true
+    v2w = subject.voxel_to_world.detach().cpu().to(torch.float64)
+    D, H, W = subject.image.shape[-3:]
+    frac = torch.rand(n_fiducials, 3, generator=g, dtype=torch.float64) * 0.8 + 0.1
+    idx = frac * torch.tensor([W - 1, H - 1, D - 1], dtype=torch.float64)
+    fid = (v2w[:3, :3] @ idx.T).T + v2w[:3, 3]
+    k_inv, sdd_t = detector(sdd, 1.0, height, width, dtype=torch.float64)
+    return RegistrationProblems(rt_true, rt_init, fid, k_inv, sdd_t, height, width)

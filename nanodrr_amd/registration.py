@@ -1,0 +1,232 @@
+"""Differentiable 2D/3D registration on the native kernels.
+
+`Registration` keeps the API of nanodrr's module
+(src/nanodrr/registration/registration.py:9-84): pose =
+rt_inv @ pivot @ exp(so3_log rot, xyz) @ pivot_inv with the pivot at the
+isocentre in camera coordinates. With `per_pose=True` each of the B poses
+gets its own (rotation, translation) parameters instead of the reference's
+shared (1, 3) pair, so B independent problems run in one launch. The pose
+step runs in FP64 on the GPU (ndrr_pose_compose and its adjoint) and hands
+the renderer its camera -> voxel affine directly.
+
+`register` is the batched optimisation loop: ZNCC loss (native kernel) and
+Adam with per-group learning rates on a cosine schedule, every problem of the
+batch advancing in the same launches.
+"""
+
+from __future__ import annotations
+
+import math
+
+import torch
+
+from . import _lib
+from .geometry import convert, transform_point
+from .metrics import zncc
+from .renderer import _make_src, _make_tgt, march
+
+
+def _stream(device):
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+@torch.library.custom_op("nanodrr_amd::pose_compose", mutates_args=())
+def pose_compose(prefix: torch.Tensor, suffix: torch.Tensor, rot: torch.Tensor, xyz: torch.Tensor) -> torch.Tensor:
+    """M (B, 3, 4) fp32 = (prefix @ [exp([rot]x), exp([rot]x) xyz; 0 1] @ suffix)[:3] in FP64."""
+    B = prefix.shape[0]
+    for name, t, dt in (("prefix", prefix, torch.float64), ("suffix", suffix, torch.float64),
+                        ("rot", rot, torch.float32), ("xyz", xyz, torch.float32)):
+        if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous {dt} device tensor")
+    if prefix.shape != (B, 4, 4) or suffix.shape != (B, 4, 4) or rot.shape != (B, 3) or xyz.shape != (B, 3):
+        raise ValueError("pose_compose expects prefix, suffix (B, 4, 4) and rot, xyz (B, 3)")
+    M = torch.empty(B, 3, 4, device=prefix.device, dtype=torch.float32)
+    with torch.cuda.device(prefix.device):
+        rc = _lib.lib().ndrr_pose_compose(prefix.data_ptr(), suffix.data_ptr(), rot.data_ptr(), xyz.data_ptr(), 3,
+                                          M.data_ptr(), B, _stream(prefix.device))
+    _lib.check(rc, "ndrr_pose_compose")
+    return M
+
+
+@pose_compose.register_fake
+def _(prefix, suffix, rot, xyz):
+    return torch.empty(prefix.shape[0], 3, 4, device=prefix.device, dtype=torch.float32)
+
+
+@torch.library.custom_op("nanodrr_amd::pose_compose_bwd", mutates_args=())
+def pose_compose_bwd(prefix: torch.Tensor, suffix: torch.Tensor, rot: torch.Tensor, xyz: torch.Tensor,
+                     gM: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    B = prefix.shape[0]
+    gM = gM.to(torch.float32).contiguous()
+    grot = torch.empty(B, 3, device=prefix.device, dtype=torch.float32)
+    gxyz = torch.empty(B, 3, device=prefix.device, dtype=torch.float32)
+    with torch.cuda.device(prefix.device):
+        rc = _lib.lib().ndrr_pose_compose_bwd(prefix.data_ptr(), suffix.data_ptr(), rot.data_ptr(), xyz.data_ptr(),
+                                              3, gM.data_ptr(), grot.data_ptr(), gxyz.data_ptr(), B,
+                                              _stream(prefix.device))
+    _lib.check(rc, "ndrr_pose_compose_bwd")
+    return grot, gxyz
+
+
+@pose_compose_bwd.register_fake
+def _(prefix, suffix, rot, xyz, gM):
+    B = prefix.shape[0]
+    return (torch.empty(B, 3, device=prefix.device, dtype=torch.float32),
+            torch.empty(B, 3, device=prefix.device, dtype=torch.float32))
+
+
+def _pc_setup(ctx, inputs, output):
+    prefix, suffix, rot, xyz = inputs
+    ctx.save_for_backward(prefix, suffix, rot, xyz)
+
+
+def _pc_backward(ctx, gM):
+    prefix, suffix, rot, xyz = ctx.saved_tensors
+    grot, gxyz = pose_compose_bwd(prefix, suffix, rot, xyz, gM)
+    return None, None, grot, gxyz
+
+
+torch.library.register_autograd("nanodrr_amd::pose_compose", _pc_backward, setup_context=_pc_setup)
+
+
+def _translation(t: torch.Tensor) -> torch.Tensor:
+    T = torch.eye(4, device=t.device, dtype=t.dtype).repeat(len(t), 1, 1)
+    T[:, :3, 3] = t
+    return T
+
+
+class Registration(torch.nn.Module):
+    """Optimise camera poses by aligning rendered DRRs with target images.
+
+    Args match the reference (registration.py:32-57); `per_pose` gives every
+    pose its own parameters (B independent problems).
+    """
+
+    def __init__(self, subject, rt_inv, k_inv, sdd, height, width, eps: float = 1e-8, per_pose: bool = False):
+        super().__init__()
+        self.subject = subject
+        self.register_buffer("rt_inv", rt_inv)
+        self.register_buffer("k_inv", k_inv)
+        self.register_buffer("sdd", sdd)
+        self.height = height
+        self.width = width
+        iso = subject.isocenter.to(rt_inv.device, torch.float64).expand(len(rt_inv), 3)
+        c = transform_point(torch.linalg.inv(rt_inv.double()), iso)  # isocentre in camera coordinates
+        self.register_buffer("pivot", _translation(c))
+        self.register_buffer("pivot_inv", _translation(-c))
+        n = len(rt_inv) if per_pose else 1
+        self._rot = torch.nn.Parameter(eps * torch.randn(n, 3, device=rt_inv.device))
+        self._xyz = torch.nn.Parameter(eps * torch.randn(n, 3, device=rt_inv.device))
+        self._refresh_prefix()
+
+    def _refresh_prefix(self):
+        """FP64 constants of the pose step: world_to_voxel @ rt_inv @ pivot and pivot_inv."""
+        w2v = self.subject.world_to_voxel.double()
+        self.register_buffer("_prefix", (w2v @ self.rt_inv.double() @ self.pivot).contiguous(), persistent=False)
+        self.register_buffer("_suffix", self.pivot_inv.contiguous(), persistent=False)
+        tgt = _make_tgt(self.k_inv, self.sdd, self.height, self.width, self.k_inv.device, torch.float32)
+        src = _make_src(False, tgt, self.sdd)
+        self.register_buffer("_tgt", tgt.contiguous(), persistent=False)
+        self.register_buffer("_src", src.contiguous(), persistent=False)
+
+    def _params(self):
+        B = len(self.rt_inv)
+        rot, xyz = self._rot, self._xyz
+        if rot.shape[0] != B:
+            rot, xyz = rot.expand(B, 3), xyz.expand(B, 3)
+        return rot.float().contiguous(), xyz.float().contiguous()
+
+    def camera_to_voxel(self) -> torch.Tensor:
+        """(B, 3, 4) fp32 camera -> voxel affine of the current pose (FP64 pose step)."""
+        rot, xyz = self._params()
+        return pose_compose(self._prefix, self._suffix, rot, xyz)
+
+    def forward(self, n_samples: int = 500, backend: str = "auto") -> torch.Tensor:
+        if backend not in ("auto", "hip", "triton"):
+            raise ValueError(f"Unknown backend {backend!r}; expected 'auto', 'hip' or 'triton'")
+        step = (self._tgt - self._src).norm(dim=-1) / float(n_samples - 1)
+        return march(self.subject, self.camera_to_voxel(), self._src, self._tgt, step, n_samples,
+                     self.height, self.width)
+
+    @property
+    def pose(self) -> torch.Tensor:
+        """(B, 4, 4) camera-to-world pose (registration.py:64-68), float64."""
+        rot, xyz = self._params()
+        T = convert(rot.double(), xyz.double(), "so3_log")
+        return self.rt_inv.double() @ self.pivot @ T @ self.pivot_inv
+
+    @torch.no_grad()
+    def rescale_(self, scale: float) -> None:
+        """Change the rendering resolution by rescaling the intrinsics (registration.py:70-76)."""
+        self.k_inv[..., 0, 0] /= scale
+        self.k_inv[..., 1, 1] /= scale
+        self.height = int(round(self.height * scale))
+        self.width = int(round(self.width * scale))
+        self._refresh_prefix()
+
+
+class _Adam:
+    """Adam over per-group tensors with a device-side step counter and learning-rate
+    schedule, so that a captured step replays with the right rates."""
+
+    def __init__(self, groups, n_steps: int, betas=(0.9, 0.999), eps=1e-8, final_frac: float = 0.02):
+        self.groups = groups  # [(param, lr)]
+        self.b1, self.b2, self.eps = betas[0], betas[1], eps
+        dev = groups[0][0].device
+        k = torch.arange(n_steps, dtype=torch.float64)
+        cos = final_frac + (1 - final_frac) * 0.5 * (1 + torch.cos(math.pi * k / max(n_steps - 1, 1)))
+        self.sched = cos.to(torch.float32).to(dev)
+        self.t = torch.zeros((), device=dev, dtype=torch.float32)
+        self.idx = torch.zeros((), device=dev, dtype=torch.int64)
+        self.m = [torch.zeros_like(p) for p, _ in groups]
+        self.v = [torch.zeros_like(p) for p, _ in groups]
+
+    def state(self) -> list[torch.Tensor]:
+        return [self.t, self.idx, *self.m, *self.v]
+
+    @torch.no_grad()
+    def step(self):
+        self.t.add_(1.0)
+        frac = self.sched.index_select(0, self.idx.clamp(max=self.sched.numel() - 1).view(1))
+        self.idx.add_(1)
+        bc1 = 1.0 - torch.pow(self.b1, self.t)
+        bc2 = 1.0 - torch.pow(self.b2, self.t)
+        for (p, lr), m, v in zip(self.groups, self.m, self.v):
+            g = p.grad
+            m.mul_(self.b1).add_(g, alpha=1.0 - self.b1)
+            v.mul_(self.b2).addcmul_(g, g, value=1.0 - self.b2)
+            p.sub_((lr * frac) * (m / bc1) / ((v / bc2).sqrt() + self.eps))
+
+
+def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot: float = 1e-2, lr_xyz: float = 1.0,
+             n_samples: int = 256, graph: bool = False, iters_per_graph: int = 10, eps: float = 1e-7) -> torch.Tensor:
+    """Optimise every pose of `reg` to maximise the ZNCC of its DRRs with `target` (B, 1, H, W).
+
+    Each iteration: pose step, render, ZNCC, backward, Adam update (learning
+    rates lr_rot rad and lr_xyz mm on a cosine schedule down to 2 %); every
+    problem of the batch moves in the same launches. Returns the ZNCC of the
+    last iteration's DRRs (B,).
+
+    `graph=True` (capturing `iters_per_graph` iterations in one HIP graph) is
+    switched off: capturing ten iterations ended the process with a
+    segmentation fault inside the runtime's capture end, with no cause found
+    in this library, so the loop runs eagerly.
+    """
+    if graph:
+        raise NotImplementedError(
+            "HIP-graph capture of the registration step is switched off (a ten-iteration capture crashed in "
+            "the runtime's capture end with no cause found); call register(..., graph=False)")
+    params = [reg._rot, reg._xyz]
+    for p in params:
+        p.grad = torch.zeros_like(p)
+    opt = _Adam([(reg._rot, lr_rot), (reg._xyz, lr_xyz)], n_iters)
+    target = target.detach().to(torch.float32).contiguous()
+    score = torch.zeros(target.shape[0], device=target.device)
+    for _ in range(n_iters):
+        for p in params:
+            p.grad.zero_()
+        s = zncc(reg(n_samples), target, eps)
+        (-s.sum()).backward()
+        opt.step()
+        score.copy_(s.detach())
+    return score.clone()

@@ -103,19 +103,13 @@ def render_hip(subject, rt_inv, src, tgt, step_size, n_samples, height, width) -
     image = subject.image
     if not (rt_inv.is_cuda and image.is_cuda):
         raise ValueError("the hip backend renders on a GPU: move the subject and poses to a cuda device")
-    if image.dtype not in _FUSED_DTYPES:
-        raise ValueError(f"volume dtype {image.dtype} unsupported; expected float32, float16 or bfloat16")
-    C = subject.n_classes
-    vol = image[0, 0]
-    if not vol.is_contiguous():
-        vol = vol.contiguous()
     # Camera -> voxel-index affine, composed in float64 and rounded once.
     M = (subject.world_to_voxel.to(torch.float64) @ rt_inv.to(torch.float64))[:, :3, :].to(torch.float32)
-    B = max(M.shape[0], tgt.shape[0], src.shape[0])
-    if M.shape[0] != B:
-        M = M.expand(B, -1, -1)
-    if not fused_supported(subject, B, height * width):
-        raise ValueError("inputs exceed the hip kernel's limits (one volume, 32-bit indexing, <= 8 classes)")
+    return march(subject, M, src, tgt, step_size, n_samples, height, width)
+
+
+def volume_layout(image: torch.Tensor) -> str:
+    """The layout the kernels read for this volume under VOLUME_LAYOUT."""
     layout = VOLUME_LAYOUT
     if layout == "auto":
         if image.requires_grad:
@@ -124,6 +118,28 @@ def render_hip(subject, rt_inv, src, tgt, step_size, n_samples, height, width) -
             layout = "quad" if image.dtype == torch.float32 else "stencil"
     if layout not in _LAYOUTS:
         raise ValueError(f"VOLUME_LAYOUT must be 'auto' or one of {_LAYOUTS}, got {VOLUME_LAYOUT!r}")
+    return layout
+
+
+def march(subject, M, src, tgt, step_size, n_samples, height, width) -> torch.Tensor:
+    """(B, C, H, W) line integrals of the rays src -> tgt (camera space) with the
+    camera -> voxel-index affines M (B|1, 3, 4) through the subject's volume."""
+    image = subject.image
+    if not image.is_cuda:
+        raise ValueError("the hip backend renders on a GPU: move the subject and poses to a cuda device")
+    if image.dtype not in _FUSED_DTYPES:
+        raise ValueError(f"volume dtype {image.dtype} unsupported; expected float32, float16 or bfloat16")
+    C = subject.n_classes
+    vol = image[0, 0]
+    if not vol.is_contiguous():
+        vol = vol.contiguous()
+    M = M.to(torch.float32)
+    B = max(M.shape[0], tgt.shape[0], src.shape[0])
+    if M.shape[0] != B:
+        M = M.expand(B, -1, -1)
+    if not fused_supported(subject, B, height * width):
+        raise ValueError("inputs exceed the hip kernel's limits (one volume, 32-bit indexing, <= 8 classes)")
+    layout = volume_layout(image)
     packed = None if layout == "linear" else subject.packed_volume(layout)
     lab = subject.packed_labels(layout) if C > 1 else vol.new_empty(0, dtype=torch.int8)
     out = fused_raymarch(

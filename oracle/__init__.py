@@ -16,8 +16,11 @@ Modules
              baseline.
   geometry   float64 numpy camera/pose helpers (intrinsics.py:53-68,
              extrinsics.py:7-57, se3.py:179-221).
+  registration float64 numpy pose step of the registration loop
+             (registration.py:64-68) with its adjoint, and ZNCC with its
+             gradient (metrics/ncc.py:8-37).
 
-Parity pinning (tests/test_oracle.py, tests/test_geometry.py):
+Parity pinning (tests/test_oracle.py, tests/test_geometry.py, tests/test_registration.py):
   raymarch.forward     unit-impulse known answer (tests/drr/test_renderer.py:52-91),
                        constant-volume chord length, torch grid_sample float64,
                        channel-sum invariant
@@ -27,5 +30,9 @@ Parity pinning (tests/test_oracle.py, tests/test_geometry.py):
   raymarch.inside_count brute force over every sample
   geometry             scipy Rotation.from_euler("ZXY"), matrix exponential,
                        round trips (tests/camera/test_extrinsics.py:11-54)
+  registration         scipy Rotation.from_rotvec, torch float64 autograd of
+                       matrix_exp, central finite differences (pose step);
+                       scipy.stats.pearsonr, affine invariance, finite
+                       differences (ZNCC)
 No function here is "parity unpinned".
 """
