@@ -211,6 +211,48 @@ def measured_traffic(case: str):
     return out or None
 
 
+def registration_leg(n_iters: int = 300, batch: int = 64):
+    """Batched registration on the 256^3 bf16 phantom: 64 problems with targets at +-30 deg /
+    +-40 mm, starts perturbed +-10 deg / +-15 mm about the isocentre, 200 x 200 DRRs, -ZNCC
+    loss, Adam (1e-2 rad, 1 mm), 300 iterations. Targets are rendered with S=1024 (a smooth
+    stand-in for a radiograph); the optimised DRRs use S=256 with the sample lattice jittered
+    by a random sub-step offset each iteration. Registrations/s = problems ending with
+    mTRE < 1 mm (32 fiducials) / wall time of the optimisation loop."""
+    from nanodrr_amd import render
+    from nanodrr_amd.metrics import mtre
+    from nanodrr_amd.phantom import make_phantom, phantom_subject, registration_problems
+    from nanodrr_amd.registration import Registration, register
+
+    dev = torch.device("cuda:0")
+    ph = make_phantom((SIZE,) * 3, (SPACING,) * 3, seed=2)
+    subject = phantom_subject(ph, n_classes=1, dtype=torch.bfloat16, device=dev)
+    prob = registration_problems(subject, batch, seed=3, height=DET, width=DET)
+    k_inv, sdd = prob.k_inv.float().to(dev), prob.sdd.float().to(dev)
+    with torch.no_grad():
+        target = render(subject, k_inv, prob.rt_true.float().to(dev), sdd, DET, DET, n_samples=4 * S)
+    torch.manual_seed(0)
+    reg = Registration(subject, prob.rt_init.float().to(dev), k_inv, sdd, DET, DET, per_pose=True)
+    fid = prob.fiducials.to(dev)
+    before = mtre(reg.pose.detach(), prob.rt_true.to(dev), fid)
+    reg(S)  # builds the packed volume once, outside the timed loop
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    score = register(reg, target, n_iters=n_iters, n_samples=S, jitter=True, seed=0)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    after = mtre(reg.pose.detach(), prob.rt_true.to(dev), fid)
+    ok = int((after < 1.0).sum())
+    return {
+        "workload": f"{batch} problems, 256^3 bf16, 200x200, S={S} jittered (targets S={4 * S}), "
+                    f"targets +-30deg/+-40mm, starts +-10deg/+-15mm, Adam 1e-2 rad / 1 mm, {n_iters} iterations, "
+                    "eager launches",
+        "registrations_per_s": ok / wall,
+        "converged": ok, "problems": batch, "wall_s": wall, "iterations_per_s": n_iters / wall,
+        "mtre_mm_median_before": float(before.median()), "mtre_mm_median_after": float(after.median()),
+        "zncc_median_after": float(score.median()),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,9 +261,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cases", default="bf16:stencil,fp32:linear,fp32:quad")
     ap.add_argument("--probe", default=None, help="run only the counter probe of one case, e.g. bf16:stencil")
+    ap.add_argument("--no-registration", action="store_true", help="skip the batched registration leg")
+    ap.add_argument("--registration-only", action="store_true", help="run only the registration leg")
     args = ap.parse_args()
     if args.probe:
         counter_probe(args.probe)
+        return
+    if args.registration_only:
+        print(json.dumps(registration_leg()), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -296,6 +343,8 @@ def main():
             "step": r["drr_per_s"] / plain["drr_per_s"],
             "kernels": (plain["ms_fwd"] + plain["ms_bwd"]) / (r["ms_fwd"] + r["ms_bwd"]),
         }
+    if not args.no_registration:
+        line["registration"] = registration_leg()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(ph)
     print(json.dumps(line), flush=True)

@@ -141,12 +141,18 @@ class Registration(torch.nn.Module):
         rot, xyz = self._params()
         return pose_compose(self._prefix, self._suffix, rot, xyz)
 
-    def forward(self, n_samples: int = 500, backend: str = "auto") -> torch.Tensor:
+    def forward(self, n_samples: int = 500, backend: str = "auto", offset: torch.Tensor | None = None) -> torch.Tensor:
+        """DRRs of the current poses (registration.py:59-62). `offset` (B,), in units of the
+        sample step, slides every ray's sample lattice along the ray (src and tgt move
+        together, the step is unchanged); None renders exactly as `render`."""
         if backend not in ("auto", "hip", "triton"):
             raise ValueError(f"Unknown backend {backend!r}; expected 'auto', 'hip' or 'triton'")
-        step = (self._tgt - self._src).norm(dim=-1) / float(n_samples - 1)
-        return march(self.subject, self.camera_to_voxel(), self._src, self._tgt, step, n_samples,
-                     self.height, self.width)
+        src, tgt = self._src, self._tgt
+        step = (tgt - src).norm(dim=-1) / float(n_samples - 1)
+        if offset is not None:
+            d = (tgt - src) * (offset.reshape(-1, 1, 1).to(tgt.dtype) / float(n_samples - 1))
+            src, tgt = src + d, tgt + d
+        return march(self.subject, self.camera_to_voxel(), src, tgt, step, n_samples, self.height, self.width)
 
     @property
     def pose(self) -> torch.Tensor:
@@ -199,13 +205,20 @@ class _Adam:
 
 
 def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot: float = 1e-2, lr_xyz: float = 1.0,
-             n_samples: int = 256, graph: bool = False, iters_per_graph: int = 10, eps: float = 1e-7) -> torch.Tensor:
+             n_samples: int = 256, jitter: bool = False, seed: int = 0, graph: bool = False,
+             iters_per_graph: int = 10, eps: float = 1e-7) -> torch.Tensor:
     """Optimise every pose of `reg` to maximise the ZNCC of its DRRs with `target` (B, 1, H, W).
 
     Each iteration: pose step, render, ZNCC, backward, Adam update (learning
     rates lr_rot rad and lr_xyz mm on a cosine schedule down to 2 %); every
-    problem of the batch moves in the same launches. Returns the ZNCC of the
-    last iteration's DRRs (B,).
+    problem of the batch advances in the same launches. Returns the ZNCC of
+    the last iteration's DRRs (B,).
+
+    A DRR sampled every `step` mm along rays fixed to the camera repeats itself
+    when the camera moves one step along its axis, so the similarity to a target
+    has local maxima spaced one step apart in depth. `jitter` slides each
+    problem's sample lattice by a fresh uniform offset in [-1/2, 1/2) step every
+    iteration (seeded by `seed`), which averages that ripple out.
 
     `graph=True` (capturing `iters_per_graph` iterations in one HIP graph) is
     switched off: capturing ten iterations ended the process with a
@@ -221,11 +234,14 @@ def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot
         p.grad = torch.zeros_like(p)
     opt = _Adam([(reg._rot, lr_rot), (reg._xyz, lr_xyz)], n_iters)
     target = target.detach().to(torch.float32).contiguous()
-    score = torch.zeros(target.shape[0], device=target.device)
+    B = target.shape[0]
+    score = torch.zeros(B, device=target.device)
+    gen = torch.Generator(device=target.device).manual_seed(int(seed)) if jitter else None
     for _ in range(n_iters):
         for p in params:
             p.grad.zero_()
-        s = zncc(reg(n_samples), target, eps)
+        offset = torch.rand(B, device=target.device, generator=gen) - 0.5 if jitter else None
+        s = zncc(reg(n_samples, offset=offset), target, eps)
         (-s.sum()).backward()
         opt.step()
         score.copy_(s.detach())

@@ -259,3 +259,24 @@ def test_registration_trajectory_matches_oracle_renderer():
     d_xyz = (reg._xyz.detach().cpu() - xyz.detach()).abs().max().item()
     print("trajectory difference: rot", d_rot, "rad, xyz", d_xyz, "mm")
     assert d_rot < 1e-3 and d_xyz < 0.1
+
+
+def test_sample_lattice_offset():
+    """offset=0 renders exactly as render(); an offset u slides src and tgt by u steps."""
+    from nanodrr_amd import render
+
+    subject, prob, reg = _phantom_registration(B=3, seed=6)
+    with torch.no_grad():
+        base = reg(96)
+        zero = reg(96, offset=torch.zeros(3, device=DEV))
+        assert torch.equal(base, zero)
+        u = torch.tensor([0.5, -0.25, 1.0], device=DEV)
+        got = reg(96, offset=u)
+        src, tgt = reg._src, reg._tgt
+        d = (tgt - src) * (u.reshape(-1, 1, 1) / 95.0)
+        ref = render(subject, reg.k_inv, reg.pose.float(), reg.sdd, reg.height, reg.width, 96, src=src + d,
+                     tgt=tgt + d)
+    err = (got - ref).abs().max() / ref.abs().max()
+    assert err < 1e-5, err
+    # a whole-step offset moves each sample onto its neighbour's position: interior sums agree
+    assert (got[2] - base[2]).abs().max() / base[2].abs().max() < 0.05
