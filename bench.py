@@ -5,19 +5,25 @@ channels (int8 labels), 16 poses (ZXY +-15 deg, +-20 mm about the isocentre
 600 mm from the source), 200 x 200 detector at 1 mm, sdd 1020 mm, 256
 samples per ray. One step = render(...) of the 16 poses plus the backward of
 a weighted sum to the poses, captured once in a HIP graph and replayed (the
-eager, uncaptured step is reported too); DRR/s = 16 * steps / elapsed. Cases (volume
-dtype : layout the kernels read): bf16:stencil (the headline `value`),
-fp32:linear (the plain fp32 kernel on the (D, H, W) volume) and fp32:quad;
-every case accumulates in float32.
+eager, uncaptured step is reported too); DRR/s = 16 * steps / elapsed.
+Cases (volume dtype : layout the kernels read : adjoint):
+  bf16:stencil:single  the headline `value`: bf16 stencil records, single-pass
+                       adjoint (the forward march stores per-ray sums, the
+                       backward contracts them);
+  fp32:linear:two      the plain fp32 kernel: the (D, H, W) volume, two-pass
+                       adjoint (the backward marches the volume again, as
+                       nanodrr's kernel does) - the baseline of the speed-ups;
+  fp32:quad:single, bf16:stencil:two  the other two corners.
+Every case accumulates in float32.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cases bf16:stencil,...] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cases bf16:stencil:single,...] [--no-cpu-baseline]
 
-Prints one JSON line. `roofline` covers the dominant kernel (the backward
-march) timed with HIP events on the stream it runs on; `cpu_baseline` times
+Prints one JSON line. `roofline` covers the dominant kernel of the headline
+case, timed with HIP events on the stream it runs on; `cpu_baseline` times
 the pure-PyTorch grid_sample restatement (fp32) of the same workload on the
-host cores for a bounded sample of DRRs.
+host cores for a bounded sample of DRRs; `registration` is the batched
+registration leg (registrations/s reaching mTRE < 1 mm).
 """
-
 from __future__ import annotations
 
 import argparse
@@ -46,6 +52,21 @@ def make_workload(dtype, device):
     rt_inv, _, _ = sample_poses(BATCH, seed=1, isocenter=subject.isocenter)
     k_inv, sdd = detector(1020.0, 1.0, DET, DET)
     return subject, k_inv.to(device), rt_inv.to(device), sdd.to(device), ph
+
+
+DTYPES = {"bf16": torch.bfloat16, "fp32": torch.float32, "fp16": torch.float16}
+ADJOINTS = {"single": "single_pass", "two": "two_pass"}
+
+
+def parse_case(case: str):
+    """"dtype:layout[:adjoint]" -> (dtype name, layout, adjoint name); the adjoint
+    defaults to the single pass."""
+    parts = case.split(":")
+    name, layout = parts[0], parts[1]
+    adjoint = parts[2] if len(parts) > 2 else "single"
+    if name not in DTYPES or adjoint not in ADJOINTS:
+        raise ValueError(f"bad case {case!r}: expected dtype:layout[:single|two]")
+    return name, layout, adjoint
 
 
 def step_fn(subject, k_inv, rt_inv, sdd, wts):
@@ -97,10 +118,12 @@ def time_steps(fn, steps, warmup):
     return t0.elapsed_time(t1) / steps  # ms per step
 
 
-def kernel_times(subject, k_inv, rt_inv, sdd, wts, reps, layout):
+def kernel_times(subject, k_inv, rt_inv, sdd, wts, reps, layout, adjoint="two"):
     """Average duration (ms) of one forward and one backward op launch, HIP events
-    recorded on the stream the ops launch on (torch's current stream)."""
-    from nanodrr_amd._ops import fused_raymarch, fused_raymarch_bwd
+    recorded on the stream the ops launch on (torch's current stream). The
+    single-pass forward also writes the adjoint's per-ray sums and its backward
+    is the contraction."""
+    from nanodrr_amd._ops import fused_raymarch, fused_raymarch_bwd, fused_raymarch_bwd_jac, fused_raymarch_jac
     from nanodrr_amd.renderer import _make_src, _make_tgt
 
     vol = subject.image[0, 0]
@@ -124,20 +147,29 @@ def kernel_times(subject, k_inv, rt_inv, sdd, wts, reps, layout):
         torch.cuda.synchronize()
         return t0.elapsed_time(t1) / reps
 
-    ms_fwd = timed(lambda: fused_raymarch(vol, lab, M, src, tgt, step, S, N_CLASSES, DET, packed, layout))
-    ms_bwd = timed(lambda: fused_raymarch_bwd(go, vol, lab, M, src, tgt, step, S, N_CLASSES, DET, False, False,
-                                              packed, layout))
+    if adjoint == "single":
+        jac = fused_raymarch_jac(vol, lab, M, src, tgt, step, S, N_CLASSES, DET, 6, packed, layout)[1]
+        ms_fwd = timed(lambda: fused_raymarch_jac(vol, lab, M, src, tgt, step, S, N_CLASSES, DET, 6, packed, layout))
+        ms_bwd = timed(lambda: fused_raymarch_bwd_jac(go, jac, M, src, tgt, step, DET, False))
+    else:
+        ms_fwd = timed(lambda: fused_raymarch(vol, lab, M, src, tgt, step, S, N_CLASSES, DET, packed, layout))
+        ms_bwd = timed(lambda: fused_raymarch_bwd(go, vol, lab, M, src, tgt, step, S, N_CLASSES, DET, False, False,
+                                                  packed, layout))
     return ms_fwd, ms_bwd, (M, src, tgt)
 
 
-def algorithmic_bytes(subject, geom, dtype):
-    from nanodrr_amd.roofline import inside_samples, march_bytes
+def algorithmic_bytes(subject, geom, dtype, adjoint="two"):
+    from nanodrr_amd.roofline import contraction_bytes, inside_samples, march_bytes
 
     M, src, tgt = geom
     n_in = int(inside_samples(M, src, tgt, S, tuple(subject.image.shape[-3:])).sum())
     n_rays = M.shape[0] * tgt.shape[1]
-    fwd = march_bytes(n_in, n_rays, M.shape[0], N_CLASSES, dtype)
-    bwd = march_bytes(n_in, n_rays, M.shape[0], N_CLASSES, dtype, backward=True)
+    if adjoint == "single":
+        fwd = march_bytes(n_in, n_rays, M.shape[0], N_CLASSES, dtype, jac_planes=6)
+        bwd = contraction_bytes(n_rays, M.shape[0], N_CLASSES, 6)
+    else:
+        fwd = march_bytes(n_in, n_rays, M.shape[0], N_CLASSES, dtype)
+        bwd = march_bytes(n_in, n_rays, M.shape[0], N_CLASSES, dtype, backward=True)
     return n_in, fwd, bwd
 
 
@@ -181,12 +213,11 @@ def counter_probe(case: str, reps: int = 4):
     copy calibrates FETCH_SIZE / WRITE_SIZE against known bytes)."""
     import nanodrr_amd.renderer as renderer
 
-    name, layout = case.split(":")
-    dtype = {"bf16": torch.bfloat16, "fp32": torch.float32, "fp16": torch.float16}[name]
+    name, layout, adjoint = parse_case(case)
     renderer.VOLUME_LAYOUT = layout
-    subject, k_inv, rt_inv, sdd, _ = make_workload(dtype, torch.device("cuda:0"))
+    subject, k_inv, rt_inv, sdd, _ = make_workload(DTYPES[name], torch.device("cuda:0"))
     wts = torch.randn(1, N_CLASSES, DET, DET, generator=torch.Generator().manual_seed(0)).cuda()
-    f, b, _ = kernel_times(subject, k_inv, rt_inv, sdd, wts, reps, layout)
+    f, b, _ = kernel_times(subject, k_inv, rt_inv, sdd, wts, reps, layout, adjoint)
     print(f"probe {case}: fwd {f * 1000:.1f} us, bwd {b * 1000:.1f} us", flush=True)
     x = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
     y = torch.empty_like(x)
@@ -199,8 +230,8 @@ def counter_probe(case: str, reps: int = 4):
 def measured_traffic(case: str):
     """HBM bytes per launch (read + write, MB) of the case's fwd and bwd kernels, from
     the committed rocprofv3 counter summary of this workload, or None."""
-    name, layout = case.split(":")
-    path = ROOT / "profiles" / f"raymarch_{name}_{layout}_counters_bench256_c4_b16.md"
+    name, layout, adjoint = parse_case(case)
+    path = ROOT / "profiles" / f"raymarch_{name}_{layout}_{adjoint}_counters_bench256_c4_b16.md"
     if not path.exists():
         return None
     out = {}
@@ -259,8 +290,10 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cases", default="bf16:stencil,fp32:linear,fp32:quad")
-    ap.add_argument("--probe", default=None, help="run only the counter probe of one case, e.g. bf16:stencil")
+    ap.add_argument("--cases", default="bf16:stencil:single,fp32:linear:two,fp32:quad:single,bf16:stencil:two",
+                    help="dtype:layout:adjoint cases; the first is the headline, fp32:linear:two the plain kernel")
+    ap.add_argument("--probe", default=None,
+                    help="run only the counter probe of one case, e.g. bf16:stencil:single")
     ap.add_argument("--no-registration", action="store_true", help="skip the batched registration leg")
     ap.add_argument("--registration-only", action="store_true", help="run only the registration leg")
     args = ap.parse_args()
@@ -279,19 +312,19 @@ def main():
     import nanodrr_amd.renderer as renderer
 
     dev = torch.device("cuda:0")
-    dtypes = {"bf16": torch.bfloat16, "fp32": torch.float32, "fp16": torch.float16}
     results = {}
     ph = None
     for case in args.cases.split(","):
-        name, layout = case.split(":")
-        dtype = dtypes[name]
+        name, layout, adjoint = parse_case(case)
+        dtype = DTYPES[name]
         renderer.VOLUME_LAYOUT = layout
+        renderer.ADJOINT = ADJOINTS[adjoint]
         subject, k_inv, rt_inv, sdd, ph = make_workload(dtype, dev)
         wts = torch.randn(1, N_CLASSES, DET, DET, generator=torch.Generator().manual_seed(0)).to(dev)
         ms_eager = time_steps(lambda: step_fn(subject, k_inv, rt_inv, sdd, wts), args.steps, args.warmup)
         ms = time_steps(graph_step(subject, k_inv, rt_inv, sdd, wts), args.steps, args.warmup)
-        ms_fwd, ms_bwd, geom = kernel_times(subject, k_inv, rt_inv, sdd, wts, max(10, args.steps), layout)
-        n_in, b_fwd, b_bwd = algorithmic_bytes(subject, geom, dtype)
+        ms_fwd, ms_bwd, geom = kernel_times(subject, k_inv, rt_inv, sdd, wts, max(10, args.steps), layout, adjoint)
+        n_in, b_fwd, b_bwd = algorithmic_bytes(subject, geom, dtype, adjoint)
         results[case] = dict(ms_per_step=ms, drr_per_s=BATCH * 1000.0 / ms, ms_eager=ms_eager,
                              drr_per_s_eager=BATCH * 1000.0 / ms_eager, ms_fwd=ms_fwd, ms_bwd=ms_bwd,
                              inside_samples=n_in, bytes_fwd=b_fwd, bytes_bwd=b_bwd,
@@ -300,8 +333,9 @@ def main():
         del subject
         torch.cuda.empty_cache()
 
+    renderer.ADJOINT = "single_pass"
     head = next(iter(results))
-    head_dtype, head_layout = head.split(":")
+    head_dtype, head_layout, head_adjoint = parse_case(head)
     r = results[head]
     dominant = "bwd" if r["ms_bwd"] >= r["ms_fwd"] else "fwd"
     achieved = r[f"gbs_{dominant}"]
@@ -321,8 +355,8 @@ def main():
         "data": "synthetic (seeded 256^3 phantom, 4 labels, 16 random poses)",
         "config": {
             "workload": "fwd+bwd (pose grads) 256^3 @1mm, C=4, B=16, 200x200, S=256",
-            "volume_storage": head_dtype, "volume_layout": head_layout, "accumulate": "fp32", "global_batch": BATCH,
-            "n_samples": S,
+            "volume_storage": head_dtype, "volume_layout": head_layout, "adjoint": ADJOINTS[head_adjoint],
+            "accumulate": "fp32", "global_batch": BATCH, "n_samples": S,
             "step": "render + weighted-sum backward captured in one HIP graph, replayed (eager also given)",
             "per_case": {k: {"drr_per_s": v["drr_per_s"], "ms_per_step": v["ms_per_step"],
                              "drr_per_s_eager": v["drr_per_s_eager"], "ms_per_step_eager": v["ms_eager"],
@@ -330,15 +364,17 @@ def main():
                               "gbs_fwd": v["gbs_fwd"], "gbs_bwd": v["gbs_bwd"]} for k, v in results.items()},
         },
         "roofline": {
-            "bound": "hbm", "kernel": f"k_{dominant} ({head_dtype} volume, {head_layout} layout)", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "bound": "hbm",
+            "kernel": f"k_{dominant} ({head_dtype} volume, {head_layout} layout, {ADJOINTS[head_adjoint]} adjoint)",
+            "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": None if traffic is None else traffic.get(f"k_{dominant}"),
             "traffic_source": "profiles/ rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated" if traffic else None,
             "algorithmic_bytes_per_launch": r[f"bytes_{dominant}"], "inside_samples": r["inside_samples"],
         },
     }
-    if "fp32:linear" in results and head != "fp32:linear":
-        plain = results["fp32:linear"]
+    if "fp32:linear:two" in results and head != "fp32:linear:two":
+        plain = results["fp32:linear:two"]
         line["config"]["speedup_over_fp32_plain"] = {
             "step": r["drr_per_s"] / plain["drr_per_s"],
             "kernels": (plain["ms_fwd"] + plain["ms_bwd"]) / (r["ms_fwd"] + r["ms_bwd"]),

@@ -102,7 +102,9 @@ void ndrr_launch_tile(int* tile_width, int* tile_height, int* tile_rays);
 /*
  * Number of fp32 elements of the pose-gradient workspace that
  * ndrr_raymarch_bwd needs for B poses of N rays at this width
- * (12 per pose per detector tile). Pure host arithmetic.
+ * (12 per pose per detector tile, or per wave of every tile, as built).
+ * Pure host arithmetic.
+ * The workspace must be 16-byte aligned (hipMalloc memory always is).
  */
 size_t ndrr_bwd_workspace_floats(int B, int N, int width);
 
@@ -132,6 +134,45 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype,
                       int64_t src_batch_stride, int64_t src_ray_stride,
                       int64_t tgt_batch_stride, int64_t step_batch_stride,
                       void* stream);
+
+/*
+ * Single-pass adjoint: the forward and backward above as one volume march
+ * plus a per-ray contraction, for callers that need pose (and camera)
+ * gradients but not the volume gradient. Together they replace the pair
+ * fused_raymarch + fused_raymarch_bwd (src/nanodrr/drr/_fused.py:349-460,
+ * 477-531) when need_gvol is false; the reference has no split like this.
+ *
+ * ndrr_raymarch_fwd_jac writes `out` exactly as ndrr_raymarch_fwd and, in the
+ * same march, jac[B][jac_planes][C][N] (fp32): per ray and class c the sums
+ * over the samples routed to c of grad v (planes 0-2, voxel units), of
+ * t * grad v (planes 3-5) and, with jac_planes == 7, of v (plane 6).
+ * jac == NULL makes it ndrr_raymarch_fwd.
+ *
+ * ndrr_raymarch_bwd_jac then gives gM (and gsrc, gtgt, gstep when
+ * need_gcam != 0, which needs jac_planes == 7) for an upstream gradient
+ * go[B][C][N] (go_batch_stride C * N, or 0 for one [C][N] gradient shared by
+ * every pose) from jac alone, without reading the volume; the geometry
+ * arguments must be those of the forward. The same workspace as
+ * ndrr_raymarch_bwd; deterministic.
+ */
+int ndrr_raymarch_fwd_jac(const void* vol, int vol_dtype, int vol_layout,
+                          const int8_t* lab, const float* M, const float* src,
+                          const float* tgt, const float* step, float* out,
+                          float* jac, int jac_planes,
+                          int B, int N, int D, int H, int W,
+                          int n_samples, int n_classes, int width,
+                          int64_t src_batch_stride, int64_t src_ray_stride,
+                          int64_t tgt_batch_stride, int64_t step_batch_stride,
+                          void* stream);
+int ndrr_raymarch_bwd_jac(const float* go, int64_t go_batch_stride,
+                          const float* jac, int jac_planes,
+                          const float* M, const float* src, const float* tgt,
+                          const float* step, float* workspace, float* gM,
+                          float* gsrc, float* gtgt, float* gstep, int need_gcam,
+                          int B, int N, int n_classes, int width,
+                          int64_t src_batch_stride, int64_t src_ray_stride,
+                          int64_t tgt_batch_stride, int64_t step_batch_stride,
+                          void* stream);
 
 /*
  * Packed copies of a [D][H][W] volume that the march reads with one load per
@@ -175,6 +216,36 @@ int ndrr_pack_labels(const int8_t* lab, int vol_layout, int D, int H, int W, voi
  */
 int ndrr_labels_to_i8(const float* src, int8_t* dst, int64_t numel,
                       int n_classes, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Ray and camera set-up
+ * ------------------------------------------------------------------------- */
+
+/*
+ * Detector rays of B cameras (fp32): tgt[b][n] = sdd_b * K^-1_b [u + 1/2,
+ * v + 1/2, 1] with n = v * width + u; src = [B][1][3] zeros (perspective) or
+ * [B][N][3] = tgt - (0, 0, sdd_b) (orthographic); step[b][n] =
+ * |tgt - src| / (n_samples - 1). k_inv is [B or 1][3][3], sdd [B or 1];
+ * a batch stride of 0 shares one across the poses. Replaces _make_tgt,
+ * _make_src and the step of src/nanodrr/drr/renderer.py:82,101-132.
+ */
+int ndrr_make_rays(const float* k_inv, int64_t k_batch_stride, const float* sdd,
+                   int64_t sdd_batch_stride, int B, int height, int width,
+                   int orthographic, int n_samples, float* tgt, float* src, float* step,
+                   void* stream);
+
+/*
+ * M[b] = (world_to_voxel @ rt_inv[b])[:3, :] computed in FP64 and rounded
+ * once to fp32 ([B][3][4]); world_to_voxel is [4][4] float64, rt_inv
+ * [B][4][4] fp32. ndrr_camera_to_voxel_bwd writes the adjoint
+ * g_rt_inv[b] = world_to_voxel[:3, :]^T @ gM[b] ([B][4][4] fp32, FP64 sums).
+ * Replaces the composition in src/nanodrr/drr/backends.py:96 and its
+ * autograd.
+ */
+int ndrr_camera_to_voxel(const double* world_to_voxel, const float* rt_inv, float* M,
+                         int B, void* stream);
+int ndrr_camera_to_voxel_bwd(const double* world_to_voxel, const float* gM,
+                             float* g_rt_inv, int B, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Registration support

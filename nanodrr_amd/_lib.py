@@ -29,6 +29,18 @@ SIGNATURES: dict[str, tuple] = {
          _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
          _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
     ),
+    "ndrr_raymarch_fwd_jac": (
+        _c_int,
+        [_c_p, _c_int, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_int,
+         _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+         _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
+    ),
+    "ndrr_raymarch_bwd_jac": (
+        _c_int,
+        [_c_p, _c_i64, _c_p, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_int,
+         _c_int, _c_int, _c_int, _c_int,
+         _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
+    ),
     "ndrr_bwd_workspace_floats": (_c_size, [_c_int, _c_int, _c_int]),
     "ndrr_launch_tile": (None, [ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
     "ndrr_raymarch_bwd": (
@@ -43,6 +55,10 @@ SIGNATURES: dict[str, tuple] = {
     "ndrr_pack_volume": (_c_int, [_c_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p]),
     "ndrr_packed_label_bytes": (_c_size, [_c_int, _c_int, _c_int, _c_int]),
     "ndrr_pack_labels": (_c_int, [_c_p, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p]),
+    "ndrr_make_rays": (_c_int, [_c_p, _c_i64, _c_p, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p, _c_p,
+                                _c_p]),
+    "ndrr_camera_to_voxel": (_c_int, [_c_p, _c_p, _c_p, _c_int, _c_p]),
+    "ndrr_camera_to_voxel_bwd": (_c_int, [_c_p, _c_p, _c_p, _c_int, _c_p]),
     "ndrr_pose_compose": (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_int, _c_p]),
     "ndrr_pose_compose_bwd": (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_int, _c_p]),
     "ndrr_zncc": (_c_int, [_c_p, _c_p, _c_int, _c_int, ctypes.c_float, _c_p, _c_p, _c_p]),

@@ -13,6 +13,11 @@ differences:
     what the kernels read when given, with `layout` naming it; `vol` still
     fixes the shape, dtype and the layout of the volume gradient;
   * the launch goes through the C ABI of include/nanodrr_amd.h.
+
+`nanodrr_amd::fused_raymarch_jac` / `nanodrr_amd::fused_raymarch_bwd_jac` are
+the single-pass adjoint of the same pair (no counterpart in nanodrr): the
+forward also stores per-ray sums from which the backward forms pose and camera
+gradients without marching the volume again.
 """
 
 from __future__ import annotations
@@ -282,6 +287,226 @@ def _backward(ctx, go):
 
 
 torch.library.register_autograd("nanodrr_amd::fused_raymarch", _backward, setup_context=_setup_context)
+
+
+@torch.library.custom_op("nanodrr_amd::fused_raymarch_jac", mutates_args=())
+def fused_raymarch_jac(
+    vol: torch.Tensor,
+    lab: torch.Tensor,
+    M: torch.Tensor,
+    src: torch.Tensor,
+    tgt: torch.Tensor,
+    step: torch.Tensor,
+    n_samples: int,
+    n_classes: int,
+    width: int,
+    jac_planes: int,
+    packed: Optional[torch.Tensor] = None,
+    layout: str = "linear",
+) -> tuple[torch.Tensor, torch.Tensor]:
+    """`fused_raymarch` plus, from the same march, the per-ray sums of the
+    single-pass adjoint: jac (B, jac_planes, C, N) fp32 (ndrr_raymarch_fwd_jac).
+    Its autograd backward reads jac instead of marching the volume again, so it
+    gives pose and camera gradients but no volume gradient."""
+    if jac_planes not in (6, 7):
+        raise ValueError(f"jac_planes must be 6 or 7, got {jac_planes}")
+    B, N, D, H, W, strides = _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width,
+                                       layout if packed is not None else "linear")
+    vptr, code = _volume_arg(vol, packed, layout)
+    out = torch.empty(B, n_classes, N, device=vol.device, dtype=torch.float32)
+    jac = torch.empty(B, jac_planes, n_classes, N, device=vol.device, dtype=torch.float32)
+    with torch.cuda.device(vol.device):
+        rc = _lib.lib().ndrr_raymarch_fwd_jac(
+            vptr, _DTYPE_CODE[vol.dtype], code, _ptr(lab) if n_classes > 1 else None,
+            M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(), out.data_ptr(), jac.data_ptr(),
+            jac_planes, B, N, D, H, W, n_samples, n_classes, width, *strides, _stream(vol.device),
+        )
+    _lib.check(rc, "ndrr_raymarch_fwd_jac")
+    return out, jac
+
+
+@fused_raymarch_jac.register_fake
+def _(vol, lab, M, src, tgt, step, n_samples, n_classes, width, jac_planes, packed=None, layout="linear"):
+    B, N = M.shape[0], tgt.shape[1]
+    return (torch.empty(B, n_classes, N, device=vol.device, dtype=torch.float32),
+            torch.empty(B, jac_planes, n_classes, N, device=vol.device, dtype=torch.float32))
+
+
+@torch.library.custom_op("nanodrr_amd::fused_raymarch_bwd_jac", mutates_args=())
+def fused_raymarch_bwd_jac(
+    go: torch.Tensor,
+    jac: torch.Tensor,
+    M: torch.Tensor,
+    src: torch.Tensor,
+    tgt: torch.Tensor,
+    step: torch.Tensor,
+    width: int,
+    need_gcam: bool,
+) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Backward of `fused_raymarch_jac` from its jac output (ndrr_raymarch_bwd_jac):
+    (gM (B,3,4), gsrc (B,N,3)|empty, gtgt (B,N,3)|empty, gstep (B,N)|empty), fp32."""
+    B, planes, C, N = jac.shape
+    if M.shape != (B, 3, 4) or tgt.shape[1] != N:
+        raise ValueError("jac does not match the geometry")
+    if need_gcam and planes != 7:
+        raise ValueError("camera gradients need the 7-plane jac (the forward ran without them)")
+    if go.shape != (B, C, N):
+        raise ValueError(f"go must be (B, C, N) = {(B, C, N)}, got {tuple(go.shape)}")
+    go = go.to(torch.float32)
+    # one (C, N) gradient broadcast over the poses is read in place (batch stride 0)
+    shared = B > 1 and go.stride(0) == 0 and go[0].is_contiguous()
+    go_bs = 0 if shared else C * N
+    if not shared:
+        go = go.contiguous()
+    dev = jac.device
+    lib = _lib.lib()
+    strides = (0 if src.shape[0] == 1 else src.shape[1] * 3, 0 if src.shape[1] == 1 else 3,
+               0 if tgt.shape[0] == 1 else N * 3, 0 if step.shape[0] == 1 else N)
+    ws = torch.empty(lib.ndrr_bwd_workspace_floats(B, N, width), device=dev, dtype=torch.float32)
+    gM = torch.empty(B, 3, 4, device=dev, dtype=torch.float32)
+    gsrc = torch.empty(B, N, 3, device=dev, dtype=torch.float32) if need_gcam else gM.new_empty(0)
+    gtgt = torch.empty(B, N, 3, device=dev, dtype=torch.float32) if need_gcam else gM.new_empty(0)
+    gstep = torch.empty(B, N, device=dev, dtype=torch.float32) if need_gcam else gM.new_empty(0)
+    with torch.cuda.device(dev):
+        rc = lib.ndrr_raymarch_bwd_jac(
+            go.data_ptr(), go_bs, jac.data_ptr(), planes, M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(),
+            ws.data_ptr(), gM.data_ptr(), _ptr(gsrc), _ptr(gtgt), _ptr(gstep), int(need_gcam), B, N, C, width,
+            *strides, _stream(dev),
+        )
+    _lib.check(rc, "ndrr_raymarch_bwd_jac")
+    return gM, gsrc, gtgt, gstep
+
+
+@fused_raymarch_bwd_jac.register_fake
+def _(go, jac, M, src, tgt, step, width, need_gcam):
+    B, _, _, N = jac.shape
+    gM = torch.empty(B, 3, 4, device=M.device, dtype=torch.float32)
+    gsrc = torch.empty(B, N, 3, device=M.device, dtype=torch.float32) if need_gcam else gM.new_empty(0)
+    gtgt = torch.empty(B, N, 3, device=M.device, dtype=torch.float32) if need_gcam else gM.new_empty(0)
+    gstep = torch.empty(B, N, device=M.device, dtype=torch.float32) if need_gcam else gM.new_empty(0)
+    return gM, gsrc, gtgt, gstep
+
+
+def _setup_context_jac(ctx, inputs, output):
+    vol, lab, M, src, tgt, step, n_samples, n_classes, width, jac_planes, packed, layout = inputs
+    ctx.save_for_backward(M, src, tgt, step, output[1])
+    ctx.mark_non_differentiable(output[1])
+    ctx.width = width
+
+
+def _backward_jac(ctx, go, g_jac):
+    M, src, tgt, step, jac = ctx.saved_tensors
+    if ctx.needs_input_grad[0]:
+        raise RuntimeError("fused_raymarch_jac gives no volume gradient; use fused_raymarch for it")
+    need_gcam = any(ctx.needs_input_grad[i] for i in (3, 4, 5))
+    gM, gsrc, gtgt, gstep = fused_raymarch_bwd_jac(go, jac, M, src, tgt, step, ctx.width, need_gcam)
+    return (
+        None,  # vol
+        None,  # lab
+        gM.to(M.dtype),
+        _reduce_to(gsrc, src) if need_gcam else None,
+        _reduce_to(gtgt, tgt) if need_gcam else None,
+        _reduce_to(gstep, step) if need_gcam else None,
+        None, None, None, None, None, None,
+    )
+
+
+torch.library.register_autograd("nanodrr_amd::fused_raymarch_jac", _backward_jac, setup_context=_setup_context_jac)
+
+
+@torch.library.custom_op("nanodrr_amd::make_rays", mutates_args=())
+def make_rays(k_inv: torch.Tensor, sdd: torch.Tensor, height: int, width: int, orthographic: bool,
+              n_samples: int) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """(tgt (B,N,3), src (B,1,3) or (B,N,3), step (B,N)) fp32 of `render`'s rays in one
+    launch (ndrr_make_rays); B = max of the batches of k_inv (B|1,3,3) and sdd (B|1,)."""
+    if k_inv.dtype != torch.float32 or sdd.dtype != torch.float32 or not k_inv.is_cuda or sdd.device != k_inv.device:
+        raise ValueError("make_rays needs float32 device tensors k_inv and sdd")
+    k = k_inv.contiguous()
+    d = sdd.contiguous()
+    if k.dim() != 3 or k.shape[1:] != (3, 3) or d.dim() != 1:
+        raise ValueError("k_inv must be (B, 3, 3) and sdd (B,)")
+    B = max(k.shape[0], d.shape[0])
+    if k.shape[0] not in (1, B) or d.shape[0] not in (1, B):
+        raise ValueError("k_inv and sdd batches must match or be 1")
+    N = height * width
+    dev = k.device
+    tgt = torch.empty(B, N, 3, device=dev, dtype=torch.float32)
+    src = torch.empty(B, N if orthographic else 1, 3, device=dev, dtype=torch.float32)
+    step = torch.empty(B, N, device=dev, dtype=torch.float32)
+    with torch.cuda.device(dev):
+        rc = _lib.lib().ndrr_make_rays(k.data_ptr(), 0 if k.shape[0] == 1 else 9, d.data_ptr(),
+                                       0 if d.shape[0] == 1 else 1, B, height, width, int(orthographic), n_samples,
+                                       tgt.data_ptr(), src.data_ptr(), step.data_ptr(), _stream(dev))
+    _lib.check(rc, "ndrr_make_rays")
+    return tgt, src, step
+
+
+@make_rays.register_fake
+def _(k_inv, sdd, height, width, orthographic, n_samples):
+    B, N = max(k_inv.shape[0], sdd.shape[0]), height * width
+    return (torch.empty(B, N, 3, device=k_inv.device, dtype=torch.float32),
+            torch.empty(B, N if orthographic else 1, 3, device=k_inv.device, dtype=torch.float32),
+            torch.empty(B, N, device=k_inv.device, dtype=torch.float32))
+
+
+def _c2v_check(world_to_voxel, t, name):
+    if world_to_voxel.shape != (4, 4) or world_to_voxel.dtype != torch.float64:
+        raise ValueError("world_to_voxel must be a (4, 4) float64 tensor")
+    if t.dtype != torch.float32 or not t.is_contiguous() or t.device != world_to_voxel.device or not t.is_cuda:
+        raise ValueError(f"{name} must be a contiguous float32 device tensor on the device of world_to_voxel")
+
+
+@torch.library.custom_op("nanodrr_amd::camera_to_voxel", mutates_args=())
+def camera_to_voxel(world_to_voxel: torch.Tensor, rt_inv: torch.Tensor) -> torch.Tensor:
+    """M = (world_to_voxel @ rt_inv)[:, :3] (B,3,4) fp32, in FP64 with one rounding (ndrr_camera_to_voxel)."""
+    world_to_voxel = world_to_voxel.contiguous()
+    _c2v_check(world_to_voxel, rt_inv, "rt_inv")
+    if rt_inv.dim() != 3 or rt_inv.shape[1:] != (4, 4):
+        raise ValueError("rt_inv must be (B, 4, 4)")
+    B = rt_inv.shape[0]
+    M = torch.empty(B, 3, 4, device=rt_inv.device, dtype=torch.float32)
+    with torch.cuda.device(rt_inv.device):
+        rc = _lib.lib().ndrr_camera_to_voxel(world_to_voxel.data_ptr(), rt_inv.data_ptr(), M.data_ptr(), B,
+                                             _stream(rt_inv.device))
+    _lib.check(rc, "ndrr_camera_to_voxel")
+    return M
+
+
+@camera_to_voxel.register_fake
+def _(world_to_voxel, rt_inv):
+    return torch.empty(rt_inv.shape[0], 3, 4, device=rt_inv.device, dtype=torch.float32)
+
+
+@torch.library.custom_op("nanodrr_amd::camera_to_voxel_bwd", mutates_args=())
+def camera_to_voxel_bwd(world_to_voxel: torch.Tensor, gM: torch.Tensor) -> torch.Tensor:
+    """Adjoint of camera_to_voxel: world_to_voxel[:3]^T @ gM (B,4,4) fp32, FP64 sums."""
+    gM = gM.to(torch.float32).contiguous()
+    world_to_voxel = world_to_voxel.contiguous()
+    _c2v_check(world_to_voxel, gM, "gM")
+    B = gM.shape[0]
+    g = torch.empty(B, 4, 4, device=gM.device, dtype=torch.float32)
+    with torch.cuda.device(gM.device):
+        rc = _lib.lib().ndrr_camera_to_voxel_bwd(world_to_voxel.data_ptr(), gM.data_ptr(), g.data_ptr(), B,
+                                                 _stream(gM.device))
+    _lib.check(rc, "ndrr_camera_to_voxel_bwd")
+    return g
+
+
+@camera_to_voxel_bwd.register_fake
+def _(world_to_voxel, gM):
+    return torch.empty(gM.shape[0], 4, 4, device=gM.device, dtype=torch.float32)
+
+
+def _c2v_setup(ctx, inputs, output):
+    ctx.save_for_backward(inputs[0])
+
+
+def _c2v_backward(ctx, gM):
+    (w2v,) = ctx.saved_tensors
+    return None, camera_to_voxel_bwd(w2v, gM)
+
+
+torch.library.register_autograd("nanodrr_amd::camera_to_voxel", _c2v_backward, setup_context=_c2v_setup)
 
 
 @torch.library.custom_op("nanodrr_amd::labels_to_i8", mutates_args=())

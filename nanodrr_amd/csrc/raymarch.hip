@@ -29,8 +29,30 @@ constexpr int kWave = 64;        // rays per wave: an 8 x 8 detector patch (or 6
 #ifndef NDRR_TILE_WAVES_Y
 #define NDRR_TILE_WAVES_Y 2
 #endif
-#ifndef NDRR_CHUNKS
-#define NDRR_CHUNKS 1
+// Chunks: wave groups per workgroup marching the same rays; group c takes the
+// c-th equal part of every ray's clipped sample range. Per kernel and layout.
+#ifndef NDRR_FWD_CHUNKS_LINEAR
+#define NDRR_FWD_CHUNKS_LINEAR 1
+#endif
+#ifndef NDRR_FWD_CHUNKS_STENCIL
+#define NDRR_FWD_CHUNKS_STENCIL 1
+#endif
+#ifndef NDRR_FWD_CHUNKS_QUAD
+#define NDRR_FWD_CHUNKS_QUAD 1
+#endif
+// Pose-gradient partials of the backward kernel: 1 = one per tile (the waves
+// of a workgroup added through LDS after a barrier), 0 = one per wave.
+#ifndef NDRR_BWD_TILE_PARTIALS
+#define NDRR_BWD_TILE_PARTIALS 1
+#endif
+#ifndef NDRR_BWD_CHUNKS_LINEAR
+#define NDRR_BWD_CHUNKS_LINEAR 1
+#endif
+#ifndef NDRR_BWD_CHUNKS_STENCIL
+#define NDRR_BWD_CHUNKS_STENCIL 1
+#endif
+#ifndef NDRR_BWD_CHUNKS_QUAD
+#define NDRR_BWD_CHUNKS_QUAD 1
 #endif
 #ifndef NDRR_IN_FLIGHT
 #define NDRR_IN_FLIGHT 4
@@ -44,8 +66,26 @@ constexpr int kWave = 64;        // rays per wave: an 8 x 8 detector patch (or 6
 constexpr int kTwx = NDRR_TILE_WAVES_X, kTwy = NDRR_TILE_WAVES_Y;  // 8 x 8 wave patches per tile
 constexpr int kTileW = 8 * kTwx, kTileH = 8 * kTwy;                // tile size in detector pixels
 constexpr int kTileRays = kWave * kTwx * kTwy;
-constexpr int kChunks = NDRR_CHUNKS;  // wave groups per workgroup; group c marches chunk c of every ray's samples
-constexpr int kBlock = kTileRays * kChunks;
+constexpr int fwd_chunks(int layout) {
+  return layout == NDRR_LAYOUT_STENCIL ? NDRR_FWD_CHUNKS_STENCIL
+                                       : (layout == NDRR_LAYOUT_QUAD ? NDRR_FWD_CHUNKS_QUAD : NDRR_FWD_CHUNKS_LINEAR);
+}
+constexpr int bwd_chunks(int layout) {
+  return layout == NDRR_LAYOUT_STENCIL ? NDRR_BWD_CHUNKS_STENCIL
+                                       : (layout == NDRR_LAYOUT_QUAD ? NDRR_BWD_CHUNKS_QUAD : NDRR_BWD_CHUNKS_LINEAR);
+}
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int kMaxFwdChunks = cmax(cmax(fwd_chunks(NDRR_LAYOUT_LINEAR), fwd_chunks(NDRR_LAYOUT_STENCIL)),
+                                   fwd_chunks(NDRR_LAYOUT_QUAD));
+constexpr int kMaxBwdChunks = cmax(cmax(bwd_chunks(NDRR_LAYOUT_LINEAR), bwd_chunks(NDRR_LAYOUT_STENCIL)),
+                                   bwd_chunks(NDRR_LAYOUT_QUAD));
+// Pose-gradient partials per tile of the backward kernel (one, or one per
+// wave of the layout with the most chunks): the workspace is sized for it.
+constexpr int bwd_parts_per_tile(int layout) {
+  return NDRR_BWD_TILE_PARTIALS ? 1 : (kTileRays / kWave) * bwd_chunks(layout);
+}
+constexpr int kBwdPartsMax = NDRR_BWD_TILE_PARTIALS ? 1 : (kTileRays / kWave) * kMaxBwdChunks;
+static_assert(kTileRays * cmax(kMaxFwdChunks, kMaxBwdChunks) <= 1024, "workgroups are limited to 1024 threads");
 constexpr int kThreads = 256;    // threads per block of the element-wise helper kernels
 constexpr float kMargin = 0.5f;  // voxels of slack around the footprint box
 constexpr int kKahanMinSamples = 512;
@@ -78,12 +118,15 @@ struct Params {
   const float* step;
   const float* go;
   float* out;
-  float* gpart;  // [B][nblocks][12] pose-gradient partials
+  float* gpart;  // [B][nblocks][parts per tile][12] pose-gradient partials
+  float* jac;    // [B][jac_q][C][N] per-ray sums of the single-pass adjoint
+  int jac_q;     // planes of jac: 6 (G, T) or 7 (G, T, V)
   float* gsrc;
   float* gtgt;
   float* gstep;
   float* gvol;
   int64_t src_bs, src_rs, tgt_bs, step_bs;
+  int64_t go_bs;  // batch stride of go in the single-pass backward (0: shared by every pose)
   uint32_t vol_bytes, lab_bytes;  // sizes of the packed volume / label buffers
   int B, N, D, H, W, S, C;
   int IW, IH, tiles_w, nblocks;
@@ -93,7 +136,7 @@ struct Params {
 };
 
 // Ray of thread `tr` (0 .. kTileRays-1) in tile `blk` and whether it is a real
-// detector pixel. Each wave holds an 8 x 8 patch of the tile; with kChunks > 1
+// detector pixel. Each wave holds an 8 x 8 patch of the tile; with chunks > 1
 // every chunk group holds the same rays (one chunk of samples each).
 __device__ inline int ray_index(const Params& p, int blk, int tr, bool& valid) {
   if (p.IW > 0) {
@@ -197,9 +240,10 @@ __device__ inline DF df_mul_df(DF a, DF b) {
   return fast_two_sum(p.hi, fmaf(a.hi, b.lo, fmaf(a.lo, b.hi, p.lo)));
 }
 
-// Per-ray set-up for the samples of chunk `chunk` of `kChunks` equal parts of
+// Per-ray set-up for the samples of chunk `chunk` of `nch` equal parts of
 // the clipped sample range: the chunk's first sample (anchor) and its count.
-__device__ inline Ray setup_ray(const Params& p, const float* __restrict__ Mb, int b, int n, bool valid, int chunk) {
+__device__ inline Ray setup_ray(const Params& p, const float* __restrict__ Mb, int b, int n, bool valid, int chunk,
+                                int nch) {
   Ray r;
   const int nn = valid ? n : 0;
   const float* sp = p.src + b * p.src_bs + nn * p.src_rs;
@@ -247,7 +291,7 @@ __device__ inline Ray setup_ray(const Params& p, const float* __restrict__ Mb, i
       n_all = static_cast<int>(f_hi) - s_lo;
     }
   }
-  const int per = (n_all + kChunks - 1) / kChunks;
+  const int per = (n_all + nch - 1) / nch;
   r.s_lo = s_lo + chunk * per;
   r.n_s = max(0, min(per, n_all - chunk * per));
   const float s0 = static_cast<float>(r.s_lo);
@@ -462,14 +506,31 @@ template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_QUAD> {  // fp32: two 16-byte rec
   }
 };
 
+// a (1 - f) + b f with the rounding fixed by an explicit fma (b f + a (1 - f)),
+// so every kernel that inlines it rounds the same way whatever else the
+// compiler contracts around it.
+__device__ inline float lerp1(float a, float b, float f) { return fmaf(b, f, a * (1.0f - f)); }
+
 __device__ inline float lerp_value(const Corners& v, const Cell& c) {
-  const float c00 = v.v000 * (1.0f - c.fx) + v.v001 * c.fx;
-  const float c01 = v.v010 * (1.0f - c.fx) + v.v011 * c.fx;
-  const float c10 = v.v100 * (1.0f - c.fx) + v.v101 * c.fx;
-  const float c11 = v.v110 * (1.0f - c.fx) + v.v111 * c.fx;
-  const float c0 = c00 * (1.0f - c.fy) + c01 * c.fy;
-  const float c1 = c10 * (1.0f - c.fy) + c11 * c.fy;
-  return c0 * (1.0f - c.fz) + c1 * c.fz;
+  const float c00 = lerp1(v.v000, v.v001, c.fx);
+  const float c01 = lerp1(v.v010, v.v011, c.fx);
+  const float c10 = lerp1(v.v100, v.v101, c.fx);
+  const float c11 = lerp1(v.v110, v.v111, c.fx);
+  const float c0 = lerp1(c00, c01, c.fy);
+  const float c1 = lerp1(c10, c11, c.fy);
+  return lerp1(c0, c1, c.fz);
+}
+
+// Gradient of the trilinear interpolant with respect to the sample position
+// (voxel units), at fractions (fx, fy, fz) of the cell.
+__device__ inline void lerp_grad(const Corners& v, const Cell& c, float& dvx, float& dvy, float& dvz) {
+  const float fx = c.fx, fy = c.fy, fz = c.fz;
+  dvx = ((v.v001 - v.v000) * (1.0f - fy) + (v.v011 - v.v010) * fy) * (1.0f - fz) +
+        ((v.v101 - v.v100) * (1.0f - fy) + (v.v111 - v.v110) * fy) * fz;
+  dvy = ((v.v010 - v.v000) * (1.0f - fx) + (v.v011 - v.v001) * fx) * (1.0f - fz) +
+        ((v.v110 - v.v100) * (1.0f - fx) + (v.v111 - v.v101) * fx) * fz;
+  dvz = ((v.v100 - v.v000) * (1.0f - fx) + (v.v101 - v.v001) * fx) * (1.0f - fy) +
+        ((v.v110 - v.v010) * (1.0f - fx) + (v.v111 - v.v011) * fx) * fy;
 }
 
 // Nearest label (round half to even, zero padded: outside the volume is class 0).
@@ -539,27 +600,36 @@ struct Batch {
 // ---------------------------------------------------------------------------
 // Forward
 // ---------------------------------------------------------------------------
-// A workgroup of kChunks waves marches one tile of 64 rays; wave c covers
-// chunk c of every ray's clipped sample range, so a launch holds kChunks x as
-// many short waves and the tail of the grid is short. The chunks' partial
-// sums are added in chunk order through LDS (deterministic).
+// A workgroup of NCH wave groups marches one tile of rays; group c covers
+// chunk c of every ray's clipped sample range, so a launch holds NCH x as
+// many waves, each with a shorter life. The chunks' partial sums are added in
+// chunk order through LDS (deterministic).
 // CV == 1: single channel (no label map). CV in {2, 4, 8}: label map, runtime C <= CV.
-template <int DT, int L, int CV, bool KAHAN>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NDRR_WAVES_PER_EU)))
+// JAC (with NCH == 1): the single-pass adjoint's per-ray, per-class sums are
+// accumulated in the same march and written to p.jac: G_c = sum grad v,
+// T_c = sum t grad v over the samples routed to class c (voxel units), and
+// with jac_q == 7 also V_c = sum v (the image before the step factor).
+template <int DT, int L, int CV, bool KAHAN, int NCH, bool JAC>
+__global__ __launch_bounds__(kTileRays * NCH) __attribute__((amdgpu_waves_per_eu(NDRR_WAVES_PER_EU)))
 void k_fwd(Params p) {
+  static_assert(!JAC || NCH == 1, "the single-pass adjoint sums are written by one wave group");
   int b, tile;
   block_coords(b, tile);
   const int tr = threadIdx.x % kTileRays, chunk = threadIdx.x / kTileRays;
   bool valid;
   const int n = ray_index(p, tile, tr, valid);
   const float* __restrict__ Mb = p.M + 12 * b;
-  const Ray r = setup_ray(p, Mb, b, n, valid, chunk);
+  const Ray r = setup_ray(p, Mb, b, n, valid, chunk, NCH);
   const Src src = make_src(p);
 
   float acc[CV];
   float comp[CV];
 #pragma unroll
   for (int c = 0; c < CV; ++c) { acc[c] = 0.0f; comp[c] = 0.0f; }
+  constexpr int kJ = JAC ? 6 * CV : 1;
+  float jg[kJ];  // per class: G x, y, z, T x, y, z
+#pragma unroll
+  for (int q = 0; q < kJ; ++q) jg[q] = 0.0f;
 
   using B = Batch<DT, L, (CV > 1)>;
   for (int k0 = 0; k0 < r.n_s; k0 += B::U) {
@@ -568,7 +638,9 @@ void k_fwd(Params p) {
 #pragma unroll
     for (int u = 0; u < B::U; ++u) {
       const Cell& c = batch.cell[u];
-      const float val = (k0 + u < r.n_s) ? lerp_value(batch.f[u].corners(), c) : 0.0f;
+      const bool live = k0 + u < r.n_s;
+      const Corners v = batch.f[u].corners();
+      const float val = live ? lerp_value(v, c) : 0.0f;
       if constexpr (CV == 1) {
         if constexpr (KAHAN) comp_add(acc[0], comp[0], val); else acc[0] += val;
       } else {
@@ -578,11 +650,32 @@ void k_fwd(Params p) {
           if constexpr (KAHAN) comp_add(acc[cl], comp[cl], contrib); else acc[cl] += contrib;
         }
       }
+      if constexpr (JAC) {
+        float dvx, dvy, dvz;
+        lerp_grad(v, c, dvx, dvy, dvz);
+        dvx = live ? dvx : 0.0f;
+        dvy = live ? dvy : 0.0f;
+        dvz = live ? dvz : 0.0f;
+        const float t = static_cast<float>(r.s_lo + k0 + u) * p.inv_s;
+        const float tx = dvx * t, ty = dvy * t, tz = dvz * t;
+        // Class routing by a 0/1 factor: fma(m, x, acc) is acc + x or acc
+        // (x finite), the same values as a select, in one instruction.
+#pragma unroll
+        for (int cl = 0; cl < CV; ++cl) {
+          const float m = (CV == 1 || batch.cls[u] == cl) ? 1.0f : 0.0f;
+          jg[6 * cl + 0] = fmaf(m, dvx, jg[6 * cl + 0]);
+          jg[6 * cl + 1] = fmaf(m, dvy, jg[6 * cl + 1]);
+          jg[6 * cl + 2] = fmaf(m, dvz, jg[6 * cl + 2]);
+          jg[6 * cl + 3] = fmaf(m, tx, jg[6 * cl + 3]);
+          jg[6 * cl + 4] = fmaf(m, ty, jg[6 * cl + 4]);
+          jg[6 * cl + 5] = fmaf(m, tz, jg[6 * cl + 5]);
+        }
+      }
     }
   }
 
-  __shared__ float part[kChunks > 1 ? kChunks - 1 : 1][CV][kTileRays];
-  if constexpr (kChunks > 1) {
+  __shared__ float part[NCH > 1 ? NCH - 1 : 1][CV][kTileRays];
+  if constexpr (NCH > 1) {
     if (chunk > 0) {
 #pragma unroll
       for (int c = 0; c < CV; ++c) part[chunk - 1][c][tr] = KAHAN ? (acc[c] + comp[c]) : acc[c];
@@ -596,10 +689,18 @@ void k_fwd(Params p) {
     if (c < p.C) {
       float total = acc[c], tcomp = comp[c];
 #pragma unroll
-      for (int k = 1; k < kChunks; ++k) {
+      for (int k = 1; k < NCH; ++k) {
         if constexpr (KAHAN) comp_add(total, tcomp, part[k - 1][c][tr]); else total += part[k - 1][c][tr];
       }
-      o[static_cast<int64_t>(c) * p.N] = (KAHAN ? (total + tcomp) : total) * r.step;
+      const float sum = KAHAN ? (total + tcomp) : total;
+      o[static_cast<int64_t>(c) * p.N] = sum * r.step;
+      if constexpr (JAC) {
+        float* jp = p.jac + (static_cast<int64_t>(b) * p.jac_q * p.C + c) * p.N + n;
+        const int64_t plane = static_cast<int64_t>(p.C) * p.N;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) jp[q * plane] = jg[6 * c + q];
+        if (p.jac_q == 7) jp[6 * plane] = sum;
+      }
     }
   }
 }
@@ -617,8 +718,8 @@ __device__ inline float wave_sum(float v) {
   return v;
 }
 
-template <int DT, int L, int CV, bool GCAM, bool GVOL>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NDRR_WAVES_PER_EU)))
+template <int DT, int L, int CV, bool GCAM, bool GVOL, int NCH>
+__global__ __launch_bounds__(kTileRays * NCH) __attribute__((amdgpu_waves_per_eu(NDRR_WAVES_PER_EU)))
 void k_bwd(Params p) {
   int b, tile;
   block_coords(b, tile);
@@ -626,7 +727,7 @@ void k_bwd(Params p) {
   bool valid;
   const int n = ray_index(p, tile, tr, valid);
   const float* __restrict__ Mb = p.M + 12 * b;
-  const Ray r = setup_ray(p, Mb, b, n, valid, chunk);
+  const Ray r = setup_ray(p, Mb, b, n, valid, chunk, NCH);
   const Src src = make_src(p);
 
   // Upstream gradient of this ray, one value per class.
@@ -660,12 +761,8 @@ void k_bwd(Params p) {
       gg = live ? gg : 0.0f;
       const float g = gg * r.step;
       const float fx = c.fx, fy = c.fy, fz = c.fz;
-      const float dvx = ((v.v001 - v.v000) * (1.0f - fy) + (v.v011 - v.v010) * fy) * (1.0f - fz) +
-                        ((v.v101 - v.v100) * (1.0f - fy) + (v.v111 - v.v110) * fy) * fz;
-      const float dvy = ((v.v010 - v.v000) * (1.0f - fx) + (v.v011 - v.v001) * fx) * (1.0f - fz) +
-                        ((v.v110 - v.v100) * (1.0f - fx) + (v.v111 - v.v101) * fx) * fz;
-      const float dvz = ((v.v100 - v.v000) * (1.0f - fx) + (v.v101 - v.v001) * fx) * (1.0f - fy) +
-                        ((v.v110 - v.v010) * (1.0f - fx) + (v.v111 - v.v011) * fx) * fy;
+      float dvx, dvy, dvz;
+      lerp_grad(v, c, dvx, dvy, dvz);
       const float gx = g * dvx, gy = g * dvy, gz = g * dvz;
       const float t = static_cast<float>(r.s_lo + k0 + u) * p.inv_s;
       Gx += gx; Gy += gy; Gz += gz;
@@ -692,8 +789,8 @@ void k_bwd(Params p) {
 
   if constexpr (GCAM) {
     // Per-ray totals over the chunks, in chunk order.
-    __shared__ float part[kChunks > 1 ? kChunks - 1 : 1][7][kTileRays];
-    if constexpr (kChunks > 1) {
+    __shared__ float part[NCH > 1 ? NCH - 1 : 1][7][kTileRays];
+    if constexpr (NCH > 1) {
       if (chunk > 0) {
         part[chunk - 1][0][tr] = Gx; part[chunk - 1][1][tr] = Gy; part[chunk - 1][2][tr] = Gz;
         part[chunk - 1][3][tr] = Tx; part[chunk - 1][4][tr] = Ty; part[chunk - 1][5][tr] = Tz;
@@ -704,7 +801,7 @@ void k_bwd(Params p) {
     if (chunk == 0 && valid) {
       float sGx = Gx, sGy = Gy, sGz = Gz, sTx = Tx, sTy = Ty, sTz = Tz, sA = astep;
 #pragma unroll
-      for (int k = 1; k < kChunks; ++k) {
+      for (int k = 1; k < NCH; ++k) {
         sGx += part[k - 1][0][tr]; sGy += part[k - 1][1][tr]; sGz += part[k - 1][2][tr];
         sTx += part[k - 1][3][tr]; sTy += part[k - 1][4][tr]; sTz += part[k - 1][5][tr];
         sA += part[k - 1][6][tr];
@@ -725,14 +822,97 @@ void k_bwd(Params p) {
     }
   }
 
-  // Deterministic per-tile pose-gradient partials: butterfly within each
-  // wave, then the wave sums in wave order.
-  constexpr int kWaves = kBlock / kWave;
+  // Deterministic pose-gradient partials: a butterfly within each wave, then
+  // either the wave sums of the tile in wave order (one partial per tile) or
+  // one partial per wave, written with no workgroup barrier; k_reduce_pose
+  // adds the partials of a pose in a fixed order.
+  constexpr int kWaves = kTileRays * NCH / kWave;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  __shared__ float red[kWaves][12];
   const float a[12] = {r.sx * Gx + r.ex * Tx, r.sy * Gx + r.ey * Tx, r.sz * Gx + r.ez * Tx, Gx,
                        r.sx * Gy + r.ex * Ty, r.sy * Gy + r.ey * Ty, r.sz * Gy + r.ez * Ty, Gy,
                        r.sx * Gz + r.ex * Tz, r.sy * Gz + r.ey * Tz, r.sz * Gz + r.ez * Tz, Gz};
+  if constexpr (NDRR_BWD_TILE_PARTIALS) {
+    __shared__ float red[kWaves][12];
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+      const float v = wave_sum(a[j]);
+      if (lane == 0) red[wave][j] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+      const int j = threadIdx.x;
+      float v = red[0][j];
+#pragma unroll
+      for (int w = 1; w < kWaves; ++w) v += red[w][j];
+      p.gpart[(static_cast<int64_t>(b) * p.nblocks + tile) * 12 + j] = v;
+    }
+  } else {
+    float* const dst = p.gpart + ((static_cast<int64_t>(b) * p.nblocks + tile) * kWaves + wave) * 12;
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+      const float v = wave_sum(a[j]);
+      if (lane == j) dst[j] = v;
+    }
+  }
+}
+
+// Single-pass adjoint: the backward as a per-ray contraction of the sums the
+// forward (k_fwd<..., JAC>) stored. With g_c = go[b][c][n],
+//   G = step sum_c g_c G_c,  T = step sum_c g_c T_c,  dL/dstep = sum_c g_c V_c,
+// which are the per-ray sums k_bwd forms sample by sample (the per-sample
+// factor go * step moved out of the sums); the pose partials, gsrc and gtgt
+// follow from G and T exactly as in k_bwd.
+template <int CV, bool GCAM>
+__global__ __launch_bounds__(kTileRays) void k_bwd_jac(Params p) {
+  int b, tile;
+  block_coords(b, tile);
+  const int tr = threadIdx.x;
+  bool valid;
+  const int n = ray_index(p, tile, tr, valid);
+  const int nn = valid ? n : 0;
+  const float* sp = p.src + b * p.src_bs + nn * p.src_rs;
+  const float* tp = p.tgt + b * p.tgt_bs + 3 * static_cast<int64_t>(nn);
+  const float sx = sp[0], sy = sp[1], sz = sp[2];
+  const float ex = tp[0] - sx, ey = tp[1] - sy, ez = tp[2] - sz;
+  const float step = p.step[b * p.step_bs + nn];
+  const int64_t plane = static_cast<int64_t>(p.C) * p.N;
+  float Gx = 0.f, Gy = 0.f, Gz = 0.f, Tx = 0.f, Ty = 0.f, Tz = 0.f, astep = 0.f;
+#pragma unroll
+  for (int c = 0; c < CV; ++c) {
+    if (c < p.C) {
+      const float g = valid ? p.go[b * p.go_bs + static_cast<int64_t>(c) * p.N + nn] : 0.0f;
+      const float* jp = p.jac + (static_cast<int64_t>(b) * p.jac_q * p.C + c) * p.N + nn;
+      Gx += g * jp[0]; Gy += g * jp[plane]; Gz += g * jp[2 * plane];
+      Tx += g * jp[3 * plane]; Ty += g * jp[4 * plane]; Tz += g * jp[5 * plane];
+      if constexpr (GCAM) astep += g * jp[6 * plane];
+    }
+  }
+  Gx *= step; Gy *= step; Gz *= step;
+  Tx *= step; Ty *= step; Tz *= step;
+  const float* __restrict__ Mb = p.M + 12 * b;
+  if constexpr (GCAM) {
+    if (valid) {
+      const float m00 = Mb[0], m01 = Mb[1], m02 = Mb[2];
+      const float m10 = Mb[4], m11 = Mb[5], m12 = Mb[6];
+      const float m20 = Mb[8], m21 = Mb[9], m22 = Mb[10];
+      const float qGx = m00 * Gx + m10 * Gy + m20 * Gz;  // M33^T G
+      const float qGy = m01 * Gx + m11 * Gy + m21 * Gz;
+      const float qGz = m02 * Gx + m12 * Gy + m22 * Gz;
+      const float qTx = m00 * Tx + m10 * Ty + m20 * Tz;  // M33^T T
+      const float qTy = m01 * Tx + m11 * Ty + m21 * Tz;
+      const float qTz = m02 * Tx + m12 * Ty + m22 * Tz;
+      const int64_t ray = static_cast<int64_t>(b) * p.N + n;
+      p.gsrc[3 * ray + 0] = qGx - qTx; p.gsrc[3 * ray + 1] = qGy - qTy; p.gsrc[3 * ray + 2] = qGz - qTz;
+      p.gtgt[3 * ray + 0] = qTx; p.gtgt[3 * ray + 1] = qTy; p.gtgt[3 * ray + 2] = qTz;
+      p.gstep[ray] = astep;
+    }
+  }
+  constexpr int kWaves = kTileRays / kWave;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const float a[12] = {sx * Gx + ex * Tx, sy * Gx + ey * Tx, sz * Gx + ez * Tx, Gx,
+                       sx * Gy + ex * Ty, sy * Gy + ey * Ty, sz * Gy + ez * Ty, Gy,
+                       sx * Gz + ex * Tz, sy * Gz + ey * Tz, sz * Gz + ez * Tz, Gz};
+  __shared__ float red[kWaves][12];
 #pragma unroll
   for (int j = 0; j < 12; ++j) {
     const float v = wave_sum(a[j]);
@@ -748,27 +928,36 @@ void k_bwd(Params p) {
   }
 }
 
-// gM[b][j] = sum over tiles of gpart[b][tile][j] in a fixed order: thread
-// (j, k) of the 192-thread block sums tiles k, k+16, ... (compensated), then
-// thread j adds the 16 partial sums in k order. Deterministic.
-constexpr int kReduceLanes = 16;
-__global__ __launch_bounds__(12 * kReduceLanes) void k_reduce_pose(const float* __restrict__ part,
-                                                                    float* __restrict__ gM, int nblocks) {
-  __shared__ float red[12][kReduceLanes];
-  const int b = blockIdx.x;
-  const int j = threadIdx.x % 12;
-  const int k = threadIdx.x / 12;
-  const float* pp = part + static_cast<int64_t>(b) * nblocks * 12 + j;
-  float sum = 0.0f, comp = 0.0f;
-  for (int t = k; t < nblocks; t += kReduceLanes) comp_add(sum, comp, pp[static_cast<int64_t>(t) * 12]);
-  red[j][k] = sum + comp;
-  __syncthreads();
-  if (threadIdx.x < 12) {
-    float s2 = 0.0f, c2 = 0.0f;
+// gM[b][j] = sum of the nparts partials gpart[b][part][j] (per tile, or per
+// wave of every tile) in a fixed order: thread t of the block sums parts t, t + 256,
+// ... (compensated, loads independent of one another), then a pairwise tree
+// over the 256 thread sums in LDS. Deterministic.
+constexpr int kReduceThreads = 256;
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_pose(const float* __restrict__ part,
+                                                               float* __restrict__ gM, int nparts) {
+  __shared__ float red[12][kReduceThreads];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const float4* pb = reinterpret_cast<const float4*>(part + static_cast<int64_t>(b) * nparts * 12);
+  float sum[12], comp[12];
 #pragma unroll
-    for (int t = 0; t < kReduceLanes; ++t) comp_add(s2, c2, red[threadIdx.x][t]);
-    gM[b * 12 + threadIdx.x] = s2 + c2;
+  for (int j = 0; j < 12; ++j) { sum[j] = 0.0f; comp[j] = 0.0f; }
+  for (int q = t; q < nparts; q += kReduceThreads) {
+    const float4 a = pb[3 * q], c = pb[3 * q + 1], d = pb[3 * q + 2];
+    const float v[12] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w};
+#pragma unroll
+    for (int j = 0; j < 12; ++j) comp_add(sum[j], comp[j], v[j]);
   }
+#pragma unroll
+  for (int j = 0; j < 12; ++j) red[j][t] = sum[j] + comp[j];
+  __syncthreads();
+  for (int w = kReduceThreads / 2; w > 0; w >>= 1) {
+    if (t < w) {
+#pragma unroll
+      for (int j = 0; j < 12; ++j) red[j][t] += red[j][t + w];
+    }
+    __syncthreads();
+  }
+  if (t < 12) gM[b * 12 + t] = red[t][0];
 }
 
 __global__ void k_labels_to_i8(const float* __restrict__ src, int8_t* __restrict__ dst, int64_t numel, int C) {
@@ -951,10 +1140,20 @@ int launch_status() {
 
 template <int DT, int L, int CV>
 void launch_fwd3(const Params& p, dim3 grid, hipStream_t st) {
+  if (p.jac) {
+    const dim3 block(kTileRays);
+    if (p.S >= kKahanMinSamples)
+      hipLaunchKernelGGL((k_fwd<DT, L, CV, true, 1, true>), grid, block, 0, st, p);
+    else
+      hipLaunchKernelGGL((k_fwd<DT, L, CV, false, 1, true>), grid, block, 0, st, p);
+    return;
+  }
+  constexpr int nch = fwd_chunks(L);
+  const dim3 block(kTileRays * nch);
   if (p.S >= kKahanMinSamples)
-    hipLaunchKernelGGL((k_fwd<DT, L, CV, true>), grid, dim3(kBlock), 0, st, p);
+    hipLaunchKernelGGL((k_fwd<DT, L, CV, true, nch, false>), grid, block, 0, st, p);
   else
-    hipLaunchKernelGGL((k_fwd<DT, L, CV, false>), grid, dim3(kBlock), 0, st, p);
+    hipLaunchKernelGGL((k_fwd<DT, L, CV, false, nch, false>), grid, block, 0, st, p);
 }
 
 template <int DT, int L>
@@ -967,10 +1166,12 @@ void launch_fwd2(const Params& p, dim3 grid, hipStream_t st) {
 
 template <int DT, int L, int CV>
 void launch_bwd3(const Params& p, bool gcam, bool gvol, dim3 grid, hipStream_t st) {
-  if (gcam && gvol) hipLaunchKernelGGL((k_bwd<DT, L, CV, true, true>), grid, dim3(kBlock), 0, st, p);
-  else if (gcam) hipLaunchKernelGGL((k_bwd<DT, L, CV, true, false>), grid, dim3(kBlock), 0, st, p);
-  else if (gvol) hipLaunchKernelGGL((k_bwd<DT, L, CV, false, true>), grid, dim3(kBlock), 0, st, p);
-  else hipLaunchKernelGGL((k_bwd<DT, L, CV, false, false>), grid, dim3(kBlock), 0, st, p);
+  constexpr int nch = bwd_chunks(L);
+  const dim3 block(kTileRays * nch);
+  if (gcam && gvol) hipLaunchKernelGGL((k_bwd<DT, L, CV, true, true, nch>), grid, block, 0, st, p);
+  else if (gcam) hipLaunchKernelGGL((k_bwd<DT, L, CV, true, false, nch>), grid, block, 0, st, p);
+  else if (gvol) hipLaunchKernelGGL((k_bwd<DT, L, CV, false, true, nch>), grid, block, 0, st, p);
+  else hipLaunchKernelGGL((k_bwd<DT, L, CV, false, false, nch>), grid, block, 0, st, p);
 }
 
 template <int DT, int L>
@@ -1019,15 +1220,29 @@ int ndrr_raymarch_fwd(const void* vol, int vol_dtype, int vol_layout, const int8
                       int H, int W, int n_samples, int n_classes, int width, int64_t src_batch_stride,
                       int64_t src_ray_stride, int64_t tgt_batch_stride, int64_t step_batch_stride,
                       void* stream) {
+  return ndrr_raymarch_fwd_jac(vol, vol_dtype, vol_layout, lab, M, src, tgt, step, out, nullptr, 0, B, N, D, H, W,
+                               n_samples, n_classes, width, src_batch_stride, src_ray_stride, tgt_batch_stride,
+                               step_batch_stride, stream);
+}
+
+int ndrr_raymarch_fwd_jac(const void* vol, int vol_dtype, int vol_layout, const int8_t* lab, const float* M,
+                          const float* src, const float* tgt, const float* step, float* out, float* jac,
+                          int jac_planes, int B, int N, int D, int H, int W, int n_samples, int n_classes, int width,
+                          int64_t src_batch_stride, int64_t src_ray_stride, int64_t tgt_batch_stride,
+                          int64_t step_batch_stride, void* stream) {
   const int rc = check_common(vol, vol_dtype, vol_layout, lab, M, src, tgt, step, B, N, D, H, W, n_samples,
                               n_classes, width);
   if (rc != NDRR_OK) return rc;
   if (!out) return NDRR_EINVAL;
+  if (jac && jac_planes != 6 && jac_planes != 7) return NDRR_EINVAL;
+  if (jac && static_cast<int64_t>(jac_planes) * n_classes * B * N >= (int64_t{1} << 31)) return NDRR_ERANGE;
   Params p = make_params(vol, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width,
                          src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride);
   p.vol_bytes = static_cast<uint32_t>(packed_volume_bytes(D, H, W, vol_dtype, vol_layout));
   p.lab_bytes = n_classes > 1 ? static_cast<uint32_t>(packed_label_bytes(D, H, W, vol_layout)) : 0u;
   p.out = out;
+  p.jac = jac;
+  p.jac_q = jac ? jac_planes : 0;
   const dim3 grid(p.nblocks, B);
   hipStream_t st = static_cast<hipStream_t>(stream);
   switch (vol_dtype) {
@@ -1047,7 +1262,7 @@ void ndrr_launch_tile(int* tile_width, int* tile_height, int* tile_rays) {
 size_t ndrr_bwd_workspace_floats(int B, int N, int width) {
   if (B <= 0 || N <= 0 || width < 0) return 0;
   int tw, ih;
-  return static_cast<size_t>(B) * nblocks_of(N, width, tw, ih) * 12;
+  return static_cast<size_t>(B) * nblocks_of(N, width, tw, ih) * kBwdPartsMax * 12;
 }
 
 int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_layout, const int8_t* lab,
@@ -1059,7 +1274,7 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_l
   const int rc = check_common(vol, vol_dtype, vol_layout, lab, M, src, tgt, step, B, N, D, H, W, n_samples,
                               n_classes, width);
   if (rc != NDRR_OK) return rc;
-  if (!go || !workspace || !gM) return NDRR_EINVAL;
+  if (!go || !workspace || !gM || reinterpret_cast<uintptr_t>(workspace) % 16 != 0) return NDRR_EINVAL;
   if (need_gcam && (!gsrc || !gtgt || !gstep)) return NDRR_EINVAL;
   Params p = make_params(vol, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width,
                          src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride);
@@ -1078,7 +1293,58 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_l
   }
   int r = launch_status();
   if (r != NDRR_OK) return r;
-  hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(12 * kReduceLanes), 0, st, workspace, gM, p.nblocks);
+  const int parts_per_tile = vol_layout == NDRR_LAYOUT_STENCIL ? bwd_parts_per_tile(NDRR_LAYOUT_STENCIL)
+                             : vol_layout == NDRR_LAYOUT_QUAD  ? bwd_parts_per_tile(NDRR_LAYOUT_QUAD)
+                                                               : bwd_parts_per_tile(NDRR_LAYOUT_LINEAR);
+  hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(kReduceThreads), 0, st, workspace, gM,
+                     p.nblocks * parts_per_tile);
+  return launch_status();
+}
+
+int ndrr_raymarch_bwd_jac(const float* go, int64_t go_batch_stride, const float* jac, int jac_planes, const float* M,
+                          const float* src, const float* tgt, const float* step, float* workspace, float* gM,
+                          float* gsrc, float* gtgt, float* gstep, int need_gcam, int B, int N, int n_classes,
+                          int width, int64_t src_batch_stride, int64_t src_ray_stride, int64_t tgt_batch_stride,
+                          int64_t step_batch_stride, void* stream) {
+  if (!go || !jac || !M || !src || !tgt || !step || !workspace || !gM) return NDRR_EINVAL;
+  if (reinterpret_cast<uintptr_t>(workspace) % 16 != 0) return NDRR_EINVAL;
+  if (B <= 0 || N <= 0 || width < 0 || (width > 0 && N % width != 0)) return NDRR_EINVAL;
+  if (jac_planes != 6 && jac_planes != 7) return NDRR_EINVAL;
+  if (need_gcam && (jac_planes != 7 || !gsrc || !gtgt || !gstep)) return NDRR_EINVAL;
+  if (go_batch_stride != 0 && go_batch_stride != static_cast<int64_t>(n_classes) * N) return NDRR_EINVAL;
+  if (n_classes < 1 || n_classes > NDRR_MAX_CLASSES) return NDRR_ECLASSES;
+  if (B > 65535 || static_cast<int64_t>(jac_planes) * n_classes * B * N >= (int64_t{1} << 31) ||
+      3 * static_cast<int64_t>(B) * N >= (int64_t{1} << 31))
+    return NDRR_ERANGE;
+  Params p{};
+  p.M = M; p.src = src; p.tgt = tgt; p.step = step; p.go = go; p.go_bs = go_batch_stride;
+  p.jac = const_cast<float*>(jac);
+  p.jac_q = jac_planes;
+  p.src_bs = src_batch_stride; p.src_rs = src_ray_stride; p.tgt_bs = tgt_batch_stride; p.step_bs = step_batch_stride;
+  p.B = B; p.N = N; p.C = n_classes; p.IW = width;
+  p.nblocks = nblocks_of(N, width, p.tiles_w, p.IH);
+  p.gpart = workspace;
+  p.gsrc = gsrc; p.gtgt = gtgt; p.gstep = gstep;
+  const dim3 grid(p.nblocks, B), block(kTileRays);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool gcam = need_gcam != 0;
+  const int C = n_classes;
+  if (C == 1) {
+    if (gcam) hipLaunchKernelGGL((k_bwd_jac<1, true>), grid, block, 0, st, p);
+    else hipLaunchKernelGGL((k_bwd_jac<1, false>), grid, block, 0, st, p);
+  } else if (C == 2) {
+    if (gcam) hipLaunchKernelGGL((k_bwd_jac<2, true>), grid, block, 0, st, p);
+    else hipLaunchKernelGGL((k_bwd_jac<2, false>), grid, block, 0, st, p);
+  } else if (C <= 4) {
+    if (gcam) hipLaunchKernelGGL((k_bwd_jac<4, true>), grid, block, 0, st, p);
+    else hipLaunchKernelGGL((k_bwd_jac<4, false>), grid, block, 0, st, p);
+  } else {
+    if (gcam) hipLaunchKernelGGL((k_bwd_jac<8, true>), grid, block, 0, st, p);
+    else hipLaunchKernelGGL((k_bwd_jac<8, false>), grid, block, 0, st, p);
+  }
+  int r = launch_status();
+  if (r != NDRR_OK) return r;
+  hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(kReduceThreads), 0, st, workspace, gM, p.nblocks);
   return launch_status();
 }
 

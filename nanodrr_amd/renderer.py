@@ -13,7 +13,7 @@ from __future__ import annotations
 import torch
 
 from . import _lib
-from ._ops import fused_raymarch
+from ._ops import camera_to_voxel, fused_raymarch, fused_raymarch_jac, make_rays
 
 _FUSED_DTYPES = (torch.float32, torch.float16, torch.bfloat16)
 _BACKENDS = ("auto", "hip", "triton")
@@ -27,6 +27,14 @@ _BACKENDS = ("auto", "hip", "triton")
 # there, at half the memory).
 VOLUME_LAYOUT = "auto"
 _LAYOUTS = ("linear", "quad", "stencil")
+
+# How pose and camera gradients are formed when the volume needs none:
+# "single_pass" (the forward march also stores per-ray sums and the backward
+# contracts them with the upstream gradient, without marching the volume
+# again) or "two_pass" (the backward marches the volume again, as nanodrr's
+# kernel does). Volume gradients always take the two-pass adjoint.
+ADJOINT = "single_pass"
+_ADJOINTS = ("single_pass", "two_pass")
 
 
 def fused_supported(subject, B: int, n_pixels: int) -> bool:
@@ -90,12 +98,22 @@ def render(
             )
         raise ValueError(f"Unknown backend {backend!r}; expected 'auto', 'hip' or 'triton'")
     device, dtype = rt_inv.device, rt_inv.dtype
-    if tgt is None:
-        tgt = _make_tgt(k_inv, sdd, height, width, device, dtype)
-    if src is None:
-        src = _make_src(orthographic, tgt, sdd)
-    step_size = (tgt - src).norm(dim=-1) / float(n_samples - 1)
+    if tgt is None and src is None and _native_rays(k_inv, sdd, dtype):
+        tgt, src, step_size = make_rays(k_inv, sdd, height, width, orthographic, n_samples)
+    else:
+        if tgt is None:
+            tgt = _make_tgt(k_inv, sdd, height, width, device, dtype)
+        if src is None:
+            src = _make_src(orthographic, tgt, sdd)
+        step_size = (tgt - src).norm(dim=-1) / float(n_samples - 1)
     return render_hip(subject, rt_inv, src, tgt, step_size, n_samples, height, width)
+
+
+def _native_rays(k_inv, sdd, dtype) -> bool:
+    """Rays from one native launch: fp32 device intrinsics that need no gradient."""
+    return (dtype == torch.float32 and k_inv.dtype == torch.float32 and sdd.dtype == torch.float32
+            and k_inv.is_cuda and sdd.is_cuda and k_inv.dim() == 3 and sdd.dim() == 1
+            and not (torch.is_grad_enabled() and (k_inv.requires_grad or sdd.requires_grad)))
 
 
 def render_hip(subject, rt_inv, src, tgt, step_size, n_samples, height, width) -> torch.Tensor:
@@ -104,7 +122,10 @@ def render_hip(subject, rt_inv, src, tgt, step_size, n_samples, height, width) -
     if not (rt_inv.is_cuda and image.is_cuda):
         raise ValueError("the hip backend renders on a GPU: move the subject and poses to a cuda device")
     # Camera -> voxel-index affine, composed in float64 and rounded once.
-    M = (subject.world_to_voxel.to(torch.float64) @ rt_inv.to(torch.float64))[:, :3, :].to(torch.float32)
+    if rt_inv.dtype == torch.float32 and rt_inv.dim() == 3 and rt_inv.shape[1:] == (4, 4):
+        M = camera_to_voxel(subject.world_to_voxel_f64(), rt_inv.contiguous())
+    else:
+        M = (subject.world_to_voxel.to(torch.float64) @ rt_inv.to(torch.float64))[:, :3, :].to(torch.float32)
     return march(subject, M, src, tgt, step_size, n_samples, height, width)
 
 
@@ -142,17 +163,15 @@ def march(subject, M, src, tgt, step_size, n_samples, height, width) -> torch.Te
     layout = volume_layout(image)
     packed = None if layout == "linear" else subject.packed_volume(layout)
     lab = subject.packed_labels(layout) if C > 1 else vol.new_empty(0, dtype=torch.int8)
-    out = fused_raymarch(
-        vol,
-        lab,
-        M.contiguous(),
-        src.to(torch.float32).contiguous(),
-        tgt.to(torch.float32).contiguous(),
-        step_size.to(torch.float32).contiguous(),
-        n_samples,
-        C,
-        width,
-        packed,
-        layout,
-    )
+    if ADJOINT not in _ADJOINTS:
+        raise ValueError(f"ADJOINT must be one of {_ADJOINTS}, got {ADJOINT!r}")
+    args = (vol, lab, M.contiguous(), src.to(torch.float32).contiguous(), tgt.to(torch.float32).contiguous(),
+            step_size.to(torch.float32).contiguous(), n_samples, C, width)
+    gcam = any(t.requires_grad for t in args[3:6])
+    single = (ADJOINT == "single_pass" and torch.is_grad_enabled() and not image.requires_grad
+              and (gcam or args[2].requires_grad))
+    if single:
+        out, _ = fused_raymarch_jac(*args, 7 if gcam else 6, packed, layout)
+    else:
+        out = fused_raymarch(*args, packed, layout)
     return out.reshape(B, C, height, width)

@@ -48,9 +48,18 @@ def inside_samples(M: torch.Tensor, src: torch.Tensor, tgt: torch.Tensor, n_samp
 
 
 def march_bytes(n_inside: int, n_rays: int, n_poses: int, n_classes: int, vol_dtype: torch.dtype,
-                backward: bool = False, need_gcam: bool = False) -> int:
-    """Algorithmic HBM bytes of one forward (or backward) launch."""
+                backward: bool = False, need_gcam: bool = False, jac_planes: int = 0) -> int:
+    """Algorithmic HBM bytes of one forward (or two-pass backward) launch; a forward
+    with jac_planes > 0 also writes the single-pass adjoint's per-ray sums."""
     ev = _ELEM[vol_dtype]
     per_sample = 8 * ev + (1 if n_classes > 1 else 0)
-    per_ray = 12 + 4 + 4 * n_classes + (28 if (backward and need_gcam) else 0)
+    per_ray = 12 + 4 + 4 * n_classes + (28 if (backward and need_gcam) else 0) + 4 * jac_planes * n_classes
     return n_inside * per_sample + n_rays * per_ray + 48 * n_poses
+
+
+def contraction_bytes(n_rays: int, n_poses: int, n_classes: int, jac_planes: int = 6, need_gcam: bool = False) -> int:
+    """Algorithmic HBM bytes of the single-pass backward: per ray the stored sums,
+    the upstream gradient, tgt and step read (plus 28 bytes of camera gradients
+    written), and M read per pose."""
+    per_ray = 4 * jac_planes * n_classes + 4 * n_classes + 12 + 4 + (28 if need_gcam else 0)
+    return n_rays * per_ray + 48 * n_poses

@@ -121,6 +121,18 @@ class Subject(torch.nn.Module):
                       self.isocenter, max_label=self.n_classes - 1, convert_to_mu=False)
         return out
 
+    def world_to_voxel_f64(self) -> torch.Tensor:
+        """`world_to_voxel` in float64 (the camera -> voxel composition runs in FP64),
+        cached until the buffer changes."""
+        w = self.world_to_voxel
+        if w.dtype == torch.float64 or torch.compiler.is_compiling():
+            return w.to(torch.float64).contiguous()
+        key = (w.data_ptr(), w.device, w._version)
+        if getattr(self, "_w2v64_key", None) != key:
+            self._w2v64 = w.to(torch.float64).contiguous()
+            self._w2v64_key = key
+        return self._w2v64
+
     def packed_volume(self, layout: str) -> torch.Tensor:
         """`image` re-laid out for the kernels ("quad" or "stencil", see
         ndrr_pack_volume), cached until the image tensor or the layout changes."""

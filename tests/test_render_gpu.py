@@ -31,10 +31,19 @@ GRAD_TOL = 1e-4
 
 @pytest.fixture(params=["quad", "stencil", "linear"], autouse=True)
 def volume_layout(request, monkeypatch):
-    """Every test runs on both volume layouts the kernels read."""
+    """Every test runs on every volume layout the kernels read."""
     import nanodrr_amd.renderer as renderer
 
     monkeypatch.setattr(renderer, "VOLUME_LAYOUT", request.param)
+    return request.param
+
+
+@pytest.fixture(params=["single_pass", "two_pass"], autouse=True)
+def adjoint(request, monkeypatch):
+    """... and with both adjoints for pose and camera gradients."""
+    import nanodrr_amd.renderer as renderer
+
+    monkeypatch.setattr(renderer, "ADJOINT", request.param)
     return request.param
 
 
