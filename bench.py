@@ -7,16 +7,16 @@ samples per ray. One step = render(...) of the 16 poses plus the backward of
 a weighted sum to the poses, captured once in a HIP graph and replayed (the
 eager, uncaptured step is reported too); DRR/s = 16 * steps / elapsed.
 Cases (volume dtype : layout the kernels read : adjoint):
-  bf16:stencil:single  the headline `value`: bf16 stencil records, single-pass
+  bf16:quad:single     the headline `value`: bf16 quad records, single-pass
                        adjoint (the forward march stores per-ray sums, the
                        backward contracts them);
   fp32:linear:two      the plain fp32 kernel: the (D, H, W) volume, two-pass
                        adjoint (the backward marches the volume again, as
                        nanodrr's kernel does) - the baseline of the speed-ups;
-  fp32:quad:single, bf16:stencil:two  the other two corners.
+  fp32:quad:single, bf16:stencil:single, bf16:stencil:two  other corners.
 Every case accumulates in float32.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cases bf16:stencil:single,...] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cases bf16:quad:single,...] [--no-cpu-baseline]
 
 Prints one JSON line. `roofline` covers the dominant kernel of the headline
 case, timed with HIP events on the stream it runs on; `cpu_baseline` times
@@ -290,7 +290,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cases", default="bf16:stencil:single,fp32:linear:two,fp32:quad:single,bf16:stencil:two",
+    ap.add_argument("--cases",
+                    default="bf16:quad:single,fp32:linear:two,fp32:quad:single,bf16:stencil:single,bf16:stencil:two",
                     help="dtype:layout:adjoint cases; the first is the headline, fp32:linear:two the plain kernel")
     ap.add_argument("--probe", default=None,
                     help="run only the counter probe of one case, e.g. bf16:stencil:single")

@@ -27,7 +27,7 @@ constexpr int kWave = 64;        // rays per wave: an 8 x 8 detector patch (or 6
 #define NDRR_TILE_WAVES_X 4
 #endif
 #ifndef NDRR_TILE_WAVES_Y
-#define NDRR_TILE_WAVES_Y 2
+#define NDRR_TILE_WAVES_Y 1
 #endif
 // Chunks: wave groups per workgroup marching the same rays; group c takes the
 // c-th equal part of every ray's clipped sample range. Per kernel and layout.
@@ -38,7 +38,7 @@ constexpr int kWave = 64;        // rays per wave: an 8 x 8 detector patch (or 6
 #define NDRR_FWD_CHUNKS_STENCIL 1
 #endif
 #ifndef NDRR_FWD_CHUNKS_QUAD
-#define NDRR_FWD_CHUNKS_QUAD 1
+#define NDRR_FWD_CHUNKS_QUAD 2
 #endif
 // Pose-gradient partials of the backward kernel: 1 = one per tile (the waves
 // of a workgroup added through LDS after a barrier), 0 = one per wave.
@@ -75,6 +75,10 @@ constexpr int bwd_chunks(int layout) {
                                        : (layout == NDRR_LAYOUT_QUAD ? NDRR_BWD_CHUNKS_QUAD : NDRR_BWD_CHUNKS_LINEAR);
 }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
+// Forward chunks for CV class slots: up to four slots only, so the LDS that
+// carries the chunks' partial sums (and the single-pass adjoint's, 6 per
+// slot) stays small.
+constexpr int fwd_chunks(int layout, int cv) { return cv <= 4 ? fwd_chunks(layout) : 1; }
 constexpr int kMaxFwdChunks = cmax(cmax(fwd_chunks(NDRR_LAYOUT_LINEAR), fwd_chunks(NDRR_LAYOUT_STENCIL)),
                                    fwd_chunks(NDRR_LAYOUT_QUAD));
 constexpr int kMaxBwdChunks = cmax(cmax(bwd_chunks(NDRR_LAYOUT_LINEAR), bwd_chunks(NDRR_LAYOUT_STENCIL)),
@@ -219,10 +223,11 @@ __device__ inline DF df_mul(DF a, float b) {
 struct Ray {
   float sx, sy, sz;     // source, camera space
   float ex, ey, ez;     // tgt - src, camera space
-  DF ax, ay, az;        // voxel-space position of sample s_lo
+  DF ax, ay, az;        // voxel-space position of the anchor: the ray's first clipped sample
   DF dx, dy, dz;        // voxel-space step per sample
   float step;
-  int s_lo, n_s;        // samples [s_lo, s_lo + n_s) can touch the volume
+  int s_lo, n_s;        // this chunk's samples [s_lo, s_lo + n_s) can touch the volume
+  int k_base;           // s_lo minus the anchor's sample index
 };
 
 // Ray-parameter interval where lo <= p0 + t d <= hi (per axis). An approximate
@@ -241,7 +246,9 @@ __device__ inline DF df_mul_df(DF a, DF b) {
 }
 
 // Per-ray set-up for the samples of chunk `chunk` of `nch` equal parts of
-// the clipped sample range: the chunk's first sample (anchor) and its count.
+// the clipped sample range: the chunk's first sample and count. Positions are
+// taken from one anchor per ray (its first clipped sample) in every chunk, so
+// a sample lands at the same bits whatever the chunking.
 __device__ inline Ray setup_ray(const Params& p, const float* __restrict__ Mb, int b, int n, bool valid, int chunk,
                                 int nch) {
   Ray r;
@@ -292,9 +299,10 @@ __device__ inline Ray setup_ray(const Params& p, const float* __restrict__ Mb, i
     }
   }
   const int per = (n_all + nch - 1) / nch;
-  r.s_lo = s_lo + chunk * per;
-  r.n_s = max(0, min(per, n_all - chunk * per));
-  const float s0 = static_cast<float>(r.s_lo);
+  r.k_base = chunk * per;
+  r.s_lo = s_lo + r.k_base;
+  r.n_s = max(0, min(per, n_all - r.k_base));
+  const float s0 = static_cast<float>(s_lo);
   const DF inv{p.inv_s, p.inv_lo};
   DF* const an[3] = {&r.ax, &r.ay, &r.az};
   DF* const st[3] = {&r.dx, &r.dy, &r.dz};
@@ -533,6 +541,25 @@ __device__ inline void lerp_grad(const Corners& v, const Cell& c, float& dvx, fl
         ((v.v110 - v.v010) * (1.0f - fx) + (v.v111 - v.v011) * fx) * fy;
 }
 
+// Value and gradient together, the gradient from the interpolation's own
+// intermediates (the x-lerped edges c_ab and y-lerped faces c_a):
+//   d/dz = c_1 - c_0,  d/dy = lerp_z(c_01 - c_00, c_11 - c_10),
+//   d/dx = lerp_z(lerp_y(v_001 - v_000, v_011 - v_010), lerp_y(v_101 - v_100, v_111 - v_110)),
+// the same derivatives as lerp_grad in about half the instructions. The value
+// is lerp_value's, bit for bit.
+__device__ inline float lerp_value_grad(const Corners& v, const Cell& c, float& dvx, float& dvy, float& dvz) {
+  const float c00 = lerp1(v.v000, v.v001, c.fx);
+  const float c01 = lerp1(v.v010, v.v011, c.fx);
+  const float c10 = lerp1(v.v100, v.v101, c.fx);
+  const float c11 = lerp1(v.v110, v.v111, c.fx);
+  const float c0 = lerp1(c00, c01, c.fy);
+  const float c1 = lerp1(c10, c11, c.fy);
+  dvz = c1 - c0;
+  dvy = lerp1(c01 - c00, c11 - c10, c.fz);
+  dvx = lerp1(lerp1(v.v001 - v.v000, v.v011 - v.v010, c.fy), lerp1(v.v101 - v.v100, v.v111 - v.v110, c.fy), c.fz);
+  return lerp1(c0, c1, c.fz);
+}
+
 // Nearest label (round half to even, zero padded: outside the volume is class 0).
 __device__ inline int nearest_label(const int8_t* __restrict__ lab, int D, int H, int W,
                                     float x, float y, float z) {
@@ -583,7 +610,7 @@ struct Batch {
   __device__ inline void issue(const Src& s, const Ray& r, int k0) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const float k = static_cast<float>(min(k0 + u, r.n_s - 1));
+      const float k = static_cast<float>(r.k_base + min(k0 + u, r.n_s - 1));
       const float x = sample_coord(r.ax, r.dx, k), y = sample_coord(r.ay, r.dy, k), z = sample_coord(r.az, r.dz, k);
       cell[u] = cell_of(x, y, z);
       f[u].load(s, cell[u]);
@@ -605,14 +632,14 @@ struct Batch {
 // many waves, each with a shorter life. The chunks' partial sums are added in
 // chunk order through LDS (deterministic).
 // CV == 1: single channel (no label map). CV in {2, 4, 8}: label map, runtime C <= CV.
-// JAC (with NCH == 1): the single-pass adjoint's per-ray, per-class sums are
-// accumulated in the same march and written to p.jac: G_c = sum grad v,
-// T_c = sum t grad v over the samples routed to class c (voxel units), and
-// with jac_q == 7 also V_c = sum v (the image before the step factor).
+// JAC: the single-pass adjoint's per-ray, per-class sums are accumulated in
+// the same march and written to p.jac: G_c = sum grad v, T_c = sum t grad v
+// over the samples routed to class c (voxel units), and with jac_q == 7 also
+// V_c = sum v (the image before the step factor); chunk partials are added
+// in chunk order through LDS like the image sums.
 template <int DT, int L, int CV, bool KAHAN, int NCH, bool JAC>
 __global__ __launch_bounds__(kTileRays * NCH) __attribute__((amdgpu_waves_per_eu(NDRR_WAVES_PER_EU)))
 void k_fwd(Params p) {
-  static_assert(!JAC || NCH == 1, "the single-pass adjoint sums are written by one wave group");
   int b, tile;
   block_coords(b, tile);
   const int tr = threadIdx.x % kTileRays, chunk = threadIdx.x / kTileRays;
@@ -640,26 +667,26 @@ void k_fwd(Params p) {
       const Cell& c = batch.cell[u];
       const bool live = k0 + u < r.n_s;
       const Corners v = batch.f[u].corners();
-      const float val = live ? lerp_value(v, c) : 0.0f;
+      float dvx, dvy, dvz;
+      const float val = live ? (JAC ? lerp_value_grad(v, c, dvx, dvy, dvz) : lerp_value(v, c)) : 0.0f;
       if constexpr (CV == 1) {
         if constexpr (KAHAN) comp_add(acc[0], comp[0], val); else acc[0] += val;
       } else {
+        // Class routing by a 0/1 factor: fma(m, x, acc) is acc + x or acc
+        // (x finite), the same values as a select and an add.
 #pragma unroll
         for (int cl = 0; cl < CV; ++cl) {
-          const float contrib = (batch.cls[u] == cl) ? val : 0.0f;
-          if constexpr (KAHAN) comp_add(acc[cl], comp[cl], contrib); else acc[cl] += contrib;
+          const float m = (batch.cls[u] == cl) ? 1.0f : 0.0f;
+          if constexpr (KAHAN) comp_add(acc[cl], comp[cl], m != 0.0f ? val : 0.0f);
+          else acc[cl] = fmaf(m, val, acc[cl]);
         }
       }
       if constexpr (JAC) {
-        float dvx, dvy, dvz;
-        lerp_grad(v, c, dvx, dvy, dvz);
         dvx = live ? dvx : 0.0f;
         dvy = live ? dvy : 0.0f;
         dvz = live ? dvz : 0.0f;
         const float t = static_cast<float>(r.s_lo + k0 + u) * p.inv_s;
         const float tx = dvx * t, ty = dvy * t, tz = dvz * t;
-        // Class routing by a 0/1 factor: fma(m, x, acc) is acc + x or acc
-        // (x finite), the same values as a select, in one instruction.
 #pragma unroll
         for (int cl = 0; cl < CV; ++cl) {
           const float m = (CV == 1 || batch.cls[u] == cl) ? 1.0f : 0.0f;
@@ -675,10 +702,16 @@ void k_fwd(Params p) {
   }
 
   __shared__ float part[NCH > 1 ? NCH - 1 : 1][CV][kTileRays];
+  constexpr bool kJpart = JAC && NCH > 1;
+  __shared__ float jpart[kJpart ? NCH - 1 : 1][kJpart ? 6 * CV : 1][kJpart ? kTileRays : 1];
   if constexpr (NCH > 1) {
     if (chunk > 0) {
 #pragma unroll
       for (int c = 0; c < CV; ++c) part[chunk - 1][c][tr] = KAHAN ? (acc[c] + comp[c]) : acc[c];
+      if constexpr (kJpart) {
+#pragma unroll
+        for (int q = 0; q < 6 * CV; ++q) jpart[chunk - 1][q][tr] = jg[q];
+      }
     }
     __syncthreads();
   }
@@ -698,7 +731,14 @@ void k_fwd(Params p) {
         float* jp = p.jac + (static_cast<int64_t>(b) * p.jac_q * p.C + c) * p.N + n;
         const int64_t plane = static_cast<int64_t>(p.C) * p.N;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) jp[q * plane] = jg[6 * c + q];
+        for (int q = 0; q < 6; ++q) {
+          float j = jg[6 * c + q];
+          if constexpr (kJpart) {
+#pragma unroll
+            for (int k = 1; k < NCH; ++k) j += jpart[k - 1][6 * c + q][tr];
+          }
+          jp[q * plane] = j;
+        }
         if (p.jac_q == 7) jp[6 * plane] = sum;
       }
     }
@@ -863,12 +903,13 @@ void k_bwd(Params p) {
 // factor go * step moved out of the sums); the pose partials, gsrc and gtgt
 // follow from G and T exactly as in k_bwd.
 template <int CV, bool GCAM>
+// Rays are taken in their linear order (kTileRays consecutive rays per
+// workgroup, grid (ceil(N / kTileRays), B)), so every plane is read in full
+// cache lines; the pose partials go to gpart[b][blockIdx.x].
 __global__ __launch_bounds__(kTileRays) void k_bwd_jac(Params p) {
-  int b, tile;
-  block_coords(b, tile);
-  const int tr = threadIdx.x;
-  bool valid;
-  const int n = ray_index(p, tile, tr, valid);
+  const int b = blockIdx.y, blk = blockIdx.x;
+  const int n = blk * kTileRays + threadIdx.x;
+  const bool valid = n < p.N;
   const int nn = valid ? n : 0;
   const float* sp = p.src + b * p.src_bs + nn * p.src_rs;
   const float* tp = p.tgt + b * p.tgt_bs + 3 * static_cast<int64_t>(nn);
@@ -924,7 +965,7 @@ __global__ __launch_bounds__(kTileRays) void k_bwd_jac(Params p) {
     float v = red[0][j];
 #pragma unroll
     for (int w = 1; w < kWaves; ++w) v += red[w][j];
-    p.gpart[(static_cast<int64_t>(b) * p.nblocks + tile) * 12 + j] = v;
+    p.gpart[(static_cast<int64_t>(b) * gridDim.x + blk) * 12 + j] = v;
   }
 }
 
@@ -1140,16 +1181,15 @@ int launch_status() {
 
 template <int DT, int L, int CV>
 void launch_fwd3(const Params& p, dim3 grid, hipStream_t st) {
+  constexpr int nch = fwd_chunks(L, CV);
+  const dim3 block(kTileRays * nch);
   if (p.jac) {
-    const dim3 block(kTileRays);
     if (p.S >= kKahanMinSamples)
-      hipLaunchKernelGGL((k_fwd<DT, L, CV, true, 1, true>), grid, block, 0, st, p);
+      hipLaunchKernelGGL((k_fwd<DT, L, CV, true, nch, true>), grid, block, 0, st, p);
     else
-      hipLaunchKernelGGL((k_fwd<DT, L, CV, false, 1, true>), grid, block, 0, st, p);
+      hipLaunchKernelGGL((k_fwd<DT, L, CV, false, nch, true>), grid, block, 0, st, p);
     return;
   }
-  constexpr int nch = fwd_chunks(L);
-  const dim3 block(kTileRays * nch);
   if (p.S >= kKahanMinSamples)
     hipLaunchKernelGGL((k_fwd<DT, L, CV, true, nch, false>), grid, block, 0, st, p);
   else
@@ -1325,7 +1365,9 @@ int ndrr_raymarch_bwd_jac(const float* go, int64_t go_batch_stride, const float*
   p.nblocks = nblocks_of(N, width, p.tiles_w, p.IH);
   p.gpart = workspace;
   p.gsrc = gsrc; p.gtgt = gtgt; p.gstep = gstep;
-  const dim3 grid(p.nblocks, B), block(kTileRays);
+  // ceil(N / kTileRays) <= nblocks: the workspace of ndrr_bwd_workspace_floats holds the partials
+  const int nlin = (N + kTileRays - 1) / kTileRays;
+  const dim3 grid(nlin, B), block(kTileRays);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const bool gcam = need_gcam != 0;
   const int C = n_classes;
@@ -1344,7 +1386,7 @@ int ndrr_raymarch_bwd_jac(const float* go, int64_t go_batch_stride, const float*
   }
   int r = launch_status();
   if (r != NDRR_OK) return r;
-  hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(kReduceThreads), 0, st, workspace, gM, p.nblocks);
+  hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(kReduceThreads), 0, st, workspace, gM, nlin);
   return launch_status();
 }
 

@@ -20,11 +20,12 @@ _BACKENDS = ("auto", "hip", "triton")
 
 # Volume layout the kernels read: "stencil" (a cached copy with all eight
 # corners of each cell: one 16-byte load per sample for 16-bit volumes),
-# "quad" (a cached copy of each voxel's (y, z) corner quads: half the memory),
-# "linear" (the (D, H, W) volume itself) or "auto": linear when the volume
-# requires grad (a packed copy would be re-made after every update), else
-# stencil for 16-bit volumes and quad for float32 (the faster of the two
-# there, at half the memory).
+# "quad" (a cached copy of each voxel's (y, z) corner quads: half the
+# stencil's memory, which lets a 256^3 16-bit volume stay in the Infinity
+# Cache), "linear" (the (D, H, W) volume itself) or "auto": linear when the
+# volume requires grad (a packed copy would be re-made after every update),
+# else quad (the fastest measured for every dtype with the single-pass
+# adjoint).
 VOLUME_LAYOUT = "auto"
 _LAYOUTS = ("linear", "quad", "stencil")
 
@@ -136,7 +137,7 @@ def volume_layout(image: torch.Tensor) -> str:
         if image.requires_grad:
             layout = "linear"
         else:
-            layout = "quad" if image.dtype == torch.float32 else "stencil"
+            layout = "quad"
     if layout not in _LAYOUTS:
         raise ValueError(f"VOLUME_LAYOUT must be 'auto' or one of {_LAYOUTS}, got {VOLUME_LAYOUT!r}")
     return layout

@@ -97,7 +97,9 @@ def test_single_pass_matches_two_pass_on_phantom(C, layout):
                 res[adjoint] = (out.detach(), rt.grad, sd.grad)
             assert torch.equal(res["single_pass"][0], res["two_pass"][0])
             assert rel_to_max(res["two_pass"][1][:, :3].cpu(), res["single_pass"][1][:, :3].cpu()) < 1e-5
-            assert rel_to_max(res["two_pass"][2].cpu(), res["single_pass"][2].cpu()) < 1e-5
+            # d/d sdd is one scalar summed over every ray with cancellation: the two fp32
+            # adjoints group its terms differently (chunks, per-class factoring)
+            assert rel_to_max(res["two_pass"][2].cpu(), res["single_pass"][2].cpu()) < 5e-5
     finally:
         renderer.VOLUME_LAYOUT, renderer.ADJOINT = old_layout, old_adjoint
 
