@@ -282,6 +282,38 @@ int ndrr_pose_compose_bwd(const double* prefix, const double* suffix, const floa
 int ndrr_zncc(const float* x, const float* y, int B, int N, float eps, float* score,
               float* grad_x, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Pose-proposal CNN: 3x3 convolutions on the bf16 matrix cores
+ * ------------------------------------------------------------------------- */
+
+/*
+ * No counterpart in the reference (the learned initialiser of the
+ * registration loop). Activations are NHWC bf16 ([N][H][W][C]), weights
+ * [Cout][3][3][Cin] bf16, bias fp32, accumulation fp32; padding 1, stride 1
+ * or 2 (Ho = (H - 1) / stride + 1). Cout must be a multiple of 32 and Cin 1
+ * or a multiple of 32.
+ *   ndrr_conv3x3_fwd: y = relu?(conv(x, w) + bias) (bias may be NULL);
+ *   ndrr_conv3x3_transpose_weights: wt[3][3][Cin][Cout] from w, for
+ *   ndrr_conv3x3_bwd_data: dx = conv^T(dy, w), times [x > 0] when relu_mask
+ *     (the ReLU of the layer that produced x); Cin a multiple of 32;
+ *   ndrr_conv3x3_bwd_weight: dw[Cout][3][3][Cin] fp32 = sum over pixels of
+ *     dy x-patch, through a workspace of
+ *     ndrr_conv3x3_bwd_weight_workspace_floats floats (split partials added
+ *     in a fixed order: deterministic); Cout a multiple of 64 unless Cin == 1.
+ */
+int ndrr_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y,
+                     int N, int H, int W, int Cin, int Cout, int stride, int relu,
+                     void* stream);
+int ndrr_conv3x3_transpose_weights(const void* w, void* wt, int Cin, int Cout, void* stream);
+int ndrr_conv3x3_bwd_data(const void* dy, const void* wt, const void* x, void* dx,
+                          int N, int H, int W, int Cin, int Cout, int stride,
+                          int relu_mask, void* stream);
+size_t ndrr_conv3x3_bwd_weight_workspace_floats(int N, int H, int W, int Cin, int Cout,
+                                                int stride);
+int ndrr_conv3x3_bwd_weight(const void* dy, const void* x, float* workspace, float* dw,
+                            int N, int H, int W, int Cin, int Cout, int stride,
+                            void* stream);
+
 #ifdef __cplusplus
 }
 #endif

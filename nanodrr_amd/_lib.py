@@ -59,6 +59,11 @@ SIGNATURES: dict[str, tuple] = {
                                 _c_p]),
     "ndrr_camera_to_voxel": (_c_int, [_c_p, _c_p, _c_p, _c_int, _c_p]),
     "ndrr_camera_to_voxel_bwd": (_c_int, [_c_p, _c_p, _c_p, _c_int, _c_p]),
+    "ndrr_conv3x3_fwd": (_c_int, [_c_p, _c_p, _c_p, _c_p] + [_c_int] * 7 + [_c_p]),
+    "ndrr_conv3x3_transpose_weights": (_c_int, [_c_p, _c_p, _c_int, _c_int, _c_p]),
+    "ndrr_conv3x3_bwd_data": (_c_int, [_c_p, _c_p, _c_p, _c_p] + [_c_int] * 7 + [_c_p]),
+    "ndrr_conv3x3_bwd_weight_workspace_floats": (_c_size, [_c_int] * 6),
+    "ndrr_conv3x3_bwd_weight": (_c_int, [_c_p, _c_p, _c_p, _c_p] + [_c_int] * 6 + [_c_p]),
     "ndrr_pose_compose": (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_int, _c_p]),
     "ndrr_pose_compose_bwd": (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_int, _c_p]),
     "ndrr_zncc": (_c_int, [_c_p, _c_p, _c_int, _c_int, ctypes.c_float, _c_p, _c_p, _c_p]),

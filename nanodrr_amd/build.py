@@ -13,7 +13,7 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
-SOURCES = [PKG / "csrc" / "raymarch.hip", PKG / "csrc" / "camera.hip", PKG / "csrc" / "register.hip"]
+SOURCES = [PKG / "csrc" / name for name in ("raymarch.hip", "camera.hip", "register.hip", "conv.hip")]
 OUTPUT = PKG / "libnanodrr_amd.so"
 
 # -packed-fp32-ops: keep the backend from forming packed fp32 instructions.

@@ -1,0 +1,453 @@
+// 3x3 convolutions of the pose-proposal CNN on AMD Instinct MI355X (gfx950):
+// implicit GEMMs on the bf16 matrix cores (v_mfma_f32_16x16x32_bf16), bf16
+// activations and weights in NHWC / [Cout][3][3][Cin], fp32 accumulation.
+//
+//   forward      y[m][co]  = relu(b[co] + sum_{tap,ci} x[m @ tap][ci] w[co][tap][ci])
+//   input grad   dx[m][ci] = [x[m][ci] > 0] sum_{tap,co} dy[m' (tap)][co] w[co][tap][ci]
+//   weight grad  dw[co][tap][ci] = sum_m dy[m][co] x[m @ tap][ci]
+//
+// with padding 1 and stride 1 or 2 (out = (in - 1) / stride + 1). The GEMM
+// rows are pixels, the columns channels and K runs over (tap, channel) in
+// chunks of 32 channels, so every MFMA operand is one 16-byte load of eight
+// consecutive channels (a tap of a pixel, or a weight row). The weight grad
+// stages 32-pixel slabs of dy and x through LDS, transposed, so that its K
+// (the pixels) is contiguous; it splits the pixels over workgroups and sums
+// the split partials in a fixed order (deterministic). The first layer (one
+// input channel, K = 9 taps) packs its taps into one padded K chunk.
+//
+// MFMA operand layout (16x16x32 bf16, wave64), checked on the device: lane l
+// holds A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16], j < 8, and
+// D[4 (l / 16) + r][l % 16] in its four accumulators r.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nanodrr_amd.h"
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+constexpr int kThreads = 256;      // four waves
+constexpr int kPixelsPerWave = 32; // two 16-row MFMA tiles
+constexpr int kSplitPixels = 4096; // pixels per weight-grad workgroup
+
+struct ConvP {
+  const __bf16* x;    // [N][H][W][Cin]
+  const __bf16* w;    // [Cout][9][Cin]      (forward)
+  const __bf16* wt;   // [9][Cin][Cout]      (input grad)
+  const float* bias;  // [Cout]
+  const __bf16* dy;   // [N][Ho][Wo][Cout]
+  __bf16* y;          // [N][Ho][Wo][Cout]   (forward)
+  __bf16* dx;         // [N][H][W][Cin]      (input grad)
+  float* part;        // [splits][9][Cout][Cin] (weight grad)
+  int N, H, W, Cin, Ho, Wo, Cout, S, relu, mask;
+};
+
+__device__ inline bf16x8 zero8() {
+  bf16x8 z;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) z[j] = static_cast<__bf16>(0.0f);
+  return z;
+}
+
+__device__ inline bf16x8 load8(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+__device__ inline f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Forward. Block: 4 waves x 32 output pixels; 16 * COG output channels.
+// ---------------------------------------------------------------------------
+template <int COG, bool CIN1>
+__global__ __launch_bounds__(kThreads) void k_conv_fwd(ConvP p) {
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int M = p.N * p.Ho * p.Wo;
+  const int m0 = (blockIdx.x * 4 + wave) * kPixelsPerWave;
+  const int co0 = blockIdx.y * 16 * COG;
+  const int kq = 8 * (l / 16);
+  int pn[2], poy[2], pox[2];
+  bool pv[2];
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    const int m = m0 + 16 * g + (l % 16);
+    pv[g] = m < M;
+    const int mm = pv[g] ? m : 0;
+    pox[g] = mm % p.Wo;
+    poy[g] = (mm / p.Wo) % p.Ho;
+    pn[g] = mm / (p.Wo * p.Ho);
+  }
+  f32x4 acc[2][COG];
+#pragma unroll
+  for (int g = 0; g < 2; ++g)
+#pragma unroll
+    for (int c = 0; c < COG; ++c) acc[g][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  if constexpr (CIN1) {
+    // K = the nine taps of the single input channel, zero-padded to 32.
+    bf16x8 a[2], b[COG];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int tap = kq + j;
+        float v = 0.0f;
+        if (tap < 9 && pv[g]) {
+          const int iy = poy[g] * p.S + tap / 3 - 1, ix = pox[g] * p.S + tap % 3 - 1;
+          if (iy >= 0 && iy < p.H && ix >= 0 && ix < p.W)
+            v = static_cast<float>(p.x[(static_cast<int64_t>(pn[g]) * p.H + iy) * p.W + ix]);
+        }
+        a[g][j] = static_cast<__bf16>(v);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < COG; ++c) {
+      const int co = co0 + 16 * c + (l % 16);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int tap = kq + j;
+        b[c][j] = tap < 9 ? p.w[co * 9 + tap] : static_cast<__bf16>(0.0f);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int c = 0; c < COG; ++c) acc[g][c] = mfma(a[g], b[c], acc[g][c]);
+  } else {
+    for (int tap = 0; tap < 9; ++tap) {
+      const int ky = tap / 3, kx = tap % 3;
+      int64_t base[2];
+      bool inb[2];
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        const int iy = poy[g] * p.S + ky - 1, ix = pox[g] * p.S + kx - 1;
+        inb[g] = pv[g] && iy >= 0 && iy < p.H && ix >= 0 && ix < p.W;
+        base[g] = inb[g] ? ((static_cast<int64_t>(pn[g]) * p.H + iy) * p.W + ix) * p.Cin + kq : 0;
+      }
+      const __bf16* wrow[COG];
+#pragma unroll
+      for (int c = 0; c < COG; ++c)
+        wrow[c] = p.w + (static_cast<int64_t>(co0 + 16 * c + (l % 16)) * 9 + tap) * p.Cin + kq;
+      for (int ci0 = 0; ci0 < p.Cin; ci0 += 32) {
+        bf16x8 a[2], b[COG];
+#pragma unroll
+        for (int g = 0; g < 2; ++g) a[g] = inb[g] ? load8(p.x + base[g] + ci0) : zero8();
+#pragma unroll
+        for (int c = 0; c < COG; ++c) b[c] = load8(wrow[c] + ci0);
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+          for (int c = 0; c < COG; ++c) acc[g][c] = mfma(a[g], b[c], acc[g][c]);
+      }
+    }
+  }
+  // Epilogue: bias, ReLU, bf16, NHWC store.
+#pragma unroll
+  for (int c = 0; c < COG; ++c) {
+    const int co = co0 + 16 * c + (l % 16);
+    const float bv = p.bias ? p.bias[co] : 0.0f;
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + 16 * g + 4 * (l / 16) + r;
+        if (m < M) {
+          float v = acc[g][c][r] + bv;
+          if (p.relu) v = fmaxf(v, 0.0f);
+          p.y[static_cast<int64_t>(m) * p.Cout + co] = static_cast<__bf16>(v);
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Input gradient: rows are input pixels, columns input channels (16 * CIG per
+// block), K over (tap, output channel). Output pixel of (iy, tap):
+// oy = (iy + 1 - ky) / S when that divides exactly and lies in [0, Ho).
+// ---------------------------------------------------------------------------
+template <int CIG>
+__global__ __launch_bounds__(kThreads) void k_conv_bwd_data(ConvP p) {
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int M = p.N * p.H * p.W;
+  const int m0 = (blockIdx.x * 4 + wave) * kPixelsPerWave;
+  const int ci0 = blockIdx.y * 16 * CIG;
+  const int kq = 8 * (l / 16);
+  int pn[2], piy[2], pix[2];
+  bool pv[2];
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    const int m = m0 + 16 * g + (l % 16);
+    pv[g] = m < M;
+    const int mm = pv[g] ? m : 0;
+    pix[g] = mm % p.W;
+    piy[g] = (mm / p.W) % p.H;
+    pn[g] = mm / (p.W * p.H);
+  }
+  f32x4 acc[2][CIG];
+#pragma unroll
+  for (int g = 0; g < 2; ++g)
+#pragma unroll
+    for (int c = 0; c < CIG; ++c) acc[g][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int tap = 0; tap < 9; ++tap) {
+    const int ky = tap / 3, kx = tap % 3;
+    int64_t base[2];
+    bool inb[2];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int ty = piy[g] + 1 - ky, tx = pix[g] + 1 - kx;
+      const int oy = ty / p.S, ox = tx / p.S;
+      inb[g] = pv[g] && ty >= 0 && tx >= 0 && ty == oy * p.S && tx == ox * p.S && oy < p.Ho && ox < p.Wo;
+      base[g] = inb[g] ? ((static_cast<int64_t>(pn[g]) * p.Ho + oy) * p.Wo + ox) * p.Cout + kq : 0;
+    }
+    const __bf16* wrow[CIG];
+#pragma unroll
+    for (int c = 0; c < CIG; ++c)
+      wrow[c] = p.wt + (static_cast<int64_t>(tap) * p.Cin + ci0 + 16 * c + (l % 16)) * p.Cout + kq;
+    for (int co0 = 0; co0 < p.Cout; co0 += 32) {
+      bf16x8 a[2], b[CIG];
+#pragma unroll
+      for (int g = 0; g < 2; ++g) a[g] = inb[g] ? load8(p.dy + base[g] + co0) : zero8();
+#pragma unroll
+      for (int c = 0; c < CIG; ++c) b[c] = load8(wrow[c] + co0);
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int c = 0; c < CIG; ++c) acc[g][c] = mfma(a[g], b[c], acc[g][c]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CIG; ++c) {
+    const int ci = ci0 + 16 * c + (l % 16);
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + 16 * g + 4 * (l / 16) + r;
+        if (m < M) {
+          const int64_t at = static_cast<int64_t>(m) * p.Cin + ci;
+          float v = acc[g][c][r];
+          if (p.mask && !(static_cast<float>(p.x[at]) > 0.0f)) v = 0.0f;
+          p.dx[at] = static_cast<__bf16>(v);
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient for Cin % 32 == 0, Cout % 64 == 0. Block (split, tap * Cin/32
+// + ci tile, co tile): 64 output channels x 32 input channels of one tap over
+// kSplitPixels output pixels, in slabs of 32 pixels staged through LDS with
+// the pixels contiguous (the MFMA's K). Wave w owns channels 16w..16w+15.
+// ---------------------------------------------------------------------------
+constexpr int kLdsRow = 48;  // 32 pixels + 16 of padding: rows stay 16-byte aligned
+
+__global__ __launch_bounds__(kThreads) void k_conv_bwd_weight(ConvP p) {
+  __shared__ __attribute__((aligned(16))) __bf16 sdy[64][kLdsRow];  // [co][pixel]
+  __shared__ __attribute__((aligned(16))) __bf16 sx[32][kLdsRow];   // [ci][pixel]
+  const int t = threadIdx.x, wave = t >> 6, l = t & 63;
+  const int split = blockIdx.x;
+  const int ci_tiles = p.Cin / 32;
+  const int tap = blockIdx.y / ci_tiles, ci0 = (blockIdx.y % ci_tiles) * 32;
+  const int ky = tap / 3, kx = tap % 3;
+  const int co0 = blockIdx.z * 64;
+  const int M = p.N * p.Ho * p.Wo;
+  const int p_begin = split * kSplitPixels, p_end = min(M, p_begin + kSplitPixels);
+  const int kq = 8 * (l / 16);
+  f32x4 acc[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
+  for (int pb = p_begin; pb < p_end; pb += 32) {
+    {  // dy slab: 32 pixels x 64 channels, 8 channels per thread
+      const int px = t / 8, c8 = 8 * (t % 8);
+      const int m = pb + px;
+      bf16x8 v = m < p_end ? load8(p.dy + static_cast<int64_t>(m) * p.Cout + co0 + c8) : zero8();
+#pragma unroll
+      for (int j = 0; j < 8; ++j) sdy[c8 + j][px] = v[j];
+    }
+    if (t < 128) {  // x slab at this tap: 32 pixels x 32 channels
+      const int px = t / 4, c8 = 8 * (t % 4);
+      const int m = pb + px;
+      bf16x8 v = zero8();
+      if (m < p_end) {
+        const int ox = m % p.Wo, oy = (m / p.Wo) % p.Ho, n = m / (p.Wo * p.Ho);
+        const int iy = oy * p.S + ky - 1, ix = ox * p.S + kx - 1;
+        if (iy >= 0 && iy < p.H && ix >= 0 && ix < p.W)
+          v = load8(p.x + ((static_cast<int64_t>(n) * p.H + iy) * p.W + ix) * p.Cin + ci0 + c8);
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) sx[c8 + j][px] = v[j];
+    }
+    __syncthreads();
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sdy[16 * wave + (l % 16)][kq]);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sx[16 * h + (l % 16)][kq]);
+      acc[h] = mfma(a, b, acc[h]);
+    }
+    __syncthreads();
+  }
+  // partial[split][tap][co][ci]
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int co = co0 + 16 * wave + 4 * (l / 16) + r, ci = ci0 + 16 * h + (l % 16);
+      p.part[((static_cast<int64_t>(split) * 9 + tap) * p.Cout + co) * p.Cin + ci] = acc[h][r];
+    }
+}
+
+// Weight gradient of the one-channel first layer: thread (co, tap) of a block
+// sums its split of output pixels; partial[split][tap][co] (Cin = 1).
+__global__ __launch_bounds__(kThreads) void k_conv_bwd_weight_cin1(ConvP p) {
+  const int split = blockIdx.x;
+  const int M = p.N * p.Ho * p.Wo;
+  const int p_begin = split * kSplitPixels, p_end = min(M, p_begin + kSplitPixels);
+  for (int e = threadIdx.x; e < 9 * p.Cout; e += kThreads) {
+    const int co = e % p.Cout, tap = e / p.Cout;
+    const int ky = tap / 3, kx = tap % 3;
+    float acc = 0.0f;
+    for (int m = p_begin; m < p_end; ++m) {
+      const int ox = m % p.Wo, oy = (m / p.Wo) % p.Ho, n = m / (p.Wo * p.Ho);
+      const int iy = oy * p.S + ky - 1, ix = ox * p.S + kx - 1;
+      if (iy >= 0 && iy < p.H && ix >= 0 && ix < p.W)
+        acc = fmaf(static_cast<float>(p.dy[static_cast<int64_t>(m) * p.Cout + co]),
+                   static_cast<float>(p.x[(static_cast<int64_t>(n) * p.H + iy) * p.W + ix]), acc);
+    }
+    p.part[(static_cast<int64_t>(split) * 9 + tap) * p.Cout + co] = acc;
+  }
+}
+
+// dw[co][tap][ci] = sum over splits of partial[split][tap][co][ci], in split order.
+__global__ __launch_bounds__(kThreads) void k_sum_splits(const float* __restrict__ part, float* __restrict__ dw,
+                                                          int splits, int Cout, int Cin) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t per = static_cast<int64_t>(9) * Cout * Cin;
+  if (e >= per) return;
+  const int ci = static_cast<int>(e % Cin);
+  const int co = static_cast<int>((e / Cin) % Cout);
+  const int tap = static_cast<int>(e / (static_cast<int64_t>(Cin) * Cout));
+  float s = 0.0f;
+  for (int k = 0; k < splits; ++k) s += part[k * per + e];
+  dw[(static_cast<int64_t>(co) * 9 + tap) * Cin + ci] = s;
+}
+
+// wt[tap][ci][co] = w[co][tap][ci]
+__global__ __launch_bounds__(kThreads) void k_transpose_w(const __bf16* __restrict__ w, __bf16* __restrict__ wt,
+                                                           int Cout, int Cin) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (e >= static_cast<int64_t>(9) * Cout * Cin) return;
+  const int co = static_cast<int>(e % Cout);
+  const int ci = static_cast<int>((e / Cout) % Cin);
+  const int tap = static_cast<int>(e / (static_cast<int64_t>(Cout) * Cin));
+  wt[e] = w[(static_cast<int64_t>(co) * 9 + tap) * Cin + ci];
+}
+
+int status() {
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? NDRR_OK : static_cast<int>(e);
+}
+
+int out_size(int in, int stride) { return (in - 1) / stride + 1; }
+
+bool shape_ok(int N, int H, int W, int Cin, int Cout, int stride) {
+  if (N <= 0 || H <= 0 || W <= 0 || Cin <= 0 || Cout <= 0 || (stride != 1 && stride != 2)) return false;
+  if (Cout % 32 != 0 || (Cin != 1 && Cin % 32 != 0)) return false;
+  return true;
+}
+
+bool range_ok(int N, int H, int W, int C) {
+  return static_cast<int64_t>(N) * H * W * C < (int64_t{1} << 31);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ndrr_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y, int N, int H, int W, int Cin,
+                     int Cout, int stride, int relu, void* stream) {
+  if (!x || !w || !y || !shape_ok(N, H, W, Cin, Cout, stride)) return NDRR_EINVAL;
+  const int Ho = out_size(H, stride), Wo = out_size(W, stride);
+  if (!range_ok(N, H, W, Cin) || !range_ok(N, Ho, Wo, Cout)) return NDRR_ERANGE;
+  ConvP p{};
+  p.x = static_cast<const __bf16*>(x); p.w = static_cast<const __bf16*>(w); p.bias = bias;
+  p.y = static_cast<__bf16*>(y);
+  p.N = N; p.H = H; p.W = W; p.Cin = Cin; p.Ho = Ho; p.Wo = Wo; p.Cout = Cout; p.S = stride; p.relu = relu != 0;
+  const int M = N * Ho * Wo;
+  const int blocks = (M + 4 * kPixelsPerWave - 1) / (4 * kPixelsPerWave);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool wide = Cout % 64 == 0;
+  const dim3 grid(blocks, Cout / (wide ? 64 : 32));
+  if (Cin == 1) {
+    if (wide) hipLaunchKernelGGL((k_conv_fwd<4, true>), grid, dim3(kThreads), 0, st, p);
+    else hipLaunchKernelGGL((k_conv_fwd<2, true>), grid, dim3(kThreads), 0, st, p);
+  } else {
+    if (wide) hipLaunchKernelGGL((k_conv_fwd<4, false>), grid, dim3(kThreads), 0, st, p);
+    else hipLaunchKernelGGL((k_conv_fwd<2, false>), grid, dim3(kThreads), 0, st, p);
+  }
+  return status();
+}
+
+int ndrr_conv3x3_transpose_weights(const void* w, void* wt, int Cin, int Cout, void* stream) {
+  if (!w || !wt || Cin <= 0 || Cout <= 0) return NDRR_EINVAL;
+  const int64_t n = static_cast<int64_t>(9) * Cin * Cout;
+  if (n >= (int64_t{1} << 31)) return NDRR_ERANGE;
+  hipLaunchKernelGGL(k_transpose_w, dim3(static_cast<unsigned>((n + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const __bf16*>(w), static_cast<__bf16*>(wt),
+                     Cout, Cin);
+  return status();
+}
+
+int ndrr_conv3x3_bwd_data(const void* dy, const void* wt, const void* x, void* dx, int N, int H, int W, int Cin,
+                          int Cout, int stride, int relu_mask, void* stream) {
+  if (!dy || !wt || !dx || !shape_ok(N, H, W, Cin, Cout, stride) || Cin % 32 != 0) return NDRR_EINVAL;
+  if (relu_mask && !x) return NDRR_EINVAL;
+  const int Ho = out_size(H, stride), Wo = out_size(W, stride);
+  if (!range_ok(N, H, W, Cin) || !range_ok(N, Ho, Wo, Cout)) return NDRR_ERANGE;
+  ConvP p{};
+  p.dy = static_cast<const __bf16*>(dy); p.wt = static_cast<const __bf16*>(wt); p.x = static_cast<const __bf16*>(x);
+  p.dx = static_cast<__bf16*>(dx);
+  p.N = N; p.H = H; p.W = W; p.Cin = Cin; p.Ho = Ho; p.Wo = Wo; p.Cout = Cout; p.S = stride;
+  p.mask = relu_mask != 0;
+  const int M = N * H * W;
+  const int blocks = (M + 4 * kPixelsPerWave - 1) / (4 * kPixelsPerWave);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool wide = Cin % 64 == 0;
+  const dim3 grid(blocks, Cin / (wide ? 64 : 32));
+  if (wide) hipLaunchKernelGGL(k_conv_bwd_data<4>, grid, dim3(kThreads), 0, st, p);
+  else hipLaunchKernelGGL(k_conv_bwd_data<2>, grid, dim3(kThreads), 0, st, p);
+  return status();
+}
+
+size_t ndrr_conv3x3_bwd_weight_workspace_floats(int N, int H, int W, int Cin, int Cout, int stride) {
+  if (!shape_ok(N, H, W, Cin, Cout, stride)) return 0;
+  const int64_t M = static_cast<int64_t>(N) * out_size(H, stride) * out_size(W, stride);
+  const int64_t splits = (M + kSplitPixels - 1) / kSplitPixels;
+  return static_cast<size_t>(splits * 9 * Cout * Cin);
+}
+
+int ndrr_conv3x3_bwd_weight(const void* dy, const void* x, float* workspace, float* dw, int N, int H, int W,
+                            int Cin, int Cout, int stride, void* stream) {
+  if (!dy || !x || !workspace || !dw || !shape_ok(N, H, W, Cin, Cout, stride)) return NDRR_EINVAL;
+  if (Cin != 1 && Cout % 64 != 0) return NDRR_EINVAL;
+  const int Ho = out_size(H, stride), Wo = out_size(W, stride);
+  if (!range_ok(N, H, W, Cin) || !range_ok(N, Ho, Wo, Cout)) return NDRR_ERANGE;
+  ConvP p{};
+  p.dy = static_cast<const __bf16*>(dy); p.x = static_cast<const __bf16*>(x); p.part = workspace;
+  p.N = N; p.H = H; p.W = W; p.Cin = Cin; p.Ho = Ho; p.Wo = Wo; p.Cout = Cout; p.S = stride;
+  const int M = N * Ho * Wo;
+  const int splits = (M + kSplitPixels - 1) / kSplitPixels;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (Cin == 1) {
+    hipLaunchKernelGGL(k_conv_bwd_weight_cin1, dim3(splits), dim3(kThreads), 0, st, p);
+  } else {
+    hipLaunchKernelGGL(k_conv_bwd_weight, dim3(splits, 9 * (Cin / 32), Cout / 64), dim3(kThreads), 0, st, p);
+  }
+  int r = status();
+  if (r != NDRR_OK) return r;
+  const int64_t per = static_cast<int64_t>(9) * Cout * Cin;
+  hipLaunchKernelGGL(k_sum_splits, dim3(static_cast<unsigned>((per + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                     st, workspace, dw, splits, Cout, Cin);
+  return status();
+}
+
+}  // extern "C"
