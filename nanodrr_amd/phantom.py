@@ -153,10 +153,23 @@ def registration_problems(subject: Subject, batch: int, seed: int, target_rot_de
     P[:, :3, :3] = R
     P[:, :3, 3] = t + iso - (R @ iso[:, None])[..., 0]  # rotate about the isocentre, then shift
     rt_init = P @ rt_true
-    v2w = subject.voxel_to_world.detach().cpu().to(torch.float64)
-    D, H, W = subject.image.shape[-3:]
-    frac = torch.rand(n_fiducials, 3, generator=g, dtype=torch.float64) * 0.8 + 0.1
-    idx = frac * torch.tensor([W - 1, H - 1, D - 1], dtype=torch.float64)
-    fid = (v2w[:3, :3] @ idx.T).T + v2w[:3, 3]
+    fid = fiducials(subject, n_fiducials, g)
     k_inv, sdd_t = detector(sdd, 1.0, height, width, dtype=torch.float64)
     return RegistrationProblems(rt_true, rt_init, fid, k_inv, sdd_t, height, width)
+
+
+def fiducials(subject: Subject, n: int, generator: torch.Generator) -> torch.Tensor:
+    """(n, 3) float64 world points uniform in the central 80 % of the volume's box."""
+    v2w = subject.voxel_to_world.detach().cpu().to(torch.float64)
+    D, H, W = subject.image.shape[-3:]
+    frac = torch.rand(n, 3, generator=generator, dtype=torch.float64) * 0.8 + 0.1
+    idx = frac * torch.tensor([W - 1, H - 1, D - 1], dtype=torch.float64)
+    return (v2w[:3, :3] @ idx.T).T + v2w[:3, 3]
+
+
+def nominal_pose(subject: Subject, distance: float = 600.0, dtype=torch.float64) -> torch.Tensor:
+    """(1, 4, 4) AP camera-to-world pose with no rotation or offset, isocentre `distance` from the source."""
+    iso = subject.isocenter.detach().cpu().to(torch.float64)
+    rt = make_rt_inv(torch.zeros(1, 3, dtype=torch.float64), torch.tensor([[0.0, distance, 0.0]], dtype=torch.float64),
+                     "AP", isocenter=iso)
+    return rt.to(dtype)

@@ -147,3 +147,191 @@ class PoseNet(torch.nn.Module):
     def split(self, out: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
         """(rotation vector, translation in mm) of the network output."""
         return out[:, :3], out[:, 3:] * self.xyz_scale
+
+
+# ---------------------------------------------------------------------------
+# Training data, training loop and the registration comparison
+# ---------------------------------------------------------------------------
+
+
+class DrrSource:
+    """DRRs of seeded 256^3 bf16 phantoms at poses perturbed about the nominal AP
+    pose, in `Registration`'s parameterisation (rotation vector about the
+    isocentre, translation in mm), rendered by the native kernels."""
+
+    def __init__(self, seeds, size: int = 256, det: int = 128, det_mm: float = 200.0, n_samples: int = 256,
+                 device="cuda:0"):
+        from .phantom import detector, make_phantom, nominal_pose, phantom_subject
+        from .registration import Registration
+
+        self.regs = []
+        self.n_samples = n_samples
+        k_inv, sdd = detector(1020.0, det_mm / det, det, det)
+        for s in seeds:
+            subject = phantom_subject(make_phantom((size,) * 3, (1.0,) * 3, seed=s), 1, torch.bfloat16, device)
+            rt = nominal_pose(subject).float().to(device)
+            self.regs.append(Registration(subject, rt, k_inv.to(device), sdd.to(device), det, det))
+
+    @torch.no_grad()
+    def render(self, k: int, rot: torch.Tensor, xyz: torch.Tensor, offset: torch.Tensor | None = None,
+               n_samples: int | None = None) -> torch.Tensor:
+        """(B, 1, det, det) DRRs of phantom k at parameters (rot, xyz) (B, 3)."""
+        from .registration import pose_compose
+
+        reg = self.regs[k]
+        B = rot.shape[0]
+        S = n_samples or self.n_samples
+        M = pose_compose(reg._prefix.expand(B, 4, 4).contiguous(), reg._suffix.expand(B, 4, 4).contiguous(),
+                         rot.float().contiguous(), xyz.float().contiguous())
+        src, tgt = reg._src, reg._tgt
+        step = (tgt - src).norm(dim=-1) / float(S - 1)
+        if offset is not None:
+            d = (tgt - src) * (offset.reshape(-1, 1, 1) / float(S - 1))
+            src, tgt = src + d, tgt + d
+        from .renderer import march
+
+        return march(reg.subject, M, src, tgt, step, S, reg.height, reg.width)
+
+
+def sample_params(n: int, gen: torch.Generator, rot_deg: float, trans_mm: float, device):
+    rot = (torch.rand(n, 3, generator=gen, device=device) * 2 - 1) * (rot_deg * math.pi / 180.0)
+    xyz = (torch.rand(n, 3, generator=gen, device=device) * 2 - 1) * trans_mm
+    return rot, xyz
+
+
+def pose_loss(net: PoseNet, out, rot, xyz, sdd: float = 1020.0):
+    """MSE on the (rotation vector, translation / xyz_scale) pair plus the mean double
+    geodesic distance (angle x sdd/2 and translation, mm) / xyz_scale."""
+    from .geometry import convert
+    from .metrics import DoubleGeodesicSE3
+
+    rot_hat, xyz_hat = net.split(out)
+    mse = (rot_hat - rot).square().mean() + ((xyz_hat - xyz) / net.xyz_scale).square().mean()
+    T_hat = convert(rot_hat.double(), xyz_hat.double(), "so3_log")
+    T = convert(rot.double(), xyz.double(), "so3_log")
+    geo = DoubleGeodesicSE3(sdd)(T_hat, T)[2].mean().float() / net.xyz_scale
+    return mse + geo, mse.detach(), geo.detach()
+
+
+def train(steps: int = 20000, batch: int = 256, seeds=tuple(range(8)), rot_deg: float = 45.0, trans_mm: float = 60.0,
+          lr: float = 1e-3, seed: int = 0, device="cuda:0", log=print, log_every: int = 500):
+    """Train PoseNet on DRRs rendered on the fly (each step: batch / len(seeds) poses per
+    phantom, a fresh sub-step lattice offset per DRR). Returns (net, history)."""
+    import time
+
+    src = DrrSource(seeds, device=device)
+    net = PoseNet(seed=seed).to(device)
+    opt = torch.optim.Adam(net.parameters(), lr=lr)
+    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, steps, eta_min=lr * 0.02)
+    gen = torch.Generator(device=device).manual_seed(seed)
+    per = batch // len(seeds)
+    hist = []
+    t0 = time.perf_counter()
+    for it in range(steps):
+        rot, xyz = sample_params(per * len(seeds), gen, rot_deg, trans_mm, device)
+        off = torch.rand(per * len(seeds), generator=gen, device=device) - 0.5
+        drr = torch.cat([src.render(k, rot[k * per:(k + 1) * per], xyz[k * per:(k + 1) * per],
+                                    off[k * per:(k + 1) * per]) for k in range(len(seeds))])
+        loss, mse, geo = pose_loss(net, net(drr), rot, xyz)
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+        sched.step()
+        if (it + 1) % log_every == 0 or it == 0:
+            torch.cuda.synchronize()
+            h = dict(step=it + 1, loss=float(loss.detach()), mse=float(mse), geo=float(geo), s=time.perf_counter() - t0)
+            hist.append(h)
+            log(h)
+    torch.cuda.synchronize()
+    return net, hist, time.perf_counter() - t0
+
+
+@torch.no_grad()
+def predict(net: PoseNet, drr: torch.Tensor):
+    return net.split(net(drr))
+
+
+def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_deg: float = 30.0,
+                         trans_mm: float = 40.0, n_iters: int = 300, device="cuda:0", log=print):
+    """On n_test unseen phantoms, register one target each (200 x 200, targets at
+    +-rot_deg / +-trans_mm about the nominal pose, S = 1024 target DRR) from the
+    nominal pose and from the CNN's proposal (its input: the target pose rendered at
+    128 x 128); both runs share one batch of two poses. Reports mTRE < 1 mm rates,
+    median mTRE and final ZNCC per start."""
+    from .metrics import mtre, zncc
+    from .phantom import detector, fiducials, make_phantom, nominal_pose, phantom_subject
+    from .registration import Registration, register
+
+    res = {"nominal": [], "cnn": [], "proposal_mtre": [], "zncc": {"nominal": [], "cnn": []}}
+    k200, sdd = detector(1020.0, 1.0, 200, 200)
+    k200, sdd = k200.to(device), sdd.to(device)
+    for i in range(n_test):
+        g = torch.Generator().manual_seed(seed0 + i)
+        subject = phantom_subject(make_phantom((256,) * 3, (1.0,) * 3, seed=seed0 + i), 1, torch.bfloat16, device)
+        rt_nom = nominal_pose(subject).float().to(device)
+        rot_t = ((torch.rand(1, 3, generator=g) * 2 - 1) * (rot_deg * math.pi / 180.0)).to(device)
+        xyz_t = ((torch.rand(1, 3, generator=g) * 2 - 1) * trans_mm).to(device)
+        fid = fiducials(subject, 32, g).to(device)
+        # target and CNN input: the same pose through Registration's own parameterisation
+        tgt_reg = Registration(subject, rt_nom, k200, sdd, 200, 200, per_pose=True)
+        tgt_reg._rot.data.copy_(rot_t)
+        tgt_reg._xyz.data.copy_(xyz_t)
+        rt_true = tgt_reg.pose.detach()
+        target = tgt_reg(1024).detach()
+        src128 = DrrSource.__new__(DrrSource)
+        k128, sdd128 = detector(1020.0, 200.0 / 128, 128, 128)
+        src128.regs = [Registration(subject, rt_nom, k128.to(device), sdd128.to(device), 128, 128)]
+        src128.n_samples = 512
+        rot_p, xyz_p = predict(net, src128.render(0, rot_t, xyz_t))
+        reg = Registration(subject, rt_nom.expand(2, 4, 4).contiguous(), k200, sdd, 200, 200, per_pose=True)
+        reg._rot.data.copy_(torch.cat([torch.zeros_like(rot_p), rot_p]))
+        reg._xyz.data.copy_(torch.cat([torch.zeros_like(xyz_p), xyz_p]))
+        res["proposal_mtre"].append(float(mtre(reg.pose.detach()[1:], rt_true, fid)[0]))
+        score = register(reg, target.expand(2, -1, -1, -1).contiguous(), n_iters=n_iters, n_samples=256,
+                         jitter=True, seed=i)
+        e = mtre(reg.pose.detach(), rt_true.expand(2, 4, 4), fid)
+        res["nominal"].append(float(e[0]))
+        res["cnn"].append(float(e[1]))
+        res["zncc"]["nominal"].append(float(score[0]))
+        res["zncc"]["cnn"].append(float(score[1]))
+        if (i + 1) % 8 == 0:
+            log(f"{i + 1}/{n_test}: nominal<1mm {sum(x < 1 for x in res['nominal'])}, "
+                f"cnn<1mm {sum(x < 1 for x in res['cnn'])}")
+    t = torch.tensor
+    return {
+        "problems": n_test,
+        "converged_from_nominal": int((t(res["nominal"]) < 1).sum()),
+        "converged_from_cnn": int((t(res["cnn"]) < 1).sum()),
+        "median_mtre_mm_nominal": float(t(res["nominal"]).median()),
+        "median_mtre_mm_cnn": float(t(res["cnn"]).median()),
+        "median_mtre_mm_cnn_proposal": float(t(res["proposal_mtre"]).median()),
+        "median_zncc_nominal": float(t(res["zncc"]["nominal"]).median()),
+        "median_zncc_cnn": float(t(res["zncc"]["cnn"]).median()),
+    }
+
+
+def main(argv=None):
+    import argparse
+    import json
+    import sys
+
+    ap = argparse.ArgumentParser(description="Train the pose-proposal CNN and compare registration starts")
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--test", type=int, default=64)
+    ap.add_argument("--out", default=None, help="write the summary JSON here")
+    args = ap.parse_args(argv)
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    net, hist, seconds = train(args.steps, args.batch, log=log)
+    summary = {"train": {"steps": args.steps, "batch": args.batch, "seconds": seconds,
+                         "drr_per_s": args.steps * args.batch / seconds, "final": hist[-1]}}
+    summary["registration"] = compare_registration(net, n_test=args.test, log=log)
+    line = json.dumps(summary)
+    print(line, flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(line + "\n")
+
+
+if __name__ == "__main__":
+    main()

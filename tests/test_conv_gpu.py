@@ -129,3 +129,15 @@ def test_posenet_gradients_match_torch_autograd():
     for i in range(len(LAYERS)):
         g, r = net.weights[i].grad, ws[i].grad
         assert (g - r).abs().max() / r.abs().max() < 5e-2, i  # bf16 activation gradients
+
+
+def test_posenet_training_and_registration_pipeline_runs():
+    """A few training steps on rendered DRRs lower the loss, and the registration comparison
+    runs end to end (one phantom, a short loop)."""
+    from nanodrr_amd.posenet import compare_registration, train
+
+    net, hist, _ = train(steps=40, batch=32, seeds=(0, 1), log=lambda m: None, log_every=39)
+    assert hist[-1]["loss"] < hist[0]["loss"]
+    r = compare_registration(net, n_test=1, n_iters=10, log=lambda m: None)
+    assert r["problems"] == 1 and 0 <= r["converged_from_cnn"] <= 1
+    assert r["median_zncc_cnn"] > 0.5
