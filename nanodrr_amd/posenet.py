@@ -325,7 +325,8 @@ def main(argv=None):
     net, hist, seconds = train(args.steps, args.batch, log=log)
     summary = {"train": {"steps": args.steps, "batch": args.batch, "seconds": seconds,
                          "drr_per_s": args.steps * args.batch / seconds, "final": hist[-1]}}
-    summary["registration"] = compare_registration(net, n_test=args.test, log=log)
+    if args.test > 0:
+        summary["registration"] = compare_registration(net, n_test=args.test, log=log)
     line = json.dumps(summary)
     print(line, flush=True)
     if args.out:

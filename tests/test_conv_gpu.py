@@ -136,8 +136,10 @@ def test_posenet_training_and_registration_pipeline_runs():
     runs end to end (one phantom, a short loop)."""
     from nanodrr_amd.posenet import compare_registration, train
 
-    net, hist, _ = train(steps=40, batch=32, seeds=(0, 1), log=lambda m: None, log_every=39)
-    assert hist[-1]["loss"] < hist[0]["loss"]
+    net, hist, _ = train(steps=100, batch=32, seeds=(0, 1), log=lambda m: None, log_every=1)
+    first = sum(h["loss"] for h in hist[:10]) / 10
+    last = sum(h["loss"] for h in hist[-10:]) / 10
+    assert last < first, (first, last)
     r = compare_registration(net, n_test=1, n_iters=10, log=lambda m: None)
     assert r["problems"] == 1 and 0 <= r["converged_from_cnn"] <= 1
     assert r["median_zncc_cnn"] > 0.5
