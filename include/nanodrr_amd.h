@@ -299,7 +299,8 @@ int ndrr_zncc(const float* x, const float* y, int B, int N, float eps, float* sc
  *   ndrr_conv3x3_bwd_weight: dw[Cout][3][3][Cin] fp32 = sum over pixels of
  *     dy x-patch, through a workspace of
  *     ndrr_conv3x3_bwd_weight_workspace_floats floats (split partials added
- *     in a fixed order: deterministic); Cout a multiple of 64 unless Cin == 1.
+ *     in a fixed order: deterministic); Cout a multiple of 64, or 32 when
+ *     Cin == 1.
  */
 int ndrr_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y,
                      int N, int H, int W, int Cin, int Cout, int stride, int relu,

@@ -296,24 +296,68 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd_weight(ConvP p) {
     }
 }
 
-// Weight gradient of the one-channel first layer: thread (co, tap) of a block
-// sums its split of output pixels; partial[split][tap][co] (Cin = 1).
+// Weight gradient of the one-channel first layer (Cout == 32): a GEMM with the
+// 32 output channels as rows, the nine taps (padded to 16) as columns and the
+// pixels as K. Each wave stages its own 32-pixel slab of dy and of the nine
+// shifted input values through LDS per step (the block's four waves take
+// consecutive slabs), and the four wave partials are added in wave order;
+// partial[split][tap][co].
 __global__ __launch_bounds__(kThreads) void k_conv_bwd_weight_cin1(ConvP p) {
+  __shared__ __attribute__((aligned(16))) __bf16 sdy[4][32][kLdsRow];  // [wave][co][pixel]
+  __shared__ __attribute__((aligned(16))) __bf16 sx[4][16][kLdsRow];   // [wave][tap][pixel]
+  __shared__ float red[4][32][16];
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int split = blockIdx.x;
   const int M = p.N * p.Ho * p.Wo;
   const int p_begin = split * kSplitPixels, p_end = min(M, p_begin + kSplitPixels);
-  for (int e = threadIdx.x; e < 9 * p.Cout; e += kThreads) {
-    const int co = e % p.Cout, tap = e / p.Cout;
-    const int ky = tap / 3, kx = tap % 3;
-    float acc = 0.0f;
-    for (int m = p_begin; m < p_end; ++m) {
-      const int ox = m % p.Wo, oy = (m / p.Wo) % p.Ho, n = m / (p.Wo * p.Ho);
-      const int iy = oy * p.S + ky - 1, ix = ox * p.S + kx - 1;
-      if (iy >= 0 && iy < p.H && ix >= 0 && ix < p.W)
-        acc = fmaf(static_cast<float>(p.dy[static_cast<int64_t>(m) * p.Cout + co]),
-                   static_cast<float>(p.x[(static_cast<int64_t>(n) * p.H + iy) * p.W + ix]), acc);
+  const int kq = 8 * (l / 16);
+  f32x4 acc[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
+  for (int base = p_begin; base < p_end; base += 4 * 32) {
+    const int pb = base + 32 * wave;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {  // dy: 32 pixels x 32 channels, two 8-channel pieces per lane
+      const int idx = l + 64 * i, px = idx / 4, c8 = 8 * (idx % 4);
+      const int m = pb + px;
+      const bf16x8 v = m < p_end ? load8(p.dy + static_cast<int64_t>(m) * p.Cout + c8) : zero8();
+#pragma unroll
+      for (int j = 0; j < 8; ++j) sdy[wave][c8 + j][px] = v[j];
     }
-    p.part[(static_cast<int64_t>(split) * 9 + tap) * p.Cout + co] = acc;
+    {  // x at the nine taps of 32 pixels: lane (px, tap group of 8)
+      const int px = l % 32, t0 = 8 * (l / 32);
+      const int m = pb + px;
+      const bool in_range = m < p_end;
+      const int mm = in_range ? m : 0;
+      const int ox = mm % p.Wo, oy = (mm / p.Wo) % p.Ho, n = mm / (p.Wo * p.Ho);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int tap = t0 + j;
+        float v = 0.0f;
+        if (in_range && tap < 9) {
+          const int iy = oy * p.S + tap / 3 - 1, ix = ox * p.S + tap % 3 - 1;
+          if (iy >= 0 && iy < p.H && ix >= 0 && ix < p.W)
+            v = static_cast<float>(p.x[(static_cast<int64_t>(n) * p.H + iy) * p.W + ix]);
+        }
+        sx[wave][tap][px] = static_cast<__bf16>(v);
+      }
+    }
+    __syncthreads();
+    const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sx[wave][l % 16][kq]);
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sdy[wave][16 * g + (l % 16)][kq]);
+      acc[g] = mfma(a, b, acc[g]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int g = 0; g < 2; ++g)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) red[wave][16 * g + 4 * (l / 16) + r][l % 16] = acc[g][r];
+  __syncthreads();
+  for (int e = threadIdx.x; e < 32 * 9; e += kThreads) {
+    const int co = e % 32, tap = e / 32;
+    const float v = ((red[0][co][tap] + red[1][co][tap]) + red[2][co][tap]) + red[3][co][tap];
+    p.part[(static_cast<int64_t>(split) * 9 + tap) * p.Cout + co] = v;
   }
 }
 
@@ -428,7 +472,7 @@ size_t ndrr_conv3x3_bwd_weight_workspace_floats(int N, int H, int W, int Cin, in
 int ndrr_conv3x3_bwd_weight(const void* dy, const void* x, float* workspace, float* dw, int N, int H, int W,
                             int Cin, int Cout, int stride, void* stream) {
   if (!dy || !x || !workspace || !dw || !shape_ok(N, H, W, Cin, Cout, stride)) return NDRR_EINVAL;
-  if (Cin != 1 && Cout % 64 != 0) return NDRR_EINVAL;
+  if (Cin == 1 ? Cout != 32 : Cout % 64 != 0) return NDRR_EINVAL;
   const int Ho = out_size(H, stride), Wo = out_size(W, stride);
   if (!range_ok(N, H, W, Cin) || !range_ok(N, Ho, Wo, Cout)) return NDRR_ERANGE;
   ConvP p{};

@@ -105,7 +105,7 @@ class _ConvStack(torch.autograd.Function):
             cin, cout, s = LAYERS[i]
             x = acts[i]
             grads[2 * i] = conv3x3_bwd_weight(dz, x, s)
-            grads[2 * i + 1] = dz.float().sum(dim=(0, 1, 2))
+            grads[2 * i + 1] = dz.sum(dim=(0, 1, 2), dtype=torch.float32)
             if i > 0:
                 dz = conv3x3_bwd_data(dz, wbf[i], x, s, (x.shape[1], x.shape[2]))
         return (None, *grads)
