@@ -31,7 +31,7 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
 constexpr int kThreads = 256;      // four waves
 constexpr int kPixelsPerWave = 32; // two 16-row MFMA tiles
-constexpr int kSplitPixels = 4096; // pixels per weight-grad workgroup
+constexpr int kWgradBlocks = 2048; // weight-grad workgroups aimed at (the split size follows)
 
 struct ConvP {
   const __bf16* x;    // [N][H][W][Cin]
@@ -43,6 +43,7 @@ struct ConvP {
   __bf16* dx;         // [N][H][W][Cin]      (input grad)
   float* part;        // [splits][9][Cout][Cin] (weight grad)
   int N, H, W, Cin, Ho, Wo, Cout, S, relu, mask;
+  int split;          // output pixels per weight-grad workgroup
 };
 
 __device__ inline bf16x8 zero8() {
@@ -166,12 +167,22 @@ __global__ __launch_bounds__(kThreads) void k_conv_fwd(ConvP p) {
 // Input gradient: rows are input pixels, columns input channels (16 * CIG per
 // block), K over (tap, output channel). Output pixel of (iy, tap):
 // oy = (iy + 1 - ky) / S when that divides exactly and lies in [0, Ho).
+// With stride 2 only the taps with ky = iy + 1 (mod 2) (and likewise in x)
+// reach a pixel, so the pixels are taken per parity class (blockIdx.z =
+// 2 (iy % 2) + ix % 2): one or two taps per axis instead of three, and no
+// multiply of structural zeros.
 // ---------------------------------------------------------------------------
 template <int CIG>
 __global__ __launch_bounds__(kThreads) void k_conv_bwd_data(ConvP p) {
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int M = p.N * p.H * p.W;
+  // pixel class: stride 1 has one class (every pixel); stride 2 four parity classes
+  const int py = p.S == 2 ? static_cast<int>(blockIdx.z) >> 1 : 0;
+  const int px = p.S == 2 ? static_cast<int>(blockIdx.z) & 1 : 0;
+  const int Hc = p.S == 2 ? (p.H - py + 1) / 2 : p.H;
+  const int Wc = p.S == 2 ? (p.W - px + 1) / 2 : p.W;
+  const int M = p.N * Hc * Wc;
   const int m0 = (blockIdx.x * 4 + wave) * kPixelsPerWave;
+  if (m0 >= M) return;  // whole wave past this class (no barrier below)
   const int ci0 = blockIdx.y * 16 * CIG;
   const int kq = 8 * (l / 16);
   int pn[2], piy[2], pix[2];
@@ -181,42 +192,49 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd_data(ConvP p) {
     const int m = m0 + 16 * g + (l % 16);
     pv[g] = m < M;
     const int mm = pv[g] ? m : 0;
-    pix[g] = mm % p.W;
-    piy[g] = (mm / p.W) % p.H;
-    pn[g] = mm / (p.W * p.H);
+    const int cx = mm % Wc, cy = (mm / Wc) % Hc;
+    pn[g] = mm / (Wc * Hc);
+    pix[g] = p.S == 2 ? 2 * cx + px : cx;
+    piy[g] = p.S == 2 ? 2 * cy + py : cy;
   }
   f32x4 acc[2][CIG];
 #pragma unroll
   for (int g = 0; g < 2; ++g)
 #pragma unroll
     for (int c = 0; c < CIG; ++c) acc[g][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  for (int tap = 0; tap < 9; ++tap) {
-    const int ky = tap / 3, kx = tap % 3;
-    int64_t base[2];
-    bool inb[2];
+  // taps of this class: stride 1 all nine; stride 2 ky in {1} (even rows) or {0, 2} (odd rows)
+  const int ky_first = p.S == 2 ? (py == 0 ? 1 : 0) : 0, ky_step = p.S == 2 ? 2 : 1;
+  const int kx_first = p.S == 2 ? (px == 0 ? 1 : 0) : 0, kx_step = p.S == 2 ? 2 : 1;
+  for (int ky = ky_first; ky < 3; ky += ky_step) {
+    for (int kx = kx_first; kx < 3; kx += kx_step) {
+      const int tap = 3 * ky + kx;
+      int64_t base[2];
+      bool inb[2];
 #pragma unroll
-    for (int g = 0; g < 2; ++g) {
-      const int ty = piy[g] + 1 - ky, tx = pix[g] + 1 - kx;
-      const int oy = ty / p.S, ox = tx / p.S;
-      inb[g] = pv[g] && ty >= 0 && tx >= 0 && ty == oy * p.S && tx == ox * p.S && oy < p.Ho && ox < p.Wo;
-      base[g] = inb[g] ? ((static_cast<int64_t>(pn[g]) * p.Ho + oy) * p.Wo + ox) * p.Cout + kq : 0;
-    }
-    const __bf16* wrow[CIG];
+      for (int g = 0; g < 2; ++g) {
+        const int ty = piy[g] + 1 - ky, tx = pix[g] + 1 - kx;
+        const int oy = ty / p.S, ox = tx / p.S;
+        inb[g] = pv[g] && ty >= 0 && tx >= 0 && ty == oy * p.S && tx == ox * p.S && oy < p.Ho && ox < p.Wo;
+        base[g] = inb[g] ? ((static_cast<int64_t>(pn[g]) * p.Ho + oy) * p.Wo + ox) * p.Cout + kq : 0;
+      }
+      const __bf16* wrow[CIG];
 #pragma unroll
-    for (int c = 0; c < CIG; ++c)
-      wrow[c] = p.wt + (static_cast<int64_t>(tap) * p.Cin + ci0 + 16 * c + (l % 16)) * p.Cout + kq;
-    for (int co0 = 0; co0 < p.Cout; co0 += 32) {
-      bf16x8 a[2], b[CIG];
+      for (int c = 0; c < CIG; ++c)
+        wrow[c] = p.wt + (static_cast<int64_t>(tap) * p.Cin + ci0 + 16 * c + (l % 16)) * p.Cout + kq;
+      for (int co0 = 0; co0 < p.Cout; co0 += 32) {
+        bf16x8 a[2], b[CIG];
 #pragma unroll
-      for (int g = 0; g < 2; ++g) a[g] = inb[g] ? load8(p.dy + base[g] + co0) : zero8();
+        for (int g = 0; g < 2; ++g) a[g] = inb[g] ? load8(p.dy + base[g] + co0) : zero8();
 #pragma unroll
-      for (int c = 0; c < CIG; ++c) b[c] = load8(wrow[c] + co0);
+        for (int c = 0; c < CIG; ++c) b[c] = load8(wrow[c] + co0);
 #pragma unroll
-      for (int g = 0; g < 2; ++g)
+        for (int g = 0; g < 2; ++g)
 #pragma unroll
-        for (int c = 0; c < CIG; ++c) acc[g][c] = mfma(a[g], b[c], acc[g][c]);
+          for (int c = 0; c < CIG; ++c) acc[g][c] = mfma(a[g], b[c], acc[g][c]);
+      }
     }
   }
+  // D rows 4 (l / 16) + r of each 16-row group; their pixels within the class
 #pragma unroll
   for (int c = 0; c < CIG; ++c) {
     const int ci = ci0 + 16 * c + (l % 16);
@@ -226,7 +244,9 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd_data(ConvP p) {
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + 16 * g + 4 * (l / 16) + r;
         if (m < M) {
-          const int64_t at = static_cast<int64_t>(m) * p.Cin + ci;
+          const int cx = m % Wc, cy = (m / Wc) % Hc, n = m / (Wc * Hc);
+          const int ix = p.S == 2 ? 2 * cx + px : cx, iy = p.S == 2 ? 2 * cy + py : cy;
+          const int64_t at = ((static_cast<int64_t>(n) * p.H + iy) * p.W + ix) * p.Cin + ci;
           float v = acc[g][c][r];
           if (p.mask && !(static_cast<float>(p.x[at]) > 0.0f)) v = 0.0f;
           p.dx[at] = static_cast<__bf16>(v);
@@ -236,64 +256,70 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd_data(ConvP p) {
 }
 
 // ---------------------------------------------------------------------------
-// Weight gradient for Cin % 32 == 0, Cout % 64 == 0. Block (split, tap * Cin/32
-// + ci tile, co tile): 64 output channels x 32 input channels of one tap over
-// kSplitPixels output pixels, in slabs of 32 pixels staged through LDS with
-// the pixels contiguous (the MFMA's K). Wave w owns channels 16w..16w+15.
+// Weight gradient for Cin % 32 == 0, Cout % 64 == 0. Block (split, ci tile,
+// co tile): 64 output channels x 32 input channels x all nine taps over
+// p.split output pixels, in slabs of 32 pixels staged through LDS with the
+// pixels contiguous (the MFMA's K): the dy slab once, the input slab once per
+// tap. Wave w owns output channels 16w..16w+15 (18 MFMAs per slab).
 // ---------------------------------------------------------------------------
 constexpr int kLdsRow = 48;  // 32 pixels + 16 of padding: rows stay 16-byte aligned
 
 __global__ __launch_bounds__(kThreads) void k_conv_bwd_weight(ConvP p) {
-  __shared__ __attribute__((aligned(16))) __bf16 sdy[64][kLdsRow];  // [co][pixel]
-  __shared__ __attribute__((aligned(16))) __bf16 sx[32][kLdsRow];   // [ci][pixel]
+  __shared__ __attribute__((aligned(16))) __bf16 sdy[64][kLdsRow];     // [co][pixel]
+  __shared__ __attribute__((aligned(16))) __bf16 sx[9][32][kLdsRow];   // [tap][ci][pixel]
   const int t = threadIdx.x, wave = t >> 6, l = t & 63;
   const int split = blockIdx.x;
-  const int ci_tiles = p.Cin / 32;
-  const int tap = blockIdx.y / ci_tiles, ci0 = (blockIdx.y % ci_tiles) * 32;
-  const int ky = tap / 3, kx = tap % 3;
-  const int co0 = blockIdx.z * 64;
+  const int ci0 = blockIdx.y * 32, co0 = blockIdx.z * 64;
   const int M = p.N * p.Ho * p.Wo;
-  const int p_begin = split * kSplitPixels, p_end = min(M, p_begin + kSplitPixels);
+  const int p_begin = split * p.split, p_end = min(M, p_begin + p.split);
   const int kq = 8 * (l / 16);
-  f32x4 acc[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
+  f32x4 acc[9][2];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) acc[k][0] = acc[k][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   for (int pb = p_begin; pb < p_end; pb += 32) {
     {  // dy slab: 32 pixels x 64 channels, 8 channels per thread
       const int px = t / 8, c8 = 8 * (t % 8);
       const int m = pb + px;
-      bf16x8 v = m < p_end ? load8(p.dy + static_cast<int64_t>(m) * p.Cout + co0 + c8) : zero8();
+      const bf16x8 v = m < p_end ? load8(p.dy + static_cast<int64_t>(m) * p.Cout + co0 + c8) : zero8();
 #pragma unroll
       for (int j = 0; j < 8; ++j) sdy[c8 + j][px] = v[j];
     }
-    if (t < 128) {  // x slab at this tap: 32 pixels x 32 channels
-      const int px = t / 4, c8 = 8 * (t % 4);
+    {  // input slabs of the nine taps: 32 pixels x 32 channels each; thread (pixel, 8 channels, tap parity)
+      const int px = (t % 128) / 4, c8 = 8 * (t % 4), half = t / 128;
       const int m = pb + px;
-      bf16x8 v = zero8();
-      if (m < p_end) {
-        const int ox = m % p.Wo, oy = (m / p.Wo) % p.Ho, n = m / (p.Wo * p.Ho);
-        const int iy = oy * p.S + ky - 1, ix = ox * p.S + kx - 1;
-        if (iy >= 0 && iy < p.H && ix >= 0 && ix < p.W)
+      const bool in_range = m < p_end;
+      const int mm = in_range ? m : 0;
+      const int ox = mm % p.Wo, oy = (mm / p.Wo) % p.Ho, n = mm / (p.Wo * p.Ho);
+      for (int tap = half; tap < 9; tap += 2) {
+        const int iy = oy * p.S + tap / 3 - 1, ix = ox * p.S + tap % 3 - 1;
+        bf16x8 v = zero8();
+        if (in_range && iy >= 0 && iy < p.H && ix >= 0 && ix < p.W)
           v = load8(p.x + ((static_cast<int64_t>(n) * p.H + iy) * p.W + ix) * p.Cin + ci0 + c8);
-      }
 #pragma unroll
-      for (int j = 0; j < 8; ++j) sx[c8 + j][px] = v[j];
+        for (int j = 0; j < 8; ++j) sx[tap][c8 + j][px] = v[j];
+      }
     }
     __syncthreads();
     const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sdy[16 * wave + (l % 16)][kq]);
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sx[16 * h + (l % 16)][kq]);
-      acc[h] = mfma(a, b, acc[h]);
-    }
+    for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sx[tap][16 * h + (l % 16)][kq]);
+        acc[tap][h] = mfma(a, b, acc[tap][h]);
+      }
     __syncthreads();
   }
   // partial[split][tap][co][ci]
 #pragma unroll
-  for (int h = 0; h < 2; ++h)
+  for (int tap = 0; tap < 9; ++tap)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int co = co0 + 16 * wave + 4 * (l / 16) + r, ci = ci0 + 16 * h + (l % 16);
-      p.part[((static_cast<int64_t>(split) * 9 + tap) * p.Cout + co) * p.Cin + ci] = acc[h][r];
-    }
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int co = co0 + 16 * wave + 4 * (l / 16) + r, ci = ci0 + 16 * h + (l % 16);
+        p.part[((static_cast<int64_t>(split) * 9 + tap) * p.Cout + co) * p.Cin + ci] = acc[tap][h][r];
+      }
 }
 
 // Weight gradient of the one-channel first layer (Cout == 32): a GEMM with the
@@ -309,7 +335,7 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd_weight_cin1(ConvP p) {
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int split = blockIdx.x;
   const int M = p.N * p.Ho * p.Wo;
-  const int p_begin = split * kSplitPixels, p_end = min(M, p_begin + kSplitPixels);
+  const int p_begin = split * p.split, p_end = min(M, p_begin + p.split);
   const int kq = 8 * (l / 16);
   f32x4 acc[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
   for (int base = p_begin; base < p_end; base += 4 * 32) {
@@ -361,18 +387,25 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd_weight_cin1(ConvP p) {
   }
 }
 
-// dw[co][tap][ci] = sum over splits of partial[split][tap][co][ci], in split order.
+// dw[co][tap][ci] = sum over splits of partial[split][tap][co][ci]: a block of
+// 8 elements x 32 lanes; lane j adds splits j, j + 32, ... in order, then a
+// fixed butterfly over the 32 lanes. Deterministic.
 __global__ __launch_bounds__(kThreads) void k_sum_splits(const float* __restrict__ part, float* __restrict__ dw,
                                                           int splits, int Cout, int Cin) {
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t per = static_cast<int64_t>(9) * Cout * Cin;
-  if (e >= per) return;
-  const int ci = static_cast<int>(e % Cin);
-  const int co = static_cast<int>((e / Cin) % Cout);
-  const int tap = static_cast<int>(e / (static_cast<int64_t>(Cin) * Cout));
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * 8 + threadIdx.x / 32;
+  const int j = threadIdx.x % 32;
   float s = 0.0f;
-  for (int k = 0; k < splits; ++k) s += part[k * per + e];
-  dw[(static_cast<int64_t>(co) * 9 + tap) * Cin + ci] = s;
+  if (e < per)
+    for (int k = j; k < splits; k += 32) s += part[k * per + e];
+#pragma unroll
+  for (int o = 16; o > 0; o >>= 1) s += __shfl_xor(s, o, 32);
+  if (e < per && j == 0) {
+    const int ci = static_cast<int>(e % Cin);
+    const int co = static_cast<int>((e / Cin) % Cout);
+    const int tap = static_cast<int>(e / (static_cast<int64_t>(Cin) * Cout));
+    dw[(static_cast<int64_t>(co) * 9 + tap) * Cin + ci] = s;
+  }
 }
 
 // wt[tap][ci][co] = w[co][tap][ci]
@@ -397,6 +430,16 @@ bool shape_ok(int N, int H, int W, int Cin, int Cout, int stride) {
   if (N <= 0 || H <= 0 || W <= 0 || Cin <= 0 || Cout <= 0 || (stride != 1 && stride != 2)) return false;
   if (Cout % 32 != 0 || (Cin != 1 && Cin % 32 != 0)) return false;
   return true;
+}
+
+// Output pixels per weight-grad workgroup: enough workgroups to fill the chip
+// (about kWgradBlocks over the channel tiles), in whole 128-pixel steps.
+int wgrad_split(int M, int Cin, int Cout) {
+  const int tiles = Cin == 1 ? 1 : (Cin / 32) * (Cout / 64);
+  const int want = (kWgradBlocks + tiles - 1) / tiles;
+  int per = (M + want - 1) / want;
+  per = ((per + 127) / 128) * 128;
+  return per < 256 ? 256 : per;
 }
 
 bool range_ok(int N, int H, int W, int C) {
@@ -452,11 +495,12 @@ int ndrr_conv3x3_bwd_data(const void* dy, const void* wt, const void* x, void* d
   p.dx = static_cast<__bf16*>(dx);
   p.N = N; p.H = H; p.W = W; p.Cin = Cin; p.Ho = Ho; p.Wo = Wo; p.Cout = Cout; p.S = stride;
   p.mask = relu_mask != 0;
-  const int M = N * H * W;
+  // pixels per class: all of them (stride 1), or the largest parity class (stride 2)
+  const int M = stride == 2 ? N * ((H + 1) / 2) * ((W + 1) / 2) : N * H * W;
   const int blocks = (M + 4 * kPixelsPerWave - 1) / (4 * kPixelsPerWave);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const bool wide = Cin % 64 == 0;
-  const dim3 grid(blocks, Cin / (wide ? 64 : 32));
+  const dim3 grid(blocks, Cin / (wide ? 64 : 32), stride == 2 ? 4 : 1);
   if (wide) hipLaunchKernelGGL(k_conv_bwd_data<4>, grid, dim3(kThreads), 0, st, p);
   else hipLaunchKernelGGL(k_conv_bwd_data<2>, grid, dim3(kThreads), 0, st, p);
   return status();
@@ -465,7 +509,9 @@ int ndrr_conv3x3_bwd_data(const void* dy, const void* wt, const void* x, void* d
 size_t ndrr_conv3x3_bwd_weight_workspace_floats(int N, int H, int W, int Cin, int Cout, int stride) {
   if (!shape_ok(N, H, W, Cin, Cout, stride)) return 0;
   const int64_t M = static_cast<int64_t>(N) * out_size(H, stride) * out_size(W, stride);
-  const int64_t splits = (M + kSplitPixels - 1) / kSplitPixels;
+  if (M >= (int64_t{1} << 31)) return 0;
+  const int per = wgrad_split(static_cast<int>(M), Cin, Cout);
+  const int64_t splits = (M + per - 1) / per;
   return static_cast<size_t>(splits * 9 * Cout * Cin);
 }
 
@@ -479,18 +525,19 @@ int ndrr_conv3x3_bwd_weight(const void* dy, const void* x, float* workspace, flo
   p.dy = static_cast<const __bf16*>(dy); p.x = static_cast<const __bf16*>(x); p.part = workspace;
   p.N = N; p.H = H; p.W = W; p.Cin = Cin; p.Ho = Ho; p.Wo = Wo; p.Cout = Cout; p.S = stride;
   const int M = N * Ho * Wo;
-  const int splits = (M + kSplitPixels - 1) / kSplitPixels;
+  p.split = wgrad_split(M, Cin, Cout);
+  const int splits = (M + p.split - 1) / p.split;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (Cin == 1) {
     hipLaunchKernelGGL(k_conv_bwd_weight_cin1, dim3(splits), dim3(kThreads), 0, st, p);
   } else {
-    hipLaunchKernelGGL(k_conv_bwd_weight, dim3(splits, 9 * (Cin / 32), Cout / 64), dim3(kThreads), 0, st, p);
+    hipLaunchKernelGGL(k_conv_bwd_weight, dim3(splits, Cin / 32, Cout / 64), dim3(kThreads), 0, st, p);
   }
   int r = status();
   if (r != NDRR_OK) return r;
   const int64_t per = static_cast<int64_t>(9) * Cout * Cin;
-  hipLaunchKernelGGL(k_sum_splits, dim3(static_cast<unsigned>((per + kThreads - 1) / kThreads)), dim3(kThreads), 0,
-                     st, workspace, dw, splits, Cout, Cin);
+  hipLaunchKernelGGL(k_sum_splits, dim3(static_cast<unsigned>((per + 7) / 8)), dim3(kThreads), 0, st, workspace, dw,
+                     splits, Cout, Cin);
   return status();
 }
 
