@@ -273,6 +273,22 @@ int ndrr_pose_compose_bwd(const double* prefix, const double* suffix, const floa
                           float* grot, float* gxyz, int B, void* stream);
 
 /*
+ * One Adam step in place on n fp32 parameters (param, first and second
+ * moments m, v) for their gradient: m = b1 m + (1 - b1) g, v = b2 v +
+ * (1 - b2) g^2, param -= lr (m / bc1) / (sqrt(v / bc2) + eps), with the bias
+ * corrections bc = 1 - beta^t and the step's learning rate from the caller.
+ * The betas are float64 so that 1 - beta is formed before rounding to fp32
+ * (as torch.optim.Adam does with its Python floats).
+ * The update of the registration loop's per-problem parameters in one
+ * launch. No counterpart in the reference: its Registration module
+ * (src/nanodrr/registration/registration.py:9-84) leaves the optimisation
+ * loop to the caller.
+ */
+int ndrr_adam_step(float* param, const float* grad, float* m, float* v, int n, float lr,
+                   double beta1, double beta2, float eps, float bias_correction1,
+                   float bias_correction2, void* stream);
+
+/*
  * Zero-normalised cross-correlation of B image pairs of N pixels (fp32):
  * score[b] = mean(xh * yh), xh = (x - mean x) / sqrt(var x + eps) (population
  * variance), yh likewise; grad_x (optional, [B][N]) = d score[b] / d x.

@@ -66,6 +66,8 @@ SIGNATURES: dict[str, tuple] = {
     "ndrr_conv3x3_bwd_weight": (_c_int, [_c_p, _c_p, _c_p, _c_p] + [_c_int] * 6 + [_c_p]),
     "ndrr_pose_compose": (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_int, _c_p]),
     "ndrr_pose_compose_bwd": (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_int, _c_p]),
+    "ndrr_adam_step": (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_int, ctypes.c_float, ctypes.c_double, ctypes.c_double]
+                       + [ctypes.c_float] * 3 + [_c_p]),
     "ndrr_zncc": (_c_int, [_c_p, _c_p, _c_int, _c_int, ctypes.c_float, _c_p, _c_p, _c_p]),
 }
 

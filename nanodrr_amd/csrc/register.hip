@@ -210,6 +210,23 @@ __global__ __launch_bounds__(kThreads) void k_zncc(const float* __restrict__ x, 
   }
 }
 
+// One Adam step in place over n parameters: m = b1 m + (1 - b1) g,
+// v = b2 v + (1 - b2) g^2, p -= lr (m / bc1) / (sqrt(v / bc2) + eps), where
+// bc1 = 1 - b1^t and bc2 = 1 - b2^t come from the host with the step's lr.
+__global__ __launch_bounds__(kThreads) void k_adam(float* __restrict__ p, const float* __restrict__ g,
+                                                   float* __restrict__ m, float* __restrict__ v, int n, float lr,
+                                                   float b1, float c1, float b2, float c2, float eps, float bc1,
+                                                   float bc2) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = m[i] * b1 + c1 * gi;  // c = 1 - b, rounded from float64 (1 - 0.999f is 1.3e-5 off)
+  const float vi = v[i] * b2 + c2 * (gi * gi);
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
 int status() {
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? NDRR_OK : static_cast<int>(e);
@@ -235,6 +252,18 @@ int ndrr_pose_compose_bwd(const double* prefix, const double* suffix, const floa
   const int blocks = (B + 63) / 64;
   hipLaunchKernelGGL(k_pose_compose_bwd, dim3(blocks), dim3(64), 0, static_cast<hipStream_t>(stream), prefix,
                      suffix, rot, xyz, gM, grot, gxyz, B, param_stride);
+  return status();
+}
+
+int ndrr_adam_step(float* param, const float* grad, float* m, float* v, int n, float lr, double beta1, double beta2,
+                   float eps, float bias_correction1, float bias_correction2, void* stream) {
+  if (!param || !grad || !m || !v || n <= 0 || !(bias_correction1 > 0.0f) || !(bias_correction2 > 0.0f))
+    return NDRR_EINVAL;
+  if (!(beta1 >= 0.0 && beta1 < 1.0) || !(beta2 >= 0.0 && beta2 < 1.0)) return NDRR_EINVAL;
+  hipLaunchKernelGGL(k_adam, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, static_cast<hipStream_t>(stream),
+                     param, grad, m, v, n, lr, static_cast<float>(beta1), static_cast<float>(1.0 - beta1),
+                     static_cast<float>(beta2), static_cast<float>(1.0 - beta2), eps, bias_correction1,
+                     bias_correction2);
   return status();
 }
 

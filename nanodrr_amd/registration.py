@@ -148,11 +148,20 @@ class Registration(torch.nn.Module):
         if backend not in ("auto", "hip", "triton"):
             raise ValueError(f"Unknown backend {backend!r}; expected 'auto', 'hip' or 'triton'")
         src, tgt = self._src, self._tgt
-        step = (tgt - src).norm(dim=-1) / float(n_samples - 1)
+        step, delta = self._ray_steps(n_samples)
         if offset is not None:
-            d = (tgt - src) * (offset.reshape(-1, 1, 1).to(tgt.dtype) / float(n_samples - 1))
-            src, tgt = src + d, tgt + d
+            u = offset.reshape(-1, 1, 1).to(tgt.dtype)
+            src, tgt = torch.addcmul(src, delta, u), torch.addcmul(tgt, delta, u)
         return march(self.subject, self.camera_to_voxel(), src, tgt, step, n_samples, self.height, self.width)
+
+    def _ray_steps(self, n_samples: int):
+        """(step (1, N), per-sample ray vector (1, N, 3)) for n_samples, kept until the rays change."""
+        key = (n_samples, self._tgt.data_ptr(), self._tgt._version)
+        if getattr(self, "_steps_key", None) != key or torch.compiler.is_compiling():
+            d = self._tgt - self._src
+            self._steps = (d.norm(dim=-1) / float(n_samples - 1), d / float(n_samples - 1))
+            self._steps_key = key
+        return self._steps
 
     @property
     def pose(self) -> torch.Tensor:
@@ -172,8 +181,8 @@ class Registration(torch.nn.Module):
 
 
 class _Adam:
-    """Adam over per-group tensors with a device-side step counter and learning-rate
-    schedule, so that a captured step replays with the right rates."""
+    """Adam with register()'s cosine schedule in torch operations, for CPU tensors:
+    the statement of the update that register() makes with ndrr_adam_step."""
 
     def __init__(self, groups, n_steps: int, betas=(0.9, 0.999), eps=1e-8, final_frac: float = 0.02):
         self.groups = groups  # [(param, lr)]
@@ -204,15 +213,25 @@ class _Adam:
             p.sub_((lr * frac) * (m / bc1) / ((v / bc2).sqrt() + self.eps))
 
 
+def _adam_step(p: torch.Tensor, g: torch.Tensor, m: torch.Tensor, v: torch.Tensor, lr: float, b1: float, b2: float,
+               eps: float, t: int) -> None:
+    """One native Adam step (ndrr_adam_step) on a contiguous fp32 device parameter."""
+    g = g.contiguous()
+    with torch.cuda.device(p.device):
+        rc = _lib.lib().ndrr_adam_step(p.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(), p.numel(), lr, b1, b2,
+                                       eps, 1.0 - b1 ** t, 1.0 - b2 ** t, _stream(p.device))
+    _lib.check(rc, "ndrr_adam_step")
+
+
 def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot: float = 1e-2, lr_xyz: float = 1.0,
              n_samples: int = 256, jitter: bool = False, seed: int = 0, graph: bool = False,
              iters_per_graph: int = 10, eps: float = 1e-7) -> torch.Tensor:
     """Optimise every pose of `reg` to maximise the ZNCC of its DRRs with `target` (B, 1, H, W).
 
     Each iteration: pose step, render, ZNCC, backward, Adam update (learning
-    rates lr_rot rad and lr_xyz mm on a cosine schedule down to 2 %); every
-    problem of the batch advances in the same launches. Returns the ZNCC of
-    the last iteration's DRRs (B,).
+    rates lr_rot rad and lr_xyz mm on a cosine schedule down to 2 %, one native
+    launch per parameter tensor); every problem of the batch advances in the
+    same launches. Returns the ZNCC of the last iteration's DRRs (B,).
 
     A DRR sampled every `step` mm along rays fixed to the camera repeats itself
     when the camera moves one step along its axis, so the similarity to a target
@@ -229,20 +248,25 @@ def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot
         raise NotImplementedError(
             "HIP-graph capture of the registration step is switched off (a ten-iteration capture crashed in "
             "the runtime's capture end with no cause found); call register(..., graph=False)")
-    params = [reg._rot, reg._xyz]
-    for p in params:
-        p.grad = torch.zeros_like(p)
-    opt = _Adam([(reg._rot, lr_rot), (reg._xyz, lr_xyz)], n_iters)
+    params = [(reg._rot, lr_rot), (reg._xyz, lr_xyz)]
+    for p, _ in params:
+        if not (p.is_cuda and p.dtype == torch.float32 and p.is_contiguous()):
+            raise ValueError("register() updates contiguous float32 device parameters")
+    moments = [(torch.zeros_like(p), torch.zeros_like(p)) for p, _ in params]
+    b1, b2, adam_eps, final_frac = 0.9, 0.999, 1e-8, 0.02
     target = target.detach().to(torch.float32).contiguous()
     B = target.shape[0]
     score = torch.zeros(B, device=target.device)
     gen = torch.Generator(device=target.device).manual_seed(int(seed)) if jitter else None
-    for _ in range(n_iters):
-        for p in params:
-            p.grad.zero_()
+    for it in range(n_iters):
+        for p, _ in params:
+            p.grad = None
         offset = torch.rand(B, device=target.device, generator=gen) - 0.5 if jitter else None
         s = zncc(reg(n_samples, offset=offset), target, eps)
         (-s.sum()).backward()
-        opt.step()
+        frac = final_frac + (1 - final_frac) * 0.5 * (1 + math.cos(math.pi * it / max(n_iters - 1, 1)))
+        with torch.no_grad():
+            for (p, lr), (m, v) in zip(params, moments):
+                _adam_step(p, p.grad, m, v, lr * frac, b1, b2, adam_eps, it + 1)
         score.copy_(s.detach())
     return score.clone()

@@ -261,6 +261,30 @@ def test_registration_trajectory_matches_oracle_renderer():
     assert d_rot < 1e-3 and d_xyz < 0.1
 
 
+def test_native_adam_step_matches_torch_optim_adam():
+    """ndrr_adam_step over five steps (learning rate changing per step) == torch.optim.Adam."""
+    from nanodrr_amd.registration import _adam_step
+
+    g = torch.Generator().manual_seed(4)
+    p0 = torch.randn(37, 3, generator=g)
+    grads = [torch.randn(37, 3, generator=g) * 10 ** (k - 2) for k in range(5)]
+    p = p0.to(DEV).contiguous()
+    m, v = torch.zeros_like(p), torch.zeros_like(p)
+    ref = p0.clone().requires_grad_(True)
+    opt = torch.optim.Adam([ref], lr=1e-2, betas=(0.9, 0.999), eps=1e-8)
+    for k, gk in enumerate(grads):
+        lr = 1e-2 * (1.0 - 0.1 * k)
+        _adam_step(p, gk.to(DEV), m, v, lr, 0.9, 0.999, 1e-8, k + 1)
+        opt.param_groups[0]["lr"] = lr
+        ref.grad = gk.clone()
+        opt.step()
+    # the same update in another order of fp32 operations: a few roundings of each quantity's
+    # largest entry per step (|p| < 4, ulp 2.4e-7), not of each entry (some end near zero)
+    for mine, theirs in ((p, ref.detach()), (m, opt.state[ref]["exp_avg"]), (v, opt.state[ref]["exp_avg_sq"])):
+        err = (mine.cpu() - theirs).abs().max() / theirs.abs().max()
+        assert err < 1e-6, err
+
+
 def test_sample_lattice_offset():
     """offset=0 renders exactly as render(); an offset u slides src and tgt by u steps."""
     from nanodrr_amd import render
