@@ -292,11 +292,14 @@ int ndrr_adam_step(float* param, const float* grad, float* m, float* v, int n, f
  * Zero-normalised cross-correlation of B image pairs of N pixels (fp32):
  * score[b] = mean(xh * yh), xh = (x - mean x) / sqrt(var x + eps) (population
  * variance), yh likewise; grad_x (optional, [B][N]) = d score[b] / d x.
- * Deterministic (fixed-order reductions). Replaces
+ * Two launches over (4096-pixel chunk, image): chunk moments into
+ * `workspace` (ndrr_zncc_workspace_floats(B, N) floats), then their merge and
+ * the gradient. Deterministic (fixed-order reductions). Replaces
  * NormalizedCrossCorrelation2d.forward, src/nanodrr/metrics/ncc.py:16-37.
  */
-int ndrr_zncc(const float* x, const float* y, int B, int N, float eps, float* score,
-              float* grad_x, void* stream);
+int64_t ndrr_zncc_workspace_floats(int B, int N);
+int ndrr_zncc(const float* x, const float* y, int B, int N, float eps, float* workspace,
+              float* score, float* grad_x, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Pose-proposal CNN: 3x3 convolutions on the bf16 matrix cores
@@ -313,10 +316,12 @@ int ndrr_zncc(const float* x, const float* y, int B, int N, float eps, float* sc
  *   ndrr_conv3x3_bwd_data: dx = conv^T(dy, w), times [x > 0] when relu_mask
  *     (the ReLU of the layer that produced x); Cin a multiple of 32;
  *   ndrr_conv3x3_bwd_weight: dw[Cout][3][3][Cin] fp32 = sum over pixels of
- *     dy x-patch, through a workspace of
+ *     dy x-patch and, when dbias is not NULL, dbias[Cout] fp32 = sum over
+ *     pixels of dy, through a 16-byte aligned workspace of
  *     ndrr_conv3x3_bwd_weight_workspace_floats floats (split partials added
  *     in a fixed order: deterministic); Cout a multiple of 64, or 32 when
- *     Cin == 1.
+ *     Cin == 1. The environment variable NDRR_WGRAD_BLOCKS (64-65536,
+ *     default 1024) sets the number of workgroups the pixels are split over.
  */
 int ndrr_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y,
                      int N, int H, int W, int Cin, int Cout, int stride, int relu,
@@ -328,8 +333,8 @@ int ndrr_conv3x3_bwd_data(const void* dy, const void* wt, const void* x, void* d
 size_t ndrr_conv3x3_bwd_weight_workspace_floats(int N, int H, int W, int Cin, int Cout,
                                                 int stride);
 int ndrr_conv3x3_bwd_weight(const void* dy, const void* x, float* workspace, float* dw,
-                            int N, int H, int W, int Cin, int Cout, int stride,
-                            void* stream);
+                            float* dbias, int N, int H, int W, int Cin, int Cout,
+                            int stride, void* stream);
 
 #ifdef __cplusplus
 }

@@ -21,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include "nanodrr_amd.h"
 
@@ -31,7 +32,7 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
 constexpr int kThreads = 256;      // four waves
 constexpr int kPixelsPerWave = 32; // two 16-row MFMA tiles
-constexpr int kWgradBlocks = 2048; // weight-grad workgroups aimed at (the split size follows)
+constexpr int kWgradBlocks = 1024; // weight-grad workgroups aimed at (the split size follows)
 
 struct ConvP {
   const __bf16* x;    // [N][H][W][Cin]
@@ -41,7 +42,8 @@ struct ConvP {
   const __bf16* dy;   // [N][Ho][Wo][Cout]
   __bf16* y;          // [N][Ho][Wo][Cout]   (forward)
   __bf16* dx;         // [N][H][W][Cin]      (input grad)
-  float* part;        // [splits][9][Cout][Cin] (weight grad)
+  float* part;        // [splits][9 Cout Cin + Cout] (weight grad, then bias grad)
+  int bias_grad;      // weight grad: also sum dy per output channel
   int N, H, W, Cin, Ho, Wo, Cout, S, relu, mask;
   int split;          // output pixels per weight-grad workgroup
 };
@@ -273,6 +275,8 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd_weight(ConvP p) {
   const int M = p.N * p.Ho * p.Wo;
   const int p_begin = split * p.split, p_end = min(M, p_begin + p.split);
   const int kq = 8 * (l / 16);
+  const bool bias = p.bias_grad && blockIdx.y == 0 && t < 64;  // thread t sums dy of channel co0 + t
+  float db = 0.0f;
   f32x4 acc[9][2];
 #pragma unroll
   for (int k = 0; k < 9; ++k) acc[k][0] = acc[k][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -300,6 +304,14 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd_weight(ConvP p) {
       }
     }
     __syncthreads();
+    if (bias) {
+#pragma unroll
+      for (int q = 0; q < 32; q += 8) {
+        const bf16x8 v = *reinterpret_cast<const bf16x8*>(&sdy[t][q]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) db += static_cast<float>(v[j]);
+      }
+    }
     const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sdy[16 * wave + (l % 16)][kq]);
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap)
@@ -310,7 +322,8 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd_weight(ConvP p) {
       }
     __syncthreads();
   }
-  // partial[split][tap][co][ci]
+  // partial[split][tap][co][ci], then partial[split][9 Cout Cin + co] for the bias
+  float* part = p.part + static_cast<int64_t>(split) * (9 * p.Cout * p.Cin + p.Cout);
 #pragma unroll
   for (int tap = 0; tap < 9; ++tap)
 #pragma unroll
@@ -318,8 +331,9 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd_weight(ConvP p) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int co = co0 + 16 * wave + 4 * (l / 16) + r, ci = ci0 + 16 * h + (l % 16);
-        p.part[((static_cast<int64_t>(split) * 9 + tap) * p.Cout + co) * p.Cin + ci] = acc[tap][h][r];
+        part[(tap * p.Cout + co) * p.Cin + ci] = acc[tap][h][r];
       }
+  if (bias) part[9 * p.Cout * p.Cin + co0 + t] = db;
 }
 
 // Weight gradient of the one-channel first layer (Cout == 32): a GEMM with the
@@ -332,12 +346,14 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd_weight_cin1(ConvP p) {
   __shared__ __attribute__((aligned(16))) __bf16 sdy[4][32][kLdsRow];  // [wave][co][pixel]
   __shared__ __attribute__((aligned(16))) __bf16 sx[4][16][kLdsRow];   // [wave][tap][pixel]
   __shared__ float red[4][32][16];
+  __shared__ float bred[4][32];
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int split = blockIdx.x;
   const int M = p.N * p.Ho * p.Wo;
   const int p_begin = split * p.split, p_end = min(M, p_begin + p.split);
   const int kq = 8 * (l / 16);
   f32x4 acc[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
+  float db = 0.0f;  // lanes 0-31: this wave's sum of dy for channel l
   for (int base = p_begin; base < p_end; base += 4 * 32) {
     const int pb = base + 32 * wave;
 #pragma unroll
@@ -367,6 +383,14 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd_weight_cin1(ConvP p) {
       }
     }
     __syncthreads();
+    if (p.bias_grad && l < 32) {
+#pragma unroll
+      for (int q = 0; q < 32; q += 8) {
+        const bf16x8 v = *reinterpret_cast<const bf16x8*>(&sdy[wave][l][q]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) db += static_cast<float>(v[j]);
+      }
+    }
     const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sx[wave][l % 16][kq]);
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
@@ -379,32 +403,50 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd_weight_cin1(ConvP p) {
   for (int g = 0; g < 2; ++g)
 #pragma unroll
     for (int r = 0; r < 4; ++r) red[wave][16 * g + 4 * (l / 16) + r][l % 16] = acc[g][r];
+  if (l < 32) bred[wave][l] = db;
   __syncthreads();
+  float* part = p.part + static_cast<int64_t>(split) * (9 * 32 + 32);
   for (int e = threadIdx.x; e < 32 * 9; e += kThreads) {
     const int co = e % 32, tap = e / 32;
-    const float v = ((red[0][co][tap] + red[1][co][tap]) + red[2][co][tap]) + red[3][co][tap];
-    p.part[(static_cast<int64_t>(split) * 9 + tap) * p.Cout + co] = v;
+    part[tap * 32 + co] = ((red[0][co][tap] + red[1][co][tap]) + red[2][co][tap]) + red[3][co][tap];
+  }
+  if (p.bias_grad && threadIdx.x < 32) {
+    const int co = threadIdx.x;
+    part[9 * 32 + co] = ((bred[0][co] + bred[1][co]) + bred[2][co]) + bred[3][co];
   }
 }
 
-// dw[co][tap][ci] = sum over splits of partial[split][tap][co][ci]: a block of
-// 8 elements x 32 lanes; lane j adds splits j, j + 32, ... in order, then a
-// fixed butterfly over the 32 lanes. Deterministic.
+// dw[co][tap][ci] = sum over splits of partial[split][tap][co][ci] (and
+// db[co] of partial[split][9 Cout Cin + co]). A block sums 64 groups of four
+// consecutive elements: wave w adds splits w, w + 4, ... in order with 16-byte
+// loads (1 KB per wave and split), then the four wave sums are added in wave
+// order. Deterministic.
 __global__ __launch_bounds__(kThreads) void k_sum_splits(const float* __restrict__ part, float* __restrict__ dw,
-                                                          int splits, int Cout, int Cin) {
-  const int64_t per = static_cast<int64_t>(9) * Cout * Cin;
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * 8 + threadIdx.x / 32;
-  const int j = threadIdx.x % 32;
-  float s = 0.0f;
-  if (e < per)
-    for (int k = j; k < splits; k += 32) s += part[k * per + e];
+                                                          float* __restrict__ db, int splits, int Cout, int Cin,
+                                                          int64_t count) {
+  __shared__ f32x4 red[4][64];
+  const int64_t stride = static_cast<int64_t>(9) * Cout * Cin + Cout;
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int64_t e = (static_cast<int64_t>(blockIdx.x) * 64 + l) * 4;
+  f32x4 s = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (e < count)
+    for (int k = wave; k < splits; k += 4) s += *reinterpret_cast<const f32x4*>(part + k * stride + e);
+  red[wave][l] = s;
+  __syncthreads();
+  if (wave != 0 || e >= count) return;
+  s = ((red[0][l] + red[1][l]) + red[2][l]) + red[3][l];
+  const int64_t nw = static_cast<int64_t>(9) * Cout * Cin;
 #pragma unroll
-  for (int o = 16; o > 0; o >>= 1) s += __shfl_xor(s, o, 32);
-  if (e < per && j == 0) {
-    const int ci = static_cast<int>(e % Cin);
-    const int co = static_cast<int>((e / Cin) % Cout);
-    const int tap = static_cast<int>(e / (static_cast<int64_t>(Cin) * Cout));
-    dw[(static_cast<int64_t>(co) * 9 + tap) * Cin + ci] = s;
+  for (int j = 0; j < 4; ++j) {
+    const int64_t q = e + j;
+    if (q >= nw) {
+      db[q - nw] = s[j];
+    } else {
+      const int ci = static_cast<int>(q % Cin);
+      const int co = static_cast<int>((q / Cin) % Cout);
+      const int tap = static_cast<int>(q / (static_cast<int64_t>(Cin) * Cout));
+      dw[(static_cast<int64_t>(co) * 9 + tap) * Cin + ci] = s[j];
+    }
   }
 }
 
@@ -434,9 +476,18 @@ bool shape_ok(int N, int H, int W, int Cin, int Cout, int stride) {
 
 // Output pixels per weight-grad workgroup: enough workgroups to fill the chip
 // (about kWgradBlocks over the channel tiles), in whole 128-pixel steps.
+int wgrad_blocks() {
+  static const int blocks = [] {
+    const char* v = getenv("NDRR_WGRAD_BLOCKS");
+    const int b = v ? atoi(v) : kWgradBlocks;
+    return b >= 64 && b <= 65536 ? b : kWgradBlocks;
+  }();
+  return blocks;
+}
+
 int wgrad_split(int M, int Cin, int Cout) {
   const int tiles = Cin == 1 ? 1 : (Cin / 32) * (Cout / 64);
-  const int want = (kWgradBlocks + tiles - 1) / tiles;
+  const int want = (wgrad_blocks() + tiles - 1) / tiles;
   int per = (M + want - 1) / want;
   per = ((per + 127) / 128) * 128;
   return per < 256 ? 256 : per;
@@ -512,18 +563,20 @@ size_t ndrr_conv3x3_bwd_weight_workspace_floats(int N, int H, int W, int Cin, in
   if (M >= (int64_t{1} << 31)) return 0;
   const int per = wgrad_split(static_cast<int>(M), Cin, Cout);
   const int64_t splits = (M + per - 1) / per;
-  return static_cast<size_t>(splits * 9 * Cout * Cin);
+  return static_cast<size_t>(splits * (9 * Cout * Cin + Cout));
 }
 
-int ndrr_conv3x3_bwd_weight(const void* dy, const void* x, float* workspace, float* dw, int N, int H, int W,
-                            int Cin, int Cout, int stride, void* stream) {
+int ndrr_conv3x3_bwd_weight(const void* dy, const void* x, float* workspace, float* dw, float* dbias, int N, int H,
+                            int W, int Cin, int Cout, int stride, void* stream) {
   if (!dy || !x || !workspace || !dw || !shape_ok(N, H, W, Cin, Cout, stride)) return NDRR_EINVAL;
+  if ((reinterpret_cast<uintptr_t>(workspace) & 15) != 0) return NDRR_EINVAL;
   if (Cin == 1 ? Cout != 32 : Cout % 64 != 0) return NDRR_EINVAL;
   const int Ho = out_size(H, stride), Wo = out_size(W, stride);
   if (!range_ok(N, H, W, Cin) || !range_ok(N, Ho, Wo, Cout)) return NDRR_ERANGE;
   ConvP p{};
   p.dy = static_cast<const __bf16*>(dy); p.x = static_cast<const __bf16*>(x); p.part = workspace;
   p.N = N; p.H = H; p.W = W; p.Cin = Cin; p.Ho = Ho; p.Wo = Wo; p.Cout = Cout; p.S = stride;
+  p.bias_grad = dbias != nullptr;
   const int M = N * Ho * Wo;
   p.split = wgrad_split(M, Cin, Cout);
   const int splits = (M + p.split - 1) / p.split;
@@ -535,9 +588,9 @@ int ndrr_conv3x3_bwd_weight(const void* dy, const void* x, float* workspace, flo
   }
   int r = status();
   if (r != NDRR_OK) return r;
-  const int64_t per = static_cast<int64_t>(9) * Cout * Cin;
-  hipLaunchKernelGGL(k_sum_splits, dim3(static_cast<unsigned>((per + 7) / 8)), dim3(kThreads), 0, st, workspace, dw,
-                     splits, Cout, Cin);
+  const int64_t count = static_cast<int64_t>(9) * Cout * Cin + (dbias ? Cout : 0);
+  hipLaunchKernelGGL(k_sum_splits, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(kThreads), 0, st,
+                     workspace, dw, dbias, splits, Cout, Cin, count);
   return status();
 }
 

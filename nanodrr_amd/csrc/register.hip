@@ -163,49 +163,136 @@ __global__ void k_pose_compose_bwd(const double* __restrict__ prefix, const doub
 
 // ZNCC per image: xh = (x - mean x) / sqrt(var x + eps), yh likewise,
 // score = mean(xh * yh); grad wrt x = (yh - score xh) / (N sigma_x).
-// One 256-thread block per image, fp32 with two passes (means first); sums in
-// fixed order (strided partials, then a tree in LDS), so results are bitwise
-// reproducible.
-__device__ inline float block_sum(float v, float* red) {
-  const int t = threadIdx.x;
-  red[t] = v;
-  __syncthreads();
-  for (int s = kThreads / 2; s > 0; s >>= 1) {
-    if (t < s) red[t] += red[t + s];
-    __syncthreads();
-  }
-  const float total = red[0];
-  __syncthreads();
-  return total;
+//
+// Two launches over (chunk, image) grids so that a batch of a few images
+// still fills the chip. k_zncc_part reduces a chunk of kZnccChunk pixels to
+// its moments (count, means, centred second moments, co-moment), two passes
+// over values held in registers. k_zncc_finish merges an image's chunk
+// moments (Chan, Golub and LeVeque's pairwise update, in a fixed order) and,
+// per chunk, writes the gradient. Every sum has a fixed order, so results are
+// bitwise reproducible.
+constexpr int kZnccPer = 16;
+constexpr int kZnccChunk = kThreads * kZnccPer;
+constexpr int kZnccStat = 8;  // floats per chunk record (6 used)
+
+struct Moments {
+  float n, mx, my, sxx, syy, sxy;
+};
+
+__device__ inline Moments merge(const Moments& a, const Moments& b) {
+  if (b.n == 0.0f) return a;
+  if (a.n == 0.0f) return b;
+  const float n = a.n + b.n;
+  const float dx = b.mx - a.mx, dy = b.my - a.my;
+  const float f = a.n * b.n / n;
+  return {n,
+          fmaf(dx, b.n / n, a.mx),
+          fmaf(dy, b.n / n, a.my),
+          a.sxx + b.sxx + dx * dx * f,
+          a.syy + b.syy + dy * dy * f,
+          a.sxy + b.sxy + dx * dy * f};
 }
 
-__global__ __launch_bounds__(kThreads) void k_zncc(const float* __restrict__ x, const float* __restrict__ y,
-                                                   float* __restrict__ score, float* __restrict__ gx, int N,
-                                                   float eps) {
-  __shared__ float red[kThreads];
-  const int b = blockIdx.x;
+// Sum over the block: a butterfly within each wave, then the four wave sums
+// in order. Every thread returns the same value.
+__device__ inline float block_sum(float v, float* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();  // red may still be read by a previous call
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(kThreads) void k_zncc_part(const float* __restrict__ x, const float* __restrict__ y,
+                                                        float* __restrict__ part, int N) {
+  __shared__ float red[kThreads / 64];
+  const int b = blockIdx.y, p = blockIdx.x, P = gridDim.x;
+  const int base = p * kZnccChunk;
   const float* xb = x + static_cast<int64_t>(b) * N;
   const float* yb = y + static_cast<int64_t>(b) * N;
-  const float inv_n = 1.0f / static_cast<float>(N);
+  const int n = min(kZnccChunk, N - base);
+  float xv[kZnccPer], yv[kZnccPer];
   float sx = 0.0f, sy = 0.0f;
-  for (int i = threadIdx.x; i < N; i += kThreads) { sx += xb[i]; sy += yb[i]; }
+#pragma unroll
+  for (int j = 0; j < kZnccPer; ++j) {
+    const int e = base + j * kThreads + threadIdx.x;
+    xv[j] = e < N ? xb[e] : 0.0f;
+    yv[j] = e < N ? yb[e] : 0.0f;
+    sx += xv[j];
+    sy += yv[j];
+  }
+  const float inv_n = 1.0f / static_cast<float>(n);
   const float mx = block_sum(sx, red) * inv_n;
   const float my = block_sum(sy, red) * inv_n;
   float vx = 0.0f, vy = 0.0f, cxy = 0.0f;
-  for (int i = threadIdx.x; i < N; i += kThreads) {
-    const float dx = xb[i] - mx, dy = yb[i] - my;
-    vx += dx * dx; vy += dy * dy; cxy += dx * dy;
+#pragma unroll
+  for (int j = 0; j < kZnccPer; ++j) {
+    if (base + j * kThreads + static_cast<int>(threadIdx.x) < N) {
+      const float dx = xv[j] - mx, dy = yv[j] - my;
+      vx = fmaf(dx, dx, vx);
+      vy = fmaf(dy, dy, vy);
+      cxy = fmaf(dx, dy, cxy);
+    }
   }
-  const float sdx = sqrtf(block_sum(vx, red) * inv_n + eps);
-  const float sdy = sqrtf(block_sum(vy, red) * inv_n + eps);
-  const float sc = block_sum(cxy, red) * inv_n / (sdx * sdy);
-  if (threadIdx.x == 0) score[b] = sc;
-  if (gx) {
-    float* gb = gx + static_cast<int64_t>(b) * N;
-    const float g = inv_n / sdx;
-    for (int i = threadIdx.x; i < N; i += kThreads) {
-      const float xh = (xb[i] - mx) / sdx, yh = (yb[i] - my) / sdy;
-      gb[i] = (yh - sc * xh) * g;
+  vx = block_sum(vx, red);
+  vy = block_sum(vy, red);
+  cxy = block_sum(cxy, red);
+  if (threadIdx.x == 0) {
+    float* r = part + (static_cast<int64_t>(b) * P + p) * kZnccStat;
+    r[0] = static_cast<float>(n);
+    r[1] = mx;
+    r[2] = my;
+    r[3] = vx;
+    r[4] = vy;
+    r[5] = cxy;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_zncc_finish(const float* __restrict__ x, const float* __restrict__ y,
+                                                          const float* __restrict__ part, int P,
+                                                          float* __restrict__ score, float* __restrict__ gx, int N,
+                                                          float eps) {
+  __shared__ float st[5];
+  const int b = blockIdx.y, p = blockIdx.x;
+  if (threadIdx.x < 64) {
+    // lane l merges chunks l, l + 64, ... in order, then a fixed tree over the lanes
+    Moments m{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int q = threadIdx.x; q < P; q += 64) {
+      const float* r = part + (static_cast<int64_t>(b) * P + q) * kZnccStat;
+      m = merge(m, Moments{r[0], r[1], r[2], r[3], r[4], r[5]});
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const Moments u{__shfl_down(m.n, o), __shfl_down(m.mx, o), __shfl_down(m.my, o),
+                      __shfl_down(m.sxx, o), __shfl_down(m.syy, o), __shfl_down(m.sxy, o)};
+      if (static_cast<int>(threadIdx.x) + o < 64) m = merge(m, u);
+    }
+    if (threadIdx.x == 0) {
+      const float inv_n = 1.0f / static_cast<float>(N);
+      const float sdx = sqrtf(m.sxx * inv_n + eps), sdy = sqrtf(m.syy * inv_n + eps);
+      const float sc = m.sxy * inv_n / (sdx * sdy);
+      st[0] = m.mx;
+      st[1] = m.my;
+      st[2] = sdx;
+      st[3] = sdy;
+      st[4] = sc;
+      if (p == 0) score[b] = sc;
+    }
+  }
+  __syncthreads();
+  if (!gx) return;
+  const float mx = st[0], my = st[1], sdx = st[2], sdy = st[3], sc = st[4];
+  const float g = 1.0f / (static_cast<float>(N) * sdx);
+  const float* xb = x + static_cast<int64_t>(b) * N;
+  const float* yb = y + static_cast<int64_t>(b) * N;
+  float* gb = gx + static_cast<int64_t>(b) * N;
+  const int base = p * kZnccChunk;
+#pragma unroll 4
+  for (int j = 0; j < kZnccPer; ++j) {
+    const int e = base + j * kThreads + threadIdx.x;
+    if (e < N) {
+      const float xh = (xb[e] - mx) / sdx, yh = (yb[e] - my) / sdy;
+      gb[e] = (yh - sc * xh) * g;
     }
   }
 }
@@ -267,11 +354,20 @@ int ndrr_adam_step(float* param, const float* grad, float* m, float* v, int n, f
   return status();
 }
 
-int ndrr_zncc(const float* x, const float* y, int B, int N, float eps, float* score, float* grad_x, void* stream) {
-  if (!x || !y || !score || B <= 0 || N <= 0 || !(eps >= 0.0f)) return NDRR_EINVAL;
-  if (B > 0x7fffffff) return NDRR_ERANGE;
-  hipLaunchKernelGGL(k_zncc, dim3(B), dim3(kThreads), 0, static_cast<hipStream_t>(stream), x, y, score, grad_x, N,
-                     eps);
+int64_t ndrr_zncc_workspace_floats(int B, int N) {
+  if (B <= 0 || N <= 0) return 0;
+  return static_cast<int64_t>(B) * ((N + kZnccChunk - 1) / kZnccChunk) * kZnccStat;
+}
+
+int ndrr_zncc(const float* x, const float* y, int B, int N, float eps, float* workspace, float* score, float* grad_x,
+              void* stream) {
+  if (!x || !y || !workspace || !score || B <= 0 || N <= 0 || !(eps >= 0.0f)) return NDRR_EINVAL;
+  if (B > 65535 || N > 0x7fffffff - kZnccChunk) return NDRR_ERANGE;
+  const int P = (N + kZnccChunk - 1) / kZnccChunk;
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_zncc_part, dim3(P, B), dim3(kThreads), 0, st, x, y, workspace, N);
+  hipLaunchKernelGGL(k_zncc_finish, dim3(grad_x ? P : 1, B), dim3(kThreads), 0, st, x, y, workspace, P, score,
+                     grad_x, N, eps);
   return status();
 }
 

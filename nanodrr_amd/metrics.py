@@ -1,8 +1,8 @@
 """Image and pose metrics for registration.
 
-`zncc` is the native zero-normalised cross-correlation (ndrr_zncc: one
-workgroup per image, deterministic, capturable in a HIP graph) with its
-gradient. `NormalizedCrossCorrelation2d`, `MultiscaleNormalizedCrossCorrelation2d`,
+`zncc` is the native zero-normalised cross-correlation (ndrr_zncc: chunk
+moments over (4096-pixel chunk, image) workgroups merged in a fixed order,
+deterministic) with its gradient. `NormalizedCrossCorrelation2d`, `MultiscaleNormalizedCrossCorrelation2d`,
 `GradientNormalizedCrossCorrelation2d` and `DoubleGeodesicSE3` keep the API of
 src/nanodrr/metrics/ncc.py:8-137 and src/nanodrr/metrics/geo.py:7-45; the
 whole-image NCC runs on the native kernel, patch and gradient variants on
@@ -30,9 +30,10 @@ def _zncc(x: torch.Tensor, y: torch.Tensor, eps: float) -> tuple[torch.Tensor, t
         raise ValueError("zncc runs on the GPU: pass device tensors")
     score = torch.empty(B, device=x.device, dtype=torch.float32)
     grad = torch.empty_like(x2)
+    ws = torch.empty(_lib.lib().ndrr_zncc_workspace_floats(B, x2.shape[1]), device=x.device, dtype=torch.float32)
     with torch.cuda.device(x.device):
-        rc = _lib.lib().ndrr_zncc(x2.data_ptr(), y2.data_ptr(), B, x2.shape[1], float(eps), score.data_ptr(),
-                                  grad.data_ptr(), torch.cuda.current_stream(x.device).cuda_stream)
+        rc = _lib.lib().ndrr_zncc(x2.data_ptr(), y2.data_ptr(), B, x2.shape[1], float(eps), ws.data_ptr(),
+                                  score.data_ptr(), grad.data_ptr(), torch.cuda.current_stream(x.device).cuda_stream)
     _lib.check(rc, "ndrr_zncc")
     return score, grad.view(x.shape)
 

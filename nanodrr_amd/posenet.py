@@ -65,19 +65,21 @@ def conv3x3_bwd_data(dy: torch.Tensor, w: torch.Tensor, x: torch.Tensor | None, 
     return dx
 
 
-def conv3x3_bwd_weight(dy: torch.Tensor, x: torch.Tensor, stride: int) -> torch.Tensor:
-    """dL/dw (Cout, 3, 3, Cin) fp32 of conv3x3 for dL/dy `dy` (bf16) and input `x` (bf16)."""
+def conv3x3_bwd_weight(dy: torch.Tensor, x: torch.Tensor, stride: int, bias: bool = False):
+    """dL/dw (Cout, 3, 3, Cin) fp32 of conv3x3 for dL/dy `dy` (bf16) and input `x` (bf16); with
+    bias=True also dL/dbias (Cout,) fp32 from the same pass, returned as (dw, db)."""
     N, H, W, Cin = x.shape
     Cout = dy.shape[-1]
     lib = _lib.lib()
     ws = torch.empty(lib.ndrr_conv3x3_bwd_weight_workspace_floats(N, H, W, Cin, Cout, stride), device=x.device,
                      dtype=torch.float32)
     dw = torch.empty(Cout, 3, 3, Cin, device=x.device, dtype=torch.float32)
+    db = torch.empty(Cout, device=x.device, dtype=torch.float32) if bias else None
     dy, x = dy.contiguous(), x.contiguous()
-    rc = lib.ndrr_conv3x3_bwd_weight(dy.data_ptr(), x.data_ptr(), ws.data_ptr(), dw.data_ptr(), N, H, W, Cin, Cout,
-                                     stride, _stream(x))
+    rc = lib.ndrr_conv3x3_bwd_weight(dy.data_ptr(), x.data_ptr(), ws.data_ptr(), dw.data_ptr(),
+                                     db.data_ptr() if bias else None, N, H, W, Cin, Cout, stride, _stream(x))
     _lib.check(rc, "ndrr_conv3x3_bwd_weight")
-    return dw
+    return (dw, db) if bias else dw
 
 
 class _ConvStack(torch.autograd.Function):
@@ -104,8 +106,7 @@ class _ConvStack(torch.autograd.Function):
         for i in reversed(range(len(LAYERS))):
             cin, cout, s = LAYERS[i]
             x = acts[i]
-            grads[2 * i] = conv3x3_bwd_weight(dz, x, s)
-            grads[2 * i + 1] = dz.sum(dim=(0, 1, 2), dtype=torch.float32)
+            grads[2 * i], grads[2 * i + 1] = conv3x3_bwd_weight(dz, x, s, bias=True)
             if i > 0:
                 dz = conv3x3_bwd_data(dz, wbf[i], x, s, (x.shape[1], x.shape[2]))
         return (None, *grads)

@@ -84,9 +84,14 @@ def test_conv_weight_gradient_matches_torch(shape):
     dy = torch.randn(N, Ho, Wo, Cout, generator=torch.Generator().manual_seed(5)).to(torch.bfloat16)
     wr = w.float().requires_grad_(True)
     _torch_conv(x, wr, b, s).backward(dy.float())
+    br = b.clone().requires_grad_(True)
+    _torch_conv(x, w, br, s).backward(dy.float())
     dw = conv3x3_bwd_weight(dy.to(DEV), x.to(DEV), s).cpu()
+    dw2, db = conv3x3_bwd_weight(dy.to(DEV), x.to(DEV), s, bias=True)
     # products of bf16 values are exact in fp32; only the summation order differs
     assert (dw - wr.grad).abs().max() / wr.grad.abs().max() < 1e-5
+    assert torch.equal(dw2.cpu(), dw)  # the bias sum does not change the weight gradient
+    assert (db.cpu() - br.grad).abs().max() / br.grad.abs().max() < 1e-5
 
 
 def test_conv_weight_gradient_many_splits_is_deterministic():
@@ -95,12 +100,14 @@ def test_conv_weight_gradient_many_splits_is_deterministic():
 
     x, w, b = _inputs(4, 64, 64, 32, 64, 99)
     dy = torch.randn(4, 32, 32, 64, generator=torch.Generator().manual_seed(9)).to(torch.bfloat16)
-    a = conv3x3_bwd_weight(dy.to(DEV), x.to(DEV), 2)
-    c = conv3x3_bwd_weight(dy.to(DEV), x.to(DEV), 2)
-    assert torch.equal(a, c)
+    a, da = conv3x3_bwd_weight(dy.to(DEV), x.to(DEV), 2, bias=True)
+    c, dc = conv3x3_bwd_weight(dy.to(DEV), x.to(DEV), 2, bias=True)
+    assert torch.equal(a, c) and torch.equal(da, dc)
     wr = w.float().requires_grad_(True)
-    _torch_conv(x, wr, b, 2).backward(dy.float())
+    br = b.clone().requires_grad_(True)
+    _torch_conv(x, wr, br, 2).backward(dy.float())
     assert (a.cpu() - wr.grad).abs().max() / wr.grad.abs().max() < 1e-5
+    assert (da.cpu() - br.grad).abs().max() / br.grad.abs().max() < 1e-5
 
 
 def test_posenet_gradients_match_torch_autograd():

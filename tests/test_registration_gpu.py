@@ -76,7 +76,10 @@ def test_pose_compose_autograd():
     np.testing.assert_allclose(v.grad.cpu().double().numpy(), ref_xyz, rtol=1e-6, atol=1e-6)
 
 
-@pytest.mark.parametrize("shape", [(3, 1, 200, 200), (2, 1, 7, 13), (1, 1, 1, 300)])
+# one chunk (7 x 13, 300, 64 x 64 = exactly 4096 pixels), ten chunks (200 x 200), and 256 chunks
+# (1024 x 1024: more chunks than lanes in the merging wave)
+@pytest.mark.parametrize("shape", [(3, 1, 200, 200), (2, 1, 7, 13), (1, 1, 1, 300), (5, 1, 64, 64),
+                                   (2, 1, 1024, 1024)])
 def test_zncc_matches_oracle(shape):
     from nanodrr_amd.metrics import zncc
 
