@@ -18,6 +18,13 @@
 // MFMA operand layout (16x16x32 bf16, wave64), checked on the device: lane l
 // holds A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16], j < 8, and
 // D[4 (l / 16) + r][l % 16] in its four accumulators r.
+//
+// The forward and input-gradient GEMMs put the channels on the MFMA rows (A,
+// from the weights) and the pixels on its columns (B), and pair 16-row groups
+// with a row order that gives each lane eight consecutive channels of one
+// pixel: row i of group 2h + e is channel 32 h + 8 (i / 4) + 4 e + i % 4, so
+// lane l ends with channels 32 h + 8 (l / 16) + 0..7 of pixel l % 16 and
+// writes them with one 16-byte store.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -59,6 +66,11 @@ __device__ inline bf16x8 load8(const __bf16* p) { return *reinterpret_cast<const
 
 __device__ inline f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Channel (within a block's channel range) of the A row that lane l loads in row group c.
+__device__ inline int pair_row(int l, int c) {
+  return 32 * (c >> 1) + 8 * ((l & 15) >> 2) + 4 * (c & 1) + (l & 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -107,7 +119,7 @@ __global__ __launch_bounds__(kThreads) void k_conv_fwd(ConvP p) {
     }
 #pragma unroll
     for (int c = 0; c < COG; ++c) {
-      const int co = co0 + 16 * c + (l % 16);
+      const int co = co0 + pair_row(l, c);
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int tap = kq + j;
@@ -117,7 +129,7 @@ __global__ __launch_bounds__(kThreads) void k_conv_fwd(ConvP p) {
 #pragma unroll
     for (int g = 0; g < 2; ++g)
 #pragma unroll
-      for (int c = 0; c < COG; ++c) acc[g][c] = mfma(a[g], b[c], acc[g][c]);
+      for (int c = 0; c < COG; ++c) acc[g][c] = mfma(b[c], a[g], acc[g][c]);
   } else {
     for (int tap = 0; tap < 9; ++tap) {
       const int ky = tap / 3, kx = tap % 3;
@@ -132,7 +144,7 @@ __global__ __launch_bounds__(kThreads) void k_conv_fwd(ConvP p) {
       const __bf16* wrow[COG];
 #pragma unroll
       for (int c = 0; c < COG; ++c)
-        wrow[c] = p.w + (static_cast<int64_t>(co0 + 16 * c + (l % 16)) * 9 + tap) * p.Cin + kq;
+        wrow[c] = p.w + (static_cast<int64_t>(co0 + pair_row(l, c)) * 9 + tap) * p.Cin + kq;
       for (int ci0 = 0; ci0 < p.Cin; ci0 += 32) {
         bf16x8 a[2], b[COG];
 #pragma unroll
@@ -142,26 +154,31 @@ __global__ __launch_bounds__(kThreads) void k_conv_fwd(ConvP p) {
 #pragma unroll
         for (int g = 0; g < 2; ++g)
 #pragma unroll
-          for (int c = 0; c < COG; ++c) acc[g][c] = mfma(a[g], b[c], acc[g][c]);
+          for (int c = 0; c < COG; ++c) acc[g][c] = mfma(b[c], a[g], acc[g][c]);
       }
     }
   }
-  // Epilogue: bias, ReLU, bf16, NHWC store.
+  // Epilogue: bias, ReLU, bf16, one 16-byte NHWC store of eight channels per pixel.
 #pragma unroll
-  for (int c = 0; c < COG; ++c) {
-    const int co = co0 + 16 * c + (l % 16);
-    const float bv = p.bias ? p.bias[co] : 0.0f;
+  for (int h = 0; h < COG / 2; ++h) {
+    const int cb = co0 + 32 * h + 8 * (l / 16);
+    float bv[8];
 #pragma unroll
-    for (int g = 0; g < 2; ++g)
+    for (int j = 0; j < 8; ++j) bv[j] = p.bias ? p.bias[cb + j] : 0.0f;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = m0 + 16 * g + 4 * (l / 16) + r;
-        if (m < M) {
-          float v = acc[g][c][r] + bv;
-          if (p.relu) v = fmaxf(v, 0.0f);
-          p.y[static_cast<int64_t>(m) * p.Cout + co] = static_cast<__bf16>(v);
+    for (int g = 0; g < 2; ++g) {
+      const int m = m0 + 16 * g + (l % 16);
+      if (m < M) {
+        bf16x8 v;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          float u = acc[g][2 * h + j / 4][j % 4] + bv[j];
+          if (p.relu) u = fmaxf(u, 0.0f);
+          v[j] = static_cast<__bf16>(u);
         }
+        *reinterpret_cast<bf16x8*>(p.y + static_cast<int64_t>(m) * p.Cout + cb) = v;
       }
+    }
   }
 }
 
@@ -222,7 +239,7 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd_data(ConvP p) {
       const __bf16* wrow[CIG];
 #pragma unroll
       for (int c = 0; c < CIG; ++c)
-        wrow[c] = p.wt + (static_cast<int64_t>(tap) * p.Cin + ci0 + 16 * c + (l % 16)) * p.Cout + kq;
+        wrow[c] = p.wt + (static_cast<int64_t>(tap) * p.Cin + ci0 + pair_row(l, c)) * p.Cout + kq;
       for (int co0 = 0; co0 < p.Cout; co0 += 32) {
         bf16x8 a[2], b[CIG];
 #pragma unroll
@@ -232,28 +249,28 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd_data(ConvP p) {
 #pragma unroll
         for (int g = 0; g < 2; ++g)
 #pragma unroll
-          for (int c = 0; c < CIG; ++c) acc[g][c] = mfma(a[g], b[c], acc[g][c]);
+          for (int c = 0; c < CIG; ++c) acc[g][c] = mfma(b[c], a[g], acc[g][c]);
       }
     }
   }
-  // D rows 4 (l / 16) + r of each 16-row group; their pixels within the class
+  // lane l: channels ci0 + 32 h + 8 (l / 16) + 0..7 of pixel l % 16 of each group
 #pragma unroll
-  for (int c = 0; c < CIG; ++c) {
-    const int ci = ci0 + 16 * c + (l % 16);
+  for (int g = 0; g < 2; ++g) {
+    if (!pv[g]) continue;
+    const int64_t at = ((static_cast<int64_t>(pn[g]) * p.H + piy[g]) * p.W + pix[g]) * p.Cin;
 #pragma unroll
-    for (int g = 0; g < 2; ++g)
+    for (int h = 0; h < CIG / 2; ++h) {
+      const int cb = ci0 + 32 * h + 8 * (l / 16);
+      const bf16x8 xm = p.mask ? load8(p.x + at + cb) : zero8();
+      bf16x8 v;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = m0 + 16 * g + 4 * (l / 16) + r;
-        if (m < M) {
-          const int cx = m % Wc, cy = (m / Wc) % Hc, n = m / (Wc * Hc);
-          const int ix = p.S == 2 ? 2 * cx + px : cx, iy = p.S == 2 ? 2 * cy + py : cy;
-          const int64_t at = ((static_cast<int64_t>(n) * p.H + iy) * p.W + ix) * p.Cin + ci;
-          float v = acc[g][c][r];
-          if (p.mask && !(static_cast<float>(p.x[at]) > 0.0f)) v = 0.0f;
-          p.dx[at] = static_cast<__bf16>(v);
-        }
+      for (int j = 0; j < 8; ++j) {
+        float u = acc[g][2 * h + j / 4][j % 4];
+        if (p.mask && !(static_cast<float>(xm[j]) > 0.0f)) u = 0.0f;
+        v[j] = static_cast<__bf16>(u);
       }
+      *reinterpret_cast<bf16x8*>(p.dx + at + cb) = v;
+    }
   }
 }
 
