@@ -8,8 +8,10 @@
 //
 // with padding 1 and stride 1 or 2 (out = (in - 1) / stride + 1). The GEMM
 // rows are pixels, the columns channels and K runs over (tap, channel) in
-// chunks of 32 channels, so every MFMA operand is one 16-byte load of eight
-// consecutive channels (a tap of a pixel, or a weight row). The weight grad
+// chunks of 32 channels, so every operand row is 16-byte pieces of eight
+// consecutive channels (a tap of a pixel, or a weight row). The forward and
+// the input gradient stage each K step's tiles through LDS, double-buffered
+// (k_igemm); the first layer's forward loads its operands directly. The weight grad
 // stages 32-pixel slabs of dy and x through LDS, transposed, so that its K
 // (the pixels) is contiguous; it splits the pixels over workgroups and sums
 // the split partials in a fixed order (deterministic). The first layer (one
@@ -74,9 +76,11 @@ __device__ inline int pair_row(int l, int c) {
 }
 
 // ---------------------------------------------------------------------------
-// Forward. Block: 4 waves x 32 output pixels; 16 * COG output channels.
+// Forward of the one-channel first layer: K = the nine taps of the single
+// input channel, zero-padded to 32, so one MFMA per tile and no K loop.
+// Block: 4 waves x 32 output pixels; 16 * COG output channels.
 // ---------------------------------------------------------------------------
-template <int COG, bool CIN1>
+template <int COG>
 __global__ __launch_bounds__(kThreads) void k_conv_fwd(ConvP p) {
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int M = p.N * p.Ho * p.Wo;
@@ -100,8 +104,7 @@ __global__ __launch_bounds__(kThreads) void k_conv_fwd(ConvP p) {
 #pragma unroll
     for (int c = 0; c < COG; ++c) acc[g][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
-  if constexpr (CIN1) {
-    // K = the nine taps of the single input channel, zero-padded to 32.
+  {
     bf16x8 a[2], b[COG];
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
@@ -130,33 +133,6 @@ __global__ __launch_bounds__(kThreads) void k_conv_fwd(ConvP p) {
     for (int g = 0; g < 2; ++g)
 #pragma unroll
       for (int c = 0; c < COG; ++c) acc[g][c] = mfma(b[c], a[g], acc[g][c]);
-  } else {
-    for (int tap = 0; tap < 9; ++tap) {
-      const int ky = tap / 3, kx = tap % 3;
-      int64_t base[2];
-      bool inb[2];
-#pragma unroll
-      for (int g = 0; g < 2; ++g) {
-        const int iy = poy[g] * p.S + ky - 1, ix = pox[g] * p.S + kx - 1;
-        inb[g] = pv[g] && iy >= 0 && iy < p.H && ix >= 0 && ix < p.W;
-        base[g] = inb[g] ? ((static_cast<int64_t>(pn[g]) * p.H + iy) * p.W + ix) * p.Cin + kq : 0;
-      }
-      const __bf16* wrow[COG];
-#pragma unroll
-      for (int c = 0; c < COG; ++c)
-        wrow[c] = p.w + (static_cast<int64_t>(co0 + pair_row(l, c)) * 9 + tap) * p.Cin + kq;
-      for (int ci0 = 0; ci0 < p.Cin; ci0 += 32) {
-        bf16x8 a[2], b[COG];
-#pragma unroll
-        for (int g = 0; g < 2; ++g) a[g] = inb[g] ? load8(p.x + base[g] + ci0) : zero8();
-#pragma unroll
-        for (int c = 0; c < COG; ++c) b[c] = load8(wrow[c] + ci0);
-#pragma unroll
-        for (int g = 0; g < 2; ++g)
-#pragma unroll
-          for (int c = 0; c < COG; ++c) acc[g][c] = mfma(b[c], a[g], acc[g][c]);
-      }
-    }
   }
   // Epilogue: bias, ReLU, bf16, one 16-byte NHWC store of eight channels per pixel.
 #pragma unroll
@@ -183,94 +159,177 @@ __global__ __launch_bounds__(kThreads) void k_conv_fwd(ConvP p) {
 }
 
 // ---------------------------------------------------------------------------
-// Input gradient: rows are input pixels, columns input channels (16 * CIG per
-// block), K over (tap, output channel). Output pixel of (iy, tap):
-// oy = (iy + 1 - ky) / S when that divides exactly and lies in [0, Ho).
-// With stride 2 only the taps with ky = iy + 1 (mod 2) (and likewise in x)
-// reach a pixel, so the pixels are taken per parity class (blockIdx.z =
-// 2 (iy % 2) + ix % 2): one or two taps per axis instead of three, and no
-// multiply of structural zeros.
+// Tiled implicit GEMM: the forward (MODE 0) and the input gradient (MODE 1)
+// of layers with Cin a multiple of 32. Block: four waves as WM x (4 / WM);
+// wave tile 64 pixels x 16 CG channels (4 x CG MFMA tiles), block tile
+// BM = 64 WM pixels x BN = 16 CG (4 / WM) channels. K runs in steps of 32
+// channels at one tap. Each step's operands are staged through LDS (A: BN
+// weight rows, B: BM activation rows, 32 K elements each in 80-byte rows, so
+// the 16-byte reads of 16 consecutive rows hit distinct banks), double
+// buffered: the next step's global loads are in flight during this step's
+// MFMAs, and one barrier ends each step.
+//   MODE 0: rows are output pixels, columns output channels, the B row of a
+//           pixel at tap (ky, kx) is x at (oy S + ky - 1, ox S + kx - 1).
+//   MODE 1: rows are input pixels of one parity class (blockIdx.z) for stride
+//           2, columns input channels, K over the class's taps and the output
+//           channels; the B row is dy at ((iy + 1 - ky) / S, (ix + 1 - kx) / S)
+//           when that divides exactly and lies inside.
 // ---------------------------------------------------------------------------
-template <int CIG>
-__global__ __launch_bounds__(kThreads) void k_conv_bwd_data(ConvP p) {
-  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-  // pixel class: stride 1 has one class (every pixel); stride 2 four parity classes
-  const int py = p.S == 2 ? static_cast<int>(blockIdx.z) >> 1 : 0;
-  const int px = p.S == 2 ? static_cast<int>(blockIdx.z) & 1 : 0;
-  const int Hc = p.S == 2 ? (p.H - py + 1) / 2 : p.H;
-  const int Wc = p.S == 2 ? (p.W - px + 1) / 2 : p.W;
-  const int M = p.N * Hc * Wc;
-  const int m0 = (blockIdx.x * 4 + wave) * kPixelsPerWave;
-  if (m0 >= M) return;  // whole wave past this class (no barrier below)
-  const int ci0 = blockIdx.y * 16 * CIG;
-  const int kq = 8 * (l / 16);
-  int pn[2], piy[2], pix[2];
-  bool pv[2];
+constexpr int kKRow = 40;  // 32 K elements + 8 of padding
+
+template <int MODE, int WM, int CG>
+__global__ __launch_bounds__(kThreads) void k_igemm(ConvP p) {
+  constexpr int WN = 4 / WM;
+  constexpr int BM = 64 * WM, BN = 16 * CG * WN;
+  constexpr int BQ = BM * 4 / kThreads;                   // 16-byte B chunks per thread
+  constexpr int AQ = (BN * 4 + kThreads - 1) / kThreads;  // 16-byte A chunks per thread
+  __shared__ __attribute__((aligned(16))) __bf16 sA[2][BN][kKRow];
+  __shared__ __attribute__((aligned(16))) __bf16 sB[2][BM][kKRow];
+  const int t = threadIdx.x, wave = t >> 6, l = t & 63;
+  const int wm = wave % WM, wn = wave / WM;
+  const int ch0 = blockIdx.y * BN;
+  const bool cls = MODE == 1 && p.S == 2;
+  const int py = cls ? static_cast<int>(blockIdx.z) >> 1 : 0;
+  const int px = cls ? static_cast<int>(blockIdx.z) & 1 : 0;
+  const int Hr = MODE == 0 ? p.Ho : (cls ? (p.H - py + 1) / 2 : p.H);
+  const int Wr = MODE == 0 ? p.Wo : (cls ? (p.W - px + 1) / 2 : p.W);
+  const int M = p.N * Hr * Wr;
+  const int m0 = blockIdx.x * BM;
+  if (m0 >= M) return;  // the whole block is past this class (before any barrier)
+  const int Ck = MODE == 0 ? p.Cin : p.Cout;
+  const int KC = Ck / 32;
+  const int ky0 = cls ? (py == 0 ? 1 : 0) : 0, kx0 = cls ? (px == 0 ? 1 : 0) : 0;
+  const int ks = cls ? 2 : 1;
+  const int nkx = (3 - kx0 + ks - 1) / ks, nky = (3 - ky0 + ks - 1) / ks;
+  const int T = nky * nkx * KC;
+
+  // this thread's B rows t / 4 + 64 i and 16-byte part t % 4 (also its A rows)
+  const int part = t & 3;
+  int rn[BQ], ry[BQ], rx[BQ];
+  bool rv[BQ];
 #pragma unroll
-  for (int g = 0; g < 2; ++g) {
-    const int m = m0 + 16 * g + (l % 16);
-    pv[g] = m < M;
-    const int mm = pv[g] ? m : 0;
-    const int cx = mm % Wc, cy = (mm / Wc) % Hc;
-    pn[g] = mm / (Wc * Hc);
-    pix[g] = p.S == 2 ? 2 * cx + px : cx;
-    piy[g] = p.S == 2 ? 2 * cy + py : cy;
+  for (int i = 0; i < BQ; ++i) {
+    const int m = m0 + t / 4 + 64 * i;
+    rv[i] = m < M;
+    const int mm = rv[i] ? m : 0;
+    const int cx = mm % Wr, cy = (mm / Wr) % Hr;
+    rn[i] = mm / (Wr * Hr);
+    rx[i] = cls ? 2 * cx + px : cx;
+    ry[i] = cls ? 2 * cy + py : cy;
   }
-  f32x4 acc[2][CIG];
+  bf16x8 rb[BQ], ra[AQ];
+  auto gload = [&](int s) {
+    const int ti = s / KC, c0 = (s - ti * KC) * 32 + 8 * part;
+    const int ky = ky0 + ks * (ti / nkx), kx = kx0 + ks * (ti % nkx);
+    const int tap = 3 * ky + kx;
 #pragma unroll
-  for (int g = 0; g < 2; ++g)
-#pragma unroll
-    for (int c = 0; c < CIG; ++c) acc[g][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  // taps of this class: stride 1 all nine; stride 2 ky in {1} (even rows) or {0, 2} (odd rows)
-  const int ky_first = p.S == 2 ? (py == 0 ? 1 : 0) : 0, ky_step = p.S == 2 ? 2 : 1;
-  const int kx_first = p.S == 2 ? (px == 0 ? 1 : 0) : 0, kx_step = p.S == 2 ? 2 : 1;
-  for (int ky = ky_first; ky < 3; ky += ky_step) {
-    for (int kx = kx_first; kx < 3; kx += kx_step) {
-      const int tap = 3 * ky + kx;
-      int64_t base[2];
-      bool inb[2];
-#pragma unroll
-      for (int g = 0; g < 2; ++g) {
-        const int ty = piy[g] + 1 - ky, tx = pix[g] + 1 - kx;
+    for (int i = 0; i < BQ; ++i) {
+      bool ok;
+      int64_t off;
+      if constexpr (MODE == 0) {
+        const int iy = ry[i] * p.S + ky - 1, ix = rx[i] * p.S + kx - 1;
+        ok = rv[i] && iy >= 0 && iy < p.H && ix >= 0 && ix < p.W;
+        off = ((static_cast<int64_t>(rn[i]) * p.H + iy) * p.W + ix) * p.Cin + c0;
+      } else {
+        const int ty = ry[i] + 1 - ky, tx = rx[i] + 1 - kx;
         const int oy = ty / p.S, ox = tx / p.S;
-        inb[g] = pv[g] && ty >= 0 && tx >= 0 && ty == oy * p.S && tx == ox * p.S && oy < p.Ho && ox < p.Wo;
-        base[g] = inb[g] ? ((static_cast<int64_t>(pn[g]) * p.Ho + oy) * p.Wo + ox) * p.Cout + kq : 0;
+        ok = rv[i] && ty >= 0 && tx >= 0 && ty == oy * p.S && tx == ox * p.S && oy < p.Ho && ox < p.Wo;
+        off = ((static_cast<int64_t>(rn[i]) * p.Ho + oy) * p.Wo + ox) * p.Cout + c0;
       }
-      const __bf16* wrow[CIG];
+      rb[i] = ok ? load8((MODE == 0 ? p.x : p.dy) + off) : zero8();
+    }
 #pragma unroll
-      for (int c = 0; c < CIG; ++c)
-        wrow[c] = p.wt + (static_cast<int64_t>(tap) * p.Cin + ci0 + pair_row(l, c)) * p.Cout + kq;
-      for (int co0 = 0; co0 < p.Cout; co0 += 32) {
-        bf16x8 a[2], b[CIG];
+    for (int i = 0; i < AQ; ++i) {
+      const int r = t / 4 + 64 * i;
+      if (r < BN) {
+        if constexpr (MODE == 0)
+          ra[i] = load8(p.w + (static_cast<int64_t>(ch0 + r) * 9 + tap) * p.Cin + c0);
+        else
+          ra[i] = load8(p.wt + (static_cast<int64_t>(tap) * p.Cin + ch0 + r) * p.Cout + c0);
+      }
+    }
+  };
+  auto lstore = [&](int b) {
 #pragma unroll
-        for (int g = 0; g < 2; ++g) a[g] = inb[g] ? load8(p.dy + base[g] + co0) : zero8();
+    for (int i = 0; i < BQ; ++i) *reinterpret_cast<bf16x8*>(&sB[b][t / 4 + 64 * i][8 * part]) = rb[i];
 #pragma unroll
-        for (int c = 0; c < CIG; ++c) b[c] = load8(wrow[c] + co0);
+    for (int i = 0; i < AQ; ++i)
+      if (t / 4 + 64 * i < BN) *reinterpret_cast<bf16x8*>(&sA[b][t / 4 + 64 * i][8 * part]) = ra[i];
+  };
+
+  f32x4 acc[4][CG];
 #pragma unroll
-        for (int g = 0; g < 2; ++g)
+  for (int g = 0; g < 4; ++g)
 #pragma unroll
-          for (int c = 0; c < CIG; ++c) acc[g][c] = mfma(b[c], a[g], acc[g][c]);
+    for (int c = 0; c < CG; ++c) acc[g][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int kq = 8 * (l >> 4);
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int s = 0; s < T; ++s) {
+    const int b = s & 1;
+    if (s + 1 < T) gload(s + 1);
+    bf16x8 bx[4], ax[CG];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) bx[g] = *reinterpret_cast<const bf16x8*>(&sB[b][wm * 64 + 16 * g + (l & 15)][kq]);
+#pragma unroll
+    for (int c = 0; c < CG; ++c)
+      ax[c] = *reinterpret_cast<const bf16x8*>(&sA[b][wn * 16 * CG + pair_row(l, c)][kq]);
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+      for (int c = 0; c < CG; ++c) acc[g][c] = mfma(ax[c], bx[g], acc[g][c]);
+    if (s + 1 < T) lstore(b ^ 1);
+    __syncthreads();
+  }
+
+  // lane l: channels cb .. cb + 7 (cb = ch0 + 16 CG wn + 32 h + 8 (l / 16)) of row 16 g + l % 16
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int m = m0 + wm * 64 + 16 * g + (l & 15);
+    if (m >= M) continue;
+    int64_t at;
+    if constexpr (MODE == 0) {
+      at = static_cast<int64_t>(m) * p.Cout;
+    } else {
+      const int cx = m % Wr, cy = (m / Wr) % Hr, n = m / (Wr * Hr);
+      const int ix = cls ? 2 * cx + px : cx, iy = cls ? 2 * cy + py : cy;
+      at = ((static_cast<int64_t>(n) * p.H + iy) * p.W + ix) * p.Cin;
+    }
+#pragma unroll
+    for (int h = 0; h < CG / 2; ++h) {
+      const int cb = ch0 + 16 * CG * wn + 32 * h + kq;
+      bf16x8 v;
+      if constexpr (MODE == 0) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          float u = acc[g][2 * h + j / 4][j % 4] + (p.bias ? p.bias[cb + j] : 0.0f);
+          if (p.relu) u = fmaxf(u, 0.0f);
+          v[j] = static_cast<__bf16>(u);
+        }
+        *reinterpret_cast<bf16x8*>(p.y + at + cb) = v;
+      } else {
+        const bf16x8 xm = p.mask ? load8(p.x + at + cb) : zero8();
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          float u = acc[g][2 * h + j / 4][j % 4];
+          if (p.mask && !(static_cast<float>(xm[j]) > 0.0f)) u = 0.0f;
+          v[j] = static_cast<__bf16>(u);
+        }
+        *reinterpret_cast<bf16x8*>(p.dx + at + cb) = v;
       }
     }
   }
-  // lane l: channels ci0 + 32 h + 8 (l / 16) + 0..7 of pixel l % 16 of each group
-#pragma unroll
-  for (int g = 0; g < 2; ++g) {
-    if (!pv[g]) continue;
-    const int64_t at = ((static_cast<int64_t>(pn[g]) * p.H + piy[g]) * p.W + pix[g]) * p.Cin;
-#pragma unroll
-    for (int h = 0; h < CIG / 2; ++h) {
-      const int cb = ci0 + 32 * h + 8 * (l / 16);
-      const bf16x8 xm = p.mask ? load8(p.x + at + cb) : zero8();
-      bf16x8 v;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        float u = acc[g][2 * h + j / 4][j % 4];
-        if (p.mask && !(static_cast<float>(xm[j]) > 0.0f)) u = 0.0f;
-        v[j] = static_cast<__bf16>(u);
-      }
-      *reinterpret_cast<bf16x8*>(p.dx + at + cb) = v;
-    }
+}
+
+template <int MODE>
+void launch_igemm(const ConvP& p, int rows, int cols, int classes, hipStream_t st) {
+  if (cols % 128 == 0) {
+    hipLaunchKernelGGL((k_igemm<MODE, 2, 4>), dim3((rows + 127) / 128, cols / 128, classes), dim3(kThreads), 0, st, p);
+  } else if (cols % 64 == 0) {
+    hipLaunchKernelGGL((k_igemm<MODE, 4, 4>), dim3((rows + 255) / 256, cols / 64, classes), dim3(kThreads), 0, st, p);
+  } else {
+    hipLaunchKernelGGL((k_igemm<MODE, 4, 2>), dim3((rows + 255) / 256, cols / 32, classes), dim3(kThreads), 0, st, p);
   }
 }
 
@@ -528,16 +587,15 @@ int ndrr_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y, i
   p.y = static_cast<__bf16*>(y);
   p.N = N; p.H = H; p.W = W; p.Cin = Cin; p.Ho = Ho; p.Wo = Wo; p.Cout = Cout; p.S = stride; p.relu = relu != 0;
   const int M = N * Ho * Wo;
-  const int blocks = (M + 4 * kPixelsPerWave - 1) / (4 * kPixelsPerWave);
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const bool wide = Cout % 64 == 0;
-  const dim3 grid(blocks, Cout / (wide ? 64 : 32));
   if (Cin == 1) {
-    if (wide) hipLaunchKernelGGL((k_conv_fwd<4, true>), grid, dim3(kThreads), 0, st, p);
-    else hipLaunchKernelGGL((k_conv_fwd<2, true>), grid, dim3(kThreads), 0, st, p);
+    const int blocks = (M + 4 * kPixelsPerWave - 1) / (4 * kPixelsPerWave);
+    const bool wide = Cout % 64 == 0;
+    const dim3 grid(blocks, Cout / (wide ? 64 : 32));
+    if (wide) hipLaunchKernelGGL((k_conv_fwd<4>), grid, dim3(kThreads), 0, st, p);
+    else hipLaunchKernelGGL((k_conv_fwd<2>), grid, dim3(kThreads), 0, st, p);
   } else {
-    if (wide) hipLaunchKernelGGL((k_conv_fwd<4, false>), grid, dim3(kThreads), 0, st, p);
-    else hipLaunchKernelGGL((k_conv_fwd<2, false>), grid, dim3(kThreads), 0, st, p);
+    launch_igemm<0>(p, M, Cout, 1, st);
   }
   return status();
 }
@@ -565,12 +623,7 @@ int ndrr_conv3x3_bwd_data(const void* dy, const void* wt, const void* x, void* d
   p.mask = relu_mask != 0;
   // pixels per class: all of them (stride 1), or the largest parity class (stride 2)
   const int M = stride == 2 ? N * ((H + 1) / 2) * ((W + 1) / 2) : N * H * W;
-  const int blocks = (M + 4 * kPixelsPerWave - 1) / (4 * kPixelsPerWave);
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  const bool wide = Cin % 64 == 0;
-  const dim3 grid(blocks, Cin / (wide ? 64 : 32), stride == 2 ? 4 : 1);
-  if (wide) hipLaunchKernelGGL(k_conv_bwd_data<4>, grid, dim3(kThreads), 0, st, p);
-  else hipLaunchKernelGGL(k_conv_bwd_data<2>, grid, dim3(kThreads), 0, st, p);
+  launch_igemm<1>(p, M, Cin, stride == 2 ? 4 : 1, static_cast<hipStream_t>(stream));
   return status();
 }
 
