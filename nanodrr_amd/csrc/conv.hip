@@ -43,6 +43,13 @@ constexpr int kThreads = 256;      // four waves
 constexpr int kPixelsPerWave = 32; // two 16-row MFMA tiles
 constexpr int kWgradBlocks = 1024; // weight-grad workgroups aimed at (the split size follows)
 
+// LDS buffers of the weight-grad slabs (1 or 2). The kernel is held to two
+// waves per SIMD (register-bound); one buffer then lets two workgroups share
+// a CU, which measured as fast as two buffers with one.
+#ifndef NDRR_WGRAD_BUFFERS
+#define NDRR_WGRAD_BUFFERS 1
+#endif
+
 struct ConvP {
   const __bf16* x;    // [N][H][W][Cin]
   const __bf16* w;    // [Cout][9][Cin]      (forward)
@@ -334,82 +341,160 @@ void launch_igemm(const ConvP& p, int rows, int cols, int classes, hipStream_t s
 }
 
 // ---------------------------------------------------------------------------
-// Weight gradient for Cin % 32 == 0, Cout % 64 == 0. Block (split, ci tile,
-// co tile): 64 output channels x 32 input channels x all nine taps over
-// p.split output pixels, in slabs of 32 pixels staged through LDS with the
-// pixels contiguous (the MFMA's K): the dy slab once, the input slab once per
-// tap. Wave w owns output channels 16w..16w+15 (18 MFMAs per slab).
+// Weight gradients.
 // ---------------------------------------------------------------------------
-constexpr int kLdsRow = 48;  // 32 pixels + 16 of padding: rows stay 16-byte aligned
+constexpr int kLdsRow = 48;  // one-channel layer: 32 pixels + 16 of padding, rows stay 16-byte aligned
 
-__global__ __launch_bounds__(kThreads) void k_conv_bwd_weight(ConvP p) {
-  __shared__ __attribute__((aligned(16))) __bf16 sdy[64][kLdsRow];     // [co][pixel]
-  __shared__ __attribute__((aligned(16))) __bf16 sx[9][32][kLdsRow];   // [tap][ci][pixel]
+// Weight gradient for Cin % 32 == 0, Cout % 64 == 0, from LDS transpose
+// reads. Block (split, CI = 16 CIT input channels, 64 output channels) over
+// p.split output pixels, in slabs of 32 pixels: the dy slab [pixel][64] and
+// the nine tap slabs of x [tap][pixel][CI] are staged in their NHWC order
+// with 16-byte stores, and the MFMA operands (8 consecutive pixels of one
+// channel per lane) come from pairs of 64-bit transpose reads: within each
+// 16-lane group the read gives lane i element i % 4 of the 8-byte pieces
+// that lanes 4 j + i / 4 address (j = 0..3, checked on the device), so lane
+// (i, b) addressing pixel row kb + 8 b + i / 4, channels 4 (i % 4)..+3 of a
+// 16-channel block, receives pixels kb + 8 b + 0..3 of channel i. Rows keep
+// their 32-byte channel blocks XOR-swizzled by the row so that these reads
+// hit distinct banks (a 64-lane 8-byte read in two 32-lane halves, modelled
+// on the CPU). Wave w takes all four 16-channel output tiles for the
+// (tap, input tile) units w, w + 4, ... (nine units per wave when CIT = 4),
+// so the dy operands are read once per wave and slab. With bias_grad, wave w
+// also sums dy of output tile w against a ones operand. The next slab's
+// global loads are in flight during this slab's MFMAs (NB = 2: two LDS
+// buffers and one barrier per slab; NB = 1: one buffer and two barriers).
+__device__ inline int swz(int row, int blk, int row_bytes) {
+  return row_bytes == 64 ? blk ^ ((row >> 3) & 1) : blk ^ (((row >> 1) & 1) | ((row >> 2) & 2));
+}
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+
+// Eight consecutive pixel rows (kb + 8 (l / 16) + 0..7) of channel 16 blk + l % 16 from a slab of
+// 32 rows of row_bytes bytes at `base`.
+__device__ inline bf16x8 read_tr(const char* base, int row_bytes, int blk, int l) {
+  const int i = l & 15, b = l >> 4;
+  s16x4 h[2];
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb) {
+    const int row = 4 * kb + 8 * b + (i >> 2);
+    const char* a = base + row * row_bytes + 32 * swz(row, blk, row_bytes) + 8 * (i & 3);
+    h[kb] = __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) s16x4*)(a));
+  }
+  const s16x8 v = __builtin_shufflevector(h[0], h[1], 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+template <int CIT, int NB>
+__global__ __launch_bounds__(kThreads, 2) void k_conv_bwd_weight(ConvP p) {
+  constexpr int CI = 16 * CIT, XRB = 2 * CI;  // x row bytes (64 or 128)
+  constexpr int UNITS = 9 * CIT, U = (UNITS + 3) / 4;
+  constexpr int XQ = (9 * 32 * CI / 8 + kThreads - 1) / kThreads;  // 16-byte x chunks per thread
+  constexpr int DYB = 32 * 128, XB = 9 * 32 * XRB;                // slab bytes
+  __shared__ __attribute__((aligned(16))) char lds[NB][DYB + XB];
   const int t = threadIdx.x, wave = t >> 6, l = t & 63;
   const int split = blockIdx.x;
-  const int ci0 = blockIdx.y * 32, co0 = blockIdx.z * 64;
+  const int ci0 = blockIdx.y * CI, co0 = blockIdx.z * 64;
   const int M = p.N * p.Ho * p.Wo;
   const int p_begin = split * p.split, p_end = min(M, p_begin + p.split);
-  const int kq = 8 * (l / 16);
-  const bool bias = p.bias_grad && blockIdx.y == 0 && t < 64;  // thread t sums dy of channel co0 + t
-  float db = 0.0f;
-  f32x4 acc[9][2];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) acc[k][0] = acc[k][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  for (int pb = p_begin; pb < p_end; pb += 32) {
-    {  // dy slab: 32 pixels x 64 channels, 8 channels per thread
-      const int px = t / 8, c8 = 8 * (t % 8);
-      const int m = pb + px;
-      const bf16x8 v = m < p_end ? load8(p.dy + static_cast<int64_t>(m) * p.Cout + co0 + c8) : zero8();
-#pragma unroll
-      for (int j = 0; j < 8; ++j) sdy[c8 + j][px] = v[j];
+  const int slabs = (p_end - p_begin + 31) / 32;
+  const bool bias = p.bias_grad && blockIdx.y == 0;
+
+  // this thread's chunks: dy (pixel t / 8, piece t % 8); x (pixel, piece) fixed over the taps
+  const int dpx = t >> 3, dpc = t & 7;
+  const int xpx = CIT == 4 ? t >> 3 : (t & 127) >> 2, xpc = CIT == 4 ? t & 7 : t & 3;
+  bf16x8 rdy, rx[XQ];
+  auto gload = [&](int s) {
+    const int pb = p_begin + 32 * s;
+    {
+      const int m = pb + dpx;
+      rdy = m < p_end ? load8(p.dy + static_cast<int64_t>(m) * p.Cout + co0 + 8 * dpc) : zero8();
     }
-    {  // input slabs of the nine taps: 32 pixels x 32 channels each; thread (pixel, 8 channels, tap parity)
-      const int px = (t % 128) / 4, c8 = 8 * (t % 4), half = t / 128;
-      const int m = pb + px;
-      const bool in_range = m < p_end;
-      const int mm = in_range ? m : 0;
-      const int ox = mm % p.Wo, oy = (mm / p.Wo) % p.Ho, n = mm / (p.Wo * p.Ho);
-      for (int tap = half; tap < 9; tap += 2) {
-        const int iy = oy * p.S + tap / 3 - 1, ix = ox * p.S + tap % 3 - 1;
-        bf16x8 v = zero8();
-        if (in_range && iy >= 0 && iy < p.H && ix >= 0 && ix < p.W)
-          v = load8(p.x + ((static_cast<int64_t>(n) * p.H + iy) * p.W + ix) * p.Cin + ci0 + c8);
+    const int m = pb + xpx;
+    const bool in_range = m < p_end;
+    const int mm = in_range ? m : 0;
+    const int ox = mm % p.Wo, oy = (mm / p.Wo) % p.Ho, n = mm / (p.Wo * p.Ho);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) sx[tap][c8 + j][px] = v[j];
+    for (int i = 0; i < XQ; ++i) {
+      const int tap = CIT == 4 ? i : 2 * i + (t >> 7);
+      const int iy = oy * p.S + tap / 3 - 1, ix = ox * p.S + tap % 3 - 1;
+      rx[i] = in_range && tap < 9 && iy >= 0 && iy < p.H && ix >= 0 && ix < p.W
+                  ? load8(p.x + ((static_cast<int64_t>(n) * p.H + iy) * p.W + ix) * p.Cin + ci0 + 8 * xpc)
+                  : zero8();
+    }
+  };
+  auto lstore = [&](int b) {
+    char* base = lds[b];
+    *reinterpret_cast<bf16x8*>(base + dpx * 128 + 32 * swz(dpx, dpc >> 1, 128) + 16 * (dpc & 1)) = rdy;
+#pragma unroll
+    for (int i = 0; i < XQ; ++i) {
+      const int tap = CIT == 4 ? i : 2 * i + (t >> 7);
+      if (tap < 9)
+        *reinterpret_cast<bf16x8*>(base + DYB + tap * 32 * XRB + xpx * XRB + 32 * swz(xpx, xpc >> 1, XRB) +
+                                   16 * (xpc & 1)) = rx[i];
+    }
+  };
+
+  f32x4 acc[U][4];
+  f32x4 accb = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int k = 0; k < U; ++k)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[k][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  bf16x8 ones;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) ones[j] = static_cast<__bf16>(1.0f);
+
+  if (slabs > 0) {
+    gload(0);
+    lstore(0);
+  }
+  __syncthreads();
+  for (int s = 0; s < slabs; ++s) {
+    const int b = NB == 2 ? (s & 1) : 0;
+    if (s + 1 < slabs) gload(s + 1);
+    const char* base = lds[b];
+    bf16x8 a[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] = read_tr(base, 128, c, l);
+    if (bias) accb = mfma(read_tr(base, 128, wave, l), ones, accb);  // a[wave] would select per element
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int u = wave + 4 * k;
+      if (u < UNITS) {
+        const int tap = u / CIT, tile = u % CIT;
+        const bf16x8 bx = read_tr(base + DYB + tap * 32 * XRB, XRB, tile, l);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[k][c] = mfma(a[c], bx, acc[k][c]);
       }
     }
-    __syncthreads();
-    if (bias) {
-#pragma unroll
-      for (int q = 0; q < 32; q += 8) {
-        const bf16x8 v = *reinterpret_cast<const bf16x8*>(&sdy[t][q]);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) db += static_cast<float>(v[j]);
-      }
+    if constexpr (NB == 2) {
+      if (s + 1 < slabs) lstore(b ^ 1);
+      __syncthreads();
+    } else {
+      __syncthreads();
+      if (s + 1 < slabs) lstore(0);
+      __syncthreads();
     }
-    const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sdy[16 * wave + (l % 16)][kq]);
-#pragma unroll
-    for (int tap = 0; tap < 9; ++tap)
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sx[tap][16 * h + (l % 16)][kq]);
-        acc[tap][h] = mfma(a, b, acc[tap][h]);
-      }
-    __syncthreads();
   }
   // partial[split][tap][co][ci], then partial[split][9 Cout Cin + co] for the bias
   float* part = p.part + static_cast<int64_t>(split) * (9 * p.Cout * p.Cin + p.Cout);
 #pragma unroll
-  for (int tap = 0; tap < 9; ++tap)
+  for (int k = 0; k < U; ++k) {
+    const int u = wave + 4 * k;
+    if (u >= UNITS) continue;
+    const int tap = u / CIT, tile = u % CIT;
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
+    for (int c = 0; c < 4; ++c)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int co = co0 + 16 * wave + 4 * (l / 16) + r, ci = ci0 + 16 * h + (l % 16);
-        part[(tap * p.Cout + co) * p.Cin + ci] = acc[tap][h][r];
+        const int co = co0 + 16 * c + 4 * (l >> 4) + r, ci = ci0 + 16 * tile + (l & 15);
+        part[(tap * p.Cout + co) * p.Cin + ci] = acc[k][c][r];
       }
-  if (bias) part[9 * p.Cout * p.Cin + co0 + t] = db;
+  }
+  if (bias && (l & 15) == 0)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) part[9 * p.Cout * p.Cin + co0 + 16 * wave + 4 * (l >> 4) + r] = accb[r];
 }
 
 // Weight gradient of the one-channel first layer (Cout == 32): a GEMM with the
@@ -561,8 +646,11 @@ int wgrad_blocks() {
   return blocks;
 }
 
+// Input channels per weight-grad workgroup: 64 when they divide Cin, else 32.
+int wgrad_ci(int Cin) { return Cin % 64 == 0 ? 64 : 32; }
+
 int wgrad_split(int M, int Cin, int Cout) {
-  const int tiles = Cin == 1 ? 1 : (Cin / 32) * (Cout / 64);
+  const int tiles = Cin == 1 ? 1 : (Cin / wgrad_ci(Cin)) * (Cout / 64);
   const int want = (wgrad_blocks() + tiles - 1) / tiles;
   int per = (M + want - 1) / want;
   per = ((per + 127) / 128) * 128;
@@ -654,7 +742,9 @@ int ndrr_conv3x3_bwd_weight(const void* dy, const void* x, float* workspace, flo
   if (Cin == 1) {
     hipLaunchKernelGGL(k_conv_bwd_weight_cin1, dim3(splits), dim3(kThreads), 0, st, p);
   } else {
-    hipLaunchKernelGGL(k_conv_bwd_weight, dim3(splits, Cin / 32, Cout / 64), dim3(kThreads), 0, st, p);
+    const dim3 grid(splits, Cin / wgrad_ci(Cin), Cout / 64);
+    if (wgrad_ci(Cin) == 64) hipLaunchKernelGGL((k_conv_bwd_weight<4, NDRR_WGRAD_BUFFERS>), grid, dim3(kThreads), 0, st, p);
+    else hipLaunchKernelGGL((k_conv_bwd_weight<2, NDRR_WGRAD_BUFFERS>), grid, dim3(kThreads), 0, st, p);
   }
   int r = status();
   if (r != NDRR_OK) return r;
