@@ -320,8 +320,9 @@ int ndrr_zncc(const float* x, const float* y, int B, int N, float eps, float* wo
  *     pixels of dy, through a 16-byte aligned workspace of
  *     ndrr_conv3x3_bwd_weight_workspace_floats floats (split partials added
  *     in a fixed order: deterministic); Cout a multiple of 64, or 32 when
- *     Cin == 1. The environment variable NDRR_WGRAD_BLOCKS (64-65536,
- *     default 1024) sets the number of workgroups the pixels are split over.
+ *     Cin == 1. The environment variable NDRR_WGRAD_BLOCKS (64-65536;
+ *     default 512, 1024 when Cin == 1) sets the number of workgroups the
+ *     pixels are split over.
  */
 int ndrr_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y,
                      int N, int H, int W, int Cin, int Cout, int stride, int relu,

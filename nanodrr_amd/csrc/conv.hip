@@ -41,7 +41,11 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
 constexpr int kThreads = 256;      // four waves
 constexpr int kPixelsPerWave = 32; // two 16-row MFMA tiles
-constexpr int kWgradBlocks = 1024; // weight-grad workgroups aimed at (the split size follows)
+// Weight-grad workgroups aimed at (the split size follows): two per CU, all
+// resident at once, for the MFMA kernel (more would only add split partials
+// to sum); four per CU for the memory-bound one-channel layer.
+constexpr int kWgradBlocks = 512;
+constexpr int kWgradBlocksCin1 = 1024;
 
 // LDS buffers of the weight-grad slabs (1 or 2). The kernel is held to two
 // waves per SIMD (register-bound); one buffer then lets two workgroups share
@@ -637,13 +641,13 @@ bool shape_ok(int N, int H, int W, int Cin, int Cout, int stride) {
 
 // Output pixels per weight-grad workgroup: enough workgroups to fill the chip
 // (about kWgradBlocks over the channel tiles), in whole 128-pixel steps.
-int wgrad_blocks() {
-  static const int blocks = [] {
+int wgrad_blocks(int Cin) {
+  static const int forced = [] {
     const char* v = getenv("NDRR_WGRAD_BLOCKS");
-    const int b = v ? atoi(v) : kWgradBlocks;
-    return b >= 64 && b <= 65536 ? b : kWgradBlocks;
+    const int b = v ? atoi(v) : 0;
+    return b >= 64 && b <= 65536 ? b : 0;
   }();
-  return blocks;
+  return forced ? forced : (Cin == 1 ? kWgradBlocksCin1 : kWgradBlocks);
 }
 
 // Input channels per weight-grad workgroup: 64 when they divide Cin, else 32.
@@ -651,7 +655,7 @@ int wgrad_ci(int Cin) { return Cin % 64 == 0 ? 64 : 32; }
 
 int wgrad_split(int M, int Cin, int Cout) {
   const int tiles = Cin == 1 ? 1 : (Cin / wgrad_ci(Cin)) * (Cout / 64);
-  const int want = (wgrad_blocks() + tiles - 1) / tiles;
+  const int want = (wgrad_blocks(Cin) + tiles - 1) / tiles;
   int per = (M + want - 1) / want;
   per = ((per + 127) / 128) * 128;
   return per < 256 ? 256 : per;
