@@ -391,10 +391,13 @@ def _setup_context_jac(ctx, inputs, output):
     vol, lab, M, src, tgt, step, n_samples, n_classes, width, jac_planes, packed, layout = inputs
     ctx.save_for_backward(M, src, tgt, step, output[1])
     ctx.mark_non_differentiable(output[1])
+    ctx.set_materialize_grads(False)  # no zero-filled gradient for the (large) jac planes
     ctx.width = width
 
 
 def _backward_jac(ctx, go, g_jac):
+    if go is None:
+        return (None,) * 12
     M, src, tgt, step, jac = ctx.saved_tensors
     if ctx.needs_input_grad[0]:
         raise RuntimeError("fused_raymarch_jac gives no volume gradient; use fused_raymarch for it")

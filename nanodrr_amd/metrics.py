@@ -45,10 +45,14 @@ def _(x, y, eps):
 
 def _zncc_setup(ctx, inputs, output):
     ctx.save_for_backward(output[1])
+    ctx.mark_non_differentiable(output[1])
+    ctx.set_materialize_grads(False)  # no zero-filled gradient for the unused per-pixel output
     ctx.x_dtype = inputs[0].dtype
 
 
 def _zncc_backward(ctx, g_score, g_grad):
+    if g_score is None:
+        return None, None, None
     (grad,) = ctx.saved_tensors
     gx = grad * g_score.reshape(-1, *([1] * (grad.dim() - 1)))
     return gx.to(ctx.x_dtype), None, None

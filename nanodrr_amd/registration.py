@@ -258,12 +258,13 @@ def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot
     B = target.shape[0]
     score = torch.zeros(B, device=target.device)
     gen = torch.Generator(device=target.device).manual_seed(int(seed)) if jitter else None
+    descent = torch.full((B,), -1.0, device=target.device)  # d(-sum of scores) / d score
     for it in range(n_iters):
         for p, _ in params:
             p.grad = None
         offset = torch.rand(B, device=target.device, generator=gen) - 0.5 if jitter else None
         s = zncc(reg(n_samples, offset=offset), target, eps)
-        (-s.sum()).backward()
+        s.backward(descent)
         frac = final_frac + (1 - final_frac) * 0.5 * (1 + math.cos(math.pi * it / max(n_iters - 1, 1)))
         with torch.no_grad():
             for (p, lr), (m, v) in zip(params, moments):
