@@ -185,10 +185,10 @@ class DrrSource:
         M = pose_compose(reg._prefix.expand(B, 4, 4).contiguous(), reg._suffix.expand(B, 4, 4).contiguous(),
                          rot.float().contiguous(), xyz.float().contiguous())
         src, tgt = reg._src, reg._tgt
-        step = (tgt - src).norm(dim=-1) / float(S - 1)
+        step, delta = reg._ray_steps(S)
         if offset is not None:
-            d = (tgt - src) * (offset.reshape(-1, 1, 1) / float(S - 1))
-            src, tgt = src + d, tgt + d
+            u = offset.reshape(-1, 1, 1).to(tgt.dtype)
+            src, tgt = torch.addcmul(src, delta, u), torch.addcmul(tgt, delta, u)
         from .renderer import march
 
         return march(reg.subject, M, src, tgt, step, S, reg.height, reg.width)
@@ -222,7 +222,7 @@ def train(steps: int = 20000, batch: int = 256, seeds=tuple(range(8)), rot_deg: 
 
     src = DrrSource(seeds, device=device)
     net = PoseNet(seed=seed).to(device)
-    opt = torch.optim.Adam(net.parameters(), lr=lr)
+    opt = torch.optim.Adam(net.parameters(), lr=lr, fused=True)  # one launch for all parameters
     sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, steps, eta_min=lr * 0.02)
     gen = torch.Generator(device=device).manual_seed(seed)
     per = batch // len(seeds)
