@@ -41,6 +41,7 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
 constexpr int kThreads = 256;      // four waves
 constexpr int kPixelsPerWave = 32; // two 16-row MFMA tiles
+constexpr int kCin1Groups = 8;     // 32-pixel groups per wave in the one-channel forward
 // Weight-grad workgroups aimed at (the split size follows): two per CU, all
 // resident at once, for the MFMA kernel (more would only add split partials
 // to sum); four per CU for the memory-bound one-channel layer.
@@ -89,81 +90,75 @@ __device__ inline int pair_row(int l, int c) {
 // ---------------------------------------------------------------------------
 // Forward of the one-channel first layer: K = the nine taps of the single
 // input channel, zero-padded to 32, so one MFMA per tile and no K loop.
-// Block: 4 waves x 32 output pixels; 16 * COG output channels.
+// Block: 4 waves, each over kCin1Groups groups of 32 output pixels; 16 * COG
+// output channels.
 // ---------------------------------------------------------------------------
 template <int COG>
 __global__ __launch_bounds__(kThreads) void k_conv_fwd(ConvP p) {
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int M = p.N * p.Ho * p.Wo;
-  const int m0 = (blockIdx.x * 4 + wave) * kPixelsPerWave;
   const int co0 = blockIdx.y * 16 * COG;
   const int kq = 8 * (l / 16);
-  int pn[2], poy[2], pox[2];
-  bool pv[2];
+  // weights and bias once per wave; the wave then walks kCin1Groups groups of 32 pixels
+  bf16x8 b[COG];
 #pragma unroll
-  for (int g = 0; g < 2; ++g) {
-    const int m = m0 + 16 * g + (l % 16);
-    pv[g] = m < M;
-    const int mm = pv[g] ? m : 0;
-    pox[g] = mm % p.Wo;
-    poy[g] = (mm / p.Wo) % p.Ho;
-    pn[g] = mm / (p.Wo * p.Ho);
+  for (int c = 0; c < COG; ++c) {
+    const int co = co0 + pair_row(l, c);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int tap = kq + j;
+      b[c][j] = tap < 9 ? p.w[co * 9 + tap] : static_cast<__bf16>(0.0f);
+    }
   }
-  f32x4 acc[2][COG];
+  float bv[COG / 2][8];
 #pragma unroll
-  for (int g = 0; g < 2; ++g)
+  for (int h = 0; h < COG / 2; ++h)
 #pragma unroll
-    for (int c = 0; c < COG; ++c) acc[g][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-
-  {
-    bf16x8 a[2], b[COG];
+    for (int j = 0; j < 8; ++j) bv[h][j] = p.bias ? p.bias[co0 + 32 * h + kq + j] : 0.0f;
+  for (int it = 0; it < kCin1Groups; ++it) {
+    const int m0 = ((blockIdx.x * 4 + wave) * kCin1Groups + it) * kPixelsPerWave;
+    if (m0 >= M) break;
+    bf16x8 a[2];
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
+      const int m = m0 + 16 * g + (l % 16);
+      const bool pv = m < M;
+      const int mm = pv ? m : 0;
+      const int ox = mm % p.Wo, oy = (mm / p.Wo) % p.Ho, n = mm / (p.Wo * p.Ho);
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int tap = kq + j;
         float v = 0.0f;
-        if (tap < 9 && pv[g]) {
-          const int iy = poy[g] * p.S + tap / 3 - 1, ix = pox[g] * p.S + tap % 3 - 1;
+        if (tap < 9 && pv) {
+          const int iy = oy * p.S + tap / 3 - 1, ix = ox * p.S + tap % 3 - 1;
           if (iy >= 0 && iy < p.H && ix >= 0 && ix < p.W)
-            v = static_cast<float>(p.x[(static_cast<int64_t>(pn[g]) * p.H + iy) * p.W + ix]);
+            v = static_cast<float>(p.x[(static_cast<int64_t>(n) * p.H + iy) * p.W + ix]);
         }
         a[g][j] = static_cast<__bf16>(v);
       }
     }
-#pragma unroll
-    for (int c = 0; c < COG; ++c) {
-      const int co = co0 + pair_row(l, c);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int tap = kq + j;
-        b[c][j] = tap < 9 ? p.w[co * 9 + tap] : static_cast<__bf16>(0.0f);
-      }
-    }
+    f32x4 acc[2][COG];
 #pragma unroll
     for (int g = 0; g < 2; ++g)
 #pragma unroll
-      for (int c = 0; c < COG; ++c) acc[g][c] = mfma(b[c], a[g], acc[g][c]);
-  }
-  // Epilogue: bias, ReLU, bf16, one 16-byte NHWC store of eight channels per pixel.
+      for (int c = 0; c < COG; ++c) acc[g][c] = mfma(b[c], a[g], f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+    // bias, ReLU, bf16, one 16-byte NHWC store of eight channels per pixel
 #pragma unroll
-  for (int h = 0; h < COG / 2; ++h) {
-    const int cb = co0 + 32 * h + 8 * (l / 16);
-    float bv[8];
+    for (int h = 0; h < COG / 2; ++h) {
+      const int cb = co0 + 32 * h + kq;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) bv[j] = p.bias ? p.bias[cb + j] : 0.0f;
+      for (int g = 0; g < 2; ++g) {
+        const int m = m0 + 16 * g + (l % 16);
+        if (m < M) {
+          bf16x8 v;
 #pragma unroll
-    for (int g = 0; g < 2; ++g) {
-      const int m = m0 + 16 * g + (l % 16);
-      if (m < M) {
-        bf16x8 v;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          float u = acc[g][2 * h + j / 4][j % 4] + bv[j];
-          if (p.relu) u = fmaxf(u, 0.0f);
-          v[j] = static_cast<__bf16>(u);
+          for (int j = 0; j < 8; ++j) {
+            float u = acc[g][2 * h + j / 4][j % 4] + bv[h][j];
+            if (p.relu) u = fmaxf(u, 0.0f);
+            v[j] = static_cast<__bf16>(u);
+          }
+          *reinterpret_cast<bf16x8*>(p.y + static_cast<int64_t>(m) * p.Cout + cb) = v;
         }
-        *reinterpret_cast<bf16x8*>(p.y + static_cast<int64_t>(m) * p.Cout + cb) = v;
       }
     }
   }
@@ -274,6 +269,23 @@ __global__ __launch_bounds__(kThreads) void k_igemm(ConvP p) {
 #pragma unroll
     for (int c = 0; c < CG; ++c) acc[g][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   const int kq = 8 * (l >> 4);
+  // Input gradient: this lane's output offsets, and the ReLU mask loaded before the K loop so
+  // that its latency hides behind the MFMAs.
+  int64_t at[4];
+  bf16x8 xm[4][CG / 2];
+  if constexpr (MODE == 1) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int m = m0 + wm * 64 + 16 * g + (l & 15);
+      const int mm = m < M ? m : 0;
+      const int cx = mm % Wr, cy = (mm / Wr) % Hr, n = mm / (Wr * Hr);
+      const int ix = cls ? 2 * cx + px : cx, iy = cls ? 2 * cy + py : cy;
+      at[g] = ((static_cast<int64_t>(n) * p.H + iy) * p.W + ix) * p.Cin;
+#pragma unroll
+      for (int h = 0; h < CG / 2; ++h)
+        xm[g][h] = p.mask && m < M ? load8(p.x + at[g] + ch0 + 16 * CG * wn + 32 * h + kq) : zero8();
+    }
+  }
   gload(0);
   lstore(0);
   __syncthreads();
@@ -299,14 +311,6 @@ __global__ __launch_bounds__(kThreads) void k_igemm(ConvP p) {
   for (int g = 0; g < 4; ++g) {
     const int m = m0 + wm * 64 + 16 * g + (l & 15);
     if (m >= M) continue;
-    int64_t at;
-    if constexpr (MODE == 0) {
-      at = static_cast<int64_t>(m) * p.Cout;
-    } else {
-      const int cx = m % Wr, cy = (m / Wr) % Hr, n = m / (Wr * Hr);
-      const int ix = cls ? 2 * cx + px : cx, iy = cls ? 2 * cy + py : cy;
-      at = ((static_cast<int64_t>(n) * p.H + iy) * p.W + ix) * p.Cin;
-    }
 #pragma unroll
     for (int h = 0; h < CG / 2; ++h) {
       const int cb = ch0 + 16 * CG * wn + 32 * h + kq;
@@ -318,16 +322,15 @@ __global__ __launch_bounds__(kThreads) void k_igemm(ConvP p) {
           if (p.relu) u = fmaxf(u, 0.0f);
           v[j] = static_cast<__bf16>(u);
         }
-        *reinterpret_cast<bf16x8*>(p.y + at + cb) = v;
+        *reinterpret_cast<bf16x8*>(p.y + static_cast<int64_t>(m) * p.Cout + cb) = v;
       } else {
-        const bf16x8 xm = p.mask ? load8(p.x + at + cb) : zero8();
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
           float u = acc[g][2 * h + j / 4][j % 4];
-          if (p.mask && !(static_cast<float>(xm[j]) > 0.0f)) u = 0.0f;
+          if (p.mask && !(static_cast<float>(xm[g][h][j]) > 0.0f)) u = 0.0f;
           v[j] = static_cast<__bf16>(u);
         }
-        *reinterpret_cast<bf16x8*>(p.dx + at + cb) = v;
+        *reinterpret_cast<bf16x8*>(p.dx + at[g] + cb) = v;
       }
     }
   }
@@ -681,7 +684,8 @@ int ndrr_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y, i
   const int M = N * Ho * Wo;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (Cin == 1) {
-    const int blocks = (M + 4 * kPixelsPerWave - 1) / (4 * kPixelsPerWave);
+    const int per_block = 4 * kCin1Groups * kPixelsPerWave;
+    const int blocks = (M + per_block - 1) / per_block;
     const bool wide = Cout % 64 == 0;
     const dim3 grid(blocks, Cout / (wide ? 64 : 32));
     if (wide) hipLaunchKernelGGL((k_conv_fwd<4>), grid, dim3(kThreads), 0, st, p);
