@@ -269,23 +269,6 @@ __global__ __launch_bounds__(kThreads) void k_igemm(ConvP p) {
 #pragma unroll
     for (int c = 0; c < CG; ++c) acc[g][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   const int kq = 8 * (l >> 4);
-  // Input gradient: this lane's output offsets, and the ReLU mask loaded before the K loop so
-  // that its latency hides behind the MFMAs.
-  int64_t at[4];
-  bf16x8 xm[4][CG / 2];
-  if constexpr (MODE == 1) {
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const int m = m0 + wm * 64 + 16 * g + (l & 15);
-      const int mm = m < M ? m : 0;
-      const int cx = mm % Wr, cy = (mm / Wr) % Hr, n = mm / (Wr * Hr);
-      const int ix = cls ? 2 * cx + px : cx, iy = cls ? 2 * cy + py : cy;
-      at[g] = ((static_cast<int64_t>(n) * p.H + iy) * p.W + ix) * p.Cin;
-#pragma unroll
-      for (int h = 0; h < CG / 2; ++h)
-        xm[g][h] = p.mask && m < M ? load8(p.x + at[g] + ch0 + 16 * CG * wn + 32 * h + kq) : zero8();
-    }
-  }
   gload(0);
   lstore(0);
   __syncthreads();
@@ -311,6 +294,14 @@ __global__ __launch_bounds__(kThreads) void k_igemm(ConvP p) {
   for (int g = 0; g < 4; ++g) {
     const int m = m0 + wm * 64 + 16 * g + (l & 15);
     if (m >= M) continue;
+    int64_t at;
+    if constexpr (MODE == 0) {
+      at = static_cast<int64_t>(m) * p.Cout;
+    } else {
+      const int cx = m % Wr, cy = (m / Wr) % Hr, n = m / (Wr * Hr);
+      const int ix = cls ? 2 * cx + px : cx, iy = cls ? 2 * cy + py : cy;
+      at = ((static_cast<int64_t>(n) * p.H + iy) * p.W + ix) * p.Cin;
+    }
 #pragma unroll
     for (int h = 0; h < CG / 2; ++h) {
       const int cb = ch0 + 16 * CG * wn + 32 * h + kq;
@@ -322,15 +313,16 @@ __global__ __launch_bounds__(kThreads) void k_igemm(ConvP p) {
           if (p.relu) u = fmaxf(u, 0.0f);
           v[j] = static_cast<__bf16>(u);
         }
-        *reinterpret_cast<bf16x8*>(p.y + static_cast<int64_t>(m) * p.Cout + cb) = v;
+        *reinterpret_cast<bf16x8*>(p.y + at + cb) = v;
       } else {
+        const bf16x8 xm = p.mask ? load8(p.x + at + cb) : zero8();
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
           float u = acc[g][2 * h + j / 4][j % 4];
-          if (p.mask && !(static_cast<float>(xm[g][h][j]) > 0.0f)) u = 0.0f;
+          if (p.mask && !(static_cast<float>(xm[j]) > 0.0f)) u = 0.0f;
           v[j] = static_cast<__bf16>(u);
         }
-        *reinterpret_cast<bf16x8*>(p.dx + at[g] + cb) = v;
+        *reinterpret_cast<bf16x8*>(p.dx + at + cb) = v;
       }
     }
   }
