@@ -151,3 +151,38 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
         monkeypatch.undo()
         _lib.lib.cache_clear()
     assert isinstance(_lib.lib(), ctypes.CDLL)
+
+
+def test_zncc_workspace_is_one_record_per_chunk():
+    lib = _lib.lib()
+    # eight floats of chunk moments per 4096-pixel chunk per image
+    assert lib.ndrr_zncc_workspace_floats(3, 40000) == 3 * 10 * 8
+    assert lib.ndrr_zncc_workspace_floats(1, 4096) == 8
+    assert lib.ndrr_zncc_workspace_floats(2, 4097) == 2 * 2 * 8
+    assert lib.ndrr_zncc_workspace_floats(0, 100) == 0
+
+
+def test_conv_weight_workspace_holds_whole_split_records():
+    lib = _lib.lib()
+    for N, H, W, Cin, Cout, s in ((2, 16, 16, 32, 64, 2), (256, 32, 32, 128, 256, 2), (4, 9, 7, 1, 32, 1)):
+        rec = 9 * Cout * Cin + Cout  # weight partials, then the bias partials
+        n = lib.ndrr_conv3x3_bwd_weight_workspace_floats(N, H, W, Cin, Cout, s)
+        assert n >= rec and n % rec == 0, (N, H, W, Cin, Cout, s, n)
+    assert lib.ndrr_conv3x3_bwd_weight_workspace_floats(2, 16, 16, 48, 64, 1) == 0  # Cin not 1 or 32k
+
+
+def test_new_entry_points_reject_bad_arguments_before_launch():
+    lib = _lib.lib()
+    EINVAL = -1
+    fake = 4096  # never dereferenced: each call returns before any launch
+    # betas outside [0, 1), bias corrections <= 0
+    assert lib.ndrr_adam_step(fake, fake, fake, fake, 8, 1e-3, 1.0, 0.999, 1e-8, 0.1, 0.1, None) == EINVAL
+    assert lib.ndrr_adam_step(fake, fake, fake, fake, 8, 1e-3, 0.9, 0.999, 1e-8, 0.0, 0.1, None) == EINVAL
+    assert lib.ndrr_adam_step(None, fake, fake, fake, 8, 1e-3, 0.9, 0.999, 1e-8, 0.1, 0.1, None) == EINVAL
+    # negative eps, missing workspace
+    assert lib.ndrr_zncc(fake, fake, 2, 100, -1.0, fake, fake, None, None) == EINVAL
+    assert lib.ndrr_zncc(fake, fake, 2, 100, 1e-7, None, fake, None, None) == EINVAL
+    # a workspace that is not 16-byte aligned
+    assert lib.ndrr_conv3x3_bwd_weight(fake, fake, fake + 4, fake, None, 2, 16, 16, 32, 64, 2, None) == EINVAL
+    # Cout not a multiple of 64 for the MFMA weight gradient
+    assert lib.ndrr_conv3x3_bwd_weight(fake, fake, fake, fake, None, 2, 16, 16, 32, 96, 2, None) == EINVAL
