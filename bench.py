@@ -42,6 +42,7 @@ if str(ROOT) not in sys.path:
 import torch  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+SPIN_CYCLES = 20_000_000  # ~10 ms of GPU clock cycles: longer than the host needs to enqueue the timed launches
 SIZE, SPACING, N_CLASSES, BATCH, DET, S = 256, 1.0, 4, 16, 200, 256
 
 
@@ -141,6 +142,10 @@ def kernel_times(subject, k_inv, rt_inv, sdd, wts, reps, layout, adjoint="two"):
         fn()
         torch.cuda.synchronize()
         t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        # A bounded spin kernel ahead of the start event keeps the GPU busy while the host enqueues
+        # every timed launch, so the events bracket back-to-back kernels and not the host's launch
+        # latency (which exceeds the 20 us contraction and delays the first forward launches).
+        torch.cuda._sleep(SPIN_CYCLES)
         t0.record(stream)
         for _ in range(reps):
             fn()
