@@ -289,6 +289,37 @@ def _backward(ctx, go):
 torch.library.register_autograd("nanodrr_amd::fused_raymarch", _backward, setup_context=_setup_context)
 
 
+def _fused_raymarch_jac_impl(
+    vol: torch.Tensor,
+    lab: torch.Tensor,
+    M: torch.Tensor,
+    src: torch.Tensor,
+    tgt: torch.Tensor,
+    step: torch.Tensor,
+    n_samples: int,
+    n_classes: int,
+    width: int,
+    jac_planes: int,
+    packed: Optional[torch.Tensor] = None,
+    layout: str = "linear",
+) -> tuple[torch.Tensor, torch.Tensor]:
+    if jac_planes not in (6, 7):
+        raise ValueError(f"jac_planes must be 6 or 7, got {jac_planes}")
+    B, N, D, H, W, strides = _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width,
+                                       layout if packed is not None else "linear")
+    vptr, code = _volume_arg(vol, packed, layout)
+    out = torch.empty(B, n_classes, N, device=vol.device, dtype=torch.float32)
+    jac = torch.empty(B, jac_planes, n_classes, N, device=vol.device, dtype=torch.float32)
+    with torch.cuda.device(vol.device):
+        rc = _lib.lib().ndrr_raymarch_fwd_jac(
+            vptr, _DTYPE_CODE[vol.dtype], code, _ptr(lab) if n_classes > 1 else None,
+            M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(), out.data_ptr(), jac.data_ptr(),
+            jac_planes, B, N, D, H, W, n_samples, n_classes, width, *strides, _stream(vol.device),
+        )
+    _lib.check(rc, "ndrr_raymarch_fwd_jac")
+    return out, jac
+
+
 @torch.library.custom_op("nanodrr_amd::fused_raymarch_jac", mutates_args=())
 def fused_raymarch_jac(
     vol: torch.Tensor,
@@ -308,21 +339,8 @@ def fused_raymarch_jac(
     single-pass adjoint: jac (B, jac_planes, C, N) fp32 (ndrr_raymarch_fwd_jac).
     Its autograd backward reads jac instead of marching the volume again, so it
     gives pose and camera gradients but no volume gradient."""
-    if jac_planes not in (6, 7):
-        raise ValueError(f"jac_planes must be 6 or 7, got {jac_planes}")
-    B, N, D, H, W, strides = _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width,
-                                       layout if packed is not None else "linear")
-    vptr, code = _volume_arg(vol, packed, layout)
-    out = torch.empty(B, n_classes, N, device=vol.device, dtype=torch.float32)
-    jac = torch.empty(B, jac_planes, n_classes, N, device=vol.device, dtype=torch.float32)
-    with torch.cuda.device(vol.device):
-        rc = _lib.lib().ndrr_raymarch_fwd_jac(
-            vptr, _DTYPE_CODE[vol.dtype], code, _ptr(lab) if n_classes > 1 else None,
-            M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(), out.data_ptr(), jac.data_ptr(),
-            jac_planes, B, N, D, H, W, n_samples, n_classes, width, *strides, _stream(vol.device),
-        )
-    _lib.check(rc, "ndrr_raymarch_fwd_jac")
-    return out, jac
+    return _fused_raymarch_jac_impl(vol, lab, M, src, tgt, step, n_samples, n_classes, width, jac_planes, packed,
+                                    layout)
 
 
 @fused_raymarch_jac.register_fake
@@ -332,8 +350,7 @@ def _(vol, lab, M, src, tgt, step, n_samples, n_classes, width, jac_planes, pack
             torch.empty(B, jac_planes, n_classes, N, device=vol.device, dtype=torch.float32))
 
 
-@torch.library.custom_op("nanodrr_amd::fused_raymarch_bwd_jac", mutates_args=())
-def fused_raymarch_bwd_jac(
+def _fused_raymarch_bwd_jac_impl(
     go: torch.Tensor,
     jac: torch.Tensor,
     M: torch.Tensor,
@@ -343,8 +360,6 @@ def fused_raymarch_bwd_jac(
     width: int,
     need_gcam: bool,
 ) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
-    """Backward of `fused_raymarch_jac` from its jac output (ndrr_raymarch_bwd_jac):
-    (gM (B,3,4), gsrc (B,N,3)|empty, gtgt (B,N,3)|empty, gstep (B,N)|empty), fp32."""
     B, planes, C, N = jac.shape
     if M.shape != (B, 3, 4) or tgt.shape[1] != N:
         raise ValueError("jac does not match the geometry")
@@ -375,6 +390,22 @@ def fused_raymarch_bwd_jac(
         )
     _lib.check(rc, "ndrr_raymarch_bwd_jac")
     return gM, gsrc, gtgt, gstep
+
+
+@torch.library.custom_op("nanodrr_amd::fused_raymarch_bwd_jac", mutates_args=())
+def fused_raymarch_bwd_jac(
+    go: torch.Tensor,
+    jac: torch.Tensor,
+    M: torch.Tensor,
+    src: torch.Tensor,
+    tgt: torch.Tensor,
+    step: torch.Tensor,
+    width: int,
+    need_gcam: bool,
+) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Backward of `fused_raymarch_jac` from its jac output (ndrr_raymarch_bwd_jac):
+    (gM (B,3,4), gsrc (B,N,3)|empty, gtgt (B,N,3)|empty, gstep (B,N)|empty), fp32."""
+    return _fused_raymarch_bwd_jac_impl(go, jac, M, src, tgt, step, width, need_gcam)
 
 
 @fused_raymarch_bwd_jac.register_fake

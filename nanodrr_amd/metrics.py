@@ -19,8 +19,7 @@ from . import _lib
 from .geometry import rotmat_geodesic_distance
 
 
-@torch.library.custom_op("nanodrr_amd::zncc", mutates_args=())
-def _zncc(x: torch.Tensor, y: torch.Tensor, eps: float) -> tuple[torch.Tensor, torch.Tensor]:
+def _zncc_impl(x: torch.Tensor, y: torch.Tensor, eps: float) -> tuple[torch.Tensor, torch.Tensor]:
     B = x.shape[0]
     x2 = x.detach().reshape(B, -1).to(torch.float32).contiguous()
     y2 = y.detach().reshape(B, -1).to(torch.float32).contiguous()
@@ -36,6 +35,11 @@ def _zncc(x: torch.Tensor, y: torch.Tensor, eps: float) -> tuple[torch.Tensor, t
                                   score.data_ptr(), grad.data_ptr(), torch.cuda.current_stream(x.device).cuda_stream)
     _lib.check(rc, "ndrr_zncc")
     return score, grad.view(x.shape)
+
+
+@torch.library.custom_op("nanodrr_amd::zncc", mutates_args=())
+def _zncc(x: torch.Tensor, y: torch.Tensor, eps: float) -> tuple[torch.Tensor, torch.Tensor]:
+    return _zncc_impl(x, y, eps)
 
 
 @_zncc.register_fake

@@ -11,7 +11,10 @@ the renderer its camera -> voxel affine directly.
 
 `register` is the batched optimisation loop: ZNCC loss (native kernel) and
 Adam with per-group learning rates on a cosine schedule, every problem of the
-batch advancing in the same launches.
+batch advancing in the same launches. It chains the kernels' adjoints itself
+(no autograd graph, no custom-op dispatch), which keeps the host's share of a
+small batch's iteration low; `adjoint="autograd"` runs the same iteration
+through autograd and gives the same parameters bit for bit.
 """
 
 from __future__ import annotations
@@ -22,17 +25,17 @@ import torch
 
 from . import _lib
 from .geometry import convert, transform_point
-from .metrics import zncc
-from .renderer import _make_src, _make_tgt, march
+from ._ops import _fused_raymarch_bwd_jac_impl, _fused_raymarch_jac_impl
+from .metrics import _zncc_impl, zncc
+from .renderer import _make_src, _make_tgt, march, march_inputs
 
 
 def _stream(device):
     return torch.cuda.current_stream(device).cuda_stream
 
 
-@torch.library.custom_op("nanodrr_amd::pose_compose", mutates_args=())
-def pose_compose(prefix: torch.Tensor, suffix: torch.Tensor, rot: torch.Tensor, xyz: torch.Tensor) -> torch.Tensor:
-    """M (B, 3, 4) fp32 = (prefix @ [exp([rot]x), exp([rot]x) xyz; 0 1] @ suffix)[:3] in FP64."""
+def _pose_compose_impl(prefix: torch.Tensor, suffix: torch.Tensor, rot: torch.Tensor,
+                       xyz: torch.Tensor) -> torch.Tensor:
     B = prefix.shape[0]
     for name, t, dt in (("prefix", prefix, torch.float64), ("suffix", suffix, torch.float64),
                         ("rot", rot, torch.float32), ("xyz", xyz, torch.float32)):
@@ -48,14 +51,19 @@ def pose_compose(prefix: torch.Tensor, suffix: torch.Tensor, rot: torch.Tensor, 
     return M
 
 
+@torch.library.custom_op("nanodrr_amd::pose_compose", mutates_args=())
+def pose_compose(prefix: torch.Tensor, suffix: torch.Tensor, rot: torch.Tensor, xyz: torch.Tensor) -> torch.Tensor:
+    """M (B, 3, 4) fp32 = (prefix @ [exp([rot]x), exp([rot]x) xyz; 0 1] @ suffix)[:3] in FP64."""
+    return _pose_compose_impl(prefix, suffix, rot, xyz)
+
+
 @pose_compose.register_fake
 def _(prefix, suffix, rot, xyz):
     return torch.empty(prefix.shape[0], 3, 4, device=prefix.device, dtype=torch.float32)
 
 
-@torch.library.custom_op("nanodrr_amd::pose_compose_bwd", mutates_args=())
-def pose_compose_bwd(prefix: torch.Tensor, suffix: torch.Tensor, rot: torch.Tensor, xyz: torch.Tensor,
-                     gM: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+def _pose_compose_bwd_impl(prefix: torch.Tensor, suffix: torch.Tensor, rot: torch.Tensor, xyz: torch.Tensor,
+                           gM: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
     B = prefix.shape[0]
     gM = gM.to(torch.float32).contiguous()
     grot = torch.empty(B, 3, device=prefix.device, dtype=torch.float32)
@@ -66,6 +74,12 @@ def pose_compose_bwd(prefix: torch.Tensor, suffix: torch.Tensor, rot: torch.Tens
                                               _stream(prefix.device))
     _lib.check(rc, "ndrr_pose_compose_bwd")
     return grot, gxyz
+
+
+@torch.library.custom_op("nanodrr_amd::pose_compose_bwd", mutates_args=())
+def pose_compose_bwd(prefix: torch.Tensor, suffix: torch.Tensor, rot: torch.Tensor, xyz: torch.Tensor,
+                     gM: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    return _pose_compose_bwd_impl(prefix, suffix, rot, xyz, gM)
 
 
 @pose_compose_bwd.register_fake
@@ -223,9 +237,32 @@ def _adam_step(p: torch.Tensor, g: torch.Tensor, m: torch.Tensor, v: torch.Tenso
     _lib.check(rc, "ndrr_adam_step")
 
 
+def _explicit_step(reg: Registration, target: torch.Tensor, n_samples: int, offset, eps: float):
+    """One iteration's score (B,) and gradients of -sum(score) for reg._rot and reg._xyz,
+    the kernels and adjoints that autograd would run, called in the same order."""
+    rot, xyz = reg._params()
+    M = _pose_compose_impl(reg._prefix, reg._suffix, rot, xyz)
+    src, tgt = reg._src, reg._tgt
+    step, delta = reg._ray_steps(n_samples)
+    if offset is not None:
+        u = offset.reshape(-1, 1, 1).to(tgt.dtype)
+        src, tgt = torch.addcmul(src, delta, u), torch.addcmul(tgt, delta, u)
+    args, packed, layout, B, C = march_inputs(reg.subject, M, src, tgt, step, n_samples, reg.height, reg.width)
+    img, jac = _fused_raymarch_jac_impl(*args, 6, packed, layout)
+    score, gimg = _zncc_impl(img, target, eps)
+    # d(-sum score)/d img = -gimg; the contraction and the pose adjoint are linear and
+    # sign-symmetric, so negating gM gives the bits that -gimg would
+    gM = _fused_raymarch_bwd_jac_impl(gimg, jac, args[2], args[3], args[4], args[5], reg.width, False)[0]
+    gM.neg_()
+    grot, gxyz = _pose_compose_bwd_impl(reg._prefix, reg._suffix, rot, xyz, gM)
+    if reg._rot.shape != grot.shape:  # parameters shared by the batch: the expand's adjoint
+        grot, gxyz = grot.sum_to_size(reg._rot.shape), gxyz.sum_to_size(reg._xyz.shape)
+    return score, grot, gxyz
+
+
 def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot: float = 1e-2, lr_xyz: float = 1.0,
              n_samples: int = 256, jitter: bool = False, seed: int = 0, graph: bool = False,
-             iters_per_graph: int = 10, eps: float = 1e-7) -> torch.Tensor:
+             iters_per_graph: int = 10, eps: float = 1e-7, adjoint: str = "explicit") -> torch.Tensor:
     """Optimise every pose of `reg` to maximise the ZNCC of its DRRs with `target` (B, 1, H, W).
 
     Each iteration: pose step, render, ZNCC, backward, Adam update (learning
@@ -239,11 +276,17 @@ def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot
     problem's sample lattice by a fresh uniform offset in [-1/2, 1/2) step every
     iteration (seeded by `seed`), which averages that ripple out.
 
+    `adjoint="explicit"` (default) calls the kernels and their adjoints
+    directly; `adjoint="autograd"` builds the same iteration with autograd
+    through the custom ops (the path any other loss would take).
+
     `graph=True` (capturing `iters_per_graph` iterations in one HIP graph) is
     switched off: capturing ten iterations ended the process with a
     segmentation fault inside the runtime's capture end, with no cause found
     in this library, so the loop runs eagerly.
     """
+    if adjoint not in ("explicit", "autograd"):
+        raise ValueError(f"adjoint must be 'explicit' or 'autograd', got {adjoint!r}")
     if graph:
         raise NotImplementedError(
             "HIP-graph capture of the registration step is switched off (a ten-iteration capture crashed in "
@@ -256,18 +299,22 @@ def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot
     b1, b2, adam_eps, final_frac = 0.9, 0.999, 1e-8, 0.02
     target = target.detach().to(torch.float32).contiguous()
     B = target.shape[0]
-    score = torch.zeros(B, device=target.device)
+    score = torch.zeros(B, device=target.device)  # n_iters == 0
     gen = torch.Generator(device=target.device).manual_seed(int(seed)) if jitter else None
     descent = torch.full((B,), -1.0, device=target.device)  # d(-sum of scores) / d score
     for it in range(n_iters):
         for p, _ in params:
             p.grad = None
         offset = torch.rand(B, device=target.device, generator=gen) - 0.5 if jitter else None
-        s = zncc(reg(n_samples, offset=offset), target, eps)
-        s.backward(descent)
+        if adjoint == "explicit":
+            with torch.no_grad():
+                s, reg._rot.grad, reg._xyz.grad = _explicit_step(reg, target, n_samples, offset, eps)
+        else:
+            s = zncc(reg(n_samples, offset=offset), target, eps)
+            s.backward(descent)
         frac = final_frac + (1 - final_frac) * 0.5 * (1 + math.cos(math.pi * it / max(n_iters - 1, 1)))
         with torch.no_grad():
             for (p, lr), (m, v) in zip(params, moments):
                 _adam_step(p, p.grad, m, v, lr * frac, b1, b2, adam_eps, it + 1)
-        score.copy_(s.detach())
+        score = s.detach()
     return score.clone()

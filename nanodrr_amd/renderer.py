@@ -143,9 +143,9 @@ def volume_layout(image: torch.Tensor) -> str:
     return layout
 
 
-def march(subject, M, src, tgt, step_size, n_samples, height, width) -> torch.Tensor:
-    """(B, C, H, W) line integrals of the rays src -> tgt (camera space) with the
-    camera -> voxel-index affines M (B|1, 3, 4) through the subject's volume."""
+def march_inputs(subject, M, src, tgt, step_size, n_samples, height, width):
+    """Checked kernel arguments of `march`: ((vol, lab, M, src, tgt, step, n_samples, C, width),
+    packed volume or None, layout name, B, C)."""
     image = subject.image
     if not image.is_cuda:
         raise ValueError("the hip backend renders on a GPU: move the subject and poses to a cuda device")
@@ -164,12 +164,19 @@ def march(subject, M, src, tgt, step_size, n_samples, height, width) -> torch.Te
     layout = volume_layout(image)
     packed = None if layout == "linear" else subject.packed_volume(layout)
     lab = subject.packed_labels(layout) if C > 1 else vol.new_empty(0, dtype=torch.int8)
-    if ADJOINT not in _ADJOINTS:
-        raise ValueError(f"ADJOINT must be one of {_ADJOINTS}, got {ADJOINT!r}")
     args = (vol, lab, M.contiguous(), src.to(torch.float32).contiguous(), tgt.to(torch.float32).contiguous(),
             step_size.to(torch.float32).contiguous(), n_samples, C, width)
+    return args, packed, layout, B, C
+
+
+def march(subject, M, src, tgt, step_size, n_samples, height, width) -> torch.Tensor:
+    """(B, C, H, W) line integrals of the rays src -> tgt (camera space) with the
+    camera -> voxel-index affines M (B|1, 3, 4) through the subject's volume."""
+    args, packed, layout, B, C = march_inputs(subject, M, src, tgt, step_size, n_samples, height, width)
+    if ADJOINT not in _ADJOINTS:
+        raise ValueError(f"ADJOINT must be one of {_ADJOINTS}, got {ADJOINT!r}")
     gcam = any(t.requires_grad for t in args[3:6])
-    single = (ADJOINT == "single_pass" and torch.is_grad_enabled() and not image.requires_grad
+    single = (ADJOINT == "single_pass" and torch.is_grad_enabled() and not subject.image.requires_grad
               and (gcam or args[2].requires_grad))
     if single:
         out, _ = fused_raymarch_jac(*args, 7 if gcam else 6, packed, layout)
