@@ -22,6 +22,7 @@ gradients without marching the volume again.
 
 from __future__ import annotations
 
+from types import SimpleNamespace
 from typing import Optional
 
 import torch
@@ -153,6 +154,33 @@ def _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width, layout="
     return B, N, D, H, W, strides
 
 
+def _fused_raymarch_impl(
+    vol: torch.Tensor,
+    lab: torch.Tensor,
+    M: torch.Tensor,
+    src: torch.Tensor,
+    tgt: torch.Tensor,
+    step: torch.Tensor,
+    n_samples: int,
+    n_classes: int,
+    width: int,
+    packed: Optional[torch.Tensor] = None,
+    layout: str = "linear",
+) -> torch.Tensor:
+    B, N, D, H, W, strides = _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width,
+                                       layout if packed is not None else "linear")
+    vptr, code = _volume_arg(vol, packed, layout)
+    out = torch.empty(B, n_classes, N, device=vol.device, dtype=torch.float32)
+    with torch.cuda.device(vol.device):
+        rc = _lib.lib().ndrr_raymarch_fwd(
+            vptr, _DTYPE_CODE[vol.dtype], code, _ptr(lab) if n_classes > 1 else None,
+            M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(), out.data_ptr(),
+            B, N, D, H, W, n_samples, n_classes, width, *strides, _stream(vol.device),
+        )
+    _lib.check(rc, "ndrr_raymarch_fwd")
+    return out
+
+
 @torch.library.custom_op("nanodrr_amd::fused_raymarch", mutates_args=())
 def fused_raymarch(
     vol: torch.Tensor,
@@ -168,18 +196,7 @@ def fused_raymarch(
     layout: str = "linear",
 ) -> torch.Tensor:
     """Line integrals (B, C, N) fp32 of the rays through `vol` (see ndrr_raymarch_fwd)."""
-    B, N, D, H, W, strides = _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width,
-                                       layout if packed is not None else "linear")
-    vptr, code = _volume_arg(vol, packed, layout)
-    out = torch.empty(B, n_classes, N, device=vol.device, dtype=torch.float32)
-    with torch.cuda.device(vol.device):
-        rc = _lib.lib().ndrr_raymarch_fwd(
-            vptr, _DTYPE_CODE[vol.dtype], code, _ptr(lab) if n_classes > 1 else None,
-            M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(), out.data_ptr(),
-            B, N, D, H, W, n_samples, n_classes, width, *strides, _stream(vol.device),
-        )
-    _lib.check(rc, "ndrr_raymarch_fwd")
-    return out
+    return _fused_raymarch_impl(vol, lab, M, src, tgt, step, n_samples, n_classes, width, packed, layout)
 
 
 @fused_raymarch.register_fake
@@ -187,8 +204,7 @@ def _(vol, lab, M, src, tgt, step, n_samples, n_classes, width, packed=None, lay
     return torch.empty(M.shape[0], n_classes, tgt.shape[1], device=vol.device, dtype=torch.float32)
 
 
-@torch.library.custom_op("nanodrr_amd::fused_raymarch_bwd", mutates_args=())
-def fused_raymarch_bwd(
+def _fused_raymarch_bwd_impl(
     go: torch.Tensor,
     vol: torch.Tensor,
     lab: torch.Tensor,
@@ -204,8 +220,6 @@ def fused_raymarch_bwd(
     packed: Optional[torch.Tensor] = None,
     layout: str = "linear",
 ) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
-    """Analytic adjoint (see ndrr_raymarch_bwd). Returns (gvol|empty, gM (B,3,4),
-    gsrc (B,N,3)|empty, gtgt (B,N,3)|empty, gstep (B,N)|empty), all fp32."""
     if need_gvol and torch.are_deterministic_algorithms_enabled():
         raise RuntimeError(
             "Volume gradients on the hip backend use nondeterministic atomics; "
@@ -237,6 +251,28 @@ def fused_raymarch_bwd(
     return gvol, gM, gsrc, gtgt, gstep
 
 
+@torch.library.custom_op("nanodrr_amd::fused_raymarch_bwd", mutates_args=())
+def fused_raymarch_bwd(
+    go: torch.Tensor,
+    vol: torch.Tensor,
+    lab: torch.Tensor,
+    M: torch.Tensor,
+    src: torch.Tensor,
+    tgt: torch.Tensor,
+    step: torch.Tensor,
+    n_samples: int,
+    n_classes: int,
+    width: int,
+    need_gvol: bool,
+    need_gcam: bool,
+    packed: Optional[torch.Tensor] = None,
+    layout: str = "linear",
+) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Analytic adjoint (see ndrr_raymarch_bwd). Returns (gvol|empty, gM (B,3,4),
+    gsrc (B,N,3)|empty, gtgt (B,N,3)|empty, gstep (B,N)|empty), all fp32."""
+    return _fused_raymarch_bwd_impl(go, vol, lab, M, src, tgt, step, n_samples, n_classes, width, need_gvol, need_gcam, packed, layout)
+
+
 @fused_raymarch_bwd.register_fake
 def _(go, vol, lab, M, src, tgt, step, n_samples, n_classes, width, need_gvol, need_gcam, packed=None,
       layout="linear"):
@@ -263,11 +299,13 @@ def _reduce_to(g: torch.Tensor, like: torch.Tensor) -> torch.Tensor:
     return g.to(like.dtype)
 
 
-def _backward(ctx, go):
+def _backward(ctx, go, bwd=None):
+    """Autograd formula of fused_raymarch; `bwd` is the adjoint to call (the custom op unless the
+    eager path passes its implementation)."""
     vol, lab, M, src, tgt, step, packed = ctx.saved_tensors
     need_gvol = ctx.needs_input_grad[0]
     need_gcam = any(ctx.needs_input_grad[i] for i in (3, 4, 5))
-    gvol, gM, gsrc, gtgt, gstep = fused_raymarch_bwd(
+    gvol, gM, gsrc, gtgt, gstep = (bwd or fused_raymarch_bwd)(
         go, vol, lab, M, src, tgt, step, ctx.n_samples, ctx.n_classes, ctx.width, need_gvol, need_gcam, packed,
         ctx.layout,
     )
@@ -426,14 +464,15 @@ def _setup_context_jac(ctx, inputs, output):
     ctx.width = width
 
 
-def _backward_jac(ctx, go, g_jac):
+def _backward_jac(ctx, go, g_jac, bwd=None):
+    """Autograd formula of fused_raymarch_jac; `bwd` as in _backward."""
     if go is None:
         return (None,) * 12
     M, src, tgt, step, jac = ctx.saved_tensors
     if ctx.needs_input_grad[0]:
         raise RuntimeError("fused_raymarch_jac gives no volume gradient; use fused_raymarch for it")
     need_gcam = any(ctx.needs_input_grad[i] for i in (3, 4, 5))
-    gM, gsrc, gtgt, gstep = fused_raymarch_bwd_jac(go, jac, M, src, tgt, step, ctx.width, need_gcam)
+    gM, gsrc, gtgt, gstep = (bwd or fused_raymarch_bwd_jac)(go, jac, M, src, tgt, step, ctx.width, need_gcam)
     return (
         None,  # vol
         None,  # lab
@@ -448,11 +487,8 @@ def _backward_jac(ctx, go, g_jac):
 torch.library.register_autograd("nanodrr_amd::fused_raymarch_jac", _backward_jac, setup_context=_setup_context_jac)
 
 
-@torch.library.custom_op("nanodrr_amd::make_rays", mutates_args=())
-def make_rays(k_inv: torch.Tensor, sdd: torch.Tensor, height: int, width: int, orthographic: bool,
-              n_samples: int) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
-    """(tgt (B,N,3), src (B,1,3) or (B,N,3), step (B,N)) fp32 of `render`'s rays in one
-    launch (ndrr_make_rays); B = max of the batches of k_inv (B|1,3,3) and sdd (B|1,)."""
+def _make_rays_impl(k_inv: torch.Tensor, sdd: torch.Tensor, height: int, width: int, orthographic: bool,
+                    n_samples: int) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
     if k_inv.dtype != torch.float32 or sdd.dtype != torch.float32 or not k_inv.is_cuda or sdd.device != k_inv.device:
         raise ValueError("make_rays needs float32 device tensors k_inv and sdd")
     k = k_inv.contiguous()
@@ -475,6 +511,14 @@ def make_rays(k_inv: torch.Tensor, sdd: torch.Tensor, height: int, width: int, o
     return tgt, src, step
 
 
+@torch.library.custom_op("nanodrr_amd::make_rays", mutates_args=())
+def make_rays(k_inv: torch.Tensor, sdd: torch.Tensor, height: int, width: int, orthographic: bool,
+              n_samples: int) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """(tgt (B,N,3), src (B,1,3) or (B,N,3), step (B,N)) fp32 of `render`'s rays in one
+    launch (ndrr_make_rays); B = max of the batches of k_inv (B|1,3,3) and sdd (B|1,)."""
+    return _make_rays_impl(k_inv, sdd, height, width, orthographic, n_samples)
+
+
 @make_rays.register_fake
 def _(k_inv, sdd, height, width, orthographic, n_samples):
     B, N = max(k_inv.shape[0], sdd.shape[0]), height * width
@@ -490,9 +534,7 @@ def _c2v_check(world_to_voxel, t, name):
         raise ValueError(f"{name} must be a contiguous float32 device tensor on the device of world_to_voxel")
 
 
-@torch.library.custom_op("nanodrr_amd::camera_to_voxel", mutates_args=())
-def camera_to_voxel(world_to_voxel: torch.Tensor, rt_inv: torch.Tensor) -> torch.Tensor:
-    """M = (world_to_voxel @ rt_inv)[:, :3] (B,3,4) fp32, in FP64 with one rounding (ndrr_camera_to_voxel)."""
+def _camera_to_voxel_impl(world_to_voxel: torch.Tensor, rt_inv: torch.Tensor) -> torch.Tensor:
     world_to_voxel = world_to_voxel.contiguous()
     _c2v_check(world_to_voxel, rt_inv, "rt_inv")
     if rt_inv.dim() != 3 or rt_inv.shape[1:] != (4, 4):
@@ -506,14 +548,18 @@ def camera_to_voxel(world_to_voxel: torch.Tensor, rt_inv: torch.Tensor) -> torch
     return M
 
 
+@torch.library.custom_op("nanodrr_amd::camera_to_voxel", mutates_args=())
+def camera_to_voxel(world_to_voxel: torch.Tensor, rt_inv: torch.Tensor) -> torch.Tensor:
+    """M = (world_to_voxel @ rt_inv)[:, :3] (B,3,4) fp32, in FP64 with one rounding (ndrr_camera_to_voxel)."""
+    return _camera_to_voxel_impl(world_to_voxel, rt_inv)
+
+
 @camera_to_voxel.register_fake
 def _(world_to_voxel, rt_inv):
     return torch.empty(rt_inv.shape[0], 3, 4, device=rt_inv.device, dtype=torch.float32)
 
 
-@torch.library.custom_op("nanodrr_amd::camera_to_voxel_bwd", mutates_args=())
-def camera_to_voxel_bwd(world_to_voxel: torch.Tensor, gM: torch.Tensor) -> torch.Tensor:
-    """Adjoint of camera_to_voxel: world_to_voxel[:3]^T @ gM (B,4,4) fp32, FP64 sums."""
+def _camera_to_voxel_bwd_impl(world_to_voxel: torch.Tensor, gM: torch.Tensor) -> torch.Tensor:
     gM = gM.to(torch.float32).contiguous()
     world_to_voxel = world_to_voxel.contiguous()
     _c2v_check(world_to_voxel, gM, "gM")
@@ -526,6 +572,12 @@ def camera_to_voxel_bwd(world_to_voxel: torch.Tensor, gM: torch.Tensor) -> torch
     return g
 
 
+@torch.library.custom_op("nanodrr_amd::camera_to_voxel_bwd", mutates_args=())
+def camera_to_voxel_bwd(world_to_voxel: torch.Tensor, gM: torch.Tensor) -> torch.Tensor:
+    """Adjoint of camera_to_voxel: world_to_voxel[:3]^T @ gM (B,4,4) fp32, FP64 sums."""
+    return _camera_to_voxel_bwd_impl(world_to_voxel, gM)
+
+
 @camera_to_voxel_bwd.register_fake
 def _(world_to_voxel, gM):
     return torch.empty(gM.shape[0], 4, 4, device=gM.device, dtype=torch.float32)
@@ -535,9 +587,9 @@ def _c2v_setup(ctx, inputs, output):
     ctx.save_for_backward(inputs[0])
 
 
-def _c2v_backward(ctx, gM):
+def _c2v_backward(ctx, gM, bwd=None):
     (w2v,) = ctx.saved_tensors
-    return None, camera_to_voxel_bwd(w2v, gM)
+    return None, (bwd or camera_to_voxel_bwd)(w2v, gM)
 
 
 torch.library.register_autograd("nanodrr_amd::camera_to_voxel", _c2v_backward, setup_context=_c2v_setup)
@@ -560,3 +612,46 @@ def labels_to_i8(label: torch.Tensor, n_classes: int) -> torch.Tensor:
 @labels_to_i8.register_fake
 def _(label, n_classes):
     return torch.empty(label.shape, device=label.device, dtype=torch.int8)
+
+
+# ---------------------------------------------------------------------------
+# Eager entry points. A torch.library custom op costs tens of microseconds of
+# host dispatch per call (more than most of the kernels it launches), so eager
+# callers reach the implementations directly: through an autograd.Function
+# that carries the op's own formula when an input needs a gradient, else as a
+# plain call. While torch.compile traces, the registered custom ops (with
+# their fake implementations) are used instead.
+# ---------------------------------------------------------------------------
+def _eager(name, op, impl, setup=None, backward=None):
+    fn = None
+    if setup is not None:
+        def forward(ctx, *args):
+            out = impl(*args)
+            setup(ctx, args, out)
+            return out
+
+        fn = type(name, (torch.autograd.Function,), {"forward": staticmethod(forward),
+                                                      "backward": staticmethod(backward)})
+
+    def call(*args):
+        if torch.compiler.is_compiling():
+            return op(*args)
+        if fn is not None and torch.is_grad_enabled() and any(
+                isinstance(a, torch.Tensor) and a.requires_grad for a in args):
+            return fn.apply(*args)
+        return impl(*args)
+
+    call.__name__ = call.__qualname__ = name
+    call.__doc__ = f"Eager entry point of the custom op {name} (same arguments and result)."
+    return call
+
+
+eager = SimpleNamespace(
+    fused_raymarch=_eager("fused_raymarch", fused_raymarch, _fused_raymarch_impl, _setup_context,
+                          lambda ctx, go: _backward(ctx, go, _fused_raymarch_bwd_impl)),
+    fused_raymarch_jac=_eager("fused_raymarch_jac", fused_raymarch_jac, _fused_raymarch_jac_impl, _setup_context_jac,
+                              lambda ctx, go, g_jac: _backward_jac(ctx, go, g_jac, _fused_raymarch_bwd_jac_impl)),
+    make_rays=_eager("make_rays", make_rays, _make_rays_impl),
+    camera_to_voxel=_eager("camera_to_voxel", camera_to_voxel, _camera_to_voxel_impl, _c2v_setup,
+                           lambda ctx, gM: _c2v_backward(ctx, gM, _camera_to_voxel_bwd_impl)),
+)

@@ -16,6 +16,7 @@ import torch
 import torch.nn.functional as F
 
 from . import _lib
+from ._ops import _eager
 from .geometry import rotmat_geodesic_distance
 
 
@@ -64,10 +65,13 @@ def _zncc_backward(ctx, g_score, g_grad):
 
 torch.library.register_autograd("nanodrr_amd::zncc", _zncc_backward, setup_context=_zncc_setup)
 
+# eager callers skip the custom-op dispatch (see nanodrr_amd._ops._eager)
+_zncc_eager = _eager("zncc", _zncc, _zncc_impl, _zncc_setup, _zncc_backward)
+
 
 def zncc(x: torch.Tensor, y: torch.Tensor, eps: float = 1e-7) -> torch.Tensor:
     """(B,) zero-normalised cross-correlation of x and y over all non-batch dims."""
-    return _zncc(x, y, eps)[0]
+    return _zncc_eager(x, y, eps)[0]
 
 
 class NormalizedCrossCorrelation2d(torch.nn.Module):

@@ -177,7 +177,7 @@ class DrrSource:
     def render(self, k: int, rot: torch.Tensor, xyz: torch.Tensor, offset: torch.Tensor | None = None,
                n_samples: int | None = None) -> torch.Tensor:
         """(B, 1, det, det) DRRs of phantom k at parameters (rot, xyz) (B, 3)."""
-        from .registration import pose_compose
+        from .registration import _pose_compose_eager as pose_compose
 
         reg = self.regs[k]
         B = rot.shape[0]

@@ -25,7 +25,7 @@ import torch
 
 from . import _lib
 from .geometry import convert, transform_point
-from ._ops import _fused_raymarch_bwd_jac_impl, _fused_raymarch_jac_impl
+from ._ops import _eager, _fused_raymarch_bwd_jac_impl, _fused_raymarch_jac_impl
 from .metrics import _zncc_impl, zncc
 from .renderer import _make_src, _make_tgt, march, march_inputs
 
@@ -102,6 +102,11 @@ def _pc_backward(ctx, gM):
 
 torch.library.register_autograd("nanodrr_amd::pose_compose", _pc_backward, setup_context=_pc_setup)
 
+# eager callers skip the custom-op dispatch (see nanodrr_amd._ops._eager)
+_pose_compose_eager = _eager(
+    "pose_compose", pose_compose, _pose_compose_impl, _pc_setup,
+    lambda ctx, gM: (None, None, *_pose_compose_bwd_impl(*ctx.saved_tensors, gM)))
+
 
 def _translation(t: torch.Tensor) -> torch.Tensor:
     T = torch.eye(4, device=t.device, dtype=t.dtype).repeat(len(t), 1, 1)
@@ -153,7 +158,7 @@ class Registration(torch.nn.Module):
     def camera_to_voxel(self) -> torch.Tensor:
         """(B, 3, 4) fp32 camera -> voxel affine of the current pose (FP64 pose step)."""
         rot, xyz = self._params()
-        return pose_compose(self._prefix, self._suffix, rot, xyz)
+        return _pose_compose_eager(self._prefix, self._suffix, rot, xyz)
 
     def forward(self, n_samples: int = 500, backend: str = "auto", offset: torch.Tensor | None = None) -> torch.Tensor:
         """DRRs of the current poses (registration.py:59-62). `offset` (B,), in units of the

@@ -13,7 +13,7 @@ from __future__ import annotations
 import torch
 
 from . import _lib
-from ._ops import camera_to_voxel, fused_raymarch, fused_raymarch_jac, make_rays
+from ._ops import eager
 
 _FUSED_DTYPES = (torch.float32, torch.float16, torch.bfloat16)
 _BACKENDS = ("auto", "hip", "triton")
@@ -100,7 +100,7 @@ def render(
         raise ValueError(f"Unknown backend {backend!r}; expected 'auto', 'hip' or 'triton'")
     device, dtype = rt_inv.device, rt_inv.dtype
     if tgt is None and src is None and _native_rays(k_inv, sdd, dtype):
-        tgt, src, step_size = make_rays(k_inv, sdd, height, width, orthographic, n_samples)
+        tgt, src, step_size = eager.make_rays(k_inv, sdd, height, width, orthographic, n_samples)
     else:
         if tgt is None:
             tgt = _make_tgt(k_inv, sdd, height, width, device, dtype)
@@ -124,7 +124,7 @@ def render_hip(subject, rt_inv, src, tgt, step_size, n_samples, height, width) -
         raise ValueError("the hip backend renders on a GPU: move the subject and poses to a cuda device")
     # Camera -> voxel-index affine, composed in float64 and rounded once.
     if rt_inv.dtype == torch.float32 and rt_inv.dim() == 3 and rt_inv.shape[1:] == (4, 4):
-        M = camera_to_voxel(subject.world_to_voxel_f64(), rt_inv.contiguous())
+        M = eager.camera_to_voxel(subject.world_to_voxel_f64(), rt_inv.contiguous())
     else:
         M = (subject.world_to_voxel.to(torch.float64) @ rt_inv.to(torch.float64))[:, :3, :].to(torch.float32)
     return march(subject, M, src, tgt, step_size, n_samples, height, width)
@@ -179,7 +179,7 @@ def march(subject, M, src, tgt, step_size, n_samples, height, width) -> torch.Te
     single = (ADJOINT == "single_pass" and torch.is_grad_enabled() and not subject.image.requires_grad
               and (gcam or args[2].requires_grad))
     if single:
-        out, _ = fused_raymarch_jac(*args, 7 if gcam else 6, packed, layout)
+        out, _ = eager.fused_raymarch_jac(*args, 7 if gcam else 6, packed, layout)
     else:
-        out = fused_raymarch(*args, packed, layout)
+        out = eager.fused_raymarch(*args, packed, layout)
     return out.reshape(B, C, height, width)
