@@ -242,27 +242,85 @@ def _adam_step(p: torch.Tensor, g: torch.Tensor, m: torch.Tensor, v: torch.Tenso
     _lib.check(rc, "ndrr_adam_step")
 
 
-def _explicit_step(reg: Registration, target: torch.Tensor, n_samples: int, offset, eps: float):
-    """One iteration's score (B,) and gradients of -sum(score) for reg._rot and reg._xyz,
-    the kernels and adjoints that autograd would run, called in the same order."""
-    rot, xyz = reg._params()
-    M = _pose_compose_impl(reg._prefix, reg._suffix, rot, xyz)
-    src, tgt = reg._src, reg._tgt
-    step, delta = reg._ray_steps(n_samples)
-    if offset is not None:
-        u = offset.reshape(-1, 1, 1).to(tgt.dtype)
-        src, tgt = torch.addcmul(src, delta, u), torch.addcmul(tgt, delta, u)
-    args, packed, layout, B, C = march_inputs(reg.subject, M, src, tgt, step, n_samples, reg.height, reg.width)
-    img, jac = _fused_raymarch_jac_impl(*args, 6, packed, layout)
-    score, gimg = _zncc_impl(img, target, eps)
-    # d(-sum score)/d img = -gimg; the contraction and the pose adjoint are linear and
-    # sign-symmetric, so negating gM gives the bits that -gimg would
-    gM = _fused_raymarch_bwd_jac_impl(gimg, jac, args[2], args[3], args[4], args[5], reg.width, False)[0]
-    gM.neg_()
-    grot, gxyz = _pose_compose_bwd_impl(reg._prefix, reg._suffix, rot, xyz, gM)
-    if reg._rot.shape != grot.shape:  # parameters shared by the batch: the expand's adjoint
-        grot, gxyz = grot.sum_to_size(reg._rot.shape), gxyz.sum_to_size(reg._xyz.shape)
-    return score, grot, gxyz
+class _ExplicitLoop:
+    """The explicit adjoint chain of one register() call. Inputs are checked and every buffer
+    is allocated once; each iteration then makes the launches that autograd would (pose step,
+    march with the single-pass sums, ZNCC, contraction, pose adjoint), in the same order and
+    with the same arguments, through the C ABI with cached pointers."""
+
+    def __init__(self, reg: Registration, target: torch.Tensor, n_samples: int, eps: float, jitter: bool):
+        from ._ops import _DTYPE_CODE, _geometry, _ptr, _volume_arg
+
+        self.reg, self.eps, self.S = reg, float(eps), n_samples
+        dev = target.device
+        self.lib = _lib.lib()
+        self.stream = _stream(dev)
+        B = len(reg.rt_inv)
+        self.B = B
+        self.shared = reg._rot.shape[0] != B
+        self.rot_b = torch.empty(B, 3, device=dev) if self.shared else reg._rot.detach()
+        self.xyz_b = torch.empty(B, 3, device=dev) if self.shared else reg._xyz.detach()
+        self.M = torch.empty(B, 3, 4, device=dev)
+        self.step, self.delta = reg._ray_steps(n_samples)
+        src, tgt = reg._src, reg._tgt
+        if jitter:  # the shifted rays are per pose and ray
+            src = torch.empty(B, tgt.shape[1], 3, device=dev)
+            tgt = torch.empty(B, tgt.shape[1], 3, device=dev)
+        self.src, self.tgt = src, tgt
+        args, packed, layout, _, C = march_inputs(reg.subject, self.M, src, tgt, self.step, n_samples, reg.height,
+                                                  reg.width)
+        vol, lab = args[0], args[1]
+        for a, b in ((args[2], self.M), (args[3], src), (args[4], tgt), (args[5], self.step)):
+            if a.data_ptr() != b.data_ptr():
+                raise ValueError("register() needs contiguous float32 rays and step")
+        _, N, D, H, W, self.strides = _geometry(*args, layout if packed is not None else "linear")
+        self.vptr, self.code = _volume_arg(vol, packed, layout)
+        self.dt, self.lab = _DTYPE_CODE[vol.dtype], (_ptr(lab) if C > 1 else None)
+        self.dims = (B, N, D, H, W, n_samples, C, reg.width)
+        self.target = target.reshape(B, -1)
+        if self.target.shape[1] != C * N:
+            raise ValueError("Input images must be the same size")
+        self.img = torch.empty(B, C, N, device=dev)
+        self.jac = torch.empty(B, 6, C, N, device=dev)
+        self.score = torch.empty(B, device=dev)
+        self.gimg = torch.empty(B, C, N, device=dev)
+        self.zws = torch.empty(self.lib.ndrr_zncc_workspace_floats(B, C * N), device=dev)
+        self.bws = torch.empty(self.lib.ndrr_bwd_workspace_floats(B, N, reg.width), device=dev)
+        self.gM = torch.empty(B, 3, 4, device=dev)
+        self.grot = torch.empty(B, 3, device=dev)
+        self.gxyz = torch.empty(B, 3, device=dev)
+
+    def step_(self, offset):
+        """One iteration: returns (score (B,), d(-sum score)/d rot, d(-sum score)/d xyz)."""
+        reg, lib, st = self.reg, self.lib, self.stream
+        B, N, D, H, W, S, C, width = self.dims
+        if offset is not None:
+            u = offset.reshape(-1, 1, 1).to(self.tgt.dtype)
+            torch.addcmul(reg._src, self.delta, u, out=self.src)
+            torch.addcmul(reg._tgt, self.delta, u, out=self.tgt)
+        if self.shared:  # the expanded parameters, as _params() hands them to the pose step
+            self.rot_b.copy_(reg._rot.detach().expand(B, 3))
+            self.xyz_b.copy_(reg._xyz.detach().expand(B, 3))
+        pre, suf = reg._prefix.data_ptr(), reg._suffix.data_ptr()
+        rot, xyz, M = self.rot_b.data_ptr(), self.xyz_b.data_ptr(), self.M.data_ptr()
+        src, tgt, step = self.src.data_ptr(), self.tgt.data_ptr(), self.step.data_ptr()
+        _lib.check(lib.ndrr_pose_compose(pre, suf, rot, xyz, 3, M, B, st), "ndrr_pose_compose")
+        _lib.check(lib.ndrr_raymarch_fwd_jac(self.vptr, self.dt, self.code, self.lab, M, src, tgt, step,
+                                             self.img.data_ptr(), self.jac.data_ptr(), 6, B, N, D, H, W, S, C,
+                                             width, *self.strides, st), "ndrr_raymarch_fwd_jac")
+        _lib.check(lib.ndrr_zncc(self.img.data_ptr(), self.target.data_ptr(), B, C * N, self.eps,
+                                 self.zws.data_ptr(), self.score.data_ptr(), self.gimg.data_ptr(), st), "ndrr_zncc")
+        # d(-sum score)/d img = -gimg; the contraction and the pose adjoint are linear and
+        # sign-symmetric, so negating gM gives the bits that -gimg would
+        _lib.check(lib.ndrr_raymarch_bwd_jac(self.gimg.data_ptr(), C * N, self.jac.data_ptr(), 6, M, src, tgt, step,
+                                             self.bws.data_ptr(), self.gM.data_ptr(), None, None, None, 0, B, N, C,
+                                             width, *self.strides, st), "ndrr_raymarch_bwd_jac")
+        self.gM.neg_()
+        _lib.check(lib.ndrr_pose_compose_bwd(pre, suf, rot, xyz, 3, self.gM.data_ptr(), self.grot.data_ptr(),
+                                             self.gxyz.data_ptr(), B, st), "ndrr_pose_compose_bwd")
+        if self.shared:  # the expand's adjoint
+            return self.score, self.grot.sum_to_size(reg._rot.shape), self.gxyz.sum_to_size(reg._xyz.shape)
+        return self.score, self.grot, self.gxyz
 
 
 def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot: float = 1e-2, lr_xyz: float = 1.0,
@@ -307,16 +365,30 @@ def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot
     score = torch.zeros(B, device=target.device)  # n_iters == 0
     gen = torch.Generator(device=target.device).manual_seed(int(seed)) if jitter else None
     descent = torch.full((B,), -1.0, device=target.device)  # d(-sum of scores) / d score
+    if adjoint == "explicit":
+        with torch.no_grad(), torch.cuda.device(target.device):
+            loop = _ExplicitLoop(reg, target, n_samples, eps, jitter)
+            lib, st = _lib.lib(), loop.stream
+            ptrs = [(p.data_ptr(), m.data_ptr(), v.data_ptr(), p.numel(), lr)
+                    for (p, lr), (m, v) in zip(params, moments)]
+            for it in range(n_iters):
+                offset = torch.rand(B, device=target.device, generator=gen) - 0.5 if jitter else None
+                s, grot, gxyz = loop.step_(offset)
+                frac = final_frac + (1 - final_frac) * 0.5 * (1 + math.cos(math.pi * it / max(n_iters - 1, 1)))
+                bc1, bc2 = 1.0 - b1 ** (it + 1), 1.0 - b2 ** (it + 1)
+                for (p, m, v, n, lr), g in zip(ptrs, (grot, gxyz)):
+                    _lib.check(lib.ndrr_adam_step(p, g.data_ptr(), m, v, n, lr * frac, b1, b2, adam_eps, bc1, bc2, st),
+                               "ndrr_adam_step")
+                score = s
+            if n_iters:
+                reg._rot.grad, reg._xyz.grad = grot.clone(), gxyz.clone()
+        return score.clone()
     for it in range(n_iters):
         for p, _ in params:
             p.grad = None
         offset = torch.rand(B, device=target.device, generator=gen) - 0.5 if jitter else None
-        if adjoint == "explicit":
-            with torch.no_grad():
-                s, reg._rot.grad, reg._xyz.grad = _explicit_step(reg, target, n_samples, offset, eps)
-        else:
-            s = zncc(reg(n_samples, offset=offset), target, eps)
-            s.backward(descent)
+        s = zncc(reg(n_samples, offset=offset), target, eps)
+        s.backward(descent)
         frac = final_frac + (1 - final_frac) * 0.5 * (1 + math.cos(math.pi * it / max(n_iters - 1, 1)))
         with torch.no_grad():
             for (p, lr), (m, v) in zip(params, moments):
