@@ -252,18 +252,34 @@ def predict(net: PoseNet, drr: torch.Tensor):
     return net.split(net(drr))
 
 
+def _settled(trace, j):
+    """First checked iteration from which problem j's mTRE stays below 1 mm to the end (None if it ends above)."""
+    settled = None
+    for k, e in trace:
+        if e[j] < 1.0:
+            if settled is None:
+                settled = k
+        else:
+            settled = None
+    return settled
+
+
 def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_deg: float = 30.0,
                          trans_mm: float = 40.0, n_iters: int = 300, device="cuda:0", log=print):
     """On n_test unseen phantoms, register one target each (200 x 200, targets at
     +-rot_deg / +-trans_mm about the nominal pose, S = 1024 target DRR) from the
     nominal pose and from the CNN's proposal (its input: the target pose rendered at
     128 x 128); both runs share one batch of two poses. Reports mTRE < 1 mm rates,
-    median mTRE and final ZNCC per start."""
+    median mTRE and final ZNCC per start, and the render iterations each start needs:
+    a problem has settled at the first check (every 5 iterations) from which its mTRE
+    stays below 1 mm to the end."""
     from .metrics import mtre, zncc
     from .phantom import detector, fiducials, make_phantom, nominal_pose, phantom_subject
     from .registration import Registration, register
 
-    res = {"nominal": [], "cnn": [], "proposal_mtre": [], "zncc": {"nominal": [], "cnn": []}}
+    res = {"nominal": [], "cnn": [], "proposal_mtre": [], "zncc": {"nominal": [], "cnn": []},
+           "settled": {"nominal": [], "cnn": []}}
+    check_every = 5
     k200, sdd = detector(1020.0, 1.0, 200, 200)
     k200, sdd = k200.to(device), sdd.to(device)
     for i in range(n_test):
@@ -288,8 +304,15 @@ def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_
         reg._rot.data.copy_(torch.cat([torch.zeros_like(rot_p), rot_p]))
         reg._xyz.data.copy_(torch.cat([torch.zeros_like(xyz_p), xyz_p]))
         res["proposal_mtre"].append(float(mtre(reg.pose.detach()[1:], rt_true, fid)[0]))
+        trace = []  # (iteration, mTRE of both starts) every check_every iterations
+
+        def monitor(k, r):
+            trace.append((k, mtre(r.pose.detach(), rt_true.expand(2, 4, 4), fid).tolist()))
+
         score = register(reg, target.expand(2, -1, -1, -1).contiguous(), n_iters=n_iters, n_samples=256,
-                         jitter=True, seed=i)
+                         jitter=True, seed=i, monitor=monitor, monitor_every=check_every)
+        for j, name in enumerate(("nominal", "cnn")):
+            res["settled"][name].append(_settled(trace, j))
         e = mtre(reg.pose.detach(), rt_true.expand(2, 4, 4), fid)
         res["nominal"].append(float(e[0]))
         res["cnn"].append(float(e[1]))
@@ -299,8 +322,25 @@ def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_
             log(f"{i + 1}/{n_test}: nominal<1mm {sum(x < 1 for x in res['nominal'])}, "
                 f"cnn<1mm {sum(x < 1 for x in res['cnn'])}")
     t = torch.tensor
+    curves = {name: [sum(1 for x in res["settled"][name] if x is not None and x <= k)
+                     for k in range(check_every, n_iters + 1, check_every)] for name in ("nominal", "cnn")}
+    plain_final = curves["nominal"][-1]
+    # render iterations after which the CNN start has as many settled problems as the plain start has at the end
+    cnn_match = next(((i + 1) * check_every for i, c in enumerate(curves["cnn"]) if c >= plain_final), None)
+
+    def med(xs):
+        xs = [x for x in xs if x is not None]
+        return float(t(xs, dtype=torch.float64).median()) if xs else None
+
     return {
         "problems": n_test,
+        "iterations": n_iters,
+        "check_every": check_every,
+        "settled_by_iteration": {"nominal": curves["nominal"], "cnn": curves["cnn"]},
+        "median_iterations_to_1mm_nominal": med(res["settled"]["nominal"]),
+        "median_iterations_to_1mm_cnn": med(res["settled"]["cnn"]),
+        "cnn_iterations_to_match_plain_final": cnn_match,
+        "cnn_over_plain_iteration_ratio": None if cnn_match is None else cnn_match / n_iters,
         "converged_from_nominal": int((t(res["nominal"]) < 1).sum()),
         "converged_from_cnn": int((t(res["cnn"]) < 1).sum()),
         "median_mtre_mm_nominal": float(t(res["nominal"]).median()),

@@ -325,7 +325,8 @@ class _ExplicitLoop:
 
 def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot: float = 1e-2, lr_xyz: float = 1.0,
              n_samples: int = 256, jitter: bool = False, seed: int = 0, graph: bool = False,
-             iters_per_graph: int = 10, eps: float = 1e-7, adjoint: str = "explicit") -> torch.Tensor:
+             iters_per_graph: int = 10, eps: float = 1e-7, adjoint: str = "explicit", monitor=None,
+             monitor_every: int = 10) -> torch.Tensor:
     """Optimise every pose of `reg` to maximise the ZNCC of its DRRs with `target` (B, 1, H, W).
 
     Each iteration: pose step, render, ZNCC, backward, Adam update (learning
@@ -342,6 +343,10 @@ def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot
     `adjoint="explicit"` (default) calls the kernels and their adjoints
     directly; `adjoint="autograd"` builds the same iteration with autograd
     through the custom ops (the path any other loss would take).
+
+    `monitor(iteration, reg)`, when given, is called after every
+    `monitor_every` iterations and after the last one, with the parameters
+    of that iteration applied (e.g. to record the mTRE trajectory).
 
     `graph=True` (capturing `iters_per_graph` iterations in one HIP graph) is
     switched off: capturing ten iterations ended the process with a
@@ -380,6 +385,8 @@ def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot
                     _lib.check(lib.ndrr_adam_step(p, g.data_ptr(), m, v, n, lr * frac, b1, b2, adam_eps, bc1, bc2, st),
                                "ndrr_adam_step")
                 score = s
+                if monitor is not None and ((it + 1) % monitor_every == 0 or it + 1 == n_iters):
+                    monitor(it + 1, reg)
             if n_iters:
                 reg._rot.grad, reg._xyz.grad = grot.clone(), gxyz.clone()
         return score.clone()
@@ -394,4 +401,6 @@ def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot
             for (p, lr), (m, v) in zip(params, moments):
                 _adam_step(p, p.grad, m, v, lr * frac, b1, b2, adam_eps, it + 1)
         score = s.detach()
+        if monitor is not None and ((it + 1) % monitor_every == 0 or it + 1 == n_iters):
+            monitor(it + 1, reg)
     return score.clone()

@@ -149,4 +149,5 @@ def test_posenet_training_and_registration_pipeline_runs():
     assert last < first, (first, last)
     r = compare_registration(net, n_test=1, n_iters=10, log=lambda m: None)
     assert r["problems"] == 1 and 0 <= r["converged_from_cnn"] <= 1
+    assert len(r["settled_by_iteration"]["cnn"]) == 2  # checks at iterations 5 and 10
     assert r["median_zncc_cnn"] > 0.5

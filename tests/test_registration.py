@@ -201,3 +201,14 @@ def test_zncc_rejects_host_tensors():
 
     with pytest.raises(ValueError, match="GPU"):
         zncc(torch.rand(2, 1, 4, 4), torch.rand(2, 1, 4, 4))
+
+
+def test_settled_iteration_is_the_start_of_the_final_run_below_1mm():
+    from nanodrr_amd.posenet import _settled
+
+    trace = [(5, [2.0, 0.5, 0.9]), (10, [0.8, 0.6, 0.8]), (15, [1.2, 0.4, 0.7]), (20, [0.9, 0.3, 0.6]),
+             (25, [0.7, 1.1, 0.5])]
+    assert _settled(trace, 0) == 20    # dipped below at 10, rose at 15, below from 20 on
+    assert _settled(trace, 1) is None  # ends above 1 mm
+    assert _settled(trace, 2) == 5     # below from the first check
+    assert _settled([], 0) is None
