@@ -433,34 +433,12 @@ __device__ inline uint32_t record_of(const Src& s, const Cell& c, int row) {
              static_cast<uint32_t>(row) + static_cast<uint32_t>(c.ix + 1);
 }
 
-// Stencil records ordered in bricks of 2 x 2 x 2 cells (eight 16-byte records,
-// one 128-byte line), so that the samples of neighbouring rays and of
-// consecutive steps along a ray share lines in every direction.
-#ifndef NDRR_STENCIL_BRICK
-#define NDRR_STENCIL_BRICK 0
-#endif
-__device__ inline uint32_t stencil_record_of(const Src& s, const Cell& c) {
-#if NDRR_STENCIL_BRICK
-  const uint32_t cx = static_cast<uint32_t>(c.ix + 1), cy = static_cast<uint32_t>(c.iy + 1);
-  const uint32_t cz = static_cast<uint32_t>(c.iz + 1);
-  const uint32_t nbx = static_cast<uint32_t>(s.W + 2) >> 1, nby = static_cast<uint32_t>(s.H + 2) >> 1;
-  return ((((cz >> 1) * nby + (cy >> 1)) * nbx + (cx >> 1)) << 3) | ((cz & 1u) << 2) | ((cy & 1u) << 1) | (cx & 1u);
-#else
-  return record_of(s, c, s.W + 1);
-#endif
-}
-
 // Byte offset of a cell's record, or one past 2 GiB (so past the end of the
 // buffer) when the cell does not exist. Bit arithmetic rather than a select,
 // which the compiler would otherwise turn into a divergent branch.
 __device__ inline uint32_t record_offset(const Src& s, const Cell& c, int row, uint32_t bytes) {
   const uint32_t gone = static_cast<uint32_t>(!cell_exists(s, c)) << 31;
   return (record_of(s, c, row) * bytes) | gone;
-}
-
-__device__ inline uint32_t stencil_offset(const Src& s, const Cell& c, uint32_t bytes) {
-  const uint32_t gone = static_cast<uint32_t>(!cell_exists(s, c)) << 31;
-  return (stencil_record_of(s, c) * bytes) | gone;
 }
 
 __device__ inline uint4 load16(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
@@ -471,7 +449,7 @@ template <int DT> struct Fetch<DT, NDRR_LAYOUT_STENCIL> {  // bf16 / fp16: 16-by
   static constexpr int kInFlight = NDRR_IN_FLIGHT;
   uint4 q;
   __device__ inline void load(const Src& s, const Cell& c) {
-    const uint32_t off = stencil_offset(s, c, 16u);
+    const uint32_t off = record_offset(s, c, s.W + 1, 16u);
     q = load16(s.vrs, off);
   }
   __device__ inline Corners corners() const {
@@ -488,7 +466,7 @@ template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_STENCIL> {  // fp32: 32-byte reco
   static constexpr int kInFlight = 2;
   uint4 a, b;
   __device__ inline void load(const Src& s, const Cell& c) {
-    const uint32_t off = stencil_offset(s, c, 32u);
+    const uint32_t off = record_offset(s, c, s.W + 1, 32u);
     a = load16(s.vrs, off);
     b = load16(s.vrs, off + 16u);
   }
@@ -1038,19 +1016,10 @@ __global__ void k_pack_stencil(const typename Vox<DT>::T* __restrict__ vol, void
   using T = typename Vox<DT>::T;
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (r >= n_rec) return;
-#if NDRR_STENCIL_BRICK
-  const int nbx = (W + 2) / 2, nby = (H + 2) / 2;
-  const int64_t b = r >> 3;
-  const int j = static_cast<int>(r & 7);
-  const int cx = 2 * static_cast<int>(b % nbx) + (j & 1);
-  const int cy = 2 * static_cast<int>((b / nbx) % nby) + ((j >> 1) & 1);
-  const int cz = 2 * static_cast<int>(b / (static_cast<int64_t>(nbx) * nby)) + (j >> 2);
-#else
   const int cx = static_cast<int>(r % (W + 1));
   const int64_t rest = r / (W + 1);
   const int cy = static_cast<int>(rest % (H + 1));
   const int cz = static_cast<int>(rest / (H + 1));
-#endif
   T v[8];
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
@@ -1144,11 +1113,7 @@ int nblocks_of(int N, int width, int& tiles_w, int& IH) {
 }
 
 int64_t stencil_records(int D, int H, int W) {
-#if NDRR_STENCIL_BRICK
-  return static_cast<int64_t>((D + 2) / 2) * ((H + 2) / 2) * ((W + 2) / 2) * 8;
-#else
   return static_cast<int64_t>(D + 1) * (H + 1) * (W + 1);
-#endif
 }
 
 int64_t quad_records(int D, int H, int W) {
