@@ -3,8 +3,9 @@
 //     vector, composition with the fixed camera/pivot transforms, and its
 //     analytic adjoint back to the (rotation, translation) parameters;
 //   * zero-normalised cross-correlation of each rendered image with its
-//     target, with the gradient, one workgroup per image and fixed-order
-//     reductions (deterministic and capturable in a HIP graph).
+//     target, with the gradient: chunk moments over (chunk, image)
+//     workgroups merged in a fixed order (deterministic);
+//   * the Adam update of the registration parameters.
 //
 // The parameterisation follows nanodrr's Registration module
 // (src/nanodrr/registration/registration.py:64-68):

@@ -113,19 +113,25 @@ class _ConvStack(torch.autograd.Function):
 
 
 class PoseNet(torch.nn.Module):
-    """(N, 1, 128, 128) DRRs -> (N, 6) = (rotation vector [rad], translation [mm] / xyz_scale)."""
+    """(N, 1, 128, 128) DRRs -> (N, 6) = (rotation vector [rad], translation [mm] / xyz_scale).
 
-    def __init__(self, xyz_scale: float = 50.0, seed: int = 0):
+    The last feature map (16 x 16 x 256 for 128 x 128 inputs) is average-pooled to a
+    `pool` x `pool` grid and flattened into the MLP, so the head sees where features
+    lie: a global average (pool = 1) makes the features nearly invariant to in-plane
+    shifts of the anatomy, i.e. to the translation the network has to estimate."""
+
+    def __init__(self, xyz_scale: float = 50.0, seed: int = 0, pool: int = 8):
         super().__init__()
         g = torch.Generator().manual_seed(seed)
         self.xyz_scale = xyz_scale
+        self.pool = pool
         self.weights = torch.nn.ParameterList()
         self.biases = torch.nn.ParameterList()
         for cin, cout, _ in LAYERS:
             std = math.sqrt(2.0 / (9 * cin))  # He initialisation for ReLU
             self.weights.append(torch.nn.Parameter(torch.randn(cout, 3, 3, cin, generator=g) * std))
             self.biases.append(torch.nn.Parameter(torch.zeros(cout)))
-        self.fc1 = torch.nn.Linear(256, 256)
+        self.fc1 = torch.nn.Linear(256 * pool * pool, 256)
         self.fc2 = torch.nn.Linear(256, 6)
         with torch.no_grad():
             self.fc2.weight.mul_(0.1)
@@ -142,8 +148,12 @@ class PoseNet(torch.nn.Module):
         x = self.prepare(drr)
         params = [t for wb in zip(self.weights, self.biases) for t in wb]
         feat = _ConvStack.apply(x, *params)            # (N, 16, 16, 256) bf16
-        h = feat.float().mean(dim=(1, 2))              # global average pool
-        return self.fc2(torch.relu(self.fc1(h)))
+        return self.fc2(torch.relu(self.fc1(self.pooled(feat))))
+
+    def pooled(self, feat: torch.Tensor) -> torch.Tensor:
+        """(N, pool * pool * 256) features: the map averaged over pool x pool cells, flattened."""
+        h = torch.nn.functional.adaptive_avg_pool2d(feat.float().permute(0, 3, 1, 2), self.pool)
+        return h.flatten(1)
 
     def split(self, out: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
         """(rotation vector, translation in mm) of the network output."""
