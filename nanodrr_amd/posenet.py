@@ -225,13 +225,13 @@ def pose_loss(net: PoseNet, out, rot, xyz, sdd: float = 1020.0):
 
 
 def train(steps: int = 20000, batch: int = 256, seeds=tuple(range(8)), rot_deg: float = 45.0, trans_mm: float = 60.0,
-          lr: float = 1e-3, seed: int = 0, device="cuda:0", log=print, log_every: int = 500):
+          lr: float = 1e-3, seed: int = 0, device="cuda:0", log=print, log_every: int = 500, pool: int = 8):
     """Train PoseNet on DRRs rendered on the fly (each step: batch / len(seeds) poses per
     phantom, a fresh sub-step lattice offset per DRR). Returns (net, history)."""
     import time
 
     src = DrrSource(seeds, device=device)
-    net = PoseNet(seed=seed).to(device)
+    net = PoseNet(seed=seed, pool=pool).to(device)
     opt = torch.optim.Adam(net.parameters(), lr=lr, fused=True)  # one launch for all parameters
     sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, steps, eta_min=lr * 0.02)
     gen = torch.Generator(device=device).manual_seed(seed)
@@ -382,14 +382,15 @@ def main(argv=None):
     ap.add_argument("--out", default=None, help="write the summary JSON here")
     ap.add_argument("--save", default=None, help="write the trained network's state dict here")
     ap.add_argument("--load", default=None, help="skip training: load a state dict written by --save")
+    ap.add_argument("--pool", type=int, default=8, help="side of the pooled feature grid fed to the MLP")
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     if args.load:
-        net = PoseNet().to("cuda:0")
+        net = PoseNet(pool=args.pool).to("cuda:0")
         net.load_state_dict(torch.load(args.load, map_location="cuda:0", weights_only=True))
         summary = {"train": {"loaded": args.load}}
     else:
-        net, hist, seconds = train(args.steps, args.batch, log=log)
+        net, hist, seconds = train(args.steps, args.batch, log=log, pool=args.pool)
         summary = {"train": {"steps": args.steps, "batch": args.batch, "seconds": seconds,
                              "drr_per_s": args.steps * args.batch / seconds, "final": hist[-1]}}
     if args.save:
