@@ -275,14 +275,18 @@ def _settled(trace, j):
 
 
 def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_deg: float = 30.0,
-                         trans_mm: float = 40.0, n_iters: int = 300, device="cuda:0", log=print):
+                         trans_mm: float = 40.0, n_iters: int = 300, device="cuda:0", log=print,
+                         lr_rot: float = 1e-2, lr_xyz: float = 1.0, short_iters: int | None = None):
     """On n_test unseen phantoms, register one target each (200 x 200, targets at
     +-rot_deg / +-trans_mm about the nominal pose, S = 1024 target DRR) from the
     nominal pose and from the CNN's proposal (its input: the target pose rendered at
     128 x 128); both runs share one batch of two poses. Reports mTRE < 1 mm rates,
     median mTRE and final ZNCC per start, and the render iterations each start needs:
     a problem has settled at the first check (every 5 iterations) from which its mTRE
-    stays below 1 mm to the end."""
+    stays below 1 mm to the end. The CNN start is also registered alone with a whole
+    schedule of `short_iters` iterations (default n_iters // 5); every run uses the
+    learning rates lr_rot (rad) and lr_xyz (mm)."""
+    short_iters = short_iters or max(n_iters // 5, 1)
     from .metrics import mtre, zncc
     from .phantom import detector, fiducials, make_phantom, nominal_pose, phantom_subject
     from .registration import Registration, register
@@ -319,15 +323,16 @@ def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_
         def monitor(k, r):
             trace.append((k, mtre(r.pose.detach(), rt_true.expand(2, 4, 4), fid).tolist()))
 
-        score = register(reg, target.expand(2, -1, -1, -1).contiguous(), n_iters=n_iters, n_samples=256,
-                         jitter=True, seed=i, monitor=monitor, monitor_every=check_every)
+        score = register(reg, target.expand(2, -1, -1, -1).contiguous(), n_iters=n_iters, lr_rot=lr_rot,
+                         lr_xyz=lr_xyz, n_samples=256, jitter=True, seed=i, monitor=monitor,
+                         monitor_every=check_every)
         for j, name in enumerate(("nominal", "cnn")):
             res["settled"][name].append(_settled(trace, j))
         # the CNN start again, with a whole schedule of a fifth of the iterations
         short = Registration(subject, rt_nom, k200, sdd, 200, 200, per_pose=True)
         short._rot.data.copy_(rot_p)
         short._xyz.data.copy_(xyz_p)
-        register(short, target, n_iters=max(n_iters // 5, 1), n_samples=256, jitter=True, seed=i)
+        register(short, target, n_iters=short_iters, lr_rot=lr_rot, lr_xyz=lr_xyz, n_samples=256, jitter=True, seed=i)
         res["cnn_fifth"].append(float(mtre(short.pose.detach(), rt_true, fid)[0]))
         e = mtre(reg.pose.detach(), rt_true.expand(2, 4, 4), fid)
         res["nominal"].append(float(e[0]))
@@ -337,7 +342,7 @@ def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_
         if (i + 1) % 8 == 0:
             log(f"{i + 1}/{n_test}: nominal<1mm {sum(x < 1 for x in res['nominal'])}, "
                 f"cnn<1mm {sum(x < 1 for x in res['cnn'])}, "
-                f"cnn<1mm in {max(n_iters // 5, 1)} iterations {sum(x < 1 for x in res['cnn_fifth'])}")
+                f"cnn<1mm in {short_iters} iterations {sum(x < 1 for x in res['cnn_fifth'])}")
     t = torch.tensor
     curves = {name: [sum(1 for x in res["settled"][name] if x is not None and x <= k)
                      for k in range(check_every, n_iters + 1, check_every)] for name in ("nominal", "cnn")}
@@ -358,8 +363,10 @@ def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_
         "median_iterations_to_1mm_cnn": med(res["settled"]["cnn"]),
         "cnn_iterations_to_match_plain_final": cnn_match,
         "cnn_over_plain_iteration_ratio": None if cnn_match is None else cnn_match / n_iters,
-        "converged_from_cnn_with_a_fifth_of_the_iterations": int((t(res["cnn_fifth"]) < 1).sum()),
-        "median_mtre_mm_cnn_with_a_fifth_of_the_iterations": float(t(res["cnn_fifth"]).median()),
+        "learning_rates": [lr_rot, lr_xyz],
+        "short_iterations": short_iters,
+        "converged_from_cnn_with_the_short_schedule": int((t(res["cnn_fifth"]) < 1).sum()),
+        "median_mtre_mm_cnn_with_the_short_schedule": float(t(res["cnn_fifth"]).median()),
         "converged_from_nominal": int((t(res["nominal"]) < 1).sum()),
         "converged_from_cnn": int((t(res["cnn"]) < 1).sum()),
         "median_mtre_mm_nominal": float(t(res["nominal"]).median()),
@@ -383,6 +390,9 @@ def main(argv=None):
     ap.add_argument("--save", default=None, help="write the trained network's state dict here")
     ap.add_argument("--load", default=None, help="skip training: load a state dict written by --save")
     ap.add_argument("--pool", type=int, default=8, help="side of the pooled feature grid fed to the MLP")
+    ap.add_argument("--iters", type=int, default=300, help="registration iterations of the comparison")
+    ap.add_argument("--short-iters", type=int, default=None, help="whole schedule of the short CNN-started run")
+    ap.add_argument("--lr-scale", type=float, default=1.0, help="multiplies the 1e-2 rad / 1 mm learning rates")
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     if args.load:
@@ -396,7 +406,9 @@ def main(argv=None):
     if args.save:
         torch.save(net.state_dict(), args.save)
     if args.test > 0:
-        summary["registration"] = compare_registration(net, n_test=args.test, log=log)
+        summary["registration"] = compare_registration(net, n_test=args.test, log=log, n_iters=args.iters,
+                                                       lr_rot=1e-2 * args.lr_scale, lr_xyz=1.0 * args.lr_scale,
+                                                       short_iters=args.short_iters)
     line = json.dumps(summary)
     print(line, flush=True)
     if args.out:
