@@ -390,6 +390,7 @@ def main(argv=None):
     ap.add_argument("--save", default=None, help="write the trained network's state dict here")
     ap.add_argument("--load", default=None, help="skip training: load a state dict written by --save")
     ap.add_argument("--pool", type=int, default=8, help="side of the pooled feature grid fed to the MLP")
+    ap.add_argument("--lr", type=float, default=1e-3, help="peak training learning rate (cosine to 2 %%)")
     ap.add_argument("--iters", type=int, default=300, help="registration iterations of the comparison")
     ap.add_argument("--short-iters", type=int, default=None, help="whole schedule of the short CNN-started run")
     ap.add_argument("--lr-scale", type=float, default=1.0, help="multiplies the 1e-2 rad / 1 mm learning rates")
@@ -400,7 +401,7 @@ def main(argv=None):
         net.load_state_dict(torch.load(args.load, map_location="cuda:0", weights_only=True))
         summary = {"train": {"loaded": args.load}}
     else:
-        net, hist, seconds = train(args.steps, args.batch, log=log, pool=args.pool)
+        net, hist, seconds = train(args.steps, args.batch, lr=args.lr, log=log, pool=args.pool)
         summary = {"train": {"steps": args.steps, "batch": args.batch, "seconds": seconds,
                              "drr_per_s": args.steps * args.batch / seconds, "final": hist[-1]}}
     if args.save:
