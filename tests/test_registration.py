@@ -212,3 +212,13 @@ def test_settled_iteration_is_the_start_of_the_final_run_below_1mm():
     assert _settled(trace, 1) is None  # ends above 1 mm
     assert _settled(trace, 2) == 5     # below from the first check
     assert _settled([], 0) is None
+
+
+def test_register_rejects_unknown_adjoint_and_graph_mode():
+    """Argument errors come back before anything touches a device."""
+    from nanodrr_amd.registration import register
+
+    with pytest.raises(ValueError, match="adjoint"):
+        register(None, None, adjoint="finite-differences")
+    with pytest.raises(NotImplementedError, match="graph"):
+        register(None, None, graph=True)
