@@ -902,57 +902,71 @@ void k_bwd(Params p) {
 // which are the per-ray sums k_bwd forms sample by sample (the per-sample
 // factor go * step moved out of the sums); the pose partials, gsrc and gtgt
 // follow from G and T exactly as in k_bwd.
+// Rays are taken in their linear order: a workgroup takes kJacRays
+// consecutive ray blocks of kTileRays rays (grid (ceil(N / (kTileRays
+// kJacRays)), B)) and each thread one ray of every block, so every plane is
+// read in full cache lines and several rays' loads are in flight per thread;
+// a thread adds its rays' terms in block order, then the workgroup reduces
+// once. The pose partials go to gpart[b][blockIdx.x].
+#ifndef NDRR_JAC_RAYS
+#define NDRR_JAC_RAYS 4
+#endif
+constexpr int kJacRays = NDRR_JAC_RAYS;
+
 template <int CV, bool GCAM>
-// Rays are taken in their linear order (kTileRays consecutive rays per
-// workgroup, grid (ceil(N / kTileRays), B)), so every plane is read in full
-// cache lines; the pose partials go to gpart[b][blockIdx.x].
 __global__ __launch_bounds__(kTileRays) void k_bwd_jac(Params p) {
   const int b = blockIdx.y, blk = blockIdx.x;
-  const int n = blk * kTileRays + threadIdx.x;
-  const bool valid = n < p.N;
-  const int nn = valid ? n : 0;
-  const float* sp = p.src + b * p.src_bs + nn * p.src_rs;
-  const float* tp = p.tgt + b * p.tgt_bs + 3 * static_cast<int64_t>(nn);
-  const float sx = sp[0], sy = sp[1], sz = sp[2];
-  const float ex = tp[0] - sx, ey = tp[1] - sy, ez = tp[2] - sz;
-  const float step = p.step[b * p.step_bs + nn];
-  const int64_t plane = static_cast<int64_t>(p.C) * p.N;
-  float Gx = 0.f, Gy = 0.f, Gz = 0.f, Tx = 0.f, Ty = 0.f, Tz = 0.f, astep = 0.f;
-#pragma unroll
-  for (int c = 0; c < CV; ++c) {
-    if (c < p.C) {
-      const float g = valid ? p.go[b * p.go_bs + static_cast<int64_t>(c) * p.N + nn] : 0.0f;
-      const float* jp = p.jac + (static_cast<int64_t>(b) * p.jac_q * p.C + c) * p.N + nn;
-      Gx += g * jp[0]; Gy += g * jp[plane]; Gz += g * jp[2 * plane];
-      Tx += g * jp[3 * plane]; Ty += g * jp[4 * plane]; Tz += g * jp[5 * plane];
-      if constexpr (GCAM) astep += g * jp[6 * plane];
-    }
-  }
-  Gx *= step; Gy *= step; Gz *= step;
-  Tx *= step; Ty *= step; Tz *= step;
   const float* __restrict__ Mb = p.M + 12 * b;
-  if constexpr (GCAM) {
-    if (valid) {
-      const float m00 = Mb[0], m01 = Mb[1], m02 = Mb[2];
-      const float m10 = Mb[4], m11 = Mb[5], m12 = Mb[6];
-      const float m20 = Mb[8], m21 = Mb[9], m22 = Mb[10];
-      const float qGx = m00 * Gx + m10 * Gy + m20 * Gz;  // M33^T G
-      const float qGy = m01 * Gx + m11 * Gy + m21 * Gz;
-      const float qGz = m02 * Gx + m12 * Gy + m22 * Gz;
-      const float qTx = m00 * Tx + m10 * Ty + m20 * Tz;  // M33^T T
-      const float qTy = m01 * Tx + m11 * Ty + m21 * Tz;
-      const float qTz = m02 * Tx + m12 * Ty + m22 * Tz;
-      const int64_t ray = static_cast<int64_t>(b) * p.N + n;
-      p.gsrc[3 * ray + 0] = qGx - qTx; p.gsrc[3 * ray + 1] = qGy - qTy; p.gsrc[3 * ray + 2] = qGz - qTz;
-      p.gtgt[3 * ray + 0] = qTx; p.gtgt[3 * ray + 1] = qTy; p.gtgt[3 * ray + 2] = qTz;
-      p.gstep[ray] = astep;
+  const int64_t plane = static_cast<int64_t>(p.C) * p.N;
+  float a[12];
+#pragma unroll
+  for (int j = 0; j < 12; ++j) a[j] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kJacRays; ++k) {
+    const int n = (blk * kJacRays + k) * kTileRays + threadIdx.x;
+    const bool valid = n < p.N;
+    const int nn = valid ? n : 0;
+    const float* sp = p.src + b * p.src_bs + nn * p.src_rs;
+    const float* tp = p.tgt + b * p.tgt_bs + 3 * static_cast<int64_t>(nn);
+    const float sx = sp[0], sy = sp[1], sz = sp[2];
+    const float ex = tp[0] - sx, ey = tp[1] - sy, ez = tp[2] - sz;
+    const float step = p.step[b * p.step_bs + nn];
+    float Gx = 0.f, Gy = 0.f, Gz = 0.f, Tx = 0.f, Ty = 0.f, Tz = 0.f, astep = 0.f;
+#pragma unroll
+    for (int c = 0; c < CV; ++c) {
+      if (c < p.C) {
+        const float g = valid ? p.go[b * p.go_bs + static_cast<int64_t>(c) * p.N + nn] : 0.0f;
+        const float* jp = p.jac + (static_cast<int64_t>(b) * p.jac_q * p.C + c) * p.N + nn;
+        Gx += g * jp[0]; Gy += g * jp[plane]; Gz += g * jp[2 * plane];
+        Tx += g * jp[3 * plane]; Ty += g * jp[4 * plane]; Tz += g * jp[5 * plane];
+        if constexpr (GCAM) astep += g * jp[6 * plane];
+      }
     }
+    Gx *= step; Gy *= step; Gz *= step;
+    Tx *= step; Ty *= step; Tz *= step;
+    if constexpr (GCAM) {
+      if (valid) {
+        const float m00 = Mb[0], m01 = Mb[1], m02 = Mb[2];
+        const float m10 = Mb[4], m11 = Mb[5], m12 = Mb[6];
+        const float m20 = Mb[8], m21 = Mb[9], m22 = Mb[10];
+        const float qGx = m00 * Gx + m10 * Gy + m20 * Gz;  // M33^T G
+        const float qGy = m01 * Gx + m11 * Gy + m21 * Gz;
+        const float qGz = m02 * Gx + m12 * Gy + m22 * Gz;
+        const float qTx = m00 * Tx + m10 * Ty + m20 * Tz;  // M33^T T
+        const float qTy = m01 * Tx + m11 * Ty + m21 * Tz;
+        const float qTz = m02 * Tx + m12 * Ty + m22 * Tz;
+        const int64_t ray = static_cast<int64_t>(b) * p.N + n;
+        p.gsrc[3 * ray + 0] = qGx - qTx; p.gsrc[3 * ray + 1] = qGy - qTy; p.gsrc[3 * ray + 2] = qGz - qTz;
+        p.gtgt[3 * ray + 0] = qTx; p.gtgt[3 * ray + 1] = qTy; p.gtgt[3 * ray + 2] = qTz;
+        p.gstep[ray] = astep;
+      }
+    }
+    a[0] += sx * Gx + ex * Tx; a[1] += sy * Gx + ey * Tx; a[2] += sz * Gx + ez * Tx; a[3] += Gx;
+    a[4] += sx * Gy + ex * Ty; a[5] += sy * Gy + ey * Ty; a[6] += sz * Gy + ez * Ty; a[7] += Gy;
+    a[8] += sx * Gz + ex * Tz; a[9] += sy * Gz + ey * Tz; a[10] += sz * Gz + ez * Tz; a[11] += Gz;
   }
   constexpr int kWaves = kTileRays / kWave;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const float a[12] = {sx * Gx + ex * Tx, sy * Gx + ey * Tx, sz * Gx + ez * Tx, Gx,
-                       sx * Gy + ex * Ty, sy * Gy + ey * Ty, sz * Gy + ez * Ty, Gy,
-                       sx * Gz + ex * Tz, sy * Gz + ey * Tz, sz * Gz + ez * Tz, Gz};
   __shared__ float red[kWaves][12];
 #pragma unroll
   for (int j = 0; j < 12; ++j) {
@@ -1365,8 +1379,8 @@ int ndrr_raymarch_bwd_jac(const float* go, int64_t go_batch_stride, const float*
   p.nblocks = nblocks_of(N, width, p.tiles_w, p.IH);
   p.gpart = workspace;
   p.gsrc = gsrc; p.gtgt = gtgt; p.gstep = gstep;
-  // ceil(N / kTileRays) <= nblocks: the workspace of ndrr_bwd_workspace_floats holds the partials
-  const int nlin = (N + kTileRays - 1) / kTileRays;
+  // ceil(N / (kTileRays kJacRays)) <= nblocks: the workspace of ndrr_bwd_workspace_floats holds the partials
+  const int nlin = (N + kTileRays * kJacRays - 1) / (kTileRays * kJacRays);
   const dim3 grid(nlin, B), block(kTileRays);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const bool gcam = need_gcam != 0;
