@@ -391,6 +391,8 @@ def main():
             "kernels": (plain["ms_fwd"] + plain["ms_bwd"]) / (r["ms_fwd"] + r["ms_bwd"]),
         }
     if not args.no_registration:
+        # the cases set the renderer's layout and adjoint per case: back to the defaults
+        renderer.VOLUME_LAYOUT, renderer.ADJOINT = "auto", "single_pass"
         gc.collect()
         torch.cuda.empty_cache()  # the march cases' volumes and graph pools
         line["registration"] = registration_leg()
