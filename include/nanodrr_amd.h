@@ -175,6 +175,38 @@ int ndrr_raymarch_bwd_jac(const float* go, int64_t go_batch_stride,
                           void* stream);
 
 /*
+ * The same pair with the camera -> voxel composition folded in: in place of
+ * M they take world_to_voxel (4 x 4 float64, row-major) and rt_inv
+ * ([B][4][4] fp32), and every workgroup composes its pose's
+ * M = (world_to_voxel @ rt_inv[b])[:3] in FP64, rounded once to fp32, with
+ * the sums of ndrr_camera_to_voxel; the backward's pose reduction returns
+ * g_rt_inv ([B][4][4] fp32) = world_to_voxel[:3]^T gM in FP64 from the
+ * rounded gM, with the sums of ndrr_camera_to_voxel_bwd. Results are those
+ * of ndrr_camera_to_voxel + ndrr_raymarch_fwd_jac and ndrr_raymarch_bwd_jac +
+ * ndrr_camera_to_voxel_bwd, bit for bit, in two launches fewer. Replaces the
+ * composition at src/nanodrr/drr/backends.py:96 together with the ops of
+ * _fused.py:415-460,477-531, and its autograd.
+ */
+int ndrr_raymarch_fwd_jac_pose(const void* vol, int vol_dtype, int vol_layout,
+                               const int8_t* lab, const double* world_to_voxel,
+                               const float* rt_inv, const float* src,
+                               const float* tgt, const float* step, float* out,
+                               float* jac, int jac_planes, int B, int N, int D,
+                               int H, int W, int n_samples, int n_classes, int width,
+                               int64_t src_batch_stride, int64_t src_ray_stride,
+                               int64_t tgt_batch_stride, int64_t step_batch_stride,
+                               void* stream);
+int ndrr_raymarch_bwd_jac_pose(const float* go, int64_t go_batch_stride,
+                               const float* jac, int jac_planes,
+                               const double* world_to_voxel, const float* rt_inv,
+                               const float* src, const float* tgt, const float* step,
+                               float* workspace, float* g_rt_inv, float* gsrc,
+                               float* gtgt, float* gstep, int need_gcam, int B, int N,
+                               int n_classes, int width, int64_t src_batch_stride,
+                               int64_t src_ray_stride, int64_t tgt_batch_stride,
+                               int64_t step_batch_stride, void* stream);
+
+/*
  * Packed copies of a [D][H][W] volume that the march reads with one load per
  * sample (two 16-byte loads for fp32):
  *   NDRR_LAYOUT_QUAD: one record per (x, y0, z0), x in [-1, W], y0 in [-1, H-1],

@@ -41,6 +41,19 @@ SIGNATURES: dict[str, tuple] = {
          _c_int, _c_int, _c_int, _c_int,
          _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
     ),
+    # the same with (world_to_voxel, rt_inv) in place of M (and the rt_inv gradient in place of gM)
+    "ndrr_raymarch_fwd_jac_pose": (
+        _c_int,
+        [_c_p, _c_int, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_int,
+         _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+         _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
+    ),
+    "ndrr_raymarch_bwd_jac_pose": (
+        _c_int,
+        [_c_p, _c_i64, _c_p, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_int,
+         _c_int, _c_int, _c_int, _c_int,
+         _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
+    ),
     "ndrr_bwd_workspace_floats": (_c_size, [_c_int, _c_int, _c_int]),
     "ndrr_launch_tile": (None, [ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
     "ndrr_raymarch_bwd": (

@@ -108,8 +108,9 @@ def _(lab, layout):
     return torch.empty(D, H, W, device=lab.device, dtype=torch.int8)
 
 
-def _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width, layout="linear"):
-    """Validate shapes/devices and return the launch geometry (raises before any launch)."""
+def _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width, layout="linear", B=None):
+    """Validate shapes/devices and return the launch geometry (raises before any launch). M may be
+    None when the kernels compose it themselves; B is then given."""
     if not vol.is_cuda:
         raise ValueError("nanodrr_amd renders on a GPU: the volume must be a device tensor")
     if vol.dim() != 3 or not vol.is_contiguous():
@@ -120,9 +121,10 @@ def _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width, layout="
         raise ValueError("n_samples must be at least 2")
     if not 1 <= n_classes <= _lib.MAX_CLASSES:
         raise ValueError(f"n_classes must lie in [1, {_lib.MAX_CLASSES}], got {n_classes}")
-    if M.dim() != 3 or M.shape[1:] != (3, 4) or M.dtype != torch.float32:
-        raise ValueError("M must be a float32 (B, 3, 4) tensor")
-    B = M.shape[0]
+    if M is not None:
+        if M.dim() != 3 or M.shape[1:] != (3, 4) or M.dtype != torch.float32:
+            raise ValueError("M must be a float32 (B, 3, 4) tensor")
+        B = M.shape[0]
     if tgt.dim() != 3 or tgt.shape[2] != 3 or tgt.shape[0] not in (1, B):
         raise ValueError("tgt must be (B or 1, N, 3)")
     N = tgt.shape[1]
@@ -131,6 +133,8 @@ def _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width, layout="
     if step.dim() != 2 or step.shape[1] != N or step.shape[0] not in (1, B):
         raise ValueError("step must be (B or 1, N)")
     for name, t in (("M", M), ("src", src), ("tgt", tgt), ("step", step)):
+        if t is None:
+            continue
         if t.dtype != torch.float32 or not t.is_contiguous() or t.device != vol.device:
             raise ValueError(f"{name} must be a contiguous float32 tensor on {vol.device}")
     if n_classes > 1:
@@ -327,6 +331,41 @@ def _backward(ctx, go, bwd=None):
 torch.library.register_autograd("nanodrr_amd::fused_raymarch", _backward, setup_context=_setup_context)
 
 
+def _check_pose(w2v: torch.Tensor, rt_inv: torch.Tensor, vol: torch.Tensor) -> None:
+    if w2v.dtype != torch.float64 or w2v.shape != (4, 4) or not w2v.is_contiguous() or w2v.device != vol.device:
+        raise ValueError(f"world_to_voxel must be a contiguous float64 (4, 4) tensor on {vol.device}")
+    if rt_inv.dtype != torch.float32 or rt_inv.dim() != 3 or rt_inv.shape[1:] != (4, 4) \
+            or not rt_inv.is_contiguous() or rt_inv.device != vol.device:
+        raise ValueError(f"rt_inv must be a contiguous float32 (B, 4, 4) tensor on {vol.device}")
+
+
+def _jac_forward(vol, lab, M, pose, src, tgt, step, n_samples, n_classes, width, jac_planes, packed, layout):
+    """The single-pass forward with M (B, 3, 4) given, or pose = (world_to_voxel, rt_inv) and M
+    composed by the kernels; jac_planes 0 (pose form only) skips the sums."""
+    if jac_planes not in ((0, 6, 7) if pose is not None else (6, 7)):
+        raise ValueError(f"jac_planes must be 6 or 7, got {jac_planes}")
+    if pose is not None:
+        _check_pose(*pose, vol)
+    B, N, D, H, W, strides = _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width,
+                                       layout if packed is not None else "linear",
+                                       B=None if pose is None else pose[1].shape[0])
+    vptr, code = _volume_arg(vol, packed, layout)
+    out = torch.empty(B, n_classes, N, device=vol.device, dtype=torch.float32)
+    jac = (torch.empty(B, jac_planes, n_classes, N, device=vol.device, dtype=torch.float32) if jac_planes
+           else out.new_empty(0))
+    lib = _lib.lib()
+    head = (vptr, _DTYPE_CODE[vol.dtype], code, _ptr(lab) if n_classes > 1 else None)
+    tail = (src.data_ptr(), tgt.data_ptr(), step.data_ptr(), out.data_ptr(), _ptr(jac), jac_planes, B, N, D, H, W,
+            n_samples, n_classes, width, *strides, _stream(vol.device))
+    with torch.cuda.device(vol.device):
+        if pose is None:
+            rc = lib.ndrr_raymarch_fwd_jac(*head, M.data_ptr(), *tail)
+        else:
+            rc = lib.ndrr_raymarch_fwd_jac_pose(*head, pose[0].data_ptr(), pose[1].data_ptr(), *tail)
+    _lib.check(rc, "ndrr_raymarch_fwd_jac" if pose is None else "ndrr_raymarch_fwd_jac_pose")
+    return out, jac
+
+
 def _fused_raymarch_jac_impl(
     vol: torch.Tensor,
     lab: torch.Tensor,
@@ -341,21 +380,7 @@ def _fused_raymarch_jac_impl(
     packed: Optional[torch.Tensor] = None,
     layout: str = "linear",
 ) -> tuple[torch.Tensor, torch.Tensor]:
-    if jac_planes not in (6, 7):
-        raise ValueError(f"jac_planes must be 6 or 7, got {jac_planes}")
-    B, N, D, H, W, strides = _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width,
-                                       layout if packed is not None else "linear")
-    vptr, code = _volume_arg(vol, packed, layout)
-    out = torch.empty(B, n_classes, N, device=vol.device, dtype=torch.float32)
-    jac = torch.empty(B, jac_planes, n_classes, N, device=vol.device, dtype=torch.float32)
-    with torch.cuda.device(vol.device):
-        rc = _lib.lib().ndrr_raymarch_fwd_jac(
-            vptr, _DTYPE_CODE[vol.dtype], code, _ptr(lab) if n_classes > 1 else None,
-            M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(), out.data_ptr(), jac.data_ptr(),
-            jac_planes, B, N, D, H, W, n_samples, n_classes, width, *strides, _stream(vol.device),
-        )
-    _lib.check(rc, "ndrr_raymarch_fwd_jac")
-    return out, jac
+    return _jac_forward(vol, lab, M, None, src, tgt, step, n_samples, n_classes, width, jac_planes, packed, layout)
 
 
 @torch.library.custom_op("nanodrr_amd::fused_raymarch_jac", mutates_args=())
@@ -388,18 +413,12 @@ def _(vol, lab, M, src, tgt, step, n_samples, n_classes, width, jac_planes, pack
             torch.empty(B, jac_planes, n_classes, N, device=vol.device, dtype=torch.float32))
 
 
-def _fused_raymarch_bwd_jac_impl(
-    go: torch.Tensor,
-    jac: torch.Tensor,
-    M: torch.Tensor,
-    src: torch.Tensor,
-    tgt: torch.Tensor,
-    step: torch.Tensor,
-    width: int,
-    need_gcam: bool,
-) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
+def _jac_backward(go, jac, M, pose, src, tgt, step, width, need_gcam):
+    """The contraction with M given (returns gM (B, 3, 4)) or pose = (world_to_voxel, rt_inv) (returns
+    the rt_inv gradient (B, 4, 4)), then gsrc, gtgt, gstep (empty without need_gcam)."""
     B, planes, C, N = jac.shape
-    if M.shape != (B, 3, 4) or tgt.shape[1] != N:
+    if (M is not None and M.shape != (B, 3, 4)) or (pose is not None and pose[1].shape != (B, 4, 4)) \
+            or tgt.shape[1] != N:
         raise ValueError("jac does not match the geometry")
     if need_gcam and planes != 7:
         raise ValueError("camera gradients need the 7-plane jac (the forward ran without them)")
@@ -416,18 +435,33 @@ def _fused_raymarch_bwd_jac_impl(
     strides = (0 if src.shape[0] == 1 else src.shape[1] * 3, 0 if src.shape[1] == 1 else 3,
                0 if tgt.shape[0] == 1 else N * 3, 0 if step.shape[0] == 1 else N)
     ws = torch.empty(lib.ndrr_bwd_workspace_floats(B, N, width), device=dev, dtype=torch.float32)
-    gM = torch.empty(B, 3, 4, device=dev, dtype=torch.float32)
-    gsrc = torch.empty(B, N, 3, device=dev, dtype=torch.float32) if need_gcam else gM.new_empty(0)
-    gtgt = torch.empty(B, N, 3, device=dev, dtype=torch.float32) if need_gcam else gM.new_empty(0)
-    gstep = torch.empty(B, N, device=dev, dtype=torch.float32) if need_gcam else gM.new_empty(0)
+    g = torch.empty((B, 3, 4) if pose is None else (B, 4, 4), device=dev, dtype=torch.float32)
+    gsrc = torch.empty(B, N, 3, device=dev, dtype=torch.float32) if need_gcam else g.new_empty(0)
+    gtgt = torch.empty(B, N, 3, device=dev, dtype=torch.float32) if need_gcam else g.new_empty(0)
+    gstep = torch.empty(B, N, device=dev, dtype=torch.float32) if need_gcam else g.new_empty(0)
+    mid = (src.data_ptr(), tgt.data_ptr(), step.data_ptr(), ws.data_ptr(), g.data_ptr(), _ptr(gsrc), _ptr(gtgt),
+           _ptr(gstep), int(need_gcam), B, N, C, width, *strides, _stream(dev))
     with torch.cuda.device(dev):
-        rc = lib.ndrr_raymarch_bwd_jac(
-            go.data_ptr(), go_bs, jac.data_ptr(), planes, M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(),
-            ws.data_ptr(), gM.data_ptr(), _ptr(gsrc), _ptr(gtgt), _ptr(gstep), int(need_gcam), B, N, C, width,
-            *strides, _stream(dev),
-        )
-    _lib.check(rc, "ndrr_raymarch_bwd_jac")
-    return gM, gsrc, gtgt, gstep
+        if pose is None:
+            rc = lib.ndrr_raymarch_bwd_jac(go.data_ptr(), go_bs, jac.data_ptr(), planes, M.data_ptr(), *mid)
+        else:
+            rc = lib.ndrr_raymarch_bwd_jac_pose(go.data_ptr(), go_bs, jac.data_ptr(), planes, pose[0].data_ptr(),
+                                                pose[1].data_ptr(), *mid)
+    _lib.check(rc, "ndrr_raymarch_bwd_jac" if pose is None else "ndrr_raymarch_bwd_jac_pose")
+    return g, gsrc, gtgt, gstep
+
+
+def _fused_raymarch_bwd_jac_impl(
+    go: torch.Tensor,
+    jac: torch.Tensor,
+    M: torch.Tensor,
+    src: torch.Tensor,
+    tgt: torch.Tensor,
+    step: torch.Tensor,
+    width: int,
+    need_gcam: bool,
+) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
+    return _jac_backward(go, jac, M, None, src, tgt, step, width, need_gcam)
 
 
 @torch.library.custom_op("nanodrr_amd::fused_raymarch_bwd_jac", mutates_args=())
@@ -485,6 +519,132 @@ def _backward_jac(ctx, go, g_jac, bwd=None):
 
 
 torch.library.register_autograd("nanodrr_amd::fused_raymarch_jac", _backward_jac, setup_context=_setup_context_jac)
+
+
+def _fused_raymarch_pose_impl(
+    vol: torch.Tensor,
+    lab: torch.Tensor,
+    world_to_voxel: torch.Tensor,
+    rt_inv: torch.Tensor,
+    src: torch.Tensor,
+    tgt: torch.Tensor,
+    step: torch.Tensor,
+    n_samples: int,
+    n_classes: int,
+    width: int,
+    jac_planes: int,
+    packed: Optional[torch.Tensor] = None,
+    layout: str = "linear",
+) -> tuple[torch.Tensor, torch.Tensor]:
+    return _jac_forward(vol, lab, None, (world_to_voxel, rt_inv), src, tgt, step, n_samples, n_classes, width,
+                        jac_planes, packed, layout)
+
+
+@torch.library.custom_op("nanodrr_amd::fused_raymarch_pose", mutates_args=())
+def fused_raymarch_pose(
+    vol: torch.Tensor,
+    lab: torch.Tensor,
+    world_to_voxel: torch.Tensor,
+    rt_inv: torch.Tensor,
+    src: torch.Tensor,
+    tgt: torch.Tensor,
+    step: torch.Tensor,
+    n_samples: int,
+    n_classes: int,
+    width: int,
+    jac_planes: int,
+    packed: Optional[torch.Tensor] = None,
+    layout: str = "linear",
+) -> tuple[torch.Tensor, torch.Tensor]:
+    """`fused_raymarch_jac` taking the pose itself: M = (world_to_voxel @ rt_inv)[:, :3] is composed
+    in FP64 by each workgroup (ndrr_raymarch_fwd_jac_pose), bitwise equal to camera_to_voxel then
+    fused_raymarch_jac. jac_planes 0 returns an empty jac (no gradient); the backward gives the
+    rt_inv gradient (B, 4, 4) and the camera gradients, no volume gradient."""
+    return _fused_raymarch_pose_impl(vol, lab, world_to_voxel, rt_inv, src, tgt, step, n_samples, n_classes, width,
+                                     jac_planes, packed, layout)
+
+
+@fused_raymarch_pose.register_fake
+def _(vol, lab, world_to_voxel, rt_inv, src, tgt, step, n_samples, n_classes, width, jac_planes, packed=None,
+      layout="linear"):
+    B, N = rt_inv.shape[0], tgt.shape[1]
+    out = torch.empty(B, n_classes, N, device=vol.device, dtype=torch.float32)
+    return out, (torch.empty(B, jac_planes, n_classes, N, device=vol.device, dtype=torch.float32) if jac_planes
+                 else out.new_empty(0))
+
+
+def _fused_raymarch_pose_bwd_impl(
+    go: torch.Tensor,
+    jac: torch.Tensor,
+    world_to_voxel: torch.Tensor,
+    rt_inv: torch.Tensor,
+    src: torch.Tensor,
+    tgt: torch.Tensor,
+    step: torch.Tensor,
+    width: int,
+    need_gcam: bool,
+) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
+    return _jac_backward(go, jac, None, (world_to_voxel, rt_inv), src, tgt, step, width, need_gcam)
+
+
+@torch.library.custom_op("nanodrr_amd::fused_raymarch_pose_bwd", mutates_args=())
+def fused_raymarch_pose_bwd(
+    go: torch.Tensor,
+    jac: torch.Tensor,
+    world_to_voxel: torch.Tensor,
+    rt_inv: torch.Tensor,
+    src: torch.Tensor,
+    tgt: torch.Tensor,
+    step: torch.Tensor,
+    width: int,
+    need_gcam: bool,
+) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Backward of `fused_raymarch_pose` from its jac output (ndrr_raymarch_bwd_jac_pose):
+    (g_rt_inv (B,4,4), gsrc (B,N,3)|empty, gtgt (B,N,3)|empty, gstep (B,N)|empty), fp32."""
+    return _fused_raymarch_pose_bwd_impl(go, jac, world_to_voxel, rt_inv, src, tgt, step, width, need_gcam)
+
+
+@fused_raymarch_pose_bwd.register_fake
+def _(go, jac, world_to_voxel, rt_inv, src, tgt, step, width, need_gcam):
+    B, _, _, N = jac.shape
+    g = torch.empty(B, 4, 4, device=jac.device, dtype=torch.float32)
+    gsrc = torch.empty(B, N, 3, device=jac.device, dtype=torch.float32) if need_gcam else g.new_empty(0)
+    gtgt = torch.empty(B, N, 3, device=jac.device, dtype=torch.float32) if need_gcam else g.new_empty(0)
+    gstep = torch.empty(B, N, device=jac.device, dtype=torch.float32) if need_gcam else g.new_empty(0)
+    return g, gsrc, gtgt, gstep
+
+
+def _setup_context_pose(ctx, inputs, output):
+    vol, lab, w2v, rt_inv, src, tgt, step, n_samples, n_classes, width, jac_planes, packed, layout = inputs
+    ctx.save_for_backward(w2v, rt_inv, src, tgt, step, output[1])
+    ctx.mark_non_differentiable(output[1])
+    ctx.set_materialize_grads(False)
+    ctx.width = width
+
+
+def _backward_pose(ctx, go, g_jac, bwd=None):
+    """Autograd formula of fused_raymarch_pose; `bwd` as in _backward."""
+    if go is None:
+        return (None,) * 13
+    w2v, rt_inv, src, tgt, step, jac = ctx.saved_tensors
+    if ctx.needs_input_grad[0] or ctx.needs_input_grad[2]:
+        raise RuntimeError("fused_raymarch_pose gives no volume or world_to_voxel gradient")
+    if jac.numel() == 0:
+        raise RuntimeError("fused_raymarch_pose ran with jac_planes=0: it has no gradient")
+    need_gcam = any(ctx.needs_input_grad[i] for i in (4, 5, 6))
+    g_rt, gsrc, gtgt, gstep = (bwd or fused_raymarch_pose_bwd)(go, jac, w2v, rt_inv, src, tgt, step, ctx.width,
+                                                               need_gcam)
+    return (
+        None, None, None,  # vol, lab, world_to_voxel
+        g_rt if ctx.needs_input_grad[3] else None,
+        _reduce_to(gsrc, src) if need_gcam else None,
+        _reduce_to(gtgt, tgt) if need_gcam else None,
+        _reduce_to(gstep, step) if need_gcam else None,
+        None, None, None, None, None, None,
+    )
+
+
+torch.library.register_autograd("nanodrr_amd::fused_raymarch_pose", _backward_pose, setup_context=_setup_context_pose)
 
 
 def _make_rays_impl(k_inv: torch.Tensor, sdd: torch.Tensor, height: int, width: int, orthographic: bool,
@@ -651,6 +811,9 @@ eager = SimpleNamespace(
                           lambda ctx, go: _backward(ctx, go, _fused_raymarch_bwd_impl)),
     fused_raymarch_jac=_eager("fused_raymarch_jac", fused_raymarch_jac, _fused_raymarch_jac_impl, _setup_context_jac,
                               lambda ctx, go, g_jac: _backward_jac(ctx, go, g_jac, _fused_raymarch_bwd_jac_impl)),
+    fused_raymarch_pose=_eager("fused_raymarch_pose", fused_raymarch_pose, _fused_raymarch_pose_impl,
+                               _setup_context_pose,
+                               lambda ctx, go, g_jac: _backward_pose(ctx, go, g_jac, _fused_raymarch_pose_bwd_impl)),
     make_rays=_eager("make_rays", make_rays, _make_rays_impl),
     camera_to_voxel=_eager("camera_to_voxel", camera_to_voxel, _camera_to_voxel_impl, _c2v_setup,
                            lambda ctx, gM: _c2v_backward(ctx, gM, _camera_to_voxel_bwd_impl)),

@@ -117,6 +117,8 @@ struct Params {
   const void* vol;
   const int8_t* lab;
   const float* M;
+  const double* w2v;  // when set, M is composed per workgroup: (w2v @ rt[b])[:3] in FP64
+  const float* rt;    // [B][4][4] camera -> world poses (rt_inv)
   const float* src;
   const float* tgt;
   const float* step;
@@ -155,6 +157,24 @@ __device__ inline int ray_index(const Params& p, int blk, int tr, bool& valid) {
   const int n = blk * kTileRays + tr;
   valid = n < p.N;
   return n;
+}
+
+// The camera -> voxel-index affine of pose b: p.M's row, or, when p.w2v is
+// set, (world_to_voxel @ rt_inv[b])[:3] composed in FP64 and rounded once to
+// fp32 (the sums of k_camera_to_voxel, in the same order) by threads 0..11
+// into sM. Every thread of the workgroup must call it (it holds a barrier).
+__device__ inline const float* pose_M(const Params& p, int b, float* sM) {
+  if (p.w2v == nullptr) return p.M + 12 * b;
+  if (threadIdx.x < 12) {
+    const int i = threadIdx.x / 4, j = threadIdx.x % 4;
+    const float* R = p.rt + 16 * static_cast<int64_t>(b);
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc = fma(p.w2v[4 * i + k], static_cast<double>(R[4 * k + j]), acc);
+    sM[threadIdx.x] = static_cast<float>(acc);
+  }
+  __syncthreads();
+  return sM;
 }
 
 // (pose, tile) of this workgroup. Workgroups are dispatched round-robin over
@@ -645,7 +665,8 @@ void k_fwd(Params p) {
   const int tr = threadIdx.x % kTileRays, chunk = threadIdx.x / kTileRays;
   bool valid;
   const int n = ray_index(p, tile, tr, valid);
-  const float* __restrict__ Mb = p.M + 12 * b;
+  __shared__ float sM[12];
+  const float* __restrict__ Mb = pose_M(p, b, sM);
   const Ray r = setup_ray(p, Mb, b, n, valid, chunk, NCH);
   const Src src = make_src(p);
 
@@ -766,7 +787,8 @@ void k_bwd(Params p) {
   const int tr = threadIdx.x % kTileRays, chunk = threadIdx.x / kTileRays;
   bool valid;
   const int n = ray_index(p, tile, tr, valid);
-  const float* __restrict__ Mb = p.M + 12 * b;
+  __shared__ float sM[12];
+  const float* __restrict__ Mb = pose_M(p, b, sM);
   const Ray r = setup_ray(p, Mb, b, n, valid, chunk, NCH);
   const Src src = make_src(p);
 
@@ -916,7 +938,8 @@ constexpr int kJacRays = NDRR_JAC_RAYS;
 template <int CV, bool GCAM>
 __global__ __launch_bounds__(kTileRays) void k_bwd_jac(Params p) {
   const int b = blockIdx.y, blk = blockIdx.x;
-  const float* __restrict__ Mb = p.M + 12 * b;
+  __shared__ float sM[12];
+  const float* __restrict__ Mb = GCAM ? pose_M(p, b, sM) : nullptr;
   const int64_t plane = static_cast<int64_t>(p.C) * p.N;
   float a[12];
 #pragma unroll
@@ -989,7 +1012,9 @@ __global__ __launch_bounds__(kTileRays) void k_bwd_jac(Params p) {
 // over the 256 thread sums in LDS. Deterministic.
 constexpr int kReduceThreads = 256;
 __global__ __launch_bounds__(kReduceThreads) void k_reduce_pose(const float* __restrict__ part,
-                                                               float* __restrict__ gM, int nparts) {
+                                                               float* __restrict__ gM, int nparts,
+                                                               const double* __restrict__ w2v,
+                                                               float* __restrict__ g_rt) {
   __shared__ float red[12][kReduceThreads];
   const int b = blockIdx.x, t = threadIdx.x;
   const float4* pb = reinterpret_cast<const float4*>(part + static_cast<int64_t>(b) * nparts * 12);
@@ -1012,7 +1037,15 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce_pose(const float* __r
     }
     __syncthreads();
   }
-  if (t < 12) gM[b * 12 + t] = red[t][0];
+  if (gM != nullptr && t < 12) gM[b * 12 + t] = red[t][0];
+  if (w2v != nullptr && t < 16) {
+    // the rt_inv gradient, W[:3]^T gM in FP64 from the rounded gM (k_camera_to_voxel_bwd's sums)
+    const int k = t / 4, j = t % 4;
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) acc = fma(w2v[4 * i + k], static_cast<double>(red[4 * i + j][0]), acc);
+    g_rt[b * 16 + t] = static_cast<float>(acc);
+  }
 }
 
 __global__ void k_labels_to_i8(const float* __restrict__ src, int8_t* __restrict__ dst, int64_t numel, int C) {
@@ -1279,13 +1312,16 @@ int ndrr_raymarch_fwd(const void* vol, int vol_dtype, int vol_layout, const int8
                                step_batch_stride, stream);
 }
 
-int ndrr_raymarch_fwd_jac(const void* vol, int vol_dtype, int vol_layout, const int8_t* lab, const float* M,
-                          const float* src, const float* tgt, const float* step, float* out, float* jac,
-                          int jac_planes, int B, int N, int D, int H, int W, int n_samples, int n_classes, int width,
-                          int64_t src_batch_stride, int64_t src_ray_stride, int64_t tgt_batch_stride,
-                          int64_t step_batch_stride, void* stream) {
-  const int rc = check_common(vol, vol_dtype, vol_layout, lab, M, src, tgt, step, B, N, D, H, W, n_samples,
-                              n_classes, width);
+// The forward with M given, or (w2v, rt) given and M composed per workgroup.
+static int raymarch_fwd_any(const void* vol, int vol_dtype, int vol_layout, const int8_t* lab, const float* M,
+                            const double* w2v, const float* rt, const float* src, const float* tgt,
+                            const float* step, float* out, float* jac, int jac_planes, int B, int N, int D, int H,
+                            int W, int n_samples, int n_classes, int width, int64_t src_batch_stride,
+                            int64_t src_ray_stride, int64_t tgt_batch_stride, int64_t step_batch_stride,
+                            void* stream) {
+  if (w2v != nullptr && (rt == nullptr || M != nullptr)) return NDRR_EINVAL;
+  const int rc = check_common(vol, vol_dtype, vol_layout, lab, w2v ? rt : M, src, tgt, step, B, N, D, H, W,
+                              n_samples, n_classes, width);
   if (rc != NDRR_OK) return rc;
   if (!out) return NDRR_EINVAL;
   if (jac && jac_planes != 6 && jac_planes != 7) return NDRR_EINVAL;
@@ -1294,6 +1330,8 @@ int ndrr_raymarch_fwd_jac(const void* vol, int vol_dtype, int vol_layout, const 
                          src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride);
   p.vol_bytes = static_cast<uint32_t>(packed_volume_bytes(D, H, W, vol_dtype, vol_layout));
   p.lab_bytes = n_classes > 1 ? static_cast<uint32_t>(packed_label_bytes(D, H, W, vol_layout)) : 0u;
+  p.w2v = w2v;
+  p.rt = rt;
   p.out = out;
   p.jac = jac;
   p.jac_q = jac ? jac_planes : 0;
@@ -1305,6 +1343,28 @@ int ndrr_raymarch_fwd_jac(const void* vol, int vol_dtype, int vol_layout, const 
     default: launch_fwd1<NDRR_F16>(p, vol_layout, grid, st); break;
   }
   return launch_status();
+}
+
+int ndrr_raymarch_fwd_jac(const void* vol, int vol_dtype, int vol_layout, const int8_t* lab, const float* M,
+                          const float* src, const float* tgt, const float* step, float* out, float* jac,
+                          int jac_planes, int B, int N, int D, int H, int W, int n_samples, int n_classes, int width,
+                          int64_t src_batch_stride, int64_t src_ray_stride, int64_t tgt_batch_stride,
+                          int64_t step_batch_stride, void* stream) {
+  return raymarch_fwd_any(vol, vol_dtype, vol_layout, lab, M, nullptr, nullptr, src, tgt, step, out, jac,
+                          jac_planes, B, N, D, H, W, n_samples, n_classes, width, src_batch_stride, src_ray_stride,
+                          tgt_batch_stride, step_batch_stride, stream);
+}
+
+int ndrr_raymarch_fwd_jac_pose(const void* vol, int vol_dtype, int vol_layout, const int8_t* lab,
+                               const double* world_to_voxel, const float* rt_inv, const float* src,
+                               const float* tgt, const float* step, float* out, float* jac, int jac_planes, int B,
+                               int N, int D, int H, int W, int n_samples, int n_classes, int width,
+                               int64_t src_batch_stride, int64_t src_ray_stride, int64_t tgt_batch_stride,
+                               int64_t step_batch_stride, void* stream) {
+  if (world_to_voxel == nullptr) return NDRR_EINVAL;
+  return raymarch_fwd_any(vol, vol_dtype, vol_layout, lab, nullptr, world_to_voxel, rt_inv, src, tgt, step, out,
+                          jac, jac_planes, B, N, D, H, W, n_samples, n_classes, width, src_batch_stride,
+                          src_ray_stride, tgt_batch_stride, step_batch_stride, stream);
 }
 
 void ndrr_launch_tile(int* tile_width, int* tile_height, int* tile_rays) {
@@ -1351,16 +1411,19 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_l
                              : vol_layout == NDRR_LAYOUT_QUAD  ? bwd_parts_per_tile(NDRR_LAYOUT_QUAD)
                                                                : bwd_parts_per_tile(NDRR_LAYOUT_LINEAR);
   hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(kReduceThreads), 0, st, workspace, gM,
-                     p.nblocks * parts_per_tile);
+                     p.nblocks * parts_per_tile, nullptr, nullptr);
   return launch_status();
 }
 
-int ndrr_raymarch_bwd_jac(const float* go, int64_t go_batch_stride, const float* jac, int jac_planes, const float* M,
-                          const float* src, const float* tgt, const float* step, float* workspace, float* gM,
-                          float* gsrc, float* gtgt, float* gstep, int need_gcam, int B, int N, int n_classes,
-                          int width, int64_t src_batch_stride, int64_t src_ray_stride, int64_t tgt_batch_stride,
-                          int64_t step_batch_stride, void* stream) {
-  if (!go || !jac || !M || !src || !tgt || !step || !workspace || !gM) return NDRR_EINVAL;
+// The contraction with M (and gM out) given, or (w2v, rt) given and the rt gradient out.
+static int raymarch_bwd_jac_any(const float* go, int64_t go_batch_stride, const float* jac, int jac_planes,
+                                const float* M, const double* w2v, const float* rt, const float* src,
+                                const float* tgt, const float* step, float* workspace, float* gM, float* g_rt,
+                                float* gsrc, float* gtgt, float* gstep, int need_gcam, int B, int N, int n_classes,
+                                int width, int64_t src_batch_stride, int64_t src_ray_stride,
+                                int64_t tgt_batch_stride, int64_t step_batch_stride, void* stream) {
+  if (w2v != nullptr ? (!rt || !g_rt || M || gM) : (!M || !gM)) return NDRR_EINVAL;
+  if (!go || !jac || !src || !tgt || !step || !workspace) return NDRR_EINVAL;
   if (reinterpret_cast<uintptr_t>(workspace) % 16 != 0) return NDRR_EINVAL;
   if (B <= 0 || N <= 0 || width < 0 || (width > 0 && N % width != 0)) return NDRR_EINVAL;
   if (jac_planes != 6 && jac_planes != 7) return NDRR_EINVAL;
@@ -1371,7 +1434,8 @@ int ndrr_raymarch_bwd_jac(const float* go, int64_t go_batch_stride, const float*
       3 * static_cast<int64_t>(B) * N >= (int64_t{1} << 31))
     return NDRR_ERANGE;
   Params p{};
-  p.M = M; p.src = src; p.tgt = tgt; p.step = step; p.go = go; p.go_bs = go_batch_stride;
+  p.M = M; p.w2v = w2v; p.rt = rt;
+  p.src = src; p.tgt = tgt; p.step = step; p.go = go; p.go_bs = go_batch_stride;
   p.jac = const_cast<float*>(jac);
   p.jac_q = jac_planes;
   p.src_bs = src_batch_stride; p.src_rs = src_ray_stride; p.tgt_bs = tgt_batch_stride; p.step_bs = step_batch_stride;
@@ -1400,8 +1464,30 @@ int ndrr_raymarch_bwd_jac(const float* go, int64_t go_batch_stride, const float*
   }
   int r = launch_status();
   if (r != NDRR_OK) return r;
-  hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(kReduceThreads), 0, st, workspace, gM, nlin);
+  hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(kReduceThreads), 0, st, workspace, gM, nlin, w2v, g_rt);
   return launch_status();
+}
+
+int ndrr_raymarch_bwd_jac(const float* go, int64_t go_batch_stride, const float* jac, int jac_planes, const float* M,
+                          const float* src, const float* tgt, const float* step, float* workspace, float* gM,
+                          float* gsrc, float* gtgt, float* gstep, int need_gcam, int B, int N, int n_classes,
+                          int width, int64_t src_batch_stride, int64_t src_ray_stride, int64_t tgt_batch_stride,
+                          int64_t step_batch_stride, void* stream) {
+  return raymarch_bwd_jac_any(go, go_batch_stride, jac, jac_planes, M, nullptr, nullptr, src, tgt, step, workspace,
+                              gM, nullptr, gsrc, gtgt, gstep, need_gcam, B, N, n_classes, width, src_batch_stride,
+                              src_ray_stride, tgt_batch_stride, step_batch_stride, stream);
+}
+
+int ndrr_raymarch_bwd_jac_pose(const float* go, int64_t go_batch_stride, const float* jac, int jac_planes,
+                               const double* world_to_voxel, const float* rt_inv, const float* src,
+                               const float* tgt, const float* step, float* workspace, float* g_rt_inv, float* gsrc,
+                               float* gtgt, float* gstep, int need_gcam, int B, int N, int n_classes, int width,
+                               int64_t src_batch_stride, int64_t src_ray_stride, int64_t tgt_batch_stride,
+                               int64_t step_batch_stride, void* stream) {
+  if (world_to_voxel == nullptr) return NDRR_EINVAL;
+  return raymarch_bwd_jac_any(go, go_batch_stride, jac, jac_planes, nullptr, world_to_voxel, rt_inv, src, tgt, step,
+                              workspace, nullptr, g_rt_inv, gsrc, gtgt, gstep, need_gcam, B, N, n_classes, width,
+                              src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride, stream);
 }
 
 size_t ndrr_packed_bytes(int D, int H, int W, int vol_dtype, int vol_layout) {

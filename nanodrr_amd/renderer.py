@@ -122,8 +122,14 @@ def render_hip(subject, rt_inv, src, tgt, step_size, n_samples, height, width) -
     image = subject.image
     if not (rt_inv.is_cuda and image.is_cuda):
         raise ValueError("the hip backend renders on a GPU: move the subject and poses to a cuda device")
-    # Camera -> voxel-index affine, composed in float64 and rounded once.
+    if ADJOINT not in _ADJOINTS:
+        raise ValueError(f"ADJOINT must be one of {_ADJOINTS}, got {ADJOINT!r}")
+    # Camera -> voxel-index affine, composed in float64 and rounded once: by the
+    # march kernels themselves when no volume gradient is asked for and pose
+    # gradients (if any) take the single-pass adjoint, else by one launch ahead.
     if rt_inv.dtype == torch.float32 and rt_inv.dim() == 3 and rt_inv.shape[1:] == (4, 4):
+        if not image.requires_grad and (ADJOINT == "single_pass" or not torch.is_grad_enabled()):
+            return march_pose(subject, rt_inv, src, tgt, step_size, n_samples, height, width)
         M = eager.camera_to_voxel(subject.world_to_voxel_f64(), rt_inv.contiguous())
     else:
         M = (subject.world_to_voxel.to(torch.float64) @ rt_inv.to(torch.float64))[:, :3, :].to(torch.float32)
@@ -145,7 +151,8 @@ def volume_layout(image: torch.Tensor) -> str:
 
 def march_inputs(subject, M, src, tgt, step_size, n_samples, height, width):
     """Checked kernel arguments of `march`: ((vol, lab, M, src, tgt, step, n_samples, C, width),
-    packed volume or None, layout name, B, C)."""
+    packed volume or None, layout name, B, C). M is the (B|1, 3, 4) affine, or for `march_pose`
+    the (B|1, 4, 4) rt_inv; either is expanded to B poses."""
     image = subject.image
     if not image.is_cuda:
         raise ValueError("the hip backend renders on a GPU: move the subject and poses to a cuda device")
@@ -182,4 +189,17 @@ def march(subject, M, src, tgt, step_size, n_samples, height, width) -> torch.Te
         out, _ = eager.fused_raymarch_jac(*args, 7 if gcam else 6, packed, layout)
     else:
         out = eager.fused_raymarch(*args, packed, layout)
+    return out.reshape(B, C, height, width)
+
+
+def march_pose(subject, rt_inv, src, tgt, step_size, n_samples, height, width) -> torch.Tensor:
+    """`march` given the pose rt_inv (B|1, 4, 4) fp32: the kernels compose the camera -> voxel-index
+    affine from it and the subject's float64 world_to_voxel, and the backward (the single-pass
+    adjoint) gives the rt_inv gradient directly. No volume gradient."""
+    args, packed, layout, B, C = march_inputs(subject, rt_inv, src, tgt, step_size, n_samples, height, width)
+    vol, lab, rt, s, t, st, _, _, _ = args
+    gcam = any(x.requires_grad for x in (s, t, st))
+    planes = (7 if gcam else 6) if torch.is_grad_enabled() and (gcam or rt.requires_grad) else 0
+    out, _ = eager.fused_raymarch_pose(vol, lab, subject.world_to_voxel_f64(), rt, s, t, st, n_samples, C, width,
+                                       planes, packed, layout)
     return out.reshape(B, C, height, width)
