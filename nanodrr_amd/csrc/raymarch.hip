@@ -441,16 +441,24 @@ __device__ inline float f16_hi(uint32_t w) {
 
 // Whether cell (ix, iy, iz) exists in a packed grid (cells -1 .. dim-1);
 // every other cell has all eight corners outside the volume.
+// Bitwise rather than short-circuit conjunction, which the compiler turns
+// into a divergent branch.
 __device__ inline bool cell_exists(const Src& s, const Cell& c) {
-  return static_cast<unsigned>(c.ix + 1) <= static_cast<unsigned>(s.W) &&
-         static_cast<unsigned>(c.iy + 1) <= static_cast<unsigned>(s.H) &&
-         static_cast<unsigned>(c.iz + 1) <= static_cast<unsigned>(s.D);
+  return (static_cast<unsigned>(c.ix + 1) <= static_cast<unsigned>(s.W)) &
+         (static_cast<unsigned>(c.iy + 1) <= static_cast<unsigned>(s.H)) &
+         (static_cast<unsigned>(c.iz + 1) <= static_cast<unsigned>(s.D));
 }
 
-// Index of a cell's record in a grid of (D + 1) x (H + 1) x row records.
+// Index of a cell's record in a grid of (D + 1) x (H + 1) x row records, with
+// 24-bit multiplies (full rate; a 32-bit integer multiply is a quarter-rate
+// instruction). check_common keeps (D + 1)(H + 1) and the row below 2^24 for
+// the packed layouts, so the operands of an existing cell fit; a cell that
+// does not exist gets an offset past the buffer whatever this returns.
 __device__ inline uint32_t record_of(const Src& s, const Cell& c, int row) {
-  return (static_cast<uint32_t>(c.iz + 1) * static_cast<uint32_t>(s.H + 1) + static_cast<uint32_t>(c.iy + 1)) *
-             static_cast<uint32_t>(row) + static_cast<uint32_t>(c.ix + 1);
+  return __umul24(__umul24(static_cast<uint32_t>(c.iz + 1), static_cast<uint32_t>(s.H + 1)) +
+                      static_cast<uint32_t>(c.iy + 1),
+                  static_cast<uint32_t>(row)) +
+         static_cast<uint32_t>(c.ix + 1);
 }
 
 // Byte offset of a cell's record, or one past 2 GiB (so past the end of the
@@ -610,6 +618,12 @@ __device__ inline int quad_label(const Src& s, const Cell& c, float x, float y, 
   return static_cast<int8_t>(static_cast<uint8_t>(word >> shift));
 }
 
+// a += x rounded once (never contracted with the product that formed x).
+__device__ inline void add_rn(float& a, float x) {
+#pragma clang fp contract(off)
+  a = a + x;
+}
+
 // Neumaier-compensated accumulation: sum + comp carries ~2x fp32 precision.
 __device__ inline void comp_add(float& sum, float& comp, float x) {
 #pragma clang fp contract(off)
@@ -690,33 +704,29 @@ void k_fwd(Params p) {
       const Corners v = batch.f[u].corners();
       float dvx, dvy, dvz;
       const float val = live ? (JAC ? lerp_value_grad(v, c, dvx, dvy, dvz) : lerp_value(v, c)) : 0.0f;
-      if constexpr (CV == 1) {
-        if constexpr (KAHAN) comp_add(acc[0], comp[0], val); else acc[0] += val;
-      } else {
-        // Class routing by a 0/1 factor: fma(m, x, acc) is acc + x or acc
-        // (x finite), the same values as a select and an add.
-#pragma unroll
-        for (int cl = 0; cl < CV; ++cl) {
-          const float m = (batch.cls[u] == cl) ? 1.0f : 0.0f;
-          if constexpr (KAHAN) comp_add(acc[cl], comp[cl], m != 0.0f ? val : 0.0f);
-          else acc[cl] = fmaf(m, val, acc[cl]);
-        }
-      }
+      float tx = 0.0f, ty = 0.0f, tz = 0.0f;
       if constexpr (JAC) {
         dvx = live ? dvx : 0.0f;
         dvy = live ? dvy : 0.0f;
         dvz = live ? dvz : 0.0f;
         const float t = static_cast<float>(r.s_lo + k0 + u) * p.inv_s;
-        const float tx = dvx * t, ty = dvy * t, tz = dvz * t;
+        tx = dvx * t; ty = dvy * t; tz = dvz * t;
+      }
+      // Class routing by branch: a class's sums are updated under the mask of
+      // the lanes whose sample it is, so a wave pays for the classes present
+      // among its lanes at this sample (mostly one or two), not for all CV.
 #pragma unroll
-        for (int cl = 0; cl < CV; ++cl) {
-          const float m = (CV == 1 || batch.cls[u] == cl) ? 1.0f : 0.0f;
-          jg[6 * cl + 0] = fmaf(m, dvx, jg[6 * cl + 0]);
-          jg[6 * cl + 1] = fmaf(m, dvy, jg[6 * cl + 1]);
-          jg[6 * cl + 2] = fmaf(m, dvz, jg[6 * cl + 2]);
-          jg[6 * cl + 3] = fmaf(m, tx, jg[6 * cl + 3]);
-          jg[6 * cl + 4] = fmaf(m, ty, jg[6 * cl + 4]);
-          jg[6 * cl + 5] = fmaf(m, tz, jg[6 * cl + 5]);
+      for (int cl = 0; cl < CV; ++cl) {
+        if (CV == 1 || batch.cls[u] == cl) {
+          if constexpr (KAHAN) comp_add(acc[cl], comp[cl], val); else add_rn(acc[cl], val);
+          if constexpr (JAC) {
+            add_rn(jg[6 * cl + 0], dvx);
+            add_rn(jg[6 * cl + 1], dvy);
+            add_rn(jg[6 * cl + 2], dvz);
+            add_rn(jg[6 * cl + 3], tx);
+            add_rn(jg[6 * cl + 4], ty);
+            add_rn(jg[6 * cl + 5], tz);
+          }
         }
       }
     }
@@ -1181,6 +1191,11 @@ int64_t packed_label_bytes(int D, int H, int W, int layout) {
   return layout == NDRR_LAYOUT_QUAD ? quad_records(D, H, W) * 4 : static_cast<int64_t>(D) * H * W;
 }
 
+// The packed layouts' record index is formed with 24-bit multiplies (record_of).
+bool packed_index_fits(int D, int H, int W) {
+  return static_cast<int64_t>(D + 1) * (H + 1) < (int64_t{1} << 24) && W + 2 < (1 << 24);
+}
+
 bool valid_dtype(int dt) { return dt == NDRR_F32 || dt == NDRR_BF16 || dt == NDRR_F16; }
 
 int check_common(const void* vol, int vol_dtype, int vol_layout, const int8_t* lab, const float* M,
@@ -1196,7 +1211,8 @@ int check_common(const void* vol, int vol_dtype, int vol_layout, const int8_t* l
   if (C > 1 && !lab) return NDRR_EINVAL;
   if (B > 65535) return NDRR_ERANGE;
   if (vol_layout != NDRR_LAYOUT_LINEAR && (packed_volume_bytes(D, H, W, vol_dtype, vol_layout) >= (int64_t{1} << 31) ||
-                                           packed_label_bytes(D, H, W, vol_layout) >= (int64_t{1} << 31)))
+                                           packed_label_bytes(D, H, W, vol_layout) >= (int64_t{1} << 31) ||
+                                           !packed_index_fits(D, H, W)))
     return NDRR_ERANGE;
   const int64_t vox = static_cast<int64_t>(D) * H * W;
   if (vox >= (int64_t{1} << 31) || stencil_records(D, H, W) >= (int64_t{1} << 31) ||

@@ -140,13 +140,19 @@ def volume_layout(image: torch.Tensor) -> str:
     """The layout the kernels read for this volume under VOLUME_LAYOUT."""
     layout = VOLUME_LAYOUT
     if layout == "auto":
-        if image.requires_grad:
+        if image.requires_grad or not packed_index_fits(*image.shape[-3:]):
             layout = "linear"
         else:
             layout = "quad"
     if layout not in _LAYOUTS:
         raise ValueError(f"VOLUME_LAYOUT must be 'auto' or one of {_LAYOUTS}, got {VOLUME_LAYOUT!r}")
     return layout
+
+
+def packed_index_fits(D: int, H: int, W: int) -> bool:
+    """Whether the packed layouts can address this (D, H, W) volume: their record index is formed
+    with 24-bit multiplies, so (D + 1)(H + 1) and W + 2 stay below 2^24."""
+    return (D + 1) * (H + 1) < 2**24 and W + 2 < 2**24
 
 
 def march_inputs(subject, M, src, tgt, step_size, n_samples, height, width):

@@ -78,6 +78,18 @@ def test_packed_layouts_limited_to_2gib():
     assert _fwd(lib, layout=1, dt=0, D=512, H=512, W=400) == -4  # fp32 stencil: 3.4 GB
 
 
+def test_packed_layouts_need_24_bit_record_index():
+    """The packed record index is formed with 24-bit multiplies: (D + 1)(H + 1) < 2^24."""
+    from nanodrr_amd.renderer import packed_index_fits
+
+    lib = _lib.lib()
+    assert _fwd(lib, layout=2, dt=1, D=4096, H=4096, W=2, C=1) == -4  # 402 MB of quads, plane too large
+    assert _fwd(lib, layout=1, dt=1, D=4096, H=4096, W=2, C=1) == -4
+    assert _fwd(lib, layout=0, dt=1, D=4096, H=4096, W=2, C=1, vol=None) == -1  # linear: only the null volume
+    assert packed_index_fits(512, 512, 512) and packed_index_fits(4094, 4095, 100)
+    assert not packed_index_fits(4095, 4096, 2) and not packed_index_fits(1, 1, 2**24 - 2)
+
+
 def _fwd(lib, **kw):
     args = dict(vol=1, dt=0, layout=0, lab=None, M=1, src=1, tgt=1, step=1, out=1, B=1, N=16, D=4, H=4, W=4, S=8, C=1,
                 width=4, sbs=0, srs=0, tbs=0, pbs=0, stream=None)
