@@ -542,10 +542,11 @@ template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_QUAD> {  // fp32: two 16-byte rec
   }
 };
 
-// a (1 - f) + b f with the rounding fixed by an explicit fma (b f + a (1 - f)),
-// so every kernel that inlines it rounds the same way whatever else the
-// compiler contracts around it.
-__device__ inline float lerp1(float a, float b, float f) { return fmaf(b, f, a * (1.0f - f)); }
+// a + f (b - a), rounded by one fma after the difference, so every kernel
+// that inlines it rounds the same way whatever else the compiler contracts
+// around it. The difference b - a is also the edge derivative the gradient
+// needs: value and gradient share it (lerp_value_grad).
+__device__ inline float lerp1(float a, float b, float f) { return fmaf(f, b - a, a); }
 
 __device__ inline float lerp_value(const Corners& v, const Cell& c) {
   const float c00 = lerp1(v.v000, v.v001, c.fx);
@@ -570,22 +571,21 @@ __device__ inline void lerp_grad(const Corners& v, const Cell& c, float& dvx, fl
 }
 
 // Value and gradient together, the gradient from the interpolation's own
-// intermediates (the x-lerped edges c_ab and y-lerped faces c_a):
-//   d/dz = c_1 - c_0,  d/dy = lerp_z(c_01 - c_00, c_11 - c_10),
-//   d/dx = lerp_z(lerp_y(v_001 - v_000, v_011 - v_010), lerp_y(v_101 - v_100, v_111 - v_110)),
-// the same derivatives as lerp_grad in about half the instructions. The value
-// is lerp_value's, bit for bit.
+// differences (the x edges d_ab = v_ab1 - v_ab0, the y differences
+// e_a = c_a1 - c_a0 of the x-lerped edges c_ab, and c_1 - c_0):
+//   d/dz = c_1 - c_0,  d/dy = lerp_z(e_0, e_1),  d/dx = lerp_z(lerp_y(d_00, d_01), lerp_y(d_10, d_11)),
+// the same derivatives as lerp_grad in about a third of the instructions. The
+// value is lerp_value's, bit for bit.
 __device__ inline float lerp_value_grad(const Corners& v, const Cell& c, float& dvx, float& dvy, float& dvz) {
-  const float c00 = lerp1(v.v000, v.v001, c.fx);
-  const float c01 = lerp1(v.v010, v.v011, c.fx);
-  const float c10 = lerp1(v.v100, v.v101, c.fx);
-  const float c11 = lerp1(v.v110, v.v111, c.fx);
-  const float c0 = lerp1(c00, c01, c.fy);
-  const float c1 = lerp1(c10, c11, c.fy);
+  const float d00 = v.v001 - v.v000, d01 = v.v011 - v.v010, d10 = v.v101 - v.v100, d11 = v.v111 - v.v110;
+  const float c00 = fmaf(c.fx, d00, v.v000), c01 = fmaf(c.fx, d01, v.v010);
+  const float c10 = fmaf(c.fx, d10, v.v100), c11 = fmaf(c.fx, d11, v.v110);
+  const float e0 = c01 - c00, e1 = c11 - c10;
+  const float c0 = fmaf(c.fy, e0, c00), c1 = fmaf(c.fy, e1, c10);
   dvz = c1 - c0;
-  dvy = lerp1(c01 - c00, c11 - c10, c.fz);
-  dvx = lerp1(lerp1(v.v001 - v.v000, v.v011 - v.v010, c.fy), lerp1(v.v101 - v.v100, v.v111 - v.v110, c.fy), c.fz);
-  return lerp1(c0, c1, c.fz);
+  dvy = lerp1(e0, e1, c.fz);
+  dvx = lerp1(lerp1(d00, d01, c.fy), lerp1(d10, d11, c.fy), c.fz);
+  return fmaf(c.fz, dvz, c0);
 }
 
 // Nearest label (round half to even, zero padded: outside the volume is class 0).
