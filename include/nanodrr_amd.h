@@ -146,7 +146,11 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype,
  * same march, jac[B][jac_planes][C][N] (fp32): per ray and class c the sums
  * over the samples routed to c of grad v (planes 0-2, voxel units), of
  * t * grad v (planes 3-5) and, with jac_planes == 7, of v (plane 6).
- * jac == NULL makes it ndrr_raymarch_fwd.
+ * jac == NULL makes it ndrr_raymarch_fwd. lattice_offset, if not NULL, is
+ * [B] fp32: pose b's samples sit at t = (s + lattice_offset[b]) / (S - 1)
+ * along src -> tgt instead of s / (S - 1) (the per-problem lattice jitter of
+ * registration, without shifted copies of the rays); the step is unchanged
+ * and the t sums use the shifted t, so ndrr_raymarch_bwd_jac needs no offset.
  *
  * ndrr_raymarch_bwd_jac then gives gM (and gsrc, gtgt, gstep when
  * need_gcam != 0, which needs jac_planes == 7) for an upstream gradient
@@ -157,7 +161,8 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype,
  */
 int ndrr_raymarch_fwd_jac(const void* vol, int vol_dtype, int vol_layout,
                           const int8_t* lab, const float* M, const float* src,
-                          const float* tgt, const float* step, float* out,
+                          const float* tgt, const float* step,
+                          const float* lattice_offset, float* out,
                           float* jac, int jac_planes,
                           int B, int N, int D, int H, int W,
                           int n_samples, int n_classes, int width,
@@ -190,7 +195,8 @@ int ndrr_raymarch_bwd_jac(const float* go, int64_t go_batch_stride,
 int ndrr_raymarch_fwd_jac_pose(const void* vol, int vol_dtype, int vol_layout,
                                const int8_t* lab, const double* world_to_voxel,
                                const float* rt_inv, const float* src,
-                               const float* tgt, const float* step, float* out,
+                               const float* tgt, const float* step,
+                               const float* lattice_offset, float* out,
                                float* jac, int jac_planes, int B, int N, int D,
                                int H, int W, int n_samples, int n_classes, int width,
                                int64_t src_batch_stride, int64_t src_ray_stride,

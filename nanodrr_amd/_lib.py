@@ -31,7 +31,7 @@ SIGNATURES: dict[str, tuple] = {
     ),
     "ndrr_raymarch_fwd_jac": (
         _c_int,
-        [_c_p, _c_int, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_int,
+        [_c_p, _c_int, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_int,
          _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
          _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
     ),
@@ -44,7 +44,7 @@ SIGNATURES: dict[str, tuple] = {
     # the same with (world_to_voxel, rt_inv) in place of M (and the rt_inv gradient in place of gM)
     "ndrr_raymarch_fwd_jac_pose": (
         _c_int,
-        [_c_p, _c_int, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_int,
+        [_c_p, _c_int, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_int,
          _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
          _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
     ),

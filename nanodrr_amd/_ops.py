@@ -339,23 +339,29 @@ def _check_pose(w2v: torch.Tensor, rt_inv: torch.Tensor, vol: torch.Tensor) -> N
         raise ValueError(f"rt_inv must be a contiguous float32 (B, 4, 4) tensor on {vol.device}")
 
 
-def _jac_forward(vol, lab, M, pose, src, tgt, step, n_samples, n_classes, width, jac_planes, packed, layout):
+def _jac_forward(vol, lab, M, pose, src, tgt, step, n_samples, n_classes, width, jac_planes, packed, layout,
+                 lattice=None):
     """The single-pass forward with M (B, 3, 4) given, or pose = (world_to_voxel, rt_inv) and M
-    composed by the kernels; jac_planes 0 (pose form only) skips the sums."""
-    if jac_planes not in ((0, 6, 7) if pose is not None else (6, 7)):
-        raise ValueError(f"jac_planes must be 6 or 7, got {jac_planes}")
+    composed by the kernels; jac_planes 0 (pose form only) skips the sums. lattice (B,) fp32, if
+    given, offsets pose b's sample lattice by lattice[b] steps along its rays."""
+    if jac_planes not in (0, 6, 7):
+        raise ValueError(f"jac_planes must be 0, 6 or 7, got {jac_planes}")
     if pose is not None:
         _check_pose(*pose, vol)
     B, N, D, H, W, strides = _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width,
                                        layout if packed is not None else "linear",
                                        B=None if pose is None else pose[1].shape[0])
+    if lattice is not None and (lattice.dtype != torch.float32 or lattice.shape != (B,)
+                                or not lattice.is_contiguous() or lattice.device != vol.device):
+        raise ValueError(f"lattice must be a contiguous float32 ({B},) tensor on {vol.device}")
     vptr, code = _volume_arg(vol, packed, layout)
     out = torch.empty(B, n_classes, N, device=vol.device, dtype=torch.float32)
     jac = (torch.empty(B, jac_planes, n_classes, N, device=vol.device, dtype=torch.float32) if jac_planes
            else out.new_empty(0))
     lib = _lib.lib()
     head = (vptr, _DTYPE_CODE[vol.dtype], code, _ptr(lab) if n_classes > 1 else None)
-    tail = (src.data_ptr(), tgt.data_ptr(), step.data_ptr(), out.data_ptr(), _ptr(jac), jac_planes, B, N, D, H, W,
+    tail = (src.data_ptr(), tgt.data_ptr(), step.data_ptr(), _ptr(lattice), out.data_ptr(), _ptr(jac), jac_planes, B,
+            N, D, H, W,
             n_samples, n_classes, width, *strides, _stream(vol.device))
     with torch.cuda.device(vol.device):
         if pose is None:
@@ -379,8 +385,10 @@ def _fused_raymarch_jac_impl(
     jac_planes: int,
     packed: Optional[torch.Tensor] = None,
     layout: str = "linear",
+    lattice: Optional[torch.Tensor] = None,
 ) -> tuple[torch.Tensor, torch.Tensor]:
-    return _jac_forward(vol, lab, M, None, src, tgt, step, n_samples, n_classes, width, jac_planes, packed, layout)
+    return _jac_forward(vol, lab, M, None, src, tgt, step, n_samples, n_classes, width, jac_planes, packed, layout,
+                        lattice)
 
 
 @torch.library.custom_op("nanodrr_amd::fused_raymarch_jac", mutates_args=())
@@ -397,17 +405,20 @@ def fused_raymarch_jac(
     jac_planes: int,
     packed: Optional[torch.Tensor] = None,
     layout: str = "linear",
+    lattice: Optional[torch.Tensor] = None,
 ) -> tuple[torch.Tensor, torch.Tensor]:
     """`fused_raymarch` plus, from the same march, the per-ray sums of the
     single-pass adjoint: jac (B, jac_planes, C, N) fp32 (ndrr_raymarch_fwd_jac).
     Its autograd backward reads jac instead of marching the volume again, so it
-    gives pose and camera gradients but no volume gradient."""
+    gives pose and camera gradients but no volume gradient. `lattice` (B,), if
+    given, slides pose b's samples by lattice[b] steps along its rays."""
     return _fused_raymarch_jac_impl(vol, lab, M, src, tgt, step, n_samples, n_classes, width, jac_planes, packed,
-                                    layout)
+                                    layout, lattice)
 
 
 @fused_raymarch_jac.register_fake
-def _(vol, lab, M, src, tgt, step, n_samples, n_classes, width, jac_planes, packed=None, layout="linear"):
+def _(vol, lab, M, src, tgt, step, n_samples, n_classes, width, jac_planes, packed=None, layout="linear",
+      lattice=None):
     B, N = M.shape[0], tgt.shape[1]
     return (torch.empty(B, n_classes, N, device=vol.device, dtype=torch.float32),
             torch.empty(B, jac_planes, n_classes, N, device=vol.device, dtype=torch.float32))
@@ -491,7 +502,7 @@ def _(go, jac, M, src, tgt, step, width, need_gcam):
 
 
 def _setup_context_jac(ctx, inputs, output):
-    vol, lab, M, src, tgt, step, n_samples, n_classes, width, jac_planes, packed, layout = inputs
+    vol, lab, M, src, tgt, step, n_samples, n_classes, width, jac_planes, packed, layout = inputs[:12]
     ctx.save_for_backward(M, src, tgt, step, output[1])
     ctx.mark_non_differentiable(output[1])
     ctx.set_materialize_grads(False)  # no zero-filled gradient for the (large) jac planes
@@ -501,10 +512,12 @@ def _setup_context_jac(ctx, inputs, output):
 def _backward_jac(ctx, go, g_jac, bwd=None):
     """Autograd formula of fused_raymarch_jac; `bwd` as in _backward."""
     if go is None:
-        return (None,) * 12
+        return (None,) * 13
     M, src, tgt, step, jac = ctx.saved_tensors
     if ctx.needs_input_grad[0]:
         raise RuntimeError("fused_raymarch_jac gives no volume gradient; use fused_raymarch for it")
+    if jac.numel() == 0:
+        raise RuntimeError("fused_raymarch_jac ran with jac_planes=0: it has no gradient")
     need_gcam = any(ctx.needs_input_grad[i] for i in (3, 4, 5))
     gM, gsrc, gtgt, gstep = (bwd or fused_raymarch_bwd_jac)(go, jac, M, src, tgt, step, ctx.width, need_gcam)
     return (
@@ -514,7 +527,7 @@ def _backward_jac(ctx, go, g_jac, bwd=None):
         _reduce_to(gsrc, src) if need_gcam else None,
         _reduce_to(gtgt, tgt) if need_gcam else None,
         _reduce_to(gstep, step) if need_gcam else None,
-        None, None, None, None, None, None,
+        None, None, None, None, None, None, None,
     )
 
 
@@ -535,9 +548,10 @@ def _fused_raymarch_pose_impl(
     jac_planes: int,
     packed: Optional[torch.Tensor] = None,
     layout: str = "linear",
+    lattice: Optional[torch.Tensor] = None,
 ) -> tuple[torch.Tensor, torch.Tensor]:
     return _jac_forward(vol, lab, None, (world_to_voxel, rt_inv), src, tgt, step, n_samples, n_classes, width,
-                        jac_planes, packed, layout)
+                        jac_planes, packed, layout, lattice)
 
 
 @torch.library.custom_op("nanodrr_amd::fused_raymarch_pose", mutates_args=())
@@ -555,18 +569,19 @@ def fused_raymarch_pose(
     jac_planes: int,
     packed: Optional[torch.Tensor] = None,
     layout: str = "linear",
+    lattice: Optional[torch.Tensor] = None,
 ) -> tuple[torch.Tensor, torch.Tensor]:
     """`fused_raymarch_jac` taking the pose itself: M = (world_to_voxel @ rt_inv)[:, :3] is composed
     in FP64 by each workgroup (ndrr_raymarch_fwd_jac_pose), bitwise equal to camera_to_voxel then
     fused_raymarch_jac. jac_planes 0 returns an empty jac (no gradient); the backward gives the
     rt_inv gradient (B, 4, 4) and the camera gradients, no volume gradient."""
     return _fused_raymarch_pose_impl(vol, lab, world_to_voxel, rt_inv, src, tgt, step, n_samples, n_classes, width,
-                                     jac_planes, packed, layout)
+                                     jac_planes, packed, layout, lattice)
 
 
 @fused_raymarch_pose.register_fake
 def _(vol, lab, world_to_voxel, rt_inv, src, tgt, step, n_samples, n_classes, width, jac_planes, packed=None,
-      layout="linear"):
+      layout="linear", lattice=None):
     B, N = rt_inv.shape[0], tgt.shape[1]
     out = torch.empty(B, n_classes, N, device=vol.device, dtype=torch.float32)
     return out, (torch.empty(B, jac_planes, n_classes, N, device=vol.device, dtype=torch.float32) if jac_planes
@@ -615,7 +630,7 @@ def _(go, jac, world_to_voxel, rt_inv, src, tgt, step, width, need_gcam):
 
 
 def _setup_context_pose(ctx, inputs, output):
-    vol, lab, w2v, rt_inv, src, tgt, step, n_samples, n_classes, width, jac_planes, packed, layout = inputs
+    vol, lab, w2v, rt_inv, src, tgt, step, n_samples, n_classes, width, jac_planes, packed, layout = inputs[:13]
     ctx.save_for_backward(w2v, rt_inv, src, tgt, step, output[1])
     ctx.mark_non_differentiable(output[1])
     ctx.set_materialize_grads(False)
@@ -625,7 +640,7 @@ def _setup_context_pose(ctx, inputs, output):
 def _backward_pose(ctx, go, g_jac, bwd=None):
     """Autograd formula of fused_raymarch_pose; `bwd` as in _backward."""
     if go is None:
-        return (None,) * 13
+        return (None,) * 14
     w2v, rt_inv, src, tgt, step, jac = ctx.saved_tensors
     if ctx.needs_input_grad[0] or ctx.needs_input_grad[2]:
         raise RuntimeError("fused_raymarch_pose gives no volume or world_to_voxel gradient")
@@ -640,7 +655,7 @@ def _backward_pose(ctx, go, g_jac, bwd=None):
         _reduce_to(gsrc, src) if need_gcam else None,
         _reduce_to(gtgt, tgt) if need_gcam else None,
         _reduce_to(gstep, step) if need_gcam else None,
-        None, None, None, None, None, None,
+        None, None, None, None, None, None, None,
     )
 
 

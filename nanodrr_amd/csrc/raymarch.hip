@@ -122,6 +122,7 @@ struct Params {
   const float* src;
   const float* tgt;
   const float* step;
+  const float* lattice;  // [B] per-pose sample-lattice offset in steps: sample s sits at t = (s + offset) / (S - 1)
   const float* go;
   float* out;
   float* gpart;  // [B][nblocks][parts per tile][12] pose-gradient partials
@@ -246,6 +247,7 @@ struct Ray {
   DF ax, ay, az;        // voxel-space position of the anchor: the ray's first clipped sample
   DF dx, dy, dz;        // voxel-space step per sample
   float step;
+  float t_off;          // the lattice offset in ray-parameter units: t(s) = s / (S - 1) + t_off
   int s_lo, n_s;        // this chunk's samples [s_lo, s_lo + n_s) can touch the volume
   int k_base;           // s_lo minus the anchor's sample index
 };
@@ -277,6 +279,7 @@ __device__ inline Ray setup_ray(const Params& p, const float* __restrict__ Mb, i
   const float* tp = p.tgt + b * p.tgt_bs + 3 * static_cast<int64_t>(nn);
   r.sx = sp[0]; r.sy = sp[1]; r.sz = sp[2];
   r.step = p.step[b * p.step_bs + nn];
+  const float off = p.lattice != nullptr ? p.lattice[b] : 0.0f;
   DF p0[3], d[3];
   if (r.sx == 0.0f && r.sy == 0.0f && r.sz == 0.0f) {
     // A source at the camera origin (every perspective render): p0 is M's
@@ -311,8 +314,8 @@ __device__ inline Ray setup_ray(const Params& p, const float* __restrict__ Mb, i
     // The slab is widened by kMargin voxels, far more than the rounding of
     // t_lo, t_hi: every sample whose footprint reaches the volume is kept; the
     // few extra ones add exact zeros.
-    const float f_lo = fmaxf(floorf(t_lo * p.s_m1), 0.0f);
-    const float f_hi = fminf(ceilf(t_hi * p.s_m1) + 1.0f, static_cast<float>(p.S));
+    const float f_lo = fmaxf(floorf(t_lo * p.s_m1 - off), 0.0f);
+    const float f_hi = fminf(ceilf(t_hi * p.s_m1 - off) + 1.0f, static_cast<float>(p.S));
     if (f_lo < f_hi) {
       s_lo = static_cast<int>(f_lo);
       n_all = static_cast<int>(f_hi) - s_lo;
@@ -331,7 +334,9 @@ __device__ inline Ray setup_ray(const Params& p, const float* __restrict__ Mb, i
     const DF delta = df_mul_df(d[i], inv);
     *st[i] = delta;
     *an[i] = df_add(p0[i], df_mul(delta, s0));
+    if (p.lattice != nullptr) *an[i] = df_add(*an[i], df_mul(delta, off));
   }
+  r.t_off = off * p.inv_s;
   return r;
 }
 
@@ -709,7 +714,7 @@ void k_fwd(Params p) {
         dvx = live ? dvx : 0.0f;
         dvy = live ? dvy : 0.0f;
         dvz = live ? dvz : 0.0f;
-        const float t = static_cast<float>(r.s_lo + k0 + u) * p.inv_s;
+        const float t = fmaf(static_cast<float>(r.s_lo + k0 + u), p.inv_s, r.t_off);
         tx = dvx * t; ty = dvy * t; tz = dvz * t;
       }
       // Class routing by branch: a class's sums are updated under the mask of
@@ -836,7 +841,7 @@ void k_bwd(Params p) {
       float dvx, dvy, dvz;
       lerp_grad(v, c, dvx, dvy, dvz);
       const float gx = g * dvx, gy = g * dvy, gz = g * dvz;
-      const float t = static_cast<float>(r.s_lo + k0 + u) * p.inv_s;
+      const float t = fmaf(static_cast<float>(r.s_lo + k0 + u), p.inv_s, r.t_off);
       Gx += gx; Gy += gy; Gz += gz;
       Tx += gx * t; Ty += gy * t; Tz += gz * t;
       if constexpr (GCAM) astep += gg * lerp_value(v, c);
@@ -1323,7 +1328,8 @@ int ndrr_raymarch_fwd(const void* vol, int vol_dtype, int vol_layout, const int8
                       int H, int W, int n_samples, int n_classes, int width, int64_t src_batch_stride,
                       int64_t src_ray_stride, int64_t tgt_batch_stride, int64_t step_batch_stride,
                       void* stream) {
-  return ndrr_raymarch_fwd_jac(vol, vol_dtype, vol_layout, lab, M, src, tgt, step, out, nullptr, 0, B, N, D, H, W,
+  return ndrr_raymarch_fwd_jac(vol, vol_dtype, vol_layout, lab, M, src, tgt, step, nullptr, out, nullptr, 0, B, N, D,
+                               H, W,
                                n_samples, n_classes, width, src_batch_stride, src_ray_stride, tgt_batch_stride,
                                step_batch_stride, stream);
 }
@@ -1331,7 +1337,8 @@ int ndrr_raymarch_fwd(const void* vol, int vol_dtype, int vol_layout, const int8
 // The forward with M given, or (w2v, rt) given and M composed per workgroup.
 static int raymarch_fwd_any(const void* vol, int vol_dtype, int vol_layout, const int8_t* lab, const float* M,
                             const double* w2v, const float* rt, const float* src, const float* tgt,
-                            const float* step, float* out, float* jac, int jac_planes, int B, int N, int D, int H,
+                            const float* step, const float* lattice, float* out, float* jac, int jac_planes, int B,
+                            int N, int D, int H,
                             int W, int n_samples, int n_classes, int width, int64_t src_batch_stride,
                             int64_t src_ray_stride, int64_t tgt_batch_stride, int64_t step_batch_stride,
                             void* stream) {
@@ -1348,6 +1355,7 @@ static int raymarch_fwd_any(const void* vol, int vol_dtype, int vol_layout, cons
   p.lab_bytes = n_classes > 1 ? static_cast<uint32_t>(packed_label_bytes(D, H, W, vol_layout)) : 0u;
   p.w2v = w2v;
   p.rt = rt;
+  p.lattice = lattice;
   p.out = out;
   p.jac = jac;
   p.jac_q = jac ? jac_planes : 0;
@@ -1362,25 +1370,25 @@ static int raymarch_fwd_any(const void* vol, int vol_dtype, int vol_layout, cons
 }
 
 int ndrr_raymarch_fwd_jac(const void* vol, int vol_dtype, int vol_layout, const int8_t* lab, const float* M,
-                          const float* src, const float* tgt, const float* step, float* out, float* jac,
-                          int jac_planes, int B, int N, int D, int H, int W, int n_samples, int n_classes, int width,
-                          int64_t src_batch_stride, int64_t src_ray_stride, int64_t tgt_batch_stride,
-                          int64_t step_batch_stride, void* stream) {
-  return raymarch_fwd_any(vol, vol_dtype, vol_layout, lab, M, nullptr, nullptr, src, tgt, step, out, jac,
-                          jac_planes, B, N, D, H, W, n_samples, n_classes, width, src_batch_stride, src_ray_stride,
-                          tgt_batch_stride, step_batch_stride, stream);
+                          const float* src, const float* tgt, const float* step, const float* lattice_offset,
+                          float* out, float* jac, int jac_planes, int B, int N, int D, int H, int W, int n_samples,
+                          int n_classes, int width, int64_t src_batch_stride, int64_t src_ray_stride,
+                          int64_t tgt_batch_stride, int64_t step_batch_stride, void* stream) {
+  return raymarch_fwd_any(vol, vol_dtype, vol_layout, lab, M, nullptr, nullptr, src, tgt, step, lattice_offset, out,
+                          jac, jac_planes, B, N, D, H, W, n_samples, n_classes, width, src_batch_stride,
+                          src_ray_stride, tgt_batch_stride, step_batch_stride, stream);
 }
 
 int ndrr_raymarch_fwd_jac_pose(const void* vol, int vol_dtype, int vol_layout, const int8_t* lab,
                                const double* world_to_voxel, const float* rt_inv, const float* src,
-                               const float* tgt, const float* step, float* out, float* jac, int jac_planes, int B,
-                               int N, int D, int H, int W, int n_samples, int n_classes, int width,
-                               int64_t src_batch_stride, int64_t src_ray_stride, int64_t tgt_batch_stride,
-                               int64_t step_batch_stride, void* stream) {
+                               const float* tgt, const float* step, const float* lattice_offset, float* out,
+                               float* jac, int jac_planes, int B, int N, int D, int H, int W, int n_samples,
+                               int n_classes, int width, int64_t src_batch_stride, int64_t src_ray_stride,
+                               int64_t tgt_batch_stride, int64_t step_batch_stride, void* stream) {
   if (world_to_voxel == nullptr) return NDRR_EINVAL;
-  return raymarch_fwd_any(vol, vol_dtype, vol_layout, lab, nullptr, world_to_voxel, rt_inv, src, tgt, step, out,
-                          jac, jac_planes, B, N, D, H, W, n_samples, n_classes, width, src_batch_stride,
-                          src_ray_stride, tgt_batch_stride, step_batch_stride, stream);
+  return raymarch_fwd_any(vol, vol_dtype, vol_layout, lab, nullptr, world_to_voxel, rt_inv, src, tgt, step,
+                          lattice_offset, out, jac, jac_planes, B, N, D, H, W, n_samples, n_classes, width,
+                          src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride, stream);
 }
 
 void ndrr_launch_tile(int* tile_width, int* tile_height, int* tile_rays) {

@@ -194,14 +194,10 @@ class DrrSource:
         S = n_samples or self.n_samples
         M = pose_compose(reg._prefix.expand(B, 4, 4).contiguous(), reg._suffix.expand(B, 4, 4).contiguous(),
                          rot.float().contiguous(), xyz.float().contiguous())
-        src, tgt = reg._src, reg._tgt
-        step, delta = reg._ray_steps(S)
-        if offset is not None:
-            u = offset.reshape(-1, 1, 1).to(tgt.dtype)
-            src, tgt = torch.addcmul(src, delta, u), torch.addcmul(tgt, delta, u)
+        step, _ = reg._ray_steps(S)
         from .renderer import march
 
-        return march(reg.subject, M, src, tgt, step, S, reg.height, reg.width)
+        return march(reg.subject, M, reg._src, reg._tgt, step, S, reg.height, reg.width, offset=offset)
 
 
 def sample_params(n: int, gen: torch.Generator, rot_deg: float, trans_mm: float, device):
@@ -276,7 +272,8 @@ def _settled(trace, j):
 
 def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_deg: float = 30.0,
                          trans_mm: float = 40.0, n_iters: int = 300, device="cuda:0", log=print,
-                         lr_rot: float = 1e-2, lr_xyz: float = 1.0, short_iters: int | None = None):
+                         lr_rot: float = 1e-2, lr_xyz: float = 1.0, short_iters: int | None = None,
+                         refine: int = 0):
     """On n_test unseen phantoms, register one target each (200 x 200, targets at
     +-rot_deg / +-trans_mm about the nominal pose, S = 1024 target DRR) from the
     nominal pose and from the CNN's proposal (its input: the target pose rendered at
@@ -285,13 +282,19 @@ def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_
     a problem has settled at the first check (every 5 iterations) from which its mTRE
     stays below 1 mm to the end. The CNN start is also registered alone with a whole
     schedule of `short_iters` iterations (default n_iters // 5); every run uses the
-    learning rates lr_rot (rad) and lr_xyz (mm)."""
+    learning rates lr_rot (rad) and lr_xyz (mm).
+
+    refine > 0 corrects the proposal p that many times by the network's own error at p:
+    with q the network's estimate from a DRR rendered at p, the next proposal is 2p - q
+    (rotation vector and translation), i.e. the target's error is taken to be the error at
+    p. Each correction renders one 128 x 128 DRR and is counted as a render iteration of
+    the CNN start."""
     short_iters = short_iters or max(n_iters // 5, 1)
     from .metrics import mtre, zncc
     from .phantom import detector, fiducials, make_phantom, nominal_pose, phantom_subject
     from .registration import Registration, register
 
-    res = {"nominal": [], "cnn": [], "proposal_mtre": [], "zncc": {"nominal": [], "cnn": []},
+    res = {"nominal": [], "cnn": [], "proposal_mtre": [], "raw_proposal_mtre": [], "zncc": {"nominal": [], "cnn": []},
            "settled": {"nominal": [], "cnn": []}, "cnn_fifth": []}
     check_every = 5
     k200, sdd = detector(1020.0, 1.0, 200, 200)
@@ -314,6 +317,13 @@ def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_
         src128.regs = [Registration(subject, rt_nom, k128.to(device), sdd128.to(device), 128, 128)]
         src128.n_samples = 512
         rot_p, xyz_p = predict(net, src128.render(0, rot_t, xyz_t))
+        raw = Registration(subject, rt_nom, k200, sdd, 200, 200, per_pose=True)
+        raw._rot.data.copy_(rot_p)
+        raw._xyz.data.copy_(xyz_p)
+        res["raw_proposal_mtre"].append(float(mtre(raw.pose.detach(), rt_true, fid)[0]))
+        for _ in range(refine):
+            rot_q, xyz_q = predict(net, src128.render(0, rot_p, xyz_p))
+            rot_p, xyz_p = 2 * rot_p - rot_q, 2 * xyz_p - xyz_q
         reg = Registration(subject, rt_nom.expand(2, 4, 4).contiguous(), k200, sdd, 200, 200, per_pose=True)
         reg._rot.data.copy_(torch.cat([torch.zeros_like(rot_p), rot_p]))
         reg._xyz.data.copy_(torch.cat([torch.zeros_like(xyz_p), xyz_p]))
@@ -347,8 +357,10 @@ def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_
     curves = {name: [sum(1 for x in res["settled"][name] if x is not None and x <= k)
                      for k in range(check_every, n_iters + 1, check_every)] for name in ("nominal", "cnn")}
     plain_final = curves["nominal"][-1]
-    # render iterations after which the CNN start has as many settled problems as the plain start has at the end
-    cnn_match = next(((i + 1) * check_every for i, c in enumerate(curves["cnn"]) if c >= plain_final), None)
+    # render iterations (the refinement renders included) after which the CNN start has as many
+    # settled problems as the plain start has at the end
+    cnn_match = next(((i + 1) * check_every + refine for i, c in enumerate(curves["cnn"]) if c >= plain_final),
+                     None)
 
     def med(xs):
         xs = [x for x in xs if x is not None]
@@ -365,6 +377,8 @@ def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_
         "cnn_over_plain_iteration_ratio": None if cnn_match is None else cnn_match / n_iters,
         "learning_rates": [lr_rot, lr_xyz],
         "short_iterations": short_iters,
+        "refine_renders": refine,
+        "render_iterations_of_the_short_cnn_run": short_iters + refine,
         "converged_from_cnn_with_the_short_schedule": int((t(res["cnn_fifth"]) < 1).sum()),
         "median_mtre_mm_cnn_with_the_short_schedule": float(t(res["cnn_fifth"]).median()),
         "converged_from_nominal": int((t(res["nominal"]) < 1).sum()),
@@ -372,6 +386,7 @@ def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_
         "median_mtre_mm_nominal": float(t(res["nominal"]).median()),
         "median_mtre_mm_cnn": float(t(res["cnn"]).median()),
         "median_mtre_mm_cnn_proposal": float(t(res["proposal_mtre"]).median()),
+        "median_mtre_mm_cnn_proposal_unrefined": float(t(res["raw_proposal_mtre"]).median()),
         "median_zncc_nominal": float(t(res["zncc"]["nominal"]).median()),
         "median_zncc_cnn": float(t(res["zncc"]["cnn"]).median()),
     }
@@ -394,6 +409,8 @@ def main(argv=None):
     ap.add_argument("--iters", type=int, default=300, help="registration iterations of the comparison")
     ap.add_argument("--short-iters", type=int, default=None, help="whole schedule of the short CNN-started run")
     ap.add_argument("--lr-scale", type=float, default=1.0, help="multiplies the 1e-2 rad / 1 mm learning rates")
+    ap.add_argument("--refine", type=int, default=0,
+                    help="corrections of the CNN proposal by the network's error at it (one 128 x 128 render each)")
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     if args.load:
@@ -409,7 +426,7 @@ def main(argv=None):
     if args.test > 0:
         summary["registration"] = compare_registration(net, n_test=args.test, log=log, n_iters=args.iters,
                                                        lr_rot=1e-2 * args.lr_scale, lr_xyz=1.0 * args.lr_scale,
-                                                       short_iters=args.short_iters)
+                                                       short_iters=args.short_iters, refine=args.refine)
     line = json.dumps(summary)
     print(line, flush=True)
     if args.out:

@@ -162,16 +162,14 @@ class Registration(torch.nn.Module):
 
     def forward(self, n_samples: int = 500, backend: str = "auto", offset: torch.Tensor | None = None) -> torch.Tensor:
         """DRRs of the current poses (registration.py:59-62). `offset` (B,), in units of the
-        sample step, slides every ray's sample lattice along the ray (src and tgt move
-        together, the step is unchanged); None renders exactly as `render`."""
+        sample step, slides each pose's sample lattice along its rays (sample s at
+        t = (s + offset) / (n_samples - 1), the step unchanged; the march kernels take it
+        directly); None renders exactly as `render`."""
         if backend not in ("auto", "hip", "triton"):
             raise ValueError(f"Unknown backend {backend!r}; expected 'auto', 'hip' or 'triton'")
-        src, tgt = self._src, self._tgt
-        step, delta = self._ray_steps(n_samples)
-        if offset is not None:
-            u = offset.reshape(-1, 1, 1).to(tgt.dtype)
-            src, tgt = torch.addcmul(src, delta, u), torch.addcmul(tgt, delta, u)
-        return march(self.subject, self.camera_to_voxel(), src, tgt, step, n_samples, self.height, self.width)
+        step, _ = self._ray_steps(n_samples)
+        return march(self.subject, self.camera_to_voxel(), self._src, self._tgt, step, n_samples, self.height,
+                     self.width, offset=offset)
 
     def _ray_steps(self, n_samples: int):
         """(step (1, N), per-sample ray vector (1, N, 3)) for n_samples, kept until the rays change."""
@@ -261,11 +259,8 @@ class _ExplicitLoop:
         self.rot_b = torch.empty(B, 3, device=dev) if self.shared else reg._rot.detach()
         self.xyz_b = torch.empty(B, 3, device=dev) if self.shared else reg._xyz.detach()
         self.M = torch.empty(B, 3, 4, device=dev)
-        self.step, self.delta = reg._ray_steps(n_samples)
+        self.step, _ = reg._ray_steps(n_samples)
         src, tgt = reg._src, reg._tgt
-        if jitter:  # the shifted rays are per pose and ray
-            src = torch.empty(B, tgt.shape[1], 3, device=dev)
-            tgt = torch.empty(B, tgt.shape[1], 3, device=dev)
         self.src, self.tgt = src, tgt
         args, packed, layout, _, C = march_inputs(reg.subject, self.M, src, tgt, self.step, n_samples, reg.height,
                                                   reg.width)
@@ -294,10 +289,9 @@ class _ExplicitLoop:
         """One iteration: returns (score (B,), d(-sum score)/d rot, d(-sum score)/d xyz)."""
         reg, lib, st = self.reg, self.lib, self.stream
         B, N, D, H, W, S, C, width = self.dims
-        if offset is not None:
-            u = offset.reshape(-1, 1, 1).to(self.tgt.dtype)
-            torch.addcmul(reg._src, self.delta, u, out=self.src)
-            torch.addcmul(reg._tgt, self.delta, u, out=self.tgt)
+        if offset is not None and (offset.dtype != torch.float32 or offset.shape != (B,) or not offset.is_contiguous()):
+            raise ValueError(f"offset must be a contiguous float32 ({B},) tensor")
+        lattice = None if offset is None else offset.data_ptr()
         if self.shared:  # the expanded parameters, as _params() hands them to the pose step
             self.rot_b.copy_(reg._rot.detach().expand(B, 3))
             self.xyz_b.copy_(reg._xyz.detach().expand(B, 3))
@@ -305,7 +299,7 @@ class _ExplicitLoop:
         rot, xyz, M = self.rot_b.data_ptr(), self.xyz_b.data_ptr(), self.M.data_ptr()
         src, tgt, step = self.src.data_ptr(), self.tgt.data_ptr(), self.step.data_ptr()
         _lib.check(lib.ndrr_pose_compose(pre, suf, rot, xyz, 3, M, B, st), "ndrr_pose_compose")
-        _lib.check(lib.ndrr_raymarch_fwd_jac(self.vptr, self.dt, self.code, self.lab, M, src, tgt, step,
+        _lib.check(lib.ndrr_raymarch_fwd_jac(self.vptr, self.dt, self.code, self.lab, M, src, tgt, step, lattice,
                                              self.img.data_ptr(), self.jac.data_ptr(), 6, B, N, D, H, W, S, C,
                                              width, *self.strides, st), "ndrr_raymarch_fwd_jac")
         _lib.check(lib.ndrr_zncc(self.img.data_ptr(), self.target.data_ptr(), B, C * N, self.eps,
@@ -377,7 +371,7 @@ def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot
             ptrs = [(p.data_ptr(), m.data_ptr(), v.data_ptr(), p.numel(), lr)
                     for (p, lr), (m, v) in zip(params, moments)]
             for it in range(n_iters):
-                offset = torch.rand(B, device=target.device, generator=gen) - 0.5 if jitter else None
+                offset = torch.empty(B, device=target.device).uniform_(-0.5, 0.5, generator=gen) if jitter else None
                 s, grot, gxyz = loop.step_(offset)
                 frac = final_frac + (1 - final_frac) * 0.5 * (1 + math.cos(math.pi * it / max(n_iters - 1, 1)))
                 bc1, bc2 = 1.0 - b1 ** (it + 1), 1.0 - b2 ** (it + 1)
@@ -393,7 +387,7 @@ def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot
     for it in range(n_iters):
         for p, _ in params:
             p.grad = None
-        offset = torch.rand(B, device=target.device, generator=gen) - 0.5 if jitter else None
+        offset = torch.empty(B, device=target.device).uniform_(-0.5, 0.5, generator=gen) if jitter else None
         s = zncc(reg(n_samples, offset=offset), target, eps)
         s.backward(descent)
         frac = final_frac + (1 - final_frac) * 0.5 * (1 + math.cos(math.pi * it / max(n_iters - 1, 1)))

@@ -182,18 +182,30 @@ def march_inputs(subject, M, src, tgt, step_size, n_samples, height, width):
     return args, packed, layout, B, C
 
 
-def march(subject, M, src, tgt, step_size, n_samples, height, width) -> torch.Tensor:
+def march(subject, M, src, tgt, step_size, n_samples, height, width, offset=None) -> torch.Tensor:
     """(B, C, H, W) line integrals of the rays src -> tgt (camera space) with the
-    camera -> voxel-index affines M (B|1, 3, 4) through the subject's volume."""
+    camera -> voxel-index affines M (B|1, 3, 4) through the subject's volume.
+    `offset` (B,), in units of the sample step, slides each pose's sample lattice
+    along its rays (sample s at t = (s + offset) / (n_samples - 1); the step is
+    unchanged); the kernels take it directly unless the two-pass adjoint runs."""
     args, packed, layout, B, C = march_inputs(subject, M, src, tgt, step_size, n_samples, height, width)
     if ADJOINT not in _ADJOINTS:
         raise ValueError(f"ADJOINT must be one of {_ADJOINTS}, got {ADJOINT!r}")
     gcam = any(t.requires_grad for t in args[3:6])
-    single = (ADJOINT == "single_pass" and torch.is_grad_enabled() and not subject.image.requires_grad
-              and (gcam or args[2].requires_grad))
-    if single:
-        out, _ = eager.fused_raymarch_jac(*args, 7 if gcam else 6, packed, layout)
+    grad = torch.is_grad_enabled() and (subject.image.requires_grad or gcam or args[2].requires_grad)
+    lattice = None
+    if offset is not None:
+        lattice = offset.detach().to(device=args[4].device, dtype=torch.float32).reshape(-1).expand(B).contiguous()
+    if grad and ADJOINT == "single_pass" and not subject.image.requires_grad:
+        out, _ = eager.fused_raymarch_jac(*args, 7 if gcam else 6, packed, layout, lattice)
+    elif not grad and lattice is not None:
+        out, _ = eager.fused_raymarch_jac(*args, 0, packed, layout, lattice)
     else:
+        if lattice is not None:  # the two-pass adjoint: shifted copies of the rays
+            vol, lab, M_, s, t, st = args[:6]
+            d = (t - s) / float(n_samples - 1)
+            u = lattice.reshape(-1, 1, 1)
+            args = (vol, lab, M_, torch.addcmul(s, d, u), torch.addcmul(t, d, u), st) + args[6:]
         out = eager.fused_raymarch(*args, packed, layout)
     return out.reshape(B, C, height, width)
 

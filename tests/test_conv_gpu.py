@@ -147,7 +147,8 @@ def test_posenet_training_and_registration_pipeline_runs():
     first = sum(h["loss"] for h in hist[:10]) / 10
     last = sum(h["loss"] for h in hist[-10:]) / 10
     assert last < first, (first, last)
-    r = compare_registration(net, n_test=1, n_iters=10, log=lambda m: None)
+    r = compare_registration(net, n_test=1, n_iters=10, log=lambda m: None, refine=1)
     assert r["problems"] == 1 and 0 <= r["converged_from_cnn"] <= 1
+    assert r["refine_renders"] == 1 and r["render_iterations_of_the_short_cnn_run"] == 3
     assert len(r["settled_by_iteration"]["cnn"]) == 2  # checks at iterations 5 and 10
     assert r["median_zncc_cnn"] > 0.5
