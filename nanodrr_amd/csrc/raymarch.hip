@@ -1,14 +1,15 @@
 // Fused differentiable DRR ray march for AMD Instinct MI355X (gfx950).
 //
-// One thread marches one ray. A workgroup covers a 32 x 16 detector tile as
-// eight wave64s of 8 x 8 rays, so the rays sharing a CU's L1 are spatially
-// coherent and their gathers land in few cache lines; workgroups are remapped
-// so each XCD (own L2) runs neighbouring tiles. Per ray the kernel clips the
-// sample range to the samples whose 2x2x2 footprint can touch the volume
-// (skipped samples contribute exactly zero, so the clip changes no bit of the
-// result), then marches: per sample the trilinear stencil (eight zero-padded
-// corners, from the linear volume or one packed record) and a nearest-label
-// route, with several samples' loads in flight per lane.
+// One thread marches one ray (or one chunk of its samples). A workgroup covers
+// a 32 x 8 detector tile as four wave64s of 8 x 8 rays per chunk group, so the
+// rays sharing a CU's L1 are spatially coherent and their gathers land in few
+// cache lines; workgroups are remapped so each XCD (own L2) runs neighbouring
+// tiles. Per ray the kernel clips the sample range to the samples whose 2x2x2
+// footprint can touch the volume (skipped samples contribute exactly zero, so
+// the clip changes no bit of the result), then marches: per sample the
+// trilinear stencil (eight zero-padded corners, from the linear volume or one
+// packed record) and a nearest-label route, with several samples' loads in
+// flight per lane.
 //
 // The arithmetic of the march and of its adjoint follows nanodrr's fused
 // kernel (src/nanodrr/drr/_fused.py:133-300); the launch layout, clip,

@@ -194,7 +194,7 @@ class DrrSource:
         S = n_samples or self.n_samples
         M = pose_compose(reg._prefix.expand(B, 4, 4).contiguous(), reg._suffix.expand(B, 4, 4).contiguous(),
                          rot.float().contiguous(), xyz.float().contiguous())
-        step, _ = reg._ray_steps(S)
+        step = reg._ray_steps(S)
         from .renderer import march
 
         return march(reg.subject, M, reg._src, reg._tgt, step, S, reg.height, reg.width, offset=offset)

@@ -167,16 +167,15 @@ class Registration(torch.nn.Module):
         directly); None renders exactly as `render`."""
         if backend not in ("auto", "hip", "triton"):
             raise ValueError(f"Unknown backend {backend!r}; expected 'auto', 'hip' or 'triton'")
-        step, _ = self._ray_steps(n_samples)
+        step = self._ray_steps(n_samples)
         return march(self.subject, self.camera_to_voxel(), self._src, self._tgt, step, n_samples, self.height,
                      self.width, offset=offset)
 
-    def _ray_steps(self, n_samples: int):
-        """(step (1, N), per-sample ray vector (1, N, 3)) for n_samples, kept until the rays change."""
+    def _ray_steps(self, n_samples: int) -> torch.Tensor:
+        """The ray step (1, N) for n_samples, kept until the rays change."""
         key = (n_samples, self._tgt.data_ptr(), self._tgt._version)
         if getattr(self, "_steps_key", None) != key or torch.compiler.is_compiling():
-            d = self._tgt - self._src
-            self._steps = (d.norm(dim=-1) / float(n_samples - 1), d / float(n_samples - 1))
+            self._steps = (self._tgt - self._src).norm(dim=-1) / float(n_samples - 1)
             self._steps_key = key
         return self._steps
 
@@ -259,7 +258,7 @@ class _ExplicitLoop:
         self.rot_b = torch.empty(B, 3, device=dev) if self.shared else reg._rot.detach()
         self.xyz_b = torch.empty(B, 3, device=dev) if self.shared else reg._xyz.detach()
         self.M = torch.empty(B, 3, 4, device=dev)
-        self.step, _ = reg._ray_steps(n_samples)
+        self.step = reg._ray_steps(n_samples)
         src, tgt = reg._src, reg._tgt
         self.src, self.tgt = src, tgt
         args, packed, layout, _, C = march_inputs(reg.subject, self.M, src, tgt, self.step, n_samples, reg.height,
