@@ -955,7 +955,7 @@ void k_bwd(Params p) {
 // a thread adds its rays' terms in block order, then the workgroup reduces
 // once. The pose partials go to gpart[b][blockIdx.x].
 #ifndef NDRR_JAC_RAYS
-#define NDRR_JAC_RAYS 4
+#define NDRR_JAC_RAYS 3
 #endif
 constexpr int kJacRays = NDRR_JAC_RAYS;
 
