@@ -179,38 +179,108 @@ def algorithmic_bytes(subject, geom, dtype, adjoint="two"):
     return n_in, fwd, bwd
 
 
-def cpu_baseline(ph, budget_s=12.0):
-    """Pure-PyTorch grid_sample restatement (oracle.torch_ref, fp32) of one DRR of the
-    workload (fwd + pose backward), repeated for about `budget_s` seconds."""
+def _cpu_drr_rate(ph, n_classes, threads, budget_s, max_runs, n_samples=S):
+    """(DRR/s, runs): median rate of one DRR (fwd + pose backward, 1 pose) through the
+    pure-PyTorch grid_sample restatement (oracle.torch_ref, fp32) on `threads` threads,
+    repeated for about `budget_s` seconds (at least 2 runs, at most `max_runs`)."""
     from nanodrr_amd.phantom import detector, phantom_subject, sample_poses
     from oracle import torch_ref
 
-    threads = min(16, os.cpu_count() or 1)
     torch.set_num_threads(threads)
-    subject = phantom_subject(ph, n_classes=N_CLASSES)
+    subject = phantom_subject(ph, n_classes=n_classes)
     rt_all, _, _ = sample_poses(BATCH, seed=1, isocenter=subject.isocenter)
     k_inv, sdd = detector(1020.0, 1.0, DET, DET)
     image, label, w2g = subject.image, subject.label, subject.world_to_grid
-    wts = torch.randn(1, N_CLASSES, DET, DET, generator=torch.Generator().manual_seed(0))
+    wts = torch.randn(1, n_classes, DET, DET, generator=torch.Generator().manual_seed(0))
 
     def one(i):
         rt = rt_all[i % BATCH: i % BATCH + 1].clone().requires_grad_(True)
-        out = torch_ref.render_torch(image, label, N_CLASSES, w2g, k_inv, rt, sdd, DET, DET, S)
+        out = torch_ref.render_torch(image, label, n_classes, w2g, k_inv, rt, sdd, DET, DET, n_samples)
         (out * wts).sum().backward()
 
     one(0)  # warm-up
     times = []
     t_start = time.perf_counter()
     i = 1
-    while (time.perf_counter() - t_start < budget_s and i <= 40) or len(times) < 2:
+    while (time.perf_counter() - t_start < budget_s and i <= max_runs) or len(times) < 2:
         t0 = time.perf_counter()
         one(i)
         times.append(time.perf_counter() - t0)
         i += 1
-    med = statistics.median(times)
-    return {"value": 1.0 / med, "unit": "DRR/s", "cores": threads, "kind": "port",
-            "sample": f"{len(times)} DRRs (1 pose each) of the 256^3/C=4/200x200/S=256 workload, fwd + pose "
-                      f"backward, oracle.torch_ref fp32 grid_sample on {threads} threads, median of runs"}
+    return 1.0 / statistics.median(times), len(times)
+
+
+def cpu_baseline(ph, budget_s=12.0):
+    """The workload's DRRs through the pure-PyTorch CPU restatement on the host's cores (up to
+    16), plus a one-thread rate and the small single-channel CPU case (128^3 at 1.5 mm, C=1,
+    200 x 200, S=256) on both thread counts."""
+    from nanodrr_amd.phantom import make_phantom
+
+    threads = min(16, os.cpu_count() or 1)
+    rate, runs = _cpu_drr_rate(ph, N_CLASSES, threads, budget_s, 40)
+    rate1, runs1 = _cpu_drr_rate(ph, N_CLASSES, 1, 4.0, 3)
+    ph0 = make_phantom((128,) * 3, (1.5,) * 3, seed=0)
+    rate0, runs0 = _cpu_drr_rate(ph0, 1, threads, 3.0, 20)
+    rate01, runs01 = _cpu_drr_rate(ph0, 1, 1, 3.0, 5)
+    torch.set_num_threads(threads)
+    return {"value": rate, "unit": "DRR/s", "cores": threads, "kind": "port",
+            "sample": f"{runs} DRRs (1 pose each) of the 256^3/C=4/200x200/S=256 workload, fwd + pose "
+                      f"backward, oracle.torch_ref fp32 grid_sample on {threads} threads, median of runs",
+            "one_thread": {"value": rate1, "sample": f"{runs1} DRRs of the same workload on 1 thread"},
+            "cpu_case_128": {"value": rate0, "value_one_thread": rate01, "unit": "DRR/s",
+                             "sample": f"{runs0} / {runs01} DRRs, 128^3 at 1.5 mm, C=1, 200x200 at 1.0 mm, "
+                                       f"S=256, fwd + pose backward, on {threads} / 1 threads"}}
+
+
+def reference_shape_leg(reps: int = 200):
+    """Forward-only DRRs at the reference benchmark's own settings (tests/benchmark/benchmark.py:
+    27-38): one 200 x 200 DRR at 2.0 mm pixels, sdd 1020, AP pose 850 mm from the isocentre,
+    n_samples 500, one channel, on the 256^3 phantom (the reference's benchmark volume is not
+    shipped). FPS from HIP events: the GPU time of back-to-back render() calls replayed from a
+    HIP graph (queued behind a spin so the host's launches stay off the clock), and the eager
+    calls' rate with the host in the loop."""
+    from nanodrr_amd import render
+    from nanodrr_amd.camera import make_k_inv, make_rt_inv
+    from nanodrr_amd.phantom import make_phantom, phantom_subject
+
+    dev = torch.device("cuda:0")
+    ph = make_phantom((SIZE,) * 3, (SPACING,) * 3, seed=0)
+    res = {"settings": "B=1, C=1, 200x200 at 2.0 mm, sdd 1020, AP at 850 mm, S=500, forward only, 256^3 phantom"}
+    for name, dt in (("fp32", torch.float32), ("bf16", torch.bfloat16)):
+        subject = phantom_subject(ph, n_classes=1, dtype=dt, device=dev)
+        k_inv = make_k_inv(1020.0, 2.0, 2.0, 0.0, 0.0, 200, 200).to(dev)
+        rt = make_rt_inv(torch.zeros(1, 3), torch.tensor([[0.0, 850.0, 0.0]]), "AP",
+                         subject.isocenter.cpu()).to(dev)
+        sdd = torch.tensor([1020.0], device=dev)
+        with torch.no_grad():
+            def call():
+                return render(subject, k_inv, rt, sdd, 200, 200, n_samples=500)
+
+            ms_eager = time_steps(call, reps, 25)
+            for _ in range(3):
+                call()
+            torch.cuda.synchronize()
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                call()
+            for _ in range(10):
+                graph.replay()
+            torch.cuda.synchronize()
+            t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda._sleep(SPIN_CYCLES)
+            t0.record()
+            for _ in range(reps):
+                graph.replay()
+            t1.record()
+            torch.cuda.synchronize()
+            ms_graph = t0.elapsed_time(t1) / reps
+        res[name] = {"fps_graph_gpu_time": 1000.0 / ms_graph, "us_per_drr_graph": ms_graph * 1000.0,
+                     "fps_eager": 1000.0 / ms_eager, "us_per_drr_eager": ms_eager * 1000.0}
+        del graph, subject
+        torch.cuda.empty_cache()
+    res["reference_published"] = ("Triton fwd on RTX 6000 Ada, its own volume: 13,332 FPS fp32 eager, "
+                                  "17,290 FPS bf16 + torch.compile (GPU time; benchmark.csv:76,81)")
+    return res
 
 
 def counter_probe(case: str, reps: int = 4):
@@ -390,11 +460,14 @@ def main():
             "step": r["drr_per_s"] / plain["drr_per_s"],
             "kernels": (plain["ms_fwd"] + plain["ms_bwd"]) / (r["ms_fwd"] + r["ms_bwd"]),
         }
+    # the cases set the renderer's layout and adjoint per case: back to the defaults
+    renderer.VOLUME_LAYOUT, renderer.ADJOINT = "auto", "single_pass"
+    gc.collect()
+    torch.cuda.empty_cache()  # the march cases' volumes and graph pools
+    line["config"]["reference_benchmark_shape"] = reference_shape_leg()
     if not args.no_registration:
-        # the cases set the renderer's layout and adjoint per case: back to the defaults
-        renderer.VOLUME_LAYOUT, renderer.ADJOINT = "auto", "single_pass"
         gc.collect()
-        torch.cuda.empty_cache()  # the march cases' volumes and graph pools
+        torch.cuda.empty_cache()
         line["registration"] = registration_leg()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(ph)
