@@ -294,7 +294,8 @@ def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_
     from .phantom import detector, fiducials, make_phantom, nominal_pose, phantom_subject
     from .registration import Registration, register
 
-    res = {"nominal": [], "cnn": [], "proposal_mtre": [], "raw_proposal_mtre": [], "zncc": {"nominal": [], "cnn": []},
+    res = {"nominal": [], "cnn": [], "proposal_mtre": [], "raw_proposal_mtre": [], "start_mtre": [],
+           "zncc": {"nominal": [], "cnn": []},
            "settled": {"nominal": [], "cnn": []}, "cnn_fifth": []}
     check_every = 5
     k200, sdd = detector(1020.0, 1.0, 200, 200)
@@ -321,6 +322,7 @@ def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_
         raw._rot.data.copy_(rot_p)
         raw._xyz.data.copy_(xyz_p)
         res["raw_proposal_mtre"].append(float(mtre(raw.pose.detach(), rt_true, fid)[0]))
+        res["start_mtre"].append(float(mtre(rt_nom.double(), rt_true, fid)[0]))
         for _ in range(refine):
             rot_q, xyz_q = predict(net, src128.render(0, rot_p, xyz_p))
             rot_p, xyz_p = 2 * rot_p - rot_q, 2 * xyz_p - xyz_q
@@ -387,6 +389,7 @@ def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_
         "median_mtre_mm_cnn": float(t(res["cnn"]).median()),
         "median_mtre_mm_cnn_proposal": float(t(res["proposal_mtre"]).median()),
         "median_mtre_mm_cnn_proposal_unrefined": float(t(res["raw_proposal_mtre"]).median()),
+        "median_mtre_mm_nominal_start": float(t(res["start_mtre"]).median()),
         "median_zncc_nominal": float(t(res["zncc"]["nominal"]).median()),
         "median_zncc_cnn": float(t(res["zncc"]["cnn"]).median()),
     }
