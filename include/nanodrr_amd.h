@@ -213,6 +213,27 @@ int ndrr_raymarch_bwd_jac_pose(const float* go, int64_t go_batch_stride,
                                int64_t step_batch_stride, void* stream);
 
 /*
+ * ndrr_raymarch_bwd_jac with registration's pose step folded into its pose
+ * reduction: in place of gM it returns the parameter gradients grot, gxyz
+ * ([B] x 3 at param_stride, as ndrr_pose_compose_bwd) of grad_scale * gM
+ * (grad_scale +1 or -1: the descent direction of a score, exactly), with the
+ * sums of ndrr_pose_compose_bwd, bit for bit, in two launches fewer (the
+ * scaling and the adjoint). No camera gradients. Replaces the autograd of
+ * Registration.pose (src/nanodrr/registration/registration.py:64-68) after
+ * the render's backward (_fused.py:477-531).
+ */
+int ndrr_raymarch_bwd_jac_se3(const float* go, int64_t go_batch_stride,
+                              const float* jac, int jac_planes, const float* M,
+                              const float* src, const float* tgt, const float* step,
+                              float* workspace, const double* prefix,
+                              const double* suffix, const float* rot, const float* xyz,
+                              int64_t param_stride, float grad_scale, float* grot,
+                              float* gxyz, int B, int N, int n_classes, int width,
+                              int64_t src_batch_stride, int64_t src_ray_stride,
+                              int64_t tgt_batch_stride, int64_t step_batch_stride,
+                              void* stream);
+
+/*
  * Packed copies of a [D][H][W] volume that the march reads with one load per
  * sample (two 16-byte loads for fp32):
  *   NDRR_LAYOUT_QUAD: one record per (x, y0, z0), x in [-1, W], y0 in [-1, H-1],

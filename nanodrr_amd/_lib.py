@@ -54,6 +54,11 @@ SIGNATURES: dict[str, tuple] = {
          _c_int, _c_int, _c_int, _c_int,
          _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
     ),
+    "ndrr_raymarch_bwd_jac_se3": (
+        _c_int,
+        [_c_p, _c_i64, _c_p, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, ctypes.c_float,
+         _c_p, _c_p, _c_int, _c_int, _c_int, _c_int, _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
+    ),
     "ndrr_bwd_workspace_floats": (_c_size, [_c_int, _c_int, _c_int]),
     "ndrr_launch_tile": (None, [ctypes.POINTER(_c_int), ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
     "ndrr_raymarch_bwd": (

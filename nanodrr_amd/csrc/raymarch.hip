@@ -19,6 +19,7 @@
 #include <stdint.h>
 
 #include "nanodrr_amd.h"
+#include "se3_pose.h"
 
 namespace {
 
@@ -1035,10 +1036,23 @@ __global__ __launch_bounds__(kTileRays) void k_bwd_jac(Params p) {
 // ... (compensated, loads independent of one another), then a pairwise tree
 // over the 256 thread sums in LDS. Deterministic.
 constexpr int kReduceThreads = 256;
+// The registration form of the reduction's output: the pose step's parameter
+// gradients from scale * gM (scale -1 for a descent direction; exact).
+struct Se3Out {
+  const double* prefix;
+  const double* suffix;
+  const float* rot;
+  const float* xyz;
+  int64_t param_stride;
+  float scale;
+  float* grot;
+  float* gxyz;
+};
+
 __global__ __launch_bounds__(kReduceThreads) void k_reduce_pose(const float* __restrict__ part,
                                                                float* __restrict__ gM, int nparts,
                                                                const double* __restrict__ w2v,
-                                                               float* __restrict__ g_rt) {
+                                                               float* __restrict__ g_rt, Se3Out se3) {
   __shared__ float red[12][kReduceThreads];
   const int b = blockIdx.x, t = threadIdx.x;
   const float4* pb = reinterpret_cast<const float4*>(part + static_cast<int64_t>(b) * nparts * 12);
@@ -1069,6 +1083,17 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce_pose(const float* __r
 #pragma unroll
     for (int i = 0; i < 3; ++i) acc = fma(w2v[4 * i + k], static_cast<double>(red[4 * i + j][0]), acc);
     g_rt[b * 16 + t] = static_cast<float>(acc);
+  }
+  if (se3.grot != nullptr && t == 0) {
+    // the pose step's adjoint (the sums of k_pose_compose_bwd) from the rounded, scaled gM
+    const int64_t ps = se3.param_stride;
+    const double w[3] = {se3.rot[b * ps], se3.rot[b * ps + 1], se3.rot[b * ps + 2]};
+    const double v[3] = {se3.xyz[b * ps], se3.xyz[b * ps + 1], se3.xyz[b * ps + 2]};
+    double G[3][4];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 4; ++j) G[i][j] = static_cast<double>(se3.scale * red[4 * i + j][0]);
+    ndrr_se3::compose_adjoint(se3.prefix + 16 * static_cast<int64_t>(b), se3.suffix + 16 * static_cast<int64_t>(b), w,
+                              v, G, se3.grot + b * ps, se3.gxyz + b * ps);
   }
 }
 
@@ -1444,7 +1469,7 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_l
                              : vol_layout == NDRR_LAYOUT_QUAD  ? bwd_parts_per_tile(NDRR_LAYOUT_QUAD)
                                                                : bwd_parts_per_tile(NDRR_LAYOUT_LINEAR);
   hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(kReduceThreads), 0, st, workspace, gM,
-                     p.nblocks * parts_per_tile, nullptr, nullptr);
+                     p.nblocks * parts_per_tile, nullptr, nullptr, Se3Out{});
   return launch_status();
 }
 
@@ -1454,8 +1479,9 @@ static int raymarch_bwd_jac_any(const float* go, int64_t go_batch_stride, const 
                                 const float* tgt, const float* step, float* workspace, float* gM, float* g_rt,
                                 float* gsrc, float* gtgt, float* gstep, int need_gcam, int B, int N, int n_classes,
                                 int width, int64_t src_batch_stride, int64_t src_ray_stride,
-                                int64_t tgt_batch_stride, int64_t step_batch_stride, void* stream) {
-  if (w2v != nullptr ? (!rt || !g_rt || M || gM) : (!M || !gM)) return NDRR_EINVAL;
+                                int64_t tgt_batch_stride, int64_t step_batch_stride, void* stream,
+                                const Se3Out& se3 = Se3Out{}) {
+  if (w2v != nullptr ? (!rt || !g_rt || M || gM) : (!M || (!gM && !se3.grot))) return NDRR_EINVAL;
   if (!go || !jac || !src || !tgt || !step || !workspace) return NDRR_EINVAL;
   if (reinterpret_cast<uintptr_t>(workspace) % 16 != 0) return NDRR_EINVAL;
   if (B <= 0 || N <= 0 || width < 0 || (width > 0 && N % width != 0)) return NDRR_EINVAL;
@@ -1497,7 +1523,7 @@ static int raymarch_bwd_jac_any(const float* go, int64_t go_batch_stride, const 
   }
   int r = launch_status();
   if (r != NDRR_OK) return r;
-  hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(kReduceThreads), 0, st, workspace, gM, nlin, w2v, g_rt);
+  hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(kReduceThreads), 0, st, workspace, gM, nlin, w2v, g_rt, se3);
   return launch_status();
 }
 
@@ -1521,6 +1547,20 @@ int ndrr_raymarch_bwd_jac_pose(const float* go, int64_t go_batch_stride, const f
   return raymarch_bwd_jac_any(go, go_batch_stride, jac, jac_planes, nullptr, world_to_voxel, rt_inv, src, tgt, step,
                               workspace, nullptr, g_rt_inv, gsrc, gtgt, gstep, need_gcam, B, N, n_classes, width,
                               src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride, stream);
+}
+
+int ndrr_raymarch_bwd_jac_se3(const float* go, int64_t go_batch_stride, const float* jac, int jac_planes,
+                              const float* M, const float* src, const float* tgt, const float* step, float* workspace,
+                              const double* prefix, const double* suffix, const float* rot, const float* xyz,
+                              int64_t param_stride, float grad_scale, float* grot, float* gxyz, int B, int N,
+                              int n_classes, int width, int64_t src_batch_stride, int64_t src_ray_stride,
+                              int64_t tgt_batch_stride, int64_t step_batch_stride, void* stream) {
+  if (!prefix || !suffix || !rot || !xyz || !grot || !gxyz || param_stride < 3) return NDRR_EINVAL;
+  if (grad_scale != 1.0f && grad_scale != -1.0f) return NDRR_EINVAL;
+  const Se3Out se3{prefix, suffix, rot, xyz, param_stride, grad_scale, grot, gxyz};
+  return raymarch_bwd_jac_any(go, go_batch_stride, jac, jac_planes, M, nullptr, nullptr, src, tgt, step, workspace,
+                              nullptr, nullptr, nullptr, nullptr, nullptr, 0, B, N, n_classes, width,
+                              src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride, stream, se3);
 }
 
 size_t ndrr_packed_bytes(int D, int H, int W, int vol_dtype, int vol_layout) {

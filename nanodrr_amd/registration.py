@@ -280,7 +280,6 @@ class _ExplicitLoop:
         self.gimg = torch.empty(B, C, N, device=dev)
         self.zws = torch.empty(self.lib.ndrr_zncc_workspace_floats(B, C * N), device=dev)
         self.bws = torch.empty(self.lib.ndrr_bwd_workspace_floats(B, N, reg.width), device=dev)
-        self.gM = torch.empty(B, 3, 4, device=dev)
         self.grot = torch.empty(B, 3, device=dev)
         self.gxyz = torch.empty(B, 3, device=dev)
 
@@ -304,13 +303,12 @@ class _ExplicitLoop:
         _lib.check(lib.ndrr_zncc(self.img.data_ptr(), self.target.data_ptr(), B, C * N, self.eps,
                                  self.zws.data_ptr(), self.score.data_ptr(), self.gimg.data_ptr(), st), "ndrr_zncc")
         # d(-sum score)/d img = -gimg; the contraction and the pose adjoint are linear and
-        # sign-symmetric, so negating gM gives the bits that -gimg would
-        _lib.check(lib.ndrr_raymarch_bwd_jac(self.gimg.data_ptr(), C * N, self.jac.data_ptr(), 6, M, src, tgt, step,
-                                             self.bws.data_ptr(), self.gM.data_ptr(), None, None, None, 0, B, N, C,
-                                             width, *self.strides, st), "ndrr_raymarch_bwd_jac")
-        self.gM.neg_()
-        _lib.check(lib.ndrr_pose_compose_bwd(pre, suf, rot, xyz, 3, self.gM.data_ptr(), self.grot.data_ptr(),
-                                             self.gxyz.data_ptr(), B, st), "ndrr_pose_compose_bwd")
+        # sign-symmetric, so scaling the reduced gM by -1 (inside the pose reduction, which also
+        # runs the pose step's adjoint) gives the bits that -gimg would
+        _lib.check(lib.ndrr_raymarch_bwd_jac_se3(self.gimg.data_ptr(), C * N, self.jac.data_ptr(), 6, M, src, tgt,
+                                                 step, self.bws.data_ptr(), pre, suf, rot, xyz, 3, -1.0,
+                                                 self.grot.data_ptr(), self.gxyz.data_ptr(), B, N, C, width,
+                                                 *self.strides, st), "ndrr_raymarch_bwd_jac_se3")
         if self.shared:  # the expand's adjoint
             return self.score, self.grot.sum_to_size(reg._rot.shape), self.gxyz.sum_to_size(reg._xyz.shape)
         return self.score, self.grot, self.gxyz

@@ -183,6 +183,16 @@ def test_conv_weight_workspace_holds_whole_split_records():
     assert lib.ndrr_conv3x3_bwd_weight_workspace_floats(2, 16, 16, 48, 64, 1) == 0  # Cin not 1 or 32k
 
 
+def test_fused_pose_step_adjoint_rejects_bad_arguments_before_launch():
+    """ndrr_raymarch_bwd_jac_se3: null pose-step inputs, a scale other than +-1, a short stride."""
+    lib = _lib.lib()
+    args = dict(go=1, gbs=16, jac=1, planes=6, M=1, src=1, tgt=1, step=1, ws=16, pre=1, suf=1, rot=1, xyz=1, ps=3,
+                scale=-1.0, grot=1, gxyz=1, B=1, N=16, C=1, width=4, sbs=0, srs=0, tbs=0, pbs=0, stream=None)
+    for key, bad in (("pre", None), ("rot", None), ("grot", None), ("scale", 0.5), ("ps", 2), ("M", None)):
+        kw = dict(args, **{key: bad})
+        assert lib.ndrr_raymarch_bwd_jac_se3(*kw.values()) == -1, key
+
+
 def test_new_entry_points_reject_bad_arguments_before_launch():
     lib = _lib.lib()
     EINVAL = -1
