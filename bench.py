@@ -232,6 +232,56 @@ def cpu_baseline(ph, budget_s=12.0):
                                        f"S=256, fwd + pose backward, on {threads} / 1 threads"}}
 
 
+def mixed_precision_leg(steps: int = 20, warmup: int = 5):
+    """The mixed-precision workload at full size: a 512 x 512 x 400 phantom at 0.8 x 0.8 x 1.0 mm
+    stored in bf16 and in fp16, 2 labels, 8 poses, 512 x 512 detector at 0.8 mm, S = 512
+    (compensated sums), fwd + pose backward (single-pass adjoint, quad layout), graph-replayed
+    steps timed with HIP events, plus the plain fp32 kernel on the fp32 volume."""
+    import nanodrr_amd.renderer as renderer
+    from nanodrr_amd import render
+    from nanodrr_amd.phantom import detector, make_phantom, phantom_subject, sample_poses
+
+    dev = torch.device("cuda:0")
+    ph = make_phantom((400, 512, 512), (0.8, 0.8, 1.0), seed=0)
+    B, det, n_s = 8, 512, 512
+    res = {"workload": "512x512x400 @0.8x0.8x1.0 mm, C=2, B=8, 512x512 @0.8 mm, S=512 (compensated), fwd + pose "
+                       "backward, graph-replayed step"}
+    for name, dt, layout, adjoint in (("bf16", torch.bfloat16, "quad", "single_pass"),
+                                      ("fp16", torch.float16, "quad", "single_pass"),
+                                      ("fp32_plain", torch.float32, "linear", "two_pass")):
+        renderer.VOLUME_LAYOUT, renderer.ADJOINT = layout, adjoint
+        subject = phantom_subject(ph, n_classes=2, dtype=dt, device=dev)
+        rt_inv, _, _ = sample_poses(B, seed=11, isocenter=subject.isocenter)
+        k_inv, sdd = detector(1020.0, 0.8, det, det)
+        k_inv, sdd, rt_inv = k_inv.to(dev), sdd.to(dev), rt_inv.to(dev)
+        wts = torch.randn(1, 2, det, det, generator=torch.Generator().manual_seed(0)).to(dev)
+        rt = rt_inv.detach().clone().requires_grad_(True)
+
+        def body():
+            rt.grad = None
+            out = render(subject, k_inv, rt, sdd, det, det, n_samples=n_s)
+            out.backward(wts.expand_as(out))
+
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(3):
+                body()
+        torch.cuda.current_stream().wait_stream(side)
+        graph = torch.cuda.CUDAGraph()
+        rt.grad = None
+        with torch.cuda.graph(graph):
+            out = render(subject, k_inv, rt, sdd, det, det, n_samples=n_s)
+            out.backward(wts.expand_as(out))
+        ms = time_steps(graph.replay, steps, warmup)
+        res[name] = {"ms_per_step": ms, "drr_per_s": B * 1000.0 / ms}
+        del graph, subject, out
+        gc.collect()
+        torch.cuda.empty_cache()
+    renderer.VOLUME_LAYOUT, renderer.ADJOINT = "auto", "single_pass"
+    return res
+
+
 def reference_shape_leg(reps: int = 200):
     """Forward-only DRRs at the reference benchmark's own settings (tests/benchmark/benchmark.py:
     27-38): one 200 x 200 DRR at 2.0 mm pixels, sdd 1020, AP pose 850 mm from the isocentre,
@@ -377,12 +427,17 @@ def main():
                     help="run only the counter probe of one case, e.g. bf16:stencil:single")
     ap.add_argument("--no-registration", action="store_true", help="skip the batched registration leg")
     ap.add_argument("--registration-only", action="store_true", help="run only the registration leg")
+    ap.add_argument("--mixed", action="store_true",
+                    help="run only the mixed-precision workload (512x512x400, C=2, B=8, 512x512, S=512)")
     args = ap.parse_args()
     if args.probe:
         counter_probe(args.probe)
         return
     if args.registration_only:
         print(json.dumps(registration_leg()), flush=True)
+        return
+    if args.mixed:
+        print(json.dumps(mixed_precision_leg()), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
