@@ -119,3 +119,30 @@ def test_lattice_offset_validates():
     with pytest.raises(ValueError, match="lattice"):
         _ops.fused_raymarch_jac(vol, lab, M, src, tgt, step, S, 1, W, 6, None, "linear",
                                 torch.zeros(4, device=DEV, dtype=torch.float64))
+
+
+@pytest.mark.parametrize("C", [1, 3])
+def test_lattice_offset_against_float64_oracle(C):
+    """The kernels' offset lattice against the float64 oracle marching the rays shifted by
+    offset_b steps in float64: the same sample positions, so the images agree to fp32 rounding."""
+    import numpy as np
+
+    from nanodrr_amd import _ops
+    from nanodrr_amd.phantom import make_phantom
+    from oracle import raymarch as orm
+
+    _, vol, lab, packed, M, _, src, tgt, step, _ = _setup(C, "linear")
+    ph = make_phantom((64, 64, 64), (4.0, 4.0, 4.0), seed=C + 20)
+    vol64 = ph.mu.double().numpy()
+    lab64 = ph.labels.long().clamp(max=C - 1).numpy() if C > 1 else None
+    u = torch.tensor([0.5, -0.25, 0.37, -0.5], device=DEV)
+    out, _ = _ops.fused_raymarch_jac(vol, lab, M, src, tgt, step, S, C, W, 6, None, "linear", u)
+    rays = np.sort(np.random.default_rng(C).choice(H * W, size=400, replace=False))
+    d = (tgt.double() - src.double()) / float(S - 1)
+    ub = u.double().reshape(-1, 1, 1)
+    s64 = (src.double() + ub * d)[:, rays].cpu().numpy()
+    t64 = (tgt.double() + ub * d)[:, rays].cpu().numpy()
+    st64 = step.double().expand(4, -1)[:, rays].cpu().numpy()
+    ref = orm.forward(vol64, lab64, C, M.double().cpu().numpy(), s64, t64, st64, S)
+    got = out[:, :, rays].double().cpu().numpy()
+    assert np.abs(ref - got).max() / np.abs(ref).max() < 1e-5
