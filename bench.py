@@ -427,6 +427,8 @@ def main():
                     help="run only the counter probe of one case, e.g. bf16:stencil:single")
     ap.add_argument("--no-registration", action="store_true", help="skip the batched registration leg")
     ap.add_argument("--registration-only", action="store_true", help="run only the registration leg")
+    ap.add_argument("--reference-shape", action="store_true",
+                    help="run only the forward at the reference benchmark's shape (B=1, C=1, 200x200 @2 mm, S=500)")
     ap.add_argument("--mixed", action="store_true",
                     help="run only the mixed-precision workload (512x512x400, C=2, B=8, 512x512, S=512)")
     args = ap.parse_args()
@@ -438,6 +440,9 @@ def main():
         return
     if args.mixed:
         print(json.dumps(mixed_precision_leg()), flush=True)
+        return
+    if args.reference_shape:
+        print(json.dumps(reference_shape_leg()), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
