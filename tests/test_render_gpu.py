@@ -354,3 +354,29 @@ def test_label_quads_hold_the_classes(volume_layout):
     pad = torch.nn.functional.pad(lab.to(torch.int32), (1, 1, 1, 1, 1, 1)).to(torch.int8)
     expect = torch.stack([pad[a:a + D + 1, b:b + H + 1, :] for a in (0, 1) for b in (0, 1)], dim=-1)
     assert torch.equal(got.view(D + 1, H + 1, W + 2, 4), expect)
+
+
+@pytest.mark.parametrize("adjoint", ["single_pass", "two_pass"])
+def test_rays_that_miss_the_volume(adjoint):
+    """A pose whose rays all miss the volume renders exact zeros with exact-zero, finite pose
+    gradients, and leaves the pose that hits it in the same batch as it renders alone."""
+    import nanodrr_amd.renderer as renderer
+
+    subject = random_subject(n_classes=2, device=DEV)
+    k_inv, _, sdd, h, w = camera()
+    rots = torch.tensor([[5.0, -3.0, 8.0], [5.0, -3.0, 8.0]])
+    trans = torch.tensor([[1.0, 80.0, -2.0], [900.0, 80.0, -2.0]])  # the second is 900 mm to the side
+    old = renderer.ADJOINT
+    try:
+        renderer.ADJOINT = adjoint
+        rt = make_rt_inv(rots, trans, "AP", torch.zeros(3)).to(DEV).requires_grad_(True)
+        out = render(subject, k_inv, rt, sdd, h, w, n_samples=200)
+        out.sum().backward()
+        rt1 = make_rt_inv(rots[:1], trans[:1], "AP", torch.zeros(3)).to(DEV).requires_grad_(True)
+        one = render(subject, k_inv, rt1, sdd, h, w, n_samples=200)
+        one.sum().backward()
+    finally:
+        renderer.ADJOINT = old
+    assert out[0].abs().max() > 0 and torch.count_nonzero(out[1]) == 0
+    assert torch.isfinite(rt.grad).all() and torch.count_nonzero(rt.grad[1]) == 0
+    assert torch.equal(out[0], one[0]) and torch.equal(rt.grad[0], rt1.grad[0])
