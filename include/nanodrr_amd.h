@@ -63,7 +63,8 @@ enum {
   NDRR_EINVAL = -1,  /* null pointer, non-positive size, S < 2, bad width */
   NDRR_EDTYPE = -2,  /* unknown volume element type */
   NDRR_ECLASSES = -3,/* n_classes outside [1, NDRR_MAX_CLASSES] */
-  NDRR_ERANGE = -4   /* a size exceeds the kernel's 32-bit index range */
+  NDRR_ERANGE = -4,  /* a size exceeds the kernel's 32-bit index range */
+  NDRR_ESCRATCH = -5 /* the work needs more than the fixed scratch array holds */
 };
 
 #define NDRR_MAX_CLASSES 8
@@ -232,6 +233,34 @@ int ndrr_raymarch_bwd_jac_se3(const float* go, int64_t go_batch_stride,
                               int64_t src_batch_stride, int64_t src_ray_stride,
                               int64_t tgt_batch_stride, int64_t step_batch_stride,
                               void* stream);
+
+/*
+ * ndrr_raymarch_bwd_jac_pose without the workspace argument: it allocates
+ * nothing. No counterpart in the reference: it is for callers that cannot
+ * call hipMalloc (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation). The pose partials go to a fixed
+ * device array of NDRR_NOALLOC_SCRATCH_FLOATS fp32 that the library holds
+ * on every GPU it runs on until the process ends; its address is looked up
+ * once per device (hipGetSymbolAddress) on the first call there, with the
+ * stream's device current. A call needs B * ceil(N / NDRR_NOALLOC_RAYS_PER_PART)
+ * * 12 floats of it and returns NDRR_ESCRATCH, before any launch, when that
+ * exceeds the array. Every partial is written before it is read, so nothing
+ * is cleared between calls. All calls on one GPU share its array: calls that
+ * overlap in time on one GPU (concurrent streams) race; serialise them.
+ * Results are bitwise those of ndrr_raymarch_bwd_jac_pose.
+ */
+#define NDRR_NOALLOC_SCRATCH_FLOATS (1u << 20)
+#define NDRR_NOALLOC_RAYS_PER_PART 768
+int ndrr_raymarch_bwd_jac_pose_noalloc(const float* go, int64_t go_batch_stride,
+                                       const float* jac, int jac_planes,
+                                       const double* world_to_voxel, const float* rt_inv,
+                                       const float* src, const float* tgt,
+                                       const float* step, float* g_rt_inv, float* gsrc,
+                                       float* gtgt, float* gstep, int need_gcam, int B,
+                                       int N, int n_classes, int width,
+                                       int64_t src_batch_stride, int64_t src_ray_stride,
+                                       int64_t tgt_batch_stride, int64_t step_batch_stride,
+                                       void* stream);
 
 /*
  * Packed copies of a [D][H][W] volume that the march reads with one load per

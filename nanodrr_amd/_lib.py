@@ -54,6 +54,13 @@ SIGNATURES: dict[str, tuple] = {
          _c_int, _c_int, _c_int, _c_int,
          _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
     ),
+    # ndrr_raymarch_bwd_jac_pose without its workspace argument (fixed device scratch array)
+    "ndrr_raymarch_bwd_jac_pose_noalloc": (
+        _c_int,
+        [_c_p, _c_i64, _c_p, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_int,
+         _c_int, _c_int, _c_int, _c_int,
+         _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
+    ),
     "ndrr_raymarch_bwd_jac_se3": (
         _c_int,
         [_c_p, _c_i64, _c_p, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64, ctypes.c_float,

@@ -18,6 +18,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+#include <mutex>
+
 #include "nanodrr_amd.h"
 #include "se3_pose.h"
 
@@ -1351,6 +1354,7 @@ const char* ndrr_strerror(int code) {
     case NDRR_EDTYPE: return "unsupported volume element type";
     case NDRR_ECLASSES: return "n_classes outside [1, 8]";
     case NDRR_ERANGE: return "a size exceeds the kernel's 32-bit index range";
+    case NDRR_ESCRATCH: return "the work needs more than the allocation-free entry point's fixed scratch array holds";
     default: break;
   }
   if (code > 0) return hipGetErrorString(static_cast<hipError_t>(code));
@@ -1561,6 +1565,67 @@ int ndrr_raymarch_bwd_jac_se3(const float* go, int64_t go_batch_stride, const fl
   return raymarch_bwd_jac_any(go, go_batch_stride, jac, jac_planes, M, nullptr, nullptr, src, tgt, step, workspace,
                               nullptr, nullptr, nullptr, nullptr, nullptr, 0, B, N, n_classes, width,
                               src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride, stream, se3);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free contraction: the pose partials of ndrr_raymarch_bwd_jac_pose
+// go to a fixed device array. In host code `scratch` names a host-side
+// stand-in, not device memory: its device address is looked up per device
+// with hipGetSymbolAddress and only that address is handed to kernels.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) float scratch[NDRR_NOALLOC_SCRATCH_FLOATS];
+static_assert(kTileRays * kJacRays == NDRR_NOALLOC_RAYS_PER_PART,
+              "the contraction writes one pose partial per NDRR_NOALLOC_RAYS_PER_PART rays");
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::atomic<float*> g_scratch[kMaxDevices];  // device address of `scratch`, per device (null until looked up)
+std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device, looked up once per device.
+int scratch_of_current_device(float** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return static_cast<int>(e);
+  if (dev < 0 || dev >= kMaxDevices) return NDRR_ERANGE;
+  float* p = g_scratch[dev].load(std::memory_order_acquire);
+  if (p == nullptr) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    p = g_scratch[dev].load(std::memory_order_relaxed);
+    if (p == nullptr) {
+      void* addr = nullptr;
+      e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess) return static_cast<int>(e);
+      if (addr == nullptr) return static_cast<int>(hipErrorInvalidSymbol);
+      p = static_cast<float*>(addr);
+      g_scratch[dev].store(p, std::memory_order_release);
+    }
+  }
+  *out = p;
+  return NDRR_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ndrr_raymarch_bwd_jac_pose_noalloc(const float* go, int64_t go_batch_stride, const float* jac, int jac_planes,
+                                       const double* world_to_voxel, const float* rt_inv, const float* src,
+                                       const float* tgt, const float* step, float* g_rt_inv, float* gsrc,
+                                       float* gtgt, float* gstep, int need_gcam, int B, int N, int n_classes,
+                                       int width, int64_t src_batch_stride, int64_t src_ray_stride,
+                                       int64_t tgt_batch_stride, int64_t step_batch_stride, void* stream) {
+  if (world_to_voxel == nullptr || B <= 0 || N <= 0) return NDRR_EINVAL;
+  const int64_t parts = static_cast<int64_t>(B) * ((static_cast<int64_t>(N) + NDRR_NOALLOC_RAYS_PER_PART - 1) /
+                                                   NDRR_NOALLOC_RAYS_PER_PART);
+  if (parts * 12 > static_cast<int64_t>(NDRR_NOALLOC_SCRATCH_FLOATS)) return NDRR_ESCRATCH;
+  float* ws = nullptr;
+  const int rc = scratch_of_current_device(&ws);
+  if (rc != NDRR_OK) return rc;
+  return raymarch_bwd_jac_any(go, go_batch_stride, jac, jac_planes, nullptr, world_to_voxel, rt_inv, src, tgt, step,
+                              ws, nullptr, g_rt_inv, gsrc, gtgt, gstep, need_gcam, B, N, n_classes, width,
+                              src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride, stream);
 }
 
 size_t ndrr_packed_bytes(int D, int H, int W, int vol_dtype, int vol_layout) {

@@ -42,7 +42,7 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_error_strings():
     lib = _lib.lib()
     assert lib.ndrr_version() >= 100
-    for code in (0, -1, -2, -3, -4):
+    for code in (0, -1, -2, -3, -4, -5):
         assert lib.ndrr_strerror(code)
     assert b"invalid" in lib.ndrr_strerror(-1)
 
@@ -208,3 +208,13 @@ def test_new_entry_points_reject_bad_arguments_before_launch():
     assert lib.ndrr_conv3x3_bwd_weight(fake, fake, fake + 4, fake, None, 2, 16, 16, 32, 64, 2, None) == EINVAL
     # Cout not a multiple of 64 for the MFMA weight gradient
     assert lib.ndrr_conv3x3_bwd_weight(fake, fake, fake, fake, None, 2, 16, 16, 32, 96, 2, None) == EINVAL
+
+
+def test_noalloc_contraction_rejects_oversized_work_on_the_host():
+    """ndrr_raymarch_bwd_jac_pose_noalloc checks the fixed scratch array's size before any HIP call."""
+    lib = _lib.lib()
+    assert lib.ndrr_raymarch_bwd_jac_pose_noalloc(1, 0, 1, 7, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2000, 40000, 1, 200,
+                                                  0, 0, 0, 0, None) == -5
+    assert lib.ndrr_raymarch_bwd_jac_pose_noalloc(1, 0, 1, 7, None, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 400, 1, 20,
+                                                  0, 0, 0, 0, None) == -1
+    assert b"scratch" in lib.ndrr_strerror(-5)
