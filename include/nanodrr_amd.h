@@ -389,6 +389,22 @@ int64_t ndrr_zncc_workspace_floats(int B, int N);
 int ndrr_zncc(const float* x, const float* y, int B, int N, float eps, float* workspace,
               float* score, float* grad_x, void* stream);
 
+/*
+ * Local (patch) ZNCC of B image pairs of H x W pixels (fp32): score[b] = the
+ * mean over every patch x patch window (stride 1, (H - patch + 1)(W - patch + 1)
+ * windows) of the window's ZNCC as ndrr_zncc defines it (population variance
+ * + eps per window); grad_x (optional, [B][H][W]) = d score[b] / d x. 1 <=
+ * patch <= min(H, W, 32). Two launches: per-window moments and gradient terms
+ * into `workspace` (ndrr_patch_zncc_workspace_floats floats, 16-byte
+ * aligned), then per-pixel sums over the windows and the score.
+ * Deterministic. Replaces NormalizedCrossCorrelation2d with a patch size,
+ * src/nanodrr/metrics/ncc.py:16-37,83-88 (unfolded windows standardised one
+ * by one), without materialising the windows.
+ */
+int64_t ndrr_patch_zncc_workspace_floats(int B, int H, int W, int patch);
+int ndrr_patch_zncc(const float* x, const float* y, int B, int H, int W, int patch, float eps,
+                    float* workspace, float* score, float* grad_x, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Pose-proposal CNN: 3x3 convolutions on the bf16 matrix cores
  * ------------------------------------------------------------------------- */

@@ -95,6 +95,8 @@ SIGNATURES: dict[str, tuple] = {
                        + [ctypes.c_float] * 3 + [_c_p]),
     "ndrr_zncc_workspace_floats": (ctypes.c_int64, [_c_int, _c_int]),
     "ndrr_zncc": (_c_int, [_c_p, _c_p, _c_int, _c_int, ctypes.c_float, _c_p, _c_p, _c_p, _c_p]),
+    "ndrr_patch_zncc_workspace_floats": (ctypes.c_int64, [_c_int, _c_int, _c_int, _c_int]),
+    "ndrr_patch_zncc": (_c_int, [_c_p, _c_p, _c_int, _c_int, _c_int, _c_int, ctypes.c_float, _c_p, _c_p, _c_p, _c_p]),
 }
 
 

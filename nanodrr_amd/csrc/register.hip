@@ -5,6 +5,7 @@
 //   * zero-normalised cross-correlation of each rendered image with its
 //     target, with the gradient: chunk moments over (chunk, image)
 //     workgroups merged in a fixed order (deterministic);
+//   * local (patch) ZNCC over every p x p window, with its gradient;
 //   * the Adam update of the registration parameters.
 //
 // The parameterisation follows nanodrr's Registration module
@@ -204,6 +205,122 @@ __global__ __launch_bounds__(kThreads) void k_adam(float* __restrict__ p, const 
   p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
 }
 
+// ---------------------------------------------------------------------------
+// Local (patch) ZNCC: the mean over every p x p window (stride 1) of the
+// window's ZNCC, as nanodrr's patch NCC (src/nanodrr/metrics/ncc.py:21-24,
+// 83-88: each unfolded window is standardised on its own, with the
+// population variance + eps), and its gradient.
+//
+// k_patch_windows: a workgroup takes a 16 x 16 block of windows and stages the
+// (16 + p - 1)^2 pixels they cover, of x and y, in LDS; each thread forms its
+// window's means, then its centred moments (two passes over LDS), its score s
+// and the window's gradient terms alpha = 1 / (P n sx sy) and
+// beta = s / (P n sx^2) (n = p^2 pixels, P windows), written with the means as
+// one float4 per window; the block's sum of s goes to a partial.
+// k_patch_pixels: per 16 x 16 pixel block, the windows covering it are staged
+// in LDS (zeros outside the window grid) and each pixel sums
+//   dS/dx = sum over its windows of alpha (y - my) - beta (x - mx),
+// the derivative of S = (1/P) sum s; block 0 of each image also adds the
+// score partials in a fixed order. Deterministic.
+// ---------------------------------------------------------------------------
+constexpr int kPatchTile = 16;
+constexpr int kPatchMax = 32;
+constexpr int kPatchSpan = kPatchTile + kPatchMax - 1;
+
+__global__ __launch_bounds__(kThreads) void k_patch_windows(const float* __restrict__ x, const float* __restrict__ y,
+                                                            int H, int W, int p, float eps, float inv_pn,
+                                                            float4* __restrict__ coef, float* __restrict__ part) {
+  __shared__ float sx[kPatchSpan][kPatchSpan + 1];
+  __shared__ float sy[kPatchSpan][kPatchSpan + 1];
+  __shared__ float red[kThreads / 64];
+  const int Ho = H - p + 1, Wo = W - p + 1;
+  const int tiles_w = (Wo + kPatchTile - 1) / kPatchTile;
+  const int b = blockIdx.y;
+  const int ty = blockIdx.x / tiles_w, tx = blockIdx.x - ty * tiles_w;
+  const int y0 = ty * kPatchTile, x0 = tx * kPatchTile;
+  const int span = kPatchTile + p - 1;
+  const float* xb = x + static_cast<int64_t>(b) * H * W;
+  const float* yb = y + static_cast<int64_t>(b) * H * W;
+  for (int e = threadIdx.x; e < span * span; e += kThreads) {
+    const int r = e / span, c = e - r * span;
+    const int gy = y0 + r, gx = x0 + c;
+    const bool in = gy < H && gx < W;
+    sx[r][c] = in ? xb[gy * W + gx] : 0.0f;
+    sy[r][c] = in ? yb[gy * W + gx] : 0.0f;
+  }
+  __syncthreads();
+  const int ly = threadIdx.x / kPatchTile, lx = threadIdx.x % kPatchTile;
+  const int wy = y0 + ly, wx = x0 + lx;
+  const bool valid = wy < Ho && wx < Wo;
+  float s = 0.0f;
+  if (valid) {
+    const float inv_n = 1.0f / static_cast<float>(p * p);
+    float ax = 0.0f, ay = 0.0f;
+    for (int i = 0; i < p; ++i)
+      for (int j = 0; j < p; ++j) {
+        ax += sx[ly + i][lx + j];
+        ay += sy[ly + i][lx + j];
+      }
+    const float mx = ax * inv_n, my = ay * inv_n;
+    float vx = 0.0f, vy = 0.0f, cxy = 0.0f;
+    for (int i = 0; i < p; ++i)
+      for (int j = 0; j < p; ++j) {
+        const float dx = sx[ly + i][lx + j] - mx, dy = sy[ly + i][lx + j] - my;
+        vx = fmaf(dx, dx, vx);
+        vy = fmaf(dy, dy, vy);
+        cxy = fmaf(dx, dy, cxy);
+      }
+    const float sdx = sqrtf(vx * inv_n + eps), sdy = sqrtf(vy * inv_n + eps);
+    s = cxy * inv_n / (sdx * sdy);
+    const float alpha = inv_pn / (sdx * sdy), beta = s * inv_pn / (sdx * sdx);
+    coef[(static_cast<int64_t>(b) * Ho + wy) * Wo + wx] = make_float4(alpha, my, beta, mx);
+  }
+  const float total = block_sum(s, red);
+  if (threadIdx.x == 0) part[static_cast<int64_t>(b) * gridDim.x + blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kThreads) void k_patch_pixels(const float* __restrict__ x, const float* __restrict__ y,
+                                                           int H, int W, int p, const float4* __restrict__ coef,
+                                                           const float* __restrict__ part, int nparts, float inv_p,
+                                                           float* __restrict__ score, float* __restrict__ gx) {
+  __shared__ float4 sc[kPatchSpan][kPatchSpan + 1];
+  __shared__ float red[kThreads / 64];
+  const int b = blockIdx.y;
+  if (blockIdx.x == 0) {  // the image's score: its window partials in a fixed order
+    float acc = 0.0f;
+    for (int q = threadIdx.x; q < nparts; q += kThreads) acc += part[static_cast<int64_t>(b) * nparts + q];
+    const float total = block_sum(acc, red);
+    if (threadIdx.x == 0) score[b] = total * inv_p;
+  }
+  if (gx == nullptr) return;
+  const int Ho = H - p + 1, Wo = W - p + 1;
+  const int tiles_w = (W + kPatchTile - 1) / kPatchTile;
+  const int ty = blockIdx.x / tiles_w, tx = blockIdx.x - ty * tiles_w;
+  const int y0 = ty * kPatchTile, x0 = tx * kPatchTile;
+  const int span = kPatchTile + p - 1;
+  // windows (y0 - p + 1 + r, x0 - p + 1 + c) for r, c < span
+  for (int e = threadIdx.x; e < span * span; e += kThreads) {
+    const int r = e / span, c = e - r * span;
+    const int wy = y0 - p + 1 + r, wx = x0 - p + 1 + c;
+    const bool in = wy >= 0 && wy < Ho && wx >= 0 && wx < Wo;
+    sc[r][c] = in ? coef[(static_cast<int64_t>(b) * Ho + wy) * Wo + wx] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  __syncthreads();
+  const int ly = threadIdx.x / kPatchTile, lx = threadIdx.x % kPatchTile;
+  const int py = y0 + ly, px = x0 + lx;
+  if (py >= H || px >= W) return;
+  const int64_t k = (static_cast<int64_t>(b) * H + py) * W + px;
+  const float xv = x[k], yv = y[k];
+  float g = 0.0f;
+  for (int i = 0; i < p; ++i)
+    for (int j = 0; j < p; ++j) {
+      const float4 c = sc[ly + i][lx + j];  // windows (py - p + 1 + i, px - p + 1 + j)
+      g = fmaf(c.x, yv - c.y, g);
+      g = fmaf(-c.z, xv - c.w, g);
+    }
+  gx[k] = g;
+}
+
 int status() {
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? NDRR_OK : static_cast<int>(e);
@@ -258,6 +375,34 @@ int ndrr_zncc(const float* x, const float* y, int B, int N, float eps, float* wo
   hipLaunchKernelGGL(k_zncc_part, dim3(P, B), dim3(kThreads), 0, st, x, y, workspace, N);
   hipLaunchKernelGGL(k_zncc_finish, dim3(grad_x ? P : 1, B), dim3(kThreads), 0, st, x, y, workspace, P, score,
                      grad_x, N, eps);
+  return status();
+}
+
+int64_t ndrr_patch_zncc_workspace_floats(int B, int H, int W, int patch) {
+  if (B <= 0 || patch < 1 || patch > kPatchMax || H < patch || W < patch) return 0;
+  const int64_t Ho = H - patch + 1, Wo = W - patch + 1;
+  const int64_t tiles = ((Ho + kPatchTile - 1) / kPatchTile) * ((Wo + kPatchTile - 1) / kPatchTile);
+  return static_cast<int64_t>(B) * (Ho * Wo * 4 + ((tiles + 3) / 4) * 4);
+}
+
+int ndrr_patch_zncc(const float* x, const float* y, int B, int H, int W, int patch, float eps, float* workspace,
+                    float* score, float* grad_x, void* stream) {
+  if (!x || !y || !workspace || !score || B <= 0 || H <= 0 || W <= 0 || !(eps >= 0.0f)) return NDRR_EINVAL;
+  if (patch < 1 || patch > kPatchMax || patch > H || patch > W) return NDRR_EINVAL;
+  if (reinterpret_cast<uintptr_t>(workspace) % 16 != 0) return NDRR_EINVAL;
+  if (B > 65535 || static_cast<int64_t>(B) * H * W >= (int64_t{1} << 31)) return NDRR_ERANGE;
+  const int Ho = H - patch + 1, Wo = W - patch + 1;
+  const int wtiles = ((Ho + kPatchTile - 1) / kPatchTile) * ((Wo + kPatchTile - 1) / kPatchTile);
+  const int ptiles = ((H + kPatchTile - 1) / kPatchTile) * ((W + kPatchTile - 1) / kPatchTile);
+  float4* coef = reinterpret_cast<float4*>(workspace);
+  float* part = workspace + static_cast<int64_t>(B) * Ho * Wo * 4;
+  const float n = static_cast<float>(patch * patch);
+  const float P = static_cast<float>(Ho) * static_cast<float>(Wo);
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_patch_windows, dim3(wtiles, B), dim3(kThreads), 0, st, x, y, H, W, patch, eps,
+                     1.0f / (P * n), coef, part);
+  hipLaunchKernelGGL(k_patch_pixels, dim3(grad_x ? ptiles : 1, B), dim3(kThreads), 0, st, x, y, H, W, patch,
+                     coef, part, wtiles, 1.0f / P, score, grad_x);
   return status();
 }
 

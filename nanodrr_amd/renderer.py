@@ -89,15 +89,7 @@ def render(
     Gradients flow to `rt_inv`, `k_inv`, `sdd`, `src`, `tgt` and, when it
     requires grad, the volume.
     """
-    if n_samples < 2:
-        raise ValueError("n_samples must be at least 2")
-    if backend not in _BACKENDS:
-        if backend == "torch":
-            raise ValueError(
-                "backend='torch' (the grid_sample path) is not part of nanodrr_amd; "
-                "use backend='hip' (or 'auto')"
-            )
-        raise ValueError(f"Unknown backend {backend!r}; expected 'auto', 'hip' or 'triton'")
+    check_render_args(n_samples, backend)
     device, dtype = rt_inv.device, rt_inv.dtype
     if tgt is None and src is None and _native_rays(k_inv, sdd, dtype):
         tgt, src, step_size = eager.make_rays(k_inv, sdd, height, width, orthographic, n_samples)
@@ -108,6 +100,19 @@ def render(
             src = _make_src(orthographic, tgt, sdd)
         step_size = (tgt - src).norm(dim=-1) / float(n_samples - 1)
     return render_hip(subject, rt_inv, src, tgt, step_size, n_samples, height, width)
+
+
+def check_render_args(n_samples: int, backend: str) -> None:
+    """The argument errors of render (renderer.py:71-72,98), raised before anything runs."""
+    if n_samples < 2:
+        raise ValueError("n_samples must be at least 2")
+    if backend not in _BACKENDS:
+        if backend == "torch":
+            raise ValueError(
+                "backend='torch' (the grid_sample path) is not part of nanodrr_amd; "
+                "use backend='hip' (or 'auto')"
+            )
+        raise ValueError(f"Unknown backend {backend!r}; expected 'auto', 'hip' or 'triton'")
 
 
 def _native_rays(k_inv, sdd, dtype) -> bool:

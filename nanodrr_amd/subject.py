@@ -1,157 +1,165 @@
-"""CT subject: attenuation volume, optional label map and their affines.
+"""CT subject: attenuation volume, optional label map and the affines between
+world, voxel-index and grid_sample coordinates.
 
-Mirrors nanodrr's `Subject` (src/nanodrr/data/subject.py:43-242) and
-`hu_to_mu` (src/nanodrr/data/preprocess.py:9-50): same constructor, buffers,
-lazily cached HU -> mu image and `align_corners=False` voxel-to-grid affine.
-File loading through torchio (subject.py:139-209) is replaced by
-`Subject.from_volume`, which builds the affines of an axis-aligned volume from
-its spacing. The kernels read the label map as int8; `label_i8` caches that
-copy per device.
+The public surface is nanodrr's `Subject` (src/nanodrr/data/subject.py:43-242)
+and `hu_to_mu` (src/nanodrr/data/preprocess.py:9-50): the constructor
+arguments, the buffer names (so a state_dict loads in either package), the
+HU -> mu parameters whose change recomputes `image`, `set_mu`, `n_classes` and
+the `align_corners=False` voxel-to-grid affine. File loading through torchio
+(subject.py:139-209) is replaced by `Subject.from_volume`, which builds the
+affines of an axis-aligned volume from its spacing.
+
+Everything the kernels read is derived from those buffers on first use and
+kept in one `Derived` table (nanodrr_amd._memo): the mu image, the float64
+world -> voxel affine, the int8 class map and the packed volume and label
+layouts. An entry stays valid while its source is the same tensor object at
+the same version, so replacing a buffer, an in-place edit, a device move or a
+change of the HU -> mu parameters makes the next read rebuild it.
 """
 
 from __future__ import annotations
 
 import torch
 
+from ._memo import Derived
+
 MU_WATER = 0.0192
 MU_BONE = 0.0573
 HU_BONE = 1000.0
 
-_MU_FIELDS = frozenset({"convert_to_mu", "mu_water", "mu_bone", "hu_bone"})
+_MU_FIELDS = ("convert_to_mu", "mu_water", "mu_bone", "hu_bone")
+_MU_DEFAULTS = {"convert_to_mu": True, "mu_water": MU_WATER, "mu_bone": MU_BONE, "hu_bone": HU_BONE}
 
 
 def hu_to_mu(data, mu_water=MU_WATER, mu_bone=MU_BONE, hu_bone=HU_BONE):
-    """Piecewise-linear HU -> linear attenuation (1/mm): air-water below 0 HU,
-    water-bone above, after clamping HU >= -1000; result clamped >= 0 (preprocess.py:44-50)."""
-    hu = data.clamp(min=-1000.0)
-    mu_low = mu_water * (1.0 + hu / 1000.0)
-    mu_high = mu_water + (hu / hu_bone) * (mu_bone - mu_water)
-    return torch.where(hu <= 0, mu_low, mu_high).clamp(min=0.0)
+    """Hounsfield units -> linear attenuation (1/mm) (preprocess.py:9-50).
+
+    HU is clamped at -1000 (air). The map is linear on each side of water
+    (0 HU -> mu_water): air to water below, mu_water / 1000 per HU; water to
+    bone above, (mu_bone - mu_water) / hu_bone per HU. The result is clamped
+    at 0.
+    """
+    hu = torch.clamp(data, min=-1000.0)
+    below = mu_water * (1.0 + hu / 1000.0)
+    above = mu_water + (mu_bone - mu_water) * (hu / hu_bone)
+    return torch.clamp(torch.where(hu > 0, above, below), min=0.0)
 
 
-class _CachedParam:
-    """Scalar that invalidates the cached mu image when it changes (subject.py:18-40)."""
+def _mu_property(name: str):
+    """A HU -> mu parameter: changing its value drops the derived mu image (and
+    what was made from it) and any `set_mu` override."""
+    slot = "_" + name
 
-    def __init__(self, default) -> None:
-        self.default = default
+    def fget(self):
+        return self.__dict__.get(slot, _MU_DEFAULTS[name])
 
-    def __set_name__(self, owner, name) -> None:
-        self.name = f"_{name}"
+    def fset(self, value):
+        if self.__dict__.get(slot, _MU_DEFAULTS[name]) != value:
+            self.__dict__[slot] = value
+            self._mu_changed()
 
-    def __get__(self, obj, objtype=None):
-        if obj is None:
-            return self
-        return getattr(obj, self.name, self.default)
-
-    def __set__(self, obj, value) -> None:
-        if getattr(obj, self.name, self.default) != value:
-            setattr(obj, self.name, value)
-            if hasattr(obj, "_image_mu_cache"):
-                obj._image_mu_cache = None
+    return property(fget, fset, doc=f"HU -> mu parameter `{name}`; setting a new value recomputes `image`.")
 
 
 class Subject(torch.nn.Module):
-    convert_to_mu: bool = _CachedParam(True)  # type: ignore[assignment]
-    mu_water: float = _CachedParam(MU_WATER)  # type: ignore[assignment]
-    mu_bone: float = _CachedParam(MU_BONE)  # type: ignore[assignment]
-    hu_bone: float = _CachedParam(HU_BONE)  # type: ignore[assignment]
+    convert_to_mu = _mu_property("convert_to_mu")
+    mu_water = _mu_property("mu_water")
+    mu_bone = _mu_property("mu_bone")
+    hu_bone = _mu_property("hu_bone")
 
     def __init__(self, imagedata, labeldata, voxel_to_world, world_to_voxel, voxel_to_grid, isocenter,
                  max_label=None, **mu) -> None:
-        unknown = mu.keys() - _MU_FIELDS
-        if unknown:
-            raise TypeError(f"Unknown mu param(s): {', '.join(repr(k) for k in unknown)}")
+        bad = sorted(set(mu) - set(_MU_FIELDS))
+        if bad:
+            raise TypeError(f"Unknown mu param(s): {', '.join(repr(k) for k in bad)}")
         super().__init__()
-        self.register_buffer("_image_hu", imagedata)
-        self.register_buffer("_image_mu_cache", None, persistent=False)
-        self.register_buffer("label", labeldata)
-        self.register_buffer("world_to_grid", voxel_to_grid @ world_to_voxel)
-        self.register_buffer("isocenter", isocenter)
-        self.register_buffer("voxel_to_world", voxel_to_world)
-        self.register_buffer("world_to_voxel", world_to_voxel)
-        self.register_buffer("voxel_to_grid", voxel_to_grid)
-        for k, v in mu.items():
-            setattr(self, k, v)
-        self.n_classes = int(max_label + 1) if max_label is not None else int(self.label.max().item()) + 1
-        self._label_i8_key = None
-        self._label_i8 = None
-        self._packed_key = None
-        self._packed = None
-        self._packed_lab_key = None
-        self._packed_lab = None
+        self._derived = Derived()
+        self._mu_override = None  # (token, mu image) after set_mu, until a parameter changes
+        for name, t in (("_image_hu", imagedata), ("label", labeldata),
+                        ("world_to_grid", voxel_to_grid @ world_to_voxel), ("isocenter", isocenter),
+                        ("voxel_to_world", voxel_to_world), ("world_to_voxel", world_to_voxel),
+                        ("voxel_to_grid", voxel_to_grid)):
+            self.register_buffer(name, t)
+        for name in _MU_FIELDS:
+            if name in mu:
+                setattr(self, name, mu[name])
+        top = max_label if max_label is not None else self.label.max().item()
+        self.n_classes = int(top) + 1
 
-    # -- volume ---------------------------------------------------------------
+    # -- HU -> mu ---------------------------------------------------------------
+    def _mu_changed(self) -> None:
+        self._mu_override = None
+        self._derived.drop("mu")
+        self._derived.drop_prefix("volume:")
+
     @property
     def image(self) -> torch.Tensor:
-        """(1, 1, D, H, W) attenuation (or raw values when convert_to_mu is False), cached."""
+        """(1, 1, D, H, W) attenuation (the stored values when convert_to_mu is False)."""
         if not self.convert_to_mu:
             return self._image_hu
-        if self._image_mu_cache is None:
-            self._image_mu_cache = hu_to_mu(self._image_hu, self.mu_water, self.mu_bone, self.hu_bone)
-        return self._image_mu_cache
+        if self._mu_override is not None:
+            return self._mu_override[1]
+        params = (self.mu_water, self.mu_bone, self.hu_bone)
+        if torch.compiler.is_compiling():  # traced graphs convert (the table is Python state)
+            return hu_to_mu(self._image_hu, *params)
+        return self._derived.get("mu", self._image_hu, params, lambda: hu_to_mu(self._image_hu, *params))
 
     def set_mu(self, mu_water=None, mu_bone=None, hu_bone=None) -> None:
-        """Recompute the mu cache from new conversion parameters (subject.py:118-137)."""
-        self._image_mu_cache = hu_to_mu(
-            self._image_hu,
-            mu_water if mu_water is not None else self.mu_water,
-            mu_bone if mu_bone is not None else self.mu_bone,
-            hu_bone if hu_bone is not None else self.hu_bone,
-        )
+        """Render with other HU -> mu parameters (floats or tensors, e.g. ones being
+        optimised) without changing the stored ones (subject.py:118-137). The image
+        made here stays in use until a stored parameter changes or the subject moves."""
+        args = (self.mu_water if mu_water is None else mu_water,
+                self.mu_bone if mu_bone is None else mu_bone,
+                self.hu_bone if hu_bone is None else hu_bone)
+        self._derived.drop_prefix("volume:")
+        self._mu_override = (object(), hu_to_mu(self._image_hu, *args))
 
-    def to(self, *args, **kwargs) -> "Subject":
-        had_cache = self._image_mu_cache is not None
-        result = super().to(*args, **kwargs)
-        if had_cache:
-            result._image_mu_cache = hu_to_mu(result._image_hu, result.mu_water, result.mu_bone, result.hu_bone)
-        result._label_i8_key = None
-        result._label_i8 = None
-        result._packed_key = None
-        result._packed = None
-        result._packed_lab_key = None
-        result._packed_lab = None
-        return result
+    def _apply(self, fn, *args, **kwargs):
+        # .to(), .cuda(), .half(), ...: buffers become new tensors, so every derived
+        # tensor (which still references the old ones) is dropped; a set_mu override
+        # does not survive the move (the stored parameters apply again).
+        self._derived.drop()
+        self._mu_override = None
+        return super()._apply(fn, *args, **kwargs)
 
     def with_volume_dtype(self, dtype: torch.dtype) -> "Subject":
-        """Store the attenuation volume in `dtype` (e.g. bfloat16) and keep the
-        affines and labels untouched (torch's `.to(dtype)` would also round the
-        affines, costing sampling accuracy)."""
-        image = self.image.detach().to(dtype)
-        out = Subject(image, self.label, self.voxel_to_world, self.world_to_voxel, self.voxel_to_grid,
-                      self.isocenter, max_label=self.n_classes - 1, convert_to_mu=False)
-        return out
+        """A subject holding `image` in `dtype` (e.g. bfloat16) as its stored values
+        (convert_to_mu=False), with the affines and labels untouched (torch's
+        `.to(dtype)` would also round the affines, costing sampling accuracy)."""
+        return Subject(self.image.detach().to(dtype), self.label, self.voxel_to_world, self.world_to_voxel,
+                       self.voxel_to_grid, self.isocenter, max_label=self.n_classes - 1, convert_to_mu=False)
 
+    # -- what the kernels read ----------------------------------------------------
     def world_to_voxel_f64(self) -> torch.Tensor:
-        """`world_to_voxel` in float64 (the camera -> voxel composition runs in FP64),
-        cached until the buffer changes."""
+        """`world_to_voxel` in float64 (the camera -> voxel composition runs in FP64)."""
         w = self.world_to_voxel
         if w.dtype == torch.float64 or torch.compiler.is_compiling():
             return w.to(torch.float64).contiguous()
-        key = (w.data_ptr(), w.device, w._version)
-        if getattr(self, "_w2v64_key", None) != key:
-            self._w2v64 = w.to(torch.float64).contiguous()
-            self._w2v64_key = key
-        return self._w2v64
+        return self._derived.get("w2v64", w, None, lambda: w.to(torch.float64).contiguous())
 
     def packed_volume(self, layout: str) -> torch.Tensor:
-        """`image` re-laid out for the kernels ("quad" or "stencil", see
-        ndrr_pack_volume), cached until the image tensor or the layout changes."""
+        """`image` re-laid out for the kernels ("quad" or "stencil", see ndrr_pack_volume)."""
         from ._ops import pack_volume
 
         img = self.image
-        if torch.compiler.is_compiling():  # traced graphs re-pack (the cache is Python state)
+        if torch.compiler.is_compiling():  # traced graphs re-pack
             return pack_volume(img[0, 0], layout)
-        key = (img.data_ptr(), img.device, img.dtype, img._version, tuple(img.shape), layout)
-        if self._packed_key != key:
-            self._packed = None  # free the old copy first
-            self._packed = pack_volume(img[0, 0], layout)
-            self._packed_key = key
-        return self._packed
+        return self._derived.get("volume:" + layout, img, None, lambda: pack_volume(img[0, 0], layout))
 
-    # -- labels ---------------------------------------------------------------
+    @property
+    def label_i8(self) -> torch.Tensor:
+        """(D, H, W) int8 class map of the kernels: trunc(label) when it lies in
+        [0, n_classes), else -1 (dropped)."""
+        from ._ops import labels_to_i8
+
+        lab = self.label
+        if torch.compiler.is_compiling():
+            return labels_to_i8(lab[0, 0], self.n_classes)
+        return self._derived.get("label_i8", lab, self.n_classes, lambda: labels_to_i8(lab[0, 0], self.n_classes))
+
     def packed_labels(self, layout: str) -> torch.Tensor:
-        """`label_i8` in the form `layout`'s march reads (label quads for "quad",
-        see ndrr_pack_labels), cached until the labels or the layout change."""
+        """`label_i8` in the form `layout`'s march reads (label quads for "quad", see
+        ndrr_pack_labels; the map itself otherwise)."""
         from ._ops import pack_labels
 
         if layout != "quad":
@@ -159,38 +167,20 @@ class Subject(torch.nn.Module):
         lab = self.label_i8
         if torch.compiler.is_compiling():
             return pack_labels(lab, layout)
-        key = (lab.data_ptr(), self._label_i8_key, layout)
-        if self._packed_lab_key != key:
-            self._packed_lab = None
-            self._packed_lab = pack_labels(lab, layout)
-            self._packed_lab_key = key
-        return self._packed_lab
+        return self._derived.get("labels:" + layout, lab, None, lambda: pack_labels(lab, layout))
 
-    @property
-    def label_i8(self) -> torch.Tensor:
-        """(D, H, W) int8 class map used by the kernels: trunc(label) when it
-        lies in [0, n_classes), else -1 (dropped). Cached until label changes."""
-        from ._ops import labels_to_i8
-
-        lab = self.label
-        if torch.compiler.is_compiling():  # traced graphs convert (the cache is Python state)
-            return labels_to_i8(lab[0, 0], self.n_classes)
-        key = (lab.data_ptr(), lab.device, lab._version, tuple(lab.shape), self.n_classes)
-        if self._label_i8_key != key:
-            self._label_i8 = labels_to_i8(lab[0, 0], self.n_classes)
-            self._label_i8_key = key
-        return self._label_i8
-
-    # -- construction -----------------------------------------------------------
+    # -- construction ---------------------------------------------------------------
     @staticmethod
     def _make_voxel_to_grid(shape) -> torch.Tensor:
-        """Voxel index -> grid_sample(align_corners=False) coordinate: 2 i / S + 1 / S - 1 (subject.py:218-242)."""
+        """Voxel index -> grid_sample(align_corners=False) coordinate per axis (x: W, y: H,
+        z: D): g = (2 i + 1) / n - 1, so the voxel extent [-1/2, n - 1/2] spans [-1, 1]
+        (subject.py:218-242)."""
         *_, D, H, W = shape
-        size = torch.tensor([W, H, D], dtype=torch.float32)
-        mat = torch.eye(4, dtype=torch.float32)
-        mat[:3, :3] = torch.diag(2.0 / size)
-        mat[:3, 3] = 1.0 / size - 1.0
-        return mat
+        n = torch.tensor([W, H, D], dtype=torch.float32)
+        out = torch.eye(4, dtype=torch.float32)
+        out[0, 0], out[1, 1], out[2, 2] = 2.0 / n
+        out[:3, 3] = 1.0 / n - 1.0
+        return out
 
     @classmethod
     def from_volume(cls, image, spacing=(1.0, 1.0, 1.0), label=None, origin=None, max_label=None, **mu) -> "Subject":

@@ -17,8 +17,9 @@ Modules
   geometry   float64 numpy camera/pose helpers (intrinsics.py:53-68,
              extrinsics.py:7-57, se3.py:179-221).
   registration float64 numpy pose step of the registration loop
-             (registration.py:64-68) with its adjoint, and ZNCC with its
-             gradient (metrics/ncc.py:8-37).
+             (registration.py:64-68) with its adjoint, ZNCC with its
+             gradient (metrics/ncc.py:8-37) and the window (patch) ZNCC
+             built from it (metrics/ncc.py:83-88).
 
 Parity pinning (tests/test_oracle.py, tests/test_geometry.py, tests/test_registration.py):
   raymarch.forward     unit-impulse known answer (tests/drr/test_renderer.py:52-91),
@@ -33,6 +34,8 @@ Parity pinning (tests/test_oracle.py, tests/test_geometry.py, tests/test_registr
   registration         scipy Rotation.from_rotvec, torch float64 autograd of
                        matrix_exp, central finite differences (pose step);
                        scipy.stats.pearsonr, affine invariance, finite
-                       differences (ZNCC)
+                       differences (ZNCC); one full-size window equals the
+                       whole-image ZNCC, a brute-force loop over windows and
+                       finite differences (window ZNCC)
 No function here is "parity unpinned".
 """

@@ -89,3 +89,24 @@ def zncc(x, y, eps: float = 1e-7):
     # dsx/dx_j = dx_j / (n sx), so dscore/dx_j = (yh_j - score xh_j) / (n sx).
     grad = (yh - score[:, None] * xh) / (n * sx)
     return score, grad
+
+
+def patch_zncc(x, y, patch: int, eps: float = 1e-7):
+    """(score (B,), grad_x (B, H, W)): the mean over every patch x patch window (stride 1) of
+    (B, H, W) images of the windows' zncc (the patch form of ncc.py:21-24,83-88: each unfolded
+    window standardised on its own), and its gradient, the windows' gradients scattered back
+    and averaged."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    B, H, W = x.shape
+    ho, wo = H - patch + 1, W - patch + 1
+    score = np.zeros(B)
+    grad = np.zeros((B, H, W))
+    for b in range(B):
+        wx = np.stack([x[b, i:i + patch, j:j + patch].ravel() for i in range(ho) for j in range(wo)])
+        wy = np.stack([y[b, i:i + patch, j:j + patch].ravel() for i in range(ho) for j in range(wo)])
+        s, g = zncc(wx, wy, eps)
+        score[b] = s.mean()
+        for k, (i, j) in enumerate((i, j) for i in range(ho) for j in range(wo)):
+            grad[b, i:i + patch, j:j + patch] += g[k].reshape(patch, patch) / (ho * wo)
+    return score, grad

@@ -218,3 +218,13 @@ def test_noalloc_contraction_rejects_oversized_work_on_the_host():
     assert lib.ndrr_raymarch_bwd_jac_pose_noalloc(1, 0, 1, 7, None, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 400, 1, 20,
                                                   0, 0, 0, 0, None) == -1
     assert b"scratch" in lib.ndrr_strerror(-5)
+
+
+def test_patch_zncc_rejects_bad_arguments_before_launch():
+    lib = _lib.lib()
+    assert lib.ndrr_patch_zncc_workspace_floats(2, 20, 20, 5) == 2 * (16 * 16 * 4 + 4)
+    assert lib.ndrr_patch_zncc_workspace_floats(2, 20, 20, 33) == 0
+    assert lib.ndrr_patch_zncc(1, 1, 2, 20, 20, 33, 1e-4, 16, 1, 1, None) == -1  # window beyond 32
+    assert lib.ndrr_patch_zncc(1, 1, 2, 4, 20, 5, 1e-4, 16, 1, 1, None) == -1   # window taller than the image
+    assert lib.ndrr_patch_zncc(1, 1, 2, 20, 20, 5, 1e-4, 8, 1, 1, None) == -1   # misaligned workspace
+    assert lib.ndrr_patch_zncc(1, 1, 70000, 20, 20, 5, 1e-4, 16, 1, 1, None) == -4

@@ -146,22 +146,37 @@ def test_zncc_grad_matches_finite_differences(eps):
     np.testing.assert_allclose(g.sum(axis=1), 0.0, atol=1e-12)
 
 
-def test_patch_ncc_matches_oracle_per_patch():
-    """The patch path of NormalizedCrossCorrelation2d (ncc.py:8-37, 85-90) is the mean over
-    every patch_size^2 window of the ZNCC of the two windows."""
-    from nanodrr_amd.metrics import NormalizedCrossCorrelation2d
-
-    torch.manual_seed(0)
-    x = torch.rand(2, 1, 9, 8, dtype=torch.float64)
-    y = torch.rand(2, 1, 9, 8, dtype=torch.float64) + x
-    got = NormalizedCrossCorrelation2d(patch_size=4, eps=1e-7)(x, y).numpy()
-    xn, yn = x.numpy()[:, 0], y.numpy()[:, 0]
-    ref = []
+def test_oracle_patch_zncc_is_pinned():
+    """The window ZNCC of the oracle: one window covering the image is the whole-image ZNCC;
+    the score is the mean of a brute-force loop over windows; the gradient matches central
+    finite differences."""
+    rng = np.random.default_rng(3)
+    x = rng.random((2, 7, 6))
+    y = rng.random((2, 7, 6)) + 0.5 * x
+    s_full, g_full = oreg.patch_zncc(x[:, :6, :6], y[:, :6, :6], 6, 1e-7)
+    s_img, g_img = oreg.zncc(x[:, :6, :6], y[:, :6, :6], 1e-7)
+    np.testing.assert_allclose(s_full, s_img, rtol=1e-13)
+    np.testing.assert_allclose(g_full.reshape(2, -1), g_img, rtol=1e-12, atol=1e-15)
+    s, g = oreg.patch_zncc(x, y, 3, 1e-4)
     for b in range(2):
-        wins_x = [xn[b, i:i + 4, j:j + 4].ravel() for i in range(6) for j in range(5)]
-        wins_y = [yn[b, i:i + 4, j:j + 4].ravel() for i in range(6) for j in range(5)]
-        ref.append(oreg.zncc(np.stack(wins_x), np.stack(wins_y), 1e-7)[0].mean())
-    np.testing.assert_allclose(got, ref, rtol=1e-10)
+        vals = []
+        for i in range(5):
+            for j in range(4):
+                vals.append(pearson_eps(x[b, i:i + 3, j:j + 3].ravel(), y[b, i:i + 3, j:j + 3].ravel(), 1e-4))
+        assert s[b] == pytest.approx(np.mean(vals), rel=1e-12)
+    h = 1e-6
+    for (b, i, j) in ((0, 0, 0), (1, 3, 2), (0, 6, 5), (1, 2, 5)):
+        xp, xm = x.copy(), x.copy()
+        xp[b, i, j] += h
+        xm[b, i, j] -= h
+        fd = (oreg.patch_zncc(xp, y, 3, 1e-4)[0][b] - oreg.patch_zncc(xm, y, 3, 1e-4)[0][b]) / (2 * h)
+        assert g[b, i, j] == pytest.approx(fd, rel=1e-6, abs=1e-9)
+
+
+def pearson_eps(a, b, eps):
+    """ZNCC of two vectors with eps added to each population variance, from numpy's covariance."""
+    c = np.cov(a, b, bias=True)
+    return c[0, 1] / np.sqrt((c[0, 0] + eps) * (c[1, 1] + eps))
 
 
 def test_double_geodesic_known_answers():

@@ -103,10 +103,10 @@ def test_zncc_deterministic_and_matches_reference_module():
     y = torch.rand(4, 1, 64, 48, generator=g).to(DEV) + x
     a, b = zncc(x, y), zncc(x, y)
     assert torch.equal(a, b)
-    # the torch statement of ncc.py:8-37 (population variance, eps in the square root)
+    # ncc.py:8-37 (population variance, eps in the square root), as the float64 oracle states it
     mod = NormalizedCrossCorrelation2d(patch_size=None, eps=1e-7)
-    ref = (mod._norm(x.double()) * mod._norm(y.double())).mean(dim=(1, 2, 3))
-    torch.testing.assert_close(mod(x, y).double(), ref, rtol=0, atol=2e-6)
+    ref = torch.from_numpy(oreg.zncc(x.cpu().numpy(), y.cpu().numpy(), 1e-7)[0])
+    torch.testing.assert_close(mod(x, y).double().cpu(), ref, rtol=0, atol=2e-6)
 
 
 def _phantom_registration(B=4, size=64, spacing=4.0, h=64, w=64, seed=0, per_pose=True):
