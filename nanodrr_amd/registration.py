@@ -24,6 +24,7 @@ import math
 import torch
 
 from . import _lib
+from ._memo import Derived
 from .geometry import convert, transform_point
 from ._ops import _eager, _fused_raymarch_bwd_jac_impl, _fused_raymarch_jac_impl
 from .metrics import _zncc_impl, zncc
@@ -172,12 +173,17 @@ class Registration(torch.nn.Module):
                      self.width, offset=offset)
 
     def _ray_steps(self, n_samples: int) -> torch.Tensor:
-        """The ray step (1, N) for n_samples, kept until the rays change."""
-        key = (n_samples, self._tgt.data_ptr(), self._tgt._version)
-        if getattr(self, "_steps_key", None) != key or torch.compiler.is_compiling():
-            self._steps = (self._tgt - self._src).norm(dim=-1) / float(n_samples - 1)
-            self._steps_key = key
-        return self._steps
+        """The ray step (1, N) for n_samples, kept until the rays change (the same tensors at the
+        same versions: a replaced or edited ray buffer is seen)."""
+        def make():
+            return (self._tgt - self._src).norm(dim=-1) / float(n_samples - 1)
+
+        if torch.compiler.is_compiling():
+            return make()
+        if not hasattr(self, "_derived"):
+            self._derived = Derived()
+        inner = self._derived.get("tgt", self._tgt, None, Derived)
+        return inner.get(f"src:{n_samples}", self._src, None, make)
 
     @property
     def pose(self) -> torch.Tensor:
