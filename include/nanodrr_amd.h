@@ -23,11 +23,12 @@
  *   out  [B][C][N] fp32 line integrals
  *   vol  the volume in NDRR_F32 / NDRR_BF16 / NDRR_F16, laid out as
  *        NDRR_LAYOUT_LINEAR:  [D][H][W], W fastest, or a packed copy made by
- *        ndrr_pack_volume: NDRR_LAYOUT_QUAD or NDRR_LAYOUT_STENCIL records
+ *        ndrr_pack_volume / ndrr_pack_coef: NDRR_LAYOUT_QUAD, NDRR_LAYOUT_STENCIL
+ *        or NDRR_LAYOUT_COEF records (vol_dtype is then the record type)
  *   lab  int8 class per voxel (values outside [0, C) are dropped) in the form
- *        vol_layout reads: the [D][H][W] map for NDRR_LAYOUT_LINEAR and
- *        NDRR_LAYOUT_STENCIL, its label quads (ndrr_pack_labels) for
- *        NDRR_LAYOUT_QUAD; ignored when n_classes == 1
+ *        vol_layout reads: the [D][H][W] map for NDRR_LAYOUT_LINEAR,
+ *        NDRR_LAYOUT_STENCIL and NDRR_LAYOUT_COEF, its label quads
+ *        (ndrr_pack_labels) for NDRR_LAYOUT_QUAD; ignored when n_classes == 1
  * Sample s of ray (b, n) sits at M[b] @ [lerp(src, tgt, s / (S - 1)), 1].
  * Rays are mapped to detector tiles of a `width`-wide image when width > 0
  * (N must be a multiple of width), else to runs of consecutive rays; see
@@ -54,7 +55,8 @@ enum {
 enum {
   NDRR_LAYOUT_LINEAR = 0,  /* [D][H][W] elements */
   NDRR_LAYOUT_STENCIL = 1, /* one record of 8 trilinear corners per cell */
-  NDRR_LAYOUT_QUAD = 2     /* one record of a (y, z) corner quad per x */
+  NDRR_LAYOUT_QUAD = 2,    /* one record of a (y, z) corner quad per x */
+  NDRR_LAYOUT_COEF = 3     /* one record of 8 multilinear coefficients per cell */
 };
 
 /* Argument errors (negative return codes). */
@@ -272,15 +274,35 @@ int ndrr_raymarch_bwd_jac_pose_noalloc(const float* go, int64_t go_batch_stride,
  *   NDRR_LAYOUT_STENCIL: one record per cell (x0, y0, z0), x0 in [-1, W-1] etc.,
  *     holding its eight corners in the order (z, y, x) = 000, 001, 010, 011,
  *     100, 101, 110, 111. 8 copies of each voxel.
+ *   NDRR_LAYOUT_COEF: a 256-byte header, then one record per cell, on the
+ *     stencil's grid, holding the coefficients of the cell's trilinear interpolant
+ *       v(f) = c0 + cx fx + cy fy + cxy fx fy + cz fz + cxz fx fz + cyz fy fz + cxyz fx fy fz
+ *     in that order (zero-padded corners vzyx: c0 = v000, cx = v001 - v000,
+ *     cy = v010 - v000, cz = v100 - v000, cxy = (v011 - v010) - cx,
+ *     cxz = (v101 - v100) - cx, cyz = (v110 - v100) - cy,
+ *     cxyz = ((v111 - v110) - (v101 - v100)) - ((v011 - v010) - cx), each in fp32
+ *     in that order). fp32 storage keeps them as they are (32-byte records).
+ *     16-bit storage keeps s times each, rounded once (16-byte records), with s
+ *     the largest power of two such that 8 max|v| s <= 65504 (so no
+ *     coefficient overflows fp16 and the differences stay clear of its
+ *     subnormals): NDRR_BF16 holds s c0 in bf16 and the seven differences in
+ *     fp16, NDRR_F16 all eight in fp16. The header holds, as 32-bit words, the
+ *     fp32 bits of max|v|, s, 1/s and the record type. The edge differences
+ *     behind the gradient are so rounded relative to their own size, not to the
+ *     voxel values.
  * ndrr_packed_bytes gives the size of `out` (0 for invalid arguments; for
  * NDRR_LAYOUT_LINEAR the volume's own size); ndrr_pack_volume fills it from the
- * linear volume on `stream`.
+ * linear volume on `stream`, in records of the volume's type.
+ * ndrr_pack_coef fills NDRR_LAYOUT_COEF records of record_dtype from a volume
+ * of vol_dtype (e.g. bf16 records from the fp32 volume, rounded once).
  * Replaces the per-sample gathers of _fused.py:205-212 (_sample, :11-16) with
  * a one-off re-layout of the subject's volume.
  */
 size_t ndrr_packed_bytes(int D, int H, int W, int vol_dtype, int vol_layout);
 int ndrr_pack_volume(const void* vol, int vol_dtype, int vol_layout, int D, int H, int W,
                      void* out, void* stream);
+int ndrr_pack_coef(const void* vol, int vol_dtype, int record_dtype, int D, int H, int W,
+                   void* out, void* stream);
 
 /*
  * Labels in the form a layout's march reads. For NDRR_LAYOUT_QUAD: one uint32

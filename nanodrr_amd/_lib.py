@@ -13,8 +13,8 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().with_name("libnanodrr_amd.so")
 
 NDRR_F32, NDRR_BF16, NDRR_F16 = 0, 1, 2
-LAYOUT_LINEAR, LAYOUT_STENCIL, LAYOUT_QUAD = 0, 1, 2
-LAYOUTS = {"linear": LAYOUT_LINEAR, "stencil": LAYOUT_STENCIL, "quad": LAYOUT_QUAD}
+LAYOUT_LINEAR, LAYOUT_STENCIL, LAYOUT_QUAD, LAYOUT_COEF = 0, 1, 2, 3
+LAYOUTS = {"linear": LAYOUT_LINEAR, "stencil": LAYOUT_STENCIL, "quad": LAYOUT_QUAD, "coef": LAYOUT_COEF}
 MAX_CLASSES = 8
 
 _c_int, _c_i64, _c_p, _c_size = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_size_t
@@ -78,6 +78,7 @@ SIGNATURES: dict[str, tuple] = {
     "ndrr_labels_to_i8": (_c_int, [_c_p, _c_p, _c_i64, _c_int, _c_p]),
     "ndrr_packed_bytes": (_c_size, [_c_int, _c_int, _c_int, _c_int, _c_int]),
     "ndrr_pack_volume": (_c_int, [_c_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p]),
+    "ndrr_pack_coef": (_c_int, [_c_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p]),
     "ndrr_packed_label_bytes": (_c_size, [_c_int, _c_int, _c_int, _c_int]),
     "ndrr_pack_labels": (_c_int, [_c_p, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p]),
     "ndrr_make_rays": (_c_int, [_c_p, _c_i64, _c_p, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p, _c_p,

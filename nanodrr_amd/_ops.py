@@ -46,7 +46,7 @@ def _volume_arg(vol, packed, layout):
         return vol.data_ptr(), _lib.LAYOUT_LINEAR
     code = _lib.LAYOUTS.get(layout)
     if code is None or code == _lib.LAYOUT_LINEAR:
-        raise ValueError(f"layout must name the packed copy ('quad' or 'stencil'), got {layout!r}")
+        raise ValueError(f"layout must name the packed copy ('quad', 'stencil' or 'coef'), got {layout!r}")
     D, H, W = vol.shape
     need = _lib.lib().ndrr_packed_bytes(D, H, W, _DTYPE_CODE[vol.dtype], code)
     if packed.dtype != torch.uint8 or packed.numel() != need or not packed.is_contiguous() \
@@ -73,12 +73,44 @@ def pack_volume(vol: torch.Tensor, layout: str) -> torch.Tensor:
     return out
 
 
+COEF_HEADER_BYTES = 256  # the coefficient layout's header: max |v|, scale s, 1/s, record type
+
+
+def _packed_bytes(D, H, W, layout, dtype):
+    e = 4 if dtype == torch.float32 else 2
+    n = {"linear": D * H * W, "stencil": (D + 1) * (H + 1) * (W + 1) * 8, "quad": (D + 1) * (H + 1) * (W + 2) * 4,
+         "coef": (D + 1) * (H + 1) * (W + 1) * 8}[layout]
+    return n * e + (COEF_HEADER_BYTES if layout == "coef" else 0)
+
+
 @pack_volume.register_fake
 def _(vol, layout):
+    return torch.empty(_packed_bytes(*vol.shape, layout, vol.dtype), device=vol.device, dtype=torch.uint8)
+
+
+@torch.library.custom_op("nanodrr_amd::pack_coef", mutates_args=())
+def pack_coef(vol: torch.Tensor, record_dtype: torch.dtype) -> torch.Tensor:
+    """Coefficient records (the "coef" layout, uint8 bytes, after a 256-byte header) for `record_dtype`
+    storage from a (D, H, W) device volume of any supported type (ndrr_pack_coef): each coefficient
+    formed in fp32 and rounded once; 16-bit records scaled by a power of two, bf16 ones holding the
+    value in bf16 and the seven edge differences in fp16."""
+    if not vol.is_cuda or vol.dim() != 3 or vol.dtype not in _DTYPE_CODE or record_dtype not in _DTYPE_CODE:
+        raise ValueError("pack_coef needs a (D, H, W) float32/bfloat16/float16 device tensor and such a record type")
+    vol = vol.detach().contiguous()
     D, H, W = vol.shape
-    e = 4 if vol.dtype == torch.float32 else 2
-    n = {"linear": D * H * W, "stencil": (D + 1) * (H + 1) * (W + 1) * 8, "quad": (D + 1) * (H + 1) * (W + 2) * 4}
-    return torch.empty(n[layout] * e, device=vol.device, dtype=torch.uint8)
+    lib = _lib.lib()
+    rd = _DTYPE_CODE[record_dtype]
+    out = torch.empty(lib.ndrr_packed_bytes(D, H, W, rd, _lib.LAYOUT_COEF), device=vol.device, dtype=torch.uint8)
+    with torch.cuda.device(vol.device):
+        rc = lib.ndrr_pack_coef(vol.data_ptr(), _DTYPE_CODE[vol.dtype], rd, D, H, W, out.data_ptr(),
+                                _stream(vol.device))
+    _lib.check(rc, "ndrr_pack_coef")
+    return out
+
+
+@pack_coef.register_fake
+def _(vol, record_dtype):
+    return torch.empty(_packed_bytes(*vol.shape, "coef", record_dtype), device=vol.device, dtype=torch.uint8)
 
 
 @torch.library.custom_op("nanodrr_amd::pack_labels", mutates_args=())

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <atomic>
 #include <mutex>
 
@@ -45,6 +46,9 @@ constexpr int kWave = 64;        // rays per wave: an 8 x 8 detector patch (or 6
 #ifndef NDRR_FWD_CHUNKS_QUAD
 #define NDRR_FWD_CHUNKS_QUAD 2
 #endif
+#ifndef NDRR_FWD_CHUNKS_COEF
+#define NDRR_FWD_CHUNKS_COEF 2
+#endif
 // Single-channel renders (no label map) carry 7 sums per ray instead of up to
 // 28 and hold fewer registers, so more chunks fit; quad layout only.
 #ifndef NDRR_FWD_CHUNKS_QUAD_1CLASS
@@ -64,6 +68,9 @@ constexpr int kWave = 64;        // rays per wave: an 8 x 8 detector patch (or 6
 #ifndef NDRR_BWD_CHUNKS_QUAD
 #define NDRR_BWD_CHUNKS_QUAD 1
 #endif
+#ifndef NDRR_BWD_CHUNKS_COEF
+#define NDRR_BWD_CHUNKS_COEF 1
+#endif
 #ifndef NDRR_IN_FLIGHT
 #define NDRR_IN_FLIGHT 4
 #endif
@@ -78,11 +85,15 @@ constexpr int kTileW = 8 * kTwx, kTileH = 8 * kTwy;                // tile size 
 constexpr int kTileRays = kWave * kTwx * kTwy;
 constexpr int fwd_chunks(int layout) {
   return layout == NDRR_LAYOUT_STENCIL ? NDRR_FWD_CHUNKS_STENCIL
-                                       : (layout == NDRR_LAYOUT_QUAD ? NDRR_FWD_CHUNKS_QUAD : NDRR_FWD_CHUNKS_LINEAR);
+         : layout == NDRR_LAYOUT_QUAD  ? NDRR_FWD_CHUNKS_QUAD
+         : layout == NDRR_LAYOUT_COEF  ? NDRR_FWD_CHUNKS_COEF
+                                       : NDRR_FWD_CHUNKS_LINEAR;
 }
 constexpr int bwd_chunks(int layout) {
   return layout == NDRR_LAYOUT_STENCIL ? NDRR_BWD_CHUNKS_STENCIL
-                                       : (layout == NDRR_LAYOUT_QUAD ? NDRR_BWD_CHUNKS_QUAD : NDRR_BWD_CHUNKS_LINEAR);
+         : layout == NDRR_LAYOUT_QUAD  ? NDRR_BWD_CHUNKS_QUAD
+         : layout == NDRR_LAYOUT_COEF  ? NDRR_BWD_CHUNKS_COEF
+                                       : NDRR_BWD_CHUNKS_LINEAR;
 }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // Forward chunks for CV class slots: up to four slots only, so the LDS that
@@ -92,10 +103,10 @@ constexpr int fwd_chunks(int layout, int cv) {
   return cv == 1 && layout == NDRR_LAYOUT_QUAD ? NDRR_FWD_CHUNKS_QUAD_1CLASS : (cv <= 4 ? fwd_chunks(layout) : 1);
 }
 constexpr int kMaxFwdChunks = cmax(cmax(cmax(fwd_chunks(NDRR_LAYOUT_LINEAR), fwd_chunks(NDRR_LAYOUT_STENCIL)),
-                                        fwd_chunks(NDRR_LAYOUT_QUAD)),
+                                        cmax(fwd_chunks(NDRR_LAYOUT_QUAD), fwd_chunks(NDRR_LAYOUT_COEF))),
                                    NDRR_FWD_CHUNKS_QUAD_1CLASS);
 constexpr int kMaxBwdChunks = cmax(cmax(bwd_chunks(NDRR_LAYOUT_LINEAR), bwd_chunks(NDRR_LAYOUT_STENCIL)),
-                                   bwd_chunks(NDRR_LAYOUT_QUAD));
+                                   cmax(bwd_chunks(NDRR_LAYOUT_QUAD), bwd_chunks(NDRR_LAYOUT_COEF)));
 // Pose-gradient partials per tile of the backward kernel (one, or one per
 // wave of the layout with the most chunks): the workspace is sized for it.
 constexpr int bwd_parts_per_tile(int layout) {
@@ -148,6 +159,7 @@ struct Params {
   int64_t src_bs, src_rs, tgt_bs, step_bs;
   int64_t go_bs;  // batch stride of go in the single-pass backward (0: shared by every pose)
   uint32_t vol_bytes, lab_bytes;  // sizes of the packed volume / label buffers
+  uint32_t vol_skip;              // bytes before the first record (the coefficient layout's header)
   int B, N, D, H, W, S, C;
   int IW, IH, tiles_w, nblocks;
   float inv_s;   // 1 / (S - 1)
@@ -394,7 +406,8 @@ __device__ inline Src make_src(const Params& p) {
   Src s;
   s.vol = p.vol;
   s.lab = p.lab;
-  s.vrs = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p.vol), 0, static_cast<int>(p.vol_bytes),
+  s.vrs = __builtin_amdgcn_make_buffer_rsrc(static_cast<char*>(const_cast<void*>(p.vol)) + p.vol_skip, 0,
+                                            static_cast<int>(p.vol_bytes - p.vol_skip),
                                             static_cast<int>(kBufferFlags));
   s.lrs = __builtin_amdgcn_make_buffer_rsrc(const_cast<int8_t*>(p.lab), 0, static_cast<int>(p.lab_bytes),
                                             static_cast<int>(kBufferFlags));
@@ -606,6 +619,106 @@ __device__ inline float lerp_value_grad(const Corners& v, const Cell& c, float& 
   return fmaf(c.fz, dvz, c0);
 }
 
+// Coefficient layout: one record per cell on the stencil's grid holding the
+// coefficients of the cell's trilinear interpolant
+//   v = c0 + cx fx + cy fy + cxy fx fy + cz fz + cxz fx fz + cyz fy fz + cxyz fx fy fz
+// in that order (16 bytes for 16-bit records, 32 for fp32), after a
+// kCoefHeader-byte header. The gradient's edge differences are stored
+// themselves, so a 16-bit record rounds them relative to their own size
+// rather than to the voxel values', and value and gradient cost 11 FMAs
+// instead of the corner form's 22 operations. 16-bit records hold every
+// coefficient times a power of two s (header: s, 1/s) chosen from the
+// volume's largest magnitude so that none overflows fp16 and the differences
+// stay clear of its subnormals; bf16 records keep the value c0 in bf16 and the
+// seven differences in fp16, fp16 records all eight in fp16. The kernels
+// multiply each ray's sums by 1/s once (exact).
+constexpr uint32_t kCoefHeader = 256;
+
+struct Coef {
+  float c0, cx, cy, cxy, cz, cxz, cyz, cxyz;
+};
+
+template <int DT> struct Fetch<DT, NDRR_LAYOUT_COEF> {  // bf16 / fp16: 16-byte records
+  static constexpr int kInFlight = NDRR_IN_FLIGHT;
+  uint4 q;
+  __device__ inline void load(const Src& s, const Cell& c) { q = load16(s.vrs, record_offset(s, c, s.W + 1, 16u)); }
+  __device__ inline Coef coef() const {
+    if constexpr (DT == NDRR_BF16)
+      return {bf16_lo(q.x), f16_hi(q.x), f16_lo(q.y), f16_hi(q.y),
+              f16_lo(q.z), f16_hi(q.z), f16_lo(q.w), f16_hi(q.w)};
+    else
+      return {f16_lo(q.x), f16_hi(q.x), f16_lo(q.y), f16_hi(q.y),
+              f16_lo(q.z), f16_hi(q.z), f16_lo(q.w), f16_hi(q.w)};
+  }
+};
+
+// 1/s of the coefficient records (1 for the other layouts), read from the header.
+template <int L> __device__ inline float record_unscale(const Params& p) {
+  if constexpr (L == NDRR_LAYOUT_COEF) return static_cast<const float*>(p.vol)[2];
+  return 1.0f;
+}
+
+template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_COEF> {  // fp32: 32-byte records
+  static constexpr int kInFlight = 2;
+  uint4 a, b;
+  __device__ inline void load(const Src& s, const Cell& c) {
+    const uint32_t off = record_offset(s, c, s.W + 1, 32u);
+    a = load16(s.vrs, off);
+    b = load16(s.vrs, off + 16u);
+  }
+  __device__ inline Coef coef() const {
+    return {__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), __uint_as_float(a.w),
+            __uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w)};
+  }
+};
+
+// Value of the interpolant, Horner-style in fz, then fy and fx: 7 FMAs.
+__device__ inline float coef_value(const Coef& k, const Cell& c) {
+  const float u = fmaf(c.fy, k.cxy, k.cx);                  // cx + fy cxy
+  const float txz = fmaf(c.fx, k.cxyz, k.cyz);              // cyz + fx cxyz
+  const float dz = fmaf(c.fy, txz, fmaf(c.fx, k.cxz, k.cz));  // d/dfz
+  return fmaf(c.fz, dz, fmaf(c.fy, k.cy, fmaf(c.fx, u, k.c0)));
+}
+
+// Value and gradient (voxel units): 11 FMAs.
+__device__ inline float coef_value_grad(const Coef& k, const Cell& c, float& dvx, float& dvy, float& dvz) {
+  const float u = fmaf(c.fy, k.cxy, k.cx);      // cx + fy cxy
+  const float tyz = fmaf(c.fy, k.cxyz, k.cxz);  // cxz + fy cxyz
+  const float txz = fmaf(c.fx, k.cxyz, k.cyz);  // cyz + fx cxyz
+  dvx = fmaf(c.fz, tyz, u);
+  dvy = fmaf(c.fz, txz, fmaf(c.fx, k.cxy, k.cy));
+  dvz = fmaf(c.fy, txz, fmaf(c.fx, k.cxz, k.cz));
+  return fmaf(c.fz, dvz, fmaf(c.fy, k.cy, fmaf(c.fx, u, k.c0)));
+}
+
+// Interpolation of a fetched cell: the corner layouts interpolate their
+// corners (the lerp order of _fused.py:214-220), the coefficient layout
+// evaluates its polynomial. interp_grad is the two-pass adjoint's gradient.
+template <int DT, int L> __device__ inline float interp_value(const Fetch<DT, L>& f, const Cell& c) {
+  return lerp_value(f.corners(), c);
+}
+template <int DT> __device__ inline float interp_value(const Fetch<DT, NDRR_LAYOUT_COEF>& f, const Cell& c) {
+  return coef_value(f.coef(), c);
+}
+template <int DT, int L>
+__device__ inline float interp_value_grad(const Fetch<DT, L>& f, const Cell& c, float& dvx, float& dvy, float& dvz) {
+  return lerp_value_grad(f.corners(), c, dvx, dvy, dvz);
+}
+template <int DT>
+__device__ inline float interp_value_grad(const Fetch<DT, NDRR_LAYOUT_COEF>& f, const Cell& c, float& dvx, float& dvy,
+                                          float& dvz) {
+  return coef_value_grad(f.coef(), c, dvx, dvy, dvz);
+}
+template <int DT, int L>
+__device__ inline void interp_grad(const Fetch<DT, L>& f, const Cell& c, float& dvx, float& dvy, float& dvz) {
+  lerp_grad(f.corners(), c, dvx, dvy, dvz);
+}
+template <int DT>
+__device__ inline void interp_grad(const Fetch<DT, NDRR_LAYOUT_COEF>& f, const Cell& c, float& dvx, float& dvy,
+                                   float& dvz) {
+  coef_value_grad(f.coef(), c, dvx, dvy, dvz);
+}
+
 // Nearest label (round half to even, zero padded: outside the volume is class 0).
 __device__ inline int nearest_label(const int8_t* __restrict__ lab, int D, int H, int W,
                                     float x, float y, float z) {
@@ -719,9 +832,9 @@ void k_fwd(Params p) {
     for (int u = 0; u < B::U; ++u) {
       const Cell& c = batch.cell[u];
       const bool live = k0 + u < r.n_s;
-      const Corners v = batch.f[u].corners();
       float dvx, dvy, dvz;
-      const float val = live ? (JAC ? lerp_value_grad(v, c, dvx, dvy, dvz) : lerp_value(v, c)) : 0.0f;
+      const float val =
+          live ? (JAC ? interp_value_grad(batch.f[u], c, dvx, dvy, dvz) : interp_value(batch.f[u], c)) : 0.0f;
       float tx = 0.0f, ty = 0.0f, tz = 0.0f;
       if constexpr (JAC) {
         dvx = live ? dvx : 0.0f;
@@ -765,6 +878,7 @@ void k_fwd(Params p) {
     __syncthreads();
   }
   if (chunk != 0 || !valid) return;
+  const float unscale = record_unscale<L>(p);
   float* o = p.out + (static_cast<int64_t>(b) * p.C) * p.N + n;
 #pragma unroll
   for (int c = 0; c < CV; ++c) {
@@ -774,7 +888,7 @@ void k_fwd(Params p) {
       for (int k = 1; k < NCH; ++k) {
         if constexpr (KAHAN) comp_add(total, tcomp, part[k - 1][c][tr]); else total += part[k - 1][c][tr];
       }
-      const float sum = KAHAN ? (total + tcomp) : total;
+      const float sum = (KAHAN ? (total + tcomp) : total) * unscale;
       o[static_cast<int64_t>(c) * p.N] = sum * r.step;
       if constexpr (JAC) {
         float* jp = p.jac + (static_cast<int64_t>(b) * p.jac_q * p.C + c) * p.N + n;
@@ -786,7 +900,7 @@ void k_fwd(Params p) {
 #pragma unroll
             for (int k = 1; k < NCH; ++k) j += jpart[k - 1][6 * c + q][tr];
           }
-          jp[q * plane] = j;
+          jp[q * plane] = j * unscale;
         }
         if (p.jac_q == 7) jp[6 * plane] = sum;
       }
@@ -839,7 +953,6 @@ void k_bwd(Params p) {
     for (int u = 0; u < B::U; ++u) {
       const bool live = k0 + u < r.n_s;
       const Cell& c = batch.cell[u];
-      const Corners v = batch.f[u].corners();
       float gg;
       if constexpr (CV == 1) {
         gg = goc[0];
@@ -852,12 +965,12 @@ void k_bwd(Params p) {
       const float g = gg * r.step;
       const float fx = c.fx, fy = c.fy, fz = c.fz;
       float dvx, dvy, dvz;
-      lerp_grad(v, c, dvx, dvy, dvz);
+      interp_grad(batch.f[u], c, dvx, dvy, dvz);
       const float gx = g * dvx, gy = g * dvy, gz = g * dvz;
       const float t = fmaf(static_cast<float>(r.s_lo + k0 + u), p.inv_s, r.t_off);
       Gx += gx; Gy += gy; Gz += gz;
       Tx += gx * t; Ty += gy * t; Tz += gz * t;
-      if constexpr (GCAM) astep += gg * lerp_value(v, c);
+      if constexpr (GCAM) astep += gg * interp_value(batch.f[u], c);
       if constexpr (GVOL) {
         if (g != 0.0f) {
           const float wx[2] = {1.0f - fx, fx};
@@ -875,6 +988,13 @@ void k_bwd(Params p) {
         }
       }
     }
+  }
+
+  if constexpr (L == NDRR_LAYOUT_COEF) {
+    const float unscale = record_unscale<L>(p);
+    Gx *= unscale; Gy *= unscale; Gz *= unscale;
+    Tx *= unscale; Ty *= unscale; Tz *= unscale;
+    astep *= unscale;
   }
 
   if constexpr (GCAM) {
@@ -1175,6 +1295,116 @@ __global__ void k_pack_quad(const typename Vox<DT>::T* __restrict__ vol, void* _
   }
 }
 
+// fp32 -> the record type, rounded to nearest even once.
+template <int DT> struct Round;
+template <> struct Round<NDRR_F32> {
+  __device__ static inline uint32_t bits(float f) { return __float_as_uint(f); }
+};
+template <> struct Round<NDRR_BF16> {
+  // the upper half of the fp32 pattern, rounded to nearest even on the dropped half
+  // (NaN stays a quiet NaN, infinities stay infinite)
+  __device__ static inline uint32_t bits(float f) {
+    const uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (u >> 16) | ((u & 0xffffu) != 0u ? 0x40u : 0u);
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+  }
+};
+template <> struct Round<NDRR_F16> {
+  __device__ static inline uint32_t bits(float f) {
+    return __builtin_bit_cast(uint16_t, static_cast<_Float16>(f));
+  }
+};
+
+// The coefficient layout's header: [0] the volume's largest |value| (fp32
+// bits, an atomic maximum), [1] the scale s of the 16-bit records, [2] 1/s,
+// [3] the record type; kCoefHeader bytes in all.
+__global__ void k_coef_header_init(uint32_t* __restrict__ hdr) {
+  if (threadIdx.x < kCoefHeader / 4) hdr[threadIdx.x] = 0u;
+}
+
+template <int DS>
+__global__ __launch_bounds__(kThreads) void k_coef_absmax(const typename Vox<DS>::T* __restrict__ vol, int64_t n,
+                                                          uint32_t* __restrict__ hdr) {
+  __shared__ uint32_t red[kThreads / 64];
+  uint32_t m = 0u;  // |value| as fp32 bits: non-negative floats order like their bit patterns
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kThreads)
+    m = max(m, __float_as_uint(Vox<DS>::ld(vol, static_cast<int>(i))) & 0x7fffffffu);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), o)));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kThreads / 64; ++w) m = max(m, red[w]);
+    atomicMax(hdr, m);
+  }
+}
+
+// The records' scale: 1 for fp32; else the largest power of two s with
+// 8 max|v| s <= 65504 (fp16's largest value; |coefficient| <= 8 max|v|),
+// within [2^-120, 2^120]; 1 for a zero or non-finite maximum.
+template <int DR> __device__ inline float coef_scale(uint32_t absmax_bits) {
+  if constexpr (DR == NDRR_F32) return 1.0f;
+  const float m = __uint_as_float(absmax_bits);
+  if (!(m > 0.0f) || !(m <= 3.0e38f)) return 1.0f;
+  const float t = fminf(fmaxf(8188.0f / m, 0x1p-120f), 0x1p120f);
+  return __uint_as_float(__float_as_uint(t) & 0x7f800000u);  // 2^floor(log2 t)
+}
+
+// One thread per coefficient record (cell x0, y0, z0 on the stencil's grid):
+// the cell's eight zero-padded corners (as fp32) become the coefficients of
+// its trilinear interpolant, in fp32 in a fixed order, times the header's
+// scale s, each rounded once to its slot's type (fp32; bf16 for c0 and fp16
+// for the rest; fp16).
+template <int DS, int DR>
+__global__ void k_pack_coef(const typename Vox<DS>::T* __restrict__ vol, uint32_t* __restrict__ hdr,
+                            void* __restrict__ rec, int D, int H, int W, int64_t n_rec) {
+#pragma clang fp contract(off)
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const float scale = coef_scale<DR>(hdr[0]);
+  if (r == 0) {
+    hdr[1] = __float_as_uint(scale);
+    hdr[2] = __float_as_uint(1.0f / scale);  // exact: a power of two
+    hdr[3] = static_cast<uint32_t>(DR);
+  }
+  if (r >= n_rec) return;
+  const int cx = static_cast<int>(r % (W + 1));
+  const int64_t rest = r / (W + 1);
+  const int cy = static_cast<int>(rest % (H + 1));
+  const int cz = static_cast<int>(rest / (H + 1));
+  float v[8];  // v[4a + 2b + c] = corner (z0 + a, y0 + b, x0 + c)
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int x = cx - 1 + (k & 1), y = cy - 1 + ((k >> 1) & 1), z = cz - 1 + (k >> 2);
+    const bool inb = static_cast<unsigned>(x) < static_cast<unsigned>(W) &&
+                     static_cast<unsigned>(y) < static_cast<unsigned>(H) &&
+                     static_cast<unsigned>(z) < static_cast<unsigned>(D);
+    v[k] = inb ? Vox<DS>::ld(vol, static_cast<int>((static_cast<int64_t>(z) * H + y) * W + x)) : 0.0f;
+  }
+  const float dx00 = v[1] - v[0], dx01 = v[3] - v[2], dx10 = v[5] - v[4], dx11 = v[7] - v[6];
+  const float cyv = v[2] - v[0];
+  float c[8];
+  c[0] = v[0];                  // c0
+  c[1] = dx00;                  // cx
+  c[2] = cyv;                   // cy
+  c[3] = dx01 - dx00;           // cxy
+  c[4] = v[4] - v[0];           // cz
+  c[5] = dx10 - dx00;           // cxz
+  c[6] = (v[6] - v[4]) - cyv;   // cyz
+  c[7] = (dx11 - dx10) - (dx01 - dx00);  // cxyz
+  if constexpr (DR == NDRR_F32) {
+    float4* o = static_cast<float4*>(rec) + 2 * r;
+    o[0] = make_float4(c[0], c[1], c[2], c[3]);
+    o[1] = make_float4(c[4], c[5], c[6], c[7]);
+  } else {
+    uint32_t h[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) h[k] = Round<NDRR_F16>::bits(c[k] * scale);  // exact product: a power of two
+    if constexpr (DR == NDRR_BF16) h[0] = Round<NDRR_BF16>::bits(c[0] * scale);
+    static_cast<uint4*>(rec)[r] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16),
+                                             h[6] | (h[7] << 16));
+  }
+}
+
 // One thread per label-quad record (x, y0, z0): bytes lab[z0+a][y0+b][x] at 2a + b.
 __global__ void k_pack_label_quads(const int8_t* __restrict__ lab, uint32_t* __restrict__ out,
                                    int D, int H, int W, int64_t n_rec) {
@@ -1225,12 +1455,19 @@ int64_t packed_volume_bytes(int D, int H, int W, int dt, int layout) {
     case NDRR_LAYOUT_LINEAR: return static_cast<int64_t>(D) * H * W * e;
     case NDRR_LAYOUT_STENCIL: return stencil_records(D, H, W) * 8 * e;
     case NDRR_LAYOUT_QUAD: return quad_records(D, H, W) * 4 * e;
+    case NDRR_LAYOUT_COEF: return kCoefHeader + stencil_records(D, H, W) * 8 * e;
     default: return 0;
   }
 }
 
 int64_t packed_label_bytes(int D, int H, int W, int layout) {
-  return layout == NDRR_LAYOUT_QUAD ? quad_records(D, H, W) * 4 : static_cast<int64_t>(D) * H * W;
+  if (layout == NDRR_LAYOUT_QUAD) return quad_records(D, H, W) * 4;
+  return static_cast<int64_t>(D) * H * W;
+}
+
+bool valid_layout(int layout) {
+  return layout == NDRR_LAYOUT_LINEAR || layout == NDRR_LAYOUT_STENCIL || layout == NDRR_LAYOUT_QUAD ||
+         layout == NDRR_LAYOUT_COEF;
 }
 
 // The packed layouts' record index is formed with 24-bit multiplies (record_of).
@@ -1246,8 +1483,7 @@ int check_common(const void* vol, int vol_dtype, int vol_layout, const int8_t* l
   if (!vol || !M || !src || !tgt || !step) return NDRR_EINVAL;
   if (B <= 0 || N <= 0 || D <= 0 || H <= 0 || W <= 0 || S < 2 || width < 0) return NDRR_EINVAL;
   if (width > 0 && N % width != 0) return NDRR_EINVAL;
-  if (vol_layout != NDRR_LAYOUT_LINEAR && vol_layout != NDRR_LAYOUT_STENCIL && vol_layout != NDRR_LAYOUT_QUAD)
-    return NDRR_EINVAL;
+  if (!valid_layout(vol_layout)) return NDRR_EINVAL;
   if (!valid_dtype(vol_dtype)) return NDRR_EDTYPE;
   if (C < 1 || C > NDRR_MAX_CLASSES) return NDRR_ECLASSES;
   if (C > 1 && !lab) return NDRR_EINVAL;
@@ -1331,6 +1567,7 @@ template <int DT>
 void launch_fwd1(const Params& p, int layout, dim3 grid, hipStream_t st) {
   if (layout == NDRR_LAYOUT_STENCIL) launch_fwd2<DT, NDRR_LAYOUT_STENCIL>(p, grid, st);
   else if (layout == NDRR_LAYOUT_QUAD) launch_fwd2<DT, NDRR_LAYOUT_QUAD>(p, grid, st);
+  else if (layout == NDRR_LAYOUT_COEF) launch_fwd2<DT, NDRR_LAYOUT_COEF>(p, grid, st);
   else launch_fwd2<DT, NDRR_LAYOUT_LINEAR>(p, grid, st);
 }
 
@@ -1338,7 +1575,27 @@ template <int DT>
 void launch_bwd1(const Params& p, int layout, bool gcam, bool gvol, dim3 grid, hipStream_t st) {
   if (layout == NDRR_LAYOUT_STENCIL) launch_bwd2<DT, NDRR_LAYOUT_STENCIL>(p, gcam, gvol, grid, st);
   else if (layout == NDRR_LAYOUT_QUAD) launch_bwd2<DT, NDRR_LAYOUT_QUAD>(p, gcam, gvol, grid, st);
+  else if (layout == NDRR_LAYOUT_COEF) launch_bwd2<DT, NDRR_LAYOUT_COEF>(p, gcam, gvol, grid, st);
   else launch_bwd2<DT, NDRR_LAYOUT_LINEAR>(p, gcam, gvol, grid, st);
+}
+
+template <int DS>
+int pack_coef_from(const void* vol, int record_dtype, int D, int H, int W, void* out, int64_t n, hipStream_t st) {
+  using T = typename Vox<DS>::T;
+  const T* v = static_cast<const T*>(vol);
+  uint32_t* hdr = static_cast<uint32_t*>(out);
+  void* rec = static_cast<char*>(out) + kCoefHeader;
+  const int64_t n_vox = static_cast<int64_t>(D) * H * W;
+  hipLaunchKernelGGL(k_coef_header_init, dim3(1), dim3(kCoefHeader / 4), 0, st, hdr);
+  const int64_t ablocks = std::min<int64_t>((n_vox + kThreads - 1) / kThreads, 4096);
+  hipLaunchKernelGGL(k_coef_absmax<DS>, dim3(static_cast<unsigned>(ablocks)), dim3(kThreads), 0, st, v, n_vox, hdr);
+  const dim3 grid(static_cast<unsigned>((n + kThreads - 1) / kThreads));
+  switch (record_dtype) {
+    case NDRR_F32: hipLaunchKernelGGL((k_pack_coef<DS, NDRR_F32>), grid, dim3(kThreads), 0, st, v, hdr, rec, D, H, W, n); break;
+    case NDRR_BF16: hipLaunchKernelGGL((k_pack_coef<DS, NDRR_BF16>), grid, dim3(kThreads), 0, st, v, hdr, rec, D, H, W, n); break;
+    default: hipLaunchKernelGGL((k_pack_coef<DS, NDRR_F16>), grid, dim3(kThreads), 0, st, v, hdr, rec, D, H, W, n); break;
+  }
+  return launch_status();
 }
 
 }  // namespace
@@ -1390,6 +1647,7 @@ static int raymarch_fwd_any(const void* vol, int vol_dtype, int vol_layout, cons
   Params p = make_params(vol, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width,
                          src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride);
   p.vol_bytes = static_cast<uint32_t>(packed_volume_bytes(D, H, W, vol_dtype, vol_layout));
+  p.vol_skip = vol_layout == NDRR_LAYOUT_COEF ? kCoefHeader : 0u;
   p.lab_bytes = n_classes > 1 ? static_cast<uint32_t>(packed_label_bytes(D, H, W, vol_layout)) : 0u;
   p.w2v = w2v;
   p.rt = rt;
@@ -1455,6 +1713,7 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_l
   Params p = make_params(vol, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width,
                          src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride);
   p.vol_bytes = static_cast<uint32_t>(packed_volume_bytes(D, H, W, vol_dtype, vol_layout));
+  p.vol_skip = vol_layout == NDRR_LAYOUT_COEF ? kCoefHeader : 0u;
   p.lab_bytes = n_classes > 1 ? static_cast<uint32_t>(packed_label_bytes(D, H, W, vol_layout)) : 0u;
   p.go = go;
   p.gpart = workspace;
@@ -1471,6 +1730,7 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_l
   if (r != NDRR_OK) return r;
   const int parts_per_tile = vol_layout == NDRR_LAYOUT_STENCIL ? bwd_parts_per_tile(NDRR_LAYOUT_STENCIL)
                              : vol_layout == NDRR_LAYOUT_QUAD  ? bwd_parts_per_tile(NDRR_LAYOUT_QUAD)
+                             : vol_layout == NDRR_LAYOUT_COEF  ? bwd_parts_per_tile(NDRR_LAYOUT_COEF)
                                                                : bwd_parts_per_tile(NDRR_LAYOUT_LINEAR);
   hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(kReduceThreads), 0, st, workspace, gM,
                      p.nblocks * parts_per_tile, nullptr, nullptr, Se3Out{});
@@ -1643,6 +1903,7 @@ int ndrr_pack_volume(const void* vol, int vol_dtype, int vol_layout, int D, int 
                                         hipMemcpyDeviceToDevice, st);
     return e == hipSuccess ? NDRR_OK : static_cast<int>(e);
   }
+  if (vol_layout == NDRR_LAYOUT_COEF) return ndrr_pack_coef(vol, vol_dtype, vol_dtype, D, H, W, out, stream);
   if (vol_layout != NDRR_LAYOUT_STENCIL && vol_layout != NDRR_LAYOUT_QUAD) return NDRR_EINVAL;
   const bool quad = vol_layout == NDRR_LAYOUT_QUAD;
   const int64_t n = quad ? quad_records(D, H, W) : stencil_records(D, H, W);
@@ -1671,17 +1932,29 @@ int ndrr_pack_volume(const void* vol, int vol_dtype, int vol_layout, int D, int 
   return launch_status();
 }
 
+int ndrr_pack_coef(const void* vol, int vol_dtype, int record_dtype, int D, int H, int W, void* out, void* stream) {
+  if (!vol || !out || D <= 0 || H <= 0 || W <= 0) return NDRR_EINVAL;
+  if (!valid_dtype(vol_dtype) || !valid_dtype(record_dtype)) return NDRR_EDTYPE;
+  const int64_t n = stencil_records(D, H, W);
+  if (n >= (int64_t{1} << 31) || static_cast<int64_t>(D) * H * W >= (int64_t{1} << 31)) return NDRR_ERANGE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (vol_dtype) {
+    case NDRR_F32: return pack_coef_from<NDRR_F32>(vol, record_dtype, D, H, W, out, n, st);
+    case NDRR_BF16: return pack_coef_from<NDRR_BF16>(vol, record_dtype, D, H, W, out, n, st);
+    default: return pack_coef_from<NDRR_F16>(vol, record_dtype, D, H, W, out, n, st);
+  }
+}
+
 size_t ndrr_packed_label_bytes(int D, int H, int W, int vol_layout) {
   if (D <= 0 || H <= 0 || W <= 0) return 0;
-  if (vol_layout != NDRR_LAYOUT_LINEAR && vol_layout != NDRR_LAYOUT_STENCIL && vol_layout != NDRR_LAYOUT_QUAD)
-    return 0;
+  if (!valid_layout(vol_layout)) return 0;
   return static_cast<size_t>(packed_label_bytes(D, H, W, vol_layout));
 }
 
 int ndrr_pack_labels(const int8_t* lab, int vol_layout, int D, int H, int W, void* out, void* stream) {
   if (!lab || !out || D <= 0 || H <= 0 || W <= 0) return NDRR_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (vol_layout == NDRR_LAYOUT_LINEAR || vol_layout == NDRR_LAYOUT_STENCIL) {
+  if (vol_layout == NDRR_LAYOUT_LINEAR || vol_layout == NDRR_LAYOUT_STENCIL || vol_layout == NDRR_LAYOUT_COEF) {
     const hipError_t e = hipMemcpyAsync(out, lab, static_cast<size_t>(D) * H * W, hipMemcpyDeviceToDevice, st);
     return e == hipSuccess ? NDRR_OK : static_cast<int>(e);
   }

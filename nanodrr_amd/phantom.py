@@ -90,13 +90,16 @@ def make_phantom(shape=(256, 256, 256), spacing=(1.0, 1.0, 1.0), seed: int = 0, 
 
 
 def phantom_subject(ph: Phantom, n_classes: int = 1, dtype=torch.float32, device="cpu") -> Subject:
-    """Subject holding mu (convert_to_mu=False) and, for n_classes > 1, the label map
-    (labels >= n_classes fold into the last class)."""
+    """Subject holding mu (convert_to_mu=False), stored in `dtype` (rounded from the fp32
+    values, which the subject keeps), and, for n_classes > 1, the label map (labels >=
+    n_classes fold into the last class)."""
     label = None
     if n_classes > 1:
         label = ph.labels.to(torch.float32).clamp(max=n_classes - 1)
-    subject = Subject.from_volume(ph.mu.to(dtype), spacing=ph.spacing, label=label, max_label=n_classes - 1,
+    subject = Subject.from_volume(ph.mu, spacing=ph.spacing, label=label, max_label=n_classes - 1,
                                   convert_to_mu=False)
+    if dtype != torch.float32:
+        subject = subject.with_volume_dtype(dtype)
     return subject.to(device)
 
 

@@ -22,12 +22,14 @@ _BACKENDS = ("auto", "hip", "triton")
 # corners of each cell: one 16-byte load per sample for 16-bit volumes),
 # "quad" (a cached copy of each voxel's (y, z) corner quads: half the
 # stencil's memory, which lets a 256^3 16-bit volume stay in the Infinity
-# Cache), "linear" (the (D, H, W) volume itself) or "auto": linear when the
-# volume requires grad (a packed copy would be re-made after every update),
-# else quad (the fastest measured for every dtype with the single-pass
-# adjoint).
+# Cache), "coef" (a cached copy of each cell's interpolant coefficients, made
+# from the unrounded volume: the gradient's edge differences rounded relative
+# to their own size), "linear" (the (D, H, W) volume itself) or "auto":
+# linear when the volume requires grad (a packed copy would be re-made after
+# every update), else coef for 16-bit storage (as fast as quad there, with
+# ~10x smaller gradient errors) and quad for fp32 (the fastest measured).
 VOLUME_LAYOUT = "auto"
-_LAYOUTS = ("linear", "quad", "stencil")
+_LAYOUTS = ("linear", "quad", "stencil", "coef")
 
 # How pose and camera gradients are formed when the volume needs none:
 # "single_pass" (the forward march also stores per-ray sums and the backward
@@ -147,11 +149,19 @@ def volume_layout(image: torch.Tensor) -> str:
     if layout == "auto":
         if image.requires_grad or not packed_index_fits(*image.shape[-3:]):
             layout = "linear"
+        elif image.dtype in (torch.bfloat16, torch.float16) and coef_fits(*image.shape[-3:]):
+            layout = "coef"
         else:
             layout = "quad"
     if layout not in _LAYOUTS:
         raise ValueError(f"VOLUME_LAYOUT must be 'auto' or one of {_LAYOUTS}, got {VOLUME_LAYOUT!r}")
     return layout
+
+
+def coef_fits(D: int, H: int, W: int) -> bool:
+    """Whether 16-bit coefficient records of a (D, H, W) volume fit the kernels' 32-bit buffer
+    offsets (a 256-byte header and 16 bytes per cell, under 2 GiB)."""
+    return 256 + (D + 1) * (H + 1) * (W + 1) * 16 < 2**31
 
 
 def packed_index_fits(D: int, H: int, W: int) -> bool:

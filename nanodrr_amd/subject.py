@@ -68,12 +68,13 @@ class Subject(torch.nn.Module):
     hu_bone = _mu_property("hu_bone")
 
     def __init__(self, imagedata, labeldata, voxel_to_world, world_to_voxel, voxel_to_grid, isocenter,
-                 max_label=None, **mu) -> None:
+                 max_label=None, *, storage_dtype: torch.dtype | None = None, **mu) -> None:
         bad = sorted(set(mu) - set(_MU_FIELDS))
         if bad:
             raise TypeError(f"Unknown mu param(s): {', '.join(repr(k) for k in bad)}")
         super().__init__()
         self._derived = Derived()
+        self.storage_dtype = storage_dtype  # None: `image` keeps the volume's own type
         self._mu_override = None  # (token, mu image) after set_mu, until a parameter changes
         for name, t in (("_image_hu", imagedata), ("label", labeldata),
                         ("world_to_grid", voxel_to_grid @ world_to_voxel), ("isocenter", isocenter),
@@ -89,20 +90,32 @@ class Subject(torch.nn.Module):
     # -- HU -> mu ---------------------------------------------------------------
     def _mu_changed(self) -> None:
         self._mu_override = None
-        self._derived.drop("mu")
+        self._derived.drop("mu", "stored")
         self._derived.drop_prefix("volume:")
 
-    @property
-    def image(self) -> torch.Tensor:
-        """(1, 1, D, H, W) attenuation (the stored values when convert_to_mu is False)."""
+    def precise_image(self) -> torch.Tensor:
+        """(1, 1, D, H, W) attenuation before any rounding to `storage_dtype` (the values the
+        coefficient layout's records are made from)."""
         if not self.convert_to_mu:
             return self._image_hu
         if self._mu_override is not None:
             return self._mu_override[1]
         params = (self.mu_water, self.mu_bone, self.hu_bone)
-        if torch.compiler.is_compiling():  # traced graphs convert (the table is Python state)
+        if torch.compiler.is_compiling() or self._image_hu.requires_grad:  # no cached autograd graphs
             return hu_to_mu(self._image_hu, *params)
         return self._derived.get("mu", self._image_hu, params, lambda: hu_to_mu(self._image_hu, *params))
+
+    @property
+    def image(self) -> torch.Tensor:
+        """(1, 1, D, H, W) attenuation (the stored values when convert_to_mu is False), in
+        `storage_dtype` when one is set."""
+        base = self.precise_image()
+        dt = self.storage_dtype
+        if dt is None or base.dtype == dt:
+            return base
+        if torch.compiler.is_compiling() or base.requires_grad:
+            return base.to(dt)
+        return self._derived.get("stored", base, dt, lambda: base.to(dt))
 
     def set_mu(self, mu_water=None, mu_bone=None, hu_bone=None) -> None:
         """Render with other HU -> mu parameters (floats or tensors, e.g. ones being
@@ -111,6 +124,7 @@ class Subject(torch.nn.Module):
         args = (self.mu_water if mu_water is None else mu_water,
                 self.mu_bone if mu_bone is None else mu_bone,
                 self.hu_bone if hu_bone is None else hu_bone)
+        self._derived.drop("stored")
         self._derived.drop_prefix("volume:")
         self._mu_override = (object(), hu_to_mu(self._image_hu, *args))
 
@@ -123,11 +137,16 @@ class Subject(torch.nn.Module):
         return super()._apply(fn, *args, **kwargs)
 
     def with_volume_dtype(self, dtype: torch.dtype) -> "Subject":
-        """A subject holding `image` in `dtype` (e.g. bfloat16) as its stored values
-        (convert_to_mu=False), with the affines and labels untouched (torch's
-        `.to(dtype)` would also round the affines, costing sampling accuracy)."""
-        return Subject(self.image.detach().to(dtype), self.label, self.voxel_to_world, self.world_to_voxel,
-                       self.voxel_to_grid, self.isocenter, max_label=self.n_classes - 1, convert_to_mu=False)
+        """The same subject with `image` stored in `dtype` (e.g. bfloat16): it shares the
+        volume, labels and affines (torch's `.to(dtype)` would also round the affines, costing
+        sampling accuracy) and rounds the attenuation once, on first use. The unrounded values
+        stay available (`precise_image`) for the coefficient layout."""
+        mu = {name: getattr(self, name) for name in _MU_FIELDS}
+        out = Subject(self._image_hu, self.label, self.voxel_to_world, self.world_to_voxel, self.voxel_to_grid,
+                      self.isocenter, max_label=self.n_classes - 1, storage_dtype=dtype, **mu)
+        if self._mu_override is not None:
+            out._mu_override = self._mu_override
+        return out
 
     # -- what the kernels read ----------------------------------------------------
     def world_to_voxel_f64(self) -> torch.Tensor:
@@ -138,10 +157,21 @@ class Subject(torch.nn.Module):
         return self._derived.get("w2v64", w, None, lambda: w.to(torch.float64).contiguous())
 
     def packed_volume(self, layout: str) -> torch.Tensor:
-        """`image` re-laid out for the kernels ("quad" or "stencil", see ndrr_pack_volume)."""
-        from ._ops import pack_volume
+        """`image` re-laid out for the kernels ("quad", "stencil" or "coef", see ndrr_pack_volume).
+        Coefficient records of a 16-bit `image` are made from `precise_image` and rounded once
+        (ndrr_pack_coef), so they carry the unrounded volume's edge differences."""
+        from ._ops import pack_coef, pack_volume
 
         img = self.image
+        if layout == "coef":
+            src = self.precise_image()
+            if src is not img:
+                def make():
+                    return pack_coef(src[0, 0].detach().to(torch.float32), img.dtype)
+
+                if torch.compiler.is_compiling():
+                    return make()
+                return self._derived.get("volume:coef", src, img.dtype, make)
         if torch.compiler.is_compiling():  # traced graphs re-pack
             return pack_volume(img[0, 0], layout)
         return self._derived.get("volume:" + layout, img, None, lambda: pack_volume(img[0, 0], layout))

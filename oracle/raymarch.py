@@ -78,6 +78,41 @@ def _corners(vol: np.ndarray, p: np.ndarray):
     return vals, f, i
 
 
+def coef_records(vol: np.ndarray, dtype=np.float32) -> np.ndarray:
+    """(D+1, H+1, W+1, 8) coefficient records of the zero-padded volume, one per cell (z0, y0, x0)
+    in [-1, D-1] x [-1, H-1] x [-1, W-1]: [c0, cx, cy, cxy, cz, cxz, cyz, cxyz] of the cell's trilinear
+    interpolant, formed with the arithmetic of `dtype` in the order ndrr_pack_coef documents
+    (float32 gives the packer's fp32 values before their final rounding; float64 the exact ones)."""
+    vol = np.asarray(vol, dtype)
+    D, H, W = vol.shape
+    P = np.zeros((D + 2, H + 2, W + 2), dtype)
+    P[1:-1, 1:-1, 1:-1] = vol
+    v = [P[a:a + D + 1, b:b + H + 1, c:c + W + 1] for a in (0, 1) for b in (0, 1) for c in (0, 1)]  # v[4a+2b+c]
+    dx00, dx01, dx10, dx11 = v[1] - v[0], v[3] - v[2], v[5] - v[4], v[7] - v[6]
+    cy = v[2] - v[0]
+    out = [v[0], dx00, cy, dx01 - dx00, v[4] - v[0], dx10 - dx00, (v[6] - v[4]) - cy, (dx11 - dx10) - (dx01 - dx00)]
+    return np.stack(out, axis=-1)
+
+
+def _coef_eval(coef: np.ndarray, p: np.ndarray):
+    """Value and gradient (voxel units) at positions p (..., 3) from coefficient records (float64)."""
+    D1, H1, W1 = coef.shape[:3]
+    fl = np.floor(p)
+    f = p - fl
+    i = fl.astype(np.int64)
+    ok = (i[..., 0] >= -1) & (i[..., 0] < W1 - 1) & (i[..., 1] >= -1) & (i[..., 1] < H1 - 1) & \
+        (i[..., 2] >= -1) & (i[..., 2] < D1 - 1)
+    rec = coef[np.clip(i[..., 2] + 1, 0, D1 - 1), np.clip(i[..., 1] + 1, 0, H1 - 1), np.clip(i[..., 0] + 1, 0, W1 - 1)]
+    rec = np.where(ok[..., None], np.asarray(rec, np.float64), 0.0)
+    c0, cx, cy, cxy, cz, cxz, cyz, cxyz = (rec[..., k] for k in range(8))
+    fx, fy, fz = f[..., 0], f[..., 1], f[..., 2]
+    val = c0 + cx * fx + cy * fy + cxy * fx * fy + cz * fz + cxz * fx * fz + cyz * fy * fz + cxyz * fx * fy * fz
+    gx = cx + cxy * fy + cxz * fz + cxyz * fy * fz
+    gy = cy + cxy * fx + cyz * fz + cxyz * fx * fz
+    gz = cz + cxz * fx + cyz * fy + cxyz * fx * fy
+    return val, gx, gy, gz
+
+
 def _lerp(vals, f):
     fx, fy, fz = f[..., 0], f[..., 1], f[..., 2]
     c00 = vals[0, 0, 0] * (1 - fx) + vals[0, 0, 1] * fx
@@ -116,8 +151,9 @@ def _sample_points(M_b, s_rows, t_rows, S):
     return t, q, p
 
 
-def forward(vol, lab, n_classes, M, src, tgt, step, n_samples, rays=None) -> np.ndarray:
-    """(B, C, R) line integrals in float64 (R = len(rays) or N)."""
+def forward(vol, lab, n_classes, M, src, tgt, step, n_samples, rays=None, coef=None) -> np.ndarray:
+    """(B, C, R) line integrals in float64 (R = len(rays) or N). With `coef` (coefficient records,
+    `coef_records`), the interpolant of each cell is the records' polynomial (vol only fixes the shape)."""
     vol = np.asarray(vol, np.float64)
     M = np.asarray(M, np.float64)
     src, tgt, step = (np.asarray(a, np.float64) for a in (src, tgt, step))
@@ -130,8 +166,11 @@ def forward(vol, lab, n_classes, M, src, tgt, step, n_samples, rays=None) -> np.
         st = _step_rows(step, b, rays)
         for lo, hi in _iter_chunks(len(rays), n_samples):
             _, _, p = _sample_points(M[b], s_all[lo:hi], t_all[lo:hi], n_samples)
-            vals, f, _ = _corners(vol, p)
-            val = _lerp(vals, f)
+            if coef is None:
+                vals, f, _ = _corners(vol, p)
+                val = _lerp(vals, f)
+            else:
+                val = _coef_eval(coef, p)[0]
             if C == 1:
                 out[b, 0, lo:hi] = val.sum(axis=1) * st[lo:hi]
             else:
@@ -141,12 +180,15 @@ def forward(vol, lab, n_classes, M, src, tgt, step, n_samples, rays=None) -> np.
     return out
 
 
-def backward(go, vol, lab, n_classes, M, src, tgt, step, n_samples, rays=None, need_gvol=True):
+def backward(go, vol, lab, n_classes, M, src, tgt, step, n_samples, rays=None, need_gvol=True, coef=None):
     """Analytic adjoint of `forward` (restates _fused.py:230-298 in float64).
 
     go: (B, C, R). Returns dict with gM (B, 3, 4), gsrc/gtgt (B, R, 3) per ray,
-    gstep (B, R) and gvol (D, H, W) (None unless need_gvol).
+    gstep (B, R) and gvol (D, H, W) (None unless need_gvol). With `coef`, values and
+    gradients come from the coefficient records (no volume gradient then).
     """
+    if coef is not None and need_gvol:
+        raise ValueError("the volume gradient is defined for the corner form only")
     vol = np.asarray(vol, np.float64)
     M = np.asarray(M, np.float64)
     go = np.asarray(go, np.float64)
@@ -175,13 +217,16 @@ def backward(go, vol, lab, n_classes, M, src, tgt, step, n_samples, rays=None, n
                 gg = np.where(ok, go[b, np.clip(cls, 0, C - 1), np.arange(lo, hi)[:, None]], 0.0)
             g = gg * st[lo:hi][:, None]
             fx, fy, fz = f[..., 0], f[..., 1], f[..., 2]
-            v = vals
-            dvx = ((v[0, 0, 1] - v[0, 0, 0]) * (1 - fy) + (v[0, 1, 1] - v[0, 1, 0]) * fy) * (1 - fz) + (
-                (v[1, 0, 1] - v[1, 0, 0]) * (1 - fy) + (v[1, 1, 1] - v[1, 1, 0]) * fy) * fz
-            dvy = ((v[0, 1, 0] - v[0, 0, 0]) * (1 - fx) + (v[0, 1, 1] - v[0, 0, 1]) * fx) * (1 - fz) + (
-                (v[1, 1, 0] - v[1, 0, 0]) * (1 - fx) + (v[1, 1, 1] - v[1, 0, 1]) * fx) * fz
-            dvz = ((v[1, 0, 0] - v[0, 0, 0]) * (1 - fx) + (v[1, 0, 1] - v[0, 0, 1]) * fx) * (1 - fy) + (
-                (v[1, 1, 0] - v[0, 1, 0]) * (1 - fx) + (v[1, 1, 1] - v[0, 1, 1]) * fx) * fy
+            if coef is not None:
+                val, dvx, dvy, dvz = _coef_eval(coef, p)
+            else:
+                v = vals
+                dvx = ((v[0, 0, 1] - v[0, 0, 0]) * (1 - fy) + (v[0, 1, 1] - v[0, 1, 0]) * fy) * (1 - fz) + (
+                    (v[1, 0, 1] - v[1, 0, 0]) * (1 - fy) + (v[1, 1, 1] - v[1, 1, 0]) * fy) * fz
+                dvy = ((v[0, 1, 0] - v[0, 0, 0]) * (1 - fx) + (v[0, 1, 1] - v[0, 0, 1]) * fx) * (1 - fz) + (
+                    (v[1, 1, 0] - v[1, 0, 0]) * (1 - fx) + (v[1, 1, 1] - v[1, 0, 1]) * fx) * fz
+                dvz = ((v[1, 0, 0] - v[0, 0, 0]) * (1 - fx) + (v[1, 0, 1] - v[0, 0, 1]) * fx) * (1 - fy) + (
+                    (v[1, 1, 0] - v[0, 1, 0]) * (1 - fx) + (v[1, 1, 1] - v[0, 1, 1]) * fx) * fy
             gp = np.stack([g * dvx, g * dvy, g * dvz], axis=-1)  # (R, S, 3) dL/dp
             qh = np.concatenate([q, np.ones(q.shape[:-1] + (1,))], axis=-1)
             gM[b] += np.einsum("rsi,rsj->ij", gp, qh)
@@ -218,4 +263,29 @@ def inside_count(M, src, tgt, n_samples, shape, rays=None) -> np.ndarray:
             _, _, p = _sample_points(M[b], s_all[lo:up], t_all[lo:up], n_samples)
             ins = np.all((p > -1.0) & (p < hi), axis=-1)
             out[b, lo:up] = ins.sum(axis=1)
+    return out
+
+
+def discontinuity_distance(M, src, tgt, n_samples, shape, rays=None, labels=True) -> np.ndarray:
+    """(B, R) smallest distance (voxels) over a ray's samples that touch the volume from a position
+    where the march is discontinuous: a cell face (integer coordinate), where the trilinear
+    derivative jumps, and, with labels, a nearest-label tie (half-integer coordinate), where the
+    class jumps. An fp32 march places samples within ~1 ulp of the float64 positions, so rays
+    whose distance exceeds a few ulps see the same cells and classes in both."""
+    D, H, W = shape
+    M = np.asarray(M, np.float64)
+    src, tgt = np.asarray(src, np.float64), np.asarray(tgt, np.float64)
+    B, N = M.shape[0], tgt.shape[1]
+    rays = np.arange(N) if rays is None else np.asarray(rays)
+    out = np.full((B, len(rays)), np.inf)
+    hi = np.array([W, H, D], np.float64)
+    for b in range(B):
+        s_all, t_all = _ray_endpoints(src, tgt, b, rays)
+        for lo, up in _iter_chunks(len(rays), n_samples):
+            _, _, p = _sample_points(M[b], s_all[lo:up], t_all[lo:up], n_samples)
+            ins = np.all((p > -1.0) & (p < hi), axis=-1)
+            step = 0.5 if labels else 1.0
+            q = p / step
+            dist = np.abs(q - np.rint(q)).min(axis=-1) * step
+            out[b, lo:up] = np.where(ins, dist, np.inf).min(axis=1)
     return out

@@ -64,3 +64,40 @@ def rel_to_max(ref, out) -> float:
     ref = np.asarray(ref, np.float64)
     out = np.asarray(out, np.float64)
     return float(np.abs(ref - out).max() / np.abs(ref).max())
+
+
+def coef_header(packed: torch.Tensor) -> tuple[float, float, int]:
+    """(scale s, 1/s, record type code) from a coefficient-layout buffer's header."""
+    hdr = packed[:16].cpu().view(torch.float32)
+    return float(hdr[1]), float(hdr[2]), int(packed[:16].cpu().view(torch.int32)[3])
+
+
+def coef_decode(packed: torch.Tensor, dtype, shape) -> np.ndarray:
+    """(D+1, H+1, W+1, 8) float64 coefficients a coefficient-layout buffer holds, unscaled: fp32
+    records as they are; bf16 storage holds c0 in bf16 and the rest in fp16; fp16 storage all in
+    fp16; 16-bit slots hold the coefficient times s."""
+    D, H, W = shape
+    body = packed[256:].cpu()
+    if dtype == torch.float32:
+        return body.view(torch.float32).reshape(D + 1, H + 1, W + 1, 8).double().numpy()
+    _, inv, _ = coef_header(packed)
+    half = body.view(torch.float16).reshape(D + 1, H + 1, W + 1, 8).double()
+    if dtype == torch.bfloat16:
+        half[..., 0] = body.view(torch.bfloat16).reshape(D + 1, H + 1, W + 1, 8)[..., 0].double()
+    return half.numpy() * inv
+
+
+def coef_expected(vol: np.ndarray, dtype) -> tuple[torch.Tensor, float]:
+    """(raw record slots as int16 (or fp32 bits for fp32 records), scale s) a coefficient pack of
+    `vol` must produce: oracle.raymarch.coef_records in fp32, times s = the largest power of two with
+    8 max|v| s <= 65504, each slot rounded to its type (c0 in bf16 for bf16 storage)."""
+    coef = orm.coef_records(vol, np.float32)
+    if dtype == torch.float32:
+        return torch.from_numpy(coef).view(torch.int32), 1.0
+    m = np.float32(np.abs(np.asarray(vol, np.float32)).max())
+    s = 1.0 if m == 0 else float(2.0 ** np.floor(np.log2(np.float32(8188.0) / m)))
+    scaled = torch.from_numpy(coef * np.float32(s))
+    slots = scaled.to(torch.float16).view(torch.int16).clone()
+    if dtype == torch.bfloat16:
+        slots[..., 0] = scaled[..., 0].to(torch.bfloat16).view(torch.int16)
+    return slots, s

@@ -67,6 +67,9 @@ def test_packed_sizes():
     assert lib.ndrr_packed_bytes(D, H, W, 1, 1) == (D + 1) * (H + 1) * (W + 1) * 16
     assert lib.ndrr_packed_bytes(D, H, W, 0, 2) == (D + 1) * (H + 1) * (W + 2) * 16
     assert lib.ndrr_packed_bytes(D, H, W, 2, 2) == (D + 1) * (H + 1) * (W + 2) * 8
+    assert lib.ndrr_packed_bytes(D, H, W, 1, 3) == 256 + (D + 1) * (H + 1) * (W + 1) * 16  # header + records
+    assert lib.ndrr_packed_bytes(D, H, W, 0, 3) == 256 + (D + 1) * (H + 1) * (W + 1) * 32
+    assert lib.ndrr_packed_label_bytes(D, H, W, 3) == D * H * W  # the int8 map itself
     assert lib.ndrr_packed_bytes(D, H, W, 1, 7) == 0
     assert lib.ndrr_packed_label_bytes(D, H, W, 2) == (D + 1) * (H + 1) * (W + 2) * 4
     assert lib.ndrr_packed_label_bytes(D, H, W, 0) == D * H * W

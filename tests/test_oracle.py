@@ -191,3 +191,38 @@ def test_inside_count_closed_form_matches_brute_force():
     assert ref.sum() > 0
     assert np.abs(got - ref).max() <= 1
     assert abs(int(got.sum()) - int(ref.sum())) <= max(2, ref.sum() // 10000)
+
+
+def test_coefficient_records_reproduce_the_corner_interpolant():
+    """Exact (float64) coefficient records evaluate to the zero-padded trilinear interpolant and its
+    gradient everywhere, inside and around the volume; float32 records differ by rounding only."""
+    rng = np.random.default_rng(11)
+    vol = rng.standard_normal((5, 6, 7))
+    coef = orm.coef_records(vol, np.float64)
+    assert coef.shape == (6, 7, 8, 8)
+    p = rng.uniform(-1.5, 7.5, size=(4000, 3))
+    val, gx, gy, gz = orm._coef_eval(coef, p)
+    corners, f, _ = orm._corners(vol, p)
+    np.testing.assert_allclose(val, orm._lerp(corners, f), atol=1e-13)
+    h = 1e-6  # the gradient, against central differences of the corner form
+    for axis, g in enumerate((gx, gy, gz)):
+        e = np.zeros(3)
+        e[axis] = h
+        fd = (orm._lerp(*orm._corners(vol, p + e)[:2]) - orm._lerp(*orm._corners(vol, p - e)[:2])) / (2 * h)
+        cellwise = np.all(np.floor(p + e) == np.floor(p - e), axis=1)  # no cell face in between
+        np.testing.assert_allclose(g[cellwise], fd[cellwise], atol=1e-7)
+    c32 = orm.coef_records(vol, np.float32).astype(np.float64)
+    assert np.abs(c32 - coef).max() < 1e-5 * np.abs(vol).max()
+
+
+def test_oracle_forward_and_backward_from_records_equal_the_corner_form():
+    image, label = random_volume(16, 3, 2)
+    k_inv, rt_inv, sdd, h, w = ref_camera_np(8, 8)
+    inp = oracle_inputs(image, label, 3, k_inv, rt_inv, sdd, h, w, 64)
+    coef = orm.coef_records(inp["vol"], np.float64)
+    np.testing.assert_allclose(orm.forward(**inp, coef=coef), orm.forward(**inp), rtol=1e-12, atol=1e-14)
+    go = np.random.default_rng(0).standard_normal((1, 3, 64))
+    a = orm.backward(go, **inp, need_gvol=False, coef=coef)
+    b = orm.backward(go, **inp, need_gvol=False)
+    for k in ("gM", "gsrc", "gtgt", "gstep"):
+        np.testing.assert_allclose(a[k], b[k], rtol=1e-10, atol=1e-12)

@@ -29,7 +29,7 @@ FWD_TOL = 1e-5
 GRAD_TOL = 1e-4
 
 
-@pytest.fixture(params=["quad", "stencil", "linear"], autouse=True)
+@pytest.fixture(params=["quad", "stencil", "coef", "linear"], autouse=True)
 def volume_layout(request, monkeypatch):
     """Every test runs on every volume layout the kernels read."""
     import nanodrr_amd.renderer as renderer
@@ -319,7 +319,8 @@ def test_hip_graph_replays_match_eager():
 def test_packed_layouts_hold_the_corners(dtype, volume_layout):
     """ndrr_pack_volume, bit for bit: stencil record (x0, y0, z0) holds the
     zero-padded corners vol[z0+a, y0+b, x0+c] in (a, b, c) order; quad record
-    (x, y0, z0) holds vol[z0+a, y0+b, x] in (a, b) order."""
+    (x, y0, z0) holds vol[z0+a, y0+b, x] in (a, b) order; coefficient record
+    (x0, y0, z0) holds the cell's interpolant coefficients (oracle.raymarch.coef_records)."""
     from nanodrr_amd._ops import pack_volume
 
     torch.manual_seed(0)
@@ -333,6 +334,13 @@ def test_packed_layouts_hold_the_corners(dtype, volume_layout):
     elif volume_layout == "quad":
         corners = [pad[a:a + D + 1, b:b + H + 1, :] for a in (0, 1) for b in (0, 1)]
         expect = torch.stack(corners, dim=-1)  # (D+1, H+1, W+2, 4)
+    elif volume_layout == "coef":  # coefficients in fp32 in the documented order, scaled, rounded once
+        from tests.helpers import coef_expected
+
+        slots, _ = coef_expected(vol.float().cpu().numpy(), dtype)
+        body = rec[256:].cpu().view(slots.dtype).reshape(slots.shape)
+        assert torch.equal(body, slots)
+        return
     else:
         expect = vol
     got = rec.view(dtype).reshape(expect.shape)
