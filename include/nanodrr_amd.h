@@ -399,6 +399,25 @@ int ndrr_adam_step(float* param, const float* grad, float* m, float* v, int n, f
                    float bias_correction2, void* stream);
 
 /*
+ * Device-side state of the registration loop, so that several iterations can
+ * be captured in one HIP graph and replayed (each launch reads the iteration
+ * from device memory instead of taking it as an argument). No counterpart in
+ * the reference (its Registration module leaves the loop to the caller).
+ *   ndrr_lattice_offsets: offsets[b] in [-1/2, 1/2), a counter-based hash of
+ *     (seed, *iteration, b): the per-problem sample-lattice jitter.
+ *   ndrr_adam_step_scheduled: ndrr_adam_step with the learning rate and bias
+ *     corrections read from row *iteration (clamped to [0, rows)) of
+ *     schedule[rows][4] = {lr of column 0, lr of column 1, 1 - beta1^t,
+ *     1 - beta2^t}; lr_column (0 or 1) picks the parameter group's rate.
+ *   ndrr_counter_advance: *counter += 1.
+ */
+int ndrr_lattice_offsets(const int* iteration, uint32_t seed, float* offsets, int B, void* stream);
+int ndrr_adam_step_scheduled(float* param, const float* grad, float* m, float* v, int n,
+                             const float* schedule, int rows, int lr_column, const int* iteration,
+                             double beta1, double beta2, float eps, void* stream);
+int ndrr_counter_advance(int* counter, void* stream);
+
+/*
  * Zero-normalised cross-correlation of B image pairs of N pixels (fp32):
  * score[b] = mean(xh * yh), xh = (x - mean x) / sqrt(var x + eps) (population
  * variance), yh likewise; grad_x (optional, [B][N]) = d score[b] / d x.

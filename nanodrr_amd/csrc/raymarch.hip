@@ -54,6 +54,9 @@ constexpr int kWave = 64;        // rays per wave: an 8 x 8 detector patch (or 6
 #ifndef NDRR_FWD_CHUNKS_QUAD_1CLASS
 #define NDRR_FWD_CHUNKS_QUAD_1CLASS 3
 #endif
+#ifndef NDRR_FWD_CHUNKS_COEF_1CLASS
+#define NDRR_FWD_CHUNKS_COEF_1CLASS 3
+#endif
 // Pose-gradient partials of the backward kernel: 1 = one per tile (the waves
 // of a workgroup added through LDS after a barrier), 0 = one per wave.
 #ifndef NDRR_BWD_TILE_PARTIALS
@@ -100,11 +103,13 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // carries the chunks' partial sums (and the single-pass adjoint's, 6 per
 // slot) stays small.
 constexpr int fwd_chunks(int layout, int cv) {
-  return cv == 1 && layout == NDRR_LAYOUT_QUAD ? NDRR_FWD_CHUNKS_QUAD_1CLASS : (cv <= 4 ? fwd_chunks(layout) : 1);
+  return cv == 1 && layout == NDRR_LAYOUT_QUAD   ? NDRR_FWD_CHUNKS_QUAD_1CLASS
+         : cv == 1 && layout == NDRR_LAYOUT_COEF ? NDRR_FWD_CHUNKS_COEF_1CLASS
+                                                 : (cv <= 4 ? fwd_chunks(layout) : 1);
 }
 constexpr int kMaxFwdChunks = cmax(cmax(cmax(fwd_chunks(NDRR_LAYOUT_LINEAR), fwd_chunks(NDRR_LAYOUT_STENCIL)),
                                         cmax(fwd_chunks(NDRR_LAYOUT_QUAD), fwd_chunks(NDRR_LAYOUT_COEF))),
-                                   NDRR_FWD_CHUNKS_QUAD_1CLASS);
+                                   cmax(NDRR_FWD_CHUNKS_QUAD_1CLASS, NDRR_FWD_CHUNKS_COEF_1CLASS));
 constexpr int kMaxBwdChunks = cmax(cmax(bwd_chunks(NDRR_LAYOUT_LINEAR), bwd_chunks(NDRR_LAYOUT_STENCIL)),
                                    cmax(bwd_chunks(NDRR_LAYOUT_QUAD), bwd_chunks(NDRR_LAYOUT_COEF)));
 // Pose-gradient partials per tile of the backward kernel (one, or one per

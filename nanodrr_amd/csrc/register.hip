@@ -6,7 +6,9 @@
 //     target, with the gradient: chunk moments over (chunk, image)
 //     workgroups merged in a fixed order (deterministic);
 //   * local (patch) ZNCC over every p x p window, with its gradient;
-//   * the Adam update of the registration parameters.
+//   * the Adam update of the registration parameters, and the device-side
+//     loop state (iteration counter, jitter offsets, schedule table) that lets
+//     several iterations be captured in one HIP graph.
 //
 // The parameterisation follows nanodrr's Registration module
 // (src/nanodrr/registration/registration.py:64-68):
@@ -321,6 +323,55 @@ __global__ __launch_bounds__(kThreads) void k_patch_pixels(const float* __restri
   gx[k] = g;
 }
 
+// Registration-loop state kept on the device, so that a captured HIP graph of
+// several iterations replays with each iteration's own values:
+//   * the iteration counter (an int the loop advances with k_advance);
+//   * the lattice-jitter offsets, drawn from a counter-based hash of
+//     (seed, iteration, problem) instead of a generator's state;
+//   * the Adam schedule, one row per iteration [lr_rot * f, lr_xyz * f,
+//     1 - beta1^t, 1 - beta2^t] (f the cosine factor, t = iteration + 1),
+//     computed once on the host in float64 and rounded to fp32.
+// 32-bit finaliser of Wellons' hash prospector ("lowbias32").
+__device__ inline uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+
+__global__ void k_lattice_offsets(const int* __restrict__ iteration, uint32_t seed, float* __restrict__ offsets,
+                                  int B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  uint32_t h = mix32(seed ^ 0x9e3779b9u);
+  h = mix32(h ^ static_cast<uint32_t>(*iteration));
+  h = mix32(h ^ static_cast<uint32_t>(b));
+  offsets[b] = static_cast<float>(h >> 8) * 0x1p-24f - 0.5f;  // uniform in [-1/2, 1/2)
+}
+
+__global__ void k_adam_scheduled(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                                 float* __restrict__ v, int n, const float* __restrict__ schedule, int rows,
+                                 int lr_column, const int* __restrict__ iteration, float b1, float c1, float b2,
+                                 float c2, float eps) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const int it = min(max(*iteration, 0), rows - 1);
+  const float* row = schedule + 4 * static_cast<int64_t>(it);
+  const float lr = row[lr_column], bc1 = row[2], bc2 = row[3];
+  const float gi = g[i];
+  const float mi = m[i] * b1 + c1 * gi;  // the same update as k_adam
+  const float vi = v[i] * b2 + c2 * (gi * gi);
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+__global__ void k_advance(int* __restrict__ counter) {
+  if (threadIdx.x == 0) counter[0] = counter[0] + 1;
+}
+
 int status() {
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? NDRR_OK : static_cast<int>(e);
@@ -403,6 +454,32 @@ int ndrr_patch_zncc(const float* x, const float* y, int B, int H, int W, int pat
                      1.0f / (P * n), coef, part);
   hipLaunchKernelGGL(k_patch_pixels, dim3(grad_x ? ptiles : 1, B), dim3(kThreads), 0, st, x, y, H, W, patch,
                      coef, part, wtiles, 1.0f / P, score, grad_x);
+  return status();
+}
+
+int ndrr_lattice_offsets(const int* iteration, uint32_t seed, float* offsets, int B, void* stream) {
+  if (!iteration || !offsets || B <= 0) return NDRR_EINVAL;
+  hipLaunchKernelGGL(k_lattice_offsets, dim3((B + 63) / 64), dim3(64), 0, static_cast<hipStream_t>(stream),
+                     iteration, seed, offsets, B);
+  return status();
+}
+
+int ndrr_adam_step_scheduled(float* param, const float* grad, float* m, float* v, int n, const float* schedule,
+                             int rows, int lr_column, const int* iteration, double beta1, double beta2, float eps,
+                             void* stream) {
+  if (!param || !grad || !m || !v || !schedule || !iteration || n <= 0 || rows <= 0) return NDRR_EINVAL;
+  if ((lr_column != 0 && lr_column != 1) || !(beta1 >= 0.0 && beta1 < 1.0) || !(beta2 >= 0.0 && beta2 < 1.0))
+    return NDRR_EINVAL;
+  hipLaunchKernelGGL(k_adam_scheduled, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                     static_cast<hipStream_t>(stream), param, grad, m, v, n, schedule, rows, lr_column, iteration,
+                     static_cast<float>(beta1), static_cast<float>(1.0 - beta1), static_cast<float>(beta2),
+                     static_cast<float>(1.0 - beta2), eps);
+  return status();
+}
+
+int ndrr_counter_advance(int* counter, void* stream) {
+  if (!counter) return NDRR_EINVAL;
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), counter);
   return status();
 }
 

@@ -20,6 +20,9 @@ from . import _lib
 
 # (Cin, Cout, stride) of the six convolutions
 LAYERS = ((1, 32, 1), (32, 64, 2), (64, 64, 1), (64, 128, 2), (128, 128, 1), (128, 256, 2))
+# Volume layout of every render here (single-channel bf16 phantoms): the quad records stay in the
+# Infinity Cache, and single-channel renders run fastest on them.
+RENDER_LAYOUT = "quad"
 
 
 def _stream(t: torch.Tensor) -> int:
@@ -181,7 +184,8 @@ class DrrSource:
         for s in seeds:
             subject = phantom_subject(make_phantom((size,) * 3, (1.0,) * 3, seed=s), 1, torch.bfloat16, device)
             rt = nominal_pose(subject).float().to(device)
-            self.regs.append(Registration(subject, rt, k_inv.to(device), sdd.to(device), det, det))
+            self.regs.append(Registration(subject, rt, k_inv.to(device), sdd.to(device), det, det,
+                                          layout=RENDER_LAYOUT))
 
     @torch.no_grad()
     def render(self, k: int, rot: torch.Tensor, xyz: torch.Tensor, offset: torch.Tensor | None = None,
@@ -197,7 +201,8 @@ class DrrSource:
         step = reg._ray_steps(S)
         from .renderer import march
 
-        return march(reg.subject, M, reg._src, reg._tgt, step, S, reg.height, reg.width, offset=offset)
+        return march(reg.subject, M, reg._src, reg._tgt, step, S, reg.height, reg.width, offset=offset,
+                     layout=RENDER_LAYOUT)
 
 
 def sample_params(n: int, gen: torch.Generator, rot_deg: float, trans_mm: float, device):
@@ -308,17 +313,18 @@ def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_
         xyz_t = ((torch.rand(1, 3, generator=g) * 2 - 1) * trans_mm).to(device)
         fid = fiducials(subject, 32, g).to(device)
         # target and CNN input: the same pose through Registration's own parameterisation
-        tgt_reg = Registration(subject, rt_nom, k200, sdd, 200, 200, per_pose=True)
+        tgt_reg = Registration(subject, rt_nom, k200, sdd, 200, 200, per_pose=True, layout=RENDER_LAYOUT)
         tgt_reg._rot.data.copy_(rot_t)
         tgt_reg._xyz.data.copy_(xyz_t)
         rt_true = tgt_reg.pose.detach()
         target = tgt_reg(1024).detach()
         src128 = DrrSource.__new__(DrrSource)
         k128, sdd128 = detector(1020.0, 200.0 / 128, 128, 128)
-        src128.regs = [Registration(subject, rt_nom, k128.to(device), sdd128.to(device), 128, 128)]
+        src128.regs = [Registration(subject, rt_nom, k128.to(device), sdd128.to(device), 128, 128,
+                                    layout=RENDER_LAYOUT)]
         src128.n_samples = 512
         rot_p, xyz_p = predict(net, src128.render(0, rot_t, xyz_t))
-        raw = Registration(subject, rt_nom, k200, sdd, 200, 200, per_pose=True)
+        raw = Registration(subject, rt_nom, k200, sdd, 200, 200, per_pose=True, layout=RENDER_LAYOUT)
         raw._rot.data.copy_(rot_p)
         raw._xyz.data.copy_(xyz_p)
         res["raw_proposal_mtre"].append(float(mtre(raw.pose.detach(), rt_true, fid)[0]))
@@ -326,7 +332,8 @@ def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_
         for _ in range(refine):
             rot_q, xyz_q = predict(net, src128.render(0, rot_p, xyz_p))
             rot_p, xyz_p = 2 * rot_p - rot_q, 2 * xyz_p - xyz_q
-        reg = Registration(subject, rt_nom.expand(2, 4, 4).contiguous(), k200, sdd, 200, 200, per_pose=True)
+        reg = Registration(subject, rt_nom.expand(2, 4, 4).contiguous(), k200, sdd, 200, 200, per_pose=True,
+                           layout=RENDER_LAYOUT)
         reg._rot.data.copy_(torch.cat([torch.zeros_like(rot_p), rot_p]))
         reg._xyz.data.copy_(torch.cat([torch.zeros_like(xyz_p), xyz_p]))
         res["proposal_mtre"].append(float(mtre(reg.pose.detach()[1:], rt_true, fid)[0]))
@@ -341,7 +348,7 @@ def compare_registration(net: PoseNet, n_test: int = 64, seed0: int = 1000, rot_
         for j, name in enumerate(("nominal", "cnn")):
             res["settled"][name].append(_settled(trace, j))
         # the CNN start again, with a whole schedule of a fifth of the iterations
-        short = Registration(subject, rt_nom, k200, sdd, 200, 200, per_pose=True)
+        short = Registration(subject, rt_nom, k200, sdd, 200, 200, per_pose=True, layout=RENDER_LAYOUT)
         short._rot.data.copy_(rot_p)
         short._xyz.data.copy_(xyz_p)
         register(short, target, n_iters=short_iters, lr_rot=lr_rot, lr_xyz=lr_xyz, n_samples=256, jitter=True, seed=i)

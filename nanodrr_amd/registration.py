@@ -122,9 +122,11 @@ class Registration(torch.nn.Module):
     pose its own parameters (B independent problems).
     """
 
-    def __init__(self, subject, rt_inv, k_inv, sdd, height, width, eps: float = 1e-8, per_pose: bool = False):
+    def __init__(self, subject, rt_inv, k_inv, sdd, height, width, eps: float = 1e-8, per_pose: bool = False,
+                 layout: str | None = None):
         super().__init__()
         self.subject = subject
+        self.layout = layout  # volume layout of the renders (None: renderer.VOLUME_LAYOUT)
         self.register_buffer("rt_inv", rt_inv)
         self.register_buffer("k_inv", k_inv)
         self.register_buffer("sdd", sdd)
@@ -170,7 +172,7 @@ class Registration(torch.nn.Module):
             raise ValueError(f"Unknown backend {backend!r}; expected 'auto', 'hip' or 'triton'")
         step = self._ray_steps(n_samples)
         return march(self.subject, self.camera_to_voxel(), self._src, self._tgt, step, n_samples, self.height,
-                     self.width, offset=offset)
+                     self.width, offset=offset, layout=self.layout)
 
     def _ray_steps(self, n_samples: int) -> torch.Tensor:
         """The ray step (1, N) for n_samples, kept until the rays change (the same tensors at the
@@ -256,8 +258,8 @@ class _ExplicitLoop:
 
         self.reg, self.eps, self.S = reg, float(eps), n_samples
         dev = target.device
+        self.device = dev
         self.lib = _lib.lib()
-        self.stream = _stream(dev)
         B = len(reg.rt_inv)
         self.B = B
         self.shared = reg._rot.shape[0] != B
@@ -268,7 +270,7 @@ class _ExplicitLoop:
         src, tgt = reg._src, reg._tgt
         self.src, self.tgt = src, tgt
         args, packed, layout, _, C = march_inputs(reg.subject, self.M, src, tgt, self.step, n_samples, reg.height,
-                                                  reg.width)
+                                                  reg.width, reg.layout)
         vol, lab = args[0], args[1]
         for a, b in ((args[2], self.M), (args[3], src), (args[4], tgt), (args[5], self.step)):
             if a.data_ptr() != b.data_ptr():
@@ -290,8 +292,9 @@ class _ExplicitLoop:
         self.gxyz = torch.empty(B, 3, device=dev)
 
     def step_(self, offset):
-        """One iteration: returns (score (B,), d(-sum score)/d rot, d(-sum score)/d xyz)."""
-        reg, lib, st = self.reg, self.lib, self.stream
+        """One iteration on torch's current stream (the capturing stream under HIP-graph capture):
+        returns (score (B,), d(-sum score)/d rot, d(-sum score)/d xyz)."""
+        reg, lib, st = self.reg, self.lib, _stream(self.device)
         B, N, D, H, W, S, C, width = self.dims
         if offset is not None and (offset.dtype != torch.float32 or offset.shape != (B,) or not offset.is_contiguous()):
             raise ValueError(f"offset must be a contiguous float32 ({B},) tensor")
@@ -320,6 +323,17 @@ class _ExplicitLoop:
         return self.score, self.grot, self.gxyz
 
 
+def _schedule(n_iters: int, lr_rot: float, lr_xyz: float, b1: float, b2: float, final_frac: float,
+              device) -> torch.Tensor:
+    """(n_iters, 4) fp32 rows [lr_rot f, lr_xyz f, 1 - b1^t, 1 - b2^t] of register()'s Adam schedule
+    (f the cosine factor down to final_frac, t = iteration + 1), computed in float64."""
+    k = torch.arange(n_iters, dtype=torch.float64)
+    f = final_frac + (1 - final_frac) * 0.5 * (1 + torch.cos(math.pi * k / max(n_iters - 1, 1)))
+    t = k + 1
+    rows = torch.stack([lr_rot * f, lr_xyz * f, 1.0 - b1 ** t, 1.0 - b2 ** t], dim=1)
+    return rows.to(torch.float32).to(device).contiguous()
+
+
 def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot: float = 1e-2, lr_xyz: float = 1.0,
              n_samples: int = 256, jitter: bool = False, seed: int = 0, graph: bool = False,
              iters_per_graph: int = 10, eps: float = 1e-7, adjoint: str = "explicit", monitor=None,
@@ -335,69 +349,107 @@ def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot
     when the camera moves one step along its axis, so the similarity to a target
     has local maxima spaced one step apart in depth. `jitter` slides each
     problem's sample lattice by a fresh uniform offset in [-1/2, 1/2) step every
-    iteration (seeded by `seed`), which averages that ripple out.
+    iteration, drawn from a hash of (seed, iteration, problem), which averages
+    that ripple out.
 
     `adjoint="explicit"` (default) calls the kernels and their adjoints
     directly; `adjoint="autograd"` builds the same iteration with autograd
     through the custom ops (the path any other loss would take).
 
-    `monitor(iteration, reg)`, when given, is called after every
-    `monitor_every` iterations and after the last one, with the parameters
-    of that iteration applied (e.g. to record the mTRE trajectory).
+    The explicit loop keeps its state on the device (iteration counter, jitter
+    offsets, the Adam schedule as a table), so `graph=True` can capture
+    `iters_per_graph` iterations in one HIP graph and replay it: the first
+    (n_iters - 1) % iters_per_graph + 1 iterations run eagerly (every kernel
+    has then run once), the rest as replays. A captured iteration is the same
+    launches as an eager one, on the capturing stream, with no allocation,
+    memset or host synchronisation, so the results are bitwise the same.
+    Graph mode needs per-problem parameters (per_pose=True).
 
-    `graph=True` (capturing `iters_per_graph` iterations in one HIP graph) is
-    switched off: capturing ten iterations ended the process with a
-    segmentation fault inside the runtime's capture end, with no cause found
-    in this library, so the loop runs eagerly.
+    `monitor(iteration, reg)`, when given, is called after every
+    `monitor_every` iterations (in graph mode: after the eager iterations and
+    after every replay) and after the last one, with the parameters of that
+    iteration applied (e.g. to record the mTRE trajectory).
     """
     if adjoint not in ("explicit", "autograd"):
         raise ValueError(f"adjoint must be 'explicit' or 'autograd', got {adjoint!r}")
-    if graph:
-        raise NotImplementedError(
-            "HIP-graph capture of the registration step is switched off (a ten-iteration capture crashed in "
-            "the runtime's capture end with no cause found); call register(..., graph=False)")
+    if graph and adjoint != "explicit":
+        raise ValueError("graph=True captures the explicit loop: use adjoint='explicit'")
+    if graph and iters_per_graph < 1:
+        raise ValueError("iters_per_graph must be at least 1")
     params = [(reg._rot, lr_rot), (reg._xyz, lr_xyz)]
     for p, _ in params:
         if not (p.is_cuda and p.dtype == torch.float32 and p.is_contiguous()):
             raise ValueError("register() updates contiguous float32 device parameters")
+    if graph and reg._rot.shape[0] != len(reg.rt_inv):
+        raise ValueError("graph=True needs per-problem parameters (Registration(..., per_pose=True))")
+    dev = target.device
     moments = [(torch.zeros_like(p), torch.zeros_like(p)) for p, _ in params]
     b1, b2, adam_eps, final_frac = 0.9, 0.999, 1e-8, 0.02
     target = target.detach().to(torch.float32).contiguous()
     B = target.shape[0]
-    score = torch.zeros(B, device=target.device)  # n_iters == 0
-    gen = torch.Generator(device=target.device).manual_seed(int(seed)) if jitter else None
-    descent = torch.full((B,), -1.0, device=target.device)  # d(-sum of scores) / d score
+    score = torch.zeros(B, device=dev)  # n_iters == 0
+    schedule = _schedule(max(n_iters, 1), lr_rot, lr_xyz, b1, b2, final_frac, dev)
+    counter = torch.zeros(1, dtype=torch.int32, device=dev)
+    offsets = torch.empty(B, device=dev) if jitter else None
+    lib = _lib.lib()
+    seed32 = int(seed) & 0xFFFFFFFF
+
+    def draw_offsets():
+        if jitter:
+            _lib.check(lib.ndrr_lattice_offsets(counter.data_ptr(), seed32, offsets.data_ptr(), B, _stream(dev)),
+                       "ndrr_lattice_offsets")
+        return offsets
+
+    def adam(grads):
+        st = _stream(dev)
+        for col, ((p, _), (m, v), g) in enumerate(zip(params, moments, grads)):
+            _lib.check(lib.ndrr_adam_step_scheduled(p.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(),
+                                                    p.numel(), schedule.data_ptr(), schedule.shape[0], col,
+                                                    counter.data_ptr(), b1, b2, adam_eps, st),
+                       "ndrr_adam_step_scheduled")
+        _lib.check(lib.ndrr_counter_advance(counter.data_ptr(), st), "ndrr_counter_advance")
+
+    def report(it):
+        if monitor is not None:
+            monitor(it, reg)
+
     if adjoint == "explicit":
-        with torch.no_grad(), torch.cuda.device(target.device):
+        with torch.no_grad(), torch.cuda.device(dev):
             loop = _ExplicitLoop(reg, target, n_samples, eps, jitter)
-            lib, st = _lib.lib(), loop.stream
-            ptrs = [(p.data_ptr(), m.data_ptr(), v.data_ptr(), p.numel(), lr)
-                    for (p, lr), (m, v) in zip(params, moments)]
-            for it in range(n_iters):
-                offset = torch.empty(B, device=target.device).uniform_(-0.5, 0.5, generator=gen) if jitter else None
-                s, grot, gxyz = loop.step_(offset)
-                frac = final_frac + (1 - final_frac) * 0.5 * (1 + math.cos(math.pi * it / max(n_iters - 1, 1)))
-                bc1, bc2 = 1.0 - b1 ** (it + 1), 1.0 - b2 ** (it + 1)
-                for (p, m, v, n, lr), g in zip(ptrs, (grot, gxyz)):
-                    _lib.check(lib.ndrr_adam_step(p, g.data_ptr(), m, v, n, lr * frac, b1, b2, adam_eps, bc1, bc2, st),
-                               "ndrr_adam_step")
-                score = s
-                if monitor is not None and ((it + 1) % monitor_every == 0 or it + 1 == n_iters):
-                    monitor(it + 1, reg)
+
+            def iteration():
+                s, grot, gxyz = loop.step_(draw_offsets())
+                adam((grot, gxyz))
+                return s, grot, gxyz
+
+            head = n_iters if not graph else (n_iters - 1) % iters_per_graph + 1 if n_iters else 0
+            for it in range(head):
+                score, grot, gxyz = iteration()
+                if (it + 1) % monitor_every == 0 or it + 1 == head:
+                    report(it + 1)
+            if n_iters > head:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    for _ in range(iters_per_graph):
+                        score, grot, gxyz = iteration()
+                done = head
+                while done < n_iters:
+                    g.replay()
+                    done += iters_per_graph
+                    report(done)
             if n_iters:
                 reg._rot.grad, reg._xyz.grad = grot.clone(), gxyz.clone()
         return score.clone()
+    descent = torch.full((B,), -1.0, device=dev)  # d(-sum of scores) / d score
     for it in range(n_iters):
         for p, _ in params:
             p.grad = None
-        offset = torch.empty(B, device=target.device).uniform_(-0.5, 0.5, generator=gen) if jitter else None
+        offset = draw_offsets().clone() if jitter else None
         s = zncc(reg(n_samples, offset=offset), target, eps)
         s.backward(descent)
-        frac = final_frac + (1 - final_frac) * 0.5 * (1 + math.cos(math.pi * it / max(n_iters - 1, 1)))
         with torch.no_grad():
-            for (p, lr), (m, v) in zip(params, moments):
-                _adam_step(p, p.grad, m, v, lr * frac, b1, b2, adam_eps, it + 1)
+            adam([p.grad.contiguous() for p, _ in params])
         score = s.detach()
-        if monitor is not None and ((it + 1) % monitor_every == 0 or it + 1 == n_iters):
-            monitor(it + 1, reg)
+        if (it + 1) % monitor_every == 0 or it + 1 == n_iters:
+            report(it + 1)
     return score.clone()

@@ -143,9 +143,9 @@ def render_hip(subject, rt_inv, src, tgt, step_size, n_samples, height, width) -
     return march(subject, M, src, tgt, step_size, n_samples, height, width)
 
 
-def volume_layout(image: torch.Tensor) -> str:
-    """The layout the kernels read for this volume under VOLUME_LAYOUT."""
-    layout = VOLUME_LAYOUT
+def volume_layout(image: torch.Tensor, layout: str | None = None) -> str:
+    """The layout the kernels read for this volume: `layout` when given, else VOLUME_LAYOUT."""
+    layout = VOLUME_LAYOUT if layout is None else layout
     if layout == "auto":
         if image.requires_grad or not packed_index_fits(*image.shape[-3:]):
             layout = "linear"
@@ -154,7 +154,7 @@ def volume_layout(image: torch.Tensor) -> str:
         else:
             layout = "quad"
     if layout not in _LAYOUTS:
-        raise ValueError(f"VOLUME_LAYOUT must be 'auto' or one of {_LAYOUTS}, got {VOLUME_LAYOUT!r}")
+        raise ValueError(f"the volume layout must be 'auto' or one of {_LAYOUTS}, got {layout!r}")
     return layout
 
 
@@ -170,7 +170,7 @@ def packed_index_fits(D: int, H: int, W: int) -> bool:
     return (D + 1) * (H + 1) < 2**24 and W + 2 < 2**24
 
 
-def march_inputs(subject, M, src, tgt, step_size, n_samples, height, width):
+def march_inputs(subject, M, src, tgt, step_size, n_samples, height, width, layout=None):
     """Checked kernel arguments of `march`: ((vol, lab, M, src, tgt, step, n_samples, C, width),
     packed volume or None, layout name, B, C). M is the (B|1, 3, 4) affine, or for `march_pose`
     the (B|1, 4, 4) rt_inv; either is expanded to B poses."""
@@ -189,7 +189,7 @@ def march_inputs(subject, M, src, tgt, step_size, n_samples, height, width):
         M = M.expand(B, -1, -1)
     if not fused_supported(subject, B, height * width):
         raise ValueError("inputs exceed the hip kernel's limits (one volume, 32-bit indexing, <= 8 classes)")
-    layout = volume_layout(image)
+    layout = volume_layout(image, layout)
     packed = None if layout == "linear" else subject.packed_volume(layout)
     lab = subject.packed_labels(layout) if C > 1 else vol.new_empty(0, dtype=torch.int8)
     args = (vol, lab, M.contiguous(), src.to(torch.float32).contiguous(), tgt.to(torch.float32).contiguous(),
@@ -197,13 +197,14 @@ def march_inputs(subject, M, src, tgt, step_size, n_samples, height, width):
     return args, packed, layout, B, C
 
 
-def march(subject, M, src, tgt, step_size, n_samples, height, width, offset=None) -> torch.Tensor:
+def march(subject, M, src, tgt, step_size, n_samples, height, width, offset=None, layout=None) -> torch.Tensor:
     """(B, C, H, W) line integrals of the rays src -> tgt (camera space) with the
     camera -> voxel-index affines M (B|1, 3, 4) through the subject's volume.
     `offset` (B,), in units of the sample step, slides each pose's sample lattice
     along its rays (sample s at t = (s + offset) / (n_samples - 1); the step is
-    unchanged); the kernels take it directly unless the two-pass adjoint runs."""
-    args, packed, layout, B, C = march_inputs(subject, M, src, tgt, step_size, n_samples, height, width)
+    unchanged); the kernels take it directly unless the two-pass adjoint runs. `layout` overrides
+    VOLUME_LAYOUT for this call."""
+    args, packed, layout, B, C = march_inputs(subject, M, src, tgt, step_size, n_samples, height, width, layout)
     if ADJOINT not in _ADJOINTS:
         raise ValueError(f"ADJOINT must be one of {_ADJOINTS}, got {ADJOINT!r}")
     gcam = any(t.requires_grad for t in args[3:6])

@@ -235,5 +235,7 @@ def test_register_rejects_unknown_adjoint_and_graph_mode():
 
     with pytest.raises(ValueError, match="adjoint"):
         register(None, None, adjoint="finite-differences")
-    with pytest.raises(NotImplementedError, match="graph"):
-        register(None, None, graph=True)
+    with pytest.raises(ValueError, match="explicit"):
+        register(None, None, graph=True, adjoint="autograd")
+    with pytest.raises(ValueError, match="iters_per_graph"):
+        register(None, None, graph=True, iters_per_graph=0)

@@ -180,21 +180,65 @@ def test_shared_parameters_sum_over_batch():
     torch.testing.assert_close(shared[0], per._rot.grad.sum(0), rtol=1e-5, atol=1e-6)
 
 
-@pytest.mark.skip(reason="HIP-graph capture of the registration step is switched off: a ten-iteration capture "
-                         "crashed (segmentation fault in the runtime's capture end) with no cause found")
-def test_graph_replay_matches_eager_iterations():
-    from nanodrr_amd import render
+class _RecordingLib:
+    """The C ABI with every call recorded: (name, whether torch's current stream was capturing, the
+    stream argument, torch's current stream, bytes torch had allocated)."""
+
+    def __init__(self, lib):
+        self._lib, self.calls = lib, []
+
+    def __getattr__(self, name):
+        fn = getattr(self._lib, name)
+
+        def call(*args):
+            capturing = torch.cuda.is_current_stream_capturing()
+            self.calls.append((name, capturing, args[-1] if args else None, torch.cuda.current_stream().cuda_stream,
+                               torch.cuda.memory_allocated()))
+            return fn(*args)
+
+        return call
+
+
+def test_graph_replay_matches_eager_iterations(monkeypatch):
+    """register(graph=True): 11 iterations as 2 eager ones and three replays of a 3-iteration
+    graph, against 11 eager iterations of the same problems. While capturing, every library call is
+    a launch on the capturing stream and torch allocates nothing; after each replay (with eager work
+    and a host synchronisation in between: the monitor) the parameters equal the eager loop's at the
+    same iteration, bit for bit, and so does the final score."""
+    from nanodrr_amd import _lib, render
+    from nanodrr_amd import registration as registration_mod
     from nanodrr_amd.registration import register
 
     subject, prob, reg_g = _phantom_registration(B=4, seed=3)
     _, _, reg_e = _phantom_registration(B=4, seed=3)
     target = render(subject, reg_g.k_inv, prob.rt_true.float().to(DEV), reg_g.sdd, reg_g.height, reg_g.width, 64)
-    s_g = register(reg_g, target, n_iters=9, n_samples=64, graph=True, iters_per_graph=3)
-    s_e = register(reg_e, target, n_iters=9, n_samples=64, graph=False)
+    traj = {"eager": {}, "graph": {}}
+
+    def monitor(name):
+        def record(it, reg):
+            traj[name][it] = (reg._rot.detach().clone(), reg._xyz.detach().clone())
+            assert torch.isfinite(reg.pose).all().item()  # eager work and a host sync between replays
+
+        return record
+
+    s_e = register(reg_e, target, n_iters=11, n_samples=64, jitter=True, seed=5, monitor=monitor("eager"),
+                   monitor_every=1)
+    rec = _RecordingLib(_lib.lib())
+    monkeypatch.setattr(registration_mod._lib, "lib", lambda: rec)
+    s_g = register(reg_g, target, n_iters=11, n_samples=64, jitter=True, seed=5, graph=True, iters_per_graph=3,
+                   monitor=monitor("graph"))
     torch.cuda.synchronize()
-    torch.testing.assert_close(reg_g._rot, reg_e._rot, rtol=1e-5, atol=1e-7)
-    torch.testing.assert_close(reg_g._xyz, reg_e._xyz, rtol=1e-5, atol=1e-5)
-    torch.testing.assert_close(s_g, s_e, rtol=1e-5, atol=1e-6)
+    captured = [c for c in rec.calls if c[1]]
+    launches = {"ndrr_lattice_offsets", "ndrr_pose_compose", "ndrr_raymarch_fwd_jac", "ndrr_zncc",
+                "ndrr_raymarch_bwd_jac_se3", "ndrr_adam_step_scheduled", "ndrr_counter_advance"}
+    assert len(captured) == 3 * 8, [c[0] for c in captured]
+    assert {c[0] for c in captured} == launches
+    assert all(c[2] == c[3] for c in captured)  # on the capturing stream
+    assert len({c[4] for c in captured}) == 1  # no torch allocation inside the capture
+    assert sorted(traj["graph"]) == [2, 5, 8, 11]
+    for it, (rot, xyz) in traj["graph"].items():
+        assert torch.equal(rot, traj["eager"][it][0]) and torch.equal(xyz, traj["eager"][it][1]), it
+    assert torch.equal(s_g, s_e)
 
 
 def test_registration_converges_on_phantom():

@@ -67,6 +67,15 @@ def _pick_rays(n, k, seed):
     return np.sort(np.random.default_rng(seed).choice(n, size=k, replace=False))
 
 
+def _clean_rays(M, src, tgt, S, shape, rays, labels):
+    """The rays with no sample within 1e-4 voxel of a cell face or (with labels) a nearest-label tie,
+    where the march is discontinuous: there an fp32 sample position (within ~1 ulp of the float64
+    one) may fall on the other side and change a derivative or a class, whatever the storage. On the
+    other rays fp32 and float64 evaluate the same cells and classes."""
+    d = orm.discontinuity_distance(_np(M), _np(src), _np(tgt), S, shape, rays, labels=labels)
+    return rays[(d > 1e-4).all(axis=0)]
+
+
 def _check_channels(ref, out, n_classes, tol):
     scale = np.abs(ref).max()
     if n_classes == 1:
@@ -126,7 +135,7 @@ def _per_ray_check(ref, got, what, frac=1e-2, tol=1e-4):
     err = np.abs(ref - got).max(axis=-1) / np.abs(ref).max()
     bad = (err > tol).mean()
     print(f"{what}: {bad:.2%} of rays beyond {tol:g}, median error {np.median(err):.1e}")
-    assert bad < frac, (what, bad)
+    assert bad <= frac, (what, bad)
 
 
 def _per_ray_gM(M, src, tgt, gsrc, gtgt):
@@ -144,29 +153,44 @@ def _per_ray_gM(M, src, tgt, gsrc, gtgt):
     return gM
 
 
-@pytest.mark.parametrize("layout", ["quad", "stencil", "linear"])
-def test_bench_workload_pose_gradient_ray_subset(bench_fp32, layout):
-    """Pose gradients of 4 poses x 768 sampled rays: per-ray endpoint and step
-    gradients vs the oracle, and the kernel's fixed-order pose reduction vs the
-    sum of its own per-ray terms."""
-    from nanodrr_amd._ops import pack_volume
+@pytest.mark.parametrize("layout", ["quad", "stencil", "coef", "linear"])
+@pytest.mark.parametrize("adjoint", ["two", "single"])
+def test_bench_workload_pose_gradient_ray_subset(bench_fp32, layout, adjoint):
+    """fp32 pose gradients of 4 poses x 768 sampled rays of the 4-label workload against the float64
+    oracle: on the clean rays (no sample within 1e-4 voxel of a cell face or label tie) gM to 2e-5
+    and every ray's endpoint and step gradients to 1e-4, no outlier allowed; on all rays the
+    per-ray checks allow the flip-prone share of rays; and the kernel's fixed-order pose
+    reduction equals the sum of its own per-ray terms."""
+    from nanodrr_amd._ops import fused_raymarch_bwd_jac, fused_raymarch_jac, pack_volume
 
     ph, subject, k_inv, rt_inv, sdd = bench_fp32
     vol_t = subject.image[0, 0]
     packed = None if layout == "linear" else pack_volume(vol_t, layout)
+    lab_t = subject.packed_labels(layout)
     M, src, tgt, step = _op_inputs(subject, k_inv, rt_inv, sdd, 200, 256)
     M = M[:4].contiguous()
-    rays = _pick_rays(40000, 768, 3)
-    tsub, ssub = _subset(tgt, step, rays)
     vol, lab = _oracle_vol(ph, 4)
-    go = torch.randn(4, 4, len(rays), generator=torch.Generator().manual_seed(5)).to(DEV)
-    _, gM, gsrc, gtgt, gstep = fused_raymarch_bwd(go, vol_t, subject.packed_labels(layout), M, src, tsub, ssub,
-                                                  256, 4, 0, False, True, packed, layout)
-    g = orm.backward(_np(go), vol, lab, 4, _np(M), _np(src), _np(tsub), _np(ssub), 256, need_gvol=False)
-    for key, got in (("gstep", gstep), ("gtgt", gtgt), ("gsrc", gsrc)):
-        _per_ray_check(g[key], _np(got), key)
-    rebuilt = _per_ray_gM(_np(M), _np(src), np.broadcast_to(_np(tsub), (4, len(rays), 3)), _np(gsrc), _np(gtgt))
-    assert rel_to_max(rebuilt, _np(gM)) < 1e-5
+    rays = _pick_rays(40000, 768, 3)
+    clean = _clean_rays(M, src, tgt, 256, vol.shape, rays, True)
+    assert len(clean) > 0.7 * len(rays)
+    for name, rs, frac in (("clean", clean, 0.0), ("all", rays, 5e-3)):
+        tsub, ssub = _subset(tgt, step, rs)
+        go = torch.randn(4, 4, len(rs), generator=torch.Generator().manual_seed(5)).to(DEV)
+        if adjoint == "two":
+            _, gM, gsrc, gtgt, gstep = fused_raymarch_bwd(go, vol_t, lab_t, M, src, tsub, ssub, 256, 4, 0, False,
+                                                          True, packed, layout)
+        else:
+            _, jac = fused_raymarch_jac(vol_t, lab_t, M, src, tsub, ssub, 256, 4, 0, 7, packed, layout)
+            gM, gsrc, gtgt, gstep = fused_raymarch_bwd_jac(go, jac, M, src, tsub, ssub, 0, True)
+        g = orm.backward(_np(go), vol, lab, 4, _np(M), _np(src), _np(tsub), _np(ssub), 256, need_gvol=False)
+        err = rel_to_max(g["gM"], _np(gM))
+        print(f"{layout} {adjoint}-pass, {name} rays ({len(rs)}): pose-gradient error {err:.2e}")
+        if name == "clean":
+            assert err < 2e-5
+        for key, got in (("gstep", gstep), ("gtgt", gtgt), ("gsrc", gsrc)):
+            _per_ray_check(g[key], _np(got), f"{name} {key}", frac=frac)
+        rebuilt = _per_ray_gM(_np(M), _np(src), np.broadcast_to(_np(tsub), (4, len(rs), 3)), _np(gsrc), _np(gtgt))
+        assert rel_to_max(rebuilt, _np(gM)) < 1e-5
 
 
 def test_bench_workload_exact_linearity(bench_fp32):
@@ -220,12 +244,16 @@ def test_render_fwd_bwd_matches_op_path(bench_fp32):
 
 
 @pytest.mark.parametrize("layout", ["quad", "stencil"])
-@pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
-def test_half_volume_error_against_float64(dtype, layout, bench_fp32):
-    """bf16/fp16 storage (stencil records), fp32 accumulation. Against the float64
-    oracle of the stored (rounded) volume the kernel's own error is <= 1e-5 (image)
-    and <= 1e-4 (pose gradient); against the full-precision volume the image error
-    stays <= 1e-3 and the pose-gradient error is what the storage rounding costs."""
+@pytest.mark.parametrize("dtype, grad_bound", [(torch.bfloat16, 2.5e-2), (torch.float16, 1.5e-3)])
+def test_half_volume_error_against_float64(dtype, grad_bound, layout, bench_fp32):
+    """bf16/fp16 storage in the corner layouts (the voxel values rounded), fp32 accumulation, 2 poses
+    x 400 sampled rays, on the clean rays (no sample within 1e-4 voxel of a cell face or label tie).
+    Against the float64 oracle of the stored (rounded) volume the kernel's own error is <= 1e-5
+    (image) and <= 2e-5 (pose gradient), every ray's endpoint gradients within 1e-4. Against the
+    fp32 volume the image error is <= 1e-3 and the pose gradient error is what rounding the voxel
+    values costs: measured 1.2e-2 (bf16) and 6e-4 (fp16) at this workload, so bf16 corner
+    records miss 1e-3 on the gradient (the coefficient layout meets it, test_coef_gpu.py); the
+    bounds are about twice the measured figures."""
     from nanodrr_amd._ops import pack_volume
 
     ph, _, k_inv, rt_inv, sdd = bench_fp32
@@ -234,24 +262,26 @@ def test_half_volume_error_against_float64(dtype, layout, bench_fp32):
     packed = pack_volume(vol_t, layout)
     M, src, tgt, step = _op_inputs(subject, k_inv, rt_inv, sdd, 200, 256)
     lab_t = subject.packed_labels(layout)
-    out = fused_raymarch(vol_t, lab_t, M, src, tgt, step, 256, 4, 200, packed, layout)
-    rays = _pick_rays(40000, 400, 6)
-    tsub, ssub = _subset(tgt, step, rays)
     Ms = M[:2].contiguous()
+    rays = _clean_rays(Ms, src, tgt, 256, vol_t.shape, _pick_rays(40000, 400, 6), True)
+    out = fused_raymarch(vol_t, lab_t, M, src, tgt, step, 256, 4, 200, packed, layout)
+    tsub, ssub = _subset(tgt, step, rays)
     go = torch.randn(2, 4, len(rays), generator=torch.Generator().manual_seed(7)).to(DEV)
     _, gM, gsrc, gtgt, _ = fused_raymarch_bwd(go, vol_t, lab_t, Ms, src, tsub, ssub, 256, 4, 0, False,
                                               True, packed, layout)
-    for name, vol_dtype, img_tol, grad_tol in (("stored", dtype, 1e-5, 5e-2), ("fp32", torch.float32, 1e-3, 5e-2)):
+    for name, vol_dtype, img_tol, grad_tol in (("stored", dtype, 1e-5, 2e-5), ("fp32", torch.float32, 1e-3,
+                                                                                  grad_bound)):
         vol, lab = _oracle_vol(ph, 4, vol_dtype)
         ref = orm.forward(vol, lab, 4, _np(M), _np(src), _np(tgt), _np(step), 256, rays=rays)
         err = rel_to_max(ref.sum(1), _np(out)[:, :, rays].sum(1))
         g = orm.backward(_np(go), vol, lab, 4, _np(Ms), _np(src), _np(tsub), _np(ssub), 256, need_gvol=False)
         gerr = rel_to_max(g["gM"], _np(gM))
-        print(f"{dtype} vs float64 oracle of the {name} volume: image {err:.2e}, pose gradient {gerr:.2e}")
+        print(f"{dtype} {layout} vs float64 oracle of the {name} volume ({len(rays)} clean rays): image {err:.2e}, "
+              f"pose gradient {gerr:.2e}")
         assert err < img_tol and gerr < grad_tol
         if name == "stored":  # the kernel's own arithmetic, ray by ray
-            _per_ray_check(g["gtgt"], _np(gtgt), f"{dtype} gtgt")
-            _per_ray_check(g["gsrc"], _np(gsrc), f"{dtype} gsrc")
+            _per_ray_check(g["gtgt"], _np(gtgt), f"{dtype} gtgt", frac=0.0)
+            _per_ray_check(g["gsrc"], _np(gsrc), f"{dtype} gsrc", frac=0.0)
 
 
 def test_compensated_accumulation_long_rays(bench_fp32):
@@ -282,10 +312,10 @@ def test_mixed_precision_workload_against_float64(dtype, layout):
     the pose-gradient error on sampled rays, against the float64 oracle of the stored volume (the
     kernels' own arithmetic) and of the fp32 volume (what the storage rounding costs).
 
-    Image: <= 1e-5 of the stored volume's. Pose gradient: the two adjoints agree to 1e-5 (the same
-    sample positions); against float64 the per-ray endpoint gradients agree ray by ray except
-    for rays with a sample within ~1 ulp of a cell face (the trilinear derivative jumps there),
-    so the pose-gradient sum of 300 rays is held to 1e-3."""
+    Image: <= 1e-5 of the stored volume's, <= 1e-3 of the fp32 volume's. Pose gradient: the two
+    adjoints agree to 1e-5 (the same sample positions); on the clean rays (no sample within 1e-4
+    voxel of a cell face or label tie) the kernels agree with float64 on the stored volume to 5e-5
+    and every ray's endpoint gradients to 1e-4."""
     from nanodrr_amd._ops import fused_raymarch_bwd_jac, fused_raymarch_jac, pack_volume
 
     ph = _c2_phantom()
@@ -298,9 +328,9 @@ def test_mixed_precision_workload_against_float64(dtype, layout):
     lab_t = subject.packed_labels(layout)
     M, src, tgt, step = _op_inputs(subject, k_inv, rt_inv, sdd, 512, 512)
     out, _ = fused_raymarch_jac(vol_t, lab_t, M, src, tgt, step, 512, 2, 512, 6, packed, layout)
-    rays = _pick_rays(512 * 512, 300, 21)
-    tsub, ssub = _subset(tgt, step, rays)
     Ms = M[:2].contiguous()
+    rays = _clean_rays(Ms, src, tgt, 512, vol_t.shape, _pick_rays(512 * 512, 300, 21), True)
+    tsub, ssub = _subset(tgt, step, rays)
     go = torch.randn(2, 2, len(rays), generator=torch.Generator().manual_seed(22)).to(DEV)
     _, jac = fused_raymarch_jac(vol_t, lab_t, Ms, src, tsub, ssub, 512, 2, 0, 6, packed, layout)
     gM_single = fused_raymarch_bwd_jac(go, jac, Ms, src, tsub, ssub, 0, False)[0]
@@ -319,10 +349,10 @@ def test_mixed_precision_workload_against_float64(dtype, layout):
               f"{pix.mean():.2e}; pose gradient {gerr:.2e}")
         if name == "stored":
             _check_channels(ref, got, 2, 1e-5)
-            assert gerr < 1e-3
-            _per_ray_check(g["gtgt"], _np(gtgt), f"{dtype} {layout} gtgt")
-            _per_ray_check(g["gsrc"], _np(gsrc), f"{dtype} {layout} gsrc")
+            assert gerr < 5e-5
+            _per_ray_check(g["gtgt"], _np(gtgt), f"{dtype} {layout} gtgt", frac=0.0)
+            _per_ray_check(g["gsrc"], _np(gsrc), f"{dtype} {layout} gsrc", frac=0.0)
         else:
-            assert pix.max() < 2e-3
+            assert pix.max() < 1e-3
     del packed, subject
     torch.cuda.empty_cache()
