@@ -482,6 +482,36 @@ int ndrr_conv3x3_bwd_weight(const void* dy, const void* x, float* workspace, flo
                             float* dbias, int N, int H, int W, int Cin, int Cout,
                             int stride, void* stream);
 
+/*
+ * The pose-proposal CNN's head (no counterpart in the reference): 2 x 2
+ * average pooling of the last feature map feat[N][FH][FW][C] (bf16, C a
+ * multiple of 64), a fully connected layer of J = 256 units with ReLU and an
+ * output layer of O <= 64 units, on the bf16 matrix cores with fp32
+ * accumulation. The pooled features are P[N][K], K = (FH/2)(FW/2)C, with
+ * k = (py (FW/2) + px) C + c, and their transpose Pt[K][Np], Np = N rounded up
+ * to a multiple of 32 (zero columns past N).
+ *   ndrr_head_cast_weights: the fp32 first-layer weights w1[J][K] as bf16, in
+ *     both orientations (w1_bf16[J][K], w1t_bf16[K][J]).
+ *   ndrr_head_pool: P and Pt from feat.
+ *   ndrr_head_fwd: H[N][J] = relu(b1 + P w1^T) (fp32) and out[N][O] = b2 + H w2^T
+ *     (fp32); workspace of ndrr_head_workspace_floats floats (k-slice
+ *     partials, added in a fixed order); K a multiple of 1024.
+ *   ndrr_head_bwd: for dout[N][O]: dh[N][J] (fp32) = [H > 0] dout w2, as bf16 in
+ *     dH_bf16[N][J] and dHt_bf16[J][Np]; dw1[J][K] = dh^T P, db1, dw2[O][J],
+ *     db2 (fp32); dfeat (bf16, feat's shape) = [feat > 0] (dh w1) spread over
+ *     the pooling windows / 4 (the ReLU mask of the layer that made feat).
+ * Deterministic (no atomics).
+ */
+size_t ndrr_head_workspace_floats(int N, int FH, int FW, int C, int J);
+int ndrr_head_cast_weights(const float* w1, void* w1_bf16, void* w1t_bf16, int J, int K, void* stream);
+int ndrr_head_pool(const void* feat, void* P, void* Pt, int N, int FH, int FW, int C, void* stream);
+int ndrr_head_fwd(const void* P, const void* w1_bf16, const float* b1, const float* w2, const float* b2,
+                  float* workspace, float* H, float* out, int N, int K, int J, int O, void* stream);
+int ndrr_head_bwd(const float* dout, const float* H, const float* w2, const void* w1t_bf16, const void* Pt,
+                  const void* feat, float* dh, void* dH_bf16, void* dHt_bf16, float* dw1, float* db1,
+                  float* dw2, float* db2, void* dfeat, int N, int FH, int FW, int C, int J, int O,
+                  void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,6 +90,11 @@ SIGNATURES: dict[str, tuple] = {
     "ndrr_conv3x3_bwd_data": (_c_int, [_c_p, _c_p, _c_p, _c_p] + [_c_int] * 7 + [_c_p]),
     "ndrr_conv3x3_bwd_weight_workspace_floats": (_c_size, [_c_int] * 6),
     "ndrr_conv3x3_bwd_weight": (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_p] + [_c_int] * 6 + [_c_p]),
+    "ndrr_head_workspace_floats": (_c_size, [_c_int] * 5),
+    "ndrr_head_cast_weights": (_c_int, [_c_p, _c_p, _c_p, _c_int, _c_int, _c_p]),
+    "ndrr_head_pool": (_c_int, [_c_p, _c_p, _c_p] + [_c_int] * 4 + [_c_p]),
+    "ndrr_head_fwd": (_c_int, [_c_p] * 8 + [_c_int] * 4 + [_c_p]),
+    "ndrr_head_bwd": (_c_int, [_c_p] * 14 + [_c_int] * 6 + [_c_p]),
     "ndrr_pose_compose": (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_int, _c_p]),
     "ndrr_pose_compose_bwd": (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_int, _c_p]),
     "ndrr_adam_step": (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_int, ctypes.c_float, ctypes.c_double, ctypes.c_double]

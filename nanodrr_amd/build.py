@@ -13,7 +13,7 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
-SOURCES = [PKG / "csrc" / name for name in ("raymarch.hip", "camera.hip", "register.hip", "conv.hip")]
+SOURCES = [PKG / "csrc" / name for name in ("raymarch.hip", "camera.hip", "register.hip", "conv.hip", "head.hip")]
 OUTPUT = PKG / "libnanodrr_amd.so"
 
 # -packed-fp32-ops: keep the backend from forming packed fp32 instructions.

@@ -1,13 +1,14 @@
 """Pose-proposal CNN: a learned initialiser for the registration loop.
 
 No counterpart in nanodrr. Six 3x3 convolutions (32 -> 256 channels, stride 2
-on every second layer) on a 128 x 128 log-normalised DRR, global average
+on every second layer) on a 128 x 128 log-normalised DRR, 2 x 2 average
 pooling and a two-layer MLP regress the 6-dof perturbation (rotation vector,
 translation in mm) of `Registration`'s parameterisation about a nominal
-pose. The convolutions run on the native bf16 matrix-core kernels
-(ndrr_conv3x3_*: NHWC bf16 activations and weights, fp32 accumulation,
-fp32 master weights); pooling, MLP and losses are small and stay in torch.
-Training DRRs are rendered on the fly by the native renderer.
+pose. The convolutions and the head (pooling and the two-layer MLP) run on
+the native bf16 matrix-core kernels (ndrr_conv3x3_*, ndrr_head_*: NHWC bf16
+activations and weights, fp32 accumulation, fp32 master weights); the losses
+and the optimiser (torch's fused Adam) stay in torch. Training DRRs are
+rendered on the fly by the native renderer.
 """
 
 from __future__ import annotations
@@ -85,6 +86,109 @@ def conv3x3_bwd_weight(dy: torch.Tensor, x: torch.Tensor, stride: int, bias: boo
     return (dw, db) if bias else dw
 
 
+HIDDEN = 256  # units of the head's fully connected layer
+OUTPUTS = 6
+
+
+def head_forward(feat: torch.Tensor, w1: torch.Tensor, b1: torch.Tensor, w2: torch.Tensor, b2: torch.Tensor):
+    """The head on the native kernels: feat (N, FH, FW, C) bf16 -> (out (N, O) fp32, saved) where
+    saved = (H, w1t_bf16, Pt) feeds head_backward. w1 (J, K) fp32 in the pooled order
+    k = (py (FW/2) + px) C + c; b1 (J,), w2 (O, J), b2 (O,) fp32."""
+    if feat.dim() != 4 or feat.dtype != torch.bfloat16 or not feat.is_cuda:
+        raise ValueError("head_forward takes an (N, FH, FW, C) bf16 device feature map")
+    N, FH, FW, C = feat.shape
+    J, K = w1.shape
+    O = w2.shape[0]
+    if FH % 2 or FW % 2 or C % 64 or K != (FH // 2) * (FW // 2) * C or K % 1024 or J != HIDDEN:
+        raise ValueError(f"head shapes: feature map {tuple(feat.shape)} (even sides, C a multiple of 64) pools to "
+                         f"{(FH // 2) * (FW // 2) * C} inputs (a multiple of 1024), w1 is {tuple(w1.shape)} "
+                         f"(needs ({HIDDEN}, that))")
+    if b1.shape != (J,) or w2.shape != (O, J) or b2.shape != (O,) or not 1 <= O <= 64:
+        raise ValueError("b1 must be (J,), w2 (O, J) and b2 (O,) with O <= 64")
+    dev = feat.device
+    lib = _lib.lib()
+    st = _stream(feat)
+    feat = feat.contiguous()
+    w1 = w1.detach().to(torch.float32).contiguous()
+    w1b = torch.empty(J, K, device=dev, dtype=torch.bfloat16)
+    w1t = torch.empty(K, J, device=dev, dtype=torch.bfloat16)
+    _lib.check(lib.ndrr_head_cast_weights(w1.data_ptr(), w1b.data_ptr(), w1t.data_ptr(), J, K, st),
+               "ndrr_head_cast_weights")
+    Np = (N + 31) // 32 * 32
+    P = torch.empty(N, K, device=dev, dtype=torch.bfloat16)
+    Pt = torch.empty(K, Np, device=dev, dtype=torch.bfloat16)
+    _lib.check(lib.ndrr_head_pool(feat.data_ptr(), P.data_ptr(), Pt.data_ptr(), N, FH, FW, C, st), "ndrr_head_pool")
+    ws = torch.empty(lib.ndrr_head_workspace_floats(N, FH, FW, C, J), device=dev, dtype=torch.float32)
+    H = torch.empty(N, J, device=dev, dtype=torch.float32)
+    out = torch.empty(N, O, device=dev, dtype=torch.float32)
+    b1, w2, b2 = (t.detach().to(torch.float32).contiguous() for t in (b1, w2, b2))
+    _lib.check(lib.ndrr_head_fwd(P.data_ptr(), w1b.data_ptr(), b1.data_ptr(), w2.data_ptr(), b2.data_ptr(),
+                                 ws.data_ptr(), H.data_ptr(), out.data_ptr(), N, K, J, O, st), "ndrr_head_fwd")
+    return out, (H, w1t, Pt)
+
+
+def head_backward(dout: torch.Tensor, feat: torch.Tensor, w2: torch.Tensor, saved):
+    """(dfeat (feat's shape, bf16, times the ReLU mask of feat), dw1, db1, dw2, db2 (fp32)) of the
+    head for the upstream gradient dout (N, O)."""
+    H, w1t, Pt = saved
+    N, FH, FW, C = feat.shape
+    K, J = w1t.shape
+    O = w2.shape[0]
+    dev = feat.device
+    Np = Pt.shape[1]
+    dout = dout.to(torch.float32).contiguous()
+    w2 = w2.detach().to(torch.float32).contiguous()
+    dh = torch.empty(N, J, device=dev, dtype=torch.float32)
+    dhb = torch.empty(N, J, device=dev, dtype=torch.bfloat16)
+    dht = torch.empty(J, Np, device=dev, dtype=torch.bfloat16)
+    dw1 = torch.empty(J, K, device=dev, dtype=torch.float32)
+    db1 = torch.empty(J, device=dev, dtype=torch.float32)
+    dw2 = torch.empty(O, J, device=dev, dtype=torch.float32)
+    db2 = torch.empty(O, device=dev, dtype=torch.float32)
+    dfeat = torch.empty_like(feat)
+    rc = _lib.lib().ndrr_head_bwd(dout.data_ptr(), H.data_ptr(), w2.data_ptr(), w1t.data_ptr(), Pt.data_ptr(),
+                                  feat.contiguous().data_ptr(), dh.data_ptr(), dhb.data_ptr(), dht.data_ptr(),
+                                  dw1.data_ptr(), db1.data_ptr(), dw2.data_ptr(), db2.data_ptr(), dfeat.data_ptr(), N,
+                                  FH, FW, C, J, O, _stream(feat))
+    _lib.check(rc, "ndrr_head_bwd")
+    return dfeat, dw1, db1, dw2, db2
+
+
+class _PoseNetFn(torch.autograd.Function):
+    """The whole network as one autograd node: the six ReLU convolutions, the pooling and the two
+    fully connected layers, forward and backward, all on the native kernels. The head's input
+    gradient comes back with the last convolution's ReLU mask applied; each convolution's
+    input-gradient kernel applies the mask of the layer below."""
+
+    @staticmethod
+    def forward(ctx, x, *params):
+        conv, (w1, b1, w2, b2) = params[:-4], params[-4:]
+        ws, bs = conv[0::2], conv[1::2]
+        acts = [x]
+        wbf = [w.to(torch.bfloat16) for w in ws]
+        for (cin, cout, s), w, b in zip(LAYERS, wbf, bs):
+            acts.append(conv3x3(acts[-1], w, b, s, relu=True))
+        out, saved = head_forward(acts[-1], w1, b1, w2, b2)
+        ctx.save_for_backward(*acts, *wbf, w2, *saved)
+        return out
+
+    @staticmethod
+    def backward(ctx, dout):
+        saved = ctx.saved_tensors
+        n = len(LAYERS)
+        acts, wbf = saved[: n + 1], saved[n + 1: 2 * n + 1]
+        w2, head_saved = saved[2 * n + 1], saved[2 * n + 2:]
+        dz, dw1, db1, dw2, db2 = head_backward(dout, acts[-1], w2, head_saved)
+        grads = [None] * (2 * n)
+        for i in reversed(range(n)):
+            cin, cout, s = LAYERS[i]
+            x = acts[i]
+            grads[2 * i], grads[2 * i + 1] = conv3x3_bwd_weight(dz, x, s, bias=True)
+            if i > 0:
+                dz = conv3x3_bwd_data(dz, wbf[i], x, s, (x.shape[1], x.shape[2]))
+        return (None, *grads, dw1, db1, dw2, db2)
+
+
 class _ConvStack(torch.autograd.Function):
     """The six ReLU convolutions as one autograd node. Backward runs the chain on
     the native kernels: the ReLU mask of each layer's input is fused into the
@@ -118,13 +222,17 @@ class _ConvStack(torch.autograd.Function):
 class PoseNet(torch.nn.Module):
     """(N, 1, 128, 128) DRRs -> (N, 6) = (rotation vector [rad], translation [mm] / xyz_scale).
 
-    The last feature map (16 x 16 x 256 for 128 x 128 inputs) is average-pooled to a
-    `pool` x `pool` grid and flattened into the MLP, so the head sees where features
-    lie: a global average (pool = 1) makes the features nearly invariant to in-plane
-    shifts of the anatomy, i.e. to the translation the network has to estimate."""
+    The last feature map (16 x 16 x 256 for 128 x 128 inputs) is average-pooled over 2 x 2
+    cells to an 8 x 8 grid and flattened into the MLP, so the head sees where features
+    lie: a global average makes the features nearly invariant to in-plane shifts of the
+    anatomy, i.e. to the translation the network has to estimate. The MLP's first layer
+    (w1: (256, 16384), inputs ordered (cell, channel)) and its output layer are native
+    bf16 matrix-core kernels with fp32 master weights, like the convolutions."""
 
     def __init__(self, xyz_scale: float = 50.0, seed: int = 0, pool: int = 8):
         super().__init__()
+        if pool != 8:
+            raise ValueError("the native head pools the 16 x 16 feature map over 2 x 2 cells: pool must be 8")
         g = torch.Generator().manual_seed(seed)
         self.xyz_scale = xyz_scale
         self.pool = pool
@@ -134,11 +242,15 @@ class PoseNet(torch.nn.Module):
             std = math.sqrt(2.0 / (9 * cin))  # He initialisation for ReLU
             self.weights.append(torch.nn.Parameter(torch.randn(cout, 3, 3, cin, generator=g) * std))
             self.biases.append(torch.nn.Parameter(torch.zeros(cout)))
-        self.fc1 = torch.nn.Linear(256 * pool * pool, 256)
-        self.fc2 = torch.nn.Linear(256, 6)
-        with torch.no_grad():
-            self.fc2.weight.mul_(0.1)
-            self.fc2.bias.zero_()
+        k = LAYERS[-1][1] * pool * pool
+
+        def uniform(*shape, fan_in):  # U(-1/sqrt(fan_in), 1/sqrt(fan_in)), torch.nn.Linear's default
+            return (torch.rand(*shape, generator=g) * 2 - 1) / math.sqrt(fan_in)
+
+        self.w1 = torch.nn.Parameter(uniform(HIDDEN, k, fan_in=k))
+        self.b1 = torch.nn.Parameter(uniform(HIDDEN, fan_in=k))
+        self.w2 = torch.nn.Parameter(0.1 * uniform(OUTPUTS, HIDDEN, fan_in=HIDDEN))
+        self.b2 = torch.nn.Parameter(torch.zeros(OUTPUTS))
 
     @staticmethod
     def prepare(drr: torch.Tensor) -> torch.Tensor:
@@ -150,13 +262,13 @@ class PoseNet(torch.nn.Module):
     def forward(self, drr: torch.Tensor) -> torch.Tensor:
         x = self.prepare(drr)
         params = [t for wb in zip(self.weights, self.biases) for t in wb]
-        feat = _ConvStack.apply(x, *params)            # (N, 16, 16, 256) bf16
-        return self.fc2(torch.relu(self.fc1(self.pooled(feat))))
+        return _PoseNetFn.apply(x, *params, self.w1, self.b1, self.w2, self.b2)
 
-    def pooled(self, feat: torch.Tensor) -> torch.Tensor:
-        """(N, pool * pool * 256) features: the map averaged over pool x pool cells, flattened."""
-        h = torch.nn.functional.adaptive_avg_pool2d(feat.float().permute(0, 3, 1, 2), self.pool)
-        return h.flatten(1)
+    def features(self, drr: torch.Tensor) -> torch.Tensor:
+        """(N, 16, 16, 256) bf16 feature map of the convolutions (no gradient)."""
+        params = [t for wb in zip(self.weights, self.biases) for t in wb]
+        with torch.no_grad():
+            return _ConvStack.apply(self.prepare(drr), *params)
 
     def split(self, out: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
         """(rotation vector, translation in mm) of the network output."""

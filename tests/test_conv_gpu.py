@@ -110,14 +110,75 @@ def test_conv_weight_gradient_many_splits_is_deterministic():
     assert (da.cpu() - br.grad).abs().max() / br.grad.abs().max() < 1e-5
 
 
+def _torch_head(feat, w1, b1, w2, b2):
+    """The head in torch fp32: 2 x 2 mean pool of the NHWC map, inputs ordered (cell, channel) and
+    rounded to bf16 as the kernel stores them; the first layer's weights rounded to bf16."""
+    N, FH, FW, C = feat.shape
+    P = feat.float().reshape(N, FH // 2, 2, FW // 2, 2, C).mean(dim=(2, 4)).reshape(N, -1)
+    P = P + (P.to(torch.bfloat16).float() - P).detach()  # bf16 pooled inputs, straight-through
+    H = torch.relu(P @ w1.to(torch.bfloat16).float().T + b1)
+    return H @ w2.T + b2, H, P
+
+
+def test_head_forward_and_backward_against_torch():
+    """The native head at full size (batch 256, 16 x 16 x 256 feature map, 16384 -> 256 -> 6)
+    against torch fp32 on the same bf16 roundings: outputs and hidden units to 1e-5; bias and
+    output-layer gradients to 1e-5; the first layer's weight gradient to 1e-4 (bf16 products are
+    exact in fp32, but an element of dH whose fp32 sum lands within an ulp of a bf16 rounding
+    boundary may round the other way in the two sums); the input gradient to 4e-3 of its
+    largest value (bf16 output)."""
+    from nanodrr_amd.posenet import head_backward, head_forward
+
+    g = torch.Generator().manual_seed(3)
+    feat = torch.relu(torch.randn(256, 16, 16, 256, generator=g)).to(torch.bfloat16)
+    w1 = torch.randn(256, 16384, generator=g) / 128
+    b1 = torch.randn(256, generator=g) * 0.1
+    w2 = torch.randn(6, 256, generator=g) / 16
+    b2 = torch.randn(6, generator=g) * 0.1
+    dout = torch.randn(256, 6, generator=g)
+    out, saved = head_forward(feat.to(DEV), w1.to(DEV), b1.to(DEV), w2.to(DEV), b2.to(DEV))
+    ref, H, P = _torch_head(feat, w1, b1, w2, b2)
+    assert (out.cpu() - ref).abs().max() / ref.abs().max() < 1e-5
+    assert (saved[0].cpu() - H).abs().max() / H.abs().max() < 1e-5
+    assert torch.equal(out, head_forward(feat.to(DEV), w1.to(DEV), b1.to(DEV), w2.to(DEV), b2.to(DEV))[0])
+    dfeat, dw1, db1, dw2, db2 = head_backward(dout.to(DEV), feat.to(DEV), w2.to(DEV), saved)
+    dH = (dout @ w2) * (H > 0)
+    dHb = dH.to(torch.bfloat16).float()
+    rel = lambda a, b: ((a.cpu().double() - b.double()).abs().max() / b.double().abs().max()).item()  # noqa: E731
+    assert rel(dw1, dHb.double().T @ P.detach().double()) < 1e-4
+    assert rel(db1, dH.sum(0)) < 1e-5
+    assert rel(dw2, dout.double().T @ H.double()) < 1e-5
+    assert rel(db2, dout.sum(0)) < 1e-5
+    dP = dHb.double() @ w1.to(torch.bfloat16).double()
+    ref_dfeat = (dP.reshape(256, 8, 1, 8, 1, 256) * 0.25).expand(-1, -1, 2, -1, 2, -1).reshape(feat.shape)
+    ref_dfeat = ref_dfeat * (feat.double() > 0)
+    assert rel(dfeat, ref_dfeat) < 4e-3
+    assert torch.equal(dfeat.cpu() != 0, ref_dfeat != 0) or ((dfeat.cpu() != 0) != (ref_dfeat != 0)).float().mean() < 1e-4
+    # the fp32 master weights are what the gradients refer to: torch autograd through the reference
+    w1r, b1r, w2r, b2r = (t.clone().requires_grad_(True) for t in (w1, b1, w2, b2))
+    r, _, _ = _torch_head(feat, w1r, b1r, w2r, b2r)
+    (r * dout).sum().backward()
+    assert rel(db1, b1r.grad) < 1e-5 and rel(dw2, w2r.grad) < 1e-5 and rel(db2, b2r.grad) < 1e-5
+    assert rel(dw1, w1r.grad) < 4e-3  # the reference keeps dH in fp32; the kernels round it to bf16
+
+
+def test_head_rejects_mismatched_shapes():
+    from nanodrr_amd.posenet import head_forward
+
+    feat = torch.zeros(2, 4, 4, 256, dtype=torch.bfloat16, device=DEV)
+    with pytest.raises(ValueError, match="pools to"):
+        head_forward(feat, torch.zeros(256, 16384, device=DEV), torch.zeros(256, device=DEV),
+                     torch.zeros(6, 256, device=DEV), torch.zeros(6, device=DEV))
+
+
 def test_posenet_gradients_match_torch_autograd():
-    """The whole conv stack (native forward and backward) against torch autograd of the same
-    network in fp32 on the bf16-rounded weights and activations, at a small batch."""
+    """The whole network (native convolutions and head, forward and backward) against torch
+    autograd of the same network in fp32 on the bf16-rounded weights and activations."""
     from nanodrr_amd.posenet import LAYERS, PoseNet
 
     torch.manual_seed(0)
     net = PoseNet().to(DEV)
-    drr = torch.rand(2, 1, 32, 32, device=DEV) * 5
+    drr = torch.rand(2, 1, 128, 128, device=DEV) * 5
     out = net(drr)
     loss = (out ** 2).sum()
     loss.backward()
@@ -125,17 +186,19 @@ def test_posenet_gradients_match_torch_autograd():
     x = PoseNet.prepare(drr).float()
     ws = [w.detach().to(torch.bfloat16).float().requires_grad_(True) for w in net.weights]
     bs = [b.detach().clone().requires_grad_(True) for b in net.biases]
+    hw = [t.detach().clone().requires_grad_(True) for t in (net.w1, net.b1, net.w2, net.b2)]
     h = x
     for (cin, cout, s), w, b in zip(LAYERS, ws, bs):
         h = torch.relu(_torch_conv(h, w, b, s))
         h = h + (h.to(torch.bfloat16).float() - h).detach()  # bf16 activations, straight-through
-    feat = torch.nn.functional.adaptive_avg_pool2d(h.permute(0, 3, 1, 2), net.pool).flatten(1)
-    ref = net.fc2(torch.relu(net.fc1(feat)))
+    ref, _, _ = _torch_head(h, *hw)
     torch.testing.assert_close(out, ref, rtol=2e-2, atol=2e-3)
     (ref ** 2).sum().backward()
     for i in range(len(LAYERS)):
         g, r = net.weights[i].grad, ws[i].grad
         assert (g - r).abs().max() / r.abs().max() < 5e-2, i  # bf16 activation gradients
+    for p, r in zip((net.w1, net.b1, net.w2, net.b2), hw):
+        assert (p.grad - r.grad).abs().max() / r.grad.abs().max() < 5e-2
 
 
 def test_posenet_training_and_registration_pipeline_runs():
