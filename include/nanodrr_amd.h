@@ -486,7 +486,7 @@ int ndrr_conv3x3_bwd_weight(const void* dy, const void* x, float* workspace, flo
  * The pose-proposal CNN's head (no counterpart in the reference): 2 x 2
  * average pooling of the last feature map feat[N][FH][FW][C] (bf16, C a
  * multiple of 64), a fully connected layer of J = 256 units with ReLU and an
- * output layer of O <= 64 units, on the bf16 matrix cores with fp32
+ * output layer of O <= 8 units, on the bf16 matrix cores with fp32
  * accumulation. The pooled features are P[N][K], K = (FH/2)(FW/2)C, with
  * k = (py (FW/2) + px) C + c, and their transpose Pt[K][Np], Np = N rounded up
  * to a multiple of 32 (zero columns past N).

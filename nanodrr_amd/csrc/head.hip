@@ -298,25 +298,48 @@ __global__ __launch_bounds__(J) void k_head_bwd_hidden(const float* __restrict__
   dHt[static_cast<int64_t>(j) * Np + n] = static_cast<__bf16>(g);
 }
 
-// Bias and output-layer gradients: thread j sums over the samples in order.
-template <int J>
-__global__ __launch_bounds__(J) void k_head_bwd_small(const float* __restrict__ dout, const float* __restrict__ H,
-                                                      const float* __restrict__ dh, int N, int O,
-                                                      float* __restrict__ db1, float* __restrict__ dw2,
-                                                      float* __restrict__ db2) {
-  const int j = threadIdx.x;
-  float s = 0.0f;
-  for (int n = 0; n < N; ++n) s += dh[static_cast<int64_t>(n) * J + j];
-  db1[j] = s;
-  for (int o = 0; o < O; ++o) {
-    float t = 0.0f;
-    for (int n = 0; n < N; ++n) t += dout[static_cast<int64_t>(n) * O + o] * H[static_cast<int64_t>(n) * J + j];
-    dw2[o * J + j] = t;
+// Bias and output-layer gradients, one workgroup per hidden unit j: thread t
+// sums samples t, t + 256, ... of dh[n][j] and dout[n][o] H[n][j], then the
+// block adds the thread sums in a fixed tree; workgroup 0 also forms db2.
+constexpr int kSmallThreads = 256;
+constexpr int kMaxOutputs = 8;
+
+__global__ __launch_bounds__(kSmallThreads) void k_head_bwd_small(const float* __restrict__ dout,
+                                                                  const float* __restrict__ H,
+                                                                  const float* __restrict__ dh, int N, int J, int O,
+                                                                  float* __restrict__ db1, float* __restrict__ dw2,
+                                                                  float* __restrict__ db2) {
+  __shared__ float red[1 + 2 * kMaxOutputs][kSmallThreads];
+  const int j = blockIdx.x, t = threadIdx.x;
+  float s[1 + 2 * kMaxOutputs];
+#pragma unroll
+  for (int q = 0; q < 1 + 2 * kMaxOutputs; ++q) s[q] = 0.0f;
+  for (int n = t; n < N; n += kSmallThreads) {
+    const float h = H[static_cast<int64_t>(n) * J + j];
+    s[0] += dh[static_cast<int64_t>(n) * J + j];
+#pragma unroll
+    for (int o = 0; o < kMaxOutputs; ++o) {
+      if (o < O) {
+        const float d = dout[static_cast<int64_t>(n) * O + o];
+        s[1 + o] += d * h;
+        s[1 + kMaxOutputs + o] += d;
+      }
+    }
   }
-  if (j < O) {
-    float t = 0.0f;
-    for (int n = 0; n < N; ++n) t += dout[static_cast<int64_t>(n) * O + j];
-    db2[j] = t;
+#pragma unroll
+  for (int q = 0; q < 1 + 2 * kMaxOutputs; ++q) red[q][t] = s[q];
+  __syncthreads();
+  for (int w = kSmallThreads / 2; w > 0; w >>= 1) {
+    if (t < w) {
+#pragma unroll
+      for (int q = 0; q < 1 + 2 * kMaxOutputs; ++q) red[q][t] += red[q][t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) db1[j] = red[0][0];
+  if (t < O) {
+    dw2[t * J + j] = red[1 + t][0];
+    if (j == 0) db2[t] = red[1 + kMaxOutputs + t][0];
   }
 }
 
@@ -324,7 +347,7 @@ constexpr int kHidden = 256;  // J: the hidden width the kernels are built for
 
 bool head_shape_ok(int N, int FH, int FW, int C, int J, int O) {
   return N > 0 && FH > 0 && FW > 0 && FH % 2 == 0 && FW % 2 == 0 && C > 0 && C % kPoolTile == 0 &&
-         J == kHidden && O > 0 && O <= 64;
+         J == kHidden && O > 0 && O <= kMaxOutputs;
 }
 
 int padded(int N) { return (N + 31) / 32 * 32; }
@@ -362,7 +385,7 @@ int ndrr_head_pool(const void* feat, void* P, void* Pt, int N, int FH, int FW, i
 int ndrr_head_fwd(const void* P, const void* w1_bf16, const float* b1, const float* w2, const float* b2,
                   float* workspace, float* H, float* out, int N, int K, int J, int O, void* stream) {
   if (!P || !w1_bf16 || !b1 || !w2 || !b2 || !workspace || !H || !out) return NDRR_EINVAL;
-  if (N <= 0 || K <= 0 || K % (32 * kSplitK) != 0 || J != kHidden || O <= 0 || O > 64) return NDRR_EINVAL;
+  if (N <= 0 || K <= 0 || K % (32 * kSplitK) != 0 || J != kHidden || O <= 0 || O > kMaxOutputs) return NDRR_EINVAL;
   if (static_cast<int64_t>(N) * K >= (int64_t{1} << 31) || N > 65535 * 2 * kTileM) return NDRR_ERANGE;
   const hipStream_t st = static_cast<hipStream_t>(stream);
   GemmP g{};
@@ -390,7 +413,7 @@ int ndrr_head_bwd(const float* dout, const float* H, const float* w2, const void
   const hipStream_t st = static_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(k_head_bwd_hidden<kHidden>, dim3(Np), dim3(kHidden), 0, st, dout, H, w2, N, Np, O, dh,
                      static_cast<__bf16*>(dH_bf16), static_cast<__bf16*>(dHt_bf16));
-  hipLaunchKernelGGL(k_head_bwd_small<kHidden>, dim3(1), dim3(kHidden), 0, st, dout, H, dh, N, O, db1, dw2, db2);
+  hipLaunchKernelGGL(k_head_bwd_small, dim3(J), dim3(kSmallThreads), 0, st, dout, H, dh, N, J, O, db1, dw2, db2);
   // dW1[j][k] = sum_n dHt[j][n] Pt[k][n]
   GemmP w{};
   w.A = static_cast<const __bf16*>(dHt_bf16);

@@ -34,6 +34,12 @@ def transform_point(xform: torch.Tensor, v: torch.Tensor) -> torch.Tensor:
 # ---------------------------------------------------------------------------
 
 
+def matmul3(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """a @ b for batches of small matrices (3 x 3, 4 x 4, 3 x 1) as broadcast products and one
+    sum: elementwise kernels on the GPU instead of a library GEMM launch per product."""
+    return (a[..., :, :, None] * b[..., None, :, :]).sum(-2)
+
+
 def hat(v: torch.Tensor) -> torch.Tensor:
     """Skew-symmetric matrix [v]x of (..., 3) vectors (se3.py:224-242)."""
     x, y, z = v.unbind(-1)
@@ -68,8 +74,8 @@ def euler_to_rotmat(convention: str, angles: torch.Tensor, degrees: bool = False
         angles = angles * (math.pi / 180.0)
     mats = [_axis_rot(ax, angles[..., k]) for k, ax in enumerate(convention.upper())]
     if convention.isupper():
-        return mats[0] @ mats[1] @ mats[2]
-    return mats[2] @ mats[1] @ mats[0]
+        return matmul3(matmul3(mats[0], mats[1]), mats[2])
+    return matmul3(matmul3(mats[2], mats[1]), mats[0])
 
 
 def rotmat_to_euler_zxy(R: torch.Tensor, degrees: bool = False) -> torch.Tensor:
@@ -97,7 +103,7 @@ def so3_exp(rotvec: torch.Tensor) -> torch.Tensor:
     K = hat(rotvec)
     A, B, _ = _series_coeffs((rotvec * rotvec).sum(-1))
     eye = torch.eye(3, dtype=rotvec.dtype, device=rotvec.device)
-    return eye + A[..., None, None] * K + B[..., None, None] * (K @ K)
+    return eye + A[..., None, None] * K + B[..., None, None] * matmul3(K, K)
 
 
 def so3_log(R: torch.Tensor) -> torch.Tensor:
@@ -200,13 +206,13 @@ def se3_V(rotvec: torch.Tensor) -> torch.Tensor:
     K = hat(rotvec)
     _, B, Cc = _series_coeffs((rotvec * rotvec).sum(-1))
     eye = torch.eye(3, dtype=rotvec.dtype, device=rotvec.device)
-    return eye + B[..., None, None] * K + Cc[..., None, None] * (K @ K)
+    return eye + B[..., None, None] * K + Cc[..., None, None] * matmul3(K, K)
 
 
 def se3_exp(log_rotation: torch.Tensor, log_translation: torch.Tensor) -> torch.Tensor:
     """SE(3) exponential of (rotation vector, translation log) pairs (se3.py:179-194)."""
     R = so3_exp(log_rotation)
-    t = (se3_V(log_rotation) @ log_translation[..., :, None])[..., 0]
+    t = matmul3(se3_V(log_rotation), log_translation[..., :, None])[..., 0]
     return make_se3(R, t)
 
 
@@ -242,7 +248,7 @@ def convert(rotation, translation, parameterization, convention=None, degrees=Tr
     if p == Parameterization.SE3_LOG:
         return se3_exp(rotation, translation)
     R = rotation_to_matrix(rotation, p, convention, degrees)
-    t = (R @ translation[..., :, None])[..., 0]
+    t = matmul3(R, translation[..., :, None])[..., 0]
     if isocenter is not None:
         t = t + isocenter
     return make_se3(R, t)

@@ -20,7 +20,7 @@ import torch.nn.functional as F
 
 from . import _lib
 from ._ops import _eager
-from .geometry import rotmat_geodesic_distance
+from .geometry import matmul3, rotmat_geodesic_distance
 
 
 def _zncc_impl(x: torch.Tensor, y: torch.Tensor, eps: float) -> tuple[torch.Tensor, torch.Tensor]:
@@ -224,7 +224,7 @@ class DoubleGeodesicSE3(torch.nn.Module):
     def forward(self, pose_1: torch.Tensor, pose_2: torch.Tensor):
         rt1 = pose_1[:, :3, :3].transpose(-1, -2)
         rt2 = pose_2[:, :3, :3].transpose(-1, -2)
-        d = (rt1 @ pose_1[:, :3, 3:] - rt2 @ pose_2[:, :3, 3:])[..., 0]
+        d = (matmul3(rt1, pose_1[:, :3, 3:]) - matmul3(rt2, pose_2[:, :3, 3:]))[..., 0]
         xyz = torch.sqrt(torch.clamp((d * d).sum(dim=-1), min=self.eps))
         rot = self.sdr * rotmat_geodesic_distance(pose_1[:, :3, :3], pose_2[:, :3, :3], clamping=1.0 - self.eps)
         return rot, xyz, torch.sqrt(rot * rot + xyz * xyz + self.eps)

@@ -103,8 +103,8 @@ def head_forward(feat: torch.Tensor, w1: torch.Tensor, b1: torch.Tensor, w2: tor
         raise ValueError(f"head shapes: feature map {tuple(feat.shape)} (even sides, C a multiple of 64) pools to "
                          f"{(FH // 2) * (FW // 2) * C} inputs (a multiple of 1024), w1 is {tuple(w1.shape)} "
                          f"(needs ({HIDDEN}, that))")
-    if b1.shape != (J,) or w2.shape != (O, J) or b2.shape != (O,) or not 1 <= O <= 64:
-        raise ValueError("b1 must be (J,), w2 (O, J) and b2 (O,) with O <= 64")
+    if b1.shape != (J,) or w2.shape != (O, J) or b2.shape != (O,) or not 1 <= O <= 8:
+        raise ValueError("b1 must be (J,), w2 (O, J) and b2 (O,) with O <= 8")
     dev = feat.device
     lib = _lib.lib()
     st = _stream(feat)
