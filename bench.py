@@ -6,22 +6,29 @@ channels (int8 labels), 16 poses (ZXY +-15 deg, +-20 mm about the isocentre
 samples per ray. One step = render(...) of the 16 poses plus the backward of
 a weighted sum to the poses, captured once in a HIP graph and replayed (the
 eager, uncaptured step is reported too); DRR/s = 16 * steps / elapsed.
-Cases (volume dtype : layout the kernels read : adjoint):
-  bf16:quad:single     the headline `value`: bf16 quad records, single-pass
-                       adjoint (the forward march stores per-ray sums, the
-                       backward contracts them);
+Cases (volume dtype : layout the kernels read : adjoint[+gvol]):
+  bf16:coef:single     the headline `value`: bf16 coefficient records (each
+                       cell's interpolant coefficients, made from the fp32
+                       volume), single-pass adjoint (the forward march stores
+                       per-ray sums, the backward contracts them);
   fp32:linear:two      the plain fp32 kernel: the (D, H, W) volume, two-pass
                        adjoint (the backward marches the volume again, as
                        nanodrr's kernel does) - the baseline of the speed-ups;
-  fp32:quad:single, bf16:stencil:single, bf16:stencil:two  other corners.
+  fp32:linear:two+gvol the same with volume gradients (fp32 atomics into a
+                       (D, H, W) gradient volume, zeroed every step);
+  bf16:quad:single, fp32:quad:single, bf16:stencil:single  other corners.
 Every case accumulates in float32.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cases bf16:quad:single,...] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cases bf16:coef:single,...]
+                    [--no-cpu-baseline] [--no-registration] [--no-mixed]
 
 Prints one JSON line. `roofline` covers the dominant kernel of the headline
-case, timed with HIP events on the stream it runs on; `cpu_baseline` times
-the pure-PyTorch grid_sample restatement (fp32) of the same workload on the
-host cores for a bounded sample of DRRs; `registration` is the batched
+case, timed with HIP events on the stream it runs on, with the HBM traffic of
+the committed rocprofv3 counter summary when that summary was measured on the
+current kernel source at the same kernel time (else null); `cpu_baseline`
+times the pure-PyTorch grid_sample restatement (fp32) of the workload and of
+the small CPU case on the host cores for bounded samples; `mixed_precision`
+is the 512 x 512 x 400 bf16 / fp16 workload; `registration` is the batched
 registration leg (registrations/s reaching mTRE < 1 mm).
 """
 from __future__ import annotations
@@ -50,11 +57,19 @@ GRAPH_ITERS = 10  # registration iterations per captured HIP graph
 REGISTRATION_LAYOUT = "quad"
 
 
-def make_workload(dtype, device):
+DEFAULT_CASES = "bf16:coef:single,fp32:linear:two,fp32:linear:two+gvol,bf16:quad:single,fp32:quad:single," \
+                "bf16:stencil:single"
+
+
+def make_workload(dtype, device, gvol: bool = False):
+    """(subject, k_inv, rt_inv, sdd, phantom) of the 256^3 / C=4 / B=16 workload; with gvol the
+    subject's fp32 volume requires grad."""
     from nanodrr_amd.phantom import detector, make_phantom, phantom_subject, sample_poses
 
     ph = make_phantom((SIZE,) * 3, (SPACING,) * 3, seed=0)
     subject = phantom_subject(ph, n_classes=N_CLASSES, dtype=dtype, device=device)
+    if gvol:
+        subject.image.requires_grad_(True)
     rt_inv, _, _ = sample_poses(BATCH, seed=1, isocenter=subject.isocenter)
     k_inv, sdd = detector(1020.0, 1.0, DET, DET)
     return subject, k_inv.to(device), rt_inv.to(device), sdd.to(device), ph
@@ -65,20 +80,32 @@ ADJOINTS = {"single": "single_pass", "two": "two_pass"}
 
 
 def parse_case(case: str):
-    """"dtype:layout[:adjoint]" -> (dtype name, layout, adjoint name); the adjoint
-    defaults to the single pass."""
+    """"dtype:layout[:adjoint[+gvol]]" -> (dtype name, layout, adjoint name, volume gradients);
+    the adjoint defaults to the single pass. Volume gradients need fp32:linear:two."""
     parts = case.split(":")
+    if len(parts) not in (2, 3):
+        raise ValueError(f"bad case {case!r}: expected dtype:layout[:single|two[+gvol]]")
     name, layout = parts[0], parts[1]
-    adjoint = parts[2] if len(parts) > 2 else "single"
-    if name not in DTYPES or adjoint not in ADJOINTS:
-        raise ValueError(f"bad case {case!r}: expected dtype:layout[:single|two]")
-    return name, layout, adjoint
+    adjoint, _, extra = (parts[2] if len(parts) > 2 else "single").partition("+")
+    gvol = extra == "gvol"
+    if name not in DTYPES or adjoint not in ADJOINTS or extra not in ("", "gvol"):
+        raise ValueError(f"bad case {case!r}: expected dtype:layout[:single|two[+gvol]]")
+    if gvol and (name, layout, adjoint) != ("fp32", "linear", "two"):
+        raise ValueError(f"bad case {case!r}: volume gradients are benchmarked as fp32:linear:two+gvol")
+    return name, layout, adjoint, gvol
+
+
+def _clear_grads(subject, rt):
+    rt.grad = None
+    if subject.image.requires_grad:
+        subject.image.grad = None
 
 
 def step_fn(subject, k_inv, rt_inv, sdd, wts):
     from nanodrr_amd import render
 
     rt = rt_inv.detach().requires_grad_(True)
+    _clear_grads(subject, rt)
     out = render(subject, k_inv, rt, sdd, DET, DET, n_samples=S)
     out.backward(wts.expand_as(out))  # d(sum(out * wts))/d(out) = wts, without a reduction kernel
     return rt.grad
@@ -93,7 +120,7 @@ def graph_step(subject, k_inv, rt_inv, sdd, wts):
     rt = rt_inv.detach().clone().requires_grad_(True)
 
     def body():
-        rt.grad = None
+        _clear_grads(subject, rt)
         out = render(subject, k_inv, rt, sdd, DET, DET, n_samples=S)
         out.backward(wts.expand_as(out))
 
@@ -104,7 +131,7 @@ def graph_step(subject, k_inv, rt_inv, sdd, wts):
             body()
     torch.cuda.current_stream().wait_stream(side)
     graph = torch.cuda.CUDAGraph()
-    rt.grad = None
+    _clear_grads(subject, rt)
     with torch.cuda.graph(graph):
         out = render(subject, k_inv, rt, sdd, DET, DET, n_samples=S)
         out.backward(wts.expand_as(out))
@@ -124,15 +151,39 @@ def time_steps(fn, steps, warmup):
     return t0.elapsed_time(t1) / steps  # ms per step
 
 
-def kernel_times(subject, k_inv, rt_inv, sdd, wts, reps, layout, adjoint="two"):
+def _gvol_backward(go, vol, lab, M, src, tgt, step):
+    """A closure launching ndrr_raymarch_bwd with volume gradients into buffers allocated once (the
+    op allocates and zero-fills its gradient volume on every call; this times the kernels)."""
+    from nanodrr_amd import _lib
+    from nanodrr_amd._ops import _DTYPE_CODE, _geometry
+
+    B, N, D, H, W, strides = _geometry(vol, lab, M, src, tgt, step, S, N_CLASSES, DET, "linear")
+    lib = _lib.lib()
+    dev = vol.device
+    ws = torch.empty(lib.ndrr_bwd_workspace_floats(B, N, DET), device=dev, dtype=torch.float32)
+    gM = torch.empty(B, 3, 4, device=dev, dtype=torch.float32)
+    gvol = torch.zeros(vol.shape, device=dev, dtype=torch.float32)  # accumulates over the timed calls
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def launch():
+        rc = lib.ndrr_raymarch_bwd(go.data_ptr(), vol.data_ptr(), _DTYPE_CODE[vol.dtype], _lib.LAYOUT_LINEAR,
+                                   lab.data_ptr(), M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(),
+                                   ws.data_ptr(), gM.data_ptr(), None, None, None, gvol.data_ptr(), 0, B, N, D, H, W,
+                                   S, N_CLASSES, DET, *strides, stream)
+        _lib.check(rc, "ndrr_raymarch_bwd")
+
+    return launch
+
+
+def kernel_times(subject, k_inv, rt_inv, sdd, wts, reps, layout, adjoint="two", gvol=False):
     """Average duration (ms) of one forward and one backward op launch, HIP events
     recorded on the stream the ops launch on (torch's current stream). The
     single-pass forward also writes the adjoint's per-ray sums and its backward
-    is the contraction."""
+    is the contraction; with gvol the two-pass backward also forms the volume gradient."""
     from nanodrr_amd._ops import fused_raymarch, fused_raymarch_bwd, fused_raymarch_bwd_jac, fused_raymarch_jac
     from nanodrr_amd.renderer import _make_src, _make_tgt
 
-    vol = subject.image[0, 0]
+    vol = subject.image[0, 0].detach()
     packed = None if layout == "linear" else subject.packed_volume(layout)
     lab = subject.packed_labels(layout)
     M = (subject.world_to_voxel.double() @ rt_inv.double())[:, :3, :].float().contiguous()
@@ -163,12 +214,15 @@ def kernel_times(subject, k_inv, rt_inv, sdd, wts, reps, layout, adjoint="two"):
         ms_bwd = timed(lambda: fused_raymarch_bwd_jac(go, jac, M, src, tgt, step, DET, False))
     else:
         ms_fwd = timed(lambda: fused_raymarch(vol, lab, M, src, tgt, step, S, N_CLASSES, DET, packed, layout))
-        ms_bwd = timed(lambda: fused_raymarch_bwd(go, vol, lab, M, src, tgt, step, S, N_CLASSES, DET, False, False,
-                                                  packed, layout))
+        if gvol:
+            ms_bwd = timed(_gvol_backward(go, vol, lab, M, src, tgt, step))
+        else:
+            ms_bwd = timed(lambda: fused_raymarch_bwd(go, vol, lab, M, src, tgt, step, S, N_CLASSES, DET, False,
+                                                      False, packed, layout))
     return ms_fwd, ms_bwd, (M, src, tgt)
 
 
-def algorithmic_bytes(subject, geom, dtype, adjoint="two"):
+def algorithmic_bytes(subject, geom, dtype, adjoint="two", gvol=False):
     from nanodrr_amd.roofline import contraction_bytes, inside_samples, march_bytes
 
     M, src, tgt = geom
@@ -179,14 +233,15 @@ def algorithmic_bytes(subject, geom, dtype, adjoint="two"):
         bwd = contraction_bytes(n_rays, M.shape[0], N_CLASSES, 6)
     else:
         fwd = march_bytes(n_in, n_rays, M.shape[0], N_CLASSES, dtype)
-        bwd = march_bytes(n_in, n_rays, M.shape[0], N_CLASSES, dtype, backward=True)
+        bwd = march_bytes(n_in, n_rays, M.shape[0], N_CLASSES, dtype, backward=True, need_gvol=gvol)
     return n_in, fwd, bwd
 
 
-def _cpu_drr_rate(ph, n_classes, threads, budget_s, max_runs, n_samples=S):
-    """(DRR/s, runs): median rate of one DRR (fwd + pose backward, 1 pose) through the
-    pure-PyTorch grid_sample restatement (oracle.torch_ref, fp32) on `threads` threads,
-    repeated for about `budget_s` seconds (at least 2 runs, at most `max_runs`)."""
+def _cpu_drr_rate(ph, n_classes, threads, budget_s, max_runs, n_samples=S, backward=True):
+    """(DRR/s, runs): median rate of one DRR (1 pose; forward + pose backward, or forward only)
+    through the pure-PyTorch grid_sample restatement (oracle.torch_ref, fp32) on `threads`
+    threads, after one warm-up, repeated for about `budget_s` seconds (at least 2 runs, at most
+    `max_runs`)."""
     from nanodrr_amd.phantom import detector, phantom_subject, sample_poses
     from oracle import torch_ref
 
@@ -198,7 +253,12 @@ def _cpu_drr_rate(ph, n_classes, threads, budget_s, max_runs, n_samples=S):
     wts = torch.randn(1, n_classes, DET, DET, generator=torch.Generator().manual_seed(0))
 
     def one(i):
-        rt = rt_all[i % BATCH: i % BATCH + 1].clone().requires_grad_(True)
+        rt = rt_all[i % BATCH: i % BATCH + 1].clone()
+        if not backward:
+            with torch.no_grad():
+                torch_ref.render_torch(image, label, n_classes, w2g, k_inv, rt, sdd, DET, DET, n_samples)
+            return
+        rt.requires_grad_(True)
         out = torch_ref.render_torch(image, label, n_classes, w2g, k_inv, rt, sdd, DET, DET, n_samples)
         (out * wts).sum().backward()
 
@@ -214,33 +274,78 @@ def _cpu_drr_rate(ph, n_classes, threads, budget_s, max_runs, n_samples=S):
     return 1.0 / statistics.median(times), len(times)
 
 
-def cpu_baseline(ph, budget_s=12.0):
-    """The workload's DRRs through the pure-PyTorch CPU restatement on the host's cores (up to
-    16), plus a one-thread rate and the small single-channel CPU case (128^3 at 1.5 mm, C=1,
-    200 x 200, S=256) on both thread counts."""
+def host_cpu() -> dict:
+    """This host's CPU: os.cpu_count(), the cores this process may run on (what `nproc`
+    prints), the cgroup CPU quota if one is set, and the model name (lscpu's "Model name",
+    read from /proc/cpuinfo)."""
+    info = {"os_cpu_count": os.cpu_count() or 1}
+    try:
+        info["nproc"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        info["nproc"] = info["os_cpu_count"]
+    try:
+        quota, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        info["cgroup_cpus"] = None if quota == "max" else int(quota) / int(period)
+    except (OSError, ValueError):
+        info["cgroup_cpus"] = None
+    info["model"] = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                info["model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return info
+
+
+def cpu_baseline(ph):
+    """The CPU restatement on the host's cores: the workload's DRRs (fwd + pose backward) with
+    torch.set_num_threads(os.cpu_count()), and - when fewer cores are usable (affinity or cgroup
+    quota) - on that many threads too, `value` being the faster (its thread count in `cores`);
+    a one-thread rate; and the small CPU case (128^3 at 1.5 mm, C=1, 200 x 200, S=256) forward
+    only and fwd + backward on the same thread counts. About 30 s in all."""
     from nanodrr_amd.phantom import make_phantom
 
-    threads = min(16, os.cpu_count() or 1)
-    rate, runs = _cpu_drr_rate(ph, N_CLASSES, threads, budget_s, 40)
+    host = host_cpu()
+    counts = [host["os_cpu_count"]]
+    usable = host["nproc"]
+    if host["cgroup_cpus"]:
+        usable = min(usable, max(1, int(host["cgroup_cpus"])))
+    if usable != counts[0]:
+        counts.append(usable)
+    per_count = {}
+    for n in counts:
+        rate, runs = _cpu_drr_rate(ph, N_CLASSES, n, 8.0, 40)
+        per_count[n] = (rate, runs)
+    best = max(per_count, key=lambda n: per_count[n][0])
+    rate, runs = per_count[best]
     rate1, runs1 = _cpu_drr_rate(ph, N_CLASSES, 1, 4.0, 3)
     ph0 = make_phantom((128,) * 3, (1.5,) * 3, seed=0)
-    rate0, runs0 = _cpu_drr_rate(ph0, 1, threads, 3.0, 20)
-    rate01, runs01 = _cpu_drr_rate(ph0, 1, 1, 3.0, 5)
-    torch.set_num_threads(threads)
-    return {"value": rate, "unit": "DRR/s", "cores": threads, "kind": "port",
+    f0, f0_runs = _cpu_drr_rate(ph0, 1, best, 3.0, 40, backward=False)
+    f01, f01_runs = _cpu_drr_rate(ph0, 1, 1, 2.0, 10, backward=False)
+    b0, b0_runs = _cpu_drr_rate(ph0, 1, best, 3.0, 20)
+    b01, b01_runs = _cpu_drr_rate(ph0, 1, 1, 2.0, 5)
+    torch.set_num_threads(best)
+    return {"value": rate, "unit": "DRR/s", "cores": best, "kind": "port",
             "sample": f"{runs} DRRs (1 pose each) of the 256^3/C=4/200x200/S=256 workload, fwd + pose "
-                      f"backward, oracle.torch_ref fp32 grid_sample on {threads} threads, median of runs",
+                      f"backward, oracle.torch_ref fp32 grid_sample on {best} threads, median of runs",
+            "host": host,
+            "per_thread_count": {str(n): {"value": r, "runs": k} for n, (r, k) in per_count.items()},
             "one_thread": {"value": rate1, "sample": f"{runs1} DRRs of the same workload on 1 thread"},
-            "cpu_case_128": {"value": rate0, "value_one_thread": rate01, "unit": "DRR/s",
-                             "sample": f"{runs0} / {runs01} DRRs, 128^3 at 1.5 mm, C=1, 200x200 at 1.0 mm, "
-                                       f"S=256, fwd + pose backward, on {threads} / 1 threads"}}
+            "cpu_case_128": {
+                "unit": "DRR/s", "threads": best,
+                "forward": f0, "forward_one_thread": f01, "fwd_bwd": b0, "fwd_bwd_one_thread": b01,
+                "sample": f"128^3 at 1.5 mm, C=1, 200x200 at 1.0 mm, S=256, 1 pose per DRR: {f0_runs} / {f01_runs} "
+                          f"forward-only and {b0_runs} / {b01_runs} fwd + pose backward DRRs on {best} / 1 threads"}}
 
 
 def mixed_precision_leg(steps: int = 20, warmup: int = 5):
     """The mixed-precision workload at full size: a 512 x 512 x 400 phantom at 0.8 x 0.8 x 1.0 mm
     stored in bf16 and in fp16, 2 labels, 8 poses, 512 x 512 detector at 0.8 mm, S = 512
-    (compensated sums), fwd + pose backward (single-pass adjoint, quad layout), graph-replayed
-    steps timed with HIP events, plus the plain fp32 kernel on the fp32 volume."""
+    (compensated sums), fwd + pose backward (single-pass adjoint; coefficient records, the
+    16-bit default, and quad records), graph-replayed steps timed with HIP events, plus the
+    plain fp32 kernel on the fp32 volume."""
     import nanodrr_amd.renderer as renderer
     from nanodrr_amd import render
     from nanodrr_amd.phantom import detector, make_phantom, phantom_subject, sample_poses
@@ -250,8 +355,10 @@ def mixed_precision_leg(steps: int = 20, warmup: int = 5):
     B, det, n_s = 8, 512, 512
     res = {"workload": "512x512x400 @0.8x0.8x1.0 mm, C=2, B=8, 512x512 @0.8 mm, S=512 (compensated), fwd + pose "
                        "backward, graph-replayed step"}
-    for name, dt, layout, adjoint in (("bf16", torch.bfloat16, "quad", "single_pass"),
-                                      ("fp16", torch.float16, "quad", "single_pass"),
+    for name, dt, layout, adjoint in (("bf16_coef", torch.bfloat16, "coef", "single_pass"),
+                                      ("fp16_coef", torch.float16, "coef", "single_pass"),
+                                      ("bf16_quad", torch.bfloat16, "quad", "single_pass"),
+                                      ("fp16_quad", torch.float16, "quad", "single_pass"),
                                       ("fp32_plain", torch.float32, "linear", "two_pass")):
         renderer.VOLUME_LAYOUT, renderer.ADJOINT = layout, adjoint
         subject = phantom_subject(ph, n_classes=2, dtype=dt, device=dev)
@@ -278,11 +385,12 @@ def mixed_precision_leg(steps: int = 20, warmup: int = 5):
             out = render(subject, k_inv, rt, sdd, det, det, n_samples=n_s)
             out.backward(wts.expand_as(out))
         ms = time_steps(graph.replay, steps, warmup)
-        res[name] = {"ms_per_step": ms, "drr_per_s": B * 1000.0 / ms}
+        res[name] = {"ms_per_step": ms, "drr_per_s": B * 1000.0 / ms, "layout": layout, "adjoint": adjoint}
         del graph, subject, out
         gc.collect()
         torch.cuda.empty_cache()
     renderer.VOLUME_LAYOUT, renderer.ADJOINT = "auto", "single_pass"
+    res["speedup_bf16_coef_over_fp32_plain"] = res["bf16_coef"]["drr_per_s"] / res["fp32_plain"]["drr_per_s"]
     return res
 
 
@@ -343,11 +451,11 @@ def counter_probe(case: str, reps: int = 4):
     copy calibrates FETCH_SIZE / WRITE_SIZE against known bytes)."""
     import nanodrr_amd.renderer as renderer
 
-    name, layout, adjoint = parse_case(case)
+    name, layout, adjoint, gvol = parse_case(case)
     renderer.VOLUME_LAYOUT = layout
-    subject, k_inv, rt_inv, sdd, _ = make_workload(DTYPES[name], torch.device("cuda:0"))
+    subject, k_inv, rt_inv, sdd, _ = make_workload(DTYPES[name], torch.device("cuda:0"), gvol)
     wts = torch.randn(1, N_CLASSES, DET, DET, generator=torch.Generator().manual_seed(0)).cuda()
-    f, b, _ = kernel_times(subject, k_inv, rt_inv, sdd, wts, reps, layout, adjoint)
+    f, b, _ = kernel_times(subject, k_inv, rt_inv, sdd, wts, reps, layout, adjoint, gvol)
     print(f"probe {case}: fwd {f * 1000:.1f} us, bwd {b * 1000:.1f} us", flush=True)
     x = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
     y = torch.empty_like(x)
@@ -357,19 +465,52 @@ def counter_probe(case: str, reps: int = 4):
     print("probe: 3 copies of 256 MiB done", flush=True)
 
 
-def measured_traffic(case: str):
-    """HBM bytes per launch (read + write, MB) of the case's fwd and bwd kernels, from
-    the committed rocprofv3 counter summary of this workload, or None."""
-    name, layout, adjoint = parse_case(case)
-    path = ROOT / "profiles" / f"raymarch_{name}_{layout}_{adjoint}_counters_bench256_c4_b16.md"
+KERNEL_SOURCE = ROOT / "nanodrr_amd" / "csrc" / "raymarch.hip"
+
+
+def kernel_source_digest() -> str:
+    """First 16 hex digits of the sha256 of the ray-march kernel source."""
+    import hashlib
+
+    return hashlib.sha256(KERNEL_SOURCE.read_bytes()).hexdigest()[:16]
+
+
+def measured_traffic(case: str, live_ms: dict, tolerance: float = 0.10, profiles: Path = ROOT / "profiles"):
+    """(traffic, note): HBM bytes per launch (read + write) of the case's fwd and bwd kernels from
+    the committed rocprofv3 counter summary of this workload, taken only when that summary was
+    measured on the current kernel source (its recorded digest of raymarch.hip matches) and its
+    kernel-trace average duration for the kernel agrees with this run's HIP-event time within
+    `tolerance`; else (None, why)."""
+    name, layout, adjoint, gvol = parse_case(case)
+    tag = f"{name}_{layout}_{adjoint}" + ("_gvol" if gvol else "")
+    path = profiles / f"raymarch_{tag}_counters_bench256_c4_b16.md"
     if not path.exists():
-        return None
-    out = {}
+        return None, f"no counter summary {path.name}"
+    digest, header, rows = None, None, {}
     for line in path.read_text().splitlines():
-        if line.startswith("| k_fwd") or line.startswith("| k_bwd"):
-            cells = [c.strip() for c in line.strip("|").split("|")]
-            out[cells[0][:5]] = (float(cells[2]) + float(cells[4])) * 1e6
-    return out or None
+        if line.startswith("Kernel source digest:"):
+            digest = line.split(":", 1)[1].strip().strip("`")
+        elif line.startswith("| role |"):
+            header = [c.strip() for c in line.strip().strip("|").split("|")]
+        elif header and line.startswith(("| fwd |", "| bwd |")):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            rows[cells[0]] = dict(zip(header, cells))
+    if digest != kernel_source_digest():
+        return None, f"{path.name} was measured on another kernel source (digest {digest})"
+    out, notes = {}, []
+    for role, row in rows.items():
+        try:
+            prof_us = float(row["avg us"])
+            traffic = (float(row["HBM read MB"]) + float(row["HBM write MB"])) * 1e6
+        except (KeyError, ValueError):
+            notes.append(f"{role}: unreadable row")
+            continue
+        live_us = live_ms[role] * 1000.0
+        if abs(prof_us - live_us) > tolerance * live_us:
+            notes.append(f"{role}: profiled at {prof_us:.1f} us, {live_us:.1f} us here")
+            continue
+        out[f"k_{role}"] = traffic
+    return (out or None), ("; ".join(notes) or f"{path.name}, kernel source digest {digest}")
 
 
 def registration_leg(n_iters: int = 300, batch: int = 64):
@@ -428,20 +569,25 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cases",
-                    default="bf16:quad:single,fp32:linear:two,fp32:quad:single,bf16:stencil:single,bf16:stencil:two",
-                    help="dtype:layout:adjoint cases; the first is the headline, fp32:linear:two the plain kernel")
+    ap.add_argument("--cases", default=DEFAULT_CASES,
+                    help="dtype:layout:adjoint[+gvol] cases; the first is the headline, fp32:linear:two the plain "
+                         "kernel")
     ap.add_argument("--probe", default=None,
-                    help="run only the counter probe of one case, e.g. bf16:stencil:single")
+                    help="run only the counter probe of one case, e.g. bf16:coef:single")
+    ap.add_argument("--probe-reps", type=int, default=4, help="launches per op in the counter probe")
     ap.add_argument("--no-registration", action="store_true", help="skip the batched registration leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--registration-only", action="store_true", help="run only the registration leg")
     ap.add_argument("--reference-shape", action="store_true",
                     help="run only the forward at the reference benchmark's shape (B=1, C=1, 200x200 @2 mm, S=500)")
     ap.add_argument("--mixed", action="store_true",
                     help="run only the mixed-precision workload (512x512x400, C=2, B=8, 512x512, S=512)")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: every figure is one GPU's; --gpus must be 1"}))
+        sys.exit(2)
     if args.probe:
-        counter_probe(args.probe)
+        counter_probe(args.probe, args.probe_reps)
         return
     if args.registration_only:
         print(json.dumps(registration_leg()), flush=True)
@@ -452,9 +598,6 @@ def main():
     if args.reference_shape:
         print(json.dumps(reference_shape_leg()), flush=True)
         return
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
-        sys.exit(2)
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(2)
@@ -464,31 +607,33 @@ def main():
     results = {}
     ph = None
     for case in args.cases.split(","):
-        name, layout, adjoint = parse_case(case)
+        name, layout, adjoint, gvol = parse_case(case)
         dtype = DTYPES[name]
         renderer.VOLUME_LAYOUT = layout
         renderer.ADJOINT = ADJOINTS[adjoint]
-        subject, k_inv, rt_inv, sdd, ph = make_workload(dtype, dev)
+        subject, k_inv, rt_inv, sdd, ph = make_workload(dtype, dev, gvol)
         wts = torch.randn(1, N_CLASSES, DET, DET, generator=torch.Generator().manual_seed(0)).to(dev)
         ms_eager = time_steps(lambda: step_fn(subject, k_inv, rt_inv, sdd, wts), args.steps, args.warmup)
         ms = time_steps(graph_step(subject, k_inv, rt_inv, sdd, wts), args.steps, args.warmup)
-        ms_fwd, ms_bwd, geom = kernel_times(subject, k_inv, rt_inv, sdd, wts, max(10, args.steps), layout, adjoint)
-        n_in, b_fwd, b_bwd = algorithmic_bytes(subject, geom, dtype, adjoint)
+        ms_fwd, ms_bwd, geom = kernel_times(subject, k_inv, rt_inv, sdd, wts, max(10, args.steps), layout, adjoint,
+                                            gvol)
+        n_in, b_fwd, b_bwd = algorithmic_bytes(subject, geom, dtype, adjoint, gvol)
         results[case] = dict(ms_per_step=ms, drr_per_s=BATCH * 1000.0 / ms, ms_eager=ms_eager,
                              drr_per_s_eager=BATCH * 1000.0 / ms_eager, ms_fwd=ms_fwd, ms_bwd=ms_bwd,
                              inside_samples=n_in, bytes_fwd=b_fwd, bytes_bwd=b_bwd,
                              gbs_fwd=b_fwd / ms_fwd / 1e6, gbs_bwd=b_bwd / ms_bwd / 1e6)
         print(f"# {case}: {results[case]}", file=sys.stderr, flush=True)
         del subject
+        gc.collect()
         torch.cuda.empty_cache()
 
     renderer.ADJOINT = "single_pass"
     head = next(iter(results))
-    head_dtype, head_layout, head_adjoint = parse_case(head)
+    head_dtype, head_layout, head_adjoint, _ = parse_case(head)
     r = results[head]
     dominant = "bwd" if r["ms_bwd"] >= r["ms_fwd"] else "fwd"
     achieved = r[f"gbs_{dominant}"]
-    traffic = measured_traffic(head)
+    traffic, traffic_note = measured_traffic(head, {"fwd": r["ms_fwd"], "bwd": r["ms_bwd"]})
     line = {
         "metric": "DRR/s fwd+bwd 200x200 @256^3 (fp32 vs bf16)",
         "value": r["drr_per_s"],
@@ -509,8 +654,11 @@ def main():
             "step": "render + weighted-sum backward captured in one HIP graph, replayed (eager also given)",
             "per_case": {k: {"drr_per_s": v["drr_per_s"], "ms_per_step": v["ms_per_step"],
                              "drr_per_s_eager": v["drr_per_s_eager"], "ms_per_step_eager": v["ms_eager"],
-                              "ms_fwd_kernel": v["ms_fwd"], "ms_bwd_kernel": v["ms_bwd"],
-                              "gbs_fwd": v["gbs_fwd"], "gbs_bwd": v["gbs_bwd"]} for k, v in results.items()},
+                             "ms_fwd_kernel": v["ms_fwd"], "ms_bwd_kernel": v["ms_bwd"],
+                             "bytes_fwd": v["bytes_fwd"], "bytes_bwd": v["bytes_bwd"],
+                             "gbs_fwd": v["gbs_fwd"], "gbs_bwd": v["gbs_bwd"],
+                             "frac_fwd": v["gbs_fwd"] / HBM_PEAK_GBS, "frac_bwd": v["gbs_bwd"] / HBM_PEAK_GBS}
+                         for k, v in results.items()},
         },
         "roofline": {
             "bound": "hbm",
@@ -518,7 +666,7 @@ def main():
             "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": None if traffic is None else traffic.get(f"k_{dominant}"),
-            "traffic_source": "profiles/ rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated" if traffic else None,
+            "traffic_source": traffic_note,
             "algorithmic_bytes_per_launch": r[f"bytes_{dominant}"], "inside_samples": r["inside_samples"],
         },
     }
@@ -533,6 +681,10 @@ def main():
     gc.collect()
     torch.cuda.empty_cache()  # the march cases' volumes and graph pools
     line["config"]["reference_benchmark_shape"] = reference_shape_leg()
+    if not args.no_mixed:
+        gc.collect()
+        torch.cuda.empty_cache()
+        line["mixed_precision"] = mixed_precision_leg()
     if not args.no_registration:
         gc.collect()
         torch.cuda.empty_cache()

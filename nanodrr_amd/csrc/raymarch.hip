@@ -161,6 +161,7 @@ struct Params {
   float* gtgt;
   float* gstep;
   float* gvol;
+  int gvol_merge;  // merge lanes' volume-gradient atomics per cell (cell keys fit 32 bits)
   int64_t src_bs, src_rs, tgt_bs, step_bs;
   int64_t go_bs;  // batch stride of go in the single-pass backward (0: shared by every pose)
   uint32_t vol_bytes, lab_bytes;  // sizes of the packed volume / label buffers
@@ -926,6 +927,81 @@ __device__ inline float wave_sum(float v) {
   return v;
 }
 
+__device__ inline int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Volume-gradient atomics of one sample per lane, merged across the wave by cell. A wave's
+// atomic instruction costs the L2 one pass per 64-byte line for every copy of an address in
+// that line (distinct addresses in one line share a pass, equal ones do not), and
+// neighbouring rays often sample the same cell, so lanes first add their contributions over
+// runs of equal cells - chains of lanes l, l + 1, ... along each 8-lane row of the wave's
+// 8 x 8 ray patch (segmented suffix sums, steps 1, 2, 4), then chains down its columns (steps
+// 8, 16, 32) - and only the first lane of a run issues the eight atomics. A run is defined by
+// lane adjacency alone, so the result is exact for any ray layout (only the merging rate
+// depends on it). Every lane of the wave must call this (it shuffles); lanes with g == 0, or
+// whose cell has no corner in the volume, take a key no other lane has and issue nothing.
+// Cell keys are ((iz + 1)(H + 2) + iy + 1)(W + 2) + ix + 1 < 2^32 - 64 (checked on the host).
+__device__ inline void gvol_add_merged(float* __restrict__ gvol, const Params& p, const Cell& c, float g) {
+  const int lane = threadIdx.x & 63, col = lane & 7, row = lane >> 3;
+  const bool touch = g != 0.0f && c.ix >= -1 && c.ix < p.W && c.iy >= -1 && c.iy < p.H && c.iz >= -1 && c.iz < p.D;
+  const uint32_t key =
+      touch ? (static_cast<uint32_t>(c.iz + 1) * static_cast<uint32_t>(p.H + 2) + static_cast<uint32_t>(c.iy + 1)) *
+                      static_cast<uint32_t>(p.W + 2) +
+                  static_cast<uint32_t>(c.ix + 1)
+            : 0xffffffffu - static_cast<uint32_t>(lane);
+  // Every shuffle below runs on all lanes: a lane reads another's value only while that lane is
+  // active, so none sits under a condition (or the right side of && / ||).
+  const int ikey = static_cast<int>(key);
+  const int nx = min(lane + 1, 63), nx2 = min(lane + 2, 63), nx4 = min(lane + 4, 63);
+  const int nz = min(lane + 8, 63), nz2 = min(lane + 16, 63), nz4 = min(lane + 32, 63);
+  const int px = max(lane - 1, 0), pz = max(lane - 8, 0);
+  // e1: lanes l and l + 1 share the cell; e2: l .. l + 2; e4: l .. l + 4 (within the row)
+  const int key_x = __shfl(ikey, nx, 64);
+  const int e1 = (col < 7 && key_x == ikey) ? 1 : 0;
+  const int e1_next = __shfl(e1, nx, 64), e1_prev = __shfl(e1, px, 64);
+  const int e2 = (e1 && e1_next) ? 1 : 0;
+  const int e2_next = __shfl(e2, nx2, 64);
+  const int e4 = (e2 && e2_next) ? 1 : 0;
+  const bool head_x = col == 0 || !e1_prev;
+  // f1: lanes l and l + 8 share the cell; f2: l, l + 8, l + 16; f4: l .. l + 32 (down the column)
+  const int key_z = __shfl(ikey, nz, 64);
+  const int f1 = (row < 7 && key_z == ikey) ? 1 : 0;
+  const int f1_next = __shfl(f1, nz, 64), f1_prev = __shfl(f1, pz, 64);
+  const int f2 = (f1 && f1_next) ? 1 : 0;
+  const int f2_next = __shfl(f2, nz2, 64);
+  const int f4 = (f2 && f2_next) ? 1 : 0;
+  const bool head_z = row == 0 || !f1_prev;
+  const float wx[2] = {1.0f - c.fx, c.fx};
+  const float wy[2] = {1.0f - c.fy, c.fy};
+  const float wz[2] = {1.0f - c.fz, c.fz};
+#pragma unroll
+  for (int corner = 0; corner < 8; ++corner) {
+    const int cx = corner & 1, cy = (corner >> 1) & 1, cz = corner >> 2;
+    float v = g * (wx[cx] * wy[cy] * wz[cz]);
+    float o = __shfl(v, nx, 64);
+    v += e1 ? o : 0.0f;
+    o = __shfl(v, nx2, 64);
+    v += e2 ? o : 0.0f;
+    o = __shfl(v, nx4, 64);
+    v += e4 ? o : 0.0f;
+    v = head_x ? v : 0.0f;
+    o = __shfl(v, nz, 64);
+    v += f1 ? o : 0.0f;
+    o = __shfl(v, nz2, 64);
+    v += f2 ? o : 0.0f;
+    o = __shfl(v, nz4, 64);
+    v += f4 ? o : 0.0f;
+    const int jx = c.ix + cx, jy = c.iy + cy, jz = c.iz + cz;
+    const bool inb = static_cast<unsigned>(jx) < static_cast<unsigned>(p.W) &&
+                     static_cast<unsigned>(jy) < static_cast<unsigned>(p.H) &&
+                     static_cast<unsigned>(jz) < static_cast<unsigned>(p.D);
+    if (head_z && inb && v != 0.0f) unsafeAtomicAdd(gvol + (jz * p.H + jy) * p.W + jx, v);
+  }
+}
+
 template <int DT, int L, int CV, bool GCAM, bool GVOL, int NCH>
 __global__ __launch_bounds__(kTileRays * NCH) __attribute__((amdgpu_waves_per_eu(NDRR_WAVES_PER_EU)))
 void k_bwd(Params p) {
@@ -951,7 +1027,12 @@ void k_bwd(Params p) {
 
   float Gx = 0.f, Gy = 0.f, Gz = 0.f, Tx = 0.f, Ty = 0.f, Tz = 0.f, astep = 0.f;
   using B = Batch<DT, L, (CV > 1)>;
-  for (int k0 = 0; k0 < r.n_s; k0 += B::U) {
+  // Merged volume-gradient atomics exchange values across the wave every sample: every lane then
+  // runs the wave's longest loop, its extra samples not live (they repeat its last sample, or for
+  // a ray that misses the volume read zeros or clamped addresses, and add nothing).
+  const bool merge = GVOL && p.gvol_merge;
+  const int n_loop = merge ? wave_max(r.n_s) : r.n_s;
+  for (int k0 = 0; k0 < n_loop; k0 += B::U) {
     B batch;
     batch.issue(src, r, k0);
 #pragma unroll
@@ -977,7 +1058,9 @@ void k_bwd(Params p) {
       Tx += gx * t; Ty += gy * t; Tz += gz * t;
       if constexpr (GCAM) astep += gg * interp_value(batch.f[u], c);
       if constexpr (GVOL) {
-        if (g != 0.0f) {
+        if (merge) {
+          gvol_add_merged(p.gvol, p, c, g);
+        } else if (g != 0.0f) {
           const float wx[2] = {1.0f - fx, fx};
           const float wy[2] = {1.0f - fy, fy};
           const float wz[2] = {1.0f - fz, fz};
@@ -1723,6 +1806,7 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_l
   p.go = go;
   p.gpart = workspace;
   p.gsrc = gsrc; p.gtgt = gtgt; p.gstep = gstep; p.gvol = gvol;
+  p.gvol_merge = static_cast<uint64_t>(D + 2) * (H + 2) * (W + 2) + 64 <= (uint64_t{1} << 32);
   const dim3 grid(p.nblocks, B);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const bool gcam = need_gcam != 0, gv = gvol != nullptr;

@@ -48,11 +48,13 @@ def inside_samples(M: torch.Tensor, src: torch.Tensor, tgt: torch.Tensor, n_samp
 
 
 def march_bytes(n_inside: int, n_rays: int, n_poses: int, n_classes: int, vol_dtype: torch.dtype,
-                backward: bool = False, need_gcam: bool = False, jac_planes: int = 0) -> int:
+                backward: bool = False, need_gcam: bool = False, jac_planes: int = 0, need_gvol: bool = False) -> int:
     """Algorithmic HBM bytes of one forward (or two-pass backward) launch; a forward
-    with jac_planes > 0 also writes the single-pass adjoint's per-ray sums."""
+    with jac_planes > 0 also writes the single-pass adjoint's per-ray sums. A backward
+    with volume gradients adds, per touching sample, the fp32 read-modify-write of the
+    eight corners' gradient entries (8 * 8 bytes)."""
     ev = _ELEM[vol_dtype]
-    per_sample = 8 * ev + (1 if n_classes > 1 else 0)
+    per_sample = 8 * ev + (1 if n_classes > 1 else 0) + (64 if backward and need_gvol else 0)
     per_ray = 12 + 4 + 4 * n_classes + (28 if (backward and need_gcam) else 0) + 4 * jac_planes * n_classes
     return n_inside * per_sample + n_rays * per_ray + 48 * n_poses
 
