@@ -26,9 +26,10 @@
  *        ndrr_pack_volume / ndrr_pack_coef: NDRR_LAYOUT_QUAD, NDRR_LAYOUT_STENCIL
  *        or NDRR_LAYOUT_COEF records (vol_dtype is then the record type)
  *   lab  int8 class per voxel (values outside [0, C) are dropped) in the form
- *        vol_layout reads: the [D][H][W] map for NDRR_LAYOUT_LINEAR,
- *        NDRR_LAYOUT_STENCIL and NDRR_LAYOUT_COEF, its label quads
- *        (ndrr_pack_labels) for NDRR_LAYOUT_QUAD; ignored when n_classes == 1
+ *        vol_layout reads: the [D][H][W] map for NDRR_LAYOUT_LINEAR, its padded
+ *        copy for NDRR_LAYOUT_STENCIL and NDRR_LAYOUT_COEF and its label quads
+ *        for NDRR_LAYOUT_QUAD (both by ndrr_pack_labels); ignored when
+ *        n_classes == 1
  * Sample s of ray (b, n) sits at M[b] @ [lerp(src, tgt, s / (S - 1)), 1].
  * Rays are mapped to detector tiles of a `width`-wide image when width > 0
  * (N must be a multiple of width), else to runs of consecutive rays; see
@@ -308,8 +309,12 @@ int ndrr_pack_coef(const void* vol, int vol_dtype, int record_dtype, int D, int 
  * Labels in the form a layout's march reads. For NDRR_LAYOUT_QUAD: one uint32
  * per quad record (x, y0, z0) holding the int8 classes lab[z0+a][y0+b][x] in
  * byte 2a + b (zero outside the volume), so the nearest-label lookup shares
- * the volume's record index; for the other layouts the [D][H][W] int8 map
- * itself (copied). ndrr_packed_label_bytes gives the size of `out`.
+ * the volume's record index; for NDRR_LAYOUT_STENCIL and NDRR_LAYOUT_COEF the
+ * padded map int8 [D + 4][H + 4][W + 4] with lab[z][y][x] at (z + 2, y + 2,
+ * x + 2) and 0 elsewhere, which the kernels read at the rounded position
+ * clamped to the border, without bounds tests; for NDRR_LAYOUT_LINEAR the
+ * [D][H][W] map itself (copied). The packed layouts also need
+ * (H + 4)(W + 4) < 2^23. ndrr_packed_label_bytes gives the size of `out`.
  * Replaces the nearest-label gather of _fused.py:222-228 (rounding,
  * zero padding, `.to(tl.int32)`), which the kernels then resolve from the
  * sample's cell fractions.

@@ -115,8 +115,9 @@ def _(vol, record_dtype):
 
 @torch.library.custom_op("nanodrr_amd::pack_labels", mutates_args=())
 def pack_labels(lab: torch.Tensor, layout: str) -> torch.Tensor:
-    """Labels in the form `layout`'s march reads (ndrr_pack_labels): the label quads
-    for "quad", a copy of the (D, H, W) int8 map otherwise."""
+    """Labels in the form `layout`'s march reads (ndrr_pack_labels): the label quads for "quad",
+    the padded (D + 4, H + 4, W + 4) map for "stencil" and "coef" (flattened, like the quads), a
+    copy of the (D, H, W) int8 map for "linear"."""
     if not lab.is_cuda or lab.dim() != 3 or lab.dtype != torch.int8:
         raise ValueError("pack_labels needs a (D, H, W) int8 device tensor (Subject.label_i8)")
     if layout not in _lib.LAYOUTS:
@@ -129,7 +130,7 @@ def pack_labels(lab: torch.Tensor, layout: str) -> torch.Tensor:
     with torch.cuda.device(lab.device):
         rc = lib.ndrr_pack_labels(lab.data_ptr(), code, D, H, W, out.data_ptr(), _stream(lab.device))
     _lib.check(rc, "ndrr_pack_labels")
-    return out if layout == "quad" else out.view(D, H, W)
+    return out.view(D, H, W) if layout == "linear" else out
 
 
 @pack_labels.register_fake
@@ -137,6 +138,8 @@ def _(lab, layout):
     D, H, W = lab.shape
     if layout == "quad":
         return torch.empty((D + 1) * (H + 1) * (W + 2) * 4, device=lab.device, dtype=torch.int8)
+    if layout in ("stencil", "coef"):
+        return torch.empty((D + 4) * (H + 4) * (W + 4), device=lab.device, dtype=torch.int8)
     return torch.empty(D, H, W, device=lab.device, dtype=torch.int8)
 
 
@@ -172,10 +175,11 @@ def _geometry(vol, lab, M, src, tgt, step, n_samples, n_classes, width, layout="
     if n_classes > 1:
         if lab.dtype != torch.int8 or not lab.is_contiguous() or lab.device != vol.device:
             raise ValueError("lab must be a contiguous int8 tensor on the volume's device")
-        if layout == "quad":
-            need = _lib.lib().ndrr_packed_label_bytes(*vol.shape, _lib.LAYOUT_QUAD)
+        if layout != "linear":
+            need = _lib.lib().ndrr_packed_label_bytes(*vol.shape, _lib.LAYOUTS[layout])
             if lab.numel() != need:
-                raise ValueError(f"the quad layout reads label quads (pack_labels): {need} bytes, got {lab.numel()}")
+                raise ValueError(f"the {layout} layout reads its packed labels (pack_labels): {need} bytes, "
+                                 f"got {lab.numel()}")
         elif lab.shape != vol.shape:
             raise ValueError("lab must be the (D, H, W) int8 label map for this layout")
     if width < 0 or (width > 0 and N % width != 0):

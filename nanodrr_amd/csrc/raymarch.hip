@@ -382,11 +382,20 @@ struct Cell {
   float fx, fy, fz;
 };
 
+// floor(x) as an int in one instruction (the compiler emits a floor and a conversion).
+__device__ inline int floor_to_int(float x) {
+  int i;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(i) : "v"(x));
+  return i;
+}
+
+// The fraction x - floor(x) comes from v_fract_f32, which keeps it below 1 where the
+// subtraction would round up to 1 (x a tiny negative number); floor_to_int saturates
+// beyond the int range like the conversion of floorf.
 __device__ inline Cell cell_of(float x, float y, float z) {
   Cell c;
-  const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
-  c.fx = x - xf; c.fy = y - yf; c.fz = z - zf;
-  c.ix = static_cast<int>(xf); c.iy = static_cast<int>(yf); c.iz = static_cast<int>(zf);
+  c.fx = __builtin_amdgcn_fractf(x); c.fy = __builtin_amdgcn_fractf(y); c.fz = __builtin_amdgcn_fractf(z);
+  c.ix = floor_to_int(x); c.iy = floor_to_int(y); c.iz = floor_to_int(z);
   return c;
 }
 
@@ -429,15 +438,16 @@ template <int DT, int L> struct Fetch;
 template <int DT> struct Fetch<DT, NDRR_LAYOUT_LINEAR> {
   static constexpr int kInFlight = 2;
   float v[8];
-  __device__ inline void load(const Src& s, const Cell& c) {
+  __device__ inline void load(const Src& s, const Cell& c, bool live) {
     const int D = s.D, H = s.H, W = s.W;
     const auto* __restrict__ vol = static_cast<const typename Vox<DT>::T*>(s.vol);
     const bool bx0 = static_cast<unsigned>(c.ix) < static_cast<unsigned>(W);
     const bool bx1 = static_cast<unsigned>(c.ix + 1) < static_cast<unsigned>(W);
     const bool by0 = static_cast<unsigned>(c.iy) < static_cast<unsigned>(H);
     const bool by1 = static_cast<unsigned>(c.iy + 1) < static_cast<unsigned>(H);
-    const bool bz0 = static_cast<unsigned>(c.iz) < static_cast<unsigned>(D);
-    const bool bz1 = static_cast<unsigned>(c.iz + 1) < static_cast<unsigned>(D);
+    // a sample that is not live reads zeros, like a corner outside the volume
+    const bool bz0 = live & (static_cast<unsigned>(c.iz) < static_cast<unsigned>(D));
+    const bool bz1 = live & (static_cast<unsigned>(c.iz + 1) < static_cast<unsigned>(D));
     const int cx0 = min(max(c.ix, 0), W - 1), cx1 = min(max(c.ix + 1, 0), W - 1);
     const int cy0 = min(max(c.iy, 0), H - 1), cy1 = min(max(c.iy + 1, 0), H - 1);
     const int cz0 = min(max(c.iz, 0), D - 1), cz1 = min(max(c.iz + 1, 0), D - 1);
@@ -491,18 +501,26 @@ __device__ inline bool cell_exists(const Src& s, const Cell& c) {
 // instruction). check_common keeps (D + 1)(H + 1) and the row below 2^24 for
 // the packed layouts, so the operands of an existing cell fit; a cell that
 // does not exist gets an offset past the buffer whatever this returns.
+__device__ inline uint32_t mul_u24(uint32_t a, uint32_t b) {
+  // the masks state the operands' width, so this is one full-rate 24-bit multiply (__umul24
+  // came out as a quarter-rate 32-bit one in these expressions)
+  return (a & 0xffffffu) * (b & 0xffffffu);
+}
+
 __device__ inline uint32_t record_of(const Src& s, const Cell& c, int row) {
-  return __umul24(__umul24(static_cast<uint32_t>(c.iz + 1), static_cast<uint32_t>(s.H + 1)) +
-                      static_cast<uint32_t>(c.iy + 1),
-                  static_cast<uint32_t>(row)) +
+  return mul_u24(mul_u24(static_cast<uint32_t>(c.iz + 1), static_cast<uint32_t>(s.H + 1)) +
+                     static_cast<uint32_t>(c.iy + 1),
+                 static_cast<uint32_t>(row)) +
          static_cast<uint32_t>(c.ix + 1);
 }
 
 // Byte offset of a cell's record, or one past 2 GiB (so past the end of the
-// buffer) when the cell does not exist. Bit arithmetic rather than a select,
-// which the compiler would otherwise turn into a divergent branch.
-__device__ inline uint32_t record_offset(const Src& s, const Cell& c, int row, uint32_t bytes) {
-  const uint32_t gone = static_cast<uint32_t>(!cell_exists(s, c)) << 31;
+// buffer, which reads zeros) when the cell does not exist or the sample is not
+// live (past the ray's last sample): its record then adds nothing without a
+// select per value. Bit arithmetic rather than a select, which the compiler
+// would otherwise turn into a divergent branch.
+__device__ inline uint32_t record_offset(const Src& s, const Cell& c, int row, uint32_t bytes, bool live) {
+  const uint32_t gone = static_cast<uint32_t>(!(cell_exists(s, c) & live)) << 31;
   return (record_of(s, c, row) * bytes) | gone;
 }
 
@@ -513,8 +531,8 @@ __device__ inline uint4 load16(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
 template <int DT> struct Fetch<DT, NDRR_LAYOUT_STENCIL> {  // bf16 / fp16: 16-byte records
   static constexpr int kInFlight = NDRR_IN_FLIGHT;
   uint4 q;
-  __device__ inline void load(const Src& s, const Cell& c) {
-    const uint32_t off = record_offset(s, c, s.W + 1, 16u);
+  __device__ inline void load(const Src& s, const Cell& c, bool live) {
+    const uint32_t off = record_offset(s, c, s.W + 1, 16u, live);
     q = load16(s.vrs, off);
   }
   __device__ inline Corners corners() const {
@@ -530,8 +548,8 @@ template <int DT> struct Fetch<DT, NDRR_LAYOUT_STENCIL> {  // bf16 / fp16: 16-by
 template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_STENCIL> {  // fp32: 32-byte records
   static constexpr int kInFlight = 2;
   uint4 a, b;
-  __device__ inline void load(const Src& s, const Cell& c) {
-    const uint32_t off = record_offset(s, c, s.W + 1, 32u);
+  __device__ inline void load(const Src& s, const Cell& c, bool live) {
+    const uint32_t off = record_offset(s, c, s.W + 1, 32u, live);
     a = load16(s.vrs, off);
     b = load16(s.vrs, off + 16u);
   }
@@ -551,8 +569,8 @@ template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_STENCIL> {  // fp32: 32-byte reco
 template <int DT> struct Fetch<DT, NDRR_LAYOUT_QUAD> {  // bf16 / fp16: two 8-byte records
   static constexpr int kInFlight = NDRR_IN_FLIGHT;
   uint4 q;  // x: (z0,y0) (z0,y0+1) at x0; y: (z0+1,y0) (z0+1,y0+1) at x0; z, w: the same at x0 + 1
-  __device__ inline void load(const Src& s, const Cell& c) {
-    const uint32_t off = record_offset(s, c, s.W + 2, 8u);
+  __device__ inline void load(const Src& s, const Cell& c, bool live) {
+    const uint32_t off = record_offset(s, c, s.W + 2, 8u, live);
     q = load16(s.vrs, off);
   }
   __device__ inline Corners corners() const {
@@ -568,8 +586,8 @@ template <int DT> struct Fetch<DT, NDRR_LAYOUT_QUAD> {  // bf16 / fp16: two 8-by
 template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_QUAD> {  // fp32: two 16-byte records
   static constexpr int kInFlight = 2;
   uint4 a, b;  // quads at x0 and x0 + 1: (z0,y0) (z0,y0+1) (z0+1,y0) (z0+1,y0+1)
-  __device__ inline void load(const Src& s, const Cell& c) {
-    const uint32_t off = record_offset(s, c, s.W + 2, 16u);
+  __device__ inline void load(const Src& s, const Cell& c, bool live) {
+    const uint32_t off = record_offset(s, c, s.W + 2, 16u, live);
     a = load16(s.vrs, off);
     b = load16(s.vrs, off + 16u);
   }
@@ -647,7 +665,7 @@ struct Coef {
 template <int DT> struct Fetch<DT, NDRR_LAYOUT_COEF> {  // bf16 / fp16: 16-byte records
   static constexpr int kInFlight = NDRR_IN_FLIGHT;
   uint4 q;
-  __device__ inline void load(const Src& s, const Cell& c) { q = load16(s.vrs, record_offset(s, c, s.W + 1, 16u)); }
+  __device__ inline void load(const Src& s, const Cell& c, bool live) { q = load16(s.vrs, record_offset(s, c, s.W + 1, 16u, live)); }
   __device__ inline Coef coef() const {
     if constexpr (DT == NDRR_BF16)
       return {bf16_lo(q.x), f16_hi(q.x), f16_lo(q.y), f16_hi(q.y),
@@ -667,8 +685,8 @@ template <int L> __device__ inline float record_unscale(const Params& p) {
 template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_COEF> {  // fp32: 32-byte records
   static constexpr int kInFlight = 2;
   uint4 a, b;
-  __device__ inline void load(const Src& s, const Cell& c) {
-    const uint32_t off = record_offset(s, c, s.W + 1, 32u);
+  __device__ inline void load(const Src& s, const Cell& c, bool live) {
+    const uint32_t off = record_offset(s, c, s.W + 1, 32u, live);
     a = load16(s.vrs, off);
     b = load16(s.vrs, off + 16u);
   }
@@ -727,13 +745,13 @@ __device__ inline void interp_grad(const Fetch<DT, NDRR_LAYOUT_COEF>& f, const C
 
 // Nearest label (round half to even, zero padded: outside the volume is class 0).
 __device__ inline int nearest_label(const int8_t* __restrict__ lab, int D, int H, int W,
-                                    float x, float y, float z) {
+                                    float x, float y, float z, bool live) {
   const int jx = static_cast<int>(rintf(x));
   const int jy = static_cast<int>(rintf(y));
   const int jz = static_cast<int>(rintf(z));
-  const bool inb = static_cast<unsigned>(jx) < static_cast<unsigned>(W) &&
-                   static_cast<unsigned>(jy) < static_cast<unsigned>(H) &&
-                   static_cast<unsigned>(jz) < static_cast<unsigned>(D);
+  const bool inb = live & (static_cast<unsigned>(jx) < static_cast<unsigned>(W)) &
+                   (static_cast<unsigned>(jy) < static_cast<unsigned>(H)) &
+                   (static_cast<unsigned>(jz) < static_cast<unsigned>(D));
   const int cx = min(max(jx, 0), W - 1), cy = min(max(jy, 0), H - 1), cz = min(max(jz, 0), D - 1);
   const int l = static_cast<int>(lab[(cz * H + cy) * W + cx]);
   return inb ? l : 0;
@@ -744,15 +762,34 @@ __device__ inline int nearest_label(const int8_t* __restrict__ lab, int D, int H
 // outside the volume). The nearest voxel, rint(p) (round half to even, as
 // grid_sample's nearest mode), is the corner of the sample's cell at offset 1
 // along each axis where rint(p) > floor(p). Branch-free.
-__device__ inline int quad_label(const Src& s, const Cell& c, float x, float y, float z) {
-  const uint32_t off = record_offset(s, c, s.W + 2, 4u);
+__device__ inline int quad_label(const Src& s, const Cell& c, float x, float y, float z, bool live) {
+  const uint32_t off = record_offset(s, c, s.W + 2, 4u, live);
   const uint2 w = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(s.lrs, off, 0, 0));
-  const bool ox = rintf(x) > floorf(x);
-  const bool oy = rintf(y) > floorf(y);
-  const bool oz = rintf(z) > floorf(z);
+  // rint(p) > floor(p) exactly when rint rounds p up, i.e. rint(p) > p
+  const bool ox = rintf(x) > x;
+  const bool oy = rintf(y) > y;
+  const bool oz = rintf(z) > z;
   const uint32_t word = ox ? w.y : w.x;
   const uint32_t shift = (oz ? 16u : 0u) | (oy ? 8u : 0u);
   return static_cast<int8_t>(static_cast<uint8_t>(word >> shift));
+}
+
+// Nearest label from the padded class map of the stencil and coefficient
+// layouts: int8 [D + 4][H + 4][W + 4] holding the class of voxel (z, y, x) at
+// (z + 2, y + 2, x + 2) and class 0 around it. The nearest voxel is rint(p)
+// (round half to even, grid_sample's nearest mode) of the position clamped to
+// [-2, dim + 1]: a nearest voxel outside the volume lands in the border, class
+// 0 as the reference gives it, so no bounds test is needed. Signed 24-bit
+// multiply-adds form the offset ((H + 4)(W + 4) < 2^23, checked on the host);
+// a sample that is not live reads past the buffer (class 0).
+__device__ inline int padded_label(const Src& s, float x, float y, float z, bool live) {
+  const int jx = static_cast<int>(rintf(__builtin_amdgcn_fmed3f(x, -2.0f, static_cast<float>(s.W + 1))));
+  const int jy = static_cast<int>(rintf(__builtin_amdgcn_fmed3f(y, -2.0f, static_cast<float>(s.H + 1))));
+  const int jz = static_cast<int>(rintf(__builtin_amdgcn_fmed3f(z, -2.0f, static_cast<float>(s.D + 1))));
+  const int w4 = s.W + 4, hw4 = (s.H + 4) * w4;
+  const int off = __mul24(jz, hw4) + __mul24(jy, w4) + jx + (2 * hw4 + 2 * w4 + 2);
+  const uint32_t o = static_cast<uint32_t>(off) | (static_cast<uint32_t>(!live) << 31);
+  return static_cast<int8_t>(__builtin_amdgcn_raw_buffer_load_b8(s.lrs, o, 0, 0));
 }
 
 // a += x rounded once (never contracted with the product that formed x).
@@ -781,15 +818,20 @@ struct Batch {
   __device__ inline void issue(const Src& s, const Ray& r, int k0) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
+      // A sample past the ray's last (not live) repeats the last one's position and reads zeros
+      // (its record and label loads go past their buffers), so it adds nothing.
+      const bool live = k0 + u < r.n_s;
       const float k = static_cast<float>(r.k_base + min(k0 + u, r.n_s - 1));
       const float x = sample_coord(r.ax, r.dx, k), y = sample_coord(r.ay, r.dy, k), z = sample_coord(r.az, r.dz, k);
       cell[u] = cell_of(x, y, z);
-      f[u].load(s, cell[u]);
+      f[u].load(s, cell[u], live);
       if constexpr (LABELS) {
         if constexpr (L == NDRR_LAYOUT_QUAD)
-          cls[u] = quad_label(s, cell[u], x, y, z);
+          cls[u] = quad_label(s, cell[u], x, y, z, live);
+        else if constexpr (L == NDRR_LAYOUT_LINEAR)
+          cls[u] = nearest_label(s.lab, s.D, s.H, s.W, x, y, z, live);
         else
-          cls[u] = nearest_label(s.lab, s.D, s.H, s.W, x, y, z);
+          cls[u] = padded_label(s, x, y, z, live);
       }
     }
   }
@@ -836,19 +878,14 @@ void k_fwd(Params p) {
     batch.issue(src, r, k0);
 #pragma unroll
     for (int u = 0; u < B::U; ++u) {
+      // A sample that is not live fetched zeros (Batch::issue): its value and gradient are 0.
       const Cell& c = batch.cell[u];
-      const bool live = k0 + u < r.n_s;
-      float dvx, dvy, dvz;
-      const float val =
-          live ? (JAC ? interp_value_grad(batch.f[u], c, dvx, dvy, dvz) : interp_value(batch.f[u], c)) : 0.0f;
-      float tx = 0.0f, ty = 0.0f, tz = 0.0f;
-      if constexpr (JAC) {
-        dvx = live ? dvx : 0.0f;
-        dvy = live ? dvy : 0.0f;
-        dvz = live ? dvz : 0.0f;
-        const float t = fmaf(static_cast<float>(r.s_lo + k0 + u), p.inv_s, r.t_off);
-        tx = dvx * t; ty = dvy * t; tz = dvz * t;
-      }
+      float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f;
+      const float val = JAC ? interp_value_grad(batch.f[u], c, dvx, dvy, dvz) : interp_value(batch.f[u], c);
+      // The t-weighted sums are kept as sums of k grad v, k = the sample's index in this chunk:
+      // t = t0 + k / (S - 1) with t0 the chunk's first t, so T = t0 G + K / (S - 1) (formed
+      // after the march), one multiply-add per component here.
+      const float kf = static_cast<float>(k0 + u);
       // Class routing by branch: a class's sums are updated under the mask of
       // the lanes whose sample it is, so a wave pays for the classes present
       // among its lanes at this sample (mostly one or two), not for all CV.
@@ -860,12 +897,20 @@ void k_fwd(Params p) {
             add_rn(jg[6 * cl + 0], dvx);
             add_rn(jg[6 * cl + 1], dvy);
             add_rn(jg[6 * cl + 2], dvz);
-            add_rn(jg[6 * cl + 3], tx);
-            add_rn(jg[6 * cl + 4], ty);
-            add_rn(jg[6 * cl + 5], tz);
+            jg[6 * cl + 3] = fmaf(kf, dvx, jg[6 * cl + 3]);
+            jg[6 * cl + 4] = fmaf(kf, dvy, jg[6 * cl + 4]);
+            jg[6 * cl + 5] = fmaf(kf, dvz, jg[6 * cl + 5]);
           }
         }
       }
+    }
+  }
+  if constexpr (JAC) {
+    const float t0 = fmaf(static_cast<float>(r.s_lo), p.inv_s, r.t_off);
+#pragma unroll
+    for (int cl = 0; cl < CV; ++cl) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) jg[6 * cl + 3 + a] = fmaf(t0, jg[6 * cl + a], p.inv_s * jg[6 * cl + 3 + a]);
     }
   }
 
@@ -1515,6 +1560,22 @@ __global__ void k_pack_label_quads(const int8_t* __restrict__ lab, uint32_t* __r
   out[r] = w;
 }
 
+// One thread per entry of the padded class map (stencil and coefficient layouts): int8
+// [D + 4][H + 4][W + 4], lab[z][y][x] at (z + 2, y + 2, x + 2), zero elsewhere.
+__global__ void k_pack_label_padded(const int8_t* __restrict__ lab, int8_t* __restrict__ out, int D, int H, int W,
+                                    int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int x = static_cast<int>(i % (W + 4)) - 2;
+  const int64_t rest = i / (W + 4);
+  const int y = static_cast<int>(rest % (H + 4)) - 2;
+  const int z = static_cast<int>(rest / (H + 4)) - 2;
+  const bool inb = static_cast<unsigned>(x) < static_cast<unsigned>(W) &&
+                   static_cast<unsigned>(y) < static_cast<unsigned>(H) &&
+                   static_cast<unsigned>(z) < static_cast<unsigned>(D);
+  out[i] = inb ? lab[(static_cast<int64_t>(z) * H + y) * W + x] : static_cast<int8_t>(0);
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -1550,6 +1611,8 @@ int64_t packed_volume_bytes(int D, int H, int W, int dt, int layout) {
 
 int64_t packed_label_bytes(int D, int H, int W, int layout) {
   if (layout == NDRR_LAYOUT_QUAD) return quad_records(D, H, W) * 4;
+  if (layout == NDRR_LAYOUT_STENCIL || layout == NDRR_LAYOUT_COEF)
+    return static_cast<int64_t>(D + 4) * (H + 4) * (W + 4);
   return static_cast<int64_t>(D) * H * W;
 }
 
@@ -1558,9 +1621,11 @@ bool valid_layout(int layout) {
          layout == NDRR_LAYOUT_COEF;
 }
 
-// The packed layouts' record index is formed with 24-bit multiplies (record_of).
+// The packed layouts' record index is formed with 24-bit multiplies (record_of), and the
+// padded class map's offset with signed ones (padded_label).
 bool packed_index_fits(int D, int H, int W) {
-  return static_cast<int64_t>(D + 1) * (H + 1) < (int64_t{1} << 24) && W + 2 < (1 << 24);
+  return static_cast<int64_t>(D + 1) * (H + 1) < (int64_t{1} << 24) && W + 2 < (1 << 24) &&
+         static_cast<int64_t>(H + 4) * (W + 4) < (int64_t{1} << 23);
 }
 
 bool valid_dtype(int dt) { return dt == NDRR_F32 || dt == NDRR_BF16 || dt == NDRR_F16; }
@@ -2043,9 +2108,16 @@ size_t ndrr_packed_label_bytes(int D, int H, int W, int vol_layout) {
 int ndrr_pack_labels(const int8_t* lab, int vol_layout, int D, int H, int W, void* out, void* stream) {
   if (!lab || !out || D <= 0 || H <= 0 || W <= 0) return NDRR_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (vol_layout == NDRR_LAYOUT_LINEAR || vol_layout == NDRR_LAYOUT_STENCIL || vol_layout == NDRR_LAYOUT_COEF) {
+  if (vol_layout == NDRR_LAYOUT_LINEAR) {
     const hipError_t e = hipMemcpyAsync(out, lab, static_cast<size_t>(D) * H * W, hipMemcpyDeviceToDevice, st);
     return e == hipSuccess ? NDRR_OK : static_cast<int>(e);
+  }
+  if (vol_layout == NDRR_LAYOUT_STENCIL || vol_layout == NDRR_LAYOUT_COEF) {
+    const int64_t n = packed_label_bytes(D, H, W, vol_layout);
+    if (n >= (int64_t{1} << 31) || !packed_index_fits(D, H, W)) return NDRR_ERANGE;
+    hipLaunchKernelGGL(k_pack_label_padded, dim3(static_cast<unsigned>((n + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, st, lab, static_cast<int8_t*>(out), D, H, W, n);
+    return launch_status();
   }
   if (vol_layout != NDRR_LAYOUT_QUAD) return NDRR_EINVAL;
   const int64_t n = quad_records(D, H, W);

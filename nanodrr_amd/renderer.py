@@ -166,8 +166,9 @@ def coef_fits(D: int, H: int, W: int) -> bool:
 
 def packed_index_fits(D: int, H: int, W: int) -> bool:
     """Whether the packed layouts can address this (D, H, W) volume: their record index is formed
-    with 24-bit multiplies, so (D + 1)(H + 1) and W + 2 stay below 2^24."""
-    return (D + 1) * (H + 1) < 2**24 and W + 2 < 2**24
+    with 24-bit multiplies, so (D + 1)(H + 1) and W + 2 stay below 2^24, and the padded label
+    map's with signed ones, so (H + 4)(W + 4) stays below 2^23."""
+    return (D + 1) * (H + 1) < 2**24 and W + 2 < 2**24 and (H + 4) * (W + 4) < 2**23
 
 
 def march_inputs(subject, M, src, tgt, step_size, n_samples, height, width, layout=None):

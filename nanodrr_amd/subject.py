@@ -188,11 +188,11 @@ class Subject(torch.nn.Module):
         return self._derived.get("label_i8", lab, self.n_classes, lambda: labels_to_i8(lab[0, 0], self.n_classes))
 
     def packed_labels(self, layout: str) -> torch.Tensor:
-        """`label_i8` in the form `layout`'s march reads (label quads for "quad", see
-        ndrr_pack_labels; the map itself otherwise)."""
+        """`label_i8` in the form `layout`'s march reads (label quads for "quad", the padded map
+        for "stencil" and "coef", see ndrr_pack_labels; the map itself for "linear")."""
         from ._ops import pack_labels
 
-        if layout != "quad":
+        if layout == "linear":
             return self.label_i8
         lab = self.label_i8
         if torch.compiler.is_compiling():

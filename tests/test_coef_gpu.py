@@ -23,7 +23,7 @@ import numpy as np
 import pytest
 import torch
 
-from nanodrr_amd._ops import (fused_raymarch, fused_raymarch_bwd, fused_raymarch_bwd_jac, fused_raymarch_jac,
+from nanodrr_amd._ops import (fused_raymarch, fused_raymarch_bwd, fused_raymarch_bwd_jac, fused_raymarch_jac, pack_labels,
                               pack_coef, pack_volume)
 from nanodrr_amd.phantom import detector, make_phantom, phantom_subject, sample_poses
 from nanodrr_amd.renderer import _make_src, _make_tgt
@@ -80,7 +80,7 @@ def test_coef_march_matches_oracle_of_its_records(dtype, n_classes):
     packed = pack_coef(vol, dtype)
     vol_t = vol.to(dtype)
     lab = label[0, 0].to(torch.int8).to(DEV)
-    lab_t = lab if n_classes > 1 else lab.new_empty(0)
+    lab_t = pack_labels(lab, "coef") if n_classes > 1 else lab.new_empty(0)
     rec = _records(packed, dtype, vol.shape)
     out = fused_raymarch(vol_t, lab_t, t["M"], t["src"], t["tgt"], t["step"], 200, n_classes, 16, packed, "coef")
     ref = orm.forward(**o, coef=rec)
@@ -104,7 +104,7 @@ def test_fp32_coef_matches_the_corner_oracle():
     volume itself the image is within 1e-5 and the pose gradient within 1e-4."""
     image, label, o, t = _golden_geometry(3)
     vol = image[0, 0].to(DEV)
-    lab_t = label[0, 0].to(torch.int8).to(DEV)
+    lab_t = pack_labels(label[0, 0].to(torch.int8).to(DEV), "coef")
     packed = pack_coef(vol, torch.float32)
     out = fused_raymarch(vol, lab_t, t["M"], t["src"], t["tgt"], t["step"], 200, 3, 16, packed, "coef")
     assert rel_to_max(orm.forward(**o).sum(1), out.double().cpu().numpy().sum(1)) < 1e-5

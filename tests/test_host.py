@@ -69,7 +69,8 @@ def test_packed_sizes():
     assert lib.ndrr_packed_bytes(D, H, W, 2, 2) == (D + 1) * (H + 1) * (W + 2) * 8
     assert lib.ndrr_packed_bytes(D, H, W, 1, 3) == 256 + (D + 1) * (H + 1) * (W + 1) * 16  # header + records
     assert lib.ndrr_packed_bytes(D, H, W, 0, 3) == 256 + (D + 1) * (H + 1) * (W + 1) * 32
-    assert lib.ndrr_packed_label_bytes(D, H, W, 3) == D * H * W  # the int8 map itself
+    assert lib.ndrr_packed_label_bytes(D, H, W, 3) == (D + 4) * (H + 4) * (W + 4)  # the padded map
+    assert lib.ndrr_packed_label_bytes(D, H, W, 1) == (D + 4) * (H + 4) * (W + 4)
     assert lib.ndrr_packed_bytes(D, H, W, 1, 7) == 0
     assert lib.ndrr_packed_label_bytes(D, H, W, 2) == (D + 1) * (H + 1) * (W + 2) * 4
     assert lib.ndrr_packed_label_bytes(D, H, W, 0) == D * H * W
@@ -91,6 +92,8 @@ def test_packed_layouts_need_24_bit_record_index():
     assert _fwd(lib, layout=0, dt=1, D=4096, H=4096, W=2, C=1, vol=None) == -1  # linear: only the null volume
     assert packed_index_fits(512, 512, 512) and packed_index_fits(4094, 4095, 100)
     assert not packed_index_fits(4095, 4096, 2) and not packed_index_fits(1, 1, 2**24 - 2)
+    # the padded label map's offsets use signed 24-bit multiplies: (H + 4)(W + 4) < 2^23
+    assert packed_index_fits(10, 2800, 2800) and not packed_index_fits(10, 2900, 2900)
 
 
 def _fwd(lib, **kw):

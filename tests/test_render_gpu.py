@@ -349,15 +349,20 @@ def test_packed_layouts_hold_the_corners(dtype, volume_layout):
 
 def test_label_quads_hold_the_classes(volume_layout):
     """ndrr_pack_labels: quad record (x, y0, z0) holds lab[z0+a, y0+b, x] in byte 2a + b
-    (zero outside); other layouts read the (D, H, W) map itself."""
+    (zero outside); the stencil and coefficient layouts read the map padded by two zero voxels
+    on every side; the linear layout reads the (D, H, W) map itself."""
     from nanodrr_amd._ops import pack_labels
 
     torch.manual_seed(1)
     D, H, W = 4, 5, 6
     lab = torch.randint(-1, 5, (D, H, W), dtype=torch.int8, device=DEV)
     got = pack_labels(lab, volume_layout)
-    if volume_layout != "quad":
+    if volume_layout == "linear":
         assert torch.equal(got, lab)
+        return
+    if volume_layout in ("stencil", "coef"):
+        expect = torch.nn.functional.pad(lab.to(torch.int32), (2, 2, 2, 2, 2, 2)).to(torch.int8)
+        assert torch.equal(got.view(D + 4, H + 4, W + 4), expect)
         return
     pad = torch.nn.functional.pad(lab.to(torch.int32), (1, 1, 1, 1, 1, 1)).to(torch.int8)
     expect = torch.stack([pad[a:a + D + 1, b:b + H + 1, :] for a in (0, 1) for b in (0, 1)], dim=-1)
