@@ -229,3 +229,64 @@ def test_posenet_proposals_beat_the_nominal_start():
           f"{r['median_mtre_mm_cnn_proposal_unrefined']:.1f} mm, refined {r['median_mtre_mm_cnn_proposal']:.1f} mm")
     assert r["median_mtre_mm_cnn_proposal_unrefined"] < 0.7 * r["median_mtre_mm_nominal_start"]
     assert r["median_mtre_mm_cnn_proposal"] < 0.7 * r["median_mtre_mm_nominal_start"]
+
+
+def _training_shapes():
+    """(N, H, W, Cin, Cout, stride) of every convolution at the training shape: batch 256 of
+    128 x 128 DRRs."""
+    from nanodrr_amd.posenet import LAYERS
+
+    out, hw = [], 128
+    for cin, cout, s in LAYERS:
+        out.append((256, hw, hw, cin, cout, s))
+        hw = (hw - 1) // s + 1
+    return out
+
+
+@pytest.mark.parametrize("layer", range(6))
+def test_every_layer_at_the_training_shape(layer):
+    """Each of the six convolutions at its real size in training (batch 256, 128 x 128 input)
+    against torch fp32 on the CPU on the same bf16 values: forward with ReLU and input
+    gradient with the ReLU mask (8e-3, one bf16 rounding of an fp32 sum), weight and bias
+    gradients (sums of exact bf16 products over 256 H W positions: 1e-4 of the largest
+    entry for the different summation order)."""
+    from nanodrr_amd.posenet import conv3x3, conv3x3_bwd_data, conv3x3_bwd_weight
+
+    N, H, W, Cin, Cout, s = _training_shapes()[layer]
+    x, w, b = _inputs(N, H, W, Cin, Cout, 100 + layer)
+    x = torch.relu(x)  # activations after a ReLU, as every layer but the first sees them
+    Ho, Wo = (H - 1) // s + 1, (W - 1) // s + 1
+    dy = torch.randn(N, Ho, Wo, Cout, generator=torch.Generator().manual_seed(layer)).to(torch.bfloat16)
+    xd, wd, dyd = x.to(DEV), w.to(DEV), dy.to(DEV)
+    y = conv3x3(xd, wd, b.to(DEV), s, True).cpu()
+    dw, db = conv3x3_bwd_weight(dyd, xd, s, bias=True)
+    dx = conv3x3_bwd_data(dyd, wd, xd, s, (H, W)).cpu() if Cin > 1 else None
+    xr = x.float().requires_grad_(True)
+    wr = w.float().requires_grad_(True)
+    br = b.clone().requires_grad_(True)
+    ref = _torch_conv(xr, wr, br, s)
+    ref.backward(dy.float())
+    ref = ref.detach().clamp(min=0)
+    err = (y.float() - ref).abs() / (ref.abs() + 1e-3 * ref.abs().max())
+    assert err.max() < 8e-3, ("forward", err.max())
+    assert (dw.cpu() - wr.grad).abs().max() / wr.grad.abs().max() < 1e-4
+    assert (db.cpu() - br.grad).abs().max() / br.grad.abs().max() < 1e-4
+    if dx is not None:
+        gref = xr.grad * (x.float() > 0)
+        err = (dx.float() - gref).abs() / (gref.abs() + 1e-3 * gref.abs().max())
+        assert err.max() < 8e-3, ("input gradient", err.max())
+
+
+def test_one_training_step_at_full_batch():
+    """Training steps at the training configuration: 256 DRRs rendered on the fly from 8
+    phantoms, the native network forward and backward and the fused Adam update. The loss is
+    finite and every parameter moves; the first step's gradients are finite and nonzero."""
+    from nanodrr_amd.posenet import train
+
+    net, hist, _ = train(steps=2, batch=256, seeds=tuple(range(8)), log=lambda m: None, log_every=1)
+    assert len(hist) == 2 and all(torch.isfinite(torch.tensor(h["loss"])) for h in hist)
+    for p in net.parameters():
+        assert torch.isfinite(p).all()
+    fresh = type(net)(seed=0, pool=8).to(DEV)
+    for (name, p), q in zip(net.named_parameters(), fresh.parameters()):
+        assert not torch.equal(p, q), name
