@@ -9,7 +9,8 @@ are checked against the float64 oracle at the poses the loop has reached:
 the FP64 pose step (oracle/registration.py:pose_compose), the march of those
 rays (oracle/raymarch.py:forward, on the bf16 values the kernels read), the
 ZNCC of the whole DRR (oracle/registration.py:zncc) and the single-pass
-contraction's pose gradient for those rays (oracle/raymarch.py:backward).
+contraction's pose gradient for those of the rays with no sample within
+1e-4 voxel of a cell face (oracle/raymarch.py:backward).
 At the end the converged count and the median mTRE are asserted at the
 values the benchmark reports (57 of 64 under 1 mm, median 0.14 mm).
 """
@@ -61,13 +62,19 @@ def _check_problem(reg, target, vol64, b, rays, log):
     got = zncc(img.reshape(1, -1), target[b:b + 1].reshape(1, -1))[0]
     want = oreg.zncc(_np(img).reshape(1, -1), _np(target[b:b + 1]).reshape(1, -1))[0][0]
     assert abs(float(got) - float(want)) < 1e-6
-    # the contraction's pose gradient of the sampled rays for a random upstream gradient
-    go = torch.randn(1, 1, len(rays), generator=torch.Generator().manual_seed(b)).to(DEV)
+    # the contraction's pose gradient for a random upstream gradient, on the sampled rays with no
+    # sample within 1e-4 voxel of a cell face: there the trilinear derivative jumps, and an fp32
+    # position within an ulp of the float64 one may take the other side's derivative
+    clean = (orm.discontinuity_distance(_np(M), _np(src), _np(tgt), S, vol64.shape, labels=False) > 1e-4).all(axis=0)
+    assert clean.mean() > 0.5
+    keep = torch.from_numpy(clean).to(DEV)
+    tgt, step = tgt[:, keep].contiguous(), step[:, keep].contiguous()
+    go = torch.randn(1, 1, tgt.shape[1], generator=torch.Generator().manual_seed(b)).to(DEV)
     _, jac = fused_raymarch_jac(vol, lab, M, src, tgt, step, S, 1, 0, 6, packed, "quad")
     gM = fused_raymarch_bwd_jac(go, jac, M, src, tgt, step, 0, False)[0]
     g_ref = orm.backward(_np(go), vol64, None, 1, _np(M), _np(src), _np(tgt), _np(step), S, need_gvol=False)["gM"]
     err = rel_to_max(g_ref, _np(gM))
-    log.append((b, float(want), err))
+    log.append((b, float(want), int(clean.sum()), err))
     assert err < 1e-4
 
 
@@ -83,6 +90,9 @@ def test_batched_registration_workload():
     k_inv, sdd = prob.k_inv.float().to(DEV), prob.sdd.float().to(DEV)
     with torch.no_grad():
         target = render(subject, k_inv, prob.rt_true.float().to(DEV), sdd, DET, DET, n_samples=4 * S)
+    # Registration draws its parameters' 1e-8 start (registration.py:55-56) from torch's generator;
+    # seeded as the benchmark seeds it, the trajectory (every kernel is deterministic) is the benchmark's
+    torch.manual_seed(0)
     reg = Registration(subject, prob.rt_init.float().to(DEV), k_inv, sdd, DET, DET, per_pose=True, layout="quad")
     vol64 = subject.image[0, 0].double().cpu().numpy()
     rays = np.random.default_rng(0).choice(DET * DET, 64, replace=False)
@@ -103,7 +113,7 @@ def test_batched_registration_workload():
     after = mtre(reg.pose.detach(), prob.rt_true.to(DEV), fid)
     ok = int((after < 1.0).sum())
     print(f"converged {ok}/{BATCH}, median mTRE {float(before.median()):.2f} -> {float(after.median()):.3f} mm, "
-          f"median ZNCC {float(score.median()):.5f}; oracle checks (problem, zncc, gM error): {log}")
+          f"median ZNCC {float(score.median()):.5f}; oracle checks (problem, zncc, clean rays, gM error): {log}")
     assert seen[0] == 10 and seen[-1] == ITERS and len(log) == 6
     assert ok >= 55
     assert float(after.median()) < 0.3
