@@ -445,6 +445,82 @@ def reference_shape_leg(reps: int = 200):
     return res
 
 
+def cnn_layer_leg(batch: int = 256, reps: int = 20):
+    """Per-layer roofline of the pose-proposal CNN at its training shape (128 x 128 inputs, batch
+    256): each convolution's forward, input gradient (with the ReLU mask of the layer below) and
+    weight + bias gradient, and the pooled MLP head's forward and backward, timed with HIP events
+    over back-to-back launches on the launch stream (queued behind a spin kernel). Algorithmic
+    bytes: every operand read once and every result written once (bf16 activations and weights,
+    fp32 bias and weight gradients); flops: 2 per multiply-add. The bound is the larger of
+    bytes / 8 TB/s and flops / 2.5 PFLOP/s (bf16 dense); frac = bound time / measured time."""
+    from nanodrr_amd import posenet as pn
+
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev).manual_seed(0)
+    stream = torch.cuda.current_stream()
+    MFMA_PEAK = 2.5e15
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda._sleep(SPIN_CYCLES)
+        t0.record(stream)
+        for _ in range(reps):
+            fn()
+        t1.record(stream)
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1) / reps * 1e-3  # seconds
+
+    def row(name, sec, nbytes, flops):
+        t_mem, t_mfma = nbytes / (HBM_PEAK_GBS * 1e9), flops / MFMA_PEAK
+        bound = "hbm" if t_mem >= t_mfma else "mfma"
+        return {"op": name, "us": sec * 1e6, "bytes": nbytes, "flops": flops, "bound": bound,
+                "tb_per_s": nbytes / sec / 1e12, "tflop_per_s": flops / sec / 1e12,
+                "frac": max(t_mem, t_mfma) / sec}
+
+    rows = []
+    H = W = 128
+    x = torch.randn(batch, H, W, 1, device=dev, generator=gen).to(torch.bfloat16)
+    for i, (cin, cout, st) in enumerate(pn.LAYERS):
+        Ho, Wo = (H - 1) // st + 1, (W - 1) // st + 1
+        w = (torch.randn(cout, 3, 3, cin, device=dev, generator=gen) * (2.0 / (9 * cin)) ** 0.5).to(torch.bfloat16)
+        b = torch.zeros(cout, device=dev)
+        x_bytes, y_bytes, w_bytes = x.numel() * 2, batch * Ho * Wo * cout * 2, w.numel() * 2
+        flops = 2.0 * batch * Ho * Wo * cout * 9 * cin
+        y = pn.conv3x3(x, w, b, st, True)
+        dy = torch.randn(y.shape, device=dev, generator=gen).to(torch.bfloat16)
+        tag = f"conv{i + 1} {cin}->{cout}{' /2' if st == 2 else ''} {H}x{W}"
+        rows.append(row(f"{tag} forward", timed(lambda: pn.conv3x3(x, w, b, st, True)),
+                        x_bytes + w_bytes + 4 * cout + y_bytes, flops))
+        if i > 0:  # the input of the first layer needs no gradient
+            rows.append(row(f"{tag} input gradient", timed(lambda: pn.conv3x3_bwd_data(dy, w, x, st, (H, W))),
+                            y_bytes + w_bytes + 2 * x_bytes, flops))
+        rows.append(row(f"{tag} weight gradient", timed(lambda: pn.conv3x3_bwd_weight(dy, x, st, bias=True)),
+                        y_bytes + x_bytes + 4 * (w.numel() + cout), flops))
+        x, H, W = y, Ho, Wo
+    feat = x
+    K = (H // 2) * (W // 2) * feat.shape[-1]
+    w1 = torch.randn(pn.HIDDEN, K, device=dev, generator=gen) / K ** 0.5
+    b1 = torch.zeros(pn.HIDDEN, device=dev)
+    w2 = torch.randn(pn.OUTPUTS, pn.HIDDEN, device=dev, generator=gen) * 0.01
+    b2 = torch.zeros(pn.OUTPUTS, device=dev)
+    out, saved = pn.head_forward(feat, w1, b1, w2, b2)
+    dout = torch.randn(out.shape, device=dev, generator=gen)
+    head_flops = 2.0 * batch * K * pn.HIDDEN
+    rows.append(row(f"head pool {H}x{W}x{feat.shape[-1]} + {K}-{pn.HIDDEN}-{pn.OUTPUTS} forward",
+                    timed(lambda: pn.head_forward(feat, w1, b1, w2, b2)),
+                    feat.numel() * 2 + w1.numel() * 4 + batch * pn.OUTPUTS * 4, head_flops))
+    rows.append(row(f"head pool {H}x{W}x{feat.shape[-1]} + {K}-{pn.HIDDEN}-{pn.OUTPUTS} backward",
+                    timed(lambda: pn.head_backward(dout, feat, w2, saved)),
+                    2 * feat.numel() * 2 + saved[1].numel() * 2 + saved[2].numel() * 2 + w1.numel() * 4,
+                    2 * head_flops))
+    total = sum(r["us"] for r in rows)
+    return {"workload": f"pose-proposal CNN, batch {batch}, 128x128 bf16 inputs, one training step's kernels "
+                        f"(weights transposed per input-gradient call; Adam and the DRR renders excluded)",
+            "layers": rows, "total_us": total}
+
+
 def counter_probe(case: str, reps: int = 4):
     """Launch the fwd and bwd ops of one case `reps` times, then copy a 256 MiB
     device buffer three times: the run that rocprofv3 --pmc passes profile (the
@@ -576,6 +652,8 @@ def main():
                     help="run only the counter probe of one case, e.g. bf16:coef:single")
     ap.add_argument("--probe-reps", type=int, default=4, help="launches per op in the counter probe")
     ap.add_argument("--no-registration", action="store_true", help="skip the batched registration leg")
+    ap.add_argument("--cnn-layers", action="store_true",
+                    help="run only the per-layer roofline of the pose-proposal CNN at batch 256")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--registration-only", action="store_true", help="run only the registration leg")
     ap.add_argument("--reference-shape", action="store_true",
@@ -588,6 +666,9 @@ def main():
         sys.exit(2)
     if args.probe:
         counter_probe(args.probe, args.probe_reps)
+        return
+    if args.cnn_layers:
+        print(json.dumps(cnn_layer_leg()), flush=True)
         return
     if args.registration_only:
         print(json.dumps(registration_leg()), flush=True)

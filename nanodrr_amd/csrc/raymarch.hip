@@ -409,12 +409,14 @@ struct Corners {
 // a cell outside the packed grid gets an offset past the end of the buffer,
 // for which the hardware returns zeros (no clamping, no 64-bit address math).
 constexpr uint32_t kBufferFlags = 0x00020000u;  // raw buffer, 32-bit data (gfx9 family)
+constexpr uint32_t kRintBias = 0x4B400000u;     // the bits of 1.5 * 2^23 (rint_bits)
 
 struct Src {
   const void* vol;
   const int8_t* lab;
   __amdgpu_buffer_rsrc_t vrs, lrs;
   int D, H, W;
+  uint32_t lab_base;  // padded class map: the border offset less the 2^22 terms of padded_label
 };
 
 __device__ inline Src make_src(const Params& p) {
@@ -427,6 +429,8 @@ __device__ inline Src make_src(const Params& p) {
   s.lrs = __builtin_amdgcn_make_buffer_rsrc(const_cast<int8_t*>(p.lab), 0, static_cast<int>(p.lab_bytes),
                                             static_cast<int>(kBufferFlags));
   s.D = p.D; s.H = p.H; s.W = p.W;
+  const uint32_t w4 = static_cast<uint32_t>(p.W + 4), hw4 = static_cast<uint32_t>(p.H + 4) * w4;
+  s.lab_base = (2u * hw4 + 2u * w4 + 2u) - (1u << 22) * (hw4 + w4) - kRintBias;
   return s;
 }
 
@@ -496,22 +500,29 @@ __device__ inline bool cell_exists(const Src& s, const Cell& c) {
          (static_cast<unsigned>(c.iz + 1) <= static_cast<unsigned>(s.D));
 }
 
-// Index of a cell's record in a grid of (D + 1) x (H + 1) x row records, with
-// 24-bit multiplies (full rate; a 32-bit integer multiply is a quarter-rate
-// instruction). check_common keeps (D + 1)(H + 1) and the row below 2^24 for
-// the packed layouts, so the operands of an existing cell fit; a cell that
-// does not exist gets an offset past the buffer whatever this returns.
-__device__ inline uint32_t mul_u24(uint32_t a, uint32_t b) {
-  // the masks state the operands' width, so this is one full-rate 24-bit multiply (__umul24
-  // came out as a quarter-rate 32-bit one in these expressions)
-  return (a & 0xffffffu) * (b & 0xffffffu);
+// a[23:0] * b[23:0] + c, unsigned and signed (the operands' low 24 bits, sign-extended for
+// the signed form): one full-rate multiply-add each. Written as instructions because the
+// compiler turns the 24-bit products of these index expressions into quarter-rate 32-bit
+// multiplies or 64-bit multiply-adds once it has distributed the +1 offsets over them.
+__device__ inline uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t d;
+  asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
+  return d;
+}
+__device__ inline int mad_i24(int a, int b, int c) {
+  int d;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
+  return d;
 }
 
+// Index of a cell's record in a grid of (D + 1) x (H + 1) x row records.
+// check_common keeps (D + 1)(H + 1) and the row below 2^24 for the packed
+// layouts, so the operands of an existing cell fit; a cell that does not exist
+// gets an offset past the buffer whatever this returns.
 __device__ inline uint32_t record_of(const Src& s, const Cell& c, int row) {
-  return mul_u24(mul_u24(static_cast<uint32_t>(c.iz + 1), static_cast<uint32_t>(s.H + 1)) +
-                     static_cast<uint32_t>(c.iy + 1),
-                 static_cast<uint32_t>(row)) +
-         static_cast<uint32_t>(c.ix + 1);
+  return mad_u24(mad_u24(static_cast<uint32_t>(c.iz + 1), static_cast<uint32_t>(s.H + 1),
+                         static_cast<uint32_t>(c.iy + 1)),
+                 static_cast<uint32_t>(row), static_cast<uint32_t>(c.ix + 1));
 }
 
 // Byte offset of a cell's record, or one past 2 GiB (so past the end of the
@@ -774,22 +785,30 @@ __device__ inline int quad_label(const Src& s, const Cell& c, float x, float y, 
   return static_cast<int8_t>(static_cast<uint8_t>(word >> shift));
 }
 
+// rint(x) (round half to even, the default rounding mode) for |x| < 2^22, in the low bits:
+// x + 1.5 * 2^23 lies in [2^23, 2^24), where the spacing of floats is 1, so the sum is x
+// rounded to an integer, whose bits are kRintBias + rint(x): bits 0..23 hold 2^22 + rint(x).
+__device__ inline int rint_bits(float x) { return __float_as_int(x + 12582912.0f); }
+
 // Nearest label from the padded class map of the stencil and coefficient
 // layouts: int8 [D + 4][H + 4][W + 4] holding the class of voxel (z, y, x) at
 // (z + 2, y + 2, x + 2) and class 0 around it. The nearest voxel is rint(p)
-// (round half to even, grid_sample's nearest mode) of the position clamped to
-// [-2, dim + 1]: a nearest voxel outside the volume lands in the border, class
-// 0 as the reference gives it, so no bounds test is needed. Signed 24-bit
-// multiply-adds form the offset ((H + 4)(W + 4) < 2^23, checked on the host);
-// a sample that is not live reads past the buffer (class 0).
-__device__ inline int padded_label(const Src& s, float x, float y, float z, bool live) {
-  const int jx = static_cast<int>(rintf(__builtin_amdgcn_fmed3f(x, -2.0f, static_cast<float>(s.W + 1))));
-  const int jy = static_cast<int>(rintf(__builtin_amdgcn_fmed3f(y, -2.0f, static_cast<float>(s.H + 1))));
-  const int jz = static_cast<int>(rintf(__builtin_amdgcn_fmed3f(z, -2.0f, static_cast<float>(s.D + 1))));
+// (round half to even, grid_sample's nearest mode). For a sample whose cell
+// exists it is a corner of that cell, at most one voxel outside the volume, so
+// it lands in the border (class 0, as the reference gives it): no bounds test.
+// A sample whose cell does not exist, or that is not live, fetched a zero
+// record (record_offset), so the class it is routed to receives exact zeros;
+// its offset may point anywhere, and buffer loads past the end read zeros.
+// The offset comes from rint_bits by signed 24-bit multiply-adds
+// ((H + 4)(W + 4) < 2^23, checked on the host), which read the 24-bit fields
+// 2^22 + rint(p) of the factors and the whole addend (kRintBias + rint(x)):
+// s.lab_base removes the 2^22 (hw4 + w4) + kRintBias this adds and adds the
+// border (both modulo 2^32).
+__device__ inline int padded_label(const Src& s, float x, float y, float z) {
   const int w4 = s.W + 4, hw4 = (s.H + 4) * w4;
-  const int off = __mul24(jz, hw4) + __mul24(jy, w4) + jx + (2 * hw4 + 2 * w4 + 2);
-  const uint32_t o = static_cast<uint32_t>(off) | (static_cast<uint32_t>(!live) << 31);
-  return static_cast<int8_t>(__builtin_amdgcn_raw_buffer_load_b8(s.lrs, o, 0, 0));
+  const int off = mad_i24(rint_bits(z), hw4, mad_i24(rint_bits(y), w4, rint_bits(x)));
+  return static_cast<int8_t>(
+      __builtin_amdgcn_raw_buffer_load_b8(s.lrs, static_cast<uint32_t>(off) + s.lab_base, 0, 0));
 }
 
 // a += x rounded once (never contracted with the product that formed x).
@@ -831,7 +850,7 @@ struct Batch {
         else if constexpr (L == NDRR_LAYOUT_LINEAR)
           cls[u] = nearest_label(s.lab, s.D, s.H, s.W, x, y, z, live);
         else
-          cls[u] = padded_label(s, x, y, z, live);
+          cls[u] = padded_label(s, x, y, z);
       }
     }
   }
