@@ -164,6 +164,106 @@ __global__ __launch_bounds__(kThreads) void k_conv_fwd(ConvP p) {
   }
 }
 
+// Forward of the one-channel first layer at stride 1 when every output row holds whole
+// 32-pixel groups (Wo % 32 == 0). A group is 32 consecutive pixels of one row, so its taps
+// are the 3 x 34 input values around it: the wave loads them once, coalesced (lane c reads
+// column ox0 - 1 + c of each of the three rows, zero outside the image), into its own LDS
+// strip, and each lane reads its operand's taps from there. The wave walks its groups along
+// the rows, advancing the row and image counters instead of dividing per pixel. Same
+// operands, products and epilogue as k_conv_fwd: bitwise the same output.
+constexpr int kStrip = 40;  // 34 columns used
+
+// LDS written by some lanes of a wave and then read by others: keep the compiler from moving
+// the accesses across this point and wait for the wave's LDS operations.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int COG>
+__global__ __launch_bounds__(kThreads) void k_conv_fwd_rows(ConvP p) {
+  __shared__ __bf16 strip[4][3][kStrip];
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int co0 = blockIdx.y * 16 * COG;
+  const int kq = 8 * (l / 16);
+  bf16x8 b[COG];
+#pragma unroll
+  for (int c = 0; c < COG; ++c) {
+    const int co = co0 + pair_row(l, c);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int tap = kq + j;
+      b[c][j] = tap < 9 ? p.w[co * 9 + tap] : static_cast<__bf16>(0.0f);
+    }
+  }
+  float bv[COG / 2][8];
+#pragma unroll
+  for (int h = 0; h < COG / 2; ++h)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) bv[h][j] = p.bias ? p.bias[co0 + 32 * h + kq + j] : 0.0f;
+  const int segs = p.Wo / 32, rows = p.N * p.Ho;
+  const int g0 = (blockIdx.x * 4 + wave) * kCin1Groups;  // this wave's first group
+  int row = g0 / segs, seg = g0 - (g0 / segs) * segs;
+  int n = row / p.Ho, oy = row - (row / p.Ho) * p.Ho;
+  for (int it = 0; it < kCin1Groups && row < rows; ++it) {
+    const int ox0 = 32 * seg;
+    if (l < 34) {
+      const int ix = ox0 - 1 + l;
+      const bool inx = ix >= 0 && ix < p.W;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const int iy = oy - 1 + r;
+        strip[wave][r][l] = (inx && iy >= 0 && iy < p.H) ? p.x[(static_cast<int64_t>(n) * p.H + iy) * p.W + ix]
+                                                         : static_cast<__bf16>(0.0f);
+      }
+    }
+    wave_lds_sync();
+    bf16x8 a[2];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int q = 16 * g + (l & 15);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int tap = kq + j;
+        a[g][j] = tap < 9 ? strip[wave][tap / 3][q + tap % 3] : static_cast<__bf16>(0.0f);
+      }
+    }
+    wave_lds_sync();  // every lane has read the strip before the next group rewrites it
+    f32x4 acc[2][COG];
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int c = 0; c < COG; ++c) acc[g][c] = mfma(b[c], a[g], f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+    const int64_t m0 = static_cast<int64_t>(row) * p.Wo + ox0;
+#pragma unroll
+    for (int h = 0; h < COG / 2; ++h) {
+      const int cb = co0 + 32 * h + kq;
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        const int64_t m = m0 + 16 * g + (l & 15);
+        bf16x8 v;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          float u = acc[g][2 * h + j / 4][j % 4] + bv[h][j];
+          if (p.relu) u = fmaxf(u, 0.0f);
+          v[j] = static_cast<__bf16>(u);
+        }
+        *reinterpret_cast<bf16x8*>(p.y + m * p.Cout + cb) = v;
+      }
+    }
+    if (++seg == segs) {
+      seg = 0;
+      ++row;
+      if (++oy == p.Ho) {
+        oy = 0;
+        ++n;
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Tiled implicit GEMM: the forward (MODE 0) and the input gradient (MODE 1)
 // of layers with Cin a multiple of 32. Block: four waves as WM x (4 / WM);
@@ -324,6 +424,110 @@ __global__ __launch_bounds__(kThreads) void k_igemm(ConvP p) {
         }
         *reinterpret_cast<bf16x8*>(p.dx + at + cb) = v;
       }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Input gradient at stride 2 from a staged dy tile. A workgroup covers an input tile of
+// 2 kDgTY x 2 kDgTX pixels of one image and BN = 16 CG input channels: the four parity
+// classes (py, px) of that tile, kDgTY x kDgTX pixels each, one class per wave. Every tap
+// of class (py, px) reads dy at (cy + (py + 1 - ky) / 2, cx + (px + 1 - kx) / 2) for class
+// pixel (cy, cx), so the whole workgroup needs the (kDgTY + 1) x (kDgTX + 1) dy pixels from
+// the tile's corner: they are staged in LDS once (all Cout channels, zero outside dy), in
+// rows of Cout + 8 channels so that the 16-byte reads of 16 consecutive pixels hit distinct
+// banks, and each wave then runs its class's taps (1, 2, 2 or 4) x Cout / 32 K steps from
+// there, with the weights read from L2. The K order (taps row-major, then channel chunks),
+// the products and the epilogue are k_igemm<1>'s: bitwise the same dx, without re-reading
+// dy per tap and class.
+// ---------------------------------------------------------------------------
+constexpr int kDgTY = 4, kDgTX = 16;
+constexpr int kDgTile = (kDgTY + 1) * (kDgTX + 1);  // dy pixels staged per workgroup
+
+template <int CG>
+__global__ __launch_bounds__(kThreads) void k_dgrad_s2(ConvP p) {
+  extern __shared__ __attribute__((aligned(16))) __bf16 sdy[];  // [kDgTile][Cout + 8]
+  constexpr int TY = kDgTY, TILE = kDgTile;
+  const int t = threadIdx.x, wave = t >> 6, l = t & 63;
+  const int row = p.Cout + 8;
+  const int ch0 = blockIdx.y * 16 * CG;
+  const int Hc = (p.H + 1) / 2, Wc = (p.W + 1) / 2;  // class rows / columns (class 0's, the largest)
+  const int tiles_x = (Wc + kDgTX - 1) / kDgTX, tiles_y = (Hc + TY - 1) / TY;
+  const int n = blockIdx.x / (tiles_x * tiles_y);
+  const int rest = blockIdx.x - n * tiles_x * tiles_y;
+  const int cy0 = (rest / tiles_x) * TY, cx0 = (rest % tiles_x) * kDgTX;
+  // stage dy[n][cy0 .. cy0 + kDgTY][cx0 .. cx0 + kDgTX][:] (16-byte pieces, zero outside)
+  const int pieces = p.Cout / 8;
+  for (int e = t; e < TILE * pieces; e += kThreads) {
+    const int pi = e / pieces, c = e - pi * pieces;
+    const int oy = cy0 + pi / (kDgTX + 1), ox = cx0 + pi % (kDgTX + 1);
+    const bf16x8 v = (oy < p.Ho && ox < p.Wo)
+                         ? load8(p.dy + ((static_cast<int64_t>(n) * p.Ho + oy) * p.Wo + ox) * p.Cout + 8 * c)
+                         : zero8();
+    *reinterpret_cast<bf16x8*>(&sdy[pi * row + 8 * c]) = v;
+  }
+  const int py = wave >> 1, px = wave & 1;
+  const int ky0 = py == 0 ? 1 : 0, kx0 = px == 0 ? 1 : 0;
+  const int nky = py == 0 ? 1 : 2, nkx = px == 0 ? 1 : 2;
+  const int kq = 8 * (l >> 4), tx = l & 15;
+  // The ReLU mask of the 32-channel layer is loaded before the barrier and the products (its
+  // latency hidden behind them); wider layers load it in the epilogue (registers).
+  constexpr bool kEarly = CG == 2;
+  bf16x8 xm_early[TY];
+  if constexpr (kEarly) {
+#pragma unroll
+    for (int g = 0; g < TY; ++g) {
+      const int iy = 2 * (cy0 + g) + py, ix = 2 * (cx0 + tx) + px;
+      xm_early[g] = (p.mask && iy < p.H && ix < p.W)
+                        ? load8(p.x + ((static_cast<int64_t>(n) * p.H + iy) * p.W + ix) * p.Cin + ch0 + kq)
+                        : zero8();
+    }
+  }
+  __syncthreads();
+  f32x4 acc[TY][CG];
+#pragma unroll
+  for (int g = 0; g < TY; ++g)
+#pragma unroll
+    for (int c = 0; c < CG; ++c) acc[g][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int a = 0; a < nky; ++a) {
+    const int ky = ky0 + 2 * a, dyo = (py + 1 - ky) / 2;
+    for (int b = 0; b < nkx; ++b) {
+      const int kx = kx0 + 2 * b, dxo = (px + 1 - kx) / 2;
+      const int tap = 3 * ky + kx;
+      for (int kc = 0; kc < p.Cout; kc += 32) {
+        bf16x8 bx[TY], ax[CG];
+#pragma unroll
+        for (int c = 0; c < CG; ++c)
+          ax[c] = load8(p.wt + (static_cast<int64_t>(tap) * p.Cin + ch0 + pair_row(l, c)) * p.Cout + kc + kq);
+#pragma unroll
+        for (int g = 0; g < TY; ++g)
+          bx[g] = *reinterpret_cast<const bf16x8*>(&sdy[((g + dyo) * (kDgTX + 1) + tx + dxo) * row + kc + kq]);
+#pragma unroll
+        for (int g = 0; g < TY; ++g)
+#pragma unroll
+          for (int c = 0; c < CG; ++c) acc[g][c] = mfma(ax[c], bx[g], acc[g][c]);
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < TY; ++g) {
+    const int iy = 2 * (cy0 + g) + py, ix = 2 * (cx0 + tx) + px;
+    if (iy >= p.H || ix >= p.W) continue;
+    const int64_t at = ((static_cast<int64_t>(n) * p.H + iy) * p.W + ix) * p.Cin;
+#pragma unroll
+    for (int h = 0; h < CG / 2; ++h) {
+      const int cb = ch0 + 32 * h + kq;
+      bf16x8 xm;
+      if constexpr (kEarly) xm = xm_early[g];
+      else xm = p.mask ? load8(p.x + at + cb) : zero8();
+      bf16x8 v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        float u = acc[g][2 * h + j / 4][j % 4];
+        if (p.mask && !(static_cast<float>(xm[j]) > 0.0f)) u = 0.0f;
+        v[j] = static_cast<__bf16>(u);
+      }
+      *reinterpret_cast<bf16x8*>(p.dx + at + cb) = v;
     }
   }
 }
@@ -576,35 +780,146 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd_weight_cin1(ConvP p) {
   }
 }
 
-// dw[co][tap][ci] = sum over splits of partial[split][tap][co][ci] (and
-// db[co] of partial[split][9 Cout Cin + co]). A block sums 64 groups of four
-// consecutive elements: wave w adds splits w, w + 4, ... in order with 16-byte
-// loads (1 KB per wave and split), then the four wave sums are added in wave
-// order. Deterministic.
+// The same gradient at stride 1 when every output row holds whole 32-pixel groups
+// (Wo % 32 == 0; splits are whole 128-pixel steps, so a group never straddles a split or a
+// row). Each wave works on its own until the final sum: per group of 32 pixels it transposes
+// the dy slab into LDS as above and stages the group's 3 x 34 input values as k_conv_fwd_rows
+// does (lane c reads column ox0 - 1 + c of each row, coalesced), reading each tap's eight
+// consecutive pixels from that strip; the next group's global loads are issued before the
+// current group's LDS work and products. Same operands, products and order of summation as
+// k_conv_bwd_weight_cin1: bitwise the same partials.
+__global__ __launch_bounds__(kThreads) void k_conv_bwd_weight_cin1_rows(ConvP p) {
+  __shared__ __attribute__((aligned(16))) __bf16 sdy[4][32][kLdsRow];  // [wave][co][pixel]
+  __shared__ __bf16 strip[4][3][kStrip];
+  __shared__ float red[4][32][16];
+  __shared__ float bred[4][32];
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int split = blockIdx.x;
+  const int M = p.N * p.Ho * p.Wo;
+  const int p_begin = split * p.split, p_end = min(M, p_begin + p.split);
+  const int kq = 8 * (l / 16);
+  f32x4 acc[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
+  float db = 0.0f;  // lanes 0-31: this wave's sum of dy for channel l
+  bf16x8 dv[2];
+  __bf16 xv[3];
+  // group pb's loads: two 8-channel dy pieces per lane, and (lanes < 34) the strip column
+  auto load = [&](int pb) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int idx = l + 64 * i, px = idx / 4, c8 = 8 * (idx % 4);
+      dv[i] = load8(p.dy + static_cast<int64_t>(pb + px) * p.Cout + c8);
+    }
+    const int row = pb / p.Wo, ox0 = pb - row * p.Wo;
+    const int n = row / p.Ho, oy = row - n * p.Ho;
+    const int ix = ox0 - 1 + l;
+    const bool inx = l < 34 && ix >= 0 && ix < p.W;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int iy = oy - 1 + r;
+      xv[r] = (inx && iy >= 0 && iy < p.H) ? p.x[(static_cast<int64_t>(n) * p.H + iy) * p.W + ix]
+                                           : static_cast<__bf16>(0.0f);
+    }
+  };
+  int pb = p_begin + 32 * wave;
+  if (pb < p_end) load(pb);
+  for (; pb < p_end; pb += 4 * 32) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int idx = l + 64 * i, px = idx / 4, c8 = 8 * (idx % 4);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) sdy[wave][c8 + j][px] = dv[i][j];
+    }
+    if (l < 34) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) strip[wave][r][l] = xv[r];
+    }
+    if (pb + 4 * 32 < p_end) load(pb + 4 * 32);  // the next group's loads, in flight meanwhile
+    wave_lds_sync();
+    if (p.bias_grad && l < 32) {
+#pragma unroll
+      for (int q = 0; q < 32; q += 8) {
+        const bf16x8 v = *reinterpret_cast<const bf16x8*>(&sdy[wave][l][q]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) db += static_cast<float>(v[j]);
+      }
+    }
+    bf16x8 b;
+    {
+      const int tap = l & 15;  // B column; its K rows are pixels kq .. kq + 7
+      const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) b[j] = tap < 9 ? strip[wave][ky][kq + j + kx] : static_cast<__bf16>(0.0f);
+    }
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sdy[wave][16 * g + (l % 16)][kq]);
+      acc[g] = mfma(a, b, acc[g]);
+    }
+    wave_lds_sync();  // every lane has read this group's LDS before the next group rewrites it
+  }
+#pragma unroll
+  for (int g = 0; g < 2; ++g)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) red[wave][16 * g + 4 * (l / 16) + r][l % 16] = acc[g][r];
+  if (l < 32) bred[wave][l] = db;
+  __syncthreads();
+  float* part = p.part + static_cast<int64_t>(split) * (9 * 32 + 32);
+  for (int e = threadIdx.x; e < 32 * 9; e += kThreads) {
+    const int co = e % 32, tap = e / 32;
+    part[tap * 32 + co] = ((red[0][co][tap] + red[1][co][tap]) + red[2][co][tap]) + red[3][co][tap];
+  }
+  if (p.bias_grad && threadIdx.x < 32) {
+    const int co = threadIdx.x;
+    part[9 * 32 + co] = ((bred[0][co] + bred[1][co]) + bred[2][co]) + bred[3][co];
+  }
+}
+
+
+// dw, db = the sums of the split partials, in a fixed order. A workgroup takes kSumElems
+// consecutive float4 elements; its thread (q, e) adds the partials of splits q, q + G, q + 2G,
+// ... (G = kSumGroups) of element e in that order, eight loads in flight at a time, and the G
+// group sums meet in a fixed pairwise tree in LDS. Deterministic.
+constexpr int kSumElems = 16;
+constexpr int kSumGroups = kThreads / kSumElems;
 __global__ __launch_bounds__(kThreads) void k_sum_splits(const float* __restrict__ part, float* __restrict__ dw,
                                                           float* __restrict__ db, int splits, int Cout, int Cin,
                                                           int64_t count) {
-  __shared__ f32x4 red[4][64];
+  __shared__ f32x4 red[kSumGroups][kSumElems];
   const int64_t stride = static_cast<int64_t>(9) * Cout * Cin + Cout;
-  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int64_t e = (static_cast<int64_t>(blockIdx.x) * 64 + l) * 4;
+  const int el = threadIdx.x % kSumElems, q = threadIdx.x / kSumElems;
+  const int64_t e = (static_cast<int64_t>(blockIdx.x) * kSumElems + el) * 4;
   f32x4 s = {0.0f, 0.0f, 0.0f, 0.0f};
-  if (e < count)
-    for (int k = wave; k < splits; k += 4) s += *reinterpret_cast<const f32x4*>(part + k * stride + e);
-  red[wave][l] = s;
+  if (e < count) {
+    const float* src = part + e;
+    int k = q;
+    for (; k + 7 * kSumGroups < splits; k += 8 * kSumGroups) {
+      f32x4 a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = *reinterpret_cast<const f32x4*>(src + (k + u * kSumGroups) * stride);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += a[u];
+    }
+    for (; k < splits; k += kSumGroups) s += *reinterpret_cast<const f32x4*>(src + k * stride);
+  }
+  red[q][el] = s;
   __syncthreads();
-  if (wave != 0 || e >= count) return;
-  s = ((red[0][l] + red[1][l]) + red[2][l]) + red[3][l];
+#pragma unroll
+  for (int w = kSumGroups / 2; w > 0; w >>= 1) {
+    if (q < w) red[q][el] += red[q + w][el];
+    __syncthreads();
+  }
+  if (q != 0 || e >= count) return;
+  s = red[0][el];
   const int64_t nw = static_cast<int64_t>(9) * Cout * Cin;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const int64_t q = e + j;
-    if (q >= nw) {
-      db[q - nw] = s[j];
+    const int64_t qq = e + j;
+    if (qq >= nw) {
+      db[qq - nw] = s[j];
     } else {
-      const int ci = static_cast<int>(q % Cin);
-      const int co = static_cast<int>((q / Cin) % Cout);
-      const int tap = static_cast<int>(q / (static_cast<int64_t>(Cin) * Cout));
+      const int ci = static_cast<int>(qq % Cin);
+      const int co = static_cast<int>((qq / Cin) % Cout);
+      const int tap = static_cast<int>(qq / (static_cast<int64_t>(Cin) * Cout));
       dw[(static_cast<int64_t>(co) * 9 + tap) * Cin + ci] = s[j];
     }
   }
@@ -680,8 +995,14 @@ int ndrr_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y, i
     const int blocks = (M + per_block - 1) / per_block;
     const bool wide = Cout % 64 == 0;
     const dim3 grid(blocks, Cout / (wide ? 64 : 32));
-    if (wide) hipLaunchKernelGGL((k_conv_fwd<4>), grid, dim3(kThreads), 0, st, p);
-    else hipLaunchKernelGGL((k_conv_fwd<2>), grid, dim3(kThreads), 0, st, p);
+    if (stride == 1 && Wo % 32 == 0) {  // whole 32-pixel groups per row: the staged-strip kernel
+      if (wide) hipLaunchKernelGGL((k_conv_fwd_rows<4>), grid, dim3(kThreads), 0, st, p);
+      else hipLaunchKernelGGL((k_conv_fwd_rows<2>), grid, dim3(kThreads), 0, st, p);
+    } else if (wide) {
+      hipLaunchKernelGGL((k_conv_fwd<4>), grid, dim3(kThreads), 0, st, p);
+    } else {
+      hipLaunchKernelGGL((k_conv_fwd<2>), grid, dim3(kThreads), 0, st, p);
+    }
   } else {
     launch_igemm<0>(p, M, Cout, 1, st);
   }
@@ -709,9 +1030,23 @@ int ndrr_conv3x3_bwd_data(const void* dy, const void* wt, const void* x, void* d
   p.dx = static_cast<__bf16*>(dx);
   p.N = N; p.H = H; p.W = W; p.Cin = Cin; p.Ho = Ho; p.Wo = Wo; p.Cout = Cout; p.S = stride;
   p.mask = relu_mask != 0;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // the staged-tile kernel (dy read once per workgroup) up to 128 output channels; above, the
+  // tiled implicit GEMM's class-per-workgroup split balances the heavier products better
+  if (stride == 2 && Cout <= 128) {
+    const int Hc = (H + 1) / 2, Wc = (W + 1) / 2;
+    const int64_t tiles = static_cast<int64_t>(N) * ((Hc + kDgTY - 1) / kDgTY) * ((Wc + kDgTX - 1) / kDgTX);
+    if (tiles > 0x7fffffff) return NDRR_ERANGE;
+    const size_t lds = static_cast<size_t>(kDgTile) * (Cout + 8) * sizeof(__bf16);
+    if (Cin % 64 == 0)
+      hipLaunchKernelGGL((k_dgrad_s2<4>), dim3(static_cast<unsigned>(tiles), Cin / 64), dim3(kThreads), lds, st, p);
+    else
+      hipLaunchKernelGGL((k_dgrad_s2<2>), dim3(static_cast<unsigned>(tiles), Cin / 32), dim3(kThreads), lds, st, p);
+    return status();
+  }
   // pixels per class: all of them (stride 1), or the largest parity class (stride 2)
   const int M = stride == 2 ? N * ((H + 1) / 2) * ((W + 1) / 2) : N * H * W;
-  launch_igemm<1>(p, M, Cin, stride == 2 ? 4 : 1, static_cast<hipStream_t>(stream));
+  launch_igemm<1>(p, M, Cin, stride == 2 ? 4 : 1, st);
   return status();
 }
 
@@ -740,7 +1075,10 @@ int ndrr_conv3x3_bwd_weight(const void* dy, const void* x, float* workspace, flo
   const int splits = (M + p.split - 1) / p.split;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (Cin == 1) {
-    hipLaunchKernelGGL(k_conv_bwd_weight_cin1, dim3(splits), dim3(kThreads), 0, st, p);
+    if (stride == 1 && Wo % 32 == 0)  // whole 32-pixel groups per row: the staged-strip kernel
+      hipLaunchKernelGGL(k_conv_bwd_weight_cin1_rows, dim3(splits), dim3(kThreads), 0, st, p);
+    else
+      hipLaunchKernelGGL(k_conv_bwd_weight_cin1, dim3(splits), dim3(kThreads), 0, st, p);
   } else {
     const dim3 grid(splits, Cin / wgrad_ci(Cin), Cout / 64);
     if (wgrad_ci(Cin) == 64) hipLaunchKernelGGL((k_conv_bwd_weight<4, NDRR_WGRAD_BUFFERS>), grid, dim3(kThreads), 0, st, p);
@@ -749,8 +1087,9 @@ int ndrr_conv3x3_bwd_weight(const void* dy, const void* x, float* workspace, flo
   int r = status();
   if (r != NDRR_OK) return r;
   const int64_t count = static_cast<int64_t>(9) * Cout * Cin + (dbias ? Cout : 0);
-  hipLaunchKernelGGL(k_sum_splits, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(kThreads), 0, st,
-                     workspace, dw, dbias, splits, Cout, Cin, count);
+  const int64_t groups4 = (count + 4 * kSumElems - 1) / (4 * kSumElems);
+  hipLaunchKernelGGL(k_sum_splits, dim3(static_cast<unsigned>(groups4)), dim3(kThreads), 0, st, workspace, dw, dbias,
+                     splits, Cout, Cin, count);
   return status();
 }
 

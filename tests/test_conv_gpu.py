@@ -14,6 +14,8 @@ DEV = "cuda:0"
 
 SHAPES = [  # N, H, W, Cin, Cout, stride
     (2, 13, 11, 1, 32, 1),
+    (3, 10, 64, 1, 32, 1),  # one-channel layer in whole 32-pixel row groups, a wave's groups crossing images
+    (2, 9, 32, 1, 64, 1),
     (2, 16, 16, 32, 64, 2),
     (3, 9, 10, 64, 64, 1),
     (2, 17, 15, 64, 128, 2),
@@ -79,6 +81,8 @@ def test_conv_weight_gradient_matches_torch(shape):
     N, H, W, Cin, Cout, s = shape
     if Cin != 1 and Cout % 64:
         pytest.skip("the weight-gradient kernel takes 64-channel output tiles")
+    if Cin == 1 and Cout != 32:
+        pytest.skip("the one-channel weight-gradient kernel takes 32 output channels (the first layer's)")
     x, w, b = _inputs(N, H, W, Cin, Cout, 11 + sum(shape))
     Ho, Wo = (H - 1) // s + 1, (W - 1) // s + 1
     dy = torch.randn(N, Ho, Wo, Cout, generator=torch.Generator().manual_seed(5)).to(torch.bfloat16)
