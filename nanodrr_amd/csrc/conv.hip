@@ -532,6 +532,124 @@ __global__ __launch_bounds__(kThreads) void k_dgrad_s2(ConvP p) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Stride-1 input gradient from a staged dy halo, for layers with up to 64 output channels (the
+// forward and wider layers measured faster on k_igemm, whose weights staged in LDS serve four
+// waves at once). A workgroup covers a kHaloT x kHaloT tile of input-gradient pixels of one
+// image and BN = 16 CG input channels; wave w takes its rows 4w .. 4w + 3 (64 pixels: row group
+// g, column l % 16). Every tap of the tile reads dy at (y + 1 - ky, x + 1 - kx), inside the
+// (kHaloT + 2)^2 pixels around the tile, so K runs over 32-channel chunks of dy, each chunk's halo
+// staged once in LDS (80-byte rows: the 16-byte reads of 16 consecutive pixels hit distinct
+// banks) and its nine taps read from there, the next chunk's halo loaded into registers
+// meanwhile (double-buffered LDS). The weights come from L1/L2, the next tap's loaded ahead.
+// Epilogue as k_igemm<1>'s.
+// ---------------------------------------------------------------------------
+constexpr int kHaloT = 16;
+constexpr int kHaloS = kHaloT + 2;                      // halo side
+constexpr int kHaloPix = kHaloS * kHaloS;               // 324 pixels
+constexpr int kHaloQ = (kHaloPix * 4 + kThreads - 1) / kThreads;  // 16-byte pieces per thread
+
+template <int CG>
+__global__ __launch_bounds__(kThreads) void k_dgrad_halo_s1(ConvP p) {
+  __shared__ __attribute__((aligned(16))) __bf16 sh[2][kHaloPix][kKRow];
+  const int t = threadIdx.x, wave = t >> 6, l = t & 63;
+  const int ch0 = blockIdx.y * 16 * CG;
+  const int tiles_x = (p.W + kHaloT - 1) / kHaloT, tiles_y = (p.H + kHaloT - 1) / kHaloT;
+  const int n = blockIdx.x / (tiles_x * tiles_y);
+  const int rest = blockIdx.x - n * tiles_x * tiles_y;
+  const int y0 = (rest / tiles_x) * kHaloT, x0 = (rest % tiles_x) * kHaloT;
+  const int K = p.Cout;  // channels of the staged source, dy
+  const __bf16* src = p.dy;
+  const int KC = K / 32;
+  // this thread's halo pieces: pixel e / 4, 16-byte part e % 4, e = t + 256 i
+  bf16x8 hv[kHaloQ];
+  auto hload = [&](int kc) {
+#pragma unroll
+    for (int i = 0; i < kHaloQ; ++i) {
+      const int e = t + kThreads * i;
+      const int pi = e >> 2, part = e & 3;
+      const int sy = y0 - 1 + pi / kHaloS, sx = x0 - 1 + pi % kHaloS;
+      const bool ok = e < kHaloPix * 4 && sy >= 0 && sy < p.H && sx >= 0 && sx < p.W;
+      hv[i] = ok ? load8(src + ((static_cast<int64_t>(n) * p.H + sy) * p.W + sx) * K + 32 * kc + 8 * part) : zero8();
+    }
+  };
+  auto hstore = [&](int b) {
+#pragma unroll
+    for (int i = 0; i < kHaloQ; ++i) {
+      const int e = t + kThreads * i;
+      if (e < kHaloPix * 4) *reinterpret_cast<bf16x8*>(&sh[b][e >> 2][8 * (e & 3)]) = hv[i];
+    }
+  };
+  const int kq = 8 * (l >> 4), tx = l & 15;
+  // weights of (tap, chunk): wt[tap][ch][k]
+  auto wload = [&](int tap, int kc, bf16x8 (&a)[CG]) {
+#pragma unroll
+    for (int c = 0; c < CG; ++c)
+      a[c] = load8(p.wt + (static_cast<int64_t>(tap) * p.Cin + ch0 + pair_row(l, c)) * p.Cout + 32 * kc + kq);
+  };
+  f32x4 acc[4][CG];
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+#pragma unroll
+    for (int c = 0; c < CG; ++c) acc[g][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  hload(0);
+  hstore(0);
+  bf16x8 ax[CG], an[CG];
+  wload(0, 0, ax);
+  __syncthreads();
+  for (int kc = 0; kc < KC; ++kc) {
+    const int b = kc & 1;
+    if (kc + 1 < KC) hload(kc + 1);
+#pragma unroll 1
+    for (int tap = 0; tap < 9; ++tap) {  // rolled: one tap's weights in flight ahead, not nine
+      // the next (tap, chunk)'s weights ahead
+      if (tap < 8) wload(tap + 1, kc, an);
+      else if (kc + 1 < KC) wload(0, kc + 1, an);
+      const int oy = 2 - tap / 3, ox = 2 - tap % 3;  // dy at (y + 1 - ky, x + 1 - kx)
+      bf16x8 bx[4];
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+        bx[g] = *reinterpret_cast<const bf16x8*>(&sh[b][(4 * wave + g + oy) * kHaloS + tx + ox][kq]);
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int c = 0; c < CG; ++c) acc[g][c] = mfma(ax[c], bx[g], acc[g][c]);
+#pragma unroll
+      for (int c = 0; c < CG; ++c) ax[c] = an[c];
+    }
+    if (kc + 1 < KC) {
+      hstore(b ^ 1);
+      __syncthreads();
+    }
+  }
+  // lane l: channels cb .. cb + 7 (cb = ch0 + 32 h + kq) of pixel (y0 + 4 wave + g, x0 + tx)
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int y = y0 + 4 * wave + g, x = x0 + tx;
+    if (y >= p.H || x >= p.W) continue;
+    const int64_t pix = (static_cast<int64_t>(n) * p.H + y) * p.W + x;
+#pragma unroll
+    for (int h = 0; h < CG / 2; ++h) {
+      const int cb = ch0 + 32 * h + kq;
+      const bf16x8 xm = p.mask ? load8(p.x + pix * p.Cin + cb) : zero8();
+      bf16x8 v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        float u = acc[g][2 * h + j / 4][j % 4];
+        if (p.mask && !(static_cast<float>(xm[j]) > 0.0f)) u = 0.0f;
+        v[j] = static_cast<__bf16>(u);
+      }
+      *reinterpret_cast<bf16x8*>(p.dx + pix * p.Cin + cb) = v;
+    }
+  }
+}
+
+void launch_dgrad_halo_s1(const ConvP& p, hipStream_t st) {
+  const unsigned tiles = static_cast<unsigned>(p.N * ((p.H + kHaloT - 1) / kHaloT) * ((p.W + kHaloT - 1) / kHaloT));
+  if (p.Cin % 64 == 0) hipLaunchKernelGGL((k_dgrad_halo_s1<4>), dim3(tiles, p.Cin / 64), dim3(kThreads), 0, st, p);
+  else hipLaunchKernelGGL((k_dgrad_halo_s1<2>), dim3(tiles, p.Cin / 32), dim3(kThreads), 0, st, p);
+}
+
 template <int MODE>
 void launch_igemm(const ConvP& p, int rows, int cols, int classes, hipStream_t st) {
   if (cols % 128 == 0) {
@@ -1042,6 +1160,13 @@ int ndrr_conv3x3_bwd_data(const void* dy, const void* wt, const void* x, void* d
       hipLaunchKernelGGL((k_dgrad_s2<4>), dim3(static_cast<unsigned>(tiles), Cin / 64), dim3(kThreads), lds, st, p);
     else
       hipLaunchKernelGGL((k_dgrad_s2<2>), dim3(static_cast<unsigned>(tiles), Cin / 32), dim3(kThreads), lds, st, p);
+    return status();
+  }
+  // the halo-staged kernel for stride 1 up to 64 output channels (each chunk of dy read once per
+  // tile); beyond, the tiled implicit GEMM, whose weights staged in LDS serve four waves at once,
+  // measured faster
+  if (stride == 1 && Cout <= 64) {
+    launch_dgrad_halo_s1(p, st);
     return status();
   }
   // pixels per class: all of them (stride 1), or the largest parity class (stride 2)
