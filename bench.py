@@ -7,10 +7,16 @@ samples per ray. One step = render(...) of the 16 poses plus the backward of
 a weighted sum to the poses, captured once in a HIP graph and replayed (the
 eager, uncaptured step is reported too); DRR/s = 16 * steps / elapsed.
 Cases (volume dtype : layout the kernels read : adjoint[+gvol]):
-  bf16:coef:single     the headline `value`: bf16 coefficient records (each
-                       cell's interpolant coefficients, made from the fp32
-                       volume), single-pass adjoint (the forward march stores
-                       per-ray sums, the backward contracts them);
+  bf16:lcoef:single    the headline `value`: bf16 labelled coefficient records
+                       (each cell's interpolant coefficients, made from the
+                       fp32 volume, with the classes of its eight corners in
+                       bits the record's scale leaves free: one 16-byte load
+                       per sample, no label map; the renderer's own choice
+                       for this subject), single-pass adjoint (the forward
+                       march stores per-ray sums, the backward contracts them);
+  bf16:coef:single     the same without the classes in the records (the label
+                       map read beside them: the choice for more than 4
+                       classes or a volume with negative values);
   fp32:linear:two      the plain fp32 kernel: the (D, H, W) volume, two-pass
                        adjoint (the backward marches the volume again, as
                        nanodrr's kernel does) - the baseline of the speed-ups;
@@ -19,7 +25,7 @@ Cases (volume dtype : layout the kernels read : adjoint[+gvol]):
   bf16:quad:single, fp32:quad:single, bf16:stencil:single  other corners.
 Every case accumulates in float32.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cases bf16:coef:single,...]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cases bf16:lcoef:single,...]
                     [--no-cpu-baseline] [--no-registration] [--no-mixed]
 
 Prints one JSON line. `roofline` covers the dominant kernel of the headline
@@ -57,8 +63,8 @@ GRAPH_ITERS = 10  # registration iterations per captured HIP graph
 REGISTRATION_LAYOUT = "quad"
 
 
-DEFAULT_CASES = "bf16:coef:single,fp32:linear:two,fp32:linear:two+gvol,bf16:quad:single,fp32:quad:single," \
-                "bf16:stencil:single"
+DEFAULT_CASES = "bf16:lcoef:single,fp32:linear:two,fp32:linear:two+gvol,bf16:coef:single,bf16:quad:single," \
+                "fp32:quad:single,bf16:stencil:single"
 
 
 def make_workload(dtype, device, gvol: bool = False):
@@ -343,9 +349,9 @@ def cpu_baseline(ph):
 def mixed_precision_leg(steps: int = 20, warmup: int = 5):
     """The mixed-precision workload at full size: a 512 x 512 x 400 phantom at 0.8 x 0.8 x 1.0 mm
     stored in bf16 and in fp16, 2 labels, 8 poses, 512 x 512 detector at 0.8 mm, S = 512
-    (compensated sums), fwd + pose backward (single-pass adjoint; coefficient records, the
-    16-bit default, and quad records), graph-replayed steps timed with HIP events, plus the
-    plain fp32 kernel on the fp32 volume."""
+    (compensated sums), fwd + pose backward (single-pass adjoint; labelled coefficient records,
+    the 16-bit default for this subject, plain coefficient records and quad records),
+    graph-replayed steps timed with HIP events, plus the plain fp32 kernel on the fp32 volume."""
     import nanodrr_amd.renderer as renderer
     from nanodrr_amd import render
     from nanodrr_amd.phantom import detector, make_phantom, phantom_subject, sample_poses
@@ -355,7 +361,9 @@ def mixed_precision_leg(steps: int = 20, warmup: int = 5):
     B, det, n_s = 8, 512, 512
     res = {"workload": "512x512x400 @0.8x0.8x1.0 mm, C=2, B=8, 512x512 @0.8 mm, S=512 (compensated), fwd + pose "
                        "backward, graph-replayed step"}
-    for name, dt, layout, adjoint in (("bf16_coef", torch.bfloat16, "coef", "single_pass"),
+    for name, dt, layout, adjoint in (("bf16_lcoef", torch.bfloat16, "lcoef", "single_pass"),
+                                      ("fp16_lcoef", torch.float16, "lcoef", "single_pass"),
+                                      ("bf16_coef", torch.bfloat16, "coef", "single_pass"),
                                       ("fp16_coef", torch.float16, "coef", "single_pass"),
                                       ("bf16_quad", torch.bfloat16, "quad", "single_pass"),
                                       ("fp16_quad", torch.float16, "quad", "single_pass"),
@@ -390,7 +398,7 @@ def mixed_precision_leg(steps: int = 20, warmup: int = 5):
         gc.collect()
         torch.cuda.empty_cache()
     renderer.VOLUME_LAYOUT, renderer.ADJOINT = "auto", "single_pass"
-    res["speedup_bf16_coef_over_fp32_plain"] = res["bf16_coef"]["drr_per_s"] / res["fp32_plain"]["drr_per_s"]
+    res["speedup_bf16_lcoef_over_fp32_plain"] = res["bf16_lcoef"]["drr_per_s"] / res["fp32_plain"]["drr_per_s"]
     return res
 
 

@@ -23,12 +23,14 @@
  *   out  [B][C][N] fp32 line integrals
  *   vol  the volume in NDRR_F32 / NDRR_BF16 / NDRR_F16, laid out as
  *        NDRR_LAYOUT_LINEAR:  [D][H][W], W fastest, or a packed copy made by
- *        ndrr_pack_volume / ndrr_pack_coef: NDRR_LAYOUT_QUAD, NDRR_LAYOUT_STENCIL
- *        or NDRR_LAYOUT_COEF records (vol_dtype is then the record type)
+ *        ndrr_pack_volume / ndrr_pack_coef / ndrr_pack_coef_labels:
+ *        NDRR_LAYOUT_QUAD, NDRR_LAYOUT_STENCIL, NDRR_LAYOUT_COEF or
+ *        NDRR_LAYOUT_LCOEF records (vol_dtype is then the record type)
  *   lab  int8 class per voxel (values outside [0, C) are dropped) in the form
  *        vol_layout reads: the [D][H][W] map for NDRR_LAYOUT_LINEAR, its padded
  *        copy for NDRR_LAYOUT_STENCIL and NDRR_LAYOUT_COEF and its label quads
- *        for NDRR_LAYOUT_QUAD (both by ndrr_pack_labels); ignored when
+ *        for NDRR_LAYOUT_QUAD (both by ndrr_pack_labels); ignored (may be NULL)
+ *        for NDRR_LAYOUT_LCOEF, whose records carry the classes, and when
  *        n_classes == 1
  * Sample s of ray (b, n) sits at M[b] @ [lerp(src, tgt, s / (S - 1)), 1].
  * Rays are mapped to detector tiles of a `width`-wide image when width > 0
@@ -57,7 +59,8 @@ enum {
   NDRR_LAYOUT_LINEAR = 0,  /* [D][H][W] elements */
   NDRR_LAYOUT_STENCIL = 1, /* one record of 8 trilinear corners per cell */
   NDRR_LAYOUT_QUAD = 2,    /* one record of a (y, z) corner quad per x */
-  NDRR_LAYOUT_COEF = 3     /* one record of 8 multilinear coefficients per cell */
+  NDRR_LAYOUT_COEF = 3,    /* one record of 8 multilinear coefficients per cell */
+  NDRR_LAYOUT_LCOEF = 4    /* the same with its 8 corners' classes (<= 4 classes) */
 };
 
 /* Argument errors (negative return codes). */
@@ -304,6 +307,38 @@ int ndrr_pack_volume(const void* vol, int vol_dtype, int vol_layout, int D, int 
                      void* out, void* stream);
 int ndrr_pack_coef(const void* vol, int vol_dtype, int record_dtype, int D, int H, int W,
                    void* out, void* stream);
+
+/*
+ * NDRR_LAYOUT_LCOEF: coefficient records (16-bit only, NDRR_LAYOUT_COEF's
+ * slots and 256-byte header) that also carry the classes of their cell's
+ * eight corners, so the march reads no label map: one 16-byte load per sample
+ * instead of a record load and a label load. The scale s is the largest power
+ * of two with max|coefficient| s <= 2^-7 - 2^-18, so every slot is below 2^-7
+ * and the top two exponent bits of each fp16 slot are zero (a bf16 c0, not
+ * negative, leaves its sign and top exponent bit zero); those two bits of
+ * slot j = x + 2y + 4z hold the class code of the corner at offsets (x, y, z):
+ * bits 13-14 of the slot, bits 14-15 for a bf16 c0. The code is lab & 3 (0
+ * outside the volume): classes 0..3, with 3 also meaning dropped (-1) when
+ * n_classes <= 3. Differences below 2^-14 / s are fp16 subnormals, rounded to
+ * 2^-25 / s absolutely. Header words: [0] max|coefficient| (fp32 bits), [1] s,
+ * [2] 1/s, [3] record type, [4] NDRR_LCOEF_* bits of what the volume and
+ * labels break (0: the records are valid), [5] n_classes.
+ * ndrr_pack_coef_labels fills such records of record_dtype (NDRR_BF16 or
+ * NDRR_F16) from a volume of vol_dtype (not negative) and its [D][H][W] int8
+ * class map (values in [-1, n_classes), no -1 when n_classes == 4), n_classes
+ * in [1, 4]. The records are then marched with vol_dtype = record_dtype,
+ * vol_layout = NDRR_LAYOUT_LCOEF and the same n_classes (at most 4). The
+ * launch does not synchronise: read header word 4 before the first march.
+ * Replaces the nearest-label gather of _fused.py:222-228 as well as the
+ * trilinear gathers of _fused.py:205-212 with one record per sample.
+ */
+enum {
+  NDRR_LCOEF_NEGATIVE = 1, /* a voxel below zero */
+  NDRR_LCOEF_CLASS = 2,    /* a class outside [-1, n_classes), or -1 with 4 classes */
+  NDRR_LCOEF_RANGE = 4     /* max|coefficient| not finite or above 2^119 */
+};
+int ndrr_pack_coef_labels(const void* vol, int vol_dtype, const int8_t* lab, int n_classes,
+                          int record_dtype, int D, int H, int W, void* out, void* stream);
 
 /*
  * Labels in the form a layout's march reads. For NDRR_LAYOUT_QUAD: one uint32

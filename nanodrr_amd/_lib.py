@@ -13,8 +13,10 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().with_name("libnanodrr_amd.so")
 
 NDRR_F32, NDRR_BF16, NDRR_F16 = 0, 1, 2
-LAYOUT_LINEAR, LAYOUT_STENCIL, LAYOUT_QUAD, LAYOUT_COEF = 0, 1, 2, 3
-LAYOUTS = {"linear": LAYOUT_LINEAR, "stencil": LAYOUT_STENCIL, "quad": LAYOUT_QUAD, "coef": LAYOUT_COEF}
+LAYOUT_LINEAR, LAYOUT_STENCIL, LAYOUT_QUAD, LAYOUT_COEF, LAYOUT_LCOEF = 0, 1, 2, 3, 4
+LAYOUTS = {"linear": LAYOUT_LINEAR, "stencil": LAYOUT_STENCIL, "quad": LAYOUT_QUAD, "coef": LAYOUT_COEF,
+           "lcoef": LAYOUT_LCOEF}
+LCOEF_NEGATIVE, LCOEF_CLASS, LCOEF_RANGE = 1, 2, 4  # header word 4 of labelled coefficient records
 MAX_CLASSES = 8
 
 _c_int, _c_i64, _c_p, _c_size = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_size_t
@@ -79,6 +81,7 @@ SIGNATURES: dict[str, tuple] = {
     "ndrr_packed_bytes": (_c_size, [_c_int, _c_int, _c_int, _c_int, _c_int]),
     "ndrr_pack_volume": (_c_int, [_c_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p]),
     "ndrr_pack_coef": (_c_int, [_c_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p]),
+    "ndrr_pack_coef_labels": (_c_int, [_c_p, _c_int, _c_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p]),
     "ndrr_packed_label_bytes": (_c_size, [_c_int, _c_int, _c_int, _c_int]),
     "ndrr_pack_labels": (_c_int, [_c_p, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p]),
     "ndrr_make_rays": (_c_int, [_c_p, _c_i64, _c_p, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p, _c_p,

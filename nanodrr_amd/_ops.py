@@ -46,7 +46,7 @@ def _volume_arg(vol, packed, layout):
         return vol.data_ptr(), _lib.LAYOUT_LINEAR
     code = _lib.LAYOUTS.get(layout)
     if code is None or code == _lib.LAYOUT_LINEAR:
-        raise ValueError(f"layout must name the packed copy ('quad', 'stencil' or 'coef'), got {layout!r}")
+        raise ValueError(f"layout must name the packed copy ('quad', 'stencil', 'coef' or 'lcoef'), got {layout!r}")
     D, H, W = vol.shape
     need = _lib.lib().ndrr_packed_bytes(D, H, W, _DTYPE_CODE[vol.dtype], code)
     if packed.dtype != torch.uint8 or packed.numel() != need or not packed.is_contiguous() \
@@ -62,6 +62,8 @@ def pack_volume(vol: torch.Tensor, layout: str) -> torch.Tensor:
         raise ValueError("pack_volume needs a (D, H, W) float32/bfloat16/float16 device tensor")
     if layout not in _lib.LAYOUTS:
         raise ValueError(f"unknown layout {layout!r}; expected one of {sorted(_lib.LAYOUTS)}")
+    if layout == "lcoef":
+        raise ValueError("labelled coefficient records are made from the volume and its labels (pack_coef_labels)")
     vol = vol.detach().contiguous()
     D, H, W = vol.shape
     lib = _lib.lib()
@@ -73,14 +75,14 @@ def pack_volume(vol: torch.Tensor, layout: str) -> torch.Tensor:
     return out
 
 
-COEF_HEADER_BYTES = 256  # the coefficient layout's header: max |v|, scale s, 1/s, record type
+COEF_HEADER_BYTES = 256  # the coefficient layouts' header: max |v| (|coefficient|), scale s, 1/s, record type, ...
 
 
 def _packed_bytes(D, H, W, layout, dtype):
     e = 4 if dtype == torch.float32 else 2
     n = {"linear": D * H * W, "stencil": (D + 1) * (H + 1) * (W + 1) * 8, "quad": (D + 1) * (H + 1) * (W + 2) * 4,
-         "coef": (D + 1) * (H + 1) * (W + 1) * 8}[layout]
-    return n * e + (COEF_HEADER_BYTES if layout == "coef" else 0)
+         "coef": (D + 1) * (H + 1) * (W + 1) * 8, "lcoef": (D + 1) * (H + 1) * (W + 1) * 8}[layout]
+    return n * e + (COEF_HEADER_BYTES if layout in ("coef", "lcoef") else 0)
 
 
 @pack_volume.register_fake
@@ -113,6 +115,54 @@ def _(vol, record_dtype):
     return torch.empty(_packed_bytes(*vol.shape, "coef", record_dtype), device=vol.device, dtype=torch.uint8)
 
 
+LCOEF_ERRORS = {_lib.LCOEF_NEGATIVE: "the volume has negative values",
+                _lib.LCOEF_CLASS: "a class lies outside [-1, n_classes), or a voxel is dropped (-1) with 4 classes",
+                _lib.LCOEF_RANGE: "the largest coefficient is not finite or too large for the records' scale"}
+
+
+def lcoef_status(packed: torch.Tensor) -> int:
+    """Header word 4 of labelled coefficient records: the LCOEF_* bits of what their volume and labels
+    break (0: valid). Reads the device header (synchronises with its stream)."""
+    return int(packed[16:20].view(torch.int32).item())
+
+
+@torch.library.custom_op("nanodrr_amd::pack_coef_labels", mutates_args=())
+def pack_coef_labels(vol: torch.Tensor, lab: torch.Tensor, n_classes: int, record_dtype: torch.dtype) -> torch.Tensor:
+    """Labelled coefficient records (the "lcoef" layout, uint8 bytes, ndrr_pack_coef_labels) of a not
+    negative (D, H, W) device volume and its int8 class map (`Subject.label_i8`, n_classes <= 4) for
+    bf16 or fp16 storage: the "coef" records at a scale that frees two bits per slot, which hold the
+    classes of the cell's eight corners, so the march reads no label map. Raises ValueError when the
+    volume or labels do not fit the format (the check reads the records' header back)."""
+    if not vol.is_cuda or vol.dim() != 3 or vol.dtype not in _DTYPE_CODE:
+        raise ValueError("pack_coef_labels needs a (D, H, W) float32/bfloat16/float16 device tensor")
+    if record_dtype not in (torch.bfloat16, torch.float16):
+        raise ValueError(f"labelled coefficient records are bfloat16 or float16, not {record_dtype}")
+    if not 1 <= n_classes <= 4:
+        raise ValueError(f"labelled coefficient records hold at most 4 classes, got {n_classes}")
+    if lab.dtype != torch.int8 or lab.shape != vol.shape or lab.device != vol.device:
+        raise ValueError("lab must be the (D, H, W) int8 class map on the volume's device")
+    vol = vol.detach().contiguous()
+    lab = lab.contiguous()
+    D, H, W = vol.shape
+    lib = _lib.lib()
+    rd = _DTYPE_CODE[record_dtype]
+    out = torch.empty(lib.ndrr_packed_bytes(D, H, W, rd, _lib.LAYOUT_LCOEF), device=vol.device, dtype=torch.uint8)
+    with torch.cuda.device(vol.device):
+        rc = lib.ndrr_pack_coef_labels(vol.data_ptr(), _DTYPE_CODE[vol.dtype], lab.data_ptr(), n_classes, rd, D, H, W,
+                                       out.data_ptr(), _stream(vol.device))
+    _lib.check(rc, "ndrr_pack_coef_labels")
+    status = lcoef_status(out)
+    if status:
+        raise ValueError("labelled coefficient records cannot hold this volume: "
+                         + "; ".join(m for bit, m in LCOEF_ERRORS.items() if status & bit))
+    return out
+
+
+@pack_coef_labels.register_fake
+def _(vol, lab, n_classes, record_dtype):
+    return torch.empty(_packed_bytes(*vol.shape, "lcoef", record_dtype), device=vol.device, dtype=torch.uint8)
+
+
 @torch.library.custom_op("nanodrr_amd::pack_labels", mutates_args=())
 def pack_labels(lab: torch.Tensor, layout: str) -> torch.Tensor:
     """Labels in the form `layout`'s march reads (ndrr_pack_labels): the label quads for "quad",
@@ -122,6 +172,8 @@ def pack_labels(lab: torch.Tensor, layout: str) -> torch.Tensor:
         raise ValueError("pack_labels needs a (D, H, W) int8 device tensor (Subject.label_i8)")
     if layout not in _lib.LAYOUTS:
         raise ValueError(f"unknown layout {layout!r}; expected one of {sorted(_lib.LAYOUTS)}")
+    if layout == "lcoef":  # the records carry the classes
+        return lab.new_empty(0)
     lab = lab.contiguous()
     D, H, W = lab.shape
     lib = _lib.lib()
@@ -136,6 +188,8 @@ def pack_labels(lab: torch.Tensor, layout: str) -> torch.Tensor:
 @pack_labels.register_fake
 def _(lab, layout):
     D, H, W = lab.shape
+    if layout == "lcoef":
+        return lab.new_empty(0)
     if layout == "quad":
         return torch.empty((D + 1) * (H + 1) * (W + 2) * 4, device=lab.device, dtype=torch.int8)
     if layout in ("stencil", "coef"):

@@ -86,16 +86,18 @@ constexpr int kWave = 64;        // rays per wave: an 8 x 8 detector patch (or 6
 constexpr int kTwx = NDRR_TILE_WAVES_X, kTwy = NDRR_TILE_WAVES_Y;  // 8 x 8 wave patches per tile
 constexpr int kTileW = 8 * kTwx, kTileH = 8 * kTwy;                // tile size in detector pixels
 constexpr int kTileRays = kWave * kTwx * kTwy;
+// The two coefficient layouts (plain and labelled records) march alike.
+constexpr bool is_coef(int layout) { return layout == NDRR_LAYOUT_COEF || layout == NDRR_LAYOUT_LCOEF; }
 constexpr int fwd_chunks(int layout) {
   return layout == NDRR_LAYOUT_STENCIL ? NDRR_FWD_CHUNKS_STENCIL
          : layout == NDRR_LAYOUT_QUAD  ? NDRR_FWD_CHUNKS_QUAD
-         : layout == NDRR_LAYOUT_COEF  ? NDRR_FWD_CHUNKS_COEF
+         : is_coef(layout)             ? NDRR_FWD_CHUNKS_COEF
                                        : NDRR_FWD_CHUNKS_LINEAR;
 }
 constexpr int bwd_chunks(int layout) {
   return layout == NDRR_LAYOUT_STENCIL ? NDRR_BWD_CHUNKS_STENCIL
          : layout == NDRR_LAYOUT_QUAD  ? NDRR_BWD_CHUNKS_QUAD
-         : layout == NDRR_LAYOUT_COEF  ? NDRR_BWD_CHUNKS_COEF
+         : is_coef(layout)             ? NDRR_BWD_CHUNKS_COEF
                                        : NDRR_BWD_CHUNKS_LINEAR;
 }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
@@ -104,7 +106,7 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // slot) stays small.
 constexpr int fwd_chunks(int layout, int cv) {
   return cv == 1 && layout == NDRR_LAYOUT_QUAD   ? NDRR_FWD_CHUNKS_QUAD_1CLASS
-         : cv == 1 && layout == NDRR_LAYOUT_COEF ? NDRR_FWD_CHUNKS_COEF_1CLASS
+         : cv == 1 && is_coef(layout)            ? NDRR_FWD_CHUNKS_COEF_1CLASS
                                                  : (cv <= 4 ? fwd_chunks(layout) : 1);
 }
 constexpr int kMaxFwdChunks = cmax(cmax(cmax(fwd_chunks(NDRR_LAYOUT_LINEAR), fwd_chunks(NDRR_LAYOUT_STENCIL)),
@@ -673,23 +675,63 @@ struct Coef {
   float c0, cx, cy, cxy, cz, cxz, cyz, cxyz;
 };
 
+template <int DT> __device__ inline Coef coef_of(const uint4& q) {
+  if constexpr (DT == NDRR_BF16)
+    return {bf16_lo(q.x), f16_hi(q.x), f16_lo(q.y), f16_hi(q.y),
+            f16_lo(q.z), f16_hi(q.z), f16_lo(q.w), f16_hi(q.w)};
+  else
+    return {f16_lo(q.x), f16_hi(q.x), f16_lo(q.y), f16_hi(q.y),
+            f16_lo(q.z), f16_hi(q.z), f16_lo(q.w), f16_hi(q.w)};
+}
+
 template <int DT> struct Fetch<DT, NDRR_LAYOUT_COEF> {  // bf16 / fp16: 16-byte records
   static constexpr int kInFlight = NDRR_IN_FLIGHT;
   uint4 q;
   __device__ inline void load(const Src& s, const Cell& c, bool live) { q = load16(s.vrs, record_offset(s, c, s.W + 1, 16u, live)); }
+  __device__ inline Coef coef() const { return coef_of<DT>(q); }
+};
+
+// Labelled coefficient records (bf16 / fp16 only): the coefficient layout's
+// records with a scale that keeps every slot below 2^-7, so that the two top
+// exponent bits of each fp16 slot are zero, and so are the sign and top
+// exponent bit of a bf16 c0 (the volume is not negative). Those two bits of
+// slot j hold the class code of the cell corner j = x + 2y + 4z (the slots'
+// own order): bits 13-14 of the 16-bit slot, bits 14-15 for a bf16 c0. The
+// class code is the class for classes 0..3; code 3 also stands for a dropped
+// voxel (-1) when there are at most three classes. A sample's nearest voxel
+// is a corner of its cell (or, for a cell with no record, a zero record:
+// class 0 and value 0), so its class comes with the record the sample loads
+// anyway, and the march reads no label map.
+constexpr uint32_t kLcoefMaskFp16 = 0x9FFF9FFFu;  // a word of two fp16 slots without their class bits
+constexpr uint32_t kLcoefMaskBf16 = 0x9FFF3FFFu;  // the bf16 c0 / fp16 cx word without its class bits
+template <int DT> struct Fetch<DT, NDRR_LAYOUT_LCOEF> {
+  static_assert(DT != NDRR_F32, "labelled coefficient records are 16-bit");
+  static constexpr int kInFlight = NDRR_IN_FLIGHT;
+  uint4 q;
+  __device__ inline void load(const Src& s, const Cell& c, bool live) { q = load16(s.vrs, record_offset(s, c, s.W + 1, 16u, live)); }
   __device__ inline Coef coef() const {
-    if constexpr (DT == NDRR_BF16)
-      return {bf16_lo(q.x), f16_hi(q.x), f16_lo(q.y), f16_hi(q.y),
-              f16_lo(q.z), f16_hi(q.z), f16_lo(q.w), f16_hi(q.w)};
-    else
-      return {f16_lo(q.x), f16_hi(q.x), f16_lo(q.y), f16_hi(q.y),
-              f16_lo(q.z), f16_hi(q.z), f16_lo(q.w), f16_hi(q.w)};
+    constexpr uint32_t m0 = DT == NDRR_BF16 ? kLcoefMaskBf16 : kLcoefMaskFp16;
+    return coef_of<DT>(make_uint4(q.x & m0, q.y & kLcoefMaskFp16, q.z & kLcoefMaskFp16, q.w & kLcoefMaskFp16));
+  }
+  // Class code of the corner at offsets (ox, oy, oz) given as xy = kRintBias * 3 + ox + 2 oy
+  // and z = kRintBias + oz (nearest_corner; mod 2^32): the corner's slot is ox + 2 oy in the
+  // record's first 8 bytes for oz = 0, its last 8 for oz = 1, so the code lies at bit
+  // 16 (ox + 2 oy) + 13 of that 64-bit half (+ 1 for a bf16 c0). (Selects of loaded values:
+  // selecting between the members themselves lets the compiler index the record through
+  // scratch memory.)
+  __device__ inline int label(uint32_t xy, uint32_t z) const {
+    const uint32_t qx = q.x, qy = q.y, qz = q.z, qw = q.w;
+    const bool hz = z != kRintBias;
+    const uint64_t half = hz ? (static_cast<uint64_t>(qw) << 32 | qz) : (static_cast<uint64_t>(qy) << 32 | qx);
+    uint32_t sh = (xy - 3u * kRintBias) * 16u + 13u;
+    if constexpr (DT == NDRR_BF16) sh += (xy == 3u * kRintBias && !hz) ? 1u : 0u;
+    return static_cast<int>((half >> sh) & 3u);
   }
 };
 
 // 1/s of the coefficient records (1 for the other layouts), read from the header.
 template <int L> __device__ inline float record_unscale(const Params& p) {
-  if constexpr (L == NDRR_LAYOUT_COEF) return static_cast<const float*>(p.vol)[2];
+  if constexpr (is_coef(L)) return static_cast<const float*>(p.vol)[2];
   return 1.0f;
 }
 
@@ -753,6 +795,19 @@ __device__ inline void interp_grad(const Fetch<DT, NDRR_LAYOUT_COEF>& f, const C
                                    float& dvz) {
   coef_value_grad(f.coef(), c, dvx, dvy, dvz);
 }
+template <int DT> __device__ inline float interp_value(const Fetch<DT, NDRR_LAYOUT_LCOEF>& f, const Cell& c) {
+  return coef_value(f.coef(), c);
+}
+template <int DT>
+__device__ inline float interp_value_grad(const Fetch<DT, NDRR_LAYOUT_LCOEF>& f, const Cell& c, float& dvx,
+                                          float& dvy, float& dvz) {
+  return coef_value_grad(f.coef(), c, dvx, dvy, dvz);
+}
+template <int DT>
+__device__ inline void interp_grad(const Fetch<DT, NDRR_LAYOUT_LCOEF>& f, const Cell& c, float& dvx, float& dvy,
+                                   float& dvz) {
+  coef_value_grad(f.coef(), c, dvx, dvy, dvz);
+}
 
 // Nearest label (round half to even, zero padded: outside the volume is class 0).
 __device__ inline int nearest_label(const int8_t* __restrict__ lab, int D, int H, int W,
@@ -789,6 +844,18 @@ __device__ inline int quad_label(const Src& s, const Cell& c, float x, float y, 
 // x + 1.5 * 2^23 lies in [2^23, 2^24), where the spacing of floats is 1, so the sum is x
 // rounded to an integer, whose bits are kRintBias + rint(x): bits 0..23 hold 2^22 + rint(x).
 __device__ inline int rint_bits(float x) { return __float_as_int(x + 12582912.0f); }
+
+// The corner of cell c nearest to sample (x, y, z): the offsets o = rint(p)
+// (round half to even) - floor(p) along each axis, 0 or 1, from rint_bits
+// (|p| < 2^22: the sample lies in the volume's box or its cell has no record,
+// whose zero record gives class 0 whatever corner is named), returned as
+// kRintBias * 3 + ox + 2 oy, with kRintBias + oz in z (both mod 2^32; the bias
+// is removed where the offsets are used).
+__device__ inline uint32_t nearest_corner(const Cell& c, float x, float y, float z, uint32_t& oz) {
+  const uint32_t bx = static_cast<uint32_t>(rint_bits(x) - c.ix), by = static_cast<uint32_t>(rint_bits(y) - c.iy);
+  oz = static_cast<uint32_t>(rint_bits(z) - c.iz);
+  return bx + 2u * by;
+}
 
 // Nearest label from the padded class map of the stencil and coefficient
 // layouts: int8 [D + 4][H + 4][W + 4] holding the class of voxel (z, y, x) at
@@ -834,6 +901,7 @@ struct Batch {
   Fetch<DT, L> f[U];
   Cell cell[U];
   int cls[U];
+  uint32_t cz[U];  // labelled records: the nearest corner's z offset (nearest_corner)
   __device__ inline void issue(const Src& s, const Ray& r, int k0) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -849,10 +917,17 @@ struct Batch {
           cls[u] = quad_label(s, cell[u], x, y, z, live);
         else if constexpr (L == NDRR_LAYOUT_LINEAR)
           cls[u] = nearest_label(s.lab, s.D, s.H, s.W, x, y, z, live);
+        else if constexpr (L == NDRR_LAYOUT_LCOEF)
+          cls[u] = static_cast<int>(nearest_corner(cell[u], x, y, z, cz[u]));
         else
           cls[u] = padded_label(s, x, y, z);
       }
     }
+  }
+  // Class of sample u (labelled records: from the loaded record, once it has arrived).
+  __device__ inline int label(int u) const {
+    if constexpr (L == NDRR_LAYOUT_LCOEF) return f[u].label(static_cast<uint32_t>(cls[u]), cz[u]);
+    else return cls[u];
   }
 };
 
@@ -910,7 +985,7 @@ void k_fwd(Params p) {
       // among its lanes at this sample (mostly one or two), not for all CV.
 #pragma unroll
       for (int cl = 0; cl < CV; ++cl) {
-        if (CV == 1 || batch.cls[u] == cl) {
+        if (CV == 1 || batch.label(u) == cl) {
           if constexpr (KAHAN) comp_add(acc[cl], comp[cl], val); else add_rn(acc[cl], val);
           if constexpr (JAC) {
             add_rn(jg[6 * cl + 0], dvx);
@@ -1109,7 +1184,7 @@ void k_bwd(Params p) {
       } else {
         gg = 0.0f;
 #pragma unroll
-        for (int cl = 0; cl < CV; ++cl) gg = (batch.cls[u] == cl) ? goc[cl] : gg;
+        for (int cl = 0; cl < CV; ++cl) gg = (batch.label(u) == cl) ? goc[cl] : gg;
       }
       gg = live ? gg : 0.0f;
       const float g = gg * r.step;
@@ -1142,7 +1217,7 @@ void k_bwd(Params p) {
     }
   }
 
-  if constexpr (L == NDRR_LAYOUT_COEF) {
+  if constexpr (is_coef(L)) {
     const float unscale = record_unscale<L>(p);
     Gx *= unscale; Gy *= unscale; Gz *= unscale;
     Tx *= unscale; Ty *= unscale; Tz *= unscale;
@@ -1507,22 +1582,20 @@ template <int DR> __device__ inline float coef_scale(uint32_t absmax_bits) {
 // its trilinear interpolant, in fp32 in a fixed order, times the header's
 // scale s, each rounded once to its slot's type (fp32; bf16 for c0 and fp16
 // for the rest; fp16).
-template <int DS, int DR>
-__global__ void k_pack_coef(const typename Vox<DS>::T* __restrict__ vol, uint32_t* __restrict__ hdr,
-                            void* __restrict__ rec, int D, int H, int W, int64_t n_rec) {
-#pragma clang fp contract(off)
-  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const float scale = coef_scale<DR>(hdr[0]);
-  if (r == 0) {
-    hdr[1] = __float_as_uint(scale);
-    hdr[2] = __float_as_uint(1.0f / scale);  // exact: a power of two
-    hdr[3] = static_cast<uint32_t>(DR);
-  }
-  if (r >= n_rec) return;
-  const int cx = static_cast<int>(r % (W + 1));
+// The cell (x0, y0, z0) = (cx - 1, cy - 1, cz - 1) of record r on the stencil's grid.
+__device__ inline void record_cell(int64_t r, int H, int W, int& cx, int& cy, int& cz) {
+  cx = static_cast<int>(r % (W + 1));
   const int64_t rest = r / (W + 1);
-  const int cy = static_cast<int>(rest % (H + 1));
-  const int cz = static_cast<int>(rest / (H + 1));
+  cy = static_cast<int>(rest % (H + 1));
+  cz = static_cast<int>(rest / (H + 1));
+}
+
+// The coefficients of record (cx, cy, cz)'s interpolant from its eight
+// zero-padded corners (as fp32), in fp32 in a fixed order.
+template <int DS>
+__device__ inline void cell_coefficients(const typename Vox<DS>::T* __restrict__ vol, int D, int H, int W, int cx,
+                                         int cy, int cz, float c[8]) {
+#pragma clang fp contract(off)
   float v[8];  // v[4a + 2b + c] = corner (z0 + a, y0 + b, x0 + c)
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
@@ -1534,7 +1607,6 @@ __global__ void k_pack_coef(const typename Vox<DS>::T* __restrict__ vol, uint32_
   }
   const float dx00 = v[1] - v[0], dx01 = v[3] - v[2], dx10 = v[5] - v[4], dx11 = v[7] - v[6];
   const float cyv = v[2] - v[0];
-  float c[8];
   c[0] = v[0];                  // c0
   c[1] = dx00;                  // cx
   c[2] = cyv;                   // cy
@@ -1543,6 +1615,23 @@ __global__ void k_pack_coef(const typename Vox<DS>::T* __restrict__ vol, uint32_
   c[5] = dx10 - dx00;           // cxz
   c[6] = (v[6] - v[4]) - cyv;   // cyz
   c[7] = (dx11 - dx10) - (dx01 - dx00);  // cxyz
+}
+
+template <int DS, int DR>
+__global__ void k_pack_coef(const typename Vox<DS>::T* __restrict__ vol, uint32_t* __restrict__ hdr,
+                            void* __restrict__ rec, int D, int H, int W, int64_t n_rec) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const float scale = coef_scale<DR>(hdr[0]);
+  if (r == 0) {
+    hdr[1] = __float_as_uint(scale);
+    hdr[2] = __float_as_uint(1.0f / scale);  // exact: a power of two
+    hdr[3] = static_cast<uint32_t>(DR);
+  }
+  if (r >= n_rec) return;
+  int cx, cy, cz;
+  record_cell(r, H, W, cx, cy, cz);
+  float c[8];
+  cell_coefficients<DS>(vol, D, H, W, cx, cy, cz, c);
   if constexpr (DR == NDRR_F32) {
     float4* o = static_cast<float4*>(rec) + 2 * r;
     o[0] = make_float4(c[0], c[1], c[2], c[3]);
@@ -1555,6 +1644,97 @@ __global__ void k_pack_coef(const typename Vox<DS>::T* __restrict__ vol, uint32_
     static_cast<uint4*>(rec)[r] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16),
                                              h[6] | (h[7] << 16));
   }
+}
+
+// Labelled coefficient records. Header: [0] the largest |coefficient| over
+// all records (fp32 bits, an atomic maximum), [1] s, [2] 1/s, [3] the record
+// type, [4] what the pack found wrong (NDRR_LCOEF_* bits, 0 when the records
+// are valid), [5] the class count they were made for.
+template <int DS>
+__global__ __launch_bounds__(kThreads) void k_lcoef_absmax(const typename Vox<DS>::T* __restrict__ vol, int D,
+                                                           int H, int W, int64_t n_rec,
+                                                           uint32_t* __restrict__ hdr) {
+  __shared__ uint32_t red[kThreads / 64];
+  uint32_t m = 0u;  // |coefficient| as fp32 bits: non-negative floats order like their bit patterns
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; r < n_rec;
+       r += static_cast<int64_t>(gridDim.x) * kThreads) {
+    int cx, cy, cz;
+    record_cell(r, H, W, cx, cy, cz);
+    float c[8];
+    cell_coefficients<DS>(vol, D, H, W, cx, cy, cz, c);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) m = max(m, __float_as_uint(c[k]) & 0x7fffffffu);
+  }
+  for (int o = 32; o > 0; o >>= 1) m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), o)));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kThreads / 64; ++w) m = max(m, red[w]);
+    atomicMax(hdr, m);
+  }
+}
+
+// The labelled records' scale: the largest power of two s with
+// max|coefficient| s <= 2^-7 - 2^-18, so that every slot rounds to below
+// 2^-7 (fp16 exponent field <= 7); within [2^-126, 2^126]; 1 for a zero
+// maximum. `fits` is false where no such s exists in that range (a
+// non-finite or too large maximum).
+__device__ inline float lcoef_scale(uint32_t absmax_bits, bool& fits) {
+  const float m = __uint_as_float(absmax_bits);
+  constexpr float kTop = 0x1p-7f - 0x1p-18f;
+  fits = m <= 0x1p119f;
+  if (!(m > 0.0f) || !fits) return 1.0f;
+  const float t = fminf(fmaxf(kTop / m, 0x1p-126f), 0x1p126f);
+  return __uint_as_float(__float_as_uint(t) & 0x7f800000u);  // 2^floor(log2 t)
+}
+
+// One thread per labelled record: the coefficient record of k_pack_coef at
+// the labelled scale, with the 2-bit class code of each corner (lab & 3 in
+// the volume, 0 outside) in its slot's free bits. Flags a negative value (a
+// negative c0, each voxel being the c0 of one record), a class outside
+// [-1, n_classes) or a dropped voxel (-1) with four classes, and a maximum
+// the scale cannot bring below 2^-7.
+template <int DS, int DR>
+__global__ void k_pack_lcoef(const typename Vox<DS>::T* __restrict__ vol, const int8_t* __restrict__ lab,
+                             int n_classes, uint32_t* __restrict__ hdr, uint4* __restrict__ rec, int D, int H,
+                             int W, int64_t n_rec) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool fits;
+  const float scale = lcoef_scale(hdr[0], fits);
+  if (r == 0) {
+    hdr[1] = __float_as_uint(scale);
+    hdr[2] = __float_as_uint(1.0f / scale);  // exact: a power of two
+    hdr[3] = static_cast<uint32_t>(DR);
+    hdr[5] = static_cast<uint32_t>(n_classes);
+    if (!fits) atomicOr(hdr + 4, static_cast<uint32_t>(NDRR_LCOEF_RANGE));
+  }
+  if (r >= n_rec) return;
+  int cx, cy, cz;
+  record_cell(r, H, W, cx, cy, cz);
+  float c[8];
+  cell_coefficients<DS>(vol, D, H, W, cx, cy, cz, c);
+  uint32_t bad = c[0] < 0.0f ? static_cast<uint32_t>(NDRR_LCOEF_NEGATIVE) : 0u;
+  uint32_t h[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    h[k] = Round<NDRR_F16>::bits(c[k] * scale);  // exact product: a power of two
+    const int x = cx - 1 + (k & 1), y = cy - 1 + ((k >> 1) & 1), z = cz - 1 + (k >> 2);
+    const bool inb = static_cast<unsigned>(x) < static_cast<unsigned>(W) &&
+                     static_cast<unsigned>(y) < static_cast<unsigned>(H) &&
+                     static_cast<unsigned>(z) < static_cast<unsigned>(D);
+    const int l = inb ? lab[(static_cast<int64_t>(z) * H + y) * W + x] : 0;
+    if (l < -1 || l >= n_classes || (l == -1 && n_classes == 4)) bad |= NDRR_LCOEF_CLASS;
+    const uint32_t code = static_cast<uint32_t>(l) & 3u;
+    if (k == 0 && DR == NDRR_BF16) {
+      const uint32_t b = Round<NDRR_BF16>::bits(fabsf(c[0]) * scale);  // bits 14-15 zero below 2
+      h[0] = b | (code << 14);
+    } else {
+      if (h[k] & 0x6000u) bad |= NDRR_LCOEF_RANGE;
+      h[k] = (h[k] & 0x9fffu) | (code << 13);
+    }
+  }
+  if (bad) atomicOr(hdr + 4, bad);
+  rec[r] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
 }
 
 // One thread per label-quad record (x, y0, z0): bytes lab[z0+a][y0+b][x] at 2a + b.
@@ -1624,11 +1804,13 @@ int64_t packed_volume_bytes(int D, int H, int W, int dt, int layout) {
     case NDRR_LAYOUT_STENCIL: return stencil_records(D, H, W) * 8 * e;
     case NDRR_LAYOUT_QUAD: return quad_records(D, H, W) * 4 * e;
     case NDRR_LAYOUT_COEF: return kCoefHeader + stencil_records(D, H, W) * 8 * e;
+    case NDRR_LAYOUT_LCOEF: return dt == NDRR_F32 ? 0 : kCoefHeader + stencil_records(D, H, W) * 16;
     default: return 0;
   }
 }
 
 int64_t packed_label_bytes(int D, int H, int W, int layout) {
+  if (layout == NDRR_LAYOUT_LCOEF) return 0;  // the records carry the classes
   if (layout == NDRR_LAYOUT_QUAD) return quad_records(D, H, W) * 4;
   if (layout == NDRR_LAYOUT_STENCIL || layout == NDRR_LAYOUT_COEF)
     return static_cast<int64_t>(D + 4) * (H + 4) * (W + 4);
@@ -1637,7 +1819,7 @@ int64_t packed_label_bytes(int D, int H, int W, int layout) {
 
 bool valid_layout(int layout) {
   return layout == NDRR_LAYOUT_LINEAR || layout == NDRR_LAYOUT_STENCIL || layout == NDRR_LAYOUT_QUAD ||
-         layout == NDRR_LAYOUT_COEF;
+         layout == NDRR_LAYOUT_COEF || layout == NDRR_LAYOUT_LCOEF;
 }
 
 // The packed layouts' record index is formed with 24-bit multiplies (record_of), and the
@@ -1658,7 +1840,9 @@ int check_common(const void* vol, int vol_dtype, int vol_layout, const int8_t* l
   if (!valid_layout(vol_layout)) return NDRR_EINVAL;
   if (!valid_dtype(vol_dtype)) return NDRR_EDTYPE;
   if (C < 1 || C > NDRR_MAX_CLASSES) return NDRR_ECLASSES;
-  if (C > 1 && !lab) return NDRR_EINVAL;
+  if (vol_layout == NDRR_LAYOUT_LCOEF && vol_dtype == NDRR_F32) return NDRR_EDTYPE;
+  if (vol_layout == NDRR_LAYOUT_LCOEF && C > 4) return NDRR_ECLASSES;
+  if (C > 1 && !lab && vol_layout != NDRR_LAYOUT_LCOEF) return NDRR_EINVAL;
   if (B > 65535) return NDRR_ERANGE;
   if (vol_layout != NDRR_LAYOUT_LINEAR && (packed_volume_bytes(D, H, W, vol_dtype, vol_layout) >= (int64_t{1} << 31) ||
                                            packed_label_bytes(D, H, W, vol_layout) >= (int64_t{1} << 31) ||
@@ -1714,7 +1898,7 @@ void launch_fwd2(const Params& p, dim3 grid, hipStream_t st) {
   if (p.C == 1) launch_fwd3<DT, L, 1>(p, grid, st);
   else if (p.C == 2) launch_fwd3<DT, L, 2>(p, grid, st);
   else if (p.C <= 4) launch_fwd3<DT, L, 4>(p, grid, st);
-  else launch_fwd3<DT, L, 8>(p, grid, st);
+  else if constexpr (L != NDRR_LAYOUT_LCOEF) launch_fwd3<DT, L, 8>(p, grid, st);  // labelled records: C <= 4
 }
 
 template <int DT, int L, int CV>
@@ -1732,7 +1916,7 @@ void launch_bwd2(const Params& p, bool gcam, bool gvol, dim3 grid, hipStream_t s
   if (p.C == 1) launch_bwd3<DT, L, 1>(p, gcam, gvol, grid, st);
   else if (p.C == 2) launch_bwd3<DT, L, 2>(p, gcam, gvol, grid, st);
   else if (p.C <= 4) launch_bwd3<DT, L, 4>(p, gcam, gvol, grid, st);
-  else launch_bwd3<DT, L, 8>(p, gcam, gvol, grid, st);
+  else if constexpr (L != NDRR_LAYOUT_LCOEF) launch_bwd3<DT, L, 8>(p, gcam, gvol, grid, st);
 }
 
 template <int DT>
@@ -1740,7 +1924,9 @@ void launch_fwd1(const Params& p, int layout, dim3 grid, hipStream_t st) {
   if (layout == NDRR_LAYOUT_STENCIL) launch_fwd2<DT, NDRR_LAYOUT_STENCIL>(p, grid, st);
   else if (layout == NDRR_LAYOUT_QUAD) launch_fwd2<DT, NDRR_LAYOUT_QUAD>(p, grid, st);
   else if (layout == NDRR_LAYOUT_COEF) launch_fwd2<DT, NDRR_LAYOUT_COEF>(p, grid, st);
-  else launch_fwd2<DT, NDRR_LAYOUT_LINEAR>(p, grid, st);
+  else if (layout == NDRR_LAYOUT_LCOEF) {
+    if constexpr (DT != NDRR_F32) launch_fwd2<DT, NDRR_LAYOUT_LCOEF>(p, grid, st);  // 16-bit records only
+  } else launch_fwd2<DT, NDRR_LAYOUT_LINEAR>(p, grid, st);
 }
 
 template <int DT>
@@ -1748,7 +1934,30 @@ void launch_bwd1(const Params& p, int layout, bool gcam, bool gvol, dim3 grid, h
   if (layout == NDRR_LAYOUT_STENCIL) launch_bwd2<DT, NDRR_LAYOUT_STENCIL>(p, gcam, gvol, grid, st);
   else if (layout == NDRR_LAYOUT_QUAD) launch_bwd2<DT, NDRR_LAYOUT_QUAD>(p, gcam, gvol, grid, st);
   else if (layout == NDRR_LAYOUT_COEF) launch_bwd2<DT, NDRR_LAYOUT_COEF>(p, gcam, gvol, grid, st);
-  else launch_bwd2<DT, NDRR_LAYOUT_LINEAR>(p, gcam, gvol, grid, st);
+  else if (layout == NDRR_LAYOUT_LCOEF) {
+    if constexpr (DT != NDRR_F32) launch_bwd2<DT, NDRR_LAYOUT_LCOEF>(p, gcam, gvol, grid, st);
+  } else launch_bwd2<DT, NDRR_LAYOUT_LINEAR>(p, gcam, gvol, grid, st);
+}
+
+template <int DS>
+int pack_lcoef_from(const void* vol, const int8_t* lab, int n_classes, int record_dtype, int D, int H, int W,
+                    void* out, int64_t n, hipStream_t st) {
+  using T = typename Vox<DS>::T;
+  const T* v = static_cast<const T*>(vol);
+  uint32_t* hdr = static_cast<uint32_t*>(out);
+  uint4* rec = reinterpret_cast<uint4*>(static_cast<char*>(out) + kCoefHeader);
+  hipLaunchKernelGGL(k_coef_header_init, dim3(1), dim3(kCoefHeader / 4), 0, st, hdr);
+  const int64_t ablocks = std::min<int64_t>((n + kThreads - 1) / kThreads, 4096);
+  hipLaunchKernelGGL(k_lcoef_absmax<DS>, dim3(static_cast<unsigned>(ablocks)), dim3(kThreads), 0, st, v, D, H, W,
+                     n, hdr);
+  const dim3 grid(static_cast<unsigned>((n + kThreads - 1) / kThreads));
+  if (record_dtype == NDRR_BF16)
+    hipLaunchKernelGGL((k_pack_lcoef<DS, NDRR_BF16>), grid, dim3(kThreads), 0, st, v, lab, n_classes, hdr, rec, D, H,
+                       W, n);
+  else
+    hipLaunchKernelGGL((k_pack_lcoef<DS, NDRR_F16>), grid, dim3(kThreads), 0, st, v, lab, n_classes, hdr, rec, D, H,
+                       W, n);
+  return launch_status();
 }
 
 template <int DS>
@@ -1819,7 +2028,7 @@ static int raymarch_fwd_any(const void* vol, int vol_dtype, int vol_layout, cons
   Params p = make_params(vol, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width,
                          src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride);
   p.vol_bytes = static_cast<uint32_t>(packed_volume_bytes(D, H, W, vol_dtype, vol_layout));
-  p.vol_skip = vol_layout == NDRR_LAYOUT_COEF ? kCoefHeader : 0u;
+  p.vol_skip = is_coef(vol_layout) ? kCoefHeader : 0u;
   p.lab_bytes = n_classes > 1 ? static_cast<uint32_t>(packed_label_bytes(D, H, W, vol_layout)) : 0u;
   p.w2v = w2v;
   p.rt = rt;
@@ -1885,7 +2094,7 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_l
   Params p = make_params(vol, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width,
                          src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride);
   p.vol_bytes = static_cast<uint32_t>(packed_volume_bytes(D, H, W, vol_dtype, vol_layout));
-  p.vol_skip = vol_layout == NDRR_LAYOUT_COEF ? kCoefHeader : 0u;
+  p.vol_skip = is_coef(vol_layout) ? kCoefHeader : 0u;
   p.lab_bytes = n_classes > 1 ? static_cast<uint32_t>(packed_label_bytes(D, H, W, vol_layout)) : 0u;
   p.go = go;
   p.gpart = workspace;
@@ -1903,7 +2112,7 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_l
   if (r != NDRR_OK) return r;
   const int parts_per_tile = vol_layout == NDRR_LAYOUT_STENCIL ? bwd_parts_per_tile(NDRR_LAYOUT_STENCIL)
                              : vol_layout == NDRR_LAYOUT_QUAD  ? bwd_parts_per_tile(NDRR_LAYOUT_QUAD)
-                             : vol_layout == NDRR_LAYOUT_COEF  ? bwd_parts_per_tile(NDRR_LAYOUT_COEF)
+                             : is_coef(vol_layout)             ? bwd_parts_per_tile(NDRR_LAYOUT_COEF)
                                                                : bwd_parts_per_tile(NDRR_LAYOUT_LINEAR);
   hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(kReduceThreads), 0, st, workspace, gM,
                      p.nblocks * parts_per_tile, nullptr, nullptr, Se3Out{});
@@ -2115,6 +2324,21 @@ int ndrr_pack_coef(const void* vol, int vol_dtype, int record_dtype, int D, int 
     case NDRR_F32: return pack_coef_from<NDRR_F32>(vol, record_dtype, D, H, W, out, n, st);
     case NDRR_BF16: return pack_coef_from<NDRR_BF16>(vol, record_dtype, D, H, W, out, n, st);
     default: return pack_coef_from<NDRR_F16>(vol, record_dtype, D, H, W, out, n, st);
+  }
+}
+
+int ndrr_pack_coef_labels(const void* vol, int vol_dtype, const int8_t* lab, int n_classes, int record_dtype, int D,
+                          int H, int W, void* out, void* stream) {
+  if (!vol || !lab || !out || D <= 0 || H <= 0 || W <= 0) return NDRR_EINVAL;
+  if (!valid_dtype(vol_dtype) || (record_dtype != NDRR_BF16 && record_dtype != NDRR_F16)) return NDRR_EDTYPE;
+  if (n_classes < 1 || n_classes > 4) return NDRR_ECLASSES;
+  const int64_t n = stencil_records(D, H, W);
+  if (n >= (int64_t{1} << 31) || static_cast<int64_t>(D) * H * W >= (int64_t{1} << 31)) return NDRR_ERANGE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (vol_dtype) {
+    case NDRR_F32: return pack_lcoef_from<NDRR_F32>(vol, lab, n_classes, record_dtype, D, H, W, out, n, st);
+    case NDRR_BF16: return pack_lcoef_from<NDRR_BF16>(vol, lab, n_classes, record_dtype, D, H, W, out, n, st);
+    default: return pack_lcoef_from<NDRR_F16>(vol, lab, n_classes, record_dtype, D, H, W, out, n, st);
   }
 }
 

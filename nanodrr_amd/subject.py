@@ -45,6 +45,20 @@ def hu_to_mu(data, mu_water=MU_WATER, mu_bone=MU_BONE, hu_bone=HU_BONE):
     return torch.clamp(torch.where(hu > 0, above, below), min=0.0)
 
 
+class _Same:
+    """A cache key part equal only to a key part holding the same object (which it keeps alive)."""
+
+    __slots__ = ("obj",)
+
+    def __init__(self, obj) -> None:
+        self.obj = obj
+
+    def __eq__(self, other) -> bool:
+        return isinstance(other, _Same) and other.obj is self.obj
+
+    __hash__ = object.__hash__
+
+
 def _mu_property(name: str):
     """A HU -> mu parameter: changing its value drops the derived mu image (and
     what was made from it) and any `set_mu` override."""
@@ -157,11 +171,18 @@ class Subject(torch.nn.Module):
         return self._derived.get("w2v64", w, None, lambda: w.to(torch.float64).contiguous())
 
     def packed_volume(self, layout: str) -> torch.Tensor:
-        """`image` re-laid out for the kernels ("quad", "stencil" or "coef", see ndrr_pack_volume).
-        Coefficient records of a 16-bit `image` are made from `precise_image` and rounded once
-        (ndrr_pack_coef), so they carry the unrounded volume's edge differences."""
+        """`image` re-laid out for the kernels ("quad", "stencil", "coef" or "lcoef", see
+        ndrr_pack_volume). Coefficient records of a 16-bit `image` are made from `precise_image` and
+        rounded once (ndrr_pack_coef), so they carry the unrounded volume's edge differences; the
+        labelled ones ("lcoef", ndrr_pack_coef_labels) also carry `label_i8` (ValueError when the
+        volume or classes do not fit them: a negative value, more than 4 classes, fp32 storage)."""
         from ._ops import pack_coef, pack_volume
 
+        if layout == "lcoef":
+            value = self._labelled_coef()
+            if isinstance(value, ValueError):
+                raise ValueError(str(value))
+            return value
         img = self.image
         if layout == "coef":
             src = self.precise_image()
@@ -175,6 +196,31 @@ class Subject(torch.nn.Module):
         if torch.compiler.is_compiling():  # traced graphs re-pack
             return pack_volume(img[0, 0], layout)
         return self._derived.get("volume:" + layout, img, None, lambda: pack_volume(img[0, 0], layout))
+
+    def _labelled_coef(self):
+        """Labelled coefficient records of (precise_image, label_i8), or the ValueError saying why
+        they cannot hold them; either is kept until the volume or labels change."""
+        from ._ops import pack_coef_labels
+
+        img, src, lab, C = self.image, self.precise_image(), self.label_i8, self.n_classes
+
+        def make():
+            try:
+                if img.dtype not in (torch.bfloat16, torch.float16):
+                    raise ValueError(f"labelled coefficient records are 16-bit; the volume is stored in {img.dtype}")
+                if C > 4:
+                    raise ValueError(f"labelled coefficient records hold at most 4 classes, not {C}")
+                return pack_coef_labels(src[0, 0].detach().to(torch.float32), lab, C, img.dtype)
+            except ValueError as e:
+                return e
+
+        if torch.compiler.is_compiling():
+            return make()
+        return self._derived.get("volume:lcoef", src, (img.dtype, C, _Same(lab), lab._version), make)
+
+    def labelled_coef_fits(self) -> bool:
+        """Whether labelled coefficient records ("lcoef") can hold this subject (packs them once)."""
+        return not isinstance(self._labelled_coef(), ValueError)
 
     @property
     def label_i8(self) -> torch.Tensor:

@@ -101,3 +101,68 @@ def coef_expected(vol: np.ndarray, dtype) -> tuple[torch.Tensor, float]:
     if dtype == torch.bfloat16:
         slots[..., 0] = scaled[..., 0].to(torch.bfloat16).view(torch.int16)
     return slots, s
+
+
+LCOEF_TOP = np.float32(2.0 ** -7 - 2.0 ** -18)  # every labelled slot stays at or below this times 1
+
+
+def lcoef_codes(lab: np.ndarray) -> np.ndarray:
+    """(D+1, H+1, W+1, 8) 2-bit class codes of labelled coefficient records: corner j = x + 2y + 4z
+    of cell (z0, y0, x0) in [-1, D-1] x ..., lab & 3 inside the volume, 0 outside."""
+    lab = np.asarray(lab, np.int64)
+    D, H, W = lab.shape
+    P = np.zeros((D + 2, H + 2, W + 2), np.int64)
+    P[1:-1, 1:-1, 1:-1] = lab & 3
+    return np.stack([P[(j >> 2):(j >> 2) + D + 1, ((j >> 1) & 1):((j >> 1) & 1) + H + 1, (j & 1):(j & 1) + W + 1]
+                     for j in range(8)], axis=-1)
+
+
+def _as_int16(bits: torch.Tensor) -> torch.Tensor:
+    """16-bit patterns held in int32 (0 .. 65535) as int16."""
+    return ((bits + 32768) % 65536 - 32768).to(torch.int16)
+
+
+def lcoef_scale(coef: np.ndarray) -> tuple[float, np.float32]:
+    """(s, max|coefficient|) of labelled records: the largest power of two with max|coefficient| s
+    <= 2^-7 - 2^-18, i.e. 2^floor(log2) of that fp32 quotient (clamped to [2^-126, 2^126]); 1 for an
+    all-zero volume."""
+    m = np.float32(np.abs(coef).max())
+    if m == 0:
+        return 1.0, m
+    t = np.clip(np.float32(LCOEF_TOP / m), np.float32(2.0 ** -126), np.float32(2.0 ** 126))
+    bits = np.array([t], np.float32).view(np.uint32) & np.uint32(0x7F800000)
+    return float(bits.view(np.float32)[0]), m
+
+
+def lcoef_expected(vol: np.ndarray, lab: np.ndarray, dtype) -> tuple[torch.Tensor, float, np.float32]:
+    """(raw record slots as int16, scale s, max|coefficient|) a labelled coefficient pack of a not
+    negative `vol` and class map `lab` must produce: oracle.raymarch.coef_records in fp32, times s
+    (lcoef_scale), rounded to fp16 (c0 to bf16 for bf16 storage), with the corners' class codes
+    (lcoef_codes) in the slots' top exponent bits: bits 13-14 of an fp16 slot, 14-15 of a bf16 c0."""
+    coef = orm.coef_records(vol, np.float32)
+    s, m = lcoef_scale(coef)
+    scaled = torch.from_numpy(coef * np.float32(s))
+    codes = torch.from_numpy(lcoef_codes(lab)).to(torch.int32)
+    slots = (scaled.to(torch.float16).view(torch.int16).to(torch.int32) & 0x9FFF) | (codes << 13)
+    if dtype == torch.bfloat16:
+        c0 = scaled[..., 0].abs().to(torch.bfloat16).view(torch.int16).to(torch.int32) & 0xFFFF
+        slots[..., 0] = c0 | (codes[..., 0] << 14)
+    return _as_int16(slots), s, m
+
+
+def lcoef_decode(packed: torch.Tensor, dtype, shape) -> tuple[np.ndarray, np.ndarray]:
+    """((D+1, H+1, W+1, 8) float64 coefficients unscaled, (D+1, H+1, W+1, 8) class codes) that a
+    labelled coefficient buffer holds: the code bits masked out of each slot before decoding."""
+    D, H, W = shape
+    raw = packed[256:].cpu().view(torch.int16).reshape(D + 1, H + 1, W + 1, 8).to(torch.int32) & 0xFFFF
+    codes = (raw >> 13) & 3
+    vals = raw & 0x9FFF
+    if dtype == torch.bfloat16:
+        codes[..., 0] = (raw[..., 0] >> 14) & 3
+        vals[..., 0] = raw[..., 0] & 0x3FFF
+    as16 = _as_int16(vals)
+    half = as16.view(torch.float16).double()
+    if dtype == torch.bfloat16:
+        half[..., 0] = as16[..., 0].view(torch.bfloat16).double()
+    _, inv, _ = coef_header(packed)
+    return half.numpy() * inv, codes.numpy()

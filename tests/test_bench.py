@@ -17,7 +17,7 @@ def test_parse_case():
             bench.parse_case(bad)
     for case in bench.DEFAULT_CASES.split(","):
         bench.parse_case(case)
-    assert bench.DEFAULT_CASES.split(",")[0] == "bf16:coef:single"
+    assert bench.DEFAULT_CASES.split(",")[0] == "bf16:lcoef:single"
 
 
 def test_volume_gradient_bytes():

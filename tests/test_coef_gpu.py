@@ -29,8 +29,8 @@ from nanodrr_amd.phantom import detector, make_phantom, phantom_subject, sample_
 from nanodrr_amd.renderer import _make_src, _make_tgt
 from oracle import raymarch as orm
 from oracle import registration as oreg
-from tests.helpers import (coef_decode, coef_expected, coef_header, oracle_inputs, random_volume, ref_camera_np,
-                           rel_to_max)
+from tests.helpers import (coef_decode, coef_expected, coef_header, lcoef_decode, oracle_inputs, random_volume,
+                           ref_camera_np, rel_to_max)
 
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda")
@@ -136,7 +136,7 @@ def _zncc_go(img, tgt_img, C):
     return np.repeat(g[:, None, :], C, axis=1)
 
 
-def _accuracy(ph, C, dtype, det, det_mm, S, n_rays, ray_seed, pose_seed, bounds):
+def _accuracy(ph, C, dtype, det, det_mm, S, n_rays, ray_seed, pose_seed, bounds, layout="coef"):
     """Errors on `n_rays` sampled rays of 2 poses. "Clean" rays have no sample within 1e-4 voxel of a
     cell face or nearest-label tie, where the march is discontinuous: there an fp32 sample position
     (within ~1 ulp of the float64 one) may fall on the other side and change a derivative or a
@@ -146,14 +146,14 @@ def _accuracy(ph, C, dtype, det, det_mm, S, n_rays, ray_seed, pose_seed, bounds)
     rt_inv, _, _ = sample_poses(4, seed=pose_seed, isocenter=subject.isocenter)
     k_inv, sdd = detector(1020.0, det_mm, det, det)
     k_inv, sdd, rt_inv = k_inv.to(DEV), sdd.to(DEV), rt_inv.to(DEV)
-    packed = subject.packed_volume("coef")  # from the fp32 values, rounded once
-    lab_t = subject.packed_labels("coef")
+    packed = subject.packed_volume(layout)  # from the fp32 values, rounded once
+    lab_t = subject.packed_labels(layout)
     vol_t = subject.image[0, 0]
     M, src, tgt, step = _op_inputs(subject, k_inv, rt_inv, sdd, det, S)
     Ms = M[:2].contiguous()
     vol = ph.mu.double().numpy()
     lab = ph.labels.long().clamp(max=C - 1).numpy()
-    rec = _records(packed, dtype, vol.shape)
+    rec = _records(packed, dtype, vol.shape) if layout == "coef" else lcoef_decode(packed, dtype, vol.shape)[0]
     Mn, srcn = Ms.double().cpu().numpy(), src.double().cpu().numpy()
     rays = np.sort(np.random.default_rng(ray_seed).choice(det * det, size=n_rays, replace=False))
     dist = orm.discontinuity_distance(Mn, srcn, tgt.double().cpu().numpy(), S, vol.shape, rays, labels=C > 1)
@@ -164,7 +164,7 @@ def _accuracy(ph, C, dtype, det, det_mm, S, n_rays, ray_seed, pose_seed, bounds)
         idx = torch.as_tensor(rs, device=DEV)
         tsub, ssub = tgt[:, idx].contiguous(), step[:, idx].contiguous()
         tn, sn = tsub.double().cpu().numpy(), ssub.double().cpu().numpy()
-        out, jac = fused_raymarch_jac(vol_t, lab_t, Ms, src, tsub, ssub, S, C, 0, 6, packed, "coef")
+        out, jac = fused_raymarch_jac(vol_t, lab_t, Ms, src, tsub, ssub, S, C, 0, 6, packed, layout)
         img = out.double().cpu().numpy().sum(1)
         ref = orm.forward(vol, lab, C, Mn, srcn, tn, sn, S)
         pix = np.abs(ref.sum(1) - img) / np.abs(ref.sum(1)).max()
@@ -181,7 +181,7 @@ def _accuracy(ph, C, dtype, det, det_mm, S, n_rays, ray_seed, pose_seed, bounds)
             e[f"own gradient ({gname} go)"] = rel_to_max(
                 orm.backward(go, vol, lab, C, Mn, srcn, tn, sn, S, need_gvol=False, coef=rec)["gM"], gM)
         report[name] = e
-        print(f"\n{dtype} coef records, {name} rays ({len(rs)}/{n_rays}): vs float64 oracle of the fp32 volume: "
+        print(f"\n{dtype} {layout} records, {name} rays ({len(rs)}/{n_rays}): vs float64 oracle of the fp32 volume: "
               + ", ".join(f"{k} {v:.2e}" for k, v in e.items() if not k.startswith("own"))
               + "; kernel vs float64 oracle of its own records: "
               + ", ".join(f"{k[4:]} {v:.2e}" for k, v in e.items() if k.startswith("own")))
@@ -202,6 +202,18 @@ def _accuracy(ph, C, dtype, det, det_mm, S, n_rays, ray_seed, pose_seed, bounds)
 def test_coef_accuracy_benchmark_workload(dtype, bounds):
     """256^3 at 1 mm, 4 labels, 200 x 200 at 1 mm, S = 256: 2 poses x 400 sampled rays."""
     _accuracy(_phantom((256,) * 3, (1.0,) * 3), 4, dtype, 200, 1.0, 256, 400, 6, 1, bounds)
+
+
+@pytest.mark.parametrize("dtype, bounds", [
+    (torch.bfloat16, {"image max": 1e-3, "pose gradient (random go)": 8e-4, "pose gradient (ones go)": 8e-4,
+                      "pose gradient (zncc go)": 3e-4}),
+    (torch.float16, {"image max": 2e-4, "pose gradient (random go)": 8e-4, "pose gradient (ones go)": 8e-4,
+                     "pose gradient (zncc go)": 3e-4}),
+])
+def test_labelled_coef_accuracy_benchmark_workload(dtype, bounds):
+    """The same workload from labelled coefficient records (the renderer's choice for it): the
+    smaller scale makes differences below 1/128 of the largest coefficient fp16 subnormals."""
+    _accuracy(_phantom((256,) * 3, (1.0,) * 3), 4, dtype, 200, 1.0, 256, 400, 6, 1, bounds, "lcoef")
 
 
 @pytest.mark.parametrize("dtype, bounds", [
