@@ -77,6 +77,10 @@ constexpr int kWave = 64;        // rays per wave: an 8 x 8 detector patch (or 6
 #ifndef NDRR_IN_FLIGHT
 #define NDRR_IN_FLIGHT 4
 #endif
+// Labelled coefficient records: no label loads, so more record loads fit in flight.
+#ifndef NDRR_IN_FLIGHT_LCOEF
+#define NDRR_IN_FLIGHT_LCOEF 6
+#endif
 #ifndef NDRR_WAVES_PER_EU
 #define NDRR_WAVES_PER_EU 1
 #endif
@@ -706,7 +710,7 @@ constexpr uint32_t kLcoefMaskFp16 = 0x9FFF9FFFu;  // a word of two fp16 slots wi
 constexpr uint32_t kLcoefMaskBf16 = 0x9FFF3FFFu;  // the bf16 c0 / fp16 cx word without its class bits
 template <int DT> struct Fetch<DT, NDRR_LAYOUT_LCOEF> {
   static_assert(DT != NDRR_F32, "labelled coefficient records are 16-bit");
-  static constexpr int kInFlight = NDRR_IN_FLIGHT;
+  static constexpr int kInFlight = NDRR_IN_FLIGHT_LCOEF;
   uint4 q;
   __device__ inline void load(const Src& s, const Cell& c, bool live) { q = load16(s.vrs, record_offset(s, c, s.W + 1, 16u, live)); }
   __device__ inline Coef coef() const {

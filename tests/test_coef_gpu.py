@@ -227,3 +227,15 @@ def test_coef_accuracy_mixed_precision_workload(dtype, bounds):
     sums): 2 poses x 300 sampled rays."""
     _accuracy(_phantom((400, 512, 512), (0.8, 0.8, 1.0)), 2, dtype, 512, 0.8, 512, 300, 21, 11, bounds)
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("dtype, bounds", [
+    (torch.bfloat16, {"image max": 1e-3, "pose gradient (random go)": 9e-4, "pose gradient (ones go)": 8e-4,
+                      "pose gradient (zncc go)": 3e-4}),
+    (torch.float16, {"image max": 1e-4, "pose gradient (random go)": 9e-4, "pose gradient (ones go)": 8e-4,
+                     "pose gradient (zncc go)": 3e-4}),
+])
+def test_labelled_coef_accuracy_mixed_precision_workload(dtype, bounds):
+    """The mixed-precision workload from labelled coefficient records (the renderer's choice for it)."""
+    _accuracy(_phantom((400, 512, 512), (0.8, 0.8, 1.0)), 2, dtype, 512, 0.8, 512, 300, 21, 11, bounds, "lcoef")
+    torch.cuda.empty_cache()
