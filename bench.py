@@ -58,9 +58,11 @@ HBM_PEAK_GBS = 8000.0
 SPIN_CYCLES = 20_000_000  # ~10 ms of GPU clock cycles: longer than the host needs to enqueue the timed launches
 SIZE, SPACING, N_CLASSES, BATCH, DET, S = 256, 1.0, 4, 16, 200, 256
 GRAPH_ITERS = 10  # registration iterations per captured HIP graph
-# Layout of the registration renders (single channel): the quad records (136 MB at 256^3 bf16) stay in
-# the Infinity Cache; the coefficient records (271 MB) do not, and ran 25 % slower with the same convergence.
-REGISTRATION_LAYOUT = "quad"
+# Layout of the registration renders (single channel): quad coefficient records (fp16, 136 MB at 256^3)
+# stay in the Infinity Cache like the quad records and need fewer operations per sample (473 against 460
+# registrations/s, the same 57 / 64 converged); the coefficient records (271 MB) do not fit, and ran 25 %
+# slower with the same convergence.
+REGISTRATION_LAYOUT = "qcoef"
 
 
 DEFAULT_CASES = "bf16:lcoef:single,fp32:linear:two,fp32:linear:two+gvol,bf16:coef:single,bf16:quad:single," \
@@ -597,8 +599,9 @@ def measured_traffic(case: str, live_ms: dict, tolerance: float = 0.10, profiles
     return (out or None), ("; ".join(notes) or f"{path.name}, kernel source digest {digest}")
 
 
-def registration_leg(n_iters: int = 300, batch: int = 64):
-    """Batched registration on the 256^3 bf16 phantom: 64 problems with targets at +-30 deg /
+def registration_leg(n_iters: int = 300, batch: int = 64, dtype: torch.dtype = torch.bfloat16,
+                     layout: str = REGISTRATION_LAYOUT):
+    """Batched registration on the 256^3 phantom stored in `dtype` (bf16 by default): 64 problems with targets at +-30 deg /
     +-40 mm, starts perturbed +-10 deg / +-15 mm about the isocentre, 200 x 200 DRRs, -ZNCC
     loss, Adam (1e-2 rad, 1 mm), 300 iterations. Targets are rendered with S=1024 (a smooth
     stand-in for a radiograph); the optimised DRRs use S=256 with the sample lattice jittered
@@ -611,20 +614,18 @@ def registration_leg(n_iters: int = 300, batch: int = 64):
 
     dev = torch.device("cuda:0")
     ph = make_phantom((SIZE,) * 3, (SPACING,) * 3, seed=2)
-    subject = phantom_subject(ph, n_classes=1, dtype=torch.bfloat16, device=dev)
+    subject = phantom_subject(ph, n_classes=1, dtype=dtype, device=dev)
     prob = registration_problems(subject, batch, seed=3, height=DET, width=DET)
     k_inv, sdd = prob.k_inv.float().to(dev), prob.sdd.float().to(dev)
     with torch.no_grad():
         target = render(subject, k_inv, prob.rt_true.float().to(dev), sdd, DET, DET, n_samples=4 * S)
     torch.manual_seed(0)
-    reg = Registration(subject, prob.rt_init.float().to(dev), k_inv, sdd, DET, DET, per_pose=True,
-                       layout=REGISTRATION_LAYOUT)
+    reg = Registration(subject, prob.rt_init.float().to(dev), k_inv, sdd, DET, DET, per_pose=True, layout=layout)
     fid = prob.fiducials.to(dev)
     before = mtre(reg.pose.detach(), prob.rt_true.to(dev), fid)
     reg(S)  # builds the packed volume once, outside the timed loop
     # warm-up: a few iterations of the same loop on a copy of the problems (first launches, allocations)
-    warm = Registration(subject, prob.rt_init.float().to(dev), k_inv, sdd, DET, DET, per_pose=True,
-                        layout=REGISTRATION_LAYOUT)
+    warm = Registration(subject, prob.rt_init.float().to(dev), k_inv, sdd, DET, DET, per_pose=True, layout=layout)
     register(warm, target, n_iters=12, n_samples=S, jitter=True, seed=1, graph=True, iters_per_graph=GRAPH_ITERS)
     del warm
     torch.cuda.synchronize()
@@ -636,10 +637,10 @@ def registration_leg(n_iters: int = 300, batch: int = 64):
     after = mtre(reg.pose.detach(), prob.rt_true.to(dev), fid)
     ok = int((after < 1.0).sum())
     return {
-        "workload": f"{batch} problems, 256^3 bf16, 200x200, S={S} jittered (targets S={4 * S}), "
+        "workload": f"{batch} problems, 256^3 {str(dtype).replace('torch.', '')}, 200x200, S={S} jittered (targets S={4 * S}), "
                     f"targets +-30deg/+-40mm, starts +-10deg/+-15mm, Adam 1e-2 rad / 1 mm, {n_iters} iterations, "
                     f"fwd+bwd+Adam step captured in a HIP graph of {GRAPH_ITERS} iterations and replayed "
-                    f"(capture inside the timed call), {REGISTRATION_LAYOUT} volume layout",
+                    f"(capture inside the timed call), {layout} volume layout",
         "registrations_per_s": ok / wall,
         "converged": ok, "problems": batch, "wall_s": wall, "iterations_per_s": n_iters / wall,
         "mtre_mm_median_before": float(before.median()), "mtre_mm_median_after": float(after.median()),

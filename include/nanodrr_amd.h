@@ -24,12 +24,14 @@
  *   vol  the volume in NDRR_F32 / NDRR_BF16 / NDRR_F16, laid out as
  *        NDRR_LAYOUT_LINEAR:  [D][H][W], W fastest, or a packed copy made by
  *        ndrr_pack_volume / ndrr_pack_coef / ndrr_pack_coef_labels:
- *        NDRR_LAYOUT_QUAD, NDRR_LAYOUT_STENCIL, NDRR_LAYOUT_COEF or
- *        NDRR_LAYOUT_LCOEF records (vol_dtype is then the record type)
+ *        NDRR_LAYOUT_QUAD, NDRR_LAYOUT_STENCIL, NDRR_LAYOUT_COEF,
+ *        NDRR_LAYOUT_LCOEF or NDRR_LAYOUT_QCOEF records (vol_dtype is then
+ *        the record type; for NDRR_LAYOUT_QCOEF, whose records are fp16,
+ *        the 16-bit type of the volume they were made for)
  *   lab  int8 class per voxel (values outside [0, C) are dropped) in the form
  *        vol_layout reads: the [D][H][W] map for NDRR_LAYOUT_LINEAR, its padded
  *        copy for NDRR_LAYOUT_STENCIL and NDRR_LAYOUT_COEF and its label quads
- *        for NDRR_LAYOUT_QUAD (both by ndrr_pack_labels); ignored (may be NULL)
+ *        for NDRR_LAYOUT_QUAD and NDRR_LAYOUT_QCOEF (both by ndrr_pack_labels); ignored (may be NULL)
  *        for NDRR_LAYOUT_LCOEF, whose records carry the classes, and when
  *        n_classes == 1
  * Sample s of ray (b, n) sits at M[b] @ [lerp(src, tgt, s / (S - 1)), 1].
@@ -60,7 +62,8 @@ enum {
   NDRR_LAYOUT_STENCIL = 1, /* one record of 8 trilinear corners per cell */
   NDRR_LAYOUT_QUAD = 2,    /* one record of a (y, z) corner quad per x */
   NDRR_LAYOUT_COEF = 3,    /* one record of 8 multilinear coefficients per cell */
-  NDRR_LAYOUT_LCOEF = 4    /* the same with its 8 corners' classes (<= 4 classes) */
+  NDRR_LAYOUT_LCOEF = 4,   /* the same with its 8 corners' classes (<= 4 classes) */
+  NDRR_LAYOUT_QCOEF = 5    /* one record of a (y, z) quad's bilinear coefficients per x */
 };
 
 /* Argument errors (negative return codes). */
@@ -294,6 +297,18 @@ int ndrr_raymarch_bwd_jac_pose_noalloc(const float* go, int64_t go_batch_stride,
  *     fp32 bits of max|v|, s, 1/s and the record type. The edge differences
  *     behind the gradient are so rounded relative to their own size, not to the
  *     voxel values.
+ *   NDRR_LAYOUT_QCOEF (16-bit volumes): a 256-byte header like NDRR_LAYOUT_COEF's
+ *     (max|v|, s, 1/s, record type NDRR_F16), then one 8-byte record per quad
+ *     record (x, y0, z0) of NDRR_LAYOUT_QUAD's grid holding the bilinear
+ *     coefficients of that quad, a = v00, by = v01 - a, bz = v10 - a,
+ *     byz = (v11 - v10) - by (vzy zero-padded at x, fp32 in that order), times
+ *     s (NDRR_LAYOUT_COEF's rule: the largest power of two with
+ *     8 max|v| s <= 65504), rounded once to fp16. A cell is the two adjacent
+ *     records x0 and x0 + 1, v = lerp_x(Q(x0), Q(x0 + 1)): the quad layout's
+ *     footprint (half the coefficient layout's) with the y and z differences
+ *     rounded relative to their own size. ndrr_pack_volume makes it from a
+ *     volume of any vol_dtype (e.g. the fp32 values a bf16 volume was rounded
+ *     from); the march then takes the volume's 16-bit type as vol_dtype.
  * ndrr_packed_bytes gives the size of `out` (0 for invalid arguments; for
  * NDRR_LAYOUT_LINEAR the volume's own size); ndrr_pack_volume fills it from the
  * linear volume on `stream`, in records of the volume's type.

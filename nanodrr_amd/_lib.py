@@ -13,9 +13,9 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().with_name("libnanodrr_amd.so")
 
 NDRR_F32, NDRR_BF16, NDRR_F16 = 0, 1, 2
-LAYOUT_LINEAR, LAYOUT_STENCIL, LAYOUT_QUAD, LAYOUT_COEF, LAYOUT_LCOEF = 0, 1, 2, 3, 4
+LAYOUT_LINEAR, LAYOUT_STENCIL, LAYOUT_QUAD, LAYOUT_COEF, LAYOUT_LCOEF, LAYOUT_QCOEF = 0, 1, 2, 3, 4, 5
 LAYOUTS = {"linear": LAYOUT_LINEAR, "stencil": LAYOUT_STENCIL, "quad": LAYOUT_QUAD, "coef": LAYOUT_COEF,
-           "lcoef": LAYOUT_LCOEF}
+           "lcoef": LAYOUT_LCOEF, "qcoef": LAYOUT_QCOEF}
 LCOEF_NEGATIVE, LCOEF_CLASS, LCOEF_RANGE = 1, 2, 4  # header word 4 of labelled coefficient records
 MAX_CLASSES = 8
 

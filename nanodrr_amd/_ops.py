@@ -46,7 +46,8 @@ def _volume_arg(vol, packed, layout):
         return vol.data_ptr(), _lib.LAYOUT_LINEAR
     code = _lib.LAYOUTS.get(layout)
     if code is None or code == _lib.LAYOUT_LINEAR:
-        raise ValueError(f"layout must name the packed copy ('quad', 'stencil', 'coef' or 'lcoef'), got {layout!r}")
+        raise ValueError(f"layout must name the packed copy ('quad', 'stencil', 'coef', 'lcoef' or 'qcoef'), "
+                         f"got {layout!r}")
     D, H, W = vol.shape
     need = _lib.lib().ndrr_packed_bytes(D, H, W, _DTYPE_CODE[vol.dtype], code)
     if packed.dtype != torch.uint8 or packed.numel() != need or not packed.is_contiguous() \
@@ -57,7 +58,8 @@ def _volume_arg(vol, packed, layout):
 
 @torch.library.custom_op("nanodrr_amd::pack_volume", mutates_args=())
 def pack_volume(vol: torch.Tensor, layout: str) -> torch.Tensor:
-    """`layout` copy (uint8 bytes) of a (D, H, W) device volume (ndrr_pack_volume)."""
+    """`layout` copy (uint8 bytes) of a (D, H, W) device volume (ndrr_pack_volume). "qcoef" records are
+    fp16 whatever `vol`'s type (pass the unrounded values of a 16-bit volume)."""
     if not vol.is_cuda or vol.dim() != 3 or vol.dtype not in _DTYPE_CODE:
         raise ValueError("pack_volume needs a (D, H, W) float32/bfloat16/float16 device tensor")
     if layout not in _lib.LAYOUTS:
@@ -81,8 +83,11 @@ COEF_HEADER_BYTES = 256  # the coefficient layouts' header: max |v| (|coefficien
 def _packed_bytes(D, H, W, layout, dtype):
     e = 4 if dtype == torch.float32 else 2
     n = {"linear": D * H * W, "stencil": (D + 1) * (H + 1) * (W + 1) * 8, "quad": (D + 1) * (H + 1) * (W + 2) * 4,
-         "coef": (D + 1) * (H + 1) * (W + 1) * 8, "lcoef": (D + 1) * (H + 1) * (W + 1) * 8}[layout]
-    return n * e + (COEF_HEADER_BYTES if layout in ("coef", "lcoef") else 0)
+         "coef": (D + 1) * (H + 1) * (W + 1) * 8, "lcoef": (D + 1) * (H + 1) * (W + 1) * 8,
+         "qcoef": (D + 1) * (H + 1) * (W + 2) * 4}[layout]
+    if layout == "qcoef":
+        e = 2  # fp16 records whatever the source
+    return n * e + (COEF_HEADER_BYTES if layout in ("coef", "lcoef", "qcoef") else 0)
 
 
 @pack_volume.register_fake
@@ -190,7 +195,7 @@ def _(lab, layout):
     D, H, W = lab.shape
     if layout == "lcoef":
         return lab.new_empty(0)
-    if layout == "quad":
+    if layout in ("quad", "qcoef"):
         return torch.empty((D + 1) * (H + 1) * (W + 2) * 4, device=lab.device, dtype=torch.int8)
     if layout in ("stencil", "coef"):
         return torch.empty((D + 4) * (H + 4) * (W + 4), device=lab.device, dtype=torch.int8)

@@ -92,15 +92,19 @@ constexpr int kTileW = 8 * kTwx, kTileH = 8 * kTwy;                // tile size 
 constexpr int kTileRays = kWave * kTwx * kTwy;
 // The two coefficient layouts (plain and labelled records) march alike.
 constexpr bool is_coef(int layout) { return layout == NDRR_LAYOUT_COEF || layout == NDRR_LAYOUT_LCOEF; }
+// The quad grid's layouts (corner quads and their coefficient form) share records, labels and chunking.
+constexpr bool is_quad(int layout) { return layout == NDRR_LAYOUT_QUAD || layout == NDRR_LAYOUT_QCOEF; }
+// Layouts whose records follow a 256-byte header and hold the coefficients times a scale s.
+constexpr bool has_header(int layout) { return is_coef(layout) || layout == NDRR_LAYOUT_QCOEF; }
 constexpr int fwd_chunks(int layout) {
   return layout == NDRR_LAYOUT_STENCIL ? NDRR_FWD_CHUNKS_STENCIL
-         : layout == NDRR_LAYOUT_QUAD  ? NDRR_FWD_CHUNKS_QUAD
+         : is_quad(layout)             ? NDRR_FWD_CHUNKS_QUAD
          : is_coef(layout)             ? NDRR_FWD_CHUNKS_COEF
                                        : NDRR_FWD_CHUNKS_LINEAR;
 }
 constexpr int bwd_chunks(int layout) {
   return layout == NDRR_LAYOUT_STENCIL ? NDRR_BWD_CHUNKS_STENCIL
-         : layout == NDRR_LAYOUT_QUAD  ? NDRR_BWD_CHUNKS_QUAD
+         : is_quad(layout)             ? NDRR_BWD_CHUNKS_QUAD
          : is_coef(layout)             ? NDRR_BWD_CHUNKS_COEF
                                        : NDRR_BWD_CHUNKS_LINEAR;
 }
@@ -109,7 +113,7 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // carries the chunks' partial sums (and the single-pass adjoint's, 6 per
 // slot) stays small.
 constexpr int fwd_chunks(int layout, int cv) {
-  return cv == 1 && layout == NDRR_LAYOUT_QUAD   ? NDRR_FWD_CHUNKS_QUAD_1CLASS
+  return cv == 1 && is_quad(layout)             ? NDRR_FWD_CHUNKS_QUAD_1CLASS
          : cv == 1 && is_coef(layout)            ? NDRR_FWD_CHUNKS_COEF_1CLASS
                                                  : (cv <= 4 ? fwd_chunks(layout) : 1);
 }
@@ -614,6 +618,50 @@ template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_QUAD> {  // fp32: two 16-byte rec
   }
 };
 
+// Quad coefficient records (bf16 / fp16 volumes): on the quad grid, record
+// (x, y0, z0) holds the bilinear coefficients of the quad at x,
+//   Q(fy, fz) = a + by fy + bz fz + byz fy fz
+// (a = v(z0, y0), by = v(z0, y0+1) - a, bz = v(z0+1, y0) - a,
+// byz = (v(z0+1, y0+1) - v(z0+1, y0)) - by), each times a power of two s in
+// fp16 (8 bytes, after a kCoefHeader-byte header). A cell is the two adjacent
+// records x0 and x0 + 1 (one 16-byte load, as for quad): v = lerp_x(Q0, Q1).
+// The fp16 slots feed mixed-precision multiply-adds directly, so value and
+// gradient cost 14 operations with no unpacking, and the y and z differences
+// are rounded relative to their own size; the records keep the quad grid's
+// half-size footprint (the coefficient layout's is twice it).
+template <int DT> struct Fetch<DT, NDRR_LAYOUT_QCOEF> {
+  static_assert(DT != NDRR_F32, "quad coefficient records serve 16-bit volumes");
+  static constexpr int kInFlight = NDRR_IN_FLIGHT;
+  uint4 q;  // x, y: a, by, bz, byz at x0; z, w: the same at x0 + 1
+  __device__ inline void load(const Src& s, const Cell& c, bool live) {
+    q = load16(s.vrs, record_offset(s, c, s.W + 2, 8u, live));
+  }
+};
+
+struct QuadCoef {
+  float a, by, bz, byz;
+};
+__device__ inline QuadCoef quad_coef(uint32_t lo, uint32_t hi) { return {f16_lo(lo), f16_hi(lo), f16_lo(hi), f16_hi(hi)}; }
+
+// Q and its fy, fz derivatives: 4 multiply-adds.
+__device__ inline float quad_coef_value(const QuadCoef& k, const Cell& c, float& dy, float& dz) {
+  dz = fmaf(c.fy, k.byz, k.bz);
+  dy = fmaf(c.fz, k.byz, k.by);
+  return fmaf(c.fz, dz, fmaf(c.fy, k.by, k.a));
+}
+
+template <int DT>
+__device__ inline float qcoef_value_grad(const Fetch<DT, NDRR_LAYOUT_QCOEF>& f, const Cell& c, float& dvx, float& dvy,
+                                         float& dvz) {
+  float dy0, dz0, dy1, dz1;
+  const float q0 = quad_coef_value(quad_coef(f.q.x, f.q.y), c, dy0, dz0);
+  const float q1 = quad_coef_value(quad_coef(f.q.z, f.q.w), c, dy1, dz1);
+  dvx = q1 - q0;
+  dvy = fmaf(c.fx, dy1 - dy0, dy0);
+  dvz = fmaf(c.fx, dz1 - dz0, dz0);
+  return fmaf(c.fx, dvx, q0);
+}
+
 // a + f (b - a), rounded by one fma after the difference, so every kernel
 // that inlines it rounds the same way whatever else the compiler contracts
 // around it. The difference b - a is also the edge derivative the gradient
@@ -735,7 +783,7 @@ template <int DT> struct Fetch<DT, NDRR_LAYOUT_LCOEF> {
 
 // 1/s of the coefficient records (1 for the other layouts), read from the header.
 template <int L> __device__ inline float record_unscale(const Params& p) {
-  if constexpr (is_coef(L)) return static_cast<const float*>(p.vol)[2];
+  if constexpr (has_header(L)) return static_cast<const float*>(p.vol)[2];
   return 1.0f;
 }
 
@@ -801,6 +849,20 @@ __device__ inline void interp_grad(const Fetch<DT, NDRR_LAYOUT_COEF>& f, const C
 }
 template <int DT> __device__ inline float interp_value(const Fetch<DT, NDRR_LAYOUT_LCOEF>& f, const Cell& c) {
   return coef_value(f.coef(), c);
+}
+template <int DT> __device__ inline float interp_value(const Fetch<DT, NDRR_LAYOUT_QCOEF>& f, const Cell& c) {
+  float dvx, dvy, dvz;  // the derivatives share the value's products; unused ones are dropped
+  return qcoef_value_grad(f, c, dvx, dvy, dvz);
+}
+template <int DT>
+__device__ inline float interp_value_grad(const Fetch<DT, NDRR_LAYOUT_QCOEF>& f, const Cell& c, float& dvx,
+                                          float& dvy, float& dvz) {
+  return qcoef_value_grad(f, c, dvx, dvy, dvz);
+}
+template <int DT>
+__device__ inline void interp_grad(const Fetch<DT, NDRR_LAYOUT_QCOEF>& f, const Cell& c, float& dvx, float& dvy,
+                                   float& dvz) {
+  qcoef_value_grad(f, c, dvx, dvy, dvz);
 }
 template <int DT>
 __device__ inline float interp_value_grad(const Fetch<DT, NDRR_LAYOUT_LCOEF>& f, const Cell& c, float& dvx,
@@ -917,7 +979,7 @@ struct Batch {
       cell[u] = cell_of(x, y, z);
       f[u].load(s, cell[u], live);
       if constexpr (LABELS) {
-        if constexpr (L == NDRR_LAYOUT_QUAD)
+        if constexpr (is_quad(L))
           cls[u] = quad_label(s, cell[u], x, y, z, live);
         else if constexpr (L == NDRR_LAYOUT_LINEAR)
           cls[u] = nearest_label(s.lab, s.D, s.H, s.W, x, y, z, live);
@@ -1221,7 +1283,7 @@ void k_bwd(Params p) {
     }
   }
 
-  if constexpr (is_coef(L)) {
+  if constexpr (has_header(L)) {
     const float unscale = record_unscale<L>(p);
     Gx *= unscale; Gy *= unscale; Gz *= unscale;
     Tx *= unscale; Ty *= unscale; Tz *= unscale;
@@ -1528,6 +1590,11 @@ __global__ void k_pack_quad(const typename Vox<DT>::T* __restrict__ vol, void* _
 
 // fp32 -> the record type, rounded to nearest even once.
 template <int DT> struct Round;
+template <> struct Round<NDRR_F16> {
+  __device__ static inline uint32_t bits(float f) {
+    return __builtin_bit_cast(uint16_t, static_cast<_Float16>(f));
+  }
+};
 template <> struct Round<NDRR_F32> {
   __device__ static inline uint32_t bits(float f) { return __float_as_uint(f); }
 };
@@ -1540,12 +1607,6 @@ template <> struct Round<NDRR_BF16> {
     return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
   }
 };
-template <> struct Round<NDRR_F16> {
-  __device__ static inline uint32_t bits(float f) {
-    return __builtin_bit_cast(uint16_t, static_cast<_Float16>(f));
-  }
-};
-
 // The coefficient layout's header: [0] the volume's largest |value| (fp32
 // bits, an atomic maximum), [1] the scale s of the 16-bit records, [2] 1/s,
 // [3] the record type; kCoefHeader bytes in all.
@@ -1741,6 +1802,43 @@ __global__ void k_pack_lcoef(const typename Vox<DS>::T* __restrict__ vol, const 
   rec[r] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
 }
 
+// One thread per quad coefficient record (x, y0, z0) on the quad grid: the
+// quad's four zero-padded corners (as fp32) become a, by, bz, byz in fp32 in
+// that order, times the header's scale s (the coefficient layout's, from
+// max|v|: |coefficient| <= 4 max|v|), each rounded once to fp16.
+template <int DS>
+__global__ void k_pack_qcoef(const typename Vox<DS>::T* __restrict__ vol, uint32_t* __restrict__ hdr,
+                             uint2* __restrict__ rec, int D, int H, int W, int64_t n_rec) {
+#pragma clang fp contract(off)
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const float scale = coef_scale<NDRR_F16>(hdr[0]);
+  if (r == 0) {
+    hdr[1] = __float_as_uint(scale);
+    hdr[2] = __float_as_uint(1.0f / scale);  // exact: a power of two
+    hdr[3] = static_cast<uint32_t>(NDRR_F16);
+  }
+  if (r >= n_rec) return;
+  const int cx = static_cast<int>(r % (W + 2));
+  const int64_t rest = r / (W + 2);
+  const int cy = static_cast<int>(rest % (H + 1));
+  const int cz = static_cast<int>(rest / (H + 1));
+  float v[4];  // v[2a + b] = (z0 + a, y0 + b) at x
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int x = cx - 1, y = cy - 1 + (k & 1), z = cz - 1 + (k >> 1);
+    const bool inb = static_cast<unsigned>(x) < static_cast<unsigned>(W) &&
+                     static_cast<unsigned>(y) < static_cast<unsigned>(H) &&
+                     static_cast<unsigned>(z) < static_cast<unsigned>(D);
+    v[k] = inb ? Vox<DS>::ld(vol, static_cast<int>((static_cast<int64_t>(z) * H + y) * W + x)) : 0.0f;
+  }
+  const float by = v[1] - v[0];
+  const float c[4] = {v[0], by, v[2] - v[0], (v[3] - v[2]) - by};
+  uint32_t h[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) h[k] = Round<NDRR_F16>::bits(c[k] * scale);  // exact product: a power of two
+  rec[r] = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+}
+
 // One thread per label-quad record (x, y0, z0): bytes lab[z0+a][y0+b][x] at 2a + b.
 __global__ void k_pack_label_quads(const int8_t* __restrict__ lab, uint32_t* __restrict__ out,
                                    int D, int H, int W, int64_t n_rec) {
@@ -1809,13 +1907,14 @@ int64_t packed_volume_bytes(int D, int H, int W, int dt, int layout) {
     case NDRR_LAYOUT_QUAD: return quad_records(D, H, W) * 4 * e;
     case NDRR_LAYOUT_COEF: return kCoefHeader + stencil_records(D, H, W) * 8 * e;
     case NDRR_LAYOUT_LCOEF: return dt == NDRR_F32 ? 0 : kCoefHeader + stencil_records(D, H, W) * 16;
+    case NDRR_LAYOUT_QCOEF: return kCoefHeader + quad_records(D, H, W) * 8;  // fp16 records whatever the source
     default: return 0;
   }
 }
 
 int64_t packed_label_bytes(int D, int H, int W, int layout) {
   if (layout == NDRR_LAYOUT_LCOEF) return 0;  // the records carry the classes
-  if (layout == NDRR_LAYOUT_QUAD) return quad_records(D, H, W) * 4;
+  if (is_quad(layout)) return quad_records(D, H, W) * 4;
   if (layout == NDRR_LAYOUT_STENCIL || layout == NDRR_LAYOUT_COEF)
     return static_cast<int64_t>(D + 4) * (H + 4) * (W + 4);
   return static_cast<int64_t>(D) * H * W;
@@ -1823,7 +1922,7 @@ int64_t packed_label_bytes(int D, int H, int W, int layout) {
 
 bool valid_layout(int layout) {
   return layout == NDRR_LAYOUT_LINEAR || layout == NDRR_LAYOUT_STENCIL || layout == NDRR_LAYOUT_QUAD ||
-         layout == NDRR_LAYOUT_COEF || layout == NDRR_LAYOUT_LCOEF;
+         layout == NDRR_LAYOUT_COEF || layout == NDRR_LAYOUT_LCOEF || layout == NDRR_LAYOUT_QCOEF;
 }
 
 // The packed layouts' record index is formed with 24-bit multiplies (record_of), and the
@@ -1844,7 +1943,7 @@ int check_common(const void* vol, int vol_dtype, int vol_layout, const int8_t* l
   if (!valid_layout(vol_layout)) return NDRR_EINVAL;
   if (!valid_dtype(vol_dtype)) return NDRR_EDTYPE;
   if (C < 1 || C > NDRR_MAX_CLASSES) return NDRR_ECLASSES;
-  if (vol_layout == NDRR_LAYOUT_LCOEF && vol_dtype == NDRR_F32) return NDRR_EDTYPE;
+  if ((vol_layout == NDRR_LAYOUT_LCOEF || vol_layout == NDRR_LAYOUT_QCOEF) && vol_dtype == NDRR_F32) return NDRR_EDTYPE;
   if (vol_layout == NDRR_LAYOUT_LCOEF && C > 4) return NDRR_ECLASSES;
   if (C > 1 && !lab && vol_layout != NDRR_LAYOUT_LCOEF) return NDRR_EINVAL;
   if (B > 65535) return NDRR_ERANGE;
@@ -1929,7 +2028,9 @@ void launch_fwd1(const Params& p, int layout, dim3 grid, hipStream_t st) {
   else if (layout == NDRR_LAYOUT_QUAD) launch_fwd2<DT, NDRR_LAYOUT_QUAD>(p, grid, st);
   else if (layout == NDRR_LAYOUT_COEF) launch_fwd2<DT, NDRR_LAYOUT_COEF>(p, grid, st);
   else if (layout == NDRR_LAYOUT_LCOEF) {
-    if constexpr (DT != NDRR_F32) launch_fwd2<DT, NDRR_LAYOUT_LCOEF>(p, grid, st);  // 16-bit records only
+    if constexpr (DT != NDRR_F32) launch_fwd2<DT, NDRR_LAYOUT_LCOEF>(p, grid, st);  // 16-bit volumes only
+  } else if (layout == NDRR_LAYOUT_QCOEF) {
+    if constexpr (DT != NDRR_F32) launch_fwd2<DT, NDRR_LAYOUT_QCOEF>(p, grid, st);
   } else launch_fwd2<DT, NDRR_LAYOUT_LINEAR>(p, grid, st);
 }
 
@@ -1940,7 +2041,24 @@ void launch_bwd1(const Params& p, int layout, bool gcam, bool gvol, dim3 grid, h
   else if (layout == NDRR_LAYOUT_COEF) launch_bwd2<DT, NDRR_LAYOUT_COEF>(p, gcam, gvol, grid, st);
   else if (layout == NDRR_LAYOUT_LCOEF) {
     if constexpr (DT != NDRR_F32) launch_bwd2<DT, NDRR_LAYOUT_LCOEF>(p, gcam, gvol, grid, st);
+  } else if (layout == NDRR_LAYOUT_QCOEF) {
+    if constexpr (DT != NDRR_F32) launch_bwd2<DT, NDRR_LAYOUT_QCOEF>(p, gcam, gvol, grid, st);
   } else launch_bwd2<DT, NDRR_LAYOUT_LINEAR>(p, gcam, gvol, grid, st);
+}
+
+template <int DS>
+int pack_qcoef_from(const void* vol, int D, int H, int W, void* out, int64_t n, hipStream_t st) {
+  using T = typename Vox<DS>::T;
+  const T* v = static_cast<const T*>(vol);
+  uint32_t* hdr = static_cast<uint32_t*>(out);
+  uint2* rec = reinterpret_cast<uint2*>(static_cast<char*>(out) + kCoefHeader);
+  const int64_t n_vox = static_cast<int64_t>(D) * H * W;
+  hipLaunchKernelGGL(k_coef_header_init, dim3(1), dim3(kCoefHeader / 4), 0, st, hdr);
+  const int64_t ablocks = std::min<int64_t>((n_vox + kThreads - 1) / kThreads, 4096);
+  hipLaunchKernelGGL(k_coef_absmax<DS>, dim3(static_cast<unsigned>(ablocks)), dim3(kThreads), 0, st, v, n_vox, hdr);
+  hipLaunchKernelGGL(k_pack_qcoef<DS>, dim3(static_cast<unsigned>((n + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                     st, v, hdr, rec, D, H, W, n);
+  return launch_status();
 }
 
 template <int DS>
@@ -2032,7 +2150,7 @@ static int raymarch_fwd_any(const void* vol, int vol_dtype, int vol_layout, cons
   Params p = make_params(vol, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width,
                          src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride);
   p.vol_bytes = static_cast<uint32_t>(packed_volume_bytes(D, H, W, vol_dtype, vol_layout));
-  p.vol_skip = is_coef(vol_layout) ? kCoefHeader : 0u;
+  p.vol_skip = has_header(vol_layout) ? kCoefHeader : 0u;
   p.lab_bytes = n_classes > 1 ? static_cast<uint32_t>(packed_label_bytes(D, H, W, vol_layout)) : 0u;
   p.w2v = w2v;
   p.rt = rt;
@@ -2098,7 +2216,7 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_l
   Params p = make_params(vol, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width,
                          src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride);
   p.vol_bytes = static_cast<uint32_t>(packed_volume_bytes(D, H, W, vol_dtype, vol_layout));
-  p.vol_skip = is_coef(vol_layout) ? kCoefHeader : 0u;
+  p.vol_skip = has_header(vol_layout) ? kCoefHeader : 0u;
   p.lab_bytes = n_classes > 1 ? static_cast<uint32_t>(packed_label_bytes(D, H, W, vol_layout)) : 0u;
   p.go = go;
   p.gpart = workspace;
@@ -2115,7 +2233,7 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_l
   int r = launch_status();
   if (r != NDRR_OK) return r;
   const int parts_per_tile = vol_layout == NDRR_LAYOUT_STENCIL ? bwd_parts_per_tile(NDRR_LAYOUT_STENCIL)
-                             : vol_layout == NDRR_LAYOUT_QUAD  ? bwd_parts_per_tile(NDRR_LAYOUT_QUAD)
+                             : is_quad(vol_layout)             ? bwd_parts_per_tile(NDRR_LAYOUT_QUAD)
                              : is_coef(vol_layout)             ? bwd_parts_per_tile(NDRR_LAYOUT_COEF)
                                                                : bwd_parts_per_tile(NDRR_LAYOUT_LINEAR);
   hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(kReduceThreads), 0, st, workspace, gM,
@@ -2290,6 +2408,15 @@ int ndrr_pack_volume(const void* vol, int vol_dtype, int vol_layout, int D, int 
     return e == hipSuccess ? NDRR_OK : static_cast<int>(e);
   }
   if (vol_layout == NDRR_LAYOUT_COEF) return ndrr_pack_coef(vol, vol_dtype, vol_dtype, D, H, W, out, stream);
+  if (vol_layout == NDRR_LAYOUT_QCOEF) {
+    const int64_t n = quad_records(D, H, W);
+    if (n >= (int64_t{1} << 31) || static_cast<int64_t>(D) * H * W >= (int64_t{1} << 31)) return NDRR_ERANGE;
+    switch (vol_dtype) {
+      case NDRR_F32: return pack_qcoef_from<NDRR_F32>(vol, D, H, W, out, n, st);
+      case NDRR_BF16: return pack_qcoef_from<NDRR_BF16>(vol, D, H, W, out, n, st);
+      default: return pack_qcoef_from<NDRR_F16>(vol, D, H, W, out, n, st);
+    }
+  }
   if (vol_layout != NDRR_LAYOUT_STENCIL && vol_layout != NDRR_LAYOUT_QUAD) return NDRR_EINVAL;
   const bool quad = vol_layout == NDRR_LAYOUT_QUAD;
   const int64_t n = quad ? quad_records(D, H, W) : stencil_records(D, H, W);
@@ -2366,7 +2493,7 @@ int ndrr_pack_labels(const int8_t* lab, int vol_layout, int D, int H, int W, voi
                        dim3(kThreads), 0, st, lab, static_cast<int8_t*>(out), D, H, W, n);
     return launch_status();
   }
-  if (vol_layout != NDRR_LAYOUT_QUAD) return NDRR_EINVAL;
+  if (!is_quad(vol_layout)) return NDRR_EINVAL;
   const int64_t n = quad_records(D, H, W);
   if (n >= (int64_t{1} << 31)) return NDRR_ERANGE;
   hipLaunchKernelGGL(k_pack_label_quads, dim3(static_cast<unsigned>((n + kThreads - 1) / kThreads)),

@@ -26,7 +26,10 @@ _BACKENDS = ("auto", "hip", "triton")
 # from the unrounded volume: the gradient's edge differences rounded relative
 # to their own size), "lcoef" (coefficient records that also carry the classes
 # of their cell's corners, so the march loads no label map: 16-bit storage,
-# 2 to 4 classes, a volume that is not negative), "linear" (the (D, H, W)
+# 2 to 4 classes, a volume that is not negative), "qcoef" (the quad grid's
+# records holding each (y, z) quad's bilinear coefficients in fp16, made from
+# the unrounded volume: quad's footprint, fewer operations per sample than
+# quad and more accurate than bf16 quad; 16-bit storage), "linear" (the (D, H, W)
 # volume itself) or "auto": linear when the volume requires grad (a packed
 # copy would be re-made after every update), else for 16-bit storage lcoef
 # where it holds the subject (11-13% faster than coef at 4 classes, 1.7x at a
@@ -34,7 +37,7 @@ _BACKENDS = ("auto", "hip", "triton")
 # otherwise (as fast as quad, with ~10x smaller gradient errors), and quad
 # for fp32 (the fastest measured).
 VOLUME_LAYOUT = "auto"
-_LAYOUTS = ("linear", "quad", "stencil", "coef", "lcoef")
+_LAYOUTS = ("linear", "quad", "stencil", "coef", "lcoef", "qcoef")
 
 # How pose and camera gradients are formed when the volume needs none:
 # "single_pass" (the forward march also stores per-ray sums and the backward

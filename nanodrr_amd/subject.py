@@ -171,7 +171,7 @@ class Subject(torch.nn.Module):
         return self._derived.get("w2v64", w, None, lambda: w.to(torch.float64).contiguous())
 
     def packed_volume(self, layout: str) -> torch.Tensor:
-        """`image` re-laid out for the kernels ("quad", "stencil", "coef" or "lcoef", see
+        """`image` re-laid out for the kernels ("quad", "stencil", "coef", "lcoef" or "qcoef", see
         ndrr_pack_volume). Coefficient records of a 16-bit `image` are made from `precise_image` and
         rounded once (ndrr_pack_coef), so they carry the unrounded volume's edge differences; the
         labelled ones ("lcoef", ndrr_pack_coef_labels) also carry `label_i8` (ValueError when the
@@ -184,6 +184,15 @@ class Subject(torch.nn.Module):
                 raise ValueError(str(value))
             return value
         img = self.image
+        if layout == "qcoef":  # fp16 records from the unrounded values
+            src = self.precise_image()
+
+            def make_q():
+                return pack_volume(src[0, 0].detach().to(torch.float32), "qcoef")
+
+            if torch.compiler.is_compiling():
+                return make_q()
+            return self._derived.get("volume:qcoef", src, None, make_q)
         if layout == "coef":
             src = self.precise_image()
             if src is not img:

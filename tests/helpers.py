@@ -150,6 +150,46 @@ def lcoef_expected(vol: np.ndarray, lab: np.ndarray, dtype) -> tuple[torch.Tenso
     return _as_int16(slots), s, m
 
 
+def quad_coef_records(vol: np.ndarray, dtype=np.float32) -> np.ndarray:
+    """(D+1, H+1, W+2, 4) quad coefficient records of the zero-padded volume, one per quad record
+    (z0, y0, x) in [-1, D-1] x [-1, H-1] x [-1, W]: [a, by, bz, byz] of the (y, z) quad at x, formed
+    with the arithmetic of `dtype` in the order ndrr_pack_volume documents (by first)."""
+    vol = np.asarray(vol, dtype)
+    D, H, W = vol.shape
+    P = np.zeros((D + 2, H + 2, W + 2), dtype)
+    P[1:-1, 1:-1, 1:-1] = vol
+    v = [P[a:a + D + 1, b:b + H + 1, :] for a in (0, 1) for b in (0, 1)]  # v[2a + b]
+    by = v[1] - v[0]
+    return np.stack([v[0], by, v[2] - v[0], (v[3] - v[2]) - by], axis=-1)
+
+
+def qcoef_expected(vol: np.ndarray) -> tuple[torch.Tensor, float]:
+    """(raw record slots as int16, scale s) of a quad coefficient pack of `vol`: quad_coef_records in
+    fp32 times s (the coefficient layout's rule from max|v|), rounded to fp16."""
+    rec = quad_coef_records(vol, np.float32)
+    m = np.float32(np.abs(np.asarray(vol, np.float32)).max())
+    s = 1.0 if m == 0 else float(2.0 ** np.floor(np.log2(np.float32(8188.0) / m)))
+    return torch.from_numpy(rec * np.float32(s)).to(torch.float16).view(torch.int16), s
+
+
+def qcoef_decode(packed: torch.Tensor, shape) -> np.ndarray:
+    """(D+1, H+1, W+2, 4) float64 quad coefficients a quad coefficient buffer holds, unscaled."""
+    D, H, W = shape
+    _, inv, _ = coef_header(packed)
+    return packed[256:].cpu().view(torch.float16).reshape(D + 1, H + 1, W + 2, 4).double().numpy() * inv
+
+
+def qcoef_to_coef(q: np.ndarray) -> np.ndarray:
+    """The coefficient records (oracle.raymarch.coef_records' layout, float64) of the interpolant
+    lerp_x(Q(x0), Q(x0 + 1)) that quad coefficient records define: c0 = a0, cx = a1 - a0,
+    cy = by0, cxy = by1 - by0, cz = bz0, cxz = bz1 - bz0, cyz = byz0, cxyz = byz1 - byz0."""
+    q = np.asarray(q, np.float64)
+    q0, q1 = q[:, :, :-1], q[:, :, 1:]
+    a0, by0, bz0, byz0 = (q0[..., k] for k in range(4))
+    a1, by1, bz1, byz1 = (q1[..., k] for k in range(4))
+    return np.stack([a0, a1 - a0, by0, by1 - by0, bz0, bz1 - bz0, byz0, byz1 - byz0], axis=-1)
+
+
 def lcoef_decode(packed: torch.Tensor, dtype, shape) -> tuple[np.ndarray, np.ndarray]:
     """((D+1, H+1, W+1, 8) float64 coefficients unscaled, (D+1, H+1, W+1, 8) class codes) that a
     labelled coefficient buffer holds: the code bits masked out of each slot before decoding."""

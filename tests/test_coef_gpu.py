@@ -29,8 +29,8 @@ from nanodrr_amd.phantom import detector, make_phantom, phantom_subject, sample_
 from nanodrr_amd.renderer import _make_src, _make_tgt
 from oracle import raymarch as orm
 from oracle import registration as oreg
-from tests.helpers import (coef_decode, coef_expected, coef_header, lcoef_decode, oracle_inputs, random_volume,
-                           ref_camera_np, rel_to_max)
+from tests.helpers import (coef_decode, coef_expected, coef_header, lcoef_decode, oracle_inputs, qcoef_decode,
+                           qcoef_to_coef, random_volume, ref_camera_np, rel_to_max)
 
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda")
@@ -153,7 +153,8 @@ def _accuracy(ph, C, dtype, det, det_mm, S, n_rays, ray_seed, pose_seed, bounds,
     Ms = M[:2].contiguous()
     vol = ph.mu.double().numpy()
     lab = ph.labels.long().clamp(max=C - 1).numpy()
-    rec = _records(packed, dtype, vol.shape) if layout == "coef" else lcoef_decode(packed, dtype, vol.shape)[0]
+    rec = {"coef": lambda: _records(packed, dtype, vol.shape), "lcoef": lambda: lcoef_decode(packed, dtype, vol.shape)[0],
+           "qcoef": lambda: qcoef_to_coef(qcoef_decode(packed, vol.shape))}[layout]()
     Mn, srcn = Ms.double().cpu().numpy(), src.double().cpu().numpy()
     rays = np.sort(np.random.default_rng(ray_seed).choice(det * det, size=n_rays, replace=False))
     dist = orm.discontinuity_distance(Mn, srcn, tgt.double().cpu().numpy(), S, vol.shape, rays, labels=C > 1)
@@ -214,6 +215,16 @@ def test_labelled_coef_accuracy_benchmark_workload(dtype, bounds):
     """The same workload from labelled coefficient records (the renderer's choice for it): the
     smaller scale makes differences below 1/128 of the largest coefficient fp16 subnormals."""
     _accuracy(_phantom((256,) * 3, (1.0,) * 3), 4, dtype, 200, 1.0, 256, 400, 6, 1, bounds, "lcoef")
+
+
+@pytest.mark.parametrize("dtype, bounds", [
+    (torch.bfloat16, {"image max": 2e-4, "pose gradient (random go)": 5e-3, "pose gradient (ones go)": 2.5e-3,
+                      "pose gradient (zncc go)": 1.2e-3}),
+])
+def test_quad_coef_accuracy_benchmark_workload(dtype, bounds):
+    """The same workload from quad coefficient records (fp16, the registration's layout): the y and z
+    differences are stored, the x difference is taken from two fp16 values."""
+    _accuracy(_phantom((256,) * 3, (1.0,) * 3), 4, dtype, 200, 1.0, 256, 400, 6, 1, bounds, "qcoef")
 
 
 @pytest.mark.parametrize("dtype, bounds", [
