@@ -473,6 +473,25 @@ int ndrr_adam_step_scheduled(float* param, const float* grad, float* m, float* v
 int ndrr_counter_advance(int* counter, void* stream);
 
 /*
+ * The same loop state in fewer launches (each result bit for bit the separate
+ * entries'; no counterpart in the reference, as above):
+ *   ndrr_adam_step_scheduled_pair: ndrr_adam_step_scheduled on two parameter
+ *     groups of n values each (group k with learning-rate column k) followed
+ *     by ndrr_counter_advance on *iteration, in one launch of one workgroup
+ *     (every thread reads the counter before a barrier; one thread advances
+ *     it after); n <= 2^20.
+ *   ndrr_pose_compose_lattice: ndrr_pose_compose and ndrr_lattice_offsets
+ *     (offsets[b] for the same B problems) in one launch.
+ */
+int ndrr_adam_step_scheduled_pair(float* param0, const float* grad0, float* m0, float* v0,
+                                  float* param1, const float* grad1, float* m1, float* v1, int n,
+                                  const float* schedule, int rows, int* iteration, double beta1,
+                                  double beta2, float eps, void* stream);
+int ndrr_pose_compose_lattice(const double* prefix, const double* suffix, const float* rot,
+                              const float* xyz, int64_t param_stride, float* M, int B,
+                              const int* iteration, uint32_t seed, float* offsets, void* stream);
+
+/*
  * Zero-normalised cross-correlation of B image pairs of N pixels (fp32):
  * score[b] = mean(xh * yh), xh = (x - mean x) / sqrt(var x + eps) (population
  * variance), yh likewise; grad_x (optional, [B][N]) = d score[b] / d x.

@@ -106,6 +106,10 @@ SIGNATURES: dict[str, tuple] = {
     "ndrr_adam_step_scheduled": (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_int, _c_p, _c_int, _c_int, _c_p,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_float, _c_p]),
     "ndrr_counter_advance": (_c_int, [_c_p, _c_p]),
+    "ndrr_adam_step_scheduled_pair": (_c_int, [_c_p] * 8 + [_c_int, _c_p, _c_int, _c_p, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_float, _c_p]),
+    "ndrr_pose_compose_lattice": (_c_int, [_c_p, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_int, _c_p, ctypes.c_uint32,
+                                           _c_p, _c_p]),
     "ndrr_zncc_workspace_floats": (ctypes.c_int64, [_c_int, _c_int]),
     "ndrr_zncc": (_c_int, [_c_p, _c_p, _c_int, _c_int, ctypes.c_float, _c_p, _c_p, _c_p, _c_p]),
     "ndrr_patch_zncc_workspace_floats": (ctypes.c_int64, [_c_int, _c_int, _c_int, _c_int]),

@@ -81,6 +81,9 @@ constexpr int kWave = 64;        // rays per wave: an 8 x 8 detector patch (or 6
 #ifndef NDRR_IN_FLIGHT_LCOEF
 #define NDRR_IN_FLIGHT_LCOEF 6
 #endif
+#ifndef NDRR_IN_FLIGHT_QCOEF
+#define NDRR_IN_FLIGHT_QCOEF 4
+#endif
 #ifndef NDRR_WAVES_PER_EU
 #define NDRR_WAVES_PER_EU 1
 #endif
@@ -631,7 +634,7 @@ template <> struct Fetch<NDRR_F32, NDRR_LAYOUT_QUAD> {  // fp32: two 16-byte rec
 // half-size footprint (the coefficient layout's is twice it).
 template <int DT> struct Fetch<DT, NDRR_LAYOUT_QCOEF> {
   static_assert(DT != NDRR_F32, "quad coefficient records serve 16-bit volumes");
-  static constexpr int kInFlight = NDRR_IN_FLIGHT;
+  static constexpr int kInFlight = NDRR_IN_FLIGHT_QCOEF;
   uint4 q;  // x, y: a, by, bz, byz at x0; z, w: the same at x0 + 1
   __device__ inline void load(const Src& s, const Cell& c, bool live) {
     q = load16(s.vrs, record_offset(s, c, s.W + 2, 8u, live));

@@ -341,14 +341,46 @@ __device__ inline uint32_t mix32(uint32_t x) {
   return x;
 }
 
+__device__ inline float lattice_offset(uint32_t seed, int iteration, int b) {
+  uint32_t h = mix32(seed ^ 0x9e3779b9u);
+  h = mix32(h ^ static_cast<uint32_t>(iteration));
+  h = mix32(h ^ static_cast<uint32_t>(b));
+  return static_cast<float>(h >> 8) * 0x1p-24f - 0.5f;  // uniform in [-1/2, 1/2)
+}
+
 __global__ void k_lattice_offsets(const int* __restrict__ iteration, uint32_t seed, float* __restrict__ offsets,
                                   int B) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
-  uint32_t h = mix32(seed ^ 0x9e3779b9u);
-  h = mix32(h ^ static_cast<uint32_t>(*iteration));
-  h = mix32(h ^ static_cast<uint32_t>(b));
-  offsets[b] = static_cast<float>(h >> 8) * 0x1p-24f - 0.5f;  // uniform in [-1/2, 1/2)
+  offsets[b] = lattice_offset(seed, *iteration, b);
+}
+
+// k_pose_compose and k_lattice_offsets in one launch (both one thread per problem).
+__global__ void k_pose_compose_lattice(const double* __restrict__ prefix, const double* __restrict__ suffix,
+                                       const float* __restrict__ rot, const float* __restrict__ xyz,
+                                       float* __restrict__ M, int B, int64_t param_stride,
+                                       const int* __restrict__ iteration, uint32_t seed, float* __restrict__ offsets) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  offsets[b] = lattice_offset(seed, *iteration, b);
+  const double w[3] = {rot[b * param_stride], rot[b * param_stride + 1], rot[b * param_stride + 2]};
+  const double v[3] = {xyz[b * param_stride], xyz[b * param_stride + 1], xyz[b * param_stride + 2]};
+  ndrr_se3::compose(prefix + 16 * static_cast<int64_t>(b), suffix + 16 * static_cast<int64_t>(b), w, v,
+                    M + 12 * static_cast<int64_t>(b));
+}
+
+// Element i's Adam update with schedule row `it` (the same update as k_adam).
+__device__ inline void adam_scheduled_one(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                                          float* __restrict__ v, int i, const float* __restrict__ schedule, int it,
+                                          int lr_column, float b1, float c1, float b2, float c2, float eps) {
+  const float* row = schedule + 4 * static_cast<int64_t>(it);
+  const float lr = row[lr_column], bc1 = row[2], bc2 = row[3];
+  const float gi = g[i];
+  const float mi = m[i] * b1 + c1 * gi;
+  const float vi = v[i] * b2 + c2 * (gi * gi);
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
 }
 
 __global__ void k_adam_scheduled(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -358,14 +390,26 @@ __global__ void k_adam_scheduled(float* __restrict__ p, const float* __restrict_
   const int i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   const int it = min(max(*iteration, 0), rows - 1);
-  const float* row = schedule + 4 * static_cast<int64_t>(it);
-  const float lr = row[lr_column], bc1 = row[2], bc2 = row[3];
-  const float gi = g[i];
-  const float mi = m[i] * b1 + c1 * gi;  // the same update as k_adam
-  const float vi = v[i] * b2 + c2 * (gi * gi);
-  m[i] = mi;
-  v[i] = vi;
-  p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  adam_scheduled_one(p, g, m, v, i, schedule, it, lr_column, b1, c1, b2, c2, eps);
+}
+
+// Both parameter groups' scheduled Adam steps (group k uses learning-rate
+// column k) and the iteration counter's advance, in one workgroup: every
+// thread reads the counter before the barrier, one thread advances it after.
+constexpr int kPairThreads = 1024;
+__global__ __launch_bounds__(kPairThreads) void k_adam_scheduled_pair(
+    float* __restrict__ p0, const float* __restrict__ g0, float* __restrict__ m0, float* __restrict__ v0,
+    float* __restrict__ p1, const float* __restrict__ g1, float* __restrict__ m1, float* __restrict__ v1, int n,
+    const float* __restrict__ schedule, int rows, int* __restrict__ iteration, float b1, float c1, float b2, float c2,
+    float eps) {
+  const int it_raw = *iteration;
+  const int it = min(max(it_raw, 0), rows - 1);
+  for (int j = threadIdx.x; j < 2 * n; j += kPairThreads) {
+    if (j < n) adam_scheduled_one(p0, g0, m0, v0, j, schedule, it, 0, b1, c1, b2, c2, eps);
+    else adam_scheduled_one(p1, g1, m1, v1, j - n, schedule, it, 1, b1, c1, b2, c2, eps);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) iteration[0] = it_raw + 1;
 }
 
 __global__ void k_advance(int* __restrict__ counter) {
@@ -480,6 +524,30 @@ int ndrr_adam_step_scheduled(float* param, const float* grad, float* m, float* v
 int ndrr_counter_advance(int* counter, void* stream) {
   if (!counter) return NDRR_EINVAL;
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), counter);
+  return status();
+}
+
+int ndrr_adam_step_scheduled_pair(float* param0, const float* grad0, float* m0, float* v0, float* param1,
+                                  const float* grad1, float* m1, float* v1, int n, const float* schedule, int rows,
+                                  int* iteration, double beta1, double beta2, float eps, void* stream) {
+  if (!param0 || !grad0 || !m0 || !v0 || !param1 || !grad1 || !m1 || !v1 || !schedule || !iteration || n <= 0 ||
+      rows <= 0)
+    return NDRR_EINVAL;
+  if (!(beta1 >= 0.0 && beta1 < 1.0) || !(beta2 >= 0.0 && beta2 < 1.0)) return NDRR_EINVAL;
+  if (n > (1 << 20)) return NDRR_ERANGE;
+  hipLaunchKernelGGL(k_adam_scheduled_pair, dim3(1), dim3(kPairThreads), 0, static_cast<hipStream_t>(stream), param0,
+                     grad0, m0, v0, param1, grad1, m1, v1, n, schedule, rows, iteration, static_cast<float>(beta1),
+                     static_cast<float>(1.0 - beta1), static_cast<float>(beta2), static_cast<float>(1.0 - beta2), eps);
+  return status();
+}
+
+int ndrr_pose_compose_lattice(const double* prefix, const double* suffix, const float* rot, const float* xyz,
+                              int64_t param_stride, float* M, int B, const int* iteration, uint32_t seed,
+                              float* offsets, void* stream) {
+  if (!prefix || !suffix || !rot || !xyz || !M || !iteration || !offsets || B <= 0 || param_stride < 3)
+    return NDRR_EINVAL;
+  hipLaunchKernelGGL(k_pose_compose_lattice, dim3((B + 63) / 64), dim3(64), 0, static_cast<hipStream_t>(stream),
+                     prefix, suffix, rot, xyz, M, B, param_stride, iteration, seed, offsets);
   return status();
 }
 

@@ -30,6 +30,11 @@ from ._ops import _eager, _fused_raymarch_bwd_jac_impl, _fused_raymarch_jac_impl
 from .metrics import _zncc_impl, zncc
 from .renderer import _make_src, _make_tgt, march, march_inputs
 
+# register()'s loop makes the pose step and the jitter draw in one launch, and both Adam updates
+# and the counter's advance in another (ndrr_pose_compose_lattice, ndrr_adam_step_scheduled_pair):
+# bit for bit the separate launches, two to three fewer per iteration. False makes the separate ones.
+FUSED_LOOP_LAUNCHES = True
+
 
 def _stream(device):
     return torch.cuda.current_stream(device).cuda_stream
@@ -291,9 +296,10 @@ class _ExplicitLoop:
         self.grot = torch.empty(B, 3, device=dev)
         self.gxyz = torch.empty(B, 3, device=dev)
 
-    def step_(self, offset):
+    def step_(self, offset, jitter=None):
         """One iteration on torch's current stream (the capturing stream under HIP-graph capture):
-        returns (score (B,), d(-sum score)/d rot, d(-sum score)/d xyz)."""
+        returns (score (B,), d(-sum score)/d rot, d(-sum score)/d xyz). With jitter = (counter,
+        seed), the pose step also draws the lattice offsets into `offset` (one launch for both)."""
         reg, lib, st = self.reg, self.lib, _stream(self.device)
         B, N, D, H, W, S, C, width = self.dims
         if offset is not None and (offset.dtype != torch.float32 or offset.shape != (B,) or not offset.is_contiguous()):
@@ -305,7 +311,12 @@ class _ExplicitLoop:
         pre, suf = reg._prefix.data_ptr(), reg._suffix.data_ptr()
         rot, xyz, M = self.rot_b.data_ptr(), self.xyz_b.data_ptr(), self.M.data_ptr()
         src, tgt, step = self.src.data_ptr(), self.tgt.data_ptr(), self.step.data_ptr()
-        _lib.check(lib.ndrr_pose_compose(pre, suf, rot, xyz, 3, M, B, st), "ndrr_pose_compose")
+        if jitter is not None:
+            counter, seed32 = jitter
+            _lib.check(lib.ndrr_pose_compose_lattice(pre, suf, rot, xyz, 3, M, B, counter.data_ptr(), seed32,
+                                                     offset.data_ptr(), st), "ndrr_pose_compose_lattice")
+        else:
+            _lib.check(lib.ndrr_pose_compose(pre, suf, rot, xyz, 3, M, B, st), "ndrr_pose_compose")
         _lib.check(lib.ndrr_raymarch_fwd_jac(self.vptr, self.dt, self.code, self.lab, M, src, tgt, step, lattice,
                                              self.img.data_ptr(), self.jac.data_ptr(), 6, B, N, D, H, W, S, C,
                                              width, *self.strides, st), "ndrr_raymarch_fwd_jac")
@@ -400,8 +411,19 @@ def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot
                        "ndrr_lattice_offsets")
         return offsets
 
+    fused = FUSED_LOOP_LAUNCHES and params[0][0].numel() == params[1][0].numel()
+
     def adam(grads):
         st = _stream(dev)
+        if fused:
+            (p0, _), (p1, _) = params
+            (m0, v0), (m1, v1) = moments
+            _lib.check(lib.ndrr_adam_step_scheduled_pair(p0.data_ptr(), grads[0].data_ptr(), m0.data_ptr(),
+                                                         v0.data_ptr(), p1.data_ptr(), grads[1].data_ptr(),
+                                                         m1.data_ptr(), v1.data_ptr(), p0.numel(),
+                                                         schedule.data_ptr(), schedule.shape[0], counter.data_ptr(),
+                                                         b1, b2, adam_eps, st), "ndrr_adam_step_scheduled_pair")
+            return
         for col, ((p, _), (m, v), g) in enumerate(zip(params, moments, grads)):
             _lib.check(lib.ndrr_adam_step_scheduled(p.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(),
                                                     p.numel(), schedule.data_ptr(), schedule.shape[0], col,
@@ -418,7 +440,10 @@ def register(reg: Registration, target: torch.Tensor, n_iters: int = 300, lr_rot
             loop = _ExplicitLoop(reg, target, n_samples, eps, jitter)
 
             def iteration():
-                s, grot, gxyz = loop.step_(draw_offsets())
+                if fused and jitter:  # the offsets are drawn by the pose step's launch
+                    s, grot, gxyz = loop.step_(offsets, jitter=(counter, seed32))
+                else:
+                    s, grot, gxyz = loop.step_(draw_offsets())
                 adam((grot, gxyz))
                 return s, grot, gxyz
 

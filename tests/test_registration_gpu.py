@@ -199,16 +199,19 @@ class _RecordingLib:
         return call
 
 
-def test_graph_replay_matches_eager_iterations(monkeypatch):
+@pytest.mark.parametrize("fused", [True, False])
+def test_graph_replay_matches_eager_iterations(monkeypatch, fused):
     """register(graph=True): 11 iterations as 2 eager ones and three replays of a 3-iteration
     graph, against 11 eager iterations of the same problems. While capturing, every library call is
     a launch on the capturing stream and torch allocates nothing; after each replay (with eager work
     and a host synchronisation in between: the monitor) the parameters equal the eager loop's at the
-    same iteration, bit for bit, and so does the final score."""
+    same iteration, bit for bit, and so does the final score. With and without the fused loop-state
+    launches (pose step + jitter draw, both Adam updates + the counter's advance)."""
     from nanodrr_amd import _lib, render
     from nanodrr_amd import registration as registration_mod
     from nanodrr_amd.registration import register
 
+    monkeypatch.setattr(registration_mod, "FUSED_LOOP_LAUNCHES", fused)
     subject, prob, reg_g = _phantom_registration(B=4, seed=3)
     _, _, reg_e = _phantom_registration(B=4, seed=3)
     target = render(subject, reg_g.k_inv, prob.rt_true.float().to(DEV), reg_g.sdd, reg_g.height, reg_g.width, 64)
@@ -229,9 +232,13 @@ def test_graph_replay_matches_eager_iterations(monkeypatch):
                    monitor=monitor("graph"))
     torch.cuda.synchronize()
     captured = [c for c in rec.calls if c[1]]
-    launches = {"ndrr_lattice_offsets", "ndrr_pose_compose", "ndrr_raymarch_fwd_jac", "ndrr_zncc",
-                "ndrr_raymarch_bwd_jac_se3", "ndrr_adam_step_scheduled", "ndrr_counter_advance"}
-    assert len(captured) == 3 * 8, [c[0] for c in captured]
+    if fused:
+        launches = {"ndrr_pose_compose_lattice", "ndrr_raymarch_fwd_jac", "ndrr_zncc", "ndrr_raymarch_bwd_jac_se3",
+                    "ndrr_adam_step_scheduled_pair"}
+    else:
+        launches = {"ndrr_lattice_offsets", "ndrr_pose_compose", "ndrr_raymarch_fwd_jac", "ndrr_zncc",
+                    "ndrr_raymarch_bwd_jac_se3", "ndrr_adam_step_scheduled", "ndrr_counter_advance"}
+    assert len(captured) == 3 * (5 if fused else 8), [c[0] for c in captured]
     assert {c[0] for c in captured} == launches
     assert all(c[2] == c[3] for c in captured)  # on the capturing stream
     assert len({c[4] for c in captured}) == 1  # no torch allocation inside the capture
@@ -239,6 +246,31 @@ def test_graph_replay_matches_eager_iterations(monkeypatch):
     for it, (rot, xyz) in traj["graph"].items():
         assert torch.equal(rot, traj["eager"][it][0]) and torch.equal(xyz, traj["eager"][it][1]), it
     assert torch.equal(s_g, s_e)
+
+
+@pytest.mark.parametrize("jitter", [True, False])
+def test_fused_loop_launches_match_the_separate_ones(monkeypatch, jitter):
+    """register() with the fused loop-state launches against the separate launches: the same
+    parameters after every iteration and the same score, bit for bit."""
+    from nanodrr_amd import render
+    from nanodrr_amd import registration as registration_mod
+    from nanodrr_amd.registration import register
+
+    subject, prob, reg_f = _phantom_registration(B=4, seed=3)
+    _, _, reg_s = _phantom_registration(B=4, seed=3)
+    target = render(subject, reg_f.k_inv, prob.rt_true.float().to(DEV), reg_f.sdd, reg_f.height, reg_f.width, 64)
+    traj = {True: [], False: []}
+    scores = {}
+    for fused, reg in ((True, reg_f), (False, reg_s)):
+        monkeypatch.setattr(registration_mod, "FUSED_LOOP_LAUNCHES", fused)
+        scores[fused] = register(reg, target, n_iters=9, n_samples=64, jitter=jitter, seed=7, monitor_every=1,
+                                 monitor=lambda it, r, f=fused: traj[f].append(
+                                     (r._rot.detach().clone(), r._xyz.detach().clone())))
+    assert len(traj[True]) == len(traj[False]) == 9
+    for (r1, x1), (r2, x2) in zip(traj[True], traj[False]):
+        assert torch.equal(r1, r2) and torch.equal(x1, x2)
+    assert torch.equal(scores[True], scores[False])
+    assert not torch.equal(traj[True][0][0], traj[True][-1][0])  # the parameters moved
 
 
 def test_registration_converges_on_phantom():
