@@ -160,8 +160,10 @@ def time_steps(fn, steps, warmup):
 
 
 def _gvol_backward(go, vol, lab, M, src, tgt, step):
-    """A closure launching ndrr_raymarch_bwd with volume gradients into buffers allocated once (the
-    op allocates and zero-fills its gradient volume on every call; this times the kernels)."""
+    """A closure launching ndrr_raymarch_bwd_bricked (the op's volume-gradient path: the march's
+    scatter into a 4 x 4 x 4-voxel brick buffer, then its addition into gvol, which clears the
+    buffer) with buffers allocated once (the op allocates and zero-fills them on every call; this
+    times the kernels)."""
     from nanodrr_amd import _lib
     from nanodrr_amd._ops import _DTYPE_CODE, _geometry
 
@@ -171,14 +173,16 @@ def _gvol_backward(go, vol, lab, M, src, tgt, step):
     ws = torch.empty(lib.ndrr_bwd_workspace_floats(B, N, DET), device=dev, dtype=torch.float32)
     gM = torch.empty(B, 3, 4, device=dev, dtype=torch.float32)
     gvol = torch.zeros(vol.shape, device=dev, dtype=torch.float32)  # accumulates over the timed calls
+    bricks = torch.zeros(lib.ndrr_gvol_brick_floats(D, H, W), device=dev, dtype=torch.float32)  # left zero
     stream = torch.cuda.current_stream(dev).cuda_stream
 
     def launch():
-        rc = lib.ndrr_raymarch_bwd(go.data_ptr(), vol.data_ptr(), _DTYPE_CODE[vol.dtype], _lib.LAYOUT_LINEAR,
-                                   lab.data_ptr(), M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(),
-                                   ws.data_ptr(), gM.data_ptr(), None, None, None, gvol.data_ptr(), 0, B, N, D, H, W,
-                                   S, N_CLASSES, DET, *strides, stream)
-        _lib.check(rc, "ndrr_raymarch_bwd")
+        rc = lib.ndrr_raymarch_bwd_bricked(go.data_ptr(), vol.data_ptr(), _DTYPE_CODE[vol.dtype], _lib.LAYOUT_LINEAR,
+                                           lab.data_ptr(), M.data_ptr(), src.data_ptr(), tgt.data_ptr(),
+                                           step.data_ptr(), ws.data_ptr(), gM.data_ptr(), None, None, None,
+                                           gvol.data_ptr(), bricks.data_ptr(), 0, B, N, D, H, W, S, N_CLASSES, DET,
+                                           *strides, stream)
+        _lib.check(rc, "ndrr_raymarch_bwd_bricked")
 
     return launch
 

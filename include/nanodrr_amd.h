@@ -146,6 +146,34 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype,
                       void* stream);
 
 /*
+ * ndrr_raymarch_bwd with the volume gradient's scatter going through a brick
+ * buffer: gvol_bricks holds ndrr_gvol_brick_floats(D, H, W) fp32 values
+ * (16-byte aligned), zero on entry: 4 x 4 x 4-voxel bricks, brick
+ * (bx, by, bz) at ((bz * ceil(H/4) + by) * ceil(W/4) + bx) * 64, voxel
+ * (x, y, z) at (z % 4) * 16 + (y % 4) * 4 + x % 4 within its brick. The march's
+ * atomics add there (a 128-byte line then holds a 4 x 4 x 2 block of voxels,
+ * so the voxels one wave touches at one step fill about half as many lines as
+ * the [D][H][W] rows do), then a second launch adds the bricks into gvol
+ * (which must not be NULL) and clears them: the buffer is zero again on
+ * return, ready for the next call on the same stream. Results equal
+ * ndrr_raymarch_bwd's up to the order of the fp32 atomic additions.
+ * No counterpart in the reference (an internal layout of its
+ * _fused.py:284-298 volume-gradient scatter).
+ */
+size_t ndrr_gvol_brick_floats(int D, int H, int W);
+int ndrr_raymarch_bwd_bricked(const float* go, const void* vol, int vol_dtype,
+                              int vol_layout, const int8_t* lab, const float* M,
+                              const float* src, const float* tgt, const float* step,
+                              float* workspace, float* gM,
+                              float* gsrc, float* gtgt, float* gstep, float* gvol,
+                              float* gvol_bricks, int need_gcam,
+                              int B, int N, int D, int H, int W,
+                              int n_samples, int n_classes, int width,
+                              int64_t src_batch_stride, int64_t src_ray_stride,
+                              int64_t tgt_batch_stride, int64_t step_batch_stride,
+                              void* stream);
+
+/*
  * Single-pass adjoint: the forward and backward above as one volume march
  * plus a per-ray contraction, for callers that need pose (and camera)
  * gradients but not the volume gradient. Together they replace the pair

@@ -77,6 +77,14 @@ SIGNATURES: dict[str, tuple] = {
          _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
          _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
     ),
+    "ndrr_gvol_brick_floats": (_c_size, [_c_int, _c_int, _c_int]),
+    "ndrr_raymarch_bwd_bricked": (
+        _c_int,
+        [_c_p, _c_p, _c_int, _c_int, _c_p, _c_p, _c_p, _c_p, _c_p,
+         _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_int,
+         _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+         _c_i64, _c_i64, _c_i64, _c_i64, _c_p],
+    ),
     "ndrr_labels_to_i8": (_c_int, [_c_p, _c_p, _c_i64, _c_int, _c_p]),
     "ndrr_packed_bytes": (_c_size, [_c_int, _c_int, _c_int, _c_int, _c_int]),
     "ndrr_pack_volume": (_c_int, [_c_p, _c_int, _c_int, _c_int, _c_int, _c_int, _c_p, _c_p]),

@@ -303,6 +303,12 @@ def _(vol, lab, M, src, tgt, step, n_samples, n_classes, width, packed=None, lay
     return torch.empty(M.shape[0], n_classes, tgt.shape[1], device=vol.device, dtype=torch.float32)
 
 
+# Volume gradients scatter through 4 x 4 x 4-voxel bricks (ndrr_raymarch_bwd_bricked): a wave's
+# atomics at one step fill about half as many cache lines as in the (D, H, W) volume. False takes
+# ndrr_raymarch_bwd (the same sums in another order of fp32 additions).
+GVOL_BRICKS = True
+
+
 def _fused_raymarch_bwd_impl(
     go: torch.Tensor,
     vol: torch.Tensor,
@@ -340,13 +346,18 @@ def _fused_raymarch_bwd_impl(
     gtgt = torch.empty(B, N, 3, device=dev, dtype=torch.float32) if need_gcam else gM.new_empty(0)
     gstep = torch.empty(B, N, device=dev, dtype=torch.float32) if need_gcam else gM.new_empty(0)
     with torch.cuda.device(dev):
-        rc = lib.ndrr_raymarch_bwd(
-            go.data_ptr(), vptr, _DTYPE_CODE[vol.dtype], code, _ptr(lab) if n_classes > 1 else None,
-            M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(),
-            ws.data_ptr(), gM.data_ptr(), _ptr(gsrc), _ptr(gtgt), _ptr(gstep), _ptr(gvol),
-            int(need_gcam), B, N, D, H, W, n_samples, n_classes, width, *strides, _stream(dev),
-        )
-    _lib.check(rc, "ndrr_raymarch_bwd")
+        args = (go.data_ptr(), vptr, _DTYPE_CODE[vol.dtype], code, _ptr(lab) if n_classes > 1 else None,
+                M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(),
+                ws.data_ptr(), gM.data_ptr(), _ptr(gsrc), _ptr(gtgt), _ptr(gstep), _ptr(gvol))
+        tail = (int(need_gcam), B, N, D, H, W, n_samples, n_classes, width, *strides, _stream(dev))
+        if need_gvol and GVOL_BRICKS:
+            # the scatter's brick buffer, zero on entry (the call leaves it zero)
+            bricks = torch.zeros(lib.ndrr_gvol_brick_floats(D, H, W), device=dev, dtype=torch.float32)
+            rc = lib.ndrr_raymarch_bwd_bricked(*args, bricks.data_ptr(), *tail)
+            _lib.check(rc, "ndrr_raymarch_bwd_bricked")
+        else:
+            rc = lib.ndrr_raymarch_bwd(*args, *tail)
+            _lib.check(rc, "ndrr_raymarch_bwd")
     return gvol, gM, gsrc, gtgt, gstep
 
 

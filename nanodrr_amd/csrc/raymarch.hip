@@ -175,6 +175,8 @@ struct Params {
   float* gstep;
   float* gvol;
   int gvol_merge;  // merge lanes' volume-gradient atomics per cell (cell keys fit 32 bits)
+  float* gvol_bricks;  // 4 x 4 x 4-voxel bricks the merged scatter adds into instead of gvol (or null)
+  int nbx, nby;        // bricks along x and y
   int64_t src_bs, src_rs, tgt_bs, step_bs;
   int64_t go_bs;  // batch stride of go in the single-pass backward (0: shared by every pose)
   uint32_t vol_bytes, lab_bytes;  // sizes of the packed volume / label buffers
@@ -1141,73 +1143,78 @@ __device__ inline int wave_max(int v) {
   return v;
 }
 
-// Volume-gradient atomics of one sample per lane, merged across the wave by cell. A wave's
-// atomic instruction costs the L2 one pass per 64-byte line for every copy of an address in
-// that line (distinct addresses in one line share a pass, equal ones do not), and
-// neighbouring rays often sample the same cell, so lanes first add their contributions over
-// runs of equal cells - chains of lanes l, l + 1, ... along each 8-lane row of the wave's
-// 8 x 8 ray patch (segmented suffix sums, steps 1, 2, 4), then chains down its columns (steps
-// 8, 16, 32) - and only the first lane of a run issues the eight atomics. A run is defined by
-// lane adjacency alone, so the result is exact for any ray layout (only the merging rate
-// depends on it). Every lane of the wave must call this (it shuffles); lanes with g == 0, or
-// whose cell has no corner in the volume, take a key no other lane has and issue nothing.
-// Cell keys are ((iz + 1)(H + 2) + iy + 1)(W + 2) + ix + 1 < 2^32 - 64 (checked on the host).
-__device__ inline void gvol_add_merged(float* __restrict__ gvol, const Params& p, const Cell& c, float g) {
-  const int lane = threadIdx.x & 63, col = lane & 7, row = lane >> 3;
-  const bool touch = g != 0.0f && c.ix >= -1 && c.ix < p.W && c.iy >= -1 && c.iy < p.H && c.iz >= -1 && c.iz < p.D;
-  const uint32_t key =
-      touch ? (static_cast<uint32_t>(c.iz + 1) * static_cast<uint32_t>(p.H + 2) + static_cast<uint32_t>(c.iy + 1)) *
-                      static_cast<uint32_t>(p.W + 2) +
-                  static_cast<uint32_t>(c.ix + 1)
-            : 0xffffffffu - static_cast<uint32_t>(lane);
-  // Every shuffle below runs on all lanes: a lane reads another's value only while that lane is
-  // active, so none sits under a condition (or the right side of && / ||).
-  const int ikey = static_cast<int>(key);
-  const int nx = min(lane + 1, 63), nx2 = min(lane + 2, 63), nx4 = min(lane + 4, 63);
-  const int nz = min(lane + 8, 63), nz2 = min(lane + 16, 63), nz4 = min(lane + 32, 63);
-  const int px = max(lane - 1, 0), pz = max(lane - 8, 0);
-  // e1: lanes l and l + 1 share the cell; e2: l .. l + 2; e4: l .. l + 4 (within the row)
-  const int key_x = __shfl(ikey, nx, 64);
-  const int e1 = (col < 7 && key_x == ikey) ? 1 : 0;
-  const int e1_next = __shfl(e1, nx, 64), e1_prev = __shfl(e1, px, 64);
-  const int e2 = (e1 && e1_next) ? 1 : 0;
-  const int e2_next = __shfl(e2, nx2, 64);
-  const int e4 = (e2 && e2_next) ? 1 : 0;
-  const bool head_x = col == 0 || !e1_prev;
-  // f1: lanes l and l + 8 share the cell; f2: l, l + 8, l + 16; f4: l .. l + 32 (down the column)
-  const int key_z = __shfl(ikey, nz, 64);
-  const int f1 = (row < 7 && key_z == ikey) ? 1 : 0;
-  const int f1_next = __shfl(f1, nz, 64), f1_prev = __shfl(f1, pz, 64);
-  const int f2 = (f1 && f1_next) ? 1 : 0;
-  const int f2_next = __shfl(f2, nz2, 64);
-  const int f4 = (f2 && f2_next) ? 1 : 0;
-  const bool head_z = row == 0 || !f1_prev;
+
+// Ordering of a wave's own LDS accesses: its earlier LDS operations have completed before
+// any lane goes on (no workgroup barrier: only this wave touches the region).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Where the merged volume-gradient scatter adds voxel (x, y, z): the brick buffer when given
+// (4 x 4 x 4-voxel bricks, x fastest within a brick and among bricks: a 128-byte line holds a
+// 4 x 4 x 2 block, so the voxels a wave touches at one step fill about half as many lines as
+// in the [D][H][W] volume, where each (y, z) row is a line), else gvol itself.
+__device__ inline float* gvol_voxel(const Params& p, int x, int y, int z) {
+  if (p.gvol_bricks == nullptr) return p.gvol + (z * p.H + y) * p.W + x;
+  const int brick = ((z >> 2) * p.nby + (y >> 2)) * p.nbx + (x >> 2);
+  return p.gvol_bricks + brick * 64 + ((z & 3) * 16 + (y & 3) * 4 + (x & 3));
+}
+
+// Volume-gradient scatter through a wave-local LDS box of kBox^3 voxels, anchored at the cell
+// of lane 36 (the middle of the wave's 8 x 8 ray patch) less 3 along each axis: each lane adds
+// its sample's eight corner contributions into the box with LDS atomics where they fall inside
+// it (the cells within 3 of the anchor's), else straight to memory; then each lane takes eight
+// box voxels, adds every nonzero one to memory with one atomic and clears them. Adjacent rays'
+// samples share corners, so a wave-step's 512 corner updates become one atomic per voxel
+// touched. Every lane of the wave runs it (the merged march's wave-wide loop).
+constexpr int kBox = 8;
+__device__ inline void gvol_add_box(const Params& p, const Cell& c, float g, float* __restrict__ box) {
+  const int lane = threadIdx.x & 63;
+  const int ax = __builtin_amdgcn_readlane(c.ix, 36) - 3;
+  const int ay = __builtin_amdgcn_readlane(c.iy, 36) - 3;
+  const int az = __builtin_amdgcn_readlane(c.iz, 36) - 3;
   const float wx[2] = {1.0f - c.fx, c.fx};
   const float wy[2] = {1.0f - c.fy, c.fy};
   const float wz[2] = {1.0f - c.fz, c.fz};
+  if (g != 0.0f) {
 #pragma unroll
-  for (int corner = 0; corner < 8; ++corner) {
-    const int cx = corner & 1, cy = (corner >> 1) & 1, cz = corner >> 2;
-    float v = g * (wx[cx] * wy[cy] * wz[cz]);
-    float o = __shfl(v, nx, 64);
-    v += e1 ? o : 0.0f;
-    o = __shfl(v, nx2, 64);
-    v += e2 ? o : 0.0f;
-    o = __shfl(v, nx4, 64);
-    v += e4 ? o : 0.0f;
-    v = head_x ? v : 0.0f;
-    o = __shfl(v, nz, 64);
-    v += f1 ? o : 0.0f;
-    o = __shfl(v, nz2, 64);
-    v += f2 ? o : 0.0f;
-    o = __shfl(v, nz4, 64);
-    v += f4 ? o : 0.0f;
-    const int jx = c.ix + cx, jy = c.iy + cy, jz = c.iz + cz;
+    for (int corner = 0; corner < 8; ++corner) {
+      const int cx = corner & 1, cy = (corner >> 1) & 1, cz = corner >> 2;
+      const int jx = c.ix + cx, jy = c.iy + cy, jz = c.iz + cz;
+      const bool inb = static_cast<unsigned>(jx) < static_cast<unsigned>(p.W) &&
+                       static_cast<unsigned>(jy) < static_cast<unsigned>(p.H) &&
+                       static_cast<unsigned>(jz) < static_cast<unsigned>(p.D);
+      if (!inb) continue;
+      const float v = g * (wx[cx] * wy[cy] * wz[cz]);
+      const unsigned bx = static_cast<unsigned>(jx - ax), by = static_cast<unsigned>(jy - ay),
+                     bz = static_cast<unsigned>(jz - az);
+      if ((bx | by | bz) < static_cast<unsigned>(kBox))
+        atomicAdd(box + (bz * kBox + by) * kBox + bx, v);
+      else
+        unsafeAtomicAdd(gvol_voxel(p, jx, jy, jz), v);
+    }
+  }
+  wave_lds_sync();
+  // lane: box row (bz, by) = (lane / 8, lane % 8), voxels bx = 0 .. 7
+  float4* row = reinterpret_cast<float4*>(box + kBox * lane);
+  const float4 lo = row[0], hi = row[1];
+  const float v8[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+  const int jy = ay + (lane & 7), jz = az + (lane >> 3);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int jx = ax + k;
+    // only voxels inside the volume were added; the test also keeps the address in range
     const bool inb = static_cast<unsigned>(jx) < static_cast<unsigned>(p.W) &&
                      static_cast<unsigned>(jy) < static_cast<unsigned>(p.H) &&
                      static_cast<unsigned>(jz) < static_cast<unsigned>(p.D);
-    if (head_z && inb && v != 0.0f) unsafeAtomicAdd(gvol + (jz * p.H + jy) * p.W + jx, v);
+    if (inb && v8[k] != 0.0f) unsafeAtomicAdd(gvol_voxel(p, jx, jy, jz), v8[k]);
   }
+  row[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  row[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  wave_lds_sync();
 }
 
 template <int DT, int L, int CV, bool GCAM, bool GVOL, int NCH>
@@ -1240,6 +1247,15 @@ void k_bwd(Params p) {
   // a ray that misses the volume read zeros or clamped addresses, and add nothing).
   const bool merge = GVOL && p.gvol_merge;
   const int n_loop = merge ? wave_max(r.n_s) : r.n_s;
+  float* box = nullptr;  // this wave's LDS box (gvol_add_box), cleared before its first use
+  if constexpr (GVOL) {
+    __shared__ __attribute__((aligned(16))) float gbox[(kTileRays / kWave) * NCH][kBox * kBox * kBox];
+    box = gbox[threadIdx.x / kWave];
+    float4* mine = reinterpret_cast<float4*>(box + kBox * (threadIdx.x & 63));
+    mine[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    mine[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    wave_lds_sync();
+  }
   for (int k0 = 0; k0 < n_loop; k0 += B::U) {
     B batch;
     batch.issue(src, r, k0);
@@ -1267,7 +1283,7 @@ void k_bwd(Params p) {
       if constexpr (GCAM) astep += gg * interp_value(batch.f[u], c);
       if constexpr (GVOL) {
         if (merge) {
-          gvol_add_merged(p.gvol, p, c, g);
+          gvol_add_box(p, c, g, box);
         } else if (g != 0.0f) {
           const float wx[2] = {1.0f - fx, fx};
           const float wy[2] = {1.0f - fy, fy};
@@ -1842,6 +1858,27 @@ __global__ void k_pack_qcoef(const typename Vox<DS>::T* __restrict__ vol, uint32
   rec[r] = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
 }
 
+// One thread per brick row (4 voxels along x): gvol[z][y][x..x+3] += the brick buffer's values
+// (gvol_voxel's layout), and the row is cleared, so the buffer is zero again for the next call.
+__global__ void k_gvol_unbrick(float* __restrict__ bricks, float* __restrict__ gvol, int D, int H, int W, int nbx,
+                               int nby, int64_t n_rows) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n_rows) return;
+  const int x4 = static_cast<int>(r % nbx);
+  const int64_t rest = r / nbx;
+  const int y = static_cast<int>(rest % H);
+  const int z = static_cast<int>(rest / H);
+  const int64_t brick = (static_cast<int64_t>(z >> 2) * nby + (y >> 2)) * nbx + x4;
+  float4* row = reinterpret_cast<float4*>(bricks + brick * 64 + (z & 3) * 16 + (y & 3) * 4);
+  const float4 v = *row;
+  const float vs[4] = {v.x, v.y, v.z, v.w};
+  float* g = gvol + (static_cast<int64_t>(z) * H + y) * W + 4 * x4;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (4 * x4 + k < W) g[k] += vs[k];
+  *row = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 // One thread per label-quad record (x, y0, z0): bytes lab[z0+a][y0+b][x] at 2a + b.
 __global__ void k_pack_label_quads(const int8_t* __restrict__ lab, uint32_t* __restrict__ out,
                                    int D, int H, int W, int64_t n_rec) {
@@ -1913,6 +1950,10 @@ int64_t packed_volume_bytes(int D, int H, int W, int dt, int layout) {
     case NDRR_LAYOUT_QCOEF: return kCoefHeader + quad_records(D, H, W) * 8;  // fp16 records whatever the source
     default: return 0;
   }
+}
+
+int64_t gvol_brick_floats(int D, int H, int W) {
+  return static_cast<int64_t>((D + 3) / 4) * ((H + 3) / 4) * ((W + 3) / 4) * 64;
 }
 
 int64_t packed_label_bytes(int D, int H, int W, int layout) {
@@ -2205,17 +2246,22 @@ size_t ndrr_bwd_workspace_floats(int B, int N, int width) {
   return static_cast<size_t>(B) * nblocks_of(N, width, tw, ih) * kBwdPartsMax * 12;
 }
 
-int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_layout, const int8_t* lab,
-                      const float* M, const float* src, const float* tgt, const float* step, float* workspace,
-                      float* gM, float* gsrc, float* gtgt, float* gstep, float* gvol, int need_gcam, int B, int N,
-                      int D, int H, int W, int n_samples, int n_classes, int width, int64_t src_batch_stride,
-                      int64_t src_ray_stride, int64_t tgt_batch_stride, int64_t step_batch_stride,
-                      void* stream) {
+// The two-pass adjoint, with the merged volume-gradient scatter adding into a brick buffer when
+// one is given (then added into gvol and cleared by k_gvol_unbrick).
+static int raymarch_bwd_any(const float* go, const void* vol, int vol_dtype, int vol_layout, const int8_t* lab,
+                            const float* M, const float* src, const float* tgt, const float* step, float* workspace,
+                            float* gM, float* gsrc, float* gtgt, float* gstep, float* gvol, float* gvol_bricks,
+                            int need_gcam, int B, int N, int D, int H, int W, int n_samples, int n_classes, int width,
+                            int64_t src_batch_stride, int64_t src_ray_stride, int64_t tgt_batch_stride,
+                            int64_t step_batch_stride, void* stream) {
   const int rc = check_common(vol, vol_dtype, vol_layout, lab, M, src, tgt, step, B, N, D, H, W, n_samples,
                               n_classes, width);
   if (rc != NDRR_OK) return rc;
   if (!go || !workspace || !gM || reinterpret_cast<uintptr_t>(workspace) % 16 != 0) return NDRR_EINVAL;
   if (need_gcam && (!gsrc || !gtgt || !gstep)) return NDRR_EINVAL;
+  if (gvol_bricks != nullptr && (gvol == nullptr || reinterpret_cast<uintptr_t>(gvol_bricks) % 16 != 0))
+    return NDRR_EINVAL;
+  if (gvol_bricks != nullptr && gvol_brick_floats(D, H, W) >= (int64_t{1} << 31)) return NDRR_ERANGE;
   Params p = make_params(vol, lab, M, src, tgt, step, B, N, D, H, W, n_samples, n_classes, width,
                          src_batch_stride, src_ray_stride, tgt_batch_stride, step_batch_stride);
   p.vol_bytes = static_cast<uint32_t>(packed_volume_bytes(D, H, W, vol_dtype, vol_layout));
@@ -2225,6 +2271,10 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_l
   p.gpart = workspace;
   p.gsrc = gsrc; p.gtgt = gtgt; p.gstep = gstep; p.gvol = gvol;
   p.gvol_merge = static_cast<uint64_t>(D + 2) * (H + 2) * (W + 2) + 64 <= (uint64_t{1} << 32);
+  // the brick buffer serves the merged scatter only (the per-lane one adds into gvol directly)
+  p.gvol_bricks = p.gvol_merge ? gvol_bricks : nullptr;
+  p.nbx = (W + 3) / 4;
+  p.nby = (H + 3) / 4;
   const dim3 grid(p.nblocks, B);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const bool gcam = need_gcam != 0, gv = gvol != nullptr;
@@ -2241,7 +2291,40 @@ int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_l
                                                                : bwd_parts_per_tile(NDRR_LAYOUT_LINEAR);
   hipLaunchKernelGGL(k_reduce_pose, dim3(B), dim3(kReduceThreads), 0, st, workspace, gM,
                      p.nblocks * parts_per_tile, nullptr, nullptr, Se3Out{});
+  if (p.gvol_bricks != nullptr) {
+    const int64_t rows = static_cast<int64_t>(D) * H * p.nbx;
+    hipLaunchKernelGGL(k_gvol_unbrick, dim3(static_cast<unsigned>((rows + kThreads - 1) / kThreads)), dim3(kThreads),
+                       0, st, p.gvol_bricks, gvol, D, H, W, p.nbx, p.nby, rows);
+  }
   return launch_status();
+}
+
+int ndrr_raymarch_bwd(const float* go, const void* vol, int vol_dtype, int vol_layout, const int8_t* lab,
+                      const float* M, const float* src, const float* tgt, const float* step, float* workspace,
+                      float* gM, float* gsrc, float* gtgt, float* gstep, float* gvol, int need_gcam, int B, int N,
+                      int D, int H, int W, int n_samples, int n_classes, int width, int64_t src_batch_stride,
+                      int64_t src_ray_stride, int64_t tgt_batch_stride, int64_t step_batch_stride,
+                      void* stream) {
+  return raymarch_bwd_any(go, vol, vol_dtype, vol_layout, lab, M, src, tgt, step, workspace, gM, gsrc, gtgt, gstep,
+                          gvol, nullptr, need_gcam, B, N, D, H, W, n_samples, n_classes, width, src_batch_stride,
+                          src_ray_stride, tgt_batch_stride, step_batch_stride, stream);
+}
+
+size_t ndrr_gvol_brick_floats(int D, int H, int W) {
+  if (D <= 0 || H <= 0 || W <= 0) return 0;
+  return static_cast<size_t>(gvol_brick_floats(D, H, W));
+}
+
+int ndrr_raymarch_bwd_bricked(const float* go, const void* vol, int vol_dtype, int vol_layout, const int8_t* lab,
+                              const float* M, const float* src, const float* tgt, const float* step,
+                              float* workspace, float* gM, float* gsrc, float* gtgt, float* gstep, float* gvol,
+                              float* gvol_bricks, int need_gcam, int B, int N, int D, int H, int W, int n_samples,
+                              int n_classes, int width, int64_t src_batch_stride, int64_t src_ray_stride,
+                              int64_t tgt_batch_stride, int64_t step_batch_stride, void* stream) {
+  if (!gvol || !gvol_bricks) return NDRR_EINVAL;
+  return raymarch_bwd_any(go, vol, vol_dtype, vol_layout, lab, M, src, tgt, step, workspace, gM, gsrc, gtgt, gstep,
+                          gvol, gvol_bricks, need_gcam, B, N, D, H, W, n_samples, n_classes, width, src_batch_stride,
+                          src_ray_stride, tgt_batch_stride, step_batch_stride, stream);
 }
 
 // The contraction with M (and gM out) given, or (w2v, rt) given and the rt gradient out.
