@@ -22,6 +22,8 @@ gradients without marching the volume again.
 
 from __future__ import annotations
 
+import contextlib
+import functools
 from types import SimpleNamespace
 from typing import Optional
 
@@ -36,8 +38,25 @@ def _ptr(t: torch.Tensor | None) -> int | None:
     return None if t is None or t.numel() == 0 else t.data_ptr()
 
 
+def _index(device: torch.device) -> int:
+    return device.index if device.index is not None else torch.cuda.current_device()
+
+
 def _stream(device: torch.device) -> int:
-    return torch.cuda.current_stream(device).cuda_stream
+    """Raw handle of torch's current stream on `device` (the capturing stream under graph capture),
+    read without building a Stream object."""
+    return torch._C._cuda_getCurrentRawStream(_index(device))
+
+
+def _on(device: torch.device):
+    """`torch.cuda.device(device)`, or nothing when `device` is already the current one."""
+    idx = _index(device)
+    return contextlib.nullcontext() if idx == torch._C._cuda_getDevice() else torch.cuda.device(idx)
+
+
+@functools.lru_cache(maxsize=64)
+def _packed_nbytes(D: int, H: int, W: int, dtype_code: int, layout_code: int) -> int:
+    return _lib.lib().ndrr_packed_bytes(D, H, W, dtype_code, layout_code)
 
 
 def _volume_arg(vol, packed, layout):
@@ -48,8 +67,7 @@ def _volume_arg(vol, packed, layout):
     if code is None or code == _lib.LAYOUT_LINEAR:
         raise ValueError(f"layout must name the packed copy ('quad', 'stencil', 'coef', 'lcoef' or 'qcoef'), "
                          f"got {layout!r}")
-    D, H, W = vol.shape
-    need = _lib.lib().ndrr_packed_bytes(D, H, W, _DTYPE_CODE[vol.dtype], code)
+    need = _packed_nbytes(*vol.shape, _DTYPE_CODE[vol.dtype], code)
     if packed.dtype != torch.uint8 or packed.numel() != need or not packed.is_contiguous() \
             or packed.device != vol.device:
         raise ValueError(f"packed must be a contiguous uint8 tensor of {need} bytes on {vol.device}")
@@ -71,7 +89,7 @@ def pack_volume(vol: torch.Tensor, layout: str) -> torch.Tensor:
     lib = _lib.lib()
     code, dt = _lib.LAYOUTS[layout], _DTYPE_CODE[vol.dtype]
     out = torch.empty(lib.ndrr_packed_bytes(D, H, W, dt, code), device=vol.device, dtype=torch.uint8)
-    with torch.cuda.device(vol.device):
+    with _on(vol.device):
         rc = lib.ndrr_pack_volume(vol.data_ptr(), dt, code, D, H, W, out.data_ptr(), _stream(vol.device))
     _lib.check(rc, "ndrr_pack_volume")
     return out
@@ -108,7 +126,7 @@ def pack_coef(vol: torch.Tensor, record_dtype: torch.dtype) -> torch.Tensor:
     lib = _lib.lib()
     rd = _DTYPE_CODE[record_dtype]
     out = torch.empty(lib.ndrr_packed_bytes(D, H, W, rd, _lib.LAYOUT_COEF), device=vol.device, dtype=torch.uint8)
-    with torch.cuda.device(vol.device):
+    with _on(vol.device):
         rc = lib.ndrr_pack_coef(vol.data_ptr(), _DTYPE_CODE[vol.dtype], rd, D, H, W, out.data_ptr(),
                                 _stream(vol.device))
     _lib.check(rc, "ndrr_pack_coef")
@@ -152,7 +170,7 @@ def pack_coef_labels(vol: torch.Tensor, lab: torch.Tensor, n_classes: int, recor
     lib = _lib.lib()
     rd = _DTYPE_CODE[record_dtype]
     out = torch.empty(lib.ndrr_packed_bytes(D, H, W, rd, _lib.LAYOUT_LCOEF), device=vol.device, dtype=torch.uint8)
-    with torch.cuda.device(vol.device):
+    with _on(vol.device):
         rc = lib.ndrr_pack_coef_labels(vol.data_ptr(), _DTYPE_CODE[vol.dtype], lab.data_ptr(), n_classes, rd, D, H, W,
                                        out.data_ptr(), _stream(vol.device))
     _lib.check(rc, "ndrr_pack_coef_labels")
@@ -184,7 +202,7 @@ def pack_labels(lab: torch.Tensor, layout: str) -> torch.Tensor:
     lib = _lib.lib()
     code = _lib.LAYOUTS[layout]
     out = torch.empty(lib.ndrr_packed_label_bytes(D, H, W, code), device=lab.device, dtype=torch.int8)
-    with torch.cuda.device(lab.device):
+    with _on(lab.device):
         rc = lib.ndrr_pack_labels(lab.data_ptr(), code, D, H, W, out.data_ptr(), _stream(lab.device))
     _lib.check(rc, "ndrr_pack_labels")
     return out.view(D, H, W) if layout == "linear" else out
@@ -270,7 +288,7 @@ def _fused_raymarch_impl(
                                        layout if packed is not None else "linear")
     vptr, code = _volume_arg(vol, packed, layout)
     out = torch.empty(B, n_classes, N, device=vol.device, dtype=torch.float32)
-    with torch.cuda.device(vol.device):
+    with _on(vol.device):
         rc = _lib.lib().ndrr_raymarch_fwd(
             vptr, _DTYPE_CODE[vol.dtype], code, _ptr(lab) if n_classes > 1 else None,
             M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(), out.data_ptr(),
@@ -345,7 +363,7 @@ def _fused_raymarch_bwd_impl(
     gsrc = torch.empty(B, N, 3, device=dev, dtype=torch.float32) if need_gcam else gM.new_empty(0)
     gtgt = torch.empty(B, N, 3, device=dev, dtype=torch.float32) if need_gcam else gM.new_empty(0)
     gstep = torch.empty(B, N, device=dev, dtype=torch.float32) if need_gcam else gM.new_empty(0)
-    with torch.cuda.device(dev):
+    with _on(dev):
         args = (go.data_ptr(), vptr, _DTYPE_CODE[vol.dtype], code, _ptr(lab) if n_classes > 1 else None,
                 M.data_ptr(), src.data_ptr(), tgt.data_ptr(), step.data_ptr(),
                 ws.data_ptr(), gM.data_ptr(), _ptr(gsrc), _ptr(gtgt), _ptr(gstep), _ptr(gvol))
@@ -469,7 +487,7 @@ def _jac_forward(vol, lab, M, pose, src, tgt, step, n_samples, n_classes, width,
     tail = (src.data_ptr(), tgt.data_ptr(), step.data_ptr(), _ptr(lattice), out.data_ptr(), _ptr(jac), jac_planes, B,
             N, D, H, W,
             n_samples, n_classes, width, *strides, _stream(vol.device))
-    with torch.cuda.device(vol.device):
+    with _on(vol.device):
         if pose is None:
             rc = lib.ndrr_raymarch_fwd_jac(*head, M.data_ptr(), *tail)
         else:
@@ -558,7 +576,7 @@ def _jac_backward(go, jac, M, pose, src, tgt, step, width, need_gcam):
     gstep = torch.empty(B, N, device=dev, dtype=torch.float32) if need_gcam else g.new_empty(0)
     mid = (src.data_ptr(), tgt.data_ptr(), step.data_ptr(), ws.data_ptr(), g.data_ptr(), _ptr(gsrc), _ptr(gtgt),
            _ptr(gstep), int(need_gcam), B, N, C, width, *strides, _stream(dev))
-    with torch.cuda.device(dev):
+    with _on(dev):
         if pose is None:
             rc = lib.ndrr_raymarch_bwd_jac(go.data_ptr(), go_bs, jac.data_ptr(), planes, M.data_ptr(), *mid)
         else:
@@ -784,7 +802,7 @@ def _make_rays_impl(k_inv: torch.Tensor, sdd: torch.Tensor, height: int, width: 
     tgt = torch.empty(B, N, 3, device=dev, dtype=torch.float32)
     src = torch.empty(B, N if orthographic else 1, 3, device=dev, dtype=torch.float32)
     step = torch.empty(B, N, device=dev, dtype=torch.float32)
-    with torch.cuda.device(dev):
+    with _on(dev):
         rc = _lib.lib().ndrr_make_rays(k.data_ptr(), 0 if k.shape[0] == 1 else 9, d.data_ptr(),
                                        0 if d.shape[0] == 1 else 1, B, height, width, int(orthographic), n_samples,
                                        tgt.data_ptr(), src.data_ptr(), step.data_ptr(), _stream(dev))
@@ -822,7 +840,7 @@ def _camera_to_voxel_impl(world_to_voxel: torch.Tensor, rt_inv: torch.Tensor) ->
         raise ValueError("rt_inv must be (B, 4, 4)")
     B = rt_inv.shape[0]
     M = torch.empty(B, 3, 4, device=rt_inv.device, dtype=torch.float32)
-    with torch.cuda.device(rt_inv.device):
+    with _on(rt_inv.device):
         rc = _lib.lib().ndrr_camera_to_voxel(world_to_voxel.data_ptr(), rt_inv.data_ptr(), M.data_ptr(), B,
                                              _stream(rt_inv.device))
     _lib.check(rc, "ndrr_camera_to_voxel")
@@ -846,7 +864,7 @@ def _camera_to_voxel_bwd_impl(world_to_voxel: torch.Tensor, gM: torch.Tensor) ->
     _c2v_check(world_to_voxel, gM, "gM")
     B = gM.shape[0]
     g = torch.empty(B, 4, 4, device=gM.device, dtype=torch.float32)
-    with torch.cuda.device(gM.device):
+    with _on(gM.device):
         rc = _lib.lib().ndrr_camera_to_voxel_bwd(world_to_voxel.data_ptr(), gM.data_ptr(), g.data_ptr(), B,
                                                  _stream(gM.device))
     _lib.check(rc, "ndrr_camera_to_voxel_bwd")
@@ -883,7 +901,7 @@ def labels_to_i8(label: torch.Tensor, n_classes: int) -> torch.Tensor:
         raise ValueError("labels_to_i8 runs on the GPU: pass a device tensor")
     src = label.detach().to(torch.float32).contiguous()
     dst = torch.empty(src.shape, device=src.device, dtype=torch.int8)
-    with torch.cuda.device(src.device):
+    with _on(src.device):
         rc = _lib.lib().ndrr_labels_to_i8(src.data_ptr(), dst.data_ptr(), src.numel(), n_classes,
                                           _stream(src.device))
     _lib.check(rc, "ndrr_labels_to_i8")

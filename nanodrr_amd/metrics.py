@@ -19,7 +19,7 @@ import torch
 import torch.nn.functional as F
 
 from . import _lib
-from ._ops import _eager
+from ._ops import _eager, _on
 from .geometry import matmul3, rotmat_geodesic_distance
 
 
@@ -34,7 +34,7 @@ def _zncc_impl(x: torch.Tensor, y: torch.Tensor, eps: float) -> tuple[torch.Tens
     score = torch.empty(B, device=x.device, dtype=torch.float32)
     grad = torch.empty_like(x2)
     ws = torch.empty(_lib.lib().ndrr_zncc_workspace_floats(B, x2.shape[1]), device=x.device, dtype=torch.float32)
-    with torch.cuda.device(x.device):
+    with _on(x.device):
         rc = _lib.lib().ndrr_zncc(x2.data_ptr(), y2.data_ptr(), B, x2.shape[1], float(eps), ws.data_ptr(),
                                   score.data_ptr(), grad.data_ptr(), torch.cuda.current_stream(x.device).cuda_stream)
     _lib.check(rc, "ndrr_zncc")
@@ -86,7 +86,7 @@ def _patch_zncc_impl(x: torch.Tensor, y: torch.Tensor, patch: int, eps: float) -
     score = torch.empty(B, device=x.device, dtype=torch.float32)
     grad = torch.empty_like(x2)
     ws = torch.empty(lib.ndrr_patch_zncc_workspace_floats(B, H, W, patch), device=x.device, dtype=torch.float32)
-    with torch.cuda.device(x.device):
+    with _on(x.device):
         rc = lib.ndrr_patch_zncc(x2.data_ptr(), y2.data_ptr(), B, H, W, patch, float(eps), ws.data_ptr(),
                                  score.data_ptr(), grad.data_ptr(), torch.cuda.current_stream(x.device).cuda_stream)
     _lib.check(rc, "ndrr_patch_zncc")

@@ -18,6 +18,7 @@ import math
 import torch
 
 from . import _lib
+from ._ops import _stream as _ops_stream
 
 # (Cin, Cout, stride) of the six convolutions
 LAYERS = ((1, 32, 1), (32, 64, 2), (64, 64, 1), (64, 128, 2), (128, 128, 1), (128, 256, 2))
@@ -27,7 +28,7 @@ RENDER_LAYOUT = "quad"
 
 
 def _stream(t: torch.Tensor) -> int:
-    return torch.cuda.current_stream(t.device).cuda_stream
+    return _ops_stream(t.device)
 
 
 def _out(n: int, stride: int) -> int:

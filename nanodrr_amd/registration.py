@@ -26,7 +26,8 @@ import torch
 from . import _lib
 from ._memo import Derived
 from .geometry import convert, transform_point
-from ._ops import _eager, _fused_raymarch_bwd_jac_impl, _fused_raymarch_jac_impl
+from ._ops import _eager, _fused_raymarch_bwd_jac_impl, _fused_raymarch_jac_impl, _on
+from ._ops import _stream as _ops_stream
 from .metrics import _zncc_impl, zncc
 from .renderer import _make_src, _make_tgt, march, march_inputs
 
@@ -37,7 +38,7 @@ FUSED_LOOP_LAUNCHES = True
 
 
 def _stream(device):
-    return torch.cuda.current_stream(device).cuda_stream
+    return _ops_stream(device)
 
 
 def _pose_compose_impl(prefix: torch.Tensor, suffix: torch.Tensor, rot: torch.Tensor,
@@ -50,7 +51,7 @@ def _pose_compose_impl(prefix: torch.Tensor, suffix: torch.Tensor, rot: torch.Te
     if prefix.shape != (B, 4, 4) or suffix.shape != (B, 4, 4) or rot.shape != (B, 3) or xyz.shape != (B, 3):
         raise ValueError("pose_compose expects prefix, suffix (B, 4, 4) and rot, xyz (B, 3)")
     M = torch.empty(B, 3, 4, device=prefix.device, dtype=torch.float32)
-    with torch.cuda.device(prefix.device):
+    with _on(prefix.device):
         rc = _lib.lib().ndrr_pose_compose(prefix.data_ptr(), suffix.data_ptr(), rot.data_ptr(), xyz.data_ptr(), 3,
                                           M.data_ptr(), B, _stream(prefix.device))
     _lib.check(rc, "ndrr_pose_compose")
@@ -74,7 +75,7 @@ def _pose_compose_bwd_impl(prefix: torch.Tensor, suffix: torch.Tensor, rot: torc
     gM = gM.to(torch.float32).contiguous()
     grot = torch.empty(B, 3, device=prefix.device, dtype=torch.float32)
     gxyz = torch.empty(B, 3, device=prefix.device, dtype=torch.float32)
-    with torch.cuda.device(prefix.device):
+    with _on(prefix.device):
         rc = _lib.lib().ndrr_pose_compose_bwd(prefix.data_ptr(), suffix.data_ptr(), rot.data_ptr(), xyz.data_ptr(),
                                               3, gM.data_ptr(), grot.data_ptr(), gxyz.data_ptr(), B,
                                               _stream(prefix.device))
@@ -246,7 +247,7 @@ def _adam_step(p: torch.Tensor, g: torch.Tensor, m: torch.Tensor, v: torch.Tenso
                eps: float, t: int) -> None:
     """One native Adam step (ndrr_adam_step) on a contiguous fp32 device parameter."""
     g = g.contiguous()
-    with torch.cuda.device(p.device):
+    with _on(p.device):
         rc = _lib.lib().ndrr_adam_step(p.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(), p.numel(), lr, b1, b2,
                                        eps, 1.0 - b1 ** t, 1.0 - b2 ** t, _stream(p.device))
     _lib.check(rc, "ndrr_adam_step")

@@ -13,7 +13,9 @@ from __future__ import annotations
 import torch
 
 from . import _lib
+from ._memo import Derived
 from ._ops import eager
+from .subject import _Same
 
 _FUSED_DTYPES = (torch.float32, torch.float16, torch.bfloat16)
 _BACKENDS = ("auto", "hip", "triton")
@@ -50,13 +52,23 @@ _ADJOINTS = ("single_pass", "two_pass")
 
 def fused_supported(subject, B: int, n_pixels: int) -> bool:
     """Hard limits of the kernels: one volume, 32-bit indexing, <= 8 classes (backends.py:66-69)."""
+    return _supported(subject.image, subject.n_classes, B, n_pixels)
+
+
+def _supported(image, C: int, B: int, n_pixels: int) -> bool:
     N = B * n_pixels
-    return (
-        subject.image.shape[0] == 1
-        and max(subject.image.numel(), subject.n_classes * N, 3 * N) < 2**31
-        and 1 <= subject.n_classes <= _lib.MAX_CLASSES
+    return image.shape[0] == 1 and max(image.numel(), C * N, 3 * N) < 2**31 and 1 <= C <= _lib.MAX_CLASSES \
         and B <= 65535
-    )
+
+
+_NO_LABELS: dict = {}  # an empty int8 tensor per device: the label argument of single-class marches
+
+
+def _no_labels(device) -> torch.Tensor:
+    t = _NO_LABELS.get(device)
+    if t is None:
+        t = _NO_LABELS[device] = torch.empty(0, dtype=torch.int8, device=device)
+    return t
 
 
 def _make_tgt(k_inv, sdd, height, width, device, dtype) -> torch.Tensor:
@@ -102,7 +114,7 @@ def render(
     check_render_args(n_samples, backend)
     device, dtype = rt_inv.device, rt_inv.dtype
     if tgt is None and src is None and _native_rays(k_inv, sdd, dtype):
-        tgt, src, step_size = eager.make_rays(k_inv, sdd, height, width, orthographic, n_samples)
+        tgt, src, step_size = _rays(k_inv, sdd, height, width, orthographic, n_samples)
     else:
         if tgt is None:
             tgt = _make_tgt(k_inv, sdd, height, width, device, dtype)
@@ -123,6 +135,21 @@ def check_render_args(n_samples: int, backend: str) -> None:
                 "use backend='hip' (or 'auto')"
             )
         raise ValueError(f"Unknown backend {backend!r}; expected 'auto', 'hip' or 'triton'")
+
+
+_RAYS = Derived()  # the rays of the last detector rendered eagerly
+
+
+def _rays(k_inv, sdd, height, width, orthographic, n_samples):
+    """eager.make_rays, kept while k_inv and sdd are the same tensors at the same versions and the
+    call is on the same stream (a fixed detector renders many poses: one launch and its host work
+    fewer per render). Made afresh while a graph is captured or torch.compile traces (tensors made
+    during a capture belong to the graph's memory)."""
+    if torch.compiler.is_compiling() or torch.cuda.is_current_stream_capturing():
+        return eager.make_rays(k_inv, sdd, height, width, orthographic, n_samples)
+    key = (_Same(sdd), sdd._version, height, width, orthographic, n_samples,
+           torch.cuda.current_stream(k_inv.device).cuda_stream)
+    return _RAYS.get("rays", k_inv, key, lambda: eager.make_rays(k_inv, sdd, height, width, orthographic, n_samples))
 
 
 def _native_rays(k_inv, sdd, dtype) -> bool:
@@ -196,7 +223,7 @@ def march_inputs(subject, M, src, tgt, step_size, n_samples, height, width, layo
     B = max(M.shape[0], tgt.shape[0], src.shape[0])
     if M.shape[0] != B:
         M = M.expand(B, -1, -1)
-    if not fused_supported(subject, B, height * width):
+    if not _supported(image, C, B, height * width):
         raise ValueError("inputs exceed the hip kernel's limits (one volume, 32-bit indexing, <= 8 classes)")
     auto = (VOLUME_LAYOUT if layout is None else layout) == "auto"
     layout = volume_layout(image, layout)
@@ -205,7 +232,7 @@ def march_inputs(subject, M, src, tgt, step_size, n_samples, height, width, layo
             and subject.labelled_coef_fits():
         layout = "lcoef"
     packed = None if layout == "linear" else subject.packed_volume(layout)
-    lab = subject.packed_labels(layout) if C > 1 else vol.new_empty(0, dtype=torch.int8)
+    lab = subject.packed_labels(layout) if C > 1 else _no_labels(vol.device)
     args = (vol, lab, M.contiguous(), src.to(torch.float32).contiguous(), tgt.to(torch.float32).contiguous(),
             step_size.to(torch.float32).contiguous(), n_samples, C, width)
     return args, packed, layout, B, C
