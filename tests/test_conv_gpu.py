@@ -22,6 +22,13 @@ SHAPES = [  # N, H, W, Cin, Cout, stride
     (2, 8, 8, 128, 128, 1),
     (2, 12, 12, 128, 256, 2),
     (1, 7, 9, 32, 32, 1),
+    (2, 11, 13, 64, 32, 1),  # stride-1 input gradient streaming 32 dy channels
+    (2, 9, 7, 128, 64, 1),  # stride-1 input gradient with 128 columns per workgroup
+    # stride 1 with whole 16-, 32- and 64-pixel rows per 64-pixel tile (side columns from lane shifts)
+    (2, 6, 64, 64, 64, 1),
+    (3, 5, 32, 128, 128, 1),
+    (2, 7, 16, 32, 64, 1),
+    (2, 4, 32, 64, 128, 1),
 ]
 
 
