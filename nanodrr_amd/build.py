@@ -17,6 +17,8 @@ SOURCES = [PKG / "csrc" / name for name in ("raymarch.hip", "camera.hip", "regis
 OUTPUT = PKG / "libnanodrr_amd.so"
 
 # -packed-fp32-ops: keep the backend from forming packed fp32 instructions.
+# -amdgpu-mfma-vgpr-form: matrix-core accumulators in ordinary VGPRs; in the accumulator
+# registers the allocator rotated each K loop's sums through copies (up to 76 moves per 32 MFMAs).
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
@@ -24,6 +26,7 @@ HIPCC_FLAGS = [
     "-fPIC",
     "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-Wno-unused-command-line-argument",
 ]
 

@@ -55,6 +55,12 @@ constexpr int kWgradBlocksCin1 = 1024;
 #define NDRR_WGRAD_BUFFERS 1
 #endif
 
+// The halo kernel (k_conv_halo) for the stride-1 layers of 64-pixel rows and 64 channels (1),
+// or the tiled and halo-per-tap kernels there too (0).
+#ifndef NDRR_CONV_HALO
+#define NDRR_CONV_HALO 1
+#endif
+
 struct ConvP {
   const __bf16* x;    // [N][H][W][Cin]
   const __bf16* w;    // [Cout][9][Cin]      (forward)
@@ -650,6 +656,170 @@ void launch_dgrad_halo_s1(const ConvP& p, hipStream_t st) {
   else hipLaunchKernelGGL((k_dgrad_halo_s1<2>), dim3(tiles, p.Cin / 32), dim3(kThreads), 0, st, p);
 }
 
+// ---------------------------------------------------------------------------
+// Halo implicit GEMM for 64-pixel rows and 64 streamed channels at stride 1: the forward
+// (MODE 0, Cin = 64) and the input gradient (MODE 1, Cout = 64) of a 64 x 64 layer. A
+// workgroup keeps its 64 output columns' weights for all nine taps in LDS (the panel: row
+// 16 c + i holds the channel that pair_row gives lane i of row group c, 9 * 64 + 8 elements
+// long, so a lane group's 16-byte reads of 16 rows hit distinct banks) and runs over tiles
+// of four image rows, one row per wave. Each tile's source rows (the four and one above and
+// below, 66 pixels with the padding columns) are staged in LDS once, with whole-line loads,
+// in pixel rows of 64 + 8 channels (16-byte reads of 16 consecutive pixels hit distinct
+// banks); the next tile's are loaded into registers while this tile's products run. Every
+// K step then reads both operands from LDS: the activations are read from memory about 1.5
+// times instead of once per tap. The K order (taps row-major, then 32-channel chunks), the
+// products and the epilogues are k_igemm's: the results are bitwise the same.
+//   MODE 0: tap (ky, kx) of output pixel (y, x) reads x at (y + ky - 1, x + kx - 1).
+//   MODE 1: tap (ky, kx) of input pixel (y, x) reads dy at (y + 1 - ky, x + 1 - kx).
+// ---------------------------------------------------------------------------
+constexpr int kHaloCols = 66;                  // 64 pixels and a padding column each side
+constexpr int kHaloPitch = 72;                 // 64 channels + 8, in elements
+constexpr int kHaloPieces = 6 * kHaloCols * 8; // 16-byte pieces of a tile's source rows
+constexpr int kHaloLoads = (kHaloPieces + kThreads - 1) / kThreads;
+constexpr int kPanelRow = 9 * 64 + 8;
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_conv_halo(ConvP p) {
+  constexpr int CG = 4;
+  __shared__ __attribute__((aligned(16))) __bf16 panel[16 * CG * kPanelRow];
+  __shared__ __attribute__((aligned(16))) __bf16 halo[6 * kHaloCols * kHaloPitch];
+  const int t = threadIdx.x, wave = t >> 6, l = t & 63;
+  const int ch0 = blockIdx.y * 16 * CG;
+  for (int e = t; e < 16 * CG * 72; e += kThreads) {  // 72 pieces of 8 per panel row
+    const int r = e / 72, q = e - r * 72;
+    const int c = r >> 4, i = r & 15;
+    const int ch = ch0 + 32 * (c >> 1) + 8 * (i >> 2) + 4 * (c & 1) + (i & 3);
+    const __bf16* src;
+    if constexpr (MODE == 0) {
+      src = p.w + static_cast<int64_t>(ch) * 576 + 8 * q;  // w[ch][tap][ci]
+    } else {
+      const int tap = q >> 3, part = q & 7;
+      src = p.wt + (static_cast<int64_t>(tap) * p.Cin + ch) * 64 + 8 * part;  // wt[tap][ch][co]
+    }
+    *reinterpret_cast<bf16x8*>(panel + r * kPanelRow + 8 * q) = load8(src);
+  }
+  // rows: output pixels (MODE 0) or input pixels (MODE 1) on an Hr x 64 grid; source Hr x 64 x 64
+  const int Hr = MODE == 0 ? p.Ho : p.H;
+  const int quads = (Hr + 3) / 4;
+  const int tiles = p.N * quads;
+  const __bf16* const srcp = MODE == 0 ? p.x : p.dy;
+  bf16x8 hv[kHaloLoads];
+  auto hload = [&](int tile) {  // the tile's source rows r0 - 1 .. r0 + 4, columns -1 .. 64
+    const int n = tile / quads, r0 = (tile - n * quads) * 4;
+#pragma unroll
+    for (int k = 0; k < kHaloLoads; ++k) {
+      const int e = t + kThreads * k;
+      const int hr = e / (kHaloCols * 8), rem = e - hr * (kHaloCols * 8);
+      const int hc = rem >> 3, part = rem & 7;
+      const int sy = r0 - 1 + hr, sx = hc - 1;
+      const bool ok = e < kHaloPieces && tile < tiles && sy >= 0 && sy < Hr && sx >= 0 && sx < 64;
+      hv[k] = ok ? load8(srcp + ((static_cast<int64_t>(n) * Hr + sy) * 64 + sx) * 64 + 8 * part) : zero8();
+    }
+  };
+  auto hstore = [&]() {
+#pragma unroll
+    for (int k = 0; k < kHaloLoads; ++k) {
+      const int e = t + kThreads * k;
+      if (e < kHaloPieces) {
+        const int hr = e / (kHaloCols * 8), rem = e - hr * (kHaloCols * 8);
+        *reinterpret_cast<bf16x8*>(halo + (hr * kHaloCols + (rem >> 3)) * kHaloPitch + 8 * (rem & 7)) = hv[k];
+      }
+    }
+  };
+  const int kq = 8 * (l >> 4), px = l & 15;
+  const __bf16* const arow = panel + px * kPanelRow + kq;
+  int tile = blockIdx.x;
+  hload(tile);
+  for (; tile < tiles; tile += gridDim.x) {
+    __syncthreads();  // the panel is staged, and every wave is done with the last tile's rows
+    hstore();
+    __syncthreads();
+    hload(tile + gridDim.x);  // in flight during this tile's products
+    f32x4 acc[4][CG];
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+      for (int c = 0; c < CG; ++c) acc[g][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    // K step s = (tap, chunk) = (s / 2, s % 2); the next step's operands are read from LDS
+    // while this step's products run
+    auto operands = [&](int s, bf16x8 (&ax)[CG], bf16x8 (&bx)[4]) {
+      const int tap = s >> 1, kc = s & 1;
+      const int ky = tap / 3, kx = tap - 3 * ky;
+      const int hr = MODE == 0 ? wave + ky : wave + 2 - ky;
+      const int hc0 = MODE == 0 ? px + kx : px + 2 - kx;  // halo column of group 0's pixel
+      const __bf16* const brow = halo + (hr * kHaloCols + hc0) * kHaloPitch + kq + 32 * kc;
+#pragma unroll
+      for (int c = 0; c < CG; ++c) ax[c] = *reinterpret_cast<const bf16x8*>(arow + 16 * c * kPanelRow + 32 * s);
+#pragma unroll
+      for (int g = 0; g < 4; ++g) bx[g] = *reinterpret_cast<const bf16x8*>(brow + 16 * g * kHaloPitch);
+    };
+    // two operand sets in turn: even steps use set 0, odd steps set 1
+    bf16x8 ax[2][CG], bx[2][4];
+    operands(0, ax[0], bx[0]);
+#pragma unroll 1
+    for (int s = 0; s < 18; s += 2) {
+      operands(s + 1, ax[1], bx[1]);
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int c = 0; c < CG; ++c) acc[g][c] = mfma(ax[0][c], bx[0][g], acc[g][c]);
+      operands(s + 2 < 18 ? s + 2 : s + 1, ax[0], bx[0]);
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int c = 0; c < CG; ++c) acc[g][c] = mfma(ax[1][c], bx[1][g], acc[g][c]);
+    }
+    // lane l: channels cb .. cb + 7 (cb = ch0 + 32 h + kq) of pixel 16 g + l % 16 of row r0 + wave
+    const int n = tile / quads, row = (tile - n * quads) * 4 + wave;
+    if (row < Hr) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int64_t m = (static_cast<int64_t>(n) * Hr + row) * 64 + 16 * g + px;
+        const int64_t at = m * (MODE == 0 ? p.Cout : p.Cin);
+#pragma unroll
+        for (int h = 0; h < CG / 2; ++h) {
+          const int cb = ch0 + 32 * h + kq;
+          bf16x8 v;
+          if constexpr (MODE == 0) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              float u = acc[g][2 * h + j / 4][j % 4] + (p.bias ? p.bias[cb + j] : 0.0f);
+              if (p.relu) u = fmaxf(u, 0.0f);
+              v[j] = static_cast<__bf16>(u);
+            }
+            *reinterpret_cast<bf16x8*>(p.y + at + cb) = v;
+          } else {
+            const bf16x8 xm = p.mask ? load8(p.x + at + cb) : zero8();
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              float u = acc[g][2 * h + j / 4][j % 4];
+              if (p.mask && !(static_cast<float>(xm[j]) > 0.0f)) u = 0.0f;
+              v[j] = static_cast<__bf16>(u);
+            }
+            *reinterpret_cast<bf16x8*>(p.dx + at + cb) = v;
+          }
+        }
+      }
+    }
+  }
+}
+
+// The halo kernel where it applies (stride 1, 64-pixel rows, 64 streamed channels, 64-channel
+// column blocks); returns false elsewhere.
+template <int MODE>
+bool try_halo(const ConvP& p, hipStream_t st) {
+  const int K = MODE == 0 ? p.Cin : p.Cout, cols = MODE == 0 ? p.Cout : p.Cin;
+  const int wr = MODE == 0 ? p.Wo : p.W;
+  if (!NDRR_CONV_HALO || p.S != 1 || K != 64 || wr != 64 || cols % 64 != 0) return false;
+  const int hr = MODE == 0 ? p.Ho : p.H;
+  const int64_t tiles = static_cast<int64_t>(p.N) * ((hr + 3) / 4);
+  const int ycols = cols / 64;
+  const int64_t want = 256 / ycols > 0 ? 256 / ycols : 1;  // one workgroup per CU (LDS-bound)
+  const unsigned blocks = static_cast<unsigned>(tiles < want ? tiles : want);
+  hipLaunchKernelGGL((k_conv_halo<MODE>), dim3(blocks, ycols), dim3(kThreads), 0, st, p);
+  return true;
+}
+
 template <int MODE>
 void launch_igemm(const ConvP& p, int rows, int cols, int classes, hipStream_t st) {
   if (cols % 128 == 0) {
@@ -1121,7 +1291,7 @@ int ndrr_conv3x3_fwd(const void* x, const void* w, const float* bias, void* y, i
     } else {
       hipLaunchKernelGGL((k_conv_fwd<2>), grid, dim3(kThreads), 0, st, p);
     }
-  } else {
+  } else if (!try_halo<0>(p, st)) {
     launch_igemm<0>(p, M, Cout, 1, st);
   }
   return status();
@@ -1165,6 +1335,7 @@ int ndrr_conv3x3_bwd_data(const void* dy, const void* wt, const void* x, void* d
   // the halo-staged kernel for stride 1 up to 64 output channels (each chunk of dy read once per
   // tile); beyond, the tiled implicit GEMM, whose weights staged in LDS serve four waves at once,
   // measured faster
+  if (try_halo<1>(p, st)) return status();
   if (stride == 1 && Cout <= 64) {
     launch_dgrad_halo_s1(p, st);
     return status();
